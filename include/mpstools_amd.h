@@ -1,0 +1,160 @@
+/*
+ * mpstools_amd.h -- C ABI of the MI355X (gfx950) DMRG hot path.
+ *
+ * Replaces, on the GPU, the block-sparse contractions of MPSTools'
+ * src/MPSfunctions.hpp that the Lanczos/DMRG engines call:
+ *   contractTwosite       (src/MPSfunctions.hpp:949-952, body :726-804)
+ *   contractSingle        (src/MPSfunctions.hpp:857-859, body :808-854)
+ *   addContractionLayer   (src/MPSfunctions.hpp:442-444 and Ap variant :499-502,
+ *                          bodies add_out_sub_2 :276-352 / add_out_sub :359-435)
+ * and the Lanczos vector algebra of TSLanzcosEngine::simulate
+ * (src/lanzcos.hpp:165-278: BlockMatrix dot/axpy/scale, BlockTensor.hpp:443-544).
+ *
+ * Conventions
+ *   - A block-sparse tensor (BlockMatrix<Ns,AbKeyType,T>) crosses the ABI as a
+ *     key table (mpsa_blocks) plus one packed device buffer: block i is the
+ *     rows[i] x cols[i] column-major matrix (ld = rows[i]) at element
+ *     offset[i].  Keys are (q[2i], q[2i+1], p[ns*i .. ns*i+ns-1]): packed
+ *     AbKeyType charges (src/keyTypes.hpp:25-30, unsigned 64-bit wrap-around
+ *     arithmetic) and physical indices (MPS / two-site psi) or the MPO aux
+ *     index (environments), exactly TensorKeyType<2,Ns,AbKeyType>.
+ *   - An MPO tensor (MPOMat<1,T>) is a list of entries (wl, wr, out, in, value):
+ *     OpKeyType(out,in) as in src/keyTypes.hpp:384-421.
+ *   - i_to_key is an array of d packed charges (state s -> charge).
+ *   - The reference's matvec reads psi at the operator's out index and writes
+ *     at its in index (src/MPSfunctions.hpp:760-792); these entry points keep
+ *     that convention, and environments keep "upper = conjugated = in".
+ *   - Every call returns 0 on success; on failure a non-zero MPSA_E* code and
+ *     mpsa_last_error() describes it (the reference prints and aborts).
+ *   - Host memory belongs to the caller; a plan owns its device workspace.
+ *     Calls on one plan must be serialised on one stream; distinct plans may
+ *     run concurrently.
+ *   - Summation order inside a plan is fixed (no floating-point atomics), so
+ *     results are bitwise reproducible run to run.
+ */
+#ifndef MPSTOOLS_AMD_H
+#define MPSTOOLS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* mpsa_stream; /* hipStream_t */
+
+enum {
+  MPSA_OK = 0,
+  MPSA_EINVAL = 1,     /* bad argument, size or key table mismatch */
+  MPSA_ENOKEY = 2,     /* an output block produced by the contraction is missing from the output table */
+  MPSA_EHIP = 3,       /* HIP runtime error */
+  MPSA_ENOMEM = 4,     /* device allocation failed */
+  MPSA_ENOTFINITE = 5  /* NaN/Inf detected where the call checks for it */
+};
+
+/* Arithmetic of the sector GEMMs.  Inputs and outputs at the ABI are always
+ * FP64 (or complex128 interleaved when is_complex is set); F32 and BF16 plans
+ * keep reduced-precision copies of the environments and convert psi on the fly.
+ * BF16 means bf16 inputs with fp32 accumulation on MFMA. */
+enum { MPSA_F64 = 0, MPSA_F32 = 1, MPSA_BF16 = 2 };
+
+typedef struct mpsa_blocks {
+  int64_t n;              /* number of blocks */
+  int32_t ns;             /* physical indices per key (1 or 2) */
+  int32_t is_complex;     /* 0: real double, 1: complex128 interleaved */
+  const uint64_t* q;      /* [2n] charges: left/upper, right/lower */
+  const uint32_t* p;      /* [ns*n] physical indices or MPO aux index */
+  const int64_t* rows;    /* [n] */
+  const int64_t* cols;    /* [n] */
+  const int64_t* offset;  /* [n] element offsets into the packed buffer */
+} mpsa_blocks;
+
+typedef struct mpsa_mpo {
+  int64_t n;              /* number of nonzero entries */
+  uint32_t Dl, Dr;        /* aux dimensions (MPOMat::Dl/Dr) */
+  int32_t is_complex;     /* values: n doubles, or 2n doubles (re, im) */
+  const uint32_t* wl;
+  const uint32_t* wr;
+  const uint32_t* out;
+  const uint32_t* in;
+  const double* val;
+} mpsa_mpo;
+
+typedef struct mpsa_plan mpsa_plan;
+
+const char* mpsa_last_error(void);
+const char* mpsa_version(void);
+
+/* ---- two-site H_eff . psi  (contractTwosite, src/MPSfunctions.hpp:949-981) ----
+ * Output key set: the keys the reference's tuple loop produces
+ * (src/MPSfunctions.hpp:792-798), in key order.  Query with cap = 0 for the
+ * count, then again with arrays of that size. */
+int mpsa_heff2_outkeys(const mpsa_blocks* L, const mpsa_blocks* R, const mpsa_blocks* x, const mpsa_mpo* Wl,
+                       const mpsa_mpo* Wr, const uint64_t* itokey_l, int dl, const uint64_t* itokey_r, int dr,
+                       int64_t cap, int64_t* nout, uint64_t* q, uint32_t* p, int64_t* rows, int64_t* cols);
+
+/* Build a grouped-GEMM plan for y = H_eff x.  The y table must contain every
+ * produced key (blocks it holds that are not produced are written as zeros);
+ * x and y may be the same table (a Lanczos vector space).  precision: MPSA_F64,
+ * MPSA_F32 or MPSA_BF16. */
+int mpsa_heff2_plan(mpsa_plan** plan, const mpsa_blocks* L, const mpsa_blocks* R, const mpsa_blocks* x,
+                    const mpsa_blocks* y, const mpsa_mpo* Wl, const mpsa_mpo* Wr, const uint64_t* itokey_l, int dl,
+                    const uint64_t* itokey_r, int dr, int precision);
+
+/* ---- one-site H_eff . A  (contractSingle, src/MPSfunctions.hpp:857-885) ---- */
+int mpsa_heff1_outkeys(const mpsa_blocks* L, const mpsa_blocks* R, const mpsa_blocks* x, const mpsa_mpo* W,
+                       const uint64_t* itokey, int d, int64_t cap, int64_t* nout, uint64_t* q, uint32_t* p,
+                       int64_t* rows, int64_t* cols);
+int mpsa_heff1_plan(mpsa_plan** plan, const mpsa_blocks* L, const mpsa_blocks* R, const mpsa_blocks* x,
+                    const mpsa_blocks* y, const mpsa_mpo* W, const uint64_t* itokey, int d, int precision);
+
+/* Bind the device environments (FP64 packed buffers in the L/R tables' layout).
+ * Reduced-precision plans convert them here, once; call again whenever the
+ * environment contents change. */
+int mpsa_plan_bind_env(mpsa_plan* plan, const double* dL, const double* dR, mpsa_stream stream);
+
+/* y[v] = H_eff x[v] for v < nvec; vector v of x starts at dx + v*x_stride
+ * elements (likewise y).  All nvec vectors go through one grouped launch per
+ * stage. */
+int mpsa_heff_apply(mpsa_plan* plan, const double* dx, int64_t x_stride, double* dy, int64_t y_stride, int nvec,
+                    mpsa_stream stream);
+
+/* ---- environment update (addContractionLayer, src/MPSfunctions.hpp:442-555) ----
+ * direction > 0: left environment grows by one site; direction <= 0: right.
+ * Ap (may be NULL) is the upper/conjugated tensor of the mixed variant
+ * (:499-555); when NULL the upper tensor is A itself. */
+int mpsa_env_outkeys(const mpsa_blocks* BL, const mpsa_blocks* A, const mpsa_blocks* Ap, const mpsa_mpo* W,
+                     const uint64_t* itokey, int d, int direction, int64_t cap, int64_t* nout, uint64_t* q,
+                     uint32_t* p, int64_t* rows, int64_t* cols);
+int mpsa_env_plan(mpsa_plan** plan, const mpsa_blocks* BL, const mpsa_blocks* A, const mpsa_blocks* Ap,
+                  const mpsa_mpo* W, const uint64_t* itokey, int d, int direction, const mpsa_blocks* out,
+                  int precision);
+int mpsa_env_apply(mpsa_plan* plan, const double* dBL, const double* dA, const double* dAp, double* dout,
+                   mpsa_stream stream);
+
+/* ---- plan queries ---- */
+int mpsa_plan_info(const mpsa_plan* plan, double* flops_alg, double* flops_ref, int64_t* n_gemm1, int64_t* n_gemm2,
+                   int64_t* workspace_bytes);
+/* Algorithmic flops of the two grouped GEMM stages (one vector) and the tile
+ * class each stage launches with (0: 128x128, 1: 64x64). */
+int mpsa_plan_stage_info(const mpsa_plan* plan, double* flops_stage1, double* flops_stage2, int* tile_class1,
+                         int* tile_class2);
+/* Per-stage HIP-event timing of mpsa_heff_apply on the call's stream.  When
+ * enabled, mpsa_plan_last_times returns, for the last apply, the milliseconds
+ * of [convert x, stage-1 GEMM, mix + zero fill, stage-2 GEMM]. */
+int mpsa_plan_set_timing(mpsa_plan* plan, int enable);
+int mpsa_plan_last_times(const mpsa_plan* plan, float* ms4);
+int mpsa_plan_destroy(mpsa_plan* plan);
+
+/* ---- Lanczos vector algebra over packed block vectors (BlockTensor.hpp:443-544) ----
+ * X*Y = sum conj(Y) X (src/BlockTensor.hpp:520-544), computed in FP64 with a
+ * fixed-order two-level reduction; result written to host *out (re, im). */
+int mpsa_dot(const double* dx, const double* dy, int64_t n, int is_complex, double* out, mpsa_stream stream);
+/* y = a*x + b*y (FP64; complex when is_complex, a and b complex as 2 doubles). */
+int mpsa_axpby(const double* a, const double* dx, const double* b, double* dy, int64_t n, int is_complex,
+               mpsa_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPSTOOLS_AMD_H */

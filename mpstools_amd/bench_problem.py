@@ -1,0 +1,111 @@
+"""Device-resident synthetic workload: the centre bond of a dense (no symmetry)
+spin-1/2 Heisenberg chain, J = 1, L sites, MPS bond dimension D.
+
+Inputs are a seeded Gaussian MPS (torch's CUDA generator; random input
+generation is plumbing), left-normalised by QR on sites < c and
+right-normalised on sites > c+1.  The environments L[c-1] and R[c+2] are then
+built layer by layer with this package's environment kernels
+(addContractionLayer on the GPU), exactly as DMRGengine builds them
+(src/MPSfunctions.hpp:2025-2052).  Every local state carries charge 0, so all
+blocks are dense: L and R hold one D x D block per MPO channel and psi four
+D x D blocks (s1, s2).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import models, native
+from .blocks import Blocks
+
+
+def _dims(L, D, d=2):
+    return [int(min(d ** min(l, L - l), D)) for l in range(L + 1)]
+
+
+def _site_table(dl, dr, d=2):
+    q = np.zeros((d, 2), np.uint64)
+    p = np.arange(d, dtype=np.uint32).reshape(d, 1)
+    return native.Table.from_keys(1, q, p, [dl] * d, [dr] * d)
+
+
+def _lmin_table():
+    return native.Table.from_keys(1, np.zeros((1, 2), np.uint64), np.zeros((1, 1), np.uint32), [1], [1])
+
+
+def dense_heisenberg_centre(D=2048, L=128, seed=20260003, build_envs="sweep", precisions=("f64",), verbose=False):
+    import torch
+    dev = "cuda"
+    mpo = models.xxz_mpo(L, 1.0, 1.0, 0.0)
+    itk = models.itokey_dense()
+    c = L // 2 - 1
+    dims = _dims(L, D)
+    g = torch.Generator(device=dev)
+    g.manual_seed(int(seed))
+    plans = {}
+
+    def env_step(Et, dE, s, direction):
+        dl, dr = dims[s], dims[s + 1]
+        At = _site_table(dl, dr)
+        key = (s if s in (0, L - 1) else -1, dl, dr, direction, tuple(Et.keys()))
+        if key not in plans:
+            outt = native.env_outkeys(Et, At, mpo[s], itk, direction)
+            plans[key] = (native.EnvPlan(Et, At, mpo[s], itk, direction, outt), outt)
+        plan, outt = plans[key]
+        # random site tensor -> isometry
+        if direction > 0:
+            G = torch.randn(2 * dl, dr, dtype=torch.float64, device=dev, generator=g)
+            Q, _ = torch.linalg.qr(G)
+            blocks = [Q[i * dl:(i + 1) * dl, :] for i in range(2)]
+        else:
+            G = torch.randn(dl, 2 * dr, dtype=torch.float64, device=dev, generator=g)
+            Q, _ = torch.linalg.qr(G.T)
+            Qh = Q.T
+            blocks = [Qh[:, i * dr:(i + 1) * dr] for i in range(2)]
+        dA = torch.zeros(max(At.total, 1), dtype=torch.float64, device=dev)
+        for i in range(2):
+            o = int(At.offsets[i])
+            dA[o:o + dl * dr] = blocks[i].T.contiguous().reshape(-1)  # column-major
+        dout = torch.zeros(max(outt.total, 1), dtype=torch.float64, device=dev)
+        plan.apply(dE, dA, dout)
+        return outt, dout
+
+    if build_envs == "sweep":
+        Et, dE = _lmin_table(), torch.ones(1, dtype=torch.float64, device=dev)
+        for s in range(0, c):
+            Et, dE = env_step(Et, dE, s, 1)
+            if verbose and s % 16 == 0:
+                print(f"[bench_problem] left env site {s}", flush=True)
+        Lt, dL = Et, dE
+        # R[L] = identity on the last bond (getRMax), aux 0
+        Et, dE = _lmin_table(), torch.ones(1, dtype=torch.float64, device=dev)
+        for s in range(L - 1, c + 1, -1):
+            Et, dE = env_step(Et, dE, s, 0)
+            if verbose and s % 16 == 0:
+                print(f"[bench_problem] right env site {s}", flush=True)
+        Rt, dR = Et, dE
+    else:
+        raise ValueError(build_envs)
+    plans.clear()
+
+    # centre two-site psi = A_c A_{c+1}, normalised
+    dl, dm, dr = dims[c], dims[c + 1], dims[c + 2]
+    A1 = torch.randn(2, dl, dm, dtype=torch.float64, device=dev, generator=g)
+    A2 = torch.randn(2, dm, dr, dtype=torch.float64, device=dev, generator=g)
+    q = np.zeros((4, 2), np.uint64)
+    p = np.array([[s1, s2] for s1 in range(2) for s2 in range(2)], np.uint32)
+    xt = native.Table.from_keys(2, q, p, [dl] * 4, [dr] * 4)
+    yt = native.heff2_outkeys(Lt, Rt, xt, mpo[c], mpo[c + 1], itk, itk)
+    vt = xt.union(yt)
+    psi = torch.zeros(vt.total, dtype=torch.float64, device=dev)
+    for i, k in enumerate(vt.keys()):
+        s1, s2 = k[2], k[3]
+        blk = A1[s1] @ A2[s2]
+        o = int(vt.offsets[i])
+        psi[o:o + dl * dr] = blk.T.contiguous().reshape(-1)
+    psi /= psi.norm()
+    out = dict(mpo=mpo, itk=itk, c=c, dims=dims, Lt=Lt, Rt=Rt, dL=dL, dR=dR, xt=xt, yt=yt, vt=vt, psi=psi, L=L, D=D)
+    for pr in precisions:
+        plan = native.HeffPlan(Lt, Rt, vt, vt, (mpo[c], mpo[c + 1]), (itk, itk), native.PREC[pr])
+        plan.bind_env(dL, dR)
+        out["plan_" + pr] = plan
+    return out

@@ -1,0 +1,72 @@
+// mpsa_internal.h -- descriptors shared by the plan builder (host) and the
+// gfx950 kernels.  All structs are plain data copied to device memory once per
+// plan.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mpsa {
+
+// One output block of a grouped GEMM stage: C (m x n, ld = m) at c_off.
+// C = sum over segments s of alpha_s * op(A_s) * op(B_s).
+struct GTask {
+  int64_t c_off;
+  int32_t m, n;
+  int32_t seg0, nseg;
+  int32_t tile0;    // first tile of this task inside one vector's tile range
+  int32_t tiles_m;  // tiles along m (tiles are numbered m-fastest)
+};
+
+struct GSeg {
+  int64_t a_off, b_off;
+  int32_t k;
+  int32_t lda, ldb;
+  int32_t pad;
+  double alpha;
+};
+
+// dst block (len elements) = sum_t coef_t * src_t (all of length len).
+struct MixTask {
+  int64_t dst_off;
+  int64_t len;
+  int32_t t0, nt;
+  int32_t chunk0;  // first 1024-element chunk of this task inside one vector
+  int32_t pad;
+};
+struct MixTerm {
+  int64_t src_off;
+  double coef;
+};
+
+struct Repack {
+  int64_t src_off, dst_off;
+  int32_t rows, cols;
+  int32_t dst_ld, transpose;
+  int32_t chunk0, pad;
+};
+
+struct Range {
+  int64_t off, len;
+};
+
+enum Prec { F64 = 0, F32 = 1, BF16 = 2 };
+
+// Launchers (mpsa_kernels.hip).  `tr` packs (transA << 1) | transB.
+// a_vs/b_vs/c_vs are per-vector strides (elements) for batched launches.
+hipError_t launch_gemm(int prec_in, int out_kind, int tr, int tile_class, const GTask* tasks, int ntasks,
+                       const GSeg* segs, int tiles_per_vec, int nvec, const void* A, int64_t a_vs, const void* B,
+                       int64_t b_vs, void* C, int64_t c_vs, hipStream_t st);
+// out_kind: 0 = same type as the input precision, 1 = double
+hipError_t launch_mix(int prec, const MixTask* tasks, int ntasks, const MixTerm* terms, int chunks_per_vec, int nvec,
+                      const void* src, void* dst, int64_t vs, hipStream_t st);
+hipError_t launch_convert(int prec, const double* src, void* dst, int64_t n, hipStream_t st);
+hipError_t launch_repack(int prec, const Repack* tasks, int ntasks, int chunks_per_vec, int nvec, const double* src,
+                         int64_t s_vs, void* dst, int64_t d_vs, hipStream_t st);
+hipError_t launch_zero_ranges(const Range* r, int nr, double* base, int64_t vs, int nvec, hipStream_t st);
+hipError_t launch_dot(const double* x, const double* y, int64_t n, int cplx, double* partial, int nparts,
+                      double* out2, hipStream_t st);
+hipError_t launch_axpby(double ar, double ai, const double* x, double br, double bi, double* y, int64_t n, int cplx,
+                        hipStream_t st);
+int gemm_max_grid(int prec_in, int tile_class);
+
+}  // namespace mpsa

@@ -1,0 +1,662 @@
+// mpsa_kernels.hip -- gfx950 kernels of the DMRG hot path.
+//
+// * gemm16   : persistent grouped, segmented-K GEMM on the 16x16x4 MFMA
+//              (v_mfma_f64_16x16x4_f64 for FP64, v_mfma_f32_16x16x4_f32 for
+//              FP32).  One launch covers every sector GEMM of a stage for all
+//              batched vectors; tiles are distributed over a persistent grid.
+// * gemm_bf16: the same on v_mfma_f32_16x16x32_bf16 (bf16 in, fp32 accumulate),
+//              for operands stored k-contiguous with leading dims padded to 8.
+// * mix      : the W-mixing between the two GEMM stages (T2 = sum c T1).
+// * convert / zero / dot / axpby: vector plumbing for the Lanczos recurrence.
+//
+// The contraction these kernels carry out is the reference's
+// doContraction/add_out_sub tuple loop (src/MPSfunctions.hpp:276-352,
+// :726-804) regrouped into unique (L, psi) products, a W mix, and one
+// segmented-K product per output block.
+#include <hip/hip_bf16.h>
+#include <hip/hip_runtime.h>
+
+#include "mpsa_internal.h"
+
+namespace mpsa {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+template <typename T>
+struct Acc4;
+template <>
+struct Acc4<double> {
+  typedef d4 type;
+};
+template <>
+struct Acc4<float> {
+  typedef f4 type;
+};
+
+__device__ __forceinline__ d4 mfma16(double a, double b, d4 c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f4 mfma16(float a, float b, f4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Find the task owning tile t (tasks sorted by tile0; tile0[0] == 0).
+template <typename Task>
+__device__ __forceinline__ int find_task(const Task* tasks, int ntasks, int t, int Task::*field) {
+  int lo = 0, hi = ntasks - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (tasks[mid].*field <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Persistent-grid mapping that keeps consecutive logical work items on one
+// XCD (workgroups are dispatched round-robin over the 8 XCDs).
+__device__ __forceinline__ int xcd_logical_id() {
+  const int G = gridDim.x, b = blockIdx.x;
+  if ((G & 7) != 0) return b;
+  return (b & 7) * (G >> 3) + (b >> 3);
+}
+
+template <typename TO, typename TA>
+__device__ __forceinline__ TO cvt(TA x) {
+  return (TO)x;
+}
+template <>
+__device__ __forceinline__ __hip_bfloat16 cvt<__hip_bfloat16, float>(float x) {
+  return __float2bfloat16(x);
+}
+template <>
+__device__ __forceinline__ __hip_bfloat16 cvt<__hip_bfloat16, double>(double x) {
+  return __float2bfloat16((float)x);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 / FP32 grouped GEMM, 16x16x4 MFMA.  256 threads = 4 waves in 2x2; each
+// wave owns a (BM/2)x(BN/2) sub-tile.  A and B tiles go through LDS (k-major);
+// the next k-step's global loads are issued into registers before the MFMAs of
+// the current step.
+// ---------------------------------------------------------------------------
+template <typename T, typename TOut, int BM, int BN, bool TA, bool TB>
+__global__ __launch_bounds__(256, 2) void gemm16_kernel(const GTask* __restrict__ tasks, int ntasks,
+                                                        const GSeg* __restrict__ segs, int tiles_per_vec,
+                                                        int total_tiles, const T* __restrict__ A, int64_t a_vs,
+                                                        const T* __restrict__ B, int64_t b_vs,
+                                                        TOut* __restrict__ C, int64_t c_vs) {
+  constexpr int BK = 16;
+  constexpr int PA = BM + 2, PB = BN + 2;
+  constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
+  constexpr int NA = BM * BK / 256, NB = BN * BK / 256;
+  typedef typename Acc4<T>::type V4;
+  __shared__ T As[BK][PA];
+  __shared__ T Bs[BK][PB];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int l16 = lane & 15, lq = lane >> 4;
+
+  for (int g = xcd_logical_id(); g < total_tiles; g += gridDim.x) {
+    const int v = g / tiles_per_vec;
+    const int t = g - v * tiles_per_vec;
+    const int ti_ = find_task(tasks, ntasks, t, &GTask::tile0);
+    const GTask tk = tasks[ti_];
+    const int local = t - tk.tile0;
+    const int tm = local % tk.tiles_m, tn = local / tk.tiles_m;
+    const int row0 = tm * BM, col0 = tn * BN;
+    const int m = tk.m, n = tk.n;
+    V4 acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) acc[i][j] = V4{0, 0, 0, 0};
+
+    // k-step iterator over (segment, k0)
+    int s = tk.seg0, send = tk.seg0 + tk.nseg;
+    while (s < send && segs[s].k <= 0) ++s;
+    int k0 = 0;
+    T ra[NA], rb[NB];
+    auto load = [&](int si, int kk0) {
+      const GSeg sg = segs[si];
+      const T* Ab = A + (int64_t)v * a_vs + sg.a_off;
+      const T* Bb = B + (int64_t)v * b_vs + sg.b_off;
+      const T alpha = (T)sg.alpha;
+#pragma unroll
+      for (int r = 0; r < NA; ++r) {
+        const int e = tid + r * 256;
+        int i, kk;
+        if (!TA) { i = e % BM; kk = e / BM; }
+        else { kk = e % BK; i = e / BK; }
+        const int gi = row0 + i, gk = kk0 + kk;
+        T val = (T)0;
+        if (gi < m && gk < sg.k) val = TA ? Ab[gk + (int64_t)gi * sg.lda] : Ab[gi + (int64_t)gk * sg.lda];
+        ra[r] = val * alpha;
+      }
+#pragma unroll
+      for (int r = 0; r < NB; ++r) {
+        const int e = tid + r * 256;
+        int j, kk;
+        if (!TB) { kk = e % BK; j = e / BK; }
+        else { j = e % BN; kk = e / BN; }
+        const int gj = col0 + j, gk = kk0 + kk;
+        T val = (T)0;
+        if (gj < n && gk < sg.k) val = TB ? Bb[gj + (int64_t)gk * sg.ldb] : Bb[gk + (int64_t)gj * sg.ldb];
+        rb[r] = val;
+      }
+    };
+    auto store = [&]() {
+#pragma unroll
+      for (int r = 0; r < NA; ++r) {
+        const int e = tid + r * 256;
+        int i, kk;
+        if (!TA) { i = e % BM; kk = e / BM; }
+        else { kk = e % BK; i = e / BK; }
+        As[kk][i] = ra[r];
+      }
+#pragma unroll
+      for (int r = 0; r < NB; ++r) {
+        const int e = tid + r * 256;
+        int j, kk;
+        if (!TB) { kk = e % BK; j = e / BK; }
+        else { j = e % BN; kk = e / BN; }
+        Bs[kk][j] = rb[r];
+      }
+    };
+    if (s < send) load(s, 0);
+    while (s < send) {
+      __syncthreads();  // previous MFMAs done reading LDS
+      store();
+      __syncthreads();
+      // advance and prefetch
+      int ns_ = s, nk = k0 + BK;
+      if (nk >= segs[ns_].k) {
+        ++ns_;
+        while (ns_ < send && segs[ns_].k <= 0) ++ns_;
+        nk = 0;
+      }
+      if (ns_ < send) load(ns_, nk);
+#pragma unroll
+      for (int kq = 0; kq < BK / 4; ++kq) {
+        T af[TM], bf[TN];
+#pragma unroll
+        for (int i = 0; i < TM; ++i) af[i] = As[kq * 4 + lq][wm * WM + i * 16 + l16];
+#pragma unroll
+        for (int j = 0; j < TN; ++j) bf[j] = Bs[kq * 4 + lq][wn * WN + j * 16 + l16];
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) acc[i][j] = mfma16(af[i], bf[j], acc[i][j]);
+      }
+      s = ns_;
+      k0 = nk;
+    }
+    // epilogue: lane holds rows 4*lq + r of column l16 of each 16x16 tile
+    TOut* Cb = C + (int64_t)v * c_vs + tk.c_off;
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int col = col0 + wn * WN + j * 16 + l16;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = row0 + wm * WM + i * 16 + lq * 4 + r;
+          if (row < m && col < n) Cb[row + (int64_t)col * m] = cvt<TOut>(acc[i][j][r]);
+        }
+      }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// BF16 grouped GEMM on v_mfma_f32_16x16x32_bf16.  Operands are stored
+// k-contiguous: op(A)(i,k) = A[i*lda + k], op(B)(k,j) = B[j*ldb + k], with
+// lda, ldb, offsets multiples of 8 and the k padding zero-filled, so every
+// fragment is one 16-byte load.  C is written either column-major (ld = m) in
+// TOut, or row-major (ld = n, padded to 8) when c_rowmajor is set (the layout
+// the next stage reads as a k-contiguous A operand).
+// ---------------------------------------------------------------------------
+template <typename TOut, int BM, int BN>
+__global__ __launch_bounds__(256, 2) void gemm_bf16_kernel(const GTask* __restrict__ tasks, int ntasks,
+                                                           const GSeg* __restrict__ segs, int tiles_per_vec,
+                                                           int total_tiles, const __hip_bfloat16* __restrict__ A,
+                                                           int64_t a_vs, const __hip_bfloat16* __restrict__ B,
+                                                           int64_t b_vs, TOut* __restrict__ C, int64_t c_vs,
+                                                           int c_rowmajor) {
+  constexpr int BK = 32, PK = BK + 8;
+  constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
+  constexpr int NA = BM * BK / 8 / 256, NB = BN * BK / 8 / 256;  // 16-byte vectors per thread
+  __shared__ __attribute__((aligned(16))) unsigned short As[BM][PK];
+  __shared__ __attribute__((aligned(16))) unsigned short Bs[BN][PK];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int l16 = lane & 15, lq = lane >> 4;
+  const unsigned short* Au = reinterpret_cast<const unsigned short*>(A);
+  const unsigned short* Bu = reinterpret_cast<const unsigned short*>(B);
+
+  for (int g = xcd_logical_id(); g < total_tiles; g += gridDim.x) {
+    const int v = g / tiles_per_vec;
+    const int t = g - v * tiles_per_vec;
+    const GTask tk = tasks[find_task(tasks, ntasks, t, &GTask::tile0)];
+    const int local = t - tk.tile0;
+    const int tm = local % tk.tiles_m, tn = local / tk.tiles_m;
+    const int row0 = tm * BM, col0 = tn * BN;
+    const int m = tk.m, n = tk.n;
+    f4 acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) acc[i][j] = f4{0, 0, 0, 0};
+
+    int s = tk.seg0, send = tk.seg0 + tk.nseg;
+    while (s < send && segs[s].k <= 0) ++s;
+    int k0 = 0;
+    u16x8 ra[NA], rb[NB];
+    float alpha = 1.f;
+    auto load = [&](int si, int kk0) {
+      const GSeg sg = segs[si];
+      const unsigned short* Ab = Au + (int64_t)v * a_vs + sg.a_off;
+      const unsigned short* Bb = Bu + (int64_t)v * b_vs + sg.b_off;
+      const int kpad = (sg.k + 7) & ~7;
+#pragma unroll
+      for (int r = 0; r < NA; ++r) {
+        const int e = tid + r * 256;
+        const int i = e / (BK / 8), kc = (e % (BK / 8)) * 8;
+        const int gi = row0 + i, gk = kk0 + kc;
+        u16x8 val = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+        if (gi < m && gk < kpad) val = *reinterpret_cast<const u16x8*>(Ab + (int64_t)gi * sg.lda + gk);
+        ra[r] = val;
+      }
+#pragma unroll
+      for (int r = 0; r < NB; ++r) {
+        const int e = tid + r * 256;
+        const int j = e / (BK / 8), kc = (e % (BK / 8)) * 8;
+        const int gj = col0 + j, gk = kk0 + kc;
+        u16x8 val = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+        if (gj < n && gk < kpad) val = *reinterpret_cast<const u16x8*>(Bb + (int64_t)gj * sg.ldb + gk);
+        rb[r] = val;
+      }
+    };
+    auto store = [&]() {
+#pragma unroll
+      for (int r = 0; r < NA; ++r) {
+        const int e = tid + r * 256;
+        *reinterpret_cast<u16x8*>(&As[e / (BK / 8)][(e % (BK / 8)) * 8]) = ra[r];
+      }
+#pragma unroll
+      for (int r = 0; r < NB; ++r) {
+        const int e = tid + r * 256;
+        *reinterpret_cast<u16x8*>(&Bs[e / (BK / 8)][(e % (BK / 8)) * 8]) = rb[r];
+      }
+    };
+    if (s < send) load(s, 0);
+    int cur = s;
+    while (s < send) {
+      __syncthreads();
+      store();
+      __syncthreads();
+      alpha = (float)segs[cur].alpha;
+      int ns_ = s, nk = k0 + BK;
+      if (nk >= segs[ns_].k) {
+        ++ns_;
+        while (ns_ < send && segs[ns_].k <= 0) ++ns_;
+        nk = 0;
+      }
+      if (ns_ < send) load(ns_, nk);
+      bf8 af[TM], bfr[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+        af[i] = *reinterpret_cast<const bf8*>(&As[wm * WM + i * 16 + l16][lq * 8]);
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+        bfr[j] = *reinterpret_cast<const bf8*>(&Bs[wn * WN + j * 16 + l16][lq * 8]);
+      if (alpha == 1.f) {
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+      } else {
+        // segment scale: accumulate this k-step separately, then add alpha * partial
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            f4 p = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], f4{0, 0, 0, 0}, 0, 0, 0);
+            acc[i][j] += alpha * p;
+          }
+      }
+      cur = ns_;
+      s = ns_;
+      k0 = nk;
+    }
+    TOut* Cb = C + (int64_t)v * c_vs + tk.c_off;
+    const int ldr = (n + 7) & ~7;
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int col = col0 + wn * WN + j * 16 + l16;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = row0 + wm * WM + i * 16 + lq * 4 + r;
+          if (row < m && col < n) {
+            const int64_t idx = c_rowmajor ? ((int64_t)row * ldr + col) : (row + (int64_t)col * m);
+            Cb[idx] = cvt<TOut>(acc[i][j][r]);
+          }
+        }
+      }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// W mix: dst = sum_t coef_t * src_t over 1024-element chunks (4 per thread).
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ float ldf(const T* p) {
+  return (float)*p;
+}
+template <>
+__device__ __forceinline__ float ldf<__hip_bfloat16>(const __hip_bfloat16* p) {
+  return __bfloat162float(*p);
+}
+
+template <typename T, typename TC>
+__global__ __launch_bounds__(256) void mix_kernel(const MixTask* __restrict__ tasks, int ntasks,
+                                                  const MixTerm* __restrict__ terms, int chunks_per_vec,
+                                                  int total_chunks, const T* __restrict__ src, T* __restrict__ dst,
+                                                  int64_t vs) {
+  for (int g = blockIdx.x; g < total_chunks; g += gridDim.x) {
+    const int v = g / chunks_per_vec;
+    const int c = g - v * chunks_per_vec;
+    const MixTask tk = tasks[find_task(tasks, ntasks, c, &MixTask::chunk0)];
+    const int64_t base = (int64_t)(c - tk.chunk0) * 1024;
+    const T* sv = src + (int64_t)v * vs;
+    T* dv = dst + (int64_t)v * vs + tk.dst_off;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int64_t e = base + threadIdx.x + r * 256;
+      if (e < tk.len) {
+        TC acc = (TC)0;
+        for (int q = 0; q < tk.nt; ++q) {
+          const MixTerm tm = terms[tk.t0 + q];
+          acc += (TC)tm.coef * (TC)ldf(sv + tm.src_off + e);
+        }
+        dv[e] = cvt<T>(acc);
+      }
+    }
+  }
+}
+template <>
+__global__ __launch_bounds__(256) void mix_kernel<double, double>(const MixTask* __restrict__ tasks, int ntasks,
+                                                                  const MixTerm* __restrict__ terms,
+                                                                  int chunks_per_vec, int total_chunks,
+                                                                  const double* __restrict__ src,
+                                                                  double* __restrict__ dst, int64_t vs) {
+  for (int g = blockIdx.x; g < total_chunks; g += gridDim.x) {
+    const int v = g / chunks_per_vec;
+    const int c = g - v * chunks_per_vec;
+    const MixTask tk = tasks[find_task(tasks, ntasks, c, &MixTask::chunk0)];
+    const int64_t base = (int64_t)(c - tk.chunk0) * 1024;
+    const double* sv = src + (int64_t)v * vs;
+    double* dv = dst + (int64_t)v * vs + tk.dst_off;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int64_t e = base + threadIdx.x + r * 256;
+      if (e < tk.len) {
+        double acc = 0.0;
+        for (int q = 0; q < tk.nt; ++q) {
+          const MixTerm tm = terms[tk.t0 + q];
+          acc += tm.coef * sv[tm.src_off + e];
+        }
+        dv[e] = acc;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+template <typename TO>
+__global__ void convert_kernel(const double* __restrict__ src, TO* __restrict__ dst, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    dst[i] = cvt<TO>(src[i]);
+}
+
+__global__ void zero_ranges_kernel(const Range* __restrict__ r, int nr, double* __restrict__ base, int64_t vs) {
+  const Range rg = r[blockIdx.y];
+  double* b = base + (int64_t)blockIdx.z * vs + rg.off;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < rg.len; i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = 0.0;
+}
+
+// Deterministic two-level dot: fixed partition, fixed tree inside a block,
+// partials summed in order by one block.
+__global__ __launch_bounds__(256) void dot_partial_kernel(const double* __restrict__ x, const double* __restrict__ y,
+                                                          int64_t n, int cplx, double* __restrict__ partial) {
+  __shared__ double sr[256], si[256];
+  double ar = 0.0, ai = 0.0;
+  if (!cplx) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+      ar += y[i] * x[i];
+  } else {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+      const double xr = x[2 * i], xi = x[2 * i + 1], yr = y[2 * i], yi = y[2 * i + 1];
+      ar += yr * xr + yi * xi;  // conj(y) * x
+      ai += yr * xi - yi * xr;
+    }
+  }
+  sr[threadIdx.x] = ar;
+  si[threadIdx.x] = ai;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sr[threadIdx.x] += sr[threadIdx.x + w];
+      si[threadIdx.x] += si[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = sr[0];
+    partial[2 * blockIdx.x + 1] = si[0];
+  }
+}
+__global__ void dot_final_kernel(const double* __restrict__ partial, int nparts, double* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double r = 0.0, i = 0.0;
+    for (int p = 0; p < nparts; ++p) {
+      r += partial[2 * p];
+      i += partial[2 * p + 1];
+    }
+    out[0] = r;
+    out[1] = i;
+  }
+}
+
+__global__ void axpby_kernel(double ar, double ai, const double* __restrict__ x, double br, double bi,
+                             double* __restrict__ y, int64_t n, int cplx) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    if (!cplx) {
+      y[i] = ar * x[i] + br * y[i];
+    } else {
+      const double xr = x[2 * i], xi = x[2 * i + 1], yr = y[2 * i], yi = y[2 * i + 1];
+      y[2 * i] = ar * xr - ai * xi + br * yr - bi * yi;
+      y[2 * i + 1] = ar * xi + ai * xr + br * yi + bi * yr;
+    }
+  }
+}
+
+
+// ---------------------------------------------------------------------------
+// Repack FP64 blocks into the compute layout (type conversion, optional
+// transpose, padded leading dimension).  One task per block, 1024-element
+// chunks of the source block.
+// ---------------------------------------------------------------------------
+template <typename TO>
+__global__ __launch_bounds__(256) void repack_kernel(const Repack* __restrict__ tasks, int ntasks, int chunks_per_vec,
+                                                     int total, const double* __restrict__ src, int64_t s_vs,
+                                                     TO* __restrict__ dst, int64_t d_vs) {
+  for (int g = blockIdx.x; g < total; g += gridDim.x) {
+    const int v = g / chunks_per_vec;
+    const int c = g - v * chunks_per_vec;
+    const Repack tk = tasks[find_task(tasks, ntasks, c, &Repack::chunk0)];
+    const int64_t len = (int64_t)tk.rows * tk.cols;
+    const int64_t base = (int64_t)(c - tk.chunk0) * 1024;
+    const double* s = src + (int64_t)v * s_vs + tk.src_off;
+    TO* d = dst + (int64_t)v * d_vs + tk.dst_off;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int64_t e = base + threadIdx.x + r * 256;
+      if (e < len) {
+        const int64_t i = e % tk.rows, j = e / tk.rows;
+        const int64_t di = tk.transpose ? (j + i * tk.dst_ld) : (i + j * tk.dst_ld);
+        d[di] = cvt<TO>(s[e]);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static int g_cu = 0;
+static int num_cu() {
+  if (!g_cu) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) == hipSuccess) g_cu = p.multiProcessorCount;
+    if (g_cu <= 0) g_cu = 256;
+  }
+  return g_cu;
+}
+
+int gemm_max_grid(int prec_in, int tile_class) {
+  (void)prec_in;
+  (void)tile_class;
+  return num_cu() * 2;
+}
+
+template <typename T, typename TOut, int BM, int BN>
+static hipError_t run16(int tr, const GTask* tasks, int ntasks, const GSeg* segs, int tpv, int nvec, const void* A,
+                        int64_t a_vs, const void* B, int64_t b_vs, void* C, int64_t c_vs, hipStream_t st) {
+  const int total = tpv * nvec;
+  if (total <= 0) return hipSuccess;
+  int grid = num_cu() * 2;
+  if (grid > total) grid = total;
+  if (grid >= 8) grid &= ~7;
+  const T* a = (const T*)A;
+  const T* b = (const T*)B;
+  TOut* c = (TOut*)C;
+  switch (tr) {
+    case 0: gemm16_kernel<T, TOut, BM, BN, false, false><<<grid, 256, 0, st>>>(tasks, ntasks, segs, tpv, total, a, a_vs, b, b_vs, c, c_vs); break;
+    case 1: gemm16_kernel<T, TOut, BM, BN, false, true><<<grid, 256, 0, st>>>(tasks, ntasks, segs, tpv, total, a, a_vs, b, b_vs, c, c_vs); break;
+    case 2: gemm16_kernel<T, TOut, BM, BN, true, false><<<grid, 256, 0, st>>>(tasks, ntasks, segs, tpv, total, a, a_vs, b, b_vs, c, c_vs); break;
+    default: gemm16_kernel<T, TOut, BM, BN, true, true><<<grid, 256, 0, st>>>(tasks, ntasks, segs, tpv, total, a, a_vs, b, b_vs, c, c_vs); break;
+  }
+  return hipGetLastError();
+}
+
+template <typename TOut, int BM, int BN>
+static hipError_t runbf(const GTask* tasks, int ntasks, const GSeg* segs, int tpv, int nvec, const void* A,
+                        int64_t a_vs, const void* B, int64_t b_vs, void* C, int64_t c_vs, int c_rowmajor,
+                        hipStream_t st) {
+  const int total = tpv * nvec;
+  if (total <= 0) return hipSuccess;
+  int grid = num_cu() * 2;
+  if (grid > total) grid = total;
+  if (grid >= 8) grid &= ~7;
+  gemm_bf16_kernel<TOut, BM, BN><<<grid, 256, 0, st>>>(tasks, ntasks, segs, tpv, total, (const __hip_bfloat16*)A,
+                                                       a_vs, (const __hip_bfloat16*)B, b_vs, (TOut*)C, c_vs,
+                                                       c_rowmajor);
+  return hipGetLastError();
+}
+
+hipError_t launch_gemm(int prec_in, int out_kind, int tr, int tile_class, const GTask* tasks, int ntasks,
+                       const GSeg* segs, int tiles_per_vec, int nvec, const void* A, int64_t a_vs, const void* B,
+                       int64_t b_vs, void* C, int64_t c_vs, hipStream_t st) {
+  if (prec_in == F64) {
+    if (tile_class == 0) return run16<double, double, 128, 128>(tr, tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, st);
+    return run16<double, double, 64, 64>(tr, tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, st);
+  }
+  if (prec_in == F32) {
+    if (out_kind == 1) {
+      if (tile_class == 0) return run16<float, double, 128, 128>(tr, tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, st);
+      return run16<float, double, 64, 64>(tr, tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, st);
+    }
+    if (tile_class == 0) return run16<float, float, 128, 128>(tr, tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, st);
+    return run16<float, float, 64, 64>(tr, tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, st);
+  }
+  // BF16: tr must be the canonical k-contiguous form; out_kind 0 -> bf16 row-major, 1 -> double col-major
+  if (out_kind == 1) {
+    if (tile_class == 0) return runbf<double, 128, 128>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, 0, st);
+    return runbf<double, 64, 64>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, 0, st);
+  }
+  if (tile_class == 0) return runbf<__hip_bfloat16, 128, 128>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, 1, st);
+  return runbf<__hip_bfloat16, 64, 64>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, 1, st);
+}
+
+hipError_t launch_mix(int prec, const MixTask* tasks, int ntasks, const MixTerm* terms, int chunks_per_vec, int nvec,
+                      const void* src, void* dst, int64_t vs, hipStream_t st) {
+  const int total = chunks_per_vec * nvec;
+  if (total <= 0 || ntasks <= 0) return hipSuccess;
+  int grid = total < 8192 ? total : 8192;
+  if (prec == F64)
+    mix_kernel<double, double><<<grid, 256, 0, st>>>(tasks, ntasks, terms, chunks_per_vec, total, (const double*)src, (double*)dst, vs);
+  else if (prec == F32)
+    mix_kernel<float, float><<<grid, 256, 0, st>>>(tasks, ntasks, terms, chunks_per_vec, total, (const float*)src, (float*)dst, vs);
+  else
+    mix_kernel<__hip_bfloat16, float><<<grid, 256, 0, st>>>(tasks, ntasks, terms, chunks_per_vec, total, (const __hip_bfloat16*)src, (__hip_bfloat16*)dst, vs);
+  return hipGetLastError();
+}
+
+hipError_t launch_convert(int prec, const double* src, void* dst, int64_t n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  int64_t blocks = (n + 255) / 256;
+  int grid = (int)(blocks < 16384 ? blocks : 16384);
+  if (prec == F32) convert_kernel<float><<<grid, 256, 0, st>>>(src, (float*)dst, n);
+  else if (prec == BF16) convert_kernel<__hip_bfloat16><<<grid, 256, 0, st>>>(src, (__hip_bfloat16*)dst, n);
+  else convert_kernel<double><<<grid, 256, 0, st>>>(src, (double*)dst, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_repack(int prec, const Repack* tasks, int ntasks, int chunks_per_vec, int nvec, const double* src,
+                         int64_t s_vs, void* dst, int64_t d_vs, hipStream_t st) {
+  const int total = chunks_per_vec * nvec;
+  if (total <= 0 || ntasks <= 0) return hipSuccess;
+  int grid = total < 8192 ? total : 8192;
+  if (prec == F32)
+    repack_kernel<float><<<grid, 256, 0, st>>>(tasks, ntasks, chunks_per_vec, total, src, s_vs, (float*)dst, d_vs);
+  else if (prec == BF16)
+    repack_kernel<__hip_bfloat16><<<grid, 256, 0, st>>>(tasks, ntasks, chunks_per_vec, total, src, s_vs, (__hip_bfloat16*)dst, d_vs);
+  else
+    repack_kernel<double><<<grid, 256, 0, st>>>(tasks, ntasks, chunks_per_vec, total, src, s_vs, (double*)dst, d_vs);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_ranges(const Range* r, int nr, double* base, int64_t vs, int nvec, hipStream_t st) {
+  if (nr <= 0 || nvec <= 0) return hipSuccess;
+  dim3 grid(64, nr, nvec);
+  zero_ranges_kernel<<<grid, 256, 0, st>>>(r, nr, base, vs);
+  return hipGetLastError();
+}
+
+hipError_t launch_dot(const double* x, const double* y, int64_t n, int cplx, double* partial, int nparts, double* out2,
+                      hipStream_t st) {
+  dot_partial_kernel<<<nparts, 256, 0, st>>>(x, y, n, cplx, partial);
+  dot_final_kernel<<<1, 64, 0, st>>>(partial, nparts, out2);
+  return hipGetLastError();
+}
+
+hipError_t launch_axpby(double ar, double ai, const double* x, double br, double bi, double* y, int64_t n, int cplx,
+                        hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  int64_t blocks = (n + 255) / 256;
+  int grid = (int)(blocks < 16384 ? blocks : 16384);
+  axpby_kernel<<<grid, 256, 0, st>>>(ar, ai, x, br, bi, y, n, cplx);
+  return hipGetLastError();
+}
+
+}  // namespace mpsa

@@ -1,0 +1,1032 @@
+// mpsa_plan.cpp -- sector plans and the C ABI (include/mpstools_amd.h).
+//
+// A plan flattens the reference's tuple loop into device task lists:
+//   contractTwosite / doContraction (src/MPSfunctions.hpp:726-804, 949-981)
+//   contractSingle  / doContraction (src/MPSfunctions.hpp:808-885)
+//   addContractionLayer / add_out_sub(_2) (src/MPSfunctions.hpp:276-555)
+// Each valid tuple (environment block, MPO entries, tensor block) of the
+// reference becomes a coefficient on a product; products that share the same
+// two operands are computed once (stage 1), the MPO coefficients are applied
+// by a mix, and every output block is one segmented-K product (stage 2).  The
+// tuples are enumerated in exactly the reference's loop order, so the output
+// key set is the reference's.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/mpstools_amd.h"
+#include "mpsa_internal.h"
+
+namespace mpsa {
+
+static thread_local std::string g_err;
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                        \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess) throw Error(MPSA_EHIP, std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// ------------------------------------------------------------------ tables
+struct BKey {
+  uint64_t q0, q1;
+  uint32_t p0, p1;
+  bool operator<(const BKey& o) const {
+    if (q0 != o.q0) return q0 < o.q0;
+    if (q1 != o.q1) return q1 < o.q1;
+    if (p0 != o.p0) return p0 < o.p0;
+    return p1 < o.p1;
+  }
+};
+
+struct Tab {
+  int64_t n = 0;
+  int ns = 1;
+  std::vector<BKey> key;
+  std::vector<int64_t> rows, cols, off;
+  std::map<BKey, int> idx;
+  int64_t total = 0;
+  int find(const BKey& k) const {
+    auto it = idx.find(k);
+    return it == idx.end() ? -1 : it->second;
+  }
+};
+
+static Tab parse(const mpsa_blocks* b, int ns_expect, const char* name) {
+  if (!b) throw Error(MPSA_EINVAL, std::string(name) + ": null table");
+  if (b->n < 0) throw Error(MPSA_EINVAL, std::string(name) + ": negative block count");
+  if (b->ns != ns_expect)
+    throw Error(MPSA_EINVAL, std::string(name) + ": expected ns=" + std::to_string(ns_expect));
+  if (b->is_complex) throw Error(MPSA_EINVAL, std::string(name) + ": complex tables are not supported by this build");
+  if (b->n > 0 && (!b->q || !b->p || !b->rows || !b->cols || !b->offset))
+    throw Error(MPSA_EINVAL, std::string(name) + ": null array");
+  Tab t;
+  t.n = b->n;
+  t.ns = b->ns;
+  for (int64_t i = 0; i < b->n; ++i) {
+    BKey k{b->q[2 * i], b->q[2 * i + 1], b->p[b->ns * i], b->ns == 2 ? b->p[b->ns * i + 1] : 0u};
+    if (b->rows[i] < 0 || b->cols[i] < 0 || b->offset[i] < 0 || b->rows[i] > (1 << 30) || b->cols[i] > (1 << 30))
+      throw Error(MPSA_EINVAL, std::string(name) + ": bad block shape/offset");
+    if (t.idx.count(k)) throw Error(MPSA_EINVAL, std::string(name) + ": duplicate key");
+    t.idx[k] = (int)i;
+    t.key.push_back(k);
+    t.rows.push_back(b->rows[i]);
+    t.cols.push_back(b->cols[i]);
+    t.off.push_back(b->offset[i]);
+    t.total = std::max(t.total, b->offset[i] + b->rows[i] * b->cols[i]);
+  }
+  return t;
+}
+
+struct Entry {
+  uint32_t in, out;
+  double v;
+};
+struct Group {
+  uint32_t wl, wr;
+  std::vector<Entry> e;
+};
+
+// MPOMat<1,T> in the reference's iteration order: ukey (wl, wr) ascending,
+// entries by OpKeyType order (in, out) (src/keyTypes.hpp:313-343, 471-483).
+static std::vector<Group> parse_mpo(const mpsa_mpo* w, int d, const char* name) {
+  if (!w) throw Error(MPSA_EINVAL, std::string(name) + ": null MPO");
+  if (w->is_complex) throw Error(MPSA_EINVAL, std::string(name) + ": complex MPO not supported by this build");
+  if (w->n < 0 || (w->n > 0 && (!w->wl || !w->wr || !w->out || !w->in || !w->val)))
+    throw Error(MPSA_EINVAL, std::string(name) + ": bad MPO arrays");
+  std::map<std::pair<uint32_t, uint32_t>, std::map<std::pair<uint32_t, uint32_t>, double>> m;
+  for (int64_t i = 0; i < w->n; ++i) {
+    if ((int)w->out[i] >= d || (int)w->in[i] >= d)
+      throw Error(MPSA_EINVAL, std::string(name) + ": physical index outside i_to_key");
+    m[{w->wl[i], w->wr[i]}][{w->in[i], w->out[i]}] = w->val[i];
+  }
+  std::vector<Group> g;
+  for (auto& kv : m) {
+    Group gr{kv.first.first, kv.first.second, {}};
+    for (auto& e : kv.second) gr.e.push_back(Entry{e.first.first, e.first.second, e.second});
+    g.push_back(gr);
+  }
+  return g;
+}
+
+// ------------------------------------------------------------------ plan
+struct Stage {
+  std::vector<GTask> tasks;
+  std::vector<GSeg> segs;
+  int tiles = 0;
+  int tile_class = 0;
+  int tr = 0;
+  GTask* d_tasks = nullptr;
+  GSeg* d_segs = nullptr;
+};
+
+struct Product {  // a T1 slot: C(m x n) = op(P) op(Q)
+  int ia, ib;
+  int64_t m, n, k;
+  int64_t ws_off;
+};
+struct T2Slot {
+  int iout, iu;
+  int64_t m, n;
+  std::vector<std::pair<int, double>> terms;  // (T1 slot, coef)
+  int64_t ws_off = -1;                        // -1: single term, read the T1 slot directly
+};
+
+}  // namespace mpsa
+
+using namespace mpsa;
+
+struct mpsa_plan {
+  int kind = 0;  // 0: heff (two- or one-site), 2: env
+  int prec = F64;
+  int direction = 1;
+  // layout facts
+  int64_t L_total = 0, R_total = 0, x_total = 0, y_total = 0;  // heff: L, R, x, y; env: BL, A, Ap, out
+  int64_t ws_per_vec = 0;                                     // workspace elements per vector (compute type)
+  int64_t xc_per_vec = 0;                                     // converted x elements per vector
+  size_t esize = 8;
+  Stage s1, s2;
+  std::vector<MixTask> mix;
+  std::vector<MixTerm> terms;
+  int mix_chunks = 0;
+  std::vector<Range> zero;
+  std::vector<Repack> rpL, rpR, rpx;
+  int rpL_chunks = 0, rpR_chunks = 0, rpx_chunks = 0;
+  int64_t Lc_elems = 0, Rc_elems = 0;
+  // device
+  MixTask* d_mix = nullptr;
+  MixTerm* d_terms = nullptr;
+  Range* d_zero = nullptr;
+  Repack *d_rpL = nullptr, *d_rpR = nullptr, *d_rpx = nullptr;
+  void* ws = nullptr;
+  int ws_nvec = 0;
+  void* xc = nullptr;
+  int xc_nvec = 0;
+  void* Lc = nullptr;
+  void* Rc = nullptr;
+  const double* dL = nullptr;
+  const double* dR = nullptr;
+  double flops_alg = 0, flops_ref = 0;
+  double flops1 = 0, flops2 = 0;  // algorithmic flops of stage 1 / stage 2 (one vector)
+  int timing = 0;
+  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  ~mpsa_plan() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+    for (void* p : {(void*)s1.d_tasks, (void*)s1.d_segs, (void*)s2.d_tasks, (void*)s2.d_segs, (void*)d_mix,
+                    (void*)d_terms, (void*)d_zero, (void*)d_rpL, (void*)d_rpR, (void*)d_rpx, ws, xc, Lc, Rc})
+      if (p) (void)hipFree(p);
+  }
+};
+
+namespace mpsa {
+
+template <typename T>
+static T* upload(const std::vector<T>& v) {
+  if (v.empty()) return nullptr;
+  T* d = nullptr;
+  if (hipMalloc((void**)&d, sizeof(T) * v.size()) != hipSuccess) throw Error(MPSA_ENOMEM, "hipMalloc descriptors");
+  HIPCHK(hipMemcpy(d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+  return d;
+}
+
+static int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
+static int64_t pad8(int64_t x) { return (x + 7) & ~int64_t(7); }
+
+// Tile class: 128x128 when it wastes little, 64x64 otherwise.
+static int pick_class(const std::vector<GTask>& t) {
+  double useful = 0, big = 0;
+  for (auto& x : t) {
+    useful += (double)x.m * x.n;
+    big += (double)((x.m + 127) / 128) * ((x.n + 127) / 128) * 128.0 * 128.0;
+  }
+  if (big <= 0) return 0;
+  return useful / big >= 0.7 ? 0 : 1;
+}
+
+static void finalize_stage(Stage& s) {
+  s.tile_class = pick_class(s.tasks);
+  const int BM = s.tile_class == 0 ? 128 : 64, BN = BM;
+  // longest tiles first (static round-robin over the persistent grid)
+  std::vector<int64_t> cost(s.tasks.size());
+  for (size_t i = 0; i < s.tasks.size(); ++i) {
+    int64_t k = 0;
+    for (int j = 0; j < s.tasks[i].nseg; ++j) k += s.segs[(size_t)(s.tasks[i].seg0 + j)].k;
+    cost[i] = k;
+  }
+  std::vector<size_t> ord(s.tasks.size());
+  for (size_t i = 0; i < ord.size(); ++i) ord[i] = i;
+  std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+  std::vector<GTask> t2;
+  int tile = 0;
+  for (size_t i : ord) {
+    GTask t = s.tasks[i];
+    if (t.m <= 0 || t.n <= 0) continue;
+    t.tiles_m = (t.m + BM - 1) / BM;
+    const int tiles_n = (t.n + BN - 1) / BN;
+    t.tile0 = tile;
+    tile += t.tiles_m * tiles_n;
+    t2.push_back(t);
+  }
+  s.tasks = t2;
+  s.tiles = tile;
+  s.d_tasks = upload(s.tasks);
+  s.d_segs = upload(s.segs);
+}
+
+static void finalize_mix(mpsa_plan* P) {
+  int chunk = 0;
+  std::vector<MixTask> m2;
+  for (auto t : P->mix) {
+    if (t.len <= 0) continue;
+    t.chunk0 = chunk;
+    chunk += (int)((t.len + 1023) / 1024);
+    m2.push_back(t);
+  }
+  P->mix = m2;
+  P->mix_chunks = chunk;
+  P->d_mix = upload(P->mix);
+  P->d_terms = upload(P->terms);
+}
+
+static int finalize_repack(std::vector<Repack>& r) {
+  int chunk = 0;
+  std::vector<Repack> r2;
+  for (auto t : r) {
+    if ((int64_t)t.rows * t.cols <= 0) continue;
+    t.chunk0 = chunk;
+    chunk += (int)(((int64_t)t.rows * t.cols + 1023) / 1024);
+    r2.push_back(t);
+  }
+  r = r2;
+  return chunk;
+}
+
+// The generic two-stage builder.  Tuples come from the enumerators below.
+struct Tuple {
+  int ia, ib, iu;  // stage-1 operands (P, Q) and stage-2 operand U
+  BKey out;
+  double coef;
+  int64_t m1, n1, k1;  // T1 dims
+  int64_t n_out_rows, n_out_cols;
+};
+
+struct Built {
+  std::vector<Product> prods;
+  std::vector<T2Slot> slots;
+  std::vector<BKey> outkeys;  // produced keys in key order
+  double flops_ref = 0;
+};
+
+static Built group_tuples(const std::vector<Tuple>& tup, const Tab* ytab, bool need_y,
+                          std::map<BKey, std::pair<int64_t, int64_t>>* out_dims) {
+  Built b;
+  std::map<std::pair<int, int>, int> pidx;
+  std::map<std::pair<int, int>, int> sidx;  // (iout, iu) -> slot; iout = y index or key ordinal
+  std::map<BKey, int> outord;
+  for (auto& t : tup) {
+    auto od = out_dims->find(t.out);
+    if (od == out_dims->end()) (*out_dims)[t.out] = {t.n_out_rows, t.n_out_cols};
+    else if (od->second.first != t.n_out_rows || od->second.second != t.n_out_cols)
+      throw Error(MPSA_EINVAL, "inconsistent output block shapes for one key");
+    if (!outord.count(t.out)) outord[t.out] = 0;
+  }
+  int ord = 0;
+  for (auto& kv : outord) kv.second = ord++;
+  for (auto& kv : outord) b.outkeys.push_back(kv.first);
+  if (!need_y) return b;
+  for (auto& t : tup) {
+    int iout = ytab->find(t.out);
+    if (iout < 0) throw Error(MPSA_ENOKEY, "output table lacks a produced key");
+    auto pk = std::make_pair(t.ia, t.ib);
+    int pi;
+    auto it = pidx.find(pk);
+    if (it == pidx.end()) {
+      pi = (int)b.prods.size();
+      pidx[pk] = pi;
+      b.prods.push_back(Product{t.ia, t.ib, t.m1, t.n1, t.k1, 0});
+    } else {
+      pi = it->second;
+    }
+    auto sk = std::make_pair(iout, t.iu);
+    int si;
+    auto jt = sidx.find(sk);
+    if (jt == sidx.end()) {
+      si = (int)b.slots.size();
+      sidx[sk] = si;
+      T2Slot s;
+      s.iout = iout;
+      s.iu = t.iu;
+      s.m = t.m1;
+      s.n = t.n1;
+      b.slots.push_back(s);
+    } else {
+      si = jt->second;
+    }
+    auto& terms = b.slots[(size_t)si].terms;
+    bool found = false;
+    for (auto& tm : terms)
+      if (tm.first == pi) {
+        tm.second += t.coef;
+        found = true;
+      }
+    if (!found) terms.push_back({pi, t.coef});
+  }
+  return b;
+}
+
+// ---- tuple enumerators (reference loop order) ----
+
+// Two-site: doContraction (src/MPSfunctions.hpp:726-804) over left charges in
+// ascending order (contractTwosite :958-976).
+static std::vector<Tuple> tuples_heff2(const Tab& L, const Tab& R, const Tab& x, const std::vector<Group>& wl,
+                                       const std::vector<Group>& wr, const uint64_t* itl, const uint64_t* itr,
+                                       double* fref) {
+  std::vector<Tuple> out;
+  *fref = 0;
+  for (int iL = 0; iL < (int)L.n; ++iL) {
+    const BKey& kl = L.key[(size_t)iL];
+    for (auto& o1 : wl) {
+      for (auto& o2 : wr) {
+        if (o1.wl != kl.p0) continue;
+        if (o2.wl != o1.wr) continue;
+        for (auto& a : o1.e) {
+          for (auto& b : o2.e) {
+            const uint64_t kylo = itl[a.out] + itr[b.out];
+            int ix = x.find(BKey{kl.q1, kl.q1 + kylo, a.out, b.out});
+            if (ix < 0) continue;
+            const uint64_t kyup = itl[a.in] + itr[b.in];
+            int iR = R.find(BKey{kl.q0 + kyup, kl.q1 + kylo, o2.wr, 0});
+            if (iR < 0) continue;
+            if (L.cols[(size_t)iL] != x.rows[(size_t)ix] || R.cols[(size_t)iR] != x.cols[(size_t)ix])
+              throw Error(MPSA_EINVAL, "heff2: block dimensions of L, psi, R do not match");
+            Tuple t;
+            t.ia = iL;
+            t.ib = ix;
+            t.iu = iR;
+            t.out = BKey{kl.q0, R.key[(size_t)iR].q0, a.in, b.in};
+            t.coef = a.v * b.v;
+            t.m1 = L.rows[(size_t)iL];
+            t.n1 = x.cols[(size_t)ix];
+            t.k1 = L.cols[(size_t)iL];
+            t.n_out_rows = L.rows[(size_t)iL];
+            t.n_out_cols = R.rows[(size_t)iR];
+            *fref += 2.0 * t.m1 * t.k1 * t.n1 + 2.0 * t.m1 * t.n1 * t.n_out_cols;
+            out.push_back(t);
+          }
+        }
+      }
+    }
+  }
+  return out;
+}
+
+// One-site: doContraction (src/MPSfunctions.hpp:808-854).
+static std::vector<Tuple> tuples_heff1(const Tab& L, const Tab& R, const Tab& x, const std::vector<Group>& w,
+                                       const uint64_t* itk, double* fref) {
+  std::vector<Tuple> out;
+  *fref = 0;
+  for (int iL = 0; iL < (int)L.n; ++iL) {
+    const BKey& kl = L.key[(size_t)iL];
+    for (auto& o : w) {
+      if (o.wl != kl.p0) continue;
+      for (auto& a : o.e) {
+        const uint64_t kylo = itk[a.out];
+        int ix = x.find(BKey{kl.q1, kl.q1 + kylo, a.out, 0});
+        if (ix < 0) continue;
+        const uint64_t kyup = itk[a.in];
+        int iR = R.find(BKey{kl.q0 + kyup, kl.q1 + kylo, o.wr, 0});
+        if (iR < 0) continue;
+        if (L.cols[(size_t)iL] != x.rows[(size_t)ix] || R.cols[(size_t)iR] != x.cols[(size_t)ix])
+          throw Error(MPSA_EINVAL, "heff1: block dimensions of L, A, R do not match");
+        Tuple t;
+        t.ia = iL;
+        t.ib = ix;
+        t.iu = iR;
+        t.out = BKey{kl.q0, R.key[(size_t)iR].q0, a.in, 0};
+        t.coef = a.v;
+        t.m1 = L.rows[(size_t)iL];
+        t.n1 = x.cols[(size_t)ix];
+        t.k1 = L.cols[(size_t)iL];
+        t.n_out_rows = L.rows[(size_t)iL];
+        t.n_out_cols = R.rows[(size_t)iR];
+        *fref += 2.0 * t.m1 * t.k1 * t.n1 + 2.0 * t.m1 * t.n1 * t.n_out_cols;
+        out.push_back(t);
+      }
+    }
+  }
+  return out;
+}
+
+// Environment: add_out_sub(_2) (src/MPSfunctions.hpp:276-435) for every
+// outgoing charge in ascending order (addContractionLayer :455-488).
+static std::vector<Tuple> tuples_env(const Tab& BL, const Tab& A, const Tab& Aup, const std::vector<Group>& w,
+                                     const uint64_t* itk, int direction, double* fref) {
+  std::vector<Tuple> out;
+  *fref = 0;
+  const int idir = direction > 0 ? 1 : 0;
+  std::map<uint64_t, int> outgoing;
+  for (auto& k : A.key) outgoing[idir ? k.q1 : k.q0] = 1;
+  std::map<uint32_t, std::vector<int>> blits;
+  for (int i = 0; i < (int)BL.n; ++i) blits[BL.key[(size_t)i].p0].push_back(i);
+  for (auto& pk : outgoing) {
+    const uint64_t p = pk.first;
+    for (auto& o : w) {
+      const uint32_t ko[2] = {o.wl, o.wr};
+      const uint32_t o_in = ko[1 - idir];
+      for (auto& oi : o.e) {
+        const uint64_t kyout = itk[oi.out];
+        BKey kb = direction > 0 ? BKey{p - kyout, p, oi.out, 0} : BKey{p, p + kyout, oi.out, 0};
+        int ib = A.find(kb);
+        if (ib < 0) continue;
+        auto bl = blits.find(o_in);
+        if (bl == blits.end()) continue;
+        for (int iBL : bl->second) {
+          const BKey& k = BL.key[(size_t)iBL];
+          if (direction > 0 && k.p0 != ko[0]) continue;
+          if (direction <= 0 && k.p0 != ko[1]) continue;
+          if (direction > 0 && k.q1 != kb.q0) continue;
+          if (direction <= 0 && k.q1 != kb.q1) continue;
+          const uint64_t kyin = itk[oi.in];
+          BKey ka = direction > 0 ? BKey{k.q0, k.q0 + kyin, oi.in, 0} : BKey{k.q0 - kyin, k.q0, oi.in, 0};
+          int ia = Aup.find(ka);
+          if (ia < 0) continue;
+          Tuple t;
+          t.ia = iBL;
+          t.ib = ib;
+          t.iu = ia;
+          t.out = BKey{idir ? ka.q1 : ka.q0, idir ? kb.q1 : kb.q0, ko[idir], 0};
+          t.coef = oi.v;
+          const int64_t ma = BL.rows[(size_t)iBL], mb = BL.cols[(size_t)iBL];
+          if (direction > 0) {
+            // T1 = BL (ma x mb) * A_low (mb x mb'); out = Aup^T (ma' x ma) * T1
+            if (A.rows[(size_t)ib] != mb || Aup.rows[(size_t)ia] != ma)
+              throw Error(MPSA_EINVAL, "env: block dimensions do not match");
+            t.m1 = ma;
+            t.n1 = A.cols[(size_t)ib];
+            t.k1 = mb;
+            t.n_out_rows = Aup.cols[(size_t)ia];
+            t.n_out_cols = A.cols[(size_t)ib];
+          } else {
+            // T1 = BR (ma x mb) * A_low^T (mb x mb'); out = Aup (ma' x ma) * T1
+            if (A.cols[(size_t)ib] != mb || Aup.cols[(size_t)ia] != ma)
+              throw Error(MPSA_EINVAL, "env: block dimensions do not match");
+            t.m1 = ma;
+            t.n1 = A.rows[(size_t)ib];
+            t.k1 = mb;
+            t.n_out_rows = Aup.rows[(size_t)ia];
+            t.n_out_cols = A.rows[(size_t)ib];
+          }
+          *fref += 2.0 * t.k1 * ma * t.n_out_rows + 2.0 * t.n_out_rows * ma * t.n1;
+          out.push_back(t);
+        }
+      }
+    }
+  }
+  return out;
+}
+
+static int64_t write_keys(const std::vector<BKey>& keys, const std::map<BKey, std::pair<int64_t, int64_t>>& dims,
+                          int ns, int64_t cap, int64_t* nout, uint64_t* q, uint32_t* p, int64_t* rows, int64_t* cols) {
+  *nout = (int64_t)keys.size();
+  if (cap <= 0) return 0;
+  if (cap < (int64_t)keys.size()) throw Error(MPSA_EINVAL, "outkeys: capacity too small");
+  for (size_t i = 0; i < keys.size(); ++i) {
+    q[2 * i] = keys[i].q0;
+    q[2 * i + 1] = keys[i].q1;
+    p[(size_t)ns * i] = keys[i].p0;
+    if (ns == 2) p[(size_t)ns * i + 1] = keys[i].p1;
+    auto d = dims.at(keys[i]);
+    if (rows) rows[i] = d.first;
+    if (cols) cols[i] = d.second;
+  }
+  return 0;
+}
+
+// Build the device plan of a two-stage contraction.
+//   heff: stage 1 T1 = P * Q  (P = L env, Q = x),  stage 2 y = T2 * U^T (U = R env)
+//   env>0: T1 = P * Q (P = BL, Q = A), out = U^T * T2 (U = Ap)
+//   env<=0: T1 = P * Q^T (P = BR, Q = A), out = U * T2 (U = Ap)
+static void build_plan(mpsa_plan* P, Built& b, const Tab& Ptab, const Tab& Qtab, const Tab& Utab, const Tab& Ytab,
+                       int kind, int direction) {
+  const bool bf = P->prec == BF16;
+  // workspace layout
+  int64_t off = 0;
+  for (auto& pr : b.prods) {
+    pr.ws_off = off;
+    off = align_up(off + pr.m * (bf ? pad8(pr.n) : pr.n), 64);
+  }
+  for (auto& s : b.slots) {
+    if (s.terms.size() == 1) continue;
+    s.ws_off = off;
+    off = align_up(off + s.m * (bf ? pad8(s.n) : s.n), 64);
+  }
+  P->ws_per_vec = std::max<int64_t>(off, 64);
+  P->flops_alg = 0;
+  for (auto& pr : b.prods) P->flops_alg += 2.0 * pr.m * pr.n * pr.k;
+  P->flops1 = P->flops_alg;
+
+  // bf16: converted operands with padded, k-contiguous layouts
+  std::vector<int64_t> Loff(Ptab.n), Roff(Utab.n), xoff(Qtab.n);
+  if (bf) {
+    int64_t o = 0;
+    for (int i = 0; i < (int)Ptab.n; ++i) {  // L^T: rows*pad8(cols) (element (i,k) at i*pad8(K)+k)
+      Loff[(size_t)i] = o;
+      P->rpL.push_back(Repack{Ptab.off[(size_t)i], o, (int32_t)Ptab.rows[(size_t)i], (int32_t)Ptab.cols[(size_t)i],
+                              (int32_t)pad8(Ptab.cols[(size_t)i]), 1, 0, 0});
+      o = align_up(o + Ptab.rows[(size_t)i] * pad8(Ptab.cols[(size_t)i]), 64);
+    }
+    P->Lc_elems = std::max<int64_t>(o, 64);
+    o = 0;
+    for (int i = 0; i < (int)Utab.n; ++i) {  // R row-major: element (j,k) at j*pad8(cols)+k
+      Roff[(size_t)i] = o;
+      P->rpR.push_back(Repack{Utab.off[(size_t)i], o, (int32_t)Utab.rows[(size_t)i], (int32_t)Utab.cols[(size_t)i],
+                              (int32_t)pad8(Utab.cols[(size_t)i]), 1, 0, 0});
+      o = align_up(o + Utab.rows[(size_t)i] * pad8(Utab.cols[(size_t)i]), 64);
+    }
+    P->Rc_elems = std::max<int64_t>(o, 64);
+    o = 0;
+    for (int i = 0; i < (int)Qtab.n; ++i) {  // x col-major with ld pad8(rows)
+      xoff[(size_t)i] = o;
+      P->rpx.push_back(Repack{Qtab.off[(size_t)i], o, (int32_t)Qtab.rows[(size_t)i], (int32_t)Qtab.cols[(size_t)i],
+                              (int32_t)pad8(Qtab.rows[(size_t)i]), 0, 0, 0});
+      o = align_up(o + pad8(Qtab.rows[(size_t)i]) * Qtab.cols[(size_t)i], 64);
+    }
+    P->xc_per_vec = std::max<int64_t>(o, 64);
+  } else if (P->prec == F32) {
+    for (int i = 0; i < (int)Ptab.n; ++i) {
+      Loff[(size_t)i] = Ptab.off[(size_t)i];
+      P->rpL.push_back(Repack{Ptab.off[(size_t)i], Ptab.off[(size_t)i], (int32_t)Ptab.rows[(size_t)i],
+                              (int32_t)Ptab.cols[(size_t)i], (int32_t)Ptab.rows[(size_t)i], 0, 0, 0});
+    }
+    for (int i = 0; i < (int)Utab.n; ++i) {
+      Roff[(size_t)i] = Utab.off[(size_t)i];
+      P->rpR.push_back(Repack{Utab.off[(size_t)i], Utab.off[(size_t)i], (int32_t)Utab.rows[(size_t)i],
+                              (int32_t)Utab.cols[(size_t)i], (int32_t)Utab.rows[(size_t)i], 0, 0, 0});
+    }
+    for (int i = 0; i < (int)Qtab.n; ++i) {
+      xoff[(size_t)i] = Qtab.off[(size_t)i];
+      P->rpx.push_back(Repack{Qtab.off[(size_t)i], Qtab.off[(size_t)i], (int32_t)Qtab.rows[(size_t)i],
+                              (int32_t)Qtab.cols[(size_t)i], (int32_t)Qtab.rows[(size_t)i], 0, 0, 0});
+    }
+    P->Lc_elems = std::max<int64_t>(P->L_total, 64);
+    P->Rc_elems = std::max<int64_t>(P->R_total, 64);
+    P->xc_per_vec = std::max<int64_t>(P->x_total, 64);
+  } else {
+    for (int i = 0; i < (int)Ptab.n; ++i) Loff[(size_t)i] = Ptab.off[(size_t)i];
+    for (int i = 0; i < (int)Utab.n; ++i) Roff[(size_t)i] = Utab.off[(size_t)i];
+    for (int i = 0; i < (int)Qtab.n; ++i) xoff[(size_t)i] = Qtab.off[(size_t)i];
+  }
+
+  // stage 1
+  for (auto& pr : b.prods) {
+    GTask t{};
+    t.c_off = pr.ws_off;
+    t.m = (int32_t)pr.m;
+    t.n = (int32_t)pr.n;
+    t.seg0 = (int32_t)P->s1.segs.size();
+    t.nseg = 1;
+    GSeg s{};
+    s.a_off = Loff[(size_t)pr.ia];
+    s.b_off = xoff[(size_t)pr.ib];
+    s.k = (int32_t)pr.k;
+    s.alpha = 1.0;
+    if (bf) {
+      s.lda = (int32_t)pad8(Ptab.cols[(size_t)pr.ia]);
+      s.ldb = (int32_t)pad8(Qtab.rows[(size_t)pr.ib]);
+    } else {
+      s.lda = (int32_t)std::max<int64_t>(1, Ptab.rows[(size_t)pr.ia]);
+      s.ldb = (int32_t)std::max<int64_t>(1, Qtab.rows[(size_t)pr.ib]);
+    }
+    P->s1.segs.push_back(s);
+    P->s1.tasks.push_back(t);
+  }
+  // heff: NN; env left: NN; env right: NT
+  P->s1.tr = (kind == 2 && direction <= 0) ? 1 : 0;
+
+  // mix
+  for (auto& sl : b.slots) {
+    if (sl.terms.size() == 1) continue;
+    MixTask mt{};
+    mt.dst_off = sl.ws_off;
+    mt.len = sl.m * (bf ? pad8(sl.n) : sl.n);
+    mt.t0 = (int32_t)P->terms.size();
+    mt.nt = (int32_t)sl.terms.size();
+    for (auto& tm : sl.terms) P->terms.push_back(MixTerm{b.prods[(size_t)tm.first].ws_off, tm.second});
+    P->mix.push_back(mt);
+  }
+
+  // stage 2: one task per produced output block; segments in slot order
+  std::map<int, std::vector<int>> per_out;
+  for (int i = 0; i < (int)b.slots.size(); ++i) per_out[b.slots[(size_t)i].iout].push_back(i);
+  std::vector<char> produced((size_t)Ytab.n, 0);
+  for (auto& kv : per_out) {
+    const int iy = kv.first;
+    produced[(size_t)iy] = 1;
+    GTask t{};
+    t.c_off = Ytab.off[(size_t)iy];
+    t.m = (int32_t)Ytab.rows[(size_t)iy];
+    t.n = (int32_t)Ytab.cols[(size_t)iy];
+    t.seg0 = (int32_t)P->s2.segs.size();
+    t.nseg = (int32_t)kv.second.size();
+    for (int si : kv.second) {
+      const T2Slot& sl = b.slots[(size_t)si];
+      GSeg s{};
+      int64_t toff;
+      double alpha;
+      if (sl.terms.size() == 1) {
+        toff = b.prods[(size_t)sl.terms[0].first].ws_off;
+        alpha = sl.terms[0].second;
+      } else {
+        toff = sl.ws_off;
+        alpha = 1.0;
+      }
+      if (kind != 2) {  // y = T * R^T: A = T (m x n_j), B = R (y.cols x n_j), k = n_j
+        s.a_off = toff;
+        s.b_off = Roff[(size_t)sl.iu];
+        s.k = (int32_t)sl.n;
+        s.lda = (int32_t)(bf ? pad8(sl.n) : std::max<int64_t>(1, sl.m));
+        s.ldb = (int32_t)(bf ? pad8(Utab.cols[(size_t)sl.iu]) : std::max<int64_t>(1, Utab.rows[(size_t)sl.iu]));
+        P->flops_alg += 2.0 * t.m * t.n * sl.n;
+      } else {  // out = op(U) * T: A = U, B = T (m_a x n), k = m_a
+        s.a_off = Utab.off[(size_t)sl.iu];
+        s.b_off = toff;
+        s.k = (int32_t)sl.m;
+        s.lda = (int32_t)std::max<int64_t>(1, Utab.rows[(size_t)sl.iu]);
+        s.ldb = (int32_t)std::max<int64_t>(1, sl.m);
+        P->flops_alg += 2.0 * t.m * t.n * sl.m;
+      }
+      s.alpha = alpha;
+      P->s2.segs.push_back(s);
+    }
+    P->s2.tasks.push_back(t);
+  }
+  P->flops2 = P->flops_alg - P->flops1;
+  if (kind != 2) P->s2.tr = 1;                // N, T
+  else P->s2.tr = direction > 0 ? 2 : 0;      // T, N  /  N, N
+  for (int i = 0; i < (int)Ytab.n; ++i)
+    if (!produced[(size_t)i] && Ytab.rows[(size_t)i] * Ytab.cols[(size_t)i] > 0)
+      P->zero.push_back(Range{Ytab.off[(size_t)i], Ytab.rows[(size_t)i] * Ytab.cols[(size_t)i]});
+
+  finalize_stage(P->s1);
+  finalize_stage(P->s2);
+  finalize_mix(P);
+  P->d_zero = upload(P->zero);
+  P->rpL_chunks = finalize_repack(P->rpL);
+  P->rpR_chunks = finalize_repack(P->rpR);
+  P->rpx_chunks = finalize_repack(P->rpx);
+  P->d_rpL = upload(P->rpL);
+  P->d_rpR = upload(P->rpR);
+  P->d_rpx = upload(P->rpx);
+  P->esize = P->prec == F64 ? 8 : (P->prec == F32 ? 4 : 2);
+}
+
+static void ensure_ws(mpsa_plan* P, int nvec) {
+  if (P->ws_nvec < nvec) {
+    if (P->ws) HIPCHK(hipFree(P->ws));
+    P->ws = nullptr;
+    const size_t bytes = (size_t)P->ws_per_vec * P->esize * (size_t)nvec;
+    if (hipMalloc(&P->ws, bytes) != hipSuccess) {
+      P->ws_nvec = 0;
+      throw Error(MPSA_ENOMEM, "workspace allocation failed");
+    }
+    HIPCHK(hipMemset(P->ws, 0, bytes));  // padding lanes must read as zero
+    P->ws_nvec = nvec;
+  }
+  if (P->prec != F64 && P->xc_nvec < nvec) {
+    if (P->xc) HIPCHK(hipFree(P->xc));
+    P->xc = nullptr;
+    const size_t bytes = (size_t)P->xc_per_vec * P->esize * (size_t)nvec;
+    if (hipMalloc(&P->xc, bytes) != hipSuccess) {
+      P->xc_nvec = 0;
+      throw Error(MPSA_ENOMEM, "converted-vector allocation failed");
+    }
+    HIPCHK(hipMemset(P->xc, 0, bytes));
+    P->xc_nvec = nvec;
+  }
+}
+
+}  // namespace mpsa
+
+// =================================================================== C ABI
+#define MPSA_TRY(...)                          \
+  try {                                        \
+    __VA_ARGS__;                               \
+    return MPSA_OK;                            \
+  } catch (const mpsa::Error& e) {             \
+    mpsa::g_err = e.what();                    \
+    return e.code;                             \
+  } catch (const std::exception& e) {          \
+    mpsa::g_err = e.what();                    \
+    return MPSA_EINVAL;                        \
+  }
+
+extern "C" {
+
+const char* mpsa_last_error(void) { return mpsa::g_err.c_str(); }
+const char* mpsa_version(void) { return "mpstools_amd 0.1 gfx950"; }
+
+static void check_itk(const uint64_t* itk, int d, const char* name) {
+  if (!itk || d <= 0) throw Error(MPSA_EINVAL, std::string(name) + ": empty i_to_key");
+}
+
+int mpsa_heff2_outkeys(const mpsa_blocks* L, const mpsa_blocks* R, const mpsa_blocks* x, const mpsa_mpo* Wl,
+                       const mpsa_mpo* Wr, const uint64_t* itl, int dl, const uint64_t* itr, int dr, int64_t cap,
+                       int64_t* nout, uint64_t* q, uint32_t* p, int64_t* rows, int64_t* cols) {
+  MPSA_TRY({
+    check_itk(itl, dl, "itokey_l");
+    check_itk(itr, dr, "itokey_r");
+    Tab tl = parse(L, 1, "L"), tr = parse(R, 1, "R"), tx = parse(x, 2, "x");
+    auto gl = parse_mpo(Wl, dl, "Wl"), gr = parse_mpo(Wr, dr, "Wr");
+    double fref;
+    auto tup = tuples_heff2(tl, tr, tx, gl, gr, itl, itr, &fref);
+    std::map<BKey, std::pair<int64_t, int64_t>> dims;
+    Built b = group_tuples(tup, nullptr, false, &dims);
+    write_keys(b.outkeys, dims, 2, cap, nout, q, p, rows, cols);
+  })
+}
+
+int mpsa_heff1_outkeys(const mpsa_blocks* L, const mpsa_blocks* R, const mpsa_blocks* x, const mpsa_mpo* W,
+                       const uint64_t* itk, int d, int64_t cap, int64_t* nout, uint64_t* q, uint32_t* p,
+                       int64_t* rows, int64_t* cols) {
+  MPSA_TRY({
+    check_itk(itk, d, "itokey");
+    Tab tl = parse(L, 1, "L"), tr = parse(R, 1, "R"), tx = parse(x, 1, "x");
+    auto g = parse_mpo(W, d, "W");
+    double fref;
+    auto tup = tuples_heff1(tl, tr, tx, g, itk, &fref);
+    std::map<BKey, std::pair<int64_t, int64_t>> dims;
+    Built b = group_tuples(tup, nullptr, false, &dims);
+    write_keys(b.outkeys, dims, 1, cap, nout, q, p, rows, cols);
+  })
+}
+
+int mpsa_env_outkeys(const mpsa_blocks* BL, const mpsa_blocks* A, const mpsa_blocks* Ap, const mpsa_mpo* W,
+                     const uint64_t* itk, int d, int direction, int64_t cap, int64_t* nout, uint64_t* q, uint32_t* p,
+                     int64_t* rows, int64_t* cols) {
+  MPSA_TRY({
+    check_itk(itk, d, "itokey");
+    Tab tb = parse(BL, 1, "BL"), ta = parse(A, 1, "A");
+    if (ta.n == 0) throw Error(MPSA_EINVAL, "Error in addContractionLayer: A is empty.");
+    Tab tap = Ap ? parse(Ap, 1, "Ap") : ta;
+    if (Ap && tap.n == 0) throw Error(MPSA_EINVAL, "Error in addContractionLayer: Ap is empty.");
+    auto g = parse_mpo(W, d, "W");
+    double fref;
+    auto tup = tuples_env(tb, ta, tap, g, itk, direction, &fref);
+    std::map<BKey, std::pair<int64_t, int64_t>> dims;
+    Built b = group_tuples(tup, nullptr, false, &dims);
+    write_keys(b.outkeys, dims, 1, cap, nout, q, p, rows, cols);
+  })
+}
+
+static int plan_heff(mpsa_plan** plan, const mpsa_blocks* L, const mpsa_blocks* R, const mpsa_blocks* x,
+                     const mpsa_blocks* y, const std::vector<Tuple>& tup, double fref, int nsx, int precision) {
+  (void)nsx;
+  if (!plan) throw Error(MPSA_EINVAL, "null plan pointer");
+  if (precision < 0 || precision > 2) throw Error(MPSA_EINVAL, "precision must be MPSA_F64, MPSA_F32 or MPSA_BF16");
+  std::unique_ptr<mpsa_plan> P(new mpsa_plan());
+  P->kind = 0;
+  P->prec = precision;
+  Tab tl = parse(L, 1, "L"), tr = parse(R, 1, "R"), tx = parse(x, x->ns, "x"), ty = parse(y, y->ns, "y");
+  P->L_total = tl.total;
+  P->R_total = tr.total;
+  P->x_total = tx.total;
+  P->y_total = ty.total;
+  std::map<BKey, std::pair<int64_t, int64_t>> dims;
+  Built b = group_tuples(tup, &ty, true, &dims);
+  for (auto& k : b.outkeys) {
+    int iy = ty.find(k);
+    auto dd = dims.at(k);
+    if (ty.rows[(size_t)iy] != dd.first || ty.cols[(size_t)iy] != dd.second)
+      throw Error(MPSA_EINVAL, "y table block shape differs from the produced block");
+  }
+  build_plan(P.get(), b, tl, tx, tr, ty, 0, 1);
+  P->flops_ref = fref;
+  *plan = P.release();
+  return MPSA_OK;
+}
+
+int mpsa_heff2_plan(mpsa_plan** plan, const mpsa_blocks* L, const mpsa_blocks* R, const mpsa_blocks* x,
+                    const mpsa_blocks* y, const mpsa_mpo* Wl, const mpsa_mpo* Wr, const uint64_t* itl, int dl,
+                    const uint64_t* itr, int dr, int precision) {
+  MPSA_TRY({
+    check_itk(itl, dl, "itokey_l");
+    check_itk(itr, dr, "itokey_r");
+    if (!x || !y || x->ns != 2 || y->ns != 2) throw Error(MPSA_EINVAL, "heff2: x and y must be two-site tables");
+    Tab tl = parse(L, 1, "L"), tr = parse(R, 1, "R"), tx = parse(x, 2, "x");
+    auto gl = parse_mpo(Wl, dl, "Wl"), gr = parse_mpo(Wr, dr, "Wr");
+    double fref;
+    auto tup = tuples_heff2(tl, tr, tx, gl, gr, itl, itr, &fref);
+    int rc = plan_heff(plan, L, R, x, y, tup, fref, 2, precision);
+    if (rc) return rc;
+  })
+}
+
+int mpsa_heff1_plan(mpsa_plan** plan, const mpsa_blocks* L, const mpsa_blocks* R, const mpsa_blocks* x,
+                    const mpsa_blocks* y, const mpsa_mpo* W, const uint64_t* itk, int d, int precision) {
+  MPSA_TRY({
+    check_itk(itk, d, "itokey");
+    if (!x || !y || x->ns != 1 || y->ns != 1) throw Error(MPSA_EINVAL, "heff1: x and y must be one-site tables");
+    Tab tl = parse(L, 1, "L"), tr = parse(R, 1, "R"), tx = parse(x, 1, "x");
+    auto g = parse_mpo(W, d, "W");
+    double fref;
+    auto tup = tuples_heff1(tl, tr, tx, g, itk, &fref);
+    int rc = plan_heff(plan, L, R, x, y, tup, fref, 1, precision);
+    if (rc) return rc;
+  })
+}
+
+int mpsa_env_plan(mpsa_plan** plan, const mpsa_blocks* BL, const mpsa_blocks* A, const mpsa_blocks* Ap,
+                  const mpsa_mpo* W, const uint64_t* itk, int d, int direction, const mpsa_blocks* out,
+                  int precision) {
+  MPSA_TRY({
+    if (!plan) throw Error(MPSA_EINVAL, "null plan pointer");
+    if (precision != MPSA_F64) throw Error(MPSA_EINVAL, "environment plans run in FP64 only");
+    check_itk(itk, d, "itokey");
+    Tab tb = parse(BL, 1, "BL"), ta = parse(A, 1, "A");
+    if (ta.n == 0) throw Error(MPSA_EINVAL, "Error in addContractionLayer: A is empty.");
+    Tab tap = Ap ? parse(Ap, 1, "Ap") : ta;
+    if (Ap && tap.n == 0) throw Error(MPSA_EINVAL, "Error in addContractionLayer: Ap is empty.");
+    Tab to = parse(out, 1, "out");
+    auto g = parse_mpo(W, d, "W");
+    double fref;
+    auto tup = tuples_env(tb, ta, tap, g, itk, direction, &fref);
+    std::unique_ptr<mpsa_plan> P(new mpsa_plan());
+    P->kind = 2;
+    P->prec = F64;
+    P->direction = direction;
+    P->L_total = tb.total;
+    P->x_total = ta.total;
+    P->R_total = tap.total;
+    P->y_total = to.total;
+    std::map<BKey, std::pair<int64_t, int64_t>> dims;
+    Built b = group_tuples(tup, &to, true, &dims);
+    for (auto& k : b.outkeys) {
+      int io = to.find(k);
+      auto dd = dims.at(k);
+      if (to.rows[(size_t)io] != dd.first || to.cols[(size_t)io] != dd.second)
+        throw Error(MPSA_EINVAL, "out table block shape differs from the produced block");
+    }
+    build_plan(P.get(), b, tb, ta, tap, to, 2, direction);
+    P->flops_ref = fref;
+    *plan = P.release();
+  })
+}
+
+int mpsa_plan_bind_env(mpsa_plan* P, const double* dL, const double* dR, mpsa_stream stream) {
+  MPSA_TRY({
+    if (!P || P->kind != 0) throw Error(MPSA_EINVAL, "bind_env: not an H_eff plan");
+    if ((P->L_total > 0 && !dL) || (P->R_total > 0 && !dR)) throw Error(MPSA_EINVAL, "bind_env: null environment");
+    hipStream_t st = (hipStream_t)stream;
+    P->dL = dL;
+    P->dR = dR;
+    if (P->prec != F64) {
+      if (!P->Lc) {
+        if (hipMalloc(&P->Lc, (size_t)P->Lc_elems * P->esize) != hipSuccess) throw Error(MPSA_ENOMEM, "env copy");
+        HIPCHK(hipMemset(P->Lc, 0, (size_t)P->Lc_elems * P->esize));
+      }
+      if (!P->Rc) {
+        if (hipMalloc(&P->Rc, (size_t)P->Rc_elems * P->esize) != hipSuccess) throw Error(MPSA_ENOMEM, "env copy");
+        HIPCHK(hipMemset(P->Rc, 0, (size_t)P->Rc_elems * P->esize));
+      }
+      HIPCHK(launch_repack(P->prec, P->d_rpL, (int)P->rpL.size(), P->rpL_chunks, 1, dL, 0, P->Lc, 0, st));
+      HIPCHK(launch_repack(P->prec, P->d_rpR, (int)P->rpR.size(), P->rpR_chunks, 1, dR, 0, P->Rc, 0, st));
+    }
+  })
+}
+
+int mpsa_heff_apply(mpsa_plan* P, const double* dx, int64_t x_stride, double* dy, int64_t y_stride, int nvec,
+                    mpsa_stream stream) {
+  MPSA_TRY({
+    if (!P || P->kind != 0) throw Error(MPSA_EINVAL, "apply: not an H_eff plan");
+    if (nvec < 1) throw Error(MPSA_EINVAL, "apply: nvec must be >= 1");
+    if (!P->dL && P->L_total > 0) throw Error(MPSA_EINVAL, "apply: environments not bound");
+    if (!dx || !dy) throw Error(MPSA_EINVAL, "apply: null vector");
+    if (nvec > 1 && (x_stride < P->x_total || y_stride < P->y_total))
+      throw Error(MPSA_EINVAL, "apply: vector stride smaller than the table extent");
+    hipStream_t st = (hipStream_t)stream;
+    ensure_ws(P, nvec);
+    if (P->timing) HIPCHK(hipEventRecord(P->ev[0], st));
+    const void* A1 = P->prec == F64 ? (const void*)P->dL : P->Lc;
+    const void* B1 = P->prec == F64 ? (const void*)dx : P->xc;
+    int64_t b1_vs = P->prec == F64 ? x_stride : P->xc_per_vec;
+    if (P->prec != F64)
+      HIPCHK(launch_repack(P->prec, P->d_rpx, (int)P->rpx.size(), P->rpx_chunks, nvec, dx, x_stride, P->xc,
+                           P->xc_per_vec, st));
+    if (P->timing) HIPCHK(hipEventRecord(P->ev[1], st));
+    HIPCHK(launch_gemm(P->prec, 0, P->s1.tr, P->s1.tile_class, P->s1.d_tasks, (int)P->s1.tasks.size(),
+                       P->s1.d_segs, P->s1.tiles, nvec, A1, 0, B1, b1_vs, P->ws, P->ws_per_vec, st));
+    if (P->timing) HIPCHK(hipEventRecord(P->ev[2], st));
+    HIPCHK(launch_mix(P->prec, P->d_mix, (int)P->mix.size(), P->d_terms, P->mix_chunks, nvec, P->ws, P->ws,
+                      P->ws_per_vec, st));
+    const void* B2 = P->prec == F64 ? (const void*)P->dR : P->Rc;
+    HIPCHK(launch_zero_ranges(P->d_zero, (int)P->zero.size(), dy, y_stride, nvec, st));
+    if (P->timing) HIPCHK(hipEventRecord(P->ev[3], st));
+    HIPCHK(launch_gemm(P->prec, 1, P->s2.tr, P->s2.tile_class, P->s2.d_tasks, (int)P->s2.tasks.size(),
+                       P->s2.d_segs, P->s2.tiles, nvec, P->ws, P->ws_per_vec, B2, 0, dy, y_stride, st));
+    if (P->timing) HIPCHK(hipEventRecord(P->ev[4], st));
+  })
+}
+
+int mpsa_env_apply(mpsa_plan* P, const double* dBL, const double* dA, const double* dAp, double* dout,
+                   mpsa_stream stream) {
+  MPSA_TRY({
+    if (!P || P->kind != 2) throw Error(MPSA_EINVAL, "env_apply: not an environment plan");
+    if (!dBL || !dA || !dout) throw Error(MPSA_EINVAL, "env_apply: null buffer");
+    const double* up = dAp ? dAp : dA;
+    hipStream_t st = (hipStream_t)stream;
+    ensure_ws(P, 1);
+    HIPCHK(launch_gemm(F64, 0, P->s1.tr, P->s1.tile_class, P->s1.d_tasks, (int)P->s1.tasks.size(), P->s1.d_segs,
+                       P->s1.tiles, 1, dBL, 0, dA, 0, P->ws, 0, st));
+    HIPCHK(launch_mix(F64, P->d_mix, (int)P->mix.size(), P->d_terms, P->mix_chunks, 1, P->ws, P->ws, 0, st));
+    HIPCHK(launch_zero_ranges(P->d_zero, (int)P->zero.size(), dout, 0, 1, st));
+    HIPCHK(launch_gemm(F64, 1, P->s2.tr, P->s2.tile_class, P->s2.d_tasks, (int)P->s2.tasks.size(), P->s2.d_segs,
+                       P->s2.tiles, 1, up, 0, P->ws, 0, dout, 0, st));
+  })
+}
+
+int mpsa_plan_info(const mpsa_plan* P, double* flops_alg, double* flops_ref, int64_t* n_gemm1, int64_t* n_gemm2,
+                   int64_t* workspace_bytes) {
+  MPSA_TRY({
+    if (!P) throw Error(MPSA_EINVAL, "null plan");
+    if (flops_alg) *flops_alg = P->flops_alg;
+    if (flops_ref) *flops_ref = P->flops_ref;
+    if (n_gemm1) *n_gemm1 = (int64_t)P->s1.tasks.size();
+    if (n_gemm2) *n_gemm2 = (int64_t)P->s2.segs.size();
+    if (workspace_bytes) *workspace_bytes = P->ws_per_vec * (int64_t)P->esize;
+  })
+}
+
+int mpsa_plan_stage_info(const mpsa_plan* P, double* flops_stage1, double* flops_stage2, int* tile_class1,
+                         int* tile_class2) {
+  MPSA_TRY({
+    if (!P) throw Error(MPSA_EINVAL, "null plan");
+    if (flops_stage1) *flops_stage1 = P->flops1;
+    if (flops_stage2) *flops_stage2 = P->flops2;
+    if (tile_class1) *tile_class1 = P->s1.tile_class;
+    if (tile_class2) *tile_class2 = P->s2.tile_class;
+  })
+}
+
+int mpsa_plan_set_timing(mpsa_plan* P, int enable) {
+  MPSA_TRY({
+    if (!P) throw Error(MPSA_EINVAL, "null plan");
+    if (enable && !P->ev[0])
+      for (auto& e : P->ev) HIPCHK(hipEventCreate(&e));
+    P->timing = enable ? 1 : 0;
+  })
+}
+
+int mpsa_plan_last_times(const mpsa_plan* P, float* ms) {
+  MPSA_TRY({
+    if (!P || !ms) throw Error(MPSA_EINVAL, "null argument");
+    if (!P->timing || !P->ev[0]) throw Error(MPSA_EINVAL, "timing not enabled on this plan");
+    HIPCHK(hipEventSynchronize(P->ev[4]));
+    for (int i = 0; i < 4; ++i) HIPCHK(hipEventElapsedTime(&ms[i], P->ev[i], P->ev[i + 1]));
+  })
+}
+
+int mpsa_plan_destroy(mpsa_plan* P) {
+  MPSA_TRY({ delete P; })
+}
+
+static double* g_partial = nullptr;
+static double* g_dotout = nullptr;
+static const int kDotParts = 1024;
+
+int mpsa_dot(const double* dx, const double* dy, int64_t n, int is_complex, double* out, mpsa_stream stream) {
+  MPSA_TRY({
+    if (n < 0 || !out || (n > 0 && (!dx || !dy))) throw Error(MPSA_EINVAL, "dot: bad arguments");
+    hipStream_t st = (hipStream_t)stream;
+    if (!g_partial) {
+      HIPCHK(hipMalloc((void**)&g_partial, sizeof(double) * 2 * kDotParts));
+      HIPCHK(hipMalloc((void**)&g_dotout, sizeof(double) * 2));
+    }
+    HIPCHK(launch_dot(dx, dy, n, is_complex, g_partial, kDotParts, g_dotout, st));
+    HIPCHK(hipMemcpyAsync(out, g_dotout, sizeof(double) * 2, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  })
+}
+
+int mpsa_axpby(const double* a, const double* dx, const double* b, double* dy, int64_t n, int is_complex,
+               mpsa_stream stream) {
+  MPSA_TRY({
+    if (n < 0 || !a || !b || (n > 0 && (!dx || !dy))) throw Error(MPSA_EINVAL, "axpby: bad arguments");
+    HIPCHK(launch_axpby(a[0], is_complex ? a[1] : 0.0, dx, b[0], is_complex ? b[1] : 0.0, dy, n, is_complex,
+                        (hipStream_t)stream));
+  })
+}
+
+}  // extern "C"

@@ -1,0 +1,197 @@
+"""MPO tensors of the models the hot path is exercised with.
+
+`MPOMat` mirrors ``MPOMat<1,T>`` (src/MPOTypes.hpp:839-899): a map from the
+aux pair ``ukey<2>(wl, wr)`` to a sparse local operator, itself a map from
+``OpKeyType<1>(out, in)`` to a value.  Iteration order is the reference's map
+order: ukey lexicographic (src/keyTypes.hpp:471-483), operator entries by
+``(in, out)`` (src/keyTypes.hpp:313-343 with OpKeyType storing in in [0]).
+
+The constructors follow src/Hamiltonians.hpp line by line (same aux layout,
+same operator products, same ``squeeze(1e-16)``), so the W tensors fed to the
+contraction are the reference's.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+class LocalOp:
+    """SparseLocalOperator<T> (src/MPOTypes.hpp:418-530); entries {(out,in): v}."""
+
+    def __init__(self, d, entries=None):
+        self.d = d
+        self.e: dict[tuple[int, int], complex] = dict(entries or {})
+
+    def items(self):
+        # map order of OpKeyType: (in, out)
+        return sorted(self.e.items(), key=lambda kv: (kv[0][1], kv[0][0]))
+
+    def __mul__(self, other):
+        if isinstance(other, LocalOp):
+            # SparseOperator::operator*= (src/MPOTypes.hpp:218-243): matrix
+            # product, key (i.out, j.in) whenever i.in == j.out.
+            out = LocalOp(self.d)
+            for (o1, i1), v1 in self.items():
+                for (o2, i2), v2 in other.items():
+                    if i1 == o2:
+                        k = (o1, i2)
+                        out.e[k] = out.e.get(k, 0.0) + v1 * v2
+            return out
+        return LocalOp(self.d, {k: v * other for k, v in self.e.items()})
+
+    __rmul__ = __mul__
+
+    def dense(self):
+        m = np.zeros((self.d, self.d), dtype=np.result_type(*[type(v) for v in self.e.values()], float)
+                     if self.e else float)
+        for (o, i), v in self.e.items():
+            m[o, i] += v
+        return m
+
+
+class MPOMat:
+    """MPOMat<1,T>: {(wl,wr): LocalOp}, with Dl/Dr."""
+
+    def __init__(self, Dl=0, Dr=0, d=2):
+        self.Dl, self.Dr, self.d = Dl, Dr, d
+        self.m: dict[tuple[int, int], LocalOp] = {}
+
+    def __setitem__(self, k, op: LocalOp):
+        self.m[tuple(k)] = LocalOp(op.d, op.e)
+
+    def squeeze(self, thresh):
+        # MPOMat::squeeze (src/MPOTypes.hpp:891-899): drops small entries, keeps keys.
+        for op in self.m.values():
+            op.e = {k: v for k, v in op.e.items() if abs(v) >= thresh}
+
+    def entries(self):
+        """(wl, wr, out, in, value) in the reference's iteration order."""
+        out = []
+        for (wl, wr) in sorted(self.m):
+            for (o, i), v in self.m[(wl, wr)].items():
+                out.append((wl, wr, o, i, v))
+        return out
+
+    def arrays(self, complex_=False):
+        e = self.entries()
+        wl = np.array([x[0] for x in e], dtype=np.uint32)
+        wr = np.array([x[1] for x in e], dtype=np.uint32)
+        o = np.array([x[2] for x in e], dtype=np.uint32)
+        i = np.array([x[3] for x in e], dtype=np.uint32)
+        if complex_:
+            v = np.array([complex(x[4]) for x in e], dtype=np.complex128)
+        else:
+            v = np.array([float(np.real(x[4])) for x in e], dtype=np.float64)
+        return wl, wr, o, i, v
+
+    def dense(self):
+        """W[wl, wr, out, in] as a dense array."""
+        W = np.zeros((self.Dl, self.Dr, self.d, self.d),
+                     dtype=np.complex128 if any(isinstance(x[4], complex) for x in self.entries()) else np.float64)
+        for wl, wr, o, i, v in self.entries():
+            W[wl, wr, o, i] += v
+        return W
+
+
+def _spin_ops():
+    Sz = LocalOp(2, {(0, 0): -0.5, (1, 1): 0.5})
+    Sp = LocalOp(2, {(1, 0): 1.0})
+    Sm = LocalOp(2, {(0, 1): 1.0})
+    Id = LocalOp(2, {(0, 0): 1.0, (1, 1): 1.0})
+    return Sz, Sp, Sm, Id
+
+
+def heisenberg_mpo(Jz, Jxy, B):
+    """HeisenbergMPO (src/Hamiltonians.hpp:86-133).  Jz, Jxy have N-1 entries, B has N."""
+    N = len(B)
+    Sz, Sp, Sm, Id = _spin_ops()
+    mpo = []
+    M = MPOMat(1, 5)
+    M[(0, 0)] = -B[0] * Sz
+    M[(0, 1)] = Jxy[0] / 2. * Sm
+    M[(0, 2)] = Jxy[0] / 2. * Sp
+    M[(0, 3)] = Jz[0] * Sz
+    M[(0, 4)] = Id
+    M.squeeze(1e-16)
+    mpo.append(M)
+    for site in range(1, N - 1):
+        Mb = MPOMat(5, 5)
+        Mb[(0, 0)] = Id
+        Mb[(1, 0)] = Sp
+        Mb[(2, 0)] = Sm
+        Mb[(3, 0)] = Sz
+        Mb[(4, 0)] = -B[site] * Sz
+        Mb[(4, 1)] = Jxy[site] / 2. * Sm
+        Mb[(4, 2)] = Jxy[site] / 2. * Sp
+        Mb[(4, 3)] = Jz[site] * Sz
+        Mb[(4, 4)] = Id
+        Mb.squeeze(1e-16)
+        mpo.append(Mb)
+    M = MPOMat(5, 1)
+    M[(0, 0)] = Id
+    M[(1, 0)] = Sp
+    M[(2, 0)] = Sm
+    M[(3, 0)] = Sz
+    M[(4, 0)] = -B[N - 1] * Sz
+    M.squeeze(1e-16)
+    mpo.append(M)
+    return mpo
+
+
+def xxz_mpo(N, Jz=1.0, Jxy=1.0, hz=0.0):
+    return heisenberg_mpo([Jz] * (N - 1), [Jxy] * (N - 1), [hz] * N)
+
+
+def spinless_fermions_mpo(V, t, mu):
+    """SpinlessFermionsMPO (src/Hamiltonians.hpp:521-569), Jordan-Wigner sign p."""
+    N = len(mu)
+    n = LocalOp(2, {(0, 0): 0.0, (1, 1): 1.0})
+    cd = LocalOp(2, {(1, 0): 1.0})
+    c = LocalOp(2, {(0, 1): 1.0})
+    Id = LocalOp(2, {(0, 0): 1.0, (1, 1): 1.0})
+    p = LocalOp(2, {(0, 0): 1.0, (1, 1): -1.0})
+    mpo = []
+    M = MPOMat(1, 5)
+    M[(0, 0)] = -mu[0] * n
+    M[(0, 1)] = t[0] * (c * p)
+    M[(0, 2)] = -t[0] * (cd * p)
+    M[(0, 3)] = V[0] * n
+    M[(0, 4)] = Id
+    M.squeeze(1e-16)
+    mpo.append(M)
+    for site in range(1, N - 1):
+        Mb = MPOMat(5, 5)
+        Mb[(0, 0)] = Id
+        Mb[(1, 0)] = cd
+        Mb[(2, 0)] = c
+        Mb[(3, 0)] = n
+        Mb[(4, 0)] = -mu[site] * n
+        Mb[(4, 1)] = t[site] * (c * p)
+        Mb[(4, 2)] = -t[site] * (cd * p)
+        Mb[(4, 3)] = V[site] * n
+        Mb[(4, 4)] = Id
+        Mb.squeeze(1e-16)
+        mpo.append(Mb)
+    M = MPOMat(5, 1)
+    M[(0, 0)] = Id
+    M[(1, 0)] = cd
+    M[(2, 0)] = c
+    M[(3, 0)] = n
+    M[(4, 0)] = -mu[N - 1] * n
+    M.squeeze(1e-16)
+    mpo.append(M)
+    return mpo
+
+
+def tv_mpo(N, t=1.0, V=2.0, mu=0.0):
+    return spinless_fermions_mpo([V] * (N - 1), [t] * (N - 1), [mu] * N)
+
+
+def itokey_u1(d=2):
+    """i_to_key for spin-1/2 / spinless fermions: state s -> charge s (N_up / N)."""
+    return [s for s in range(d)]
+
+
+def itokey_dense(d=2):
+    """No symmetry: every local state carries charge 0 (dense blocks)."""
+    return [0] * d

@@ -1,0 +1,351 @@
+"""ctypes binding of the gfx950 library (libmpstools_amd.so, include/mpstools_amd.h).
+
+This module is the Python mirror of the reference's contraction interface for
+the hot path: `contract_twosite`, `contract_single` and `add_contraction_layer`
+take the same arguments as the reference's contractTwosite / contractSingle /
+addContractionLayer (src/MPSfunctions.hpp:949, 857, 442) -- environments,
+tensor, MPO matrices, i_to_key -- as host `Blocks` and return host `Blocks`.
+The plan classes keep everything device-resident for repeated application
+(Lanczos).  Device memory comes from torch (plumbing only); every contraction
+runs in the HIP library, and importing this module fails loudly when the
+library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .blocks import Blocks
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libmpstools_amd.so")
+
+F64, F32, BF16 = 0, 1, 2
+PREC = {"f64": F64, "fp64": F64, "f32": F32, "fp32": F32, "bf16": BF16}
+
+
+class MpsaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mpsa error {code}: {msg}")
+        self.code = code
+
+
+class _Blocks(C.Structure):
+    _fields_ = [("n", C.c_int64), ("ns", C.c_int32), ("is_complex", C.c_int32), ("q", C.c_void_p),
+                ("p", C.c_void_p), ("rows", C.c_void_p), ("cols", C.c_void_p), ("offset", C.c_void_p)]
+
+
+class _MPO(C.Structure):
+    _fields_ = [("n", C.c_int64), ("Dl", C.c_uint32), ("Dr", C.c_uint32), ("is_complex", C.c_int32),
+                ("wl", C.c_void_p), ("wr", C.c_void_p), ("out", C.c_void_p), ("in_", C.c_void_p),
+                ("val", C.c_void_p)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError(f"HIP library not built: {LIBPATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIBPATH)
+        P = C.c_void_p
+        L.mpsa_last_error.restype = C.c_char_p
+        L.mpsa_version.restype = C.c_char_p
+        L.mpsa_heff2_outkeys.argtypes = [P, P, P, P, P, P, C.c_int, P, C.c_int, C.c_int64, P, P, P, P, P]
+        L.mpsa_heff1_outkeys.argtypes = [P, P, P, P, P, C.c_int, C.c_int64, P, P, P, P, P]
+        L.mpsa_env_outkeys.argtypes = [P, P, P, P, P, C.c_int, C.c_int, C.c_int64, P, P, P, P, P]
+        L.mpsa_heff2_plan.argtypes = [P, P, P, P, P, P, P, P, C.c_int, P, C.c_int, C.c_int]
+        L.mpsa_heff1_plan.argtypes = [P, P, P, P, P, P, P, C.c_int, C.c_int]
+        L.mpsa_env_plan.argtypes = [P, P, P, P, P, P, C.c_int, C.c_int, P, C.c_int]
+        L.mpsa_plan_bind_env.argtypes = [P, P, P, P]
+        L.mpsa_heff_apply.argtypes = [P, P, C.c_int64, P, C.c_int64, C.c_int, P]
+        L.mpsa_env_apply.argtypes = [P, P, P, P, P, P]
+        L.mpsa_plan_info.argtypes = [P, P, P, P, P, P]
+        L.mpsa_plan_destroy.argtypes = [P]
+        L.mpsa_plan_stage_info.argtypes = [P, P, P, P, P]
+        L.mpsa_plan_set_timing.argtypes = [P, C.c_int]
+        L.mpsa_plan_last_times.argtypes = [P, P]
+        L.mpsa_dot.argtypes = [P, P, C.c_int64, C.c_int, P, P]
+        L.mpsa_axpby.argtypes = [P, P, P, P, C.c_int64, C.c_int, P]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise MpsaError(rc, lib().mpsa_last_error().decode())
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Table:
+    """A key table (mpsa_blocks) with its arrays kept alive."""
+
+    def __init__(self, ns, q, p, rows, cols, offsets):
+        self.ns = ns
+        self.q = np.ascontiguousarray(q, dtype=np.uint64).reshape(-1, 2)
+        self.p = np.ascontiguousarray(p, dtype=np.uint32).reshape(-1, ns)
+        self.rows = np.ascontiguousarray(rows, dtype=np.int64)
+        self.cols = np.ascontiguousarray(cols, dtype=np.int64)
+        self.offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+        self.n = len(self.rows)
+        self.total = int(max([0] + [int(o + r * c) for o, r, c in zip(self.offsets, self.rows, self.cols)]))
+        self.s = _Blocks(self.n, ns, 0, self.q.ctypes.data, self.p.ctypes.data, self.rows.ctypes.data,
+                         self.cols.ctypes.data, self.offsets.ctypes.data)
+
+    @staticmethod
+    def from_blocks(B: Blocks, align=64):
+        t = B.table(align)
+        return Table(B.ns, t["q"], t["p"], t["rows"], t["cols"], t["offsets"])
+
+    @staticmethod
+    def from_keys(ns, q, p, rows, cols, align=64):
+        offs = np.zeros(len(rows), np.int64)
+        off = 0
+        for i in range(len(rows)):
+            off = (off + align - 1) // align * align
+            offs[i] = off
+            off += int(rows[i]) * int(cols[i])
+        return Table(ns, q, p, rows, cols, offs)
+
+    def as_dict(self):
+        return dict(q=self.q, p=self.p, rows=self.rows, cols=self.cols, offsets=self.offsets, total=self.total)
+
+    def keys(self):
+        return [(int(self.q[i, 0]), int(self.q[i, 1])) + tuple(int(x) for x in self.p[i]) for i in range(self.n)]
+
+    def union(self, other: "Table", align=64) -> "Table":
+        """Table over the union of both key sets (key order)."""
+        d = {}
+        for t in (self, other):
+            for i, k in enumerate(t.keys()):
+                d[k] = (int(t.rows[i]), int(t.cols[i]))
+        ks = sorted(d)
+        q = np.array([[k[0], k[1]] for k in ks], np.uint64).reshape(-1, 2)
+        p = np.array([k[2:] for k in ks], np.uint32).reshape(-1, self.ns)
+        return Table.from_keys(self.ns, q, p, [d[k][0] for k in ks], [d[k][1] for k in ks], align)
+
+    def pack(self, B: Blocks, dtype=np.float64):
+        return B.pack(table=self.as_dict()).astype(dtype)
+
+    def unpack(self, buf, keep=None) -> Blocks:
+        return Blocks.unpack(self.ns, self.as_dict(), np.asarray(buf), keep=keep)
+
+
+class MPOArg:
+    def __init__(self, W):
+        self.wl, self.wr, self.o, self.i, self.v = W.arrays()
+        self.s = _MPO(len(self.wl), W.Dl, W.Dr, 0, self.wl.ctypes.data, self.wr.ctypes.data, self.o.ctypes.data,
+                      self.i.ctypes.data, self.v.ctypes.data)
+
+
+def _itk(itk):
+    return np.ascontiguousarray(np.array(itk, dtype=np.uint64))
+
+
+def _outkeys(fn, args, ns):
+    n = np.zeros(1, np.int64)
+    _check(fn(*args, 0, _ptr(n), None, None, None, None))
+    m = int(n[0])
+    q = np.zeros((max(m, 1), 2), np.uint64)
+    p = np.zeros((max(m, 1), ns), np.uint32)
+    rows = np.zeros(max(m, 1), np.int64)
+    cols = np.zeros(max(m, 1), np.int64)
+    if m:
+        _check(fn(*args, m, _ptr(n), _ptr(q), _ptr(p), _ptr(rows), _ptr(cols)))
+    return q[:m], p[:m], rows[:m], cols[:m]
+
+
+def heff2_outkeys(Lt: Table, Rt: Table, xt: Table, Wl, Wr, itl, itr, align=64) -> Table:
+    a, b = _itk(itl), _itk(itr)
+    ml, mr = MPOArg(Wl), MPOArg(Wr)
+    q, p, r, c = _outkeys(lib().mpsa_heff2_outkeys,
+                          [C.byref(Lt.s), C.byref(Rt.s), C.byref(xt.s), C.byref(ml.s), C.byref(mr.s), _ptr(a),
+                           len(a), _ptr(b), len(b)], 2)
+    return Table.from_keys(2, q, p, r, c, align)
+
+
+def heff1_outkeys(Lt: Table, Rt: Table, xt: Table, W, itk, align=64) -> Table:
+    a = _itk(itk)
+    m = MPOArg(W)
+    q, p, r, c = _outkeys(lib().mpsa_heff1_outkeys,
+                          [C.byref(Lt.s), C.byref(Rt.s), C.byref(xt.s), C.byref(m.s), _ptr(a), len(a)], 1)
+    return Table.from_keys(1, q, p, r, c, align)
+
+
+def env_outkeys(BLt: Table, At: Table, W, itk, direction, Apt: Table | None = None, align=64) -> Table:
+    a = _itk(itk)
+    m = MPOArg(W)
+    q, p, r, c = _outkeys(lib().mpsa_env_outkeys,
+                          [C.byref(BLt.s), C.byref(At.s), C.byref(Apt.s) if Apt is not None else None,
+                           C.byref(m.s), _ptr(a), len(a), int(direction)], 1)
+    return Table.from_keys(1, q, p, r, c, align)
+
+
+def _stream():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _dptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class HeffPlan:
+    """Grouped-GEMM plan of y = H_eff x (two-site or one-site)."""
+
+    def __init__(self, Lt: Table, Rt: Table, xt: Table, yt: Table, W, itk, precision=F64):
+        self.Lt, self.Rt, self.xt, self.yt = Lt, Rt, xt, yt
+        self.h = C.c_void_p()
+        self.precision = PREC.get(precision, precision) if isinstance(precision, str) else precision
+        if xt.ns == 2:
+            Wl, Wr = W
+            itl, itr = itk
+            a, b = _itk(itl), _itk(itr)
+            ml, mr = MPOArg(Wl), MPOArg(Wr)
+            _check(lib().mpsa_heff2_plan(C.byref(self.h), C.byref(Lt.s), C.byref(Rt.s), C.byref(xt.s),
+                                         C.byref(yt.s), C.byref(ml.s), C.byref(mr.s), _ptr(a), len(a), _ptr(b),
+                                         len(b), self.precision))
+        else:
+            a = _itk(itk)
+            m = MPOArg(W)
+            _check(lib().mpsa_heff1_plan(C.byref(self.h), C.byref(Lt.s), C.byref(Rt.s), C.byref(xt.s),
+                                         C.byref(yt.s), C.byref(m.s), _ptr(a), len(a), self.precision))
+        self._env = None
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value and _lib is not None:
+            _lib.mpsa_plan_destroy(self.h)
+            self.h = None
+
+    def info(self):
+        fa, fr = np.zeros(1), np.zeros(1)
+        g1, g2, ws = np.zeros(1, np.int64), np.zeros(1, np.int64), np.zeros(1, np.int64)
+        _check(lib().mpsa_plan_info(self.h, _ptr(fa), _ptr(fr), _ptr(g1), _ptr(g2), _ptr(ws)))
+        return dict(flops_alg=float(fa[0]), flops_ref=float(fr[0]), n_gemm1=int(g1[0]), n_gemm2=int(g2[0]),
+                    workspace_bytes=int(ws[0]))
+
+    def stage_info(self):
+        f1, f2 = np.zeros(1), np.zeros(1)
+        c1, c2 = np.zeros(1, np.int32), np.zeros(1, np.int32)
+        _check(lib().mpsa_plan_stage_info(self.h, _ptr(f1), _ptr(f2), _ptr(c1), _ptr(c2)))
+        return dict(flops1=float(f1[0]), flops2=float(f2[0]), tile_class1=int(c1[0]), tile_class2=int(c2[0]))
+
+    def set_timing(self, on=True):
+        _check(lib().mpsa_plan_set_timing(self.h, int(on)))
+
+    def last_times(self):
+        """ms of [convert x, stage-1 GEMM, mix + zero fill, stage-2 GEMM] of the last apply."""
+        ms = np.zeros(4, np.float32)
+        _check(lib().mpsa_plan_last_times(self.h, _ptr(ms)))
+        return [float(x) for x in ms]
+
+    def bind_env(self, dL, dR):
+        self._env = (dL, dR)
+        _check(lib().mpsa_plan_bind_env(self.h, _dptr(dL), _dptr(dR), _stream()))
+
+    def apply(self, dx, dy, nvec=1, x_stride=0, y_stride=0):
+        _check(lib().mpsa_heff_apply(self.h, _dptr(dx), int(x_stride), _dptr(dy), int(y_stride), int(nvec),
+                                     _stream()))
+
+
+class EnvPlan:
+    """Plan of one environment update (addContractionLayer)."""
+
+    def __init__(self, BLt: Table, At: Table, W, itk, direction, outt: Table, Apt: Table | None = None):
+        self.BLt, self.At, self.Apt, self.outt = BLt, At, Apt, outt
+        self.h = C.c_void_p()
+        a = _itk(itk)
+        m = MPOArg(W)
+        _check(lib().mpsa_env_plan(C.byref(self.h), C.byref(BLt.s), C.byref(At.s),
+                                   C.byref(Apt.s) if Apt is not None else None, C.byref(m.s), _ptr(a), len(a),
+                                   int(direction), C.byref(outt.s), F64))
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value and _lib is not None:
+            _lib.mpsa_plan_destroy(self.h)
+            self.h = None
+
+    def info(self):
+        fa, fr = np.zeros(1), np.zeros(1)
+        g1, g2, ws = np.zeros(1, np.int64), np.zeros(1, np.int64), np.zeros(1, np.int64)
+        _check(lib().mpsa_plan_info(self.h, _ptr(fa), _ptr(fr), _ptr(g1), _ptr(g2), _ptr(ws)))
+        return dict(flops_alg=float(fa[0]), flops_ref=float(fr[0]), n_gemm1=int(g1[0]), n_gemm2=int(g2[0]),
+                    workspace_bytes=int(ws[0]))
+
+    def apply(self, dBL, dA, dout, dAp=None):
+        _check(lib().mpsa_env_apply(self.h, _dptr(dBL), _dptr(dA), _dptr(dAp), _dptr(dout), _stream()))
+
+
+def dot(dx, dy, n=None, cplx=False):
+    """<y|x> on device (fixed-order FP64 reduction)."""
+    out = np.zeros(2)
+    n = dx.numel() if n is None else n
+    _check(lib().mpsa_dot(_dptr(dx), _dptr(dy), int(n), int(cplx), _ptr(out), _stream()))
+    return complex(out[0], out[1]) if cplx else float(out[0])
+
+
+def axpby(a, dx, b, dy, n=None):
+    """y = a x + b y on device (real)."""
+    av = np.array([float(a), 0.0])
+    bv = np.array([float(b), 0.0])
+    n = dx.numel() if n is None else n
+    _check(lib().mpsa_axpby(_ptr(av), _dptr(dx), _ptr(bv), _dptr(dy), int(n), 0, _stream()))
+
+
+# ---------------------------------------------------------------- drop-in calls
+def _to_dev(t: Table, B: Blocks):
+    import torch
+    return torch.from_numpy(t.pack(B)).cuda()
+
+
+def contract_twosite(L: Blocks, R: Blocks, psi: Blocks, Wl, Wr, itl, itr, precision=F64) -> Blocks:
+    """contractTwosite (src/MPSfunctions.hpp:949-981): returns H_eff psi with the
+    reference's output key set."""
+    import torch
+    Lt, Rt, xt = Table.from_blocks(L), Table.from_blocks(R), Table.from_blocks(psi)
+    yt = heff2_outkeys(Lt, Rt, xt, Wl, Wr, itl, itr)
+    plan = HeffPlan(Lt, Rt, xt, yt, (Wl, Wr), (itl, itr), precision)
+    dL, dR, dx = _to_dev(Lt, L), _to_dev(Rt, R), _to_dev(xt, psi)
+    dy = torch.zeros(max(yt.total, 1), dtype=torch.float64, device="cuda")
+    plan.bind_env(dL, dR)
+    plan.apply(dx, dy)
+    torch.cuda.synchronize()
+    return yt.unpack(dy.cpu().numpy())
+
+
+def contract_single(L: Blocks, A: Blocks, W, R: Blocks, itk, precision=F64) -> Blocks:
+    """contractSingle (src/MPSfunctions.hpp:857-885)."""
+    import torch
+    Lt, Rt, xt = Table.from_blocks(L), Table.from_blocks(R), Table.from_blocks(A)
+    yt = heff1_outkeys(Lt, Rt, xt, W, itk)
+    plan = HeffPlan(Lt, Rt, xt, yt, W, itk, precision)
+    dL, dR, dx = _to_dev(Lt, L), _to_dev(Rt, R), _to_dev(xt, A)
+    dy = torch.zeros(max(yt.total, 1), dtype=torch.float64, device="cuda")
+    plan.bind_env(dL, dR)
+    plan.apply(dx, dy)
+    torch.cuda.synchronize()
+    return yt.unpack(dy.cpu().numpy())
+
+
+def add_contraction_layer(BL: Blocks, A: Blocks, W, itk, direction, Ap: Blocks | None = None) -> Blocks:
+    """addContractionLayer (src/MPSfunctions.hpp:442-555)."""
+    import torch
+    if len(A) == 0:
+        raise MpsaError(1, "Error in addContractionLayer: A is empty.")
+    BLt, At = Table.from_blocks(BL), Table.from_blocks(A)
+    Apt = Table.from_blocks(Ap) if Ap is not None else None
+    outt = env_outkeys(BLt, At, W, itk, direction, Apt)
+    plan = EnvPlan(BLt, At, W, itk, direction, outt, Apt)
+    dBL, dA = _to_dev(BLt, BL), _to_dev(At, A)
+    dAp = _to_dev(Apt, Ap) if Ap is not None else None
+    dout = torch.zeros(max(outt.total, 1), dtype=torch.float64, device="cuda")
+    plan.apply(dBL, dA, dout, dAp)
+    torch.cuda.synchronize()
+    return outt.unpack(dout.cpu().numpy())

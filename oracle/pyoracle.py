@@ -1,0 +1,248 @@
+"""ctypes binding of the CPU oracle (oracle.cpp / oracle.hpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and
+by bench.py's CPU-baseline leg; the product package never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"oracle library not built: {LIB} (run make -C oracle)")
+        L = C.CDLL(LIB)
+        P = C.c_void_p
+        L.orc_last_error.restype = C.c_char_p
+        L.orc_blas_name.restype = C.c_char_p
+        L.orc_bm_new.restype = P
+        L.orc_bm_new.argtypes = [C.c_int, C.c_int]
+        L.orc_bm_free.argtypes = [P]
+        L.orc_bm_load.argtypes = [P, C.c_int64, P, P, P, P, P]
+        L.orc_bm_nblocks.restype = C.c_int64
+        L.orc_bm_nblocks.argtypes = [P]
+        L.orc_bm_nelem.restype = C.c_int64
+        L.orc_bm_nelem.argtypes = [P]
+        L.orc_bm_dump.argtypes = [P, P, P, P, P, P]
+        L.orc_mpo_new.restype = P
+        L.orc_mpo_new.argtypes = [C.c_int, C.c_uint32, C.c_uint32, C.c_int64, P, P, P, P, P]
+        L.orc_mpo_free.argtypes = [P]
+        L.orc_contract_twosite.argtypes = [P, P, P, P, P, P, P, C.c_int, P, C.c_int, C.c_int]
+        L.orc_contract_single.argtypes = [P, P, P, P, P, P, C.c_int, C.c_int]
+        L.orc_add_layer.argtypes = [P, P, P, P, P, C.c_int, C.c_int, C.c_int, P]
+        L.orc_dot.argtypes = [P, P, P, P]
+        L.orc_twosite.argtypes = [P, P, P]
+        L.orc_lmin.argtypes = [P]
+        L.orc_rmax.argtypes = [P, P]
+        L.orc_measure.argtypes = [C.c_int, P, P, P, C.c_int, C.c_int, P]
+        L.orc_lanczos_twosite.argtypes = [P, P, P, P, P, P, C.c_int, P, C.c_int, C.c_int, C.c_double,
+                                          C.c_int, C.c_int, C.c_int, P, C.c_int, P, P, P]
+        L.orc_tridiag.argtypes = [C.c_int, P, P, P]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError("oracle: " + lib().orc_last_error().decode())
+
+
+def blas_name() -> str:
+    return lib().orc_blas_name().decode()
+
+
+class BM:
+    """Handle to an oracle BlockMatrix<Ns>."""
+
+    def __init__(self, ns, cplx=False):
+        self.ns, self.cplx = ns, cplx
+        self.h = lib().orc_bm_new(ns, int(cplx))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_bm_free(self.h)
+            self.h = None
+
+    @staticmethod
+    def from_blocks(B):
+        cplx = np.issubdtype(B.dtype, np.complexfloating)
+        o = BM(B.ns, cplx)
+        t = B.table()
+        data = B.pack(table=t).astype(np.complex128 if cplx else np.float64)
+        q = np.ascontiguousarray(t["q"])
+        p = np.ascontiguousarray(t["p"])
+        _check(lib().orc_bm_load(o.h, len(t["rows"]), _p(q), _p(p), _p(t["rows"]), _p(t["cols"]),
+                                 _p(data.view(np.float64))))
+        return o
+
+    def to_blocks(self):
+        from mpstools_amd.blocks import Blocks
+        n = lib().orc_bm_nblocks(self.h)
+        ne = lib().orc_bm_nelem(self.h)
+        q = np.zeros((n, 2), np.uint64)
+        p = np.zeros((n, self.ns), np.uint32)
+        rows = np.zeros(n, np.int64)
+        cols = np.zeros(n, np.int64)
+        data = np.zeros(ne, np.complex128 if self.cplx else np.float64)
+        _check(lib().orc_bm_dump(self.h, _p(q), _p(p), _p(rows), _p(cols), _p(data.view(np.float64))))
+        return Blocks.from_table(self.ns, q, p, rows, cols, data, data.dtype)
+
+
+class MPO:
+    def __init__(self, W, cplx=False):
+        wl, wr, o, i, v = W.arrays(complex_=cplx)
+        vv = v.view(np.float64) if cplx else v
+        self.h = lib().orc_mpo_new(int(cplx), W.Dl, W.Dr, len(wl), _p(wl), _p(wr), _p(o), _p(i),
+                                   _p(np.ascontiguousarray(vv)))
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.orc_mpo_free(self.h)
+            self.h = None
+
+
+def _itk(itk):
+    return np.ascontiguousarray(np.array(itk, dtype=np.uint64))
+
+
+def contract_twosite(L, R, psi, Wl, Wr, itl, itr, nthreads=1, cplx=False):
+    """contractTwosite (src/MPSfunctions.hpp:949-981) on Blocks inputs."""
+    out = BM(2, cplx)
+    a, b = _itk(itl), _itk(itr)
+    hl, hr, hp = BM.from_blocks(L), BM.from_blocks(R), BM.from_blocks(psi)
+    ml, mr = MPO(Wl, cplx), MPO(Wr, cplx)
+    _check(lib().orc_contract_twosite(out.h, hl.h, hr.h, hp.h, ml.h, mr.h, _p(a), len(itl), _p(b), len(itr),
+                                      nthreads))
+    return out.to_blocks()
+
+
+def contract_single(L, A, W, R, itk, nthreads=1, cplx=False):
+    """contractSingle (src/MPSfunctions.hpp:857-885)."""
+    out = BM(1, cplx)
+    a = _itk(itk)
+    hl, ha, hr = BM.from_blocks(L), BM.from_blocks(A), BM.from_blocks(R)
+    m = MPO(W, cplx)
+    _check(lib().orc_contract_single(out.h, hl.h, ha.h, m.h, hr.h, _p(a), len(itk), nthreads))
+    return out.to_blocks()
+
+
+def add_layer(BL, A, W, itk, direction, nthreads=1, Ap=None, cplx=False):
+    """addContractionLayer (src/MPSfunctions.hpp:442-493; Ap variant :499-555)."""
+    out = BM(1, cplx)
+    a = _itk(itk)
+    hb, ha = BM.from_blocks(BL), BM.from_blocks(A)
+    hap = BM.from_blocks(Ap) if Ap is not None else None
+    m = MPO(W, cplx)
+    _check(lib().orc_add_layer(out.h, hb.h, ha.h, m.h, _p(a), len(itk), direction, nthreads,
+                               hap.h if hap is not None else None))
+    return out.to_blocks()
+
+
+def lmin(cplx=False):
+    from mpstools_amd.blocks import Blocks
+    B = Blocks(1, np.complex128 if cplx else np.float64)
+    B[(0, 0, 0)] = np.ones((1, 1))
+    return B
+
+
+def rmax(Alast):
+    out = BM(1, np.issubdtype(Alast.dtype, np.complexfloating))
+    ha = BM.from_blocks(Alast)
+    _check(lib().orc_rmax(out.h, ha.h))
+    return out.to_blocks()
+
+
+def left_envs(mps, mpo, itk, upto, nthreads=1):
+    """L[-1..upto] as a dict (computeL, src/MPSfunctions.hpp:2040-2052)."""
+    Ls = {-1: lmin(np.issubdtype(mps[0].dtype, np.complexfloating))}
+    for s in range(0, upto + 1):
+        Ls[s] = add_layer(Ls[s - 1], mps[s], mpo[s], itk[s], 1, nthreads)
+    return Ls
+
+
+def right_envs(mps, mpo, itk, downto, nthreads=1):
+    """R[downto..N] as a dict (computeR, src/MPSfunctions.hpp:2025-2037)."""
+    N = len(mps)
+    Rs = {N: rmax(mps[N - 1])}
+    for s in range(N - 1, downto - 1, -1):
+        Rs[s] = add_layer(Rs[s + 1], mps[s], mpo[s], itk[s], 0, nthreads)
+    return Rs
+
+
+def measure(mps, mpo, itk, nthreads=1):
+    """MPO<T>::measure (src/MPOTypes.hpp:990-1017), real MPS."""
+    hs = [BM.from_blocks(A) for A in mps]
+    ms = [MPO(W) for W in mpo]
+    d = len(itk[0])
+    I = np.ascontiguousarray(np.array(itk, dtype=np.uint64).reshape(-1))
+    arr_b = (C.c_void_p * len(hs))(*[h.h for h in hs])
+    arr_m = (C.c_void_p * len(ms))(*[m.h for m in ms])
+    E = np.zeros(1)
+    _check(lib().orc_measure(len(hs), arr_b, arr_m, _p(I), d, nthreads, _p(E)))
+    return float(E[0])
+
+
+def dot(X, Y):
+    hx, hy = BM.from_blocks(X), BM.from_blocks(Y)
+    re, im = np.zeros(1), np.zeros(1)
+    _check(lib().orc_dot(hx.h, hy.h, _p(re), _p(im)))
+    return complex(re[0], im[0]) if X.dtype.kind == "c" else float(re[0])
+
+
+def lanczos_twosite(L, R, psi0, Wl, Wr, itl, itr, itmax, delta, testenergy=True, nthreads=1, target=0):
+    """TSLanzcosEngine::simulate + getState (src/lanzcos.hpp:165-278, 81-92)."""
+    a, b = _itk(itl), _itk(itr)
+    hl, hr, hp = BM.from_blocks(L), BM.from_blocks(R), BM.from_blocks(psi0)
+    ml, mr = MPO(Wl), MPO(Wr)
+    cap = max(itmax + 2, 8)
+    E = np.zeros(cap)
+    nit = np.zeros(1, np.int32)
+    nr = np.zeros(1, np.int32)
+    gs = BM(2)
+    _check(lib().orc_lanczos_twosite(hl.h, hr.h, hp.h, ml.h, mr.h, _p(a), len(itl), _p(b), len(itr), itmax,
+                                      delta, int(testenergy), nthreads, target, _p(E), cap, _p(nit), _p(nr),
+                                      gs.h))
+    return dict(it=int(nit[0]), energies=E[:int(nr[0])].copy(), gs=gs.to_blocks())
+
+
+def tridiag(d, e, vectors=True):
+    n = len(d)
+    dd = np.ascontiguousarray(np.array(d, dtype=np.float64))
+    ee = np.ascontiguousarray(np.array(e, dtype=np.float64))
+    z = np.zeros(n * n) if vectors else None
+    _check(lib().orc_tridiag(n, _p(dd), _p(ee), _p(z) if vectors else None))
+    return dd, (z.reshape((n, n), order="F") if vectors else None)
+
+
+class Handles:
+    """Pre-loaded oracle handles for repeated timing (CPU baseline)."""
+
+    def __init__(self, L, R, psi, Wl, Wr, itl, itr):
+        self.a, self.b = _itk(itl), _itk(itr)
+        self.dl, self.dr = len(itl), len(itr)
+        self.hl, self.hr, self.hp = BM.from_blocks(L), BM.from_blocks(R), BM.from_blocks(psi)
+        self.ml, self.mr = MPO(Wl), MPO(Wr)
+        self.out = BM(2)
+
+    def matvec(self, nthreads):
+        _check(lib().orc_contract_twosite(self.out.h, self.hl.h, self.hr.h, self.hp.h, self.ml.h, self.mr.h,
+                                          _p(self.a), self.dl, _p(self.b), self.dr, nthreads))

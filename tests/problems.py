@@ -1,0 +1,60 @@
+"""Seeded problem builders shared by the tests (inputs only; no expectations)."""
+from __future__ import annotations
+
+from mpstools_amd import models, synth
+from mpstools_amd.models import LocalOp, MPOMat
+
+
+def asym_mpo(N, a=0.7, b=0.3, Jz=0.9, hz=0.2):
+    """A charge-conserving MPO whose Hamiltonian is NOT symmetric
+    (a Sm_i Sp_j + b Sp_i Sm_j with a != b); it separates H from H^T."""
+    Sz = LocalOp(2, {(0, 0): -0.5, (1, 1): 0.5})
+    Sp = LocalOp(2, {(1, 0): 1.0})
+    Sm = LocalOp(2, {(0, 1): 1.0})
+    Id = LocalOp(2, {(0, 0): 1.0, (1, 1): 1.0})
+    mpo = []
+    M = MPOMat(1, 5)
+    M[(0, 0)] = -hz * Sz
+    M[(0, 1)] = a * Sm
+    M[(0, 2)] = b * Sp
+    M[(0, 3)] = Jz * Sz
+    M[(0, 4)] = Id
+    mpo.append(M)
+    for _ in range(1, N - 1):
+        Mb = MPOMat(5, 5)
+        Mb[(0, 0)] = Id
+        Mb[(1, 0)] = Sp
+        Mb[(2, 0)] = Sm
+        Mb[(3, 0)] = Sz
+        Mb[(4, 0)] = -hz * Sz
+        Mb[(4, 1)] = a * Sm
+        Mb[(4, 2)] = b * Sp
+        Mb[(4, 3)] = Jz * Sz
+        Mb[(4, 4)] = Id
+        mpo.append(Mb)
+    M = MPOMat(5, 1)
+    M[(0, 0)] = Id
+    M[(1, 0)] = Sp
+    M[(2, 0)] = Sm
+    M[(3, 0)] = Sz
+    M[(4, 0)] = -hz * Sz
+    mpo.append(M)
+    return mpo
+
+
+def build(oracle, model, L, N, D, seed, c=None, dense=False, **kw):
+    """Random mixed-canonical MPS around sites (c, c+1), its environments
+    (built by the oracle) and the normalised two-site psi."""
+    if model == "xxz":
+        mpo = models.xxz_mpo(L, kw.get("Jz", 1.0), kw.get("Jxy", 1.0), kw.get("hz", 0.0))
+    elif model == "tv":
+        mpo = models.tv_mpo(L, kw.get("t", 1.0), kw.get("V", 2.0), kw.get("mu", 0.0))
+    elif model == "asym":
+        mpo = asym_mpo(L)
+    else:
+        raise ValueError(model)
+    itk = [models.itokey_dense() if dense else models.itokey_u1()] * L
+    mps, psi, c, dims = synth.centre_problem(L, itk, 0 if dense else N, D, seed, c)
+    Ls = oracle.left_envs(mps, mpo, itk, c - 1)
+    Rs = oracle.right_envs(mps, mpo, itk, c + 1)
+    return dict(mpo=mpo, itk=itk, mps=mps, psi=psi, c=c, dims=dims, Ls=Ls, Rs=Rs)
