@@ -55,8 +55,14 @@ enum {
 /* Arithmetic of the sector GEMMs.  Inputs and outputs at the ABI are always
  * FP64 (or complex128 interleaved when is_complex is set); F32 and BF16 plans
  * keep reduced-precision copies of the environments and convert psi on the fly.
- * BF16 means bf16 inputs with fp32 accumulation on MFMA. */
-enum { MPSA_F64 = 0, MPSA_F32 = 1, MPSA_BF16 = 2 };
+ * BF16 means bf16 inputs with fp32 accumulation on MFMA; BF16X3 splits every
+ * operand into hi + lo bf16 parts and accumulates hi*hi + hi*lo + lo*hi in fp32
+ * (about 16 significant bits per operand, three bf16 MFMAs per product). */
+enum { MPSA_F64 = 0, MPSA_F32 = 1, MPSA_BF16 = 2, MPSA_BF16X3 = 3 };
+/* OR-ed into an H_eff plan's precision: keep the reference's schedule of one
+ * product pair per tuple (its "plain method", F_ref flops) instead of sharing
+ * the (L, psi) products across MPO paths.  For baselines. */
+enum { MPSA_PLAIN = 0x100 };
 
 typedef struct mpsa_blocks {
   int64_t n;              /* number of blocks */
@@ -143,8 +149,41 @@ int mpsa_plan_stage_info(const mpsa_plan* plan, double* flops_stage1, double* fl
  * enabled, mpsa_plan_last_times returns, for the last apply, the milliseconds
  * of [convert x, stage-1 GEMM, mix + zero fill, stage-2 GEMM]. */
 int mpsa_plan_set_timing(mpsa_plan* plan, int enable);
+/* Copy out a plan's device schedule for inspection: what = 0/1 stage-1 tasks /
+ * segments, 2/3 stage-2 tasks / segments, 4/5 mix tasks / terms, 6 zero-filled
+ * output ranges.  Records are the packed structs
+ *   task   {int64 c_off; int32 m, n, seg0, nseg, tile0, tiles_m}            (32 B)
+ *   seg    {int64 a_off, b_off; int32 k, lda, ldb, pad; double alpha}       (40 B)
+ *   mix    {int64 dst_off, len; int32 t0, nt, chunk0, pad}                  (32 B)
+ *   term   {int64 src_off; double coef}                                     (16 B)
+ *   range  {int64 off, len}                                                 (16 B)
+ * *n receives the record count (cap = 0 queries it; cap is in bytes); *extra
+ * the stage's transpose code (tasks), tile class (segments) or precision (mix). */
+int mpsa_plan_describe(const mpsa_plan* plan, int what, void* buf, int64_t cap, int64_t* n, int* extra);
 int mpsa_plan_last_times(const mpsa_plan* plan, float* ms4);
 int mpsa_plan_destroy(mpsa_plan* plan);
+
+/* Extents (elements) of the x and y layouts and whether they are the same
+ * vector space (same keys, shapes and offsets). */
+int mpsa_plan_vector_lengths(const mpsa_plan* plan, int64_t* x_len, int64_t* y_len, int* same_layout);
+
+/* ---- Lanczos (TSLanzcosEngine::simulate + getState, src/lanzcos.hpp:165-278, 81-92) ----
+ * Plain three-term Lanczos on the device in FP64 with the plan as matvec (the
+ * plan's x and y tables must coincide).  Same recurrence, stopping rules
+ * (|k| < delta, Ritz-value test every 5 steps when testenergy, Ritz-vector
+ * residual otherwise, itmax) and getState(target) as the reference; the
+ * tridiagonal eigensolve runs on the host in FP64.  energies receives the
+ * sorted Ritz values (at most cap), *nit the iterations, *nritz their count;
+ * dstate (may be NULL) the Ritz vector `target`.  The sign of a Ritz vector
+ * is that of the tridiagonal eigensolver (as with dstev, arbitrary). */
+int mpsa_lanczos(mpsa_plan* plan, const double* dpsi0, double* dstate, int itmax, double delta, int testenergy,
+                 int target, double* energies, int cap, int* nit, int* nritz, mpsa_stream stream);
+/* Host FP64 symmetric tridiagonal eigensolver used by mpsa_lanczos (replaces
+ * dstev, blas/lapack_routines.cpp:435-446): d (n) diagonal -> ascending
+ * eigenvalues, e (n-1) off-diagonal, z (n*n, column-major, may be NULL). */
+int mpsa_tridiag_eig(int n, double* d, const double* e, double* z);
+/* Free the device vector pool kept between Lanczos calls. */
+int mpsa_release_pools(void);
 
 /* ---- Lanczos vector algebra over packed block vectors (BlockTensor.hpp:443-544) ----
  * X*Y = sum conj(Y) X (src/BlockTensor.hpp:520-544), computed in FP64 with a

@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <limits>
 
 namespace mpsa {
 
@@ -49,24 +50,31 @@ struct Range {
   int64_t off, len;
 };
 
-enum Prec { F64 = 0, F32 = 1, BF16 = 2 };
+enum Prec { F64 = 0, F32 = 1, BF16 = 2, BF16X3 = 3 };
+
+// Plane offsets (elements) of hi/lo split operands (BF16X3); zero otherwise.
+struct Planes {
+  int64_t a = 0, b = 0, c = 0;
+};
 
 // Launchers (mpsa_kernels.hip).  `tr` packs (transA << 1) | transB.
 // a_vs/b_vs/c_vs are per-vector strides (elements) for batched launches.
 hipError_t launch_gemm(int prec_in, int out_kind, int tr, int tile_class, const GTask* tasks, int ntasks,
                        const GSeg* segs, int tiles_per_vec, int nvec, const void* A, int64_t a_vs, const void* B,
-                       int64_t b_vs, void* C, int64_t c_vs, hipStream_t st);
+                       int64_t b_vs, void* C, int64_t c_vs, hipStream_t st, Planes planes = Planes());
 // out_kind: 0 = same type as the input precision, 1 = double
 hipError_t launch_mix(int prec, const MixTask* tasks, int ntasks, const MixTerm* terms, int chunks_per_vec, int nvec,
-                      const void* src, void* dst, int64_t vs, hipStream_t st);
+                      const void* src, void* dst, int64_t vs, hipStream_t st, int64_t plane = 0);
 hipError_t launch_convert(int prec, const double* src, void* dst, int64_t n, hipStream_t st);
 hipError_t launch_repack(int prec, const Repack* tasks, int ntasks, int chunks_per_vec, int nvec, const double* src,
-                         int64_t s_vs, void* dst, int64_t d_vs, hipStream_t st);
+                         int64_t s_vs, void* dst, int64_t d_vs, hipStream_t st, int64_t plane = 0);
+hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t st);
 hipError_t launch_zero_ranges(const Range* r, int nr, double* base, int64_t vs, int nvec, hipStream_t st);
 hipError_t launch_dot(const double* x, const double* y, int64_t n, int cplx, double* partial, int nparts,
                       double* out2, hipStream_t st);
 hipError_t launch_axpby(double ar, double ai, const double* x, double br, double bi, double* y, int64_t n, int cplx,
                         hipStream_t st);
 int gemm_max_grid(int prec_in, int tile_class);
+void set_last_error(const char* msg);
 
 }  // namespace mpsa

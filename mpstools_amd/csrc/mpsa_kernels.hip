@@ -43,6 +43,21 @@ __device__ __forceinline__ f4 mfma16(float a, float b, f4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
+// Row (within a 16x16 output tile) of accumulator register r in lanes 16*lq..16*lq+15;
+// the column is lane % 16.  v_mfma_f32_16x16x4_f32 and the 16x16x32 bf16 MFMA
+// give each lane four consecutive rows (4*lq + r); v_mfma_f64_16x16x4_f64
+// interleaves them (4*r + lq).
+template <typename T>
+__device__ __forceinline__ int acc_row(int lq, int r);
+template <>
+__device__ __forceinline__ int acc_row<double>(int lq, int r) {
+  return 4 * r + lq;
+}
+template <>
+__device__ __forceinline__ int acc_row<float>(int lq, int r) {
+  return 4 * lq + r;
+}
+
 // Find the task owning tile t (tasks sorted by tile0; tile0[0] == 0).
 template <typename Task>
 __device__ __forceinline__ int find_task(const Task* tasks, int ntasks, int t, int Task::*field) {
@@ -193,7 +208,7 @@ __global__ __launch_bounds__(256, 2) void gemm16_kernel(const GTask* __restrict_
       s = ns_;
       k0 = nk;
     }
-    // epilogue: lane holds rows 4*lq + r of column l16 of each 16x16 tile
+    // epilogue: see acc_row for the row each accumulator register holds
     TOut* Cb = C + (int64_t)v * c_vs + tk.c_off;
 #pragma unroll
     for (int i = 0; i < TM; ++i)
@@ -202,12 +217,36 @@ __global__ __launch_bounds__(256, 2) void gemm16_kernel(const GTask* __restrict_
         const int col = col0 + wn * WN + j * 16 + l16;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int row = row0 + wm * WM + i * 16 + lq * 4 + r;
+          const int row = row0 + wm * WM + i * 16 + acc_row<T>(lq, r);
           if (row < m && col < n) Cb[row + (int64_t)col * m] = cvt<TOut>(acc[i][j][r]);
         }
       }
     __syncthreads();
   }
+}
+
+// Output store of the bf16 kernels: doubles as-is, bf16 as one plane or a
+// hi/lo pair (hi = bf16(x), lo = bf16(x - hi)).
+template <typename TOut, bool SPLIT>
+__device__ __forceinline__ void store_out(TOut* C, int64_t idx, int64_t plane, float x);
+template <>
+__device__ __forceinline__ void store_out<double, false>(double* C, int64_t idx, int64_t, float x) {
+  C[idx] = (double)x;
+}
+template <>
+__device__ __forceinline__ void store_out<double, true>(double* C, int64_t idx, int64_t, float x) {
+  C[idx] = (double)x;
+}
+template <>
+__device__ __forceinline__ void store_out<__hip_bfloat16, false>(__hip_bfloat16* C, int64_t idx, int64_t, float x) {
+  C[idx] = __float2bfloat16(x);
+}
+template <>
+__device__ __forceinline__ void store_out<__hip_bfloat16, true>(__hip_bfloat16* C, int64_t idx, int64_t plane,
+                                                                float x) {
+  const __hip_bfloat16 h = __float2bfloat16(x);
+  C[idx] = h;
+  C[idx + plane] = __float2bfloat16(x - __bfloat162float(h));
 }
 
 // ---------------------------------------------------------------------------
@@ -217,19 +256,26 @@ __global__ __launch_bounds__(256, 2) void gemm16_kernel(const GTask* __restrict_
 // fragment is one 16-byte load.  C is written either column-major (ld = m) in
 // TOut, or row-major (ld = n, padded to 8) when c_rowmajor is set (the layout
 // the next stage reads as a k-contiguous A operand).
+//
+// SPLIT: every operand is a pair of bf16 planes x = hi + lo (lo at a fixed
+// plane offset), and each product is hi*hi + hi*lo + lo*hi on MFMA with fp32
+// accumulation: about 16 significant bits per operand instead of 8.  Outputs
+// in bf16 are written back as hi/lo planes.
 // ---------------------------------------------------------------------------
-template <typename TOut, int BM, int BN>
+template <typename TOut, int BM, int BN, bool SPLIT>
 __global__ __launch_bounds__(256, 2) void gemm_bf16_kernel(const GTask* __restrict__ tasks, int ntasks,
                                                            const GSeg* __restrict__ segs, int tiles_per_vec,
                                                            int total_tiles, const __hip_bfloat16* __restrict__ A,
-                                                           int64_t a_vs, const __hip_bfloat16* __restrict__ B,
-                                                           int64_t b_vs, TOut* __restrict__ C, int64_t c_vs,
-                                                           int c_rowmajor) {
+                                                           int64_t a_vs, int64_t a_plane,
+                                                           const __hip_bfloat16* __restrict__ B, int64_t b_vs,
+                                                           int64_t b_plane, TOut* __restrict__ C, int64_t c_vs,
+                                                           int64_t c_plane, int c_rowmajor) {
   constexpr int BK = 32, PK = BK + 8;
+  constexpr int NP = SPLIT ? 2 : 1;
   constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
-  constexpr int NA = BM * BK / 8 / 256, NB = BN * BK / 8 / 256;  // 16-byte vectors per thread
-  __shared__ __attribute__((aligned(16))) unsigned short As[BM][PK];
-  __shared__ __attribute__((aligned(16))) unsigned short Bs[BN][PK];
+  constexpr int NA = BM * BK / 8 / 256, NB = BN * BK / 8 / 256;  // 16-byte vectors per thread and plane
+  __shared__ __attribute__((aligned(16))) unsigned short As[NP][BM][PK];
+  __shared__ __attribute__((aligned(16))) unsigned short Bs[NP][BN][PK];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
   const int l16 = lane & 15, lq = lane >> 4;
@@ -253,7 +299,7 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_kernel(const GTask* __restri
     int s = tk.seg0, send = tk.seg0 + tk.nseg;
     while (s < send && segs[s].k <= 0) ++s;
     int k0 = 0;
-    u16x8 ra[NA], rb[NB];
+    u16x8 ra[NP][NA], rb[NP][NB];
     float alpha = 1.f;
     auto load = [&](int si, int kk0) {
       const GSeg sg = segs[si];
@@ -265,30 +311,41 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_kernel(const GTask* __restri
         const int e = tid + r * 256;
         const int i = e / (BK / 8), kc = (e % (BK / 8)) * 8;
         const int gi = row0 + i, gk = kk0 + kc;
-        u16x8 val = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
-        if (gi < m && gk < kpad) val = *reinterpret_cast<const u16x8*>(Ab + (int64_t)gi * sg.lda + gk);
-        ra[r] = val;
+        const bool ok = gi < m && gk < kpad;
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+          u16x8 val = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+          if (ok) val = *reinterpret_cast<const u16x8*>(Ab + p * a_plane + (int64_t)gi * sg.lda + gk);
+          ra[p][r] = val;
+        }
       }
 #pragma unroll
       for (int r = 0; r < NB; ++r) {
         const int e = tid + r * 256;
         const int j = e / (BK / 8), kc = (e % (BK / 8)) * 8;
         const int gj = col0 + j, gk = kk0 + kc;
-        u16x8 val = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
-        if (gj < n && gk < kpad) val = *reinterpret_cast<const u16x8*>(Bb + (int64_t)gj * sg.ldb + gk);
-        rb[r] = val;
+        const bool ok = gj < n && gk < kpad;
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+          u16x8 val = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+          if (ok) val = *reinterpret_cast<const u16x8*>(Bb + p * b_plane + (int64_t)gj * sg.ldb + gk);
+          rb[p][r] = val;
+        }
       }
     };
     auto store = [&]() {
 #pragma unroll
-      for (int r = 0; r < NA; ++r) {
-        const int e = tid + r * 256;
-        *reinterpret_cast<u16x8*>(&As[e / (BK / 8)][(e % (BK / 8)) * 8]) = ra[r];
-      }
+      for (int p = 0; p < NP; ++p) {
 #pragma unroll
-      for (int r = 0; r < NB; ++r) {
-        const int e = tid + r * 256;
-        *reinterpret_cast<u16x8*>(&Bs[e / (BK / 8)][(e % (BK / 8)) * 8]) = rb[r];
+        for (int r = 0; r < NA; ++r) {
+          const int e = tid + r * 256;
+          *reinterpret_cast<u16x8*>(&As[p][e / (BK / 8)][(e % (BK / 8)) * 8]) = ra[p][r];
+        }
+#pragma unroll
+        for (int r = 0; r < NB; ++r) {
+          const int e = tid + r * 256;
+          *reinterpret_cast<u16x8*>(&Bs[p][e / (BK / 8)][(e % (BK / 8)) * 8]) = rb[p][r];
+        }
       }
     };
     if (s < send) load(s, 0);
@@ -305,28 +362,49 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_kernel(const GTask* __restri
         nk = 0;
       }
       if (ns_ < send) load(ns_, nk);
-      bf8 af[TM], bfr[TN];
+      bf8 ah[TM], bh[TN];
+#pragma unroll
+      for (int i = 0; i < TM; ++i) ah[i] = *reinterpret_cast<const bf8*>(&As[0][wm * WM + i * 16 + l16][lq * 8]);
+#pragma unroll
+      for (int j = 0; j < TN; ++j) bh[j] = *reinterpret_cast<const bf8*>(&Bs[0][wn * WN + j * 16 + l16][lq * 8]);
+      f4 part[TM][TN];
 #pragma unroll
       for (int i = 0; i < TM; ++i)
-        af[i] = *reinterpret_cast<const bf8*>(&As[wm * WM + i * 16 + l16][lq * 8]);
 #pragma unroll
-      for (int j = 0; j < TN; ++j)
-        bfr[j] = *reinterpret_cast<const bf8*>(&Bs[wn * WN + j * 16 + l16][lq * 8]);
-      if (alpha == 1.f) {
+        for (int j = 0; j < TN; ++j)
+          part[i][j] = (alpha == 1.f) ? acc[i][j] : f4{0, 0, 0, 0};
+      if (SPLIT) {
+        bf8 al[TM], bl[TN];
 #pragma unroll
         for (int i = 0; i < TM; ++i)
+          al[i] = *reinterpret_cast<const bf8*>(&As[NP - 1][wm * WM + i * 16 + l16][lq * 8]);
 #pragma unroll
-          for (int j = 0; j < TN; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
-      } else {
-        // segment scale: accumulate this k-step separately, then add alpha * partial
+        for (int j = 0; j < TN; ++j)
+          bl[j] = *reinterpret_cast<const bf8*>(&Bs[NP - 1][wn * WN + j * 16 + l16][lq * 8]);
+        // small cross terms first, then hi*hi
 #pragma unroll
         for (int i = 0; i < TM; ++i)
 #pragma unroll
           for (int j = 0; j < TN; ++j) {
-            f4 p = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], f4{0, 0, 0, 0}, 0, 0, 0);
-            acc[i][j] += alpha * p;
+            part[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], part[i][j], 0, 0, 0);
+            part[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], part[i][j], 0, 0, 0);
           }
+      }
+#pragma unroll
+      for (int i = 0; i < TM; ++i)
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+          part[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], part[i][j], 0, 0, 0);
+      if (alpha == 1.f) {
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) acc[i][j] = part[i][j];
+      } else {
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j) acc[i][j] += alpha * part[i][j];
       }
       cur = ns_;
       s = ns_;
@@ -344,7 +422,7 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_kernel(const GTask* __restri
           const int row = row0 + wm * WM + i * 16 + lq * 4 + r;
           if (row < m && col < n) {
             const int64_t idx = c_rowmajor ? ((int64_t)row * ldr + col) : (row + (int64_t)col * m);
-            Cb[idx] = cvt<TOut>(acc[i][j][r]);
+            store_out<TOut, SPLIT>(Cb, idx, c_plane, acc[i][j][r]);
           }
         }
       }
@@ -418,11 +496,45 @@ __global__ __launch_bounds__(256) void mix_kernel<double, double>(const MixTask*
   }
 }
 
+// W mix on hi/lo bf16 planes: dst = split(sum_t coef_t * (hi_t + lo_t)).
+__global__ __launch_bounds__(256) void mix_split_kernel(const MixTask* __restrict__ tasks, int ntasks,
+                                                        const MixTerm* __restrict__ terms, int chunks_per_vec,
+                                                        int total_chunks, const __hip_bfloat16* __restrict__ src,
+                                                        __hip_bfloat16* __restrict__ dst, int64_t vs, int64_t plane) {
+  for (int g = blockIdx.x; g < total_chunks; g += gridDim.x) {
+    const int v = g / chunks_per_vec;
+    const int c = g - v * chunks_per_vec;
+    const MixTask tk = tasks[find_task(tasks, ntasks, c, &MixTask::chunk0)];
+    const int64_t base = (int64_t)(c - tk.chunk0) * 1024;
+    const __hip_bfloat16* sv = src + (int64_t)v * vs;
+    __hip_bfloat16* dv = dst + (int64_t)v * vs + tk.dst_off;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int64_t e = base + threadIdx.x + r * 256;
+      if (e < tk.len) {
+        float acc = 0.f;
+        for (int q = 0; q < tk.nt; ++q) {
+          const MixTerm tm = terms[tk.t0 + q];
+          const float x = __bfloat162float(sv[tm.src_off + e]) + __bfloat162float(sv[tm.src_off + e + plane]);
+          acc += (float)tm.coef * x;
+        }
+        const __hip_bfloat16 h = __float2bfloat16(acc);
+        dv[e] = h;
+        dv[e + plane] = __float2bfloat16(acc - __bfloat162float(h));
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 template <typename TO>
 __global__ void convert_kernel(const double* __restrict__ src, TO* __restrict__ dst, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     dst[i] = cvt<TO>(src[i]);
+}
+
+__global__ void fill_kernel(double* __restrict__ p, int64_t n, double v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
 __global__ void zero_ranges_kernel(const Range* __restrict__ r, int nr, double* __restrict__ base, int64_t vs) {
@@ -518,6 +630,34 @@ __global__ __launch_bounds__(256) void repack_kernel(const Repack* __restrict__ 
   }
 }
 
+// Repack into hi/lo bf16 planes: hi = bf16(x), lo = bf16(x - hi).
+__global__ __launch_bounds__(256) void repack_split_kernel(const Repack* __restrict__ tasks, int ntasks,
+                                                           int chunks_per_vec, int total, const double* __restrict__ src,
+                                                           int64_t s_vs, __hip_bfloat16* __restrict__ dst,
+                                                           int64_t d_vs, int64_t plane) {
+  for (int g = blockIdx.x; g < total; g += gridDim.x) {
+    const int v = g / chunks_per_vec;
+    const int c = g - v * chunks_per_vec;
+    const Repack tk = tasks[find_task(tasks, ntasks, c, &Repack::chunk0)];
+    const int64_t len = (int64_t)tk.rows * tk.cols;
+    const int64_t base = (int64_t)(c - tk.chunk0) * 1024;
+    const double* sp = src + (int64_t)v * s_vs + tk.src_off;
+    __hip_bfloat16* d = dst + (int64_t)v * d_vs + tk.dst_off;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int64_t e = base + threadIdx.x + r * 256;
+      if (e < len) {
+        const int64_t i = e % tk.rows, j = e / tk.rows;
+        const int64_t di = tk.transpose ? (j + i * tk.dst_ld) : (i + j * tk.dst_ld);
+        const double x = sp[e];
+        const __hip_bfloat16 h = __float2bfloat16((float)x);
+        d[di] = h;
+        d[di + plane] = __float2bfloat16((float)(x - (double)__bfloat162float(h)));
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
@@ -559,24 +699,24 @@ static hipError_t run16(int tr, const GTask* tasks, int ntasks, const GSeg* segs
   return hipGetLastError();
 }
 
-template <typename TOut, int BM, int BN>
+template <typename TOut, int BM, int BN, bool SPLIT>
 static hipError_t runbf(const GTask* tasks, int ntasks, const GSeg* segs, int tpv, int nvec, const void* A,
-                        int64_t a_vs, const void* B, int64_t b_vs, void* C, int64_t c_vs, int c_rowmajor,
-                        hipStream_t st) {
+                        int64_t a_vs, int64_t a_pl, const void* B, int64_t b_vs, int64_t b_pl, void* C, int64_t c_vs,
+                        int64_t c_pl, int c_rowmajor, hipStream_t st) {
   const int total = tpv * nvec;
   if (total <= 0) return hipSuccess;
   int grid = num_cu() * 2;
   if (grid > total) grid = total;
   if (grid >= 8) grid &= ~7;
-  gemm_bf16_kernel<TOut, BM, BN><<<grid, 256, 0, st>>>(tasks, ntasks, segs, tpv, total, (const __hip_bfloat16*)A,
-                                                       a_vs, (const __hip_bfloat16*)B, b_vs, (TOut*)C, c_vs,
-                                                       c_rowmajor);
+  gemm_bf16_kernel<TOut, BM, BN, SPLIT><<<grid, 256, 0, st>>>(
+      tasks, ntasks, segs, tpv, total, (const __hip_bfloat16*)A, a_vs, a_pl, (const __hip_bfloat16*)B, b_vs, b_pl,
+      (TOut*)C, c_vs, c_pl, c_rowmajor);
   return hipGetLastError();
 }
 
 hipError_t launch_gemm(int prec_in, int out_kind, int tr, int tile_class, const GTask* tasks, int ntasks,
                        const GSeg* segs, int tiles_per_vec, int nvec, const void* A, int64_t a_vs, const void* B,
-                       int64_t b_vs, void* C, int64_t c_vs, hipStream_t st) {
+                       int64_t b_vs, void* C, int64_t c_vs, hipStream_t st, Planes planes) {
   if (prec_in == F64) {
     if (tile_class == 0) return run16<double, double, 128, 128>(tr, tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, st);
     return run16<double, double, 64, 64>(tr, tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, st);
@@ -589,20 +729,34 @@ hipError_t launch_gemm(int prec_in, int out_kind, int tr, int tile_class, const 
     if (tile_class == 0) return run16<float, float, 128, 128>(tr, tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, st);
     return run16<float, float, 64, 64>(tr, tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, st);
   }
-  // BF16: tr must be the canonical k-contiguous form; out_kind 0 -> bf16 row-major, 1 -> double col-major
-  if (out_kind == 1) {
-    if (tile_class == 0) return runbf<double, 128, 128>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, 0, st);
-    return runbf<double, 64, 64>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, 0, st);
+  // BF16 / BF16X3: operands in the k-contiguous layout; out_kind 0 -> bf16
+  // row-major (hi/lo planes for BF16X3), 1 -> double column-major.
+  if (prec_in == BF16) {
+    if (out_kind == 1) {
+      if (tile_class == 0) return runbf<double, 128, 128, false>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, 0, B, b_vs, 0, C, c_vs, 0, 0, st);
+      return runbf<double, 64, 64, false>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, 0, B, b_vs, 0, C, c_vs, 0, 0, st);
+    }
+    if (tile_class == 0) return runbf<__hip_bfloat16, 128, 128, false>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, 0, B, b_vs, 0, C, c_vs, 0, 1, st);
+    return runbf<__hip_bfloat16, 64, 64, false>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, 0, B, b_vs, 0, C, c_vs, 0, 1, st);
   }
-  if (tile_class == 0) return runbf<__hip_bfloat16, 128, 128>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, 1, st);
-  return runbf<__hip_bfloat16, 64, 64>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, 1, st);
+  if (out_kind == 1) {
+    if (tile_class == 0) return runbf<double, 128, 128, true>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, planes.a, B, b_vs, planes.b, C, c_vs, 0, 0, st);
+    return runbf<double, 64, 64, true>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, planes.a, B, b_vs, planes.b, C, c_vs, 0, 0, st);
+  }
+  if (tile_class == 0) return runbf<__hip_bfloat16, 128, 128, true>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, planes.a, B, b_vs, planes.b, C, c_vs, planes.c, 1, st);
+  return runbf<__hip_bfloat16, 64, 64, true>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, planes.a, B, b_vs, planes.b, C, c_vs, planes.c, 1, st);
 }
 
 hipError_t launch_mix(int prec, const MixTask* tasks, int ntasks, const MixTerm* terms, int chunks_per_vec, int nvec,
-                      const void* src, void* dst, int64_t vs, hipStream_t st) {
+                      const void* src, void* dst, int64_t vs, hipStream_t st, int64_t plane) {
   const int total = chunks_per_vec * nvec;
   if (total <= 0 || ntasks <= 0) return hipSuccess;
   int grid = total < 8192 ? total : 8192;
+  if (prec == BF16X3) {
+    mix_split_kernel<<<grid, 256, 0, st>>>(tasks, ntasks, terms, chunks_per_vec, total, (const __hip_bfloat16*)src,
+                                           (__hip_bfloat16*)dst, vs, plane);
+    return hipGetLastError();
+  }
   if (prec == F64)
     mix_kernel<double, double><<<grid, 256, 0, st>>>(tasks, ntasks, terms, chunks_per_vec, total, (const double*)src, (double*)dst, vs);
   else if (prec == F32)
@@ -623,16 +777,29 @@ hipError_t launch_convert(int prec, const double* src, void* dst, int64_t n, hip
 }
 
 hipError_t launch_repack(int prec, const Repack* tasks, int ntasks, int chunks_per_vec, int nvec, const double* src,
-                         int64_t s_vs, void* dst, int64_t d_vs, hipStream_t st) {
+                         int64_t s_vs, void* dst, int64_t d_vs, hipStream_t st, int64_t plane) {
   const int total = chunks_per_vec * nvec;
   if (total <= 0 || ntasks <= 0) return hipSuccess;
   int grid = total < 8192 ? total : 8192;
+  if (prec == BF16X3) {
+    repack_split_kernel<<<grid, 256, 0, st>>>(tasks, ntasks, chunks_per_vec, total, src, s_vs, (__hip_bfloat16*)dst,
+                                              d_vs, plane);
+    return hipGetLastError();
+  }
   if (prec == F32)
     repack_kernel<float><<<grid, 256, 0, st>>>(tasks, ntasks, chunks_per_vec, total, src, s_vs, (float*)dst, d_vs);
   else if (prec == BF16)
     repack_kernel<__hip_bfloat16><<<grid, 256, 0, st>>>(tasks, ntasks, chunks_per_vec, total, src, s_vs, (__hip_bfloat16*)dst, d_vs);
   else
     repack_kernel<double><<<grid, 256, 0, st>>>(tasks, ntasks, chunks_per_vec, total, src, s_vs, (double*)dst, d_vs);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  int64_t blocks = (n + 255) / 256;
+  int grid = (int)(blocks < 16384 ? blocks : 16384);
+  fill_kernel<<<grid, 256, 0, st>>>(p, n, v);
   return hipGetLastError();
 }
 

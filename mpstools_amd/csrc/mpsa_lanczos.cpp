@@ -1,0 +1,267 @@
+// mpsa_lanczos.cpp -- device-resident plain Lanczos of TSLanzcosEngine
+// (src/lanzcos.hpp:165-278) and getState (:81-92), with the host-side FP64
+// symmetric tridiagonal eigensolver replacing LAPACK dstev
+// (blas/lapack_routines.cpp:435-446).
+//
+// The recurrence is the reference's, step for step: k = ||A1||, A1 /= k,
+// A2 = H A1, e = <A1|A2>, A3 = (-e A1 + A2) + (-k A0), every _Ndiag = 5 steps a
+// convergence test on the lowest Ritz value (testenergy) or on the Ritz-vector
+// residual, stop on |k| < delta or it == itmax.  All vectors live on the device
+// in the plan's vector layout; the Krylov basis is kept (the reference keeps
+// _As) and the requested Ritz state is assembled on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/mpstools_amd.h"
+#include "mpsa_internal.h"
+
+namespace mpsa {
+
+// Implicit QL iteration with Wilkinson-type shifts for a symmetric tridiagonal
+// matrix (Bowdler, Martin, Reinsch & Wilkinson, Numer. Math. 11 (1968) 293),
+// accumulating the rotations into z (n x n column-major, starts as identity).
+// d: diagonal (eigenvalues on exit, ascending); e: n-1 off-diagonals.
+int tridiag_ql(int n, double* d, const double* e_in, double* z) {
+  if (n <= 0) return 0;
+  std::vector<double> e((size_t)n, 0.0);
+  for (int i = 0; i + 1 < n; ++i) e[(size_t)i] = e_in[i];
+  if (z) {
+    std::fill(z, z + (size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i) z[(size_t)i * n + i] = 1.0;
+  }
+  const double eps = std::numeric_limits<double>::epsilon();
+  for (int l = 0; l < n; ++l) {
+    int iter = 0;
+    int m;
+    do {
+      for (m = l; m < n - 1; ++m) {
+        const double dd = std::fabs(d[m]) + std::fabs(d[m + 1]);
+        if (std::fabs(e[(size_t)m]) <= eps * dd) break;
+      }
+      if (m != l) {
+        if (++iter > 100) return 1;
+        double g = (d[l + 1] - d[l]) / (2.0 * e[(size_t)l]);
+        double r = std::hypot(g, 1.0);
+        g = d[m] - d[l] + e[(size_t)l] / (g + std::copysign(r, g));
+        double s = 1.0, c = 1.0, p = 0.0;
+        int i;
+        for (i = m - 1; i >= l; --i) {
+          double f = s * e[(size_t)i];
+          const double b = c * e[(size_t)i];
+          r = std::hypot(f, g);
+          e[(size_t)i + 1] = r;
+          if (r == 0.0) {
+            d[i + 1] -= p;
+            e[(size_t)m] = 0.0;
+            break;
+          }
+          s = f / r;
+          c = g / r;
+          g = d[i + 1] - p;
+          r = (d[i] - g) * s + 2.0 * c * b;
+          p = s * r;
+          d[i + 1] = g + p;
+          g = c * r - b;
+          if (z)
+            for (int k = 0; k < n; ++k) {
+              f = z[(size_t)(i + 1) * n + k];
+              z[(size_t)(i + 1) * n + k] = s * z[(size_t)i * n + k] + c * f;
+              z[(size_t)i * n + k] = c * z[(size_t)i * n + k] - s * f;
+            }
+        }
+        if (r == 0.0 && i >= l) continue;
+        d[l] -= p;
+        e[(size_t)l] = g;
+        e[(size_t)m] = 0.0;
+      }
+    } while (m != l);
+  }
+  // ascending order (selection sort keeps it simple for the small n of Lanczos)
+  for (int i = 0; i < n - 1; ++i) {
+    int k = i;
+    for (int j = i + 1; j < n; ++j)
+      if (d[j] < d[k]) k = j;
+    if (k != i) {
+      std::swap(d[i], d[k]);
+      if (z)
+        for (int r = 0; r < n; ++r) std::swap(z[(size_t)i * n + r], z[(size_t)k * n + r]);
+    }
+  }
+  return 0;
+}
+
+// Pool of device vectors (Krylov bases are reused across Lanczos calls).
+struct VecPool {
+  std::map<int64_t, std::vector<double*>> free_;
+  double* get(int64_t n) {
+    auto& f = free_[n];
+    if (!f.empty()) {
+      double* p = f.back();
+      f.pop_back();
+      return p;
+    }
+    double* p = nullptr;
+    if (hipMalloc((void**)&p, sizeof(double) * (size_t)std::max<int64_t>(n, 1)) != hipSuccess) return nullptr;
+    return p;
+  }
+  void put(int64_t n, double* p) {
+    if (p) free_[n].push_back(p);
+  }
+  void release() {
+    for (auto& kv : free_)
+      for (double* p : kv.second) (void)hipFree(p);
+    free_.clear();
+  }
+};
+static VecPool g_pool;
+
+}  // namespace mpsa
+
+using namespace mpsa;
+
+namespace {
+struct LErr : std::runtime_error {
+  int code;
+  LErr(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void chk(int rc) {
+  if (rc != 0) throw LErr(rc, mpsa_last_error());
+}
+void hchk(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw LErr(MPSA_EHIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// Vectors handed out zero-filled: layouts can have alignment gaps between
+// blocks that no kernel writes, and every dot/axpby runs over the full extent.
+struct Vecs {
+  std::vector<double*> v;
+  int64_t n;
+  hipStream_t st;
+  ~Vecs() {
+    for (double* p : v) g_pool.put(n, p);
+  }
+  double* get() {
+    double* p = g_pool.get(n);
+    if (!p) throw LErr(MPSA_ENOMEM, "Lanczos vector allocation failed");
+    v.push_back(p);
+    hchk(launch_fill(p, n, 0.0, st), "zero vector");
+    return p;
+  }
+};
+
+double ddot(const double* x, const double* y, int64_t n, hipStream_t st) {
+  double out[2];
+  chk(mpsa_dot(x, y, n, 0, out, (mpsa_stream)st));
+  return out[0];
+}
+void axpby(double a, const double* x, double b, double* y, int64_t n, hipStream_t st) {
+  double av[2] = {a, 0.0}, bv[2] = {b, 0.0};
+  chk(mpsa_axpby(av, x, bv, y, n, 0, (mpsa_stream)st));
+}
+}  // namespace
+
+extern "C" {
+
+int mpsa_tridiag_eig(int n, double* d, const double* e, double* z) {
+  if (n < 0 || (n > 0 && !d) || (n > 1 && !e)) return MPSA_EINVAL;
+  return tridiag_ql(n, d, e, z) == 0 ? MPSA_OK : MPSA_EINVAL;
+}
+
+int mpsa_lanczos(mpsa_plan* plan, const double* dpsi0, double* dstate, int itmax, double delta, int testenergy,
+                 int target, double* energies, int cap, int* nit, int* nritz, mpsa_stream stream) {
+  try {
+    if (!plan || !dpsi0 || itmax < 1 || !nit || !nritz) throw LErr(MPSA_EINVAL, "lanczos: bad arguments");
+    int64_t xl = 0, yl = 0;
+    int same = 0;
+    chk(mpsa_plan_vector_lengths(plan, &xl, &yl, &same));
+    if (!same) throw LErr(MPSA_EINVAL, "lanczos: the plan's x and y tables must be the same vector space");
+    const int64_t n = xl;
+    hipStream_t st = (hipStream_t)stream;
+    Vecs V{{}, n, st};
+    std::vector<double*> As;
+    double* A0 = nullptr;
+    double* A1 = V.get();
+    double* A2 = V.get();
+    hchk(hipMemcpyAsync(A1, dpsi0, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, st), "copy psi0");
+    std::vector<double> es, ks;
+    int it = 0;
+    double k = 0.0, e = 0.0, eold = 1e100;
+    bool first = true;
+    const int Ndiag = 5;
+    while (true) {
+      k = std::sqrt(ddot(A1, A1, n, st));
+      if (std::fabs(k) < delta) break;
+      if (!first) ks.push_back(k);
+      axpby(0.0, A1, 1.0 / k, A1, n, st);                  // A1 /= k
+      chk(mpsa_heff_apply(plan, A1, 0, A2, 0, 1, stream));  // A2 = H A1
+      As.push_back(A1);                                      // the basis keeps A1
+      e = ddot(A2, A1, n, st);
+      es.push_back(e);
+      if (it % Ndiag == 0 && it != 0) {
+        std::vector<double> d(es), sub(ks);
+        if (testenergy) {
+          if (tridiag_ql((int)d.size(), d.data(), sub.data(), nullptr)) throw LErr(MPSA_EINVAL, "tridiag failed");
+          if (std::fabs(d[0] - eold) < delta) break;
+          eold = d[0];
+        } else {
+          std::vector<double> z(d.size() * d.size());
+          if (tridiag_ql((int)d.size(), d.data(), sub.data(), z.data())) throw LErr(MPSA_EINVAL, "tridiag failed");
+          double* A4 = V.get();
+          double* A5 = V.get();
+          axpby(z[0], As[0], 0.0, A4, n, st);
+          for (size_t i = 1; i < d.size(); ++i) axpby(z[i], As[i], 1.0, A4, n, st);
+          chk(mpsa_heff_apply(plan, A4, 0, A5, 0, 1, stream));
+          axpby(-d[0], A4, 1.0, A5, n, st);
+          const double resi = std::sqrt(ddot(A5, A5, n, st));
+          if (resi < delta) break;
+        }
+      }
+      // A3 = (-e A1 + A2) + (-k A0), built in A2's storage
+      axpby(-e, A1, 1.0, A2, n, st);
+      if (!first) axpby(-k, A0, 1.0, A2, n, st);
+      first = false;
+      A0 = A1;
+      A1 = A2;
+      A2 = V.get();
+      ++it;
+      if (it >= itmax) break;
+    }
+    *nit = it;
+    *nritz = (int)es.size();
+    if (!es.empty()) {
+      const int m = (int)es.size();
+      std::vector<double> d(es), sub(ks), z((size_t)m * m);
+      if (tridiag_ql(m, d.data(), sub.data(), z.data())) throw LErr(MPSA_EINVAL, "tridiag failed");
+      for (int i = 0; i < m && i < cap; ++i) energies[i] = d[(size_t)i];
+      if (dstate) {
+        if (target < 0 || target >= m) throw LErr(MPSA_EINVAL, "LanzcosEngine::getState(const int j): j<_eigs.size2() failed");
+        hchk(launch_fill(dstate, n, 0.0, st), "zero state");
+        axpby(z[(size_t)target * m + 0], As[0], 0.0, dstate, n, st);
+        for (int i = 1; i < m; ++i) axpby(z[(size_t)target * m + i], As[(size_t)i], 1.0, dstate, n, st);
+      }
+    }
+    hchk(hipStreamSynchronize(st), "lanczos sync");
+    return MPSA_OK;
+  } catch (const LErr& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return MPSA_EINVAL;
+  }
+}
+
+int mpsa_release_pools(void) {
+  g_pool.release();
+  return MPSA_OK;
+}
+
+}  // extern "C"

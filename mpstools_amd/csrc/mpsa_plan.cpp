@@ -27,6 +27,7 @@
 namespace mpsa {
 
 static thread_local std::string g_err;
+void set_last_error(const char* msg) { g_err = msg; }
 
 struct Error : std::runtime_error {
   int code;
@@ -49,6 +50,7 @@ struct BKey {
     if (p0 != o.p0) return p0 < o.p0;
     return p1 < o.p1;
   }
+  bool operator==(const BKey& o) const { return q0 == o.q0 && q1 == o.q1 && p0 == o.p0 && p1 == o.p1; }
 };
 
 struct Tab {
@@ -165,6 +167,7 @@ struct mpsa_plan {
   std::vector<Repack> rpL, rpR, rpx;
   int rpL_chunks = 0, rpR_chunks = 0, rpx_chunks = 0;
   int64_t Lc_elems = 0, Rc_elems = 0;
+  int planes = 1;  // 2: hi/lo bf16 planes (BF16X3)
   // device
   MixTask* d_mix = nullptr;
   MixTerm* d_terms = nullptr;
@@ -181,6 +184,8 @@ struct mpsa_plan {
   double flops_alg = 0, flops_ref = 0;
   double flops1 = 0, flops2 = 0;  // algorithmic flops of stage 1 / stage 2 (one vector)
   int timing = 0;
+  int on_device = 0;
+  int xy_same = 0;
   hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   ~mpsa_plan() {
     for (auto& e : ev)
@@ -242,8 +247,6 @@ static void finalize_stage(Stage& s) {
   }
   s.tasks = t2;
   s.tiles = tile;
-  s.d_tasks = upload(s.tasks);
-  s.d_segs = upload(s.segs);
 }
 
 static void finalize_mix(mpsa_plan* P) {
@@ -257,8 +260,6 @@ static void finalize_mix(mpsa_plan* P) {
   }
   P->mix = m2;
   P->mix_chunks = chunk;
-  P->d_mix = upload(P->mix);
-  P->d_terms = upload(P->terms);
 }
 
 static int finalize_repack(std::vector<Repack>& r) {
@@ -290,8 +291,10 @@ struct Built {
   double flops_ref = 0;
 };
 
+// plain = true keeps the reference's schedule: one product per tuple
+// (F_ref flops), no sharing of (P, Q) products and no mix.
 static Built group_tuples(const std::vector<Tuple>& tup, const Tab* ytab, bool need_y,
-                          std::map<BKey, std::pair<int64_t, int64_t>>* out_dims) {
+                          std::map<BKey, std::pair<int64_t, int64_t>>* out_dims, bool plain = false) {
   Built b;
   std::map<std::pair<int, int>, int> pidx;
   std::map<std::pair<int, int>, int> sidx;  // (iout, iu) -> slot; iout = y index or key ordinal
@@ -307,10 +310,11 @@ static Built group_tuples(const std::vector<Tuple>& tup, const Tab* ytab, bool n
   for (auto& kv : outord) kv.second = ord++;
   for (auto& kv : outord) b.outkeys.push_back(kv.first);
   if (!need_y) return b;
+  int tix = 0;
   for (auto& t : tup) {
     int iout = ytab->find(t.out);
     if (iout < 0) throw Error(MPSA_ENOKEY, "output table lacks a produced key");
-    auto pk = std::make_pair(t.ia, t.ib);
+    auto pk = plain ? std::make_pair(tix, -1) : std::make_pair(t.ia, t.ib);
     int pi;
     auto it = pidx.find(pk);
     if (it == pidx.end()) {
@@ -320,7 +324,8 @@ static Built group_tuples(const std::vector<Tuple>& tup, const Tab* ytab, bool n
     } else {
       pi = it->second;
     }
-    auto sk = std::make_pair(iout, t.iu);
+    auto sk = plain ? std::make_pair(iout, tix) : std::make_pair(iout, t.iu);
+    ++tix;
     int si;
     auto jt = sidx.find(sk);
     if (jt == sidx.end()) {
@@ -521,7 +526,7 @@ static int64_t write_keys(const std::vector<BKey>& keys, const std::map<BKey, st
 //   env<=0: T1 = P * Q^T (P = BR, Q = A), out = U * T2 (U = Ap)
 static void build_plan(mpsa_plan* P, Built& b, const Tab& Ptab, const Tab& Qtab, const Tab& Utab, const Tab& Ytab,
                        int kind, int direction) {
-  const bool bf = P->prec == BF16;
+  const bool bf = P->prec == BF16 || P->prec == BF16X3;
   // workspace layout
   int64_t off = 0;
   for (auto& pr : b.prods) {
@@ -683,21 +688,35 @@ static void build_plan(mpsa_plan* P, Built& b, const Tab& Ptab, const Tab& Qtab,
   finalize_stage(P->s1);
   finalize_stage(P->s2);
   finalize_mix(P);
-  P->d_zero = upload(P->zero);
   P->rpL_chunks = finalize_repack(P->rpL);
   P->rpR_chunks = finalize_repack(P->rpR);
   P->rpx_chunks = finalize_repack(P->rpx);
+  P->esize = P->prec == F64 ? 8 : (P->prec == F32 ? 4 : 2);
+  P->planes = P->prec == BF16X3 ? 2 : 1;
+}
+
+// Plans are built on the host only; descriptors move to the device on first use.
+static void ensure_device(mpsa_plan* P) {
+  if (P->on_device) return;
+  P->s1.d_tasks = upload(P->s1.tasks);
+  P->s1.d_segs = upload(P->s1.segs);
+  P->s2.d_tasks = upload(P->s2.tasks);
+  P->s2.d_segs = upload(P->s2.segs);
+  P->d_mix = upload(P->mix);
+  P->d_terms = upload(P->terms);
+  P->d_zero = upload(P->zero);
   P->d_rpL = upload(P->rpL);
   P->d_rpR = upload(P->rpR);
   P->d_rpx = upload(P->rpx);
-  P->esize = P->prec == F64 ? 8 : (P->prec == F32 ? 4 : 2);
+  P->on_device = 1;
 }
 
 static void ensure_ws(mpsa_plan* P, int nvec) {
+  ensure_device(P);
   if (P->ws_nvec < nvec) {
     if (P->ws) HIPCHK(hipFree(P->ws));
     P->ws = nullptr;
-    const size_t bytes = (size_t)P->ws_per_vec * P->esize * (size_t)nvec;
+    const size_t bytes = (size_t)P->ws_per_vec * P->planes * P->esize * (size_t)nvec;
     if (hipMalloc(&P->ws, bytes) != hipSuccess) {
       P->ws_nvec = 0;
       throw Error(MPSA_ENOMEM, "workspace allocation failed");
@@ -708,7 +727,7 @@ static void ensure_ws(mpsa_plan* P, int nvec) {
   if (P->prec != F64 && P->xc_nvec < nvec) {
     if (P->xc) HIPCHK(hipFree(P->xc));
     P->xc = nullptr;
-    const size_t bytes = (size_t)P->xc_per_vec * P->esize * (size_t)nvec;
+    const size_t bytes = (size_t)P->xc_per_vec * P->planes * P->esize * (size_t)nvec;
     if (hipMalloc(&P->xc, bytes) != hipSuccess) {
       P->xc_nvec = 0;
       throw Error(MPSA_ENOMEM, "converted-vector allocation failed");
@@ -795,7 +814,10 @@ static int plan_heff(mpsa_plan** plan, const mpsa_blocks* L, const mpsa_blocks* 
                      const mpsa_blocks* y, const std::vector<Tuple>& tup, double fref, int nsx, int precision) {
   (void)nsx;
   if (!plan) throw Error(MPSA_EINVAL, "null plan pointer");
-  if (precision < 0 || precision > 2) throw Error(MPSA_EINVAL, "precision must be MPSA_F64, MPSA_F32 or MPSA_BF16");
+  const bool plain = (precision & MPSA_PLAIN) != 0;
+  precision &= ~MPSA_PLAIN;
+  if (precision < 0 || precision > 3)
+    throw Error(MPSA_EINVAL, "precision must be MPSA_F64, MPSA_F32, MPSA_BF16 or MPSA_BF16X3");
   std::unique_ptr<mpsa_plan> P(new mpsa_plan());
   P->kind = 0;
   P->prec = precision;
@@ -805,7 +827,7 @@ static int plan_heff(mpsa_plan** plan, const mpsa_blocks* L, const mpsa_blocks* 
   P->x_total = tx.total;
   P->y_total = ty.total;
   std::map<BKey, std::pair<int64_t, int64_t>> dims;
-  Built b = group_tuples(tup, &ty, true, &dims);
+  Built b = group_tuples(tup, &ty, true, &dims, plain);
   for (auto& k : b.outkeys) {
     int iy = ty.find(k);
     auto dd = dims.at(k);
@@ -814,6 +836,8 @@ static int plan_heff(mpsa_plan** plan, const mpsa_blocks* L, const mpsa_blocks* 
   }
   build_plan(P.get(), b, tl, tx, tr, ty, 0, 1);
   P->flops_ref = fref;
+  P->xy_same = (tx.n == ty.n && tx.ns == ty.ns && tx.rows == ty.rows && tx.cols == ty.cols && tx.off == ty.off &&
+                tx.key == ty.key) ? 1 : 0;
   *plan = P.release();
   return MPSA_OK;
 }
@@ -890,19 +914,21 @@ int mpsa_plan_bind_env(mpsa_plan* P, const double* dL, const double* dR, mpsa_st
     if (!P || P->kind != 0) throw Error(MPSA_EINVAL, "bind_env: not an H_eff plan");
     if ((P->L_total > 0 && !dL) || (P->R_total > 0 && !dR)) throw Error(MPSA_EINVAL, "bind_env: null environment");
     hipStream_t st = (hipStream_t)stream;
+    ensure_device(P);
     P->dL = dL;
     P->dR = dR;
     if (P->prec != F64) {
+      const size_t lb = (size_t)P->Lc_elems * P->planes * P->esize, rb = (size_t)P->Rc_elems * P->planes * P->esize;
       if (!P->Lc) {
-        if (hipMalloc(&P->Lc, (size_t)P->Lc_elems * P->esize) != hipSuccess) throw Error(MPSA_ENOMEM, "env copy");
-        HIPCHK(hipMemset(P->Lc, 0, (size_t)P->Lc_elems * P->esize));
+        if (hipMalloc(&P->Lc, lb) != hipSuccess) throw Error(MPSA_ENOMEM, "env copy");
+        HIPCHK(hipMemset(P->Lc, 0, lb));
       }
       if (!P->Rc) {
-        if (hipMalloc(&P->Rc, (size_t)P->Rc_elems * P->esize) != hipSuccess) throw Error(MPSA_ENOMEM, "env copy");
-        HIPCHK(hipMemset(P->Rc, 0, (size_t)P->Rc_elems * P->esize));
+        if (hipMalloc(&P->Rc, rb) != hipSuccess) throw Error(MPSA_ENOMEM, "env copy");
+        HIPCHK(hipMemset(P->Rc, 0, rb));
       }
-      HIPCHK(launch_repack(P->prec, P->d_rpL, (int)P->rpL.size(), P->rpL_chunks, 1, dL, 0, P->Lc, 0, st));
-      HIPCHK(launch_repack(P->prec, P->d_rpR, (int)P->rpR.size(), P->rpR_chunks, 1, dR, 0, P->Rc, 0, st));
+      HIPCHK(launch_repack(P->prec, P->d_rpL, (int)P->rpL.size(), P->rpL_chunks, 1, dL, 0, P->Lc, 0, st, P->Lc_elems));
+      HIPCHK(launch_repack(P->prec, P->d_rpR, (int)P->rpR.size(), P->rpR_chunks, 1, dR, 0, P->Rc, 0, st, P->Rc_elems));
     }
   })
 }
@@ -921,21 +947,29 @@ int mpsa_heff_apply(mpsa_plan* P, const double* dx, int64_t x_stride, double* dy
     if (P->timing) HIPCHK(hipEventRecord(P->ev[0], st));
     const void* A1 = P->prec == F64 ? (const void*)P->dL : P->Lc;
     const void* B1 = P->prec == F64 ? (const void*)dx : P->xc;
-    int64_t b1_vs = P->prec == F64 ? x_stride : P->xc_per_vec;
+    const int64_t xcs = P->xc_per_vec * P->planes, wss = P->ws_per_vec * P->planes;
+    int64_t b1_vs = P->prec == F64 ? x_stride : xcs;
     if (P->prec != F64)
-      HIPCHK(launch_repack(P->prec, P->d_rpx, (int)P->rpx.size(), P->rpx_chunks, nvec, dx, x_stride, P->xc,
-                           P->xc_per_vec, st));
+      HIPCHK(launch_repack(P->prec, P->d_rpx, (int)P->rpx.size(), P->rpx_chunks, nvec, dx, x_stride, P->xc, xcs, st,
+                           P->xc_per_vec));
     if (P->timing) HIPCHK(hipEventRecord(P->ev[1], st));
+    Planes pl1;
+    pl1.a = P->Lc_elems;
+    pl1.b = P->xc_per_vec;
+    pl1.c = P->ws_per_vec;
     HIPCHK(launch_gemm(P->prec, 0, P->s1.tr, P->s1.tile_class, P->s1.d_tasks, (int)P->s1.tasks.size(),
-                       P->s1.d_segs, P->s1.tiles, nvec, A1, 0, B1, b1_vs, P->ws, P->ws_per_vec, st));
+                       P->s1.d_segs, P->s1.tiles, nvec, A1, 0, B1, b1_vs, P->ws, wss, st, pl1));
     if (P->timing) HIPCHK(hipEventRecord(P->ev[2], st));
-    HIPCHK(launch_mix(P->prec, P->d_mix, (int)P->mix.size(), P->d_terms, P->mix_chunks, nvec, P->ws, P->ws,
-                      P->ws_per_vec, st));
+    HIPCHK(launch_mix(P->prec, P->d_mix, (int)P->mix.size(), P->d_terms, P->mix_chunks, nvec, P->ws, P->ws, wss, st,
+                      P->ws_per_vec));
     const void* B2 = P->prec == F64 ? (const void*)P->dR : P->Rc;
     HIPCHK(launch_zero_ranges(P->d_zero, (int)P->zero.size(), dy, y_stride, nvec, st));
     if (P->timing) HIPCHK(hipEventRecord(P->ev[3], st));
+    Planes pl2;
+    pl2.a = P->ws_per_vec;
+    pl2.b = P->Rc_elems;
     HIPCHK(launch_gemm(P->prec, 1, P->s2.tr, P->s2.tile_class, P->s2.d_tasks, (int)P->s2.tasks.size(),
-                       P->s2.d_segs, P->s2.tiles, nvec, P->ws, P->ws_per_vec, B2, 0, dy, y_stride, st));
+                       P->s2.d_segs, P->s2.tiles, nvec, P->ws, wss, B2, 0, dy, y_stride, st, pl2));
     if (P->timing) HIPCHK(hipEventRecord(P->ev[4], st));
   })
 }
@@ -969,6 +1003,15 @@ int mpsa_plan_info(const mpsa_plan* P, double* flops_alg, double* flops_ref, int
   })
 }
 
+int mpsa_plan_vector_lengths(const mpsa_plan* P, int64_t* x_len, int64_t* y_len, int* same_layout) {
+  MPSA_TRY({
+    if (!P) throw Error(MPSA_EINVAL, "null plan");
+    if (x_len) *x_len = P->x_total;
+    if (y_len) *y_len = P->y_total;
+    if (same_layout) *same_layout = P->xy_same;
+  })
+}
+
 int mpsa_plan_stage_info(const mpsa_plan* P, double* flops_stage1, double* flops_stage2, int* tile_class1,
                          int* tile_class2) {
   MPSA_TRY({
@@ -977,6 +1020,31 @@ int mpsa_plan_stage_info(const mpsa_plan* P, double* flops_stage1, double* flops
     if (flops_stage2) *flops_stage2 = P->flops2;
     if (tile_class1) *tile_class1 = P->s1.tile_class;
     if (tile_class2) *tile_class2 = P->s2.tile_class;
+  })
+}
+
+int mpsa_plan_describe(const mpsa_plan* P, int what, void* buf, int64_t cap, int64_t* n, int* extra) {
+  MPSA_TRY({
+    if (!P || !n) throw Error(MPSA_EINVAL, "null argument");
+    const void* src = nullptr;
+    size_t es = 0, cnt = 0;
+    int ex = 0;
+    switch (what) {
+      case 0: src = P->s1.tasks.data(); es = sizeof(GTask); cnt = P->s1.tasks.size(); ex = P->s1.tr; break;
+      case 1: src = P->s1.segs.data(); es = sizeof(GSeg); cnt = P->s1.segs.size(); ex = P->s1.tile_class; break;
+      case 2: src = P->s2.tasks.data(); es = sizeof(GTask); cnt = P->s2.tasks.size(); ex = P->s2.tr; break;
+      case 3: src = P->s2.segs.data(); es = sizeof(GSeg); cnt = P->s2.segs.size(); ex = P->s2.tile_class; break;
+      case 4: src = P->mix.data(); es = sizeof(MixTask); cnt = P->mix.size(); ex = P->prec; break;
+      case 5: src = P->terms.data(); es = sizeof(MixTerm); cnt = P->terms.size(); break;
+      case 6: src = P->zero.data(); es = sizeof(Range); cnt = P->zero.size(); break;
+      default: throw Error(MPSA_EINVAL, "describe: unknown table");
+    }
+    *n = (int64_t)cnt;
+    if (extra) *extra = ex;
+    if (cap > 0) {
+      if (!buf || cap < (int64_t)(cnt * es)) throw Error(MPSA_EINVAL, "describe: buffer too small");
+      if (cnt) std::memcpy(buf, src, cnt * es);
+    }
   })
 }
 

@@ -22,8 +22,9 @@ from .blocks import Blocks
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libmpstools_amd.so")
 
-F64, F32, BF16 = 0, 1, 2
-PREC = {"f64": F64, "fp64": F64, "f32": F32, "fp32": F32, "bf16": BF16}
+F64, F32, BF16, BF16X3 = 0, 1, 2, 3
+PLAIN = 0x100
+PREC = {"f64": F64, "fp64": F64, "f32": F32, "fp32": F32, "bf16": BF16, "bf16x3": BF16X3, "f64plain": F64 | PLAIN}
 
 
 class MpsaError(RuntimeError):
@@ -68,7 +69,11 @@ def lib():
         L.mpsa_plan_destroy.argtypes = [P]
         L.mpsa_plan_stage_info.argtypes = [P, P, P, P, P]
         L.mpsa_plan_set_timing.argtypes = [P, C.c_int]
+        L.mpsa_plan_describe.argtypes = [P, C.c_int, P, C.c_int64, P, P]
         L.mpsa_plan_last_times.argtypes = [P, P]
+        L.mpsa_plan_vector_lengths.argtypes = [P, P, P, P]
+        L.mpsa_lanczos.argtypes = [P, P, P, C.c_int, C.c_double, C.c_int, C.c_int, P, C.c_int, P, P, P]
+        L.mpsa_tridiag_eig.argtypes = [C.c_int, P, P, P]
         L.mpsa_dot.argtypes = [P, P, C.c_int64, C.c_int, P, P]
         L.mpsa_axpby.argtypes = [P, P, P, P, C.c_int64, C.c_int, P]
         _lib = L
@@ -237,6 +242,10 @@ class HeffPlan:
         _check(lib().mpsa_plan_stage_info(self.h, _ptr(f1), _ptr(f2), _ptr(c1), _ptr(c2)))
         return dict(flops1=float(f1[0]), flops2=float(f2[0]), tile_class1=int(c1[0]), tile_class2=int(c2[0]))
 
+    def describe(self):
+        """The plan's device schedule as numpy record arrays (see mpsa_plan_describe)."""
+        return describe(self.h)
+
     def set_timing(self, on=True):
         _check(lib().mpsa_plan_set_timing(self.h, int(on)))
 
@@ -253,6 +262,34 @@ class HeffPlan:
     def apply(self, dx, dy, nvec=1, x_stride=0, y_stride=0):
         _check(lib().mpsa_heff_apply(self.h, _dptr(dx), int(x_stride), _dptr(dy), int(y_stride), int(nvec),
                                      _stream()))
+
+
+_REC = {
+    0: np.dtype([("c_off", "<i8"), ("m", "<i4"), ("n", "<i4"), ("seg0", "<i4"), ("nseg", "<i4"),
+                 ("tile0", "<i4"), ("tiles_m", "<i4")]),
+    1: np.dtype([("a_off", "<i8"), ("b_off", "<i8"), ("k", "<i4"), ("lda", "<i4"), ("ldb", "<i4"),
+                 ("pad", "<i4"), ("alpha", "<f8")]),
+    4: np.dtype([("dst_off", "<i8"), ("len", "<i8"), ("t0", "<i4"), ("nt", "<i4"), ("chunk0", "<i4"),
+                 ("pad", "<i4")]),
+    5: np.dtype([("src_off", "<i8"), ("coef", "<f8")]),
+    6: np.dtype([("off", "<i8"), ("len", "<i8")]),
+}
+_REC[2], _REC[3] = _REC[0], _REC[1]
+
+
+def describe(h):
+    out = {}
+    names = ["s1_tasks", "s1_segs", "s2_tasks", "s2_segs", "mix", "terms", "zero"]
+    for w in range(7):
+        n = np.zeros(1, np.int64)
+        ex = np.zeros(1, np.int32)
+        _check(lib().mpsa_plan_describe(h, w, None, 0, _ptr(n), _ptr(ex)))
+        arr = np.zeros(int(n[0]), dtype=_REC[w])
+        if n[0]:
+            _check(lib().mpsa_plan_describe(h, w, _ptr(arr), arr.nbytes, _ptr(n), _ptr(ex)))
+        out[names[w]] = arr
+        out[names[w] + "_extra"] = int(ex[0])
+    return out
 
 
 class EnvPlan:
@@ -279,8 +316,34 @@ class EnvPlan:
         return dict(flops_alg=float(fa[0]), flops_ref=float(fr[0]), n_gemm1=int(g1[0]), n_gemm2=int(g2[0]),
                     workspace_bytes=int(ws[0]))
 
+    def describe(self):
+        return describe(self.h)
+
     def apply(self, dBL, dA, dout, dAp=None):
         _check(lib().mpsa_env_apply(self.h, _dptr(dBL), _dptr(dA), _dptr(dAp), _dptr(dout), _stream()))
+
+
+def lanczos(plan, dpsi0, itmax, delta, testenergy=True, target=0, want_state=True):
+    """TSLanzcosEngine::simulate + getState on the device (src/lanzcos.hpp:165-278)."""
+    import torch
+    cap = itmax + 2
+    E = np.zeros(cap)
+    nit = np.zeros(1, np.int32)
+    nr = np.zeros(1, np.int32)
+    state = torch.zeros_like(dpsi0) if want_state else None
+    _check(lib().mpsa_lanczos(plan.h, _dptr(dpsi0), _dptr(state), int(itmax), float(delta), int(testenergy),
+                              int(target), _ptr(E), cap, _ptr(nit), _ptr(nr), _stream()))
+    return dict(it=int(nit[0]), energies=E[:int(nr[0])].copy(), state=state)
+
+
+def tridiag_eig(d, e, vectors=True):
+    """Host FP64 tridiagonal eigensolver of the library (replaces dstev)."""
+    n = len(d)
+    dd = np.ascontiguousarray(np.array(d, dtype=np.float64))
+    ee = np.ascontiguousarray(np.array(e, dtype=np.float64))
+    z = np.zeros(n * n) if vectors else None
+    _check(lib().mpsa_tridiag_eig(n, _ptr(dd), _ptr(ee) if n > 1 else None, _ptr(z) if vectors else None))
+    return dd, (z.reshape((n, n), order="F") if vectors else None)
 
 
 def dot(dx, dy, n=None, cplx=False):

@@ -94,11 +94,14 @@ def test_env_mixed_variant_matches_oracle(gpu, oracle):
     assert _rel(out, ref) <= 1e-12
 
 
-@pytest.mark.parametrize("prec,tol", [(native.F32, 2e-6), (native.BF16, 3e-2)])
+@pytest.mark.parametrize("prec,tol", [(native.F32, 2e-6), (native.BF16, 3e-2), (native.BF16X3, 1e-4),
+                                      (native.F64 | native.PLAIN, 1e-12)])
 @pytest.mark.parametrize("model,L,N,D,seed,dense", [CASES[2], CASES[3], CASES[4]])
 def test_heff2_reduced_precision_bounds(gpu, oracle, model, L, N, D, seed, dense, prec, tol):
     """FP32 inputs/accumulate: relative error <= 2e-6; BF16 inputs with FP32
-    accumulate: <= 3e-2 (bf16 has an 8-bit significand, unit roundoff 2^-9)."""
+    accumulate: <= 3e-2 (bf16 has an 8-bit significand, unit roundoff 2^-9);
+    BF16X3 (hi/lo split, three MFMAs): <= 1e-4; the FP64 plain-method schedule:
+    <= 1e-12."""
     _say("heff2 reduced", prec, model, L, N, D, seed)
     P = problems.build(oracle, model, L, N, D, seed, dense=dense)
     c, mpo, itk = P["c"], P["mpo"], P["itk"]
@@ -180,3 +183,54 @@ def test_full_size_dense_properties(gpu, oracle):
     assert lin.item() <= 1e-13
     sym = abs(native.dot(Hy, x) - native.dot(y, Hx)) / (Hx.norm() * y.norm()).item()
     assert sym <= 1e-13
+
+
+def _lanczos_setup(gpu, oracle, model, L, N, D, seed):
+    torch = gpu
+    P = problems.build(oracle, model, L, N, D, seed)
+    c, mpo, itk = P["c"], P["mpo"], P["itk"]
+    Lb, Rb, psi = P["Ls"][c - 1], P["Rs"][c + 2], P["psi"]
+    Lt, Rt, xt = native.Table.from_blocks(Lb), native.Table.from_blocks(Rb), native.Table.from_blocks(psi)
+    yt = native.heff2_outkeys(Lt, Rt, xt, mpo[c], mpo[c + 1], itk[c], itk[c + 1])
+    vt = xt.union(yt)
+    dL = torch.from_numpy(Lt.pack(Lb)).cuda()
+    dR = torch.from_numpy(Rt.pack(Rb)).cuda()
+    x0 = torch.from_numpy(vt.pack(psi)).cuda()
+    return P, (Lt, Rt, vt, dL, dR, x0)
+
+
+@pytest.mark.parametrize("model,L,N,D,seed", [("xxz", 12, 6, 24, 111), ("tv", 20, 10, 60, 112)])
+def test_lanczos_matches_oracle(gpu, oracle, model, L, N, D, seed):
+    """Device Lanczos (same recurrence and stopping rules as TSLanzcosEngine)
+    against the oracle's: 10 fixed steps give the same Ritz values; run to
+    convergence, the same ground-state energy and state (up to sign)."""
+    _say("lanczos", model, L, N, D, seed)
+    P, (Lt, Rt, vt, dL, dR, x0) = _lanczos_setup(gpu, oracle, model, L, N, D, seed)
+    c, mpo, itk = P["c"], P["mpo"], P["itk"]
+    args = (P["Ls"][c - 1], P["Rs"][c + 2], P["psi"], mpo[c], mpo[c + 1], itk[c], itk[c + 1])
+    plan = native.HeffPlan(Lt, Rt, vt, vt, (mpo[c], mpo[c + 1]), (itk[c], itk[c + 1]), native.F64)
+    plan.bind_env(dL, dR)
+    r10 = oracle.lanczos_twosite(*args, itmax=10, delta=1e-12, testenergy=True)
+    g10 = native.lanczos(plan, x0, 10, 1e-12, True)
+    assert g10["it"] == r10["it"] == 10
+    assert np.max(np.abs(g10["energies"] - r10["energies"])) <= 1e-10 * np.max(np.abs(r10["energies"]))
+    r = oracle.lanczos_twosite(*args, itmax=300, delta=1e-10, testenergy=True)
+    g = native.lanczos(plan, x0, 300, 1e-10, True)
+    assert abs(g["energies"][0] - r["energies"][0]) <= 1e-9
+    gs = vt.unpack(g["state"].cpu().numpy())
+    ov = sum(float(np.sum(gs[k] * r["gs"][k])) for k in r["gs"].keys())
+    assert abs(abs(ov) - 1.0) <= 1e-6
+
+
+def test_lanczos_residual_mode_and_itmax_cap(gpu, oracle):
+    _say("lanczos residual mode")
+    P, (Lt, Rt, vt, dL, dR, x0) = _lanczos_setup(gpu, oracle, "xxz", 12, 6, 24, 113)
+    c, mpo, itk = P["c"], P["mpo"], P["itk"]
+    plan = native.HeffPlan(Lt, Rt, vt, vt, (mpo[c], mpo[c + 1]), (itk[c], itk[c + 1]), native.F64)
+    plan.bind_env(dL, dR)
+    g = native.lanczos(plan, x0, 400, 1e-8, False)
+    r = oracle.lanczos_twosite(P["Ls"][c - 1], P["Rs"][c + 2], P["psi"], mpo[c], mpo[c + 1], itk[c], itk[c + 1],
+                               itmax=400, delta=1e-8, testenergy=False)
+    assert abs(g["energies"][0] - r["energies"][0]) <= 1e-9
+    g3 = native.lanczos(plan, x0, 3, 1e-12, True)
+    assert g3["it"] == 3 and len(g3["energies"]) == 3
