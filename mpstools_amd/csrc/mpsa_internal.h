@@ -75,6 +75,22 @@ hipError_t launch_dot(const double* x, const double* y, int64_t n, int cplx, dou
 hipError_t launch_axpby(double ar, double ai, const double* x, double br, double bi, double* y, int64_t n, int cplx,
                         hipStream_t st);
 int gemm_max_grid(int prec_in, int tile_class);
+
+// Workgroup tile of each (precision, tile class).  FP64/FP32: 0 = 128x128,
+// 1 = 64x64.  BF16/BF16X3: 0 = 256x128, 1 = 64x64, 2 = 128x128.
+inline void tile_dims(int prec, int cls, int* bm, int* bn) {
+  if (cls == 1) {
+    *bm = 64;
+    *bn = 64;
+  } else if ((prec == BF16 || prec == BF16X3) && cls == 0) {
+    *bm = 256;
+    *bn = 128;
+  } else {
+    *bm = 128;
+    *bn = 128;
+  }
+}
+inline int num_tile_classes(int prec) { return (prec == BF16 || prec == BF16X3) ? 3 : 2; }
 void set_last_error(const char* msg);
 
 }  // namespace mpsa

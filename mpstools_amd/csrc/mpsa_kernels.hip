@@ -431,6 +431,187 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_kernel(const GTask* __restri
 }
 
 // ---------------------------------------------------------------------------
+// Large-tile BF16 grouped GEMM: 256x128 workgroup tile, four waves of 128x64
+// (8x4 MFMA 16x16x32 tiles each), BK = 32, LDS double-buffered with one
+// barrier per k-step.  The wave tile halves the LDS bytes per MFMA relative to
+// 64x64 wave tiles.  Segment scales alpha != 1 are applied to the A tile on its
+// way into LDS (re-split into hi/lo for SPLIT).  Same operand layouts as
+// gemm_bf16_kernel.
+// ---------------------------------------------------------------------------
+template <typename TOut, bool SPLIT>
+__global__ __launch_bounds__(256, 1) void gemm_bf16_big_kernel(
+    const GTask* __restrict__ tasks, int ntasks, const GSeg* __restrict__ segs, int tiles_per_vec, int total_tiles,
+    const __hip_bfloat16* __restrict__ A, int64_t a_vs, int64_t a_plane, const __hip_bfloat16* __restrict__ B,
+    int64_t b_vs, int64_t b_plane, TOut* __restrict__ C, int64_t c_vs, int64_t c_plane, int c_rowmajor) {
+  constexpr int BM = 256, BN = 128, BK = 32, PK = BK + 8;
+  constexpr int NP = SPLIT ? 2 : 1;
+  constexpr int WM = 128, WN = 64, TM = WM / 16, TN = WN / 16;
+  constexpr int NA = BM * BK / 8 / 256, NB = BN * BK / 8 / 256;  // 4 and 2 vectors per thread and plane
+  __shared__ __attribute__((aligned(16))) unsigned short As[2][NP][BM][PK];
+  __shared__ __attribute__((aligned(16))) unsigned short Bs[2][NP][BN][PK];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int l16 = lane & 15, lq = lane >> 4;
+  const unsigned short* Au = reinterpret_cast<const unsigned short*>(A);
+  const unsigned short* Bu = reinterpret_cast<const unsigned short*>(B);
+
+  for (int g = xcd_logical_id(); g < total_tiles; g += gridDim.x) {
+    const int v = g / tiles_per_vec;
+    const int t = g - v * tiles_per_vec;
+    const GTask tk = tasks[find_task(tasks, ntasks, t, &GTask::tile0)];
+    const int local = t - tk.tile0;
+    const int tm = local % tk.tiles_m, tn = local / tk.tiles_m;
+    const int row0 = tm * BM, col0 = tn * BN;
+    const int m = tk.m, n = tk.n;
+    f4 acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) acc[i][j] = f4{0, 0, 0, 0};
+
+    int s = tk.seg0;
+    const int send = tk.seg0 + tk.nseg;
+    while (s < send && segs[s].k <= 0) ++s;
+    int k0 = 0;
+    u16x8 ra[NP][NA], rb[NP][NB];
+    float ralpha = 1.f;
+    auto load = [&](int si, int kk0) {
+      const GSeg sg = segs[si];
+      ralpha = (float)sg.alpha;
+      const unsigned short* Ab = Au + (int64_t)v * a_vs + sg.a_off;
+      const unsigned short* Bb = Bu + (int64_t)v * b_vs + sg.b_off;
+      const int kpad = (sg.k + 7) & ~7;
+#pragma unroll
+      for (int r = 0; r < NA; ++r) {
+        const int e = tid + r * 256;
+        const int i = e >> 2, kc = (e & 3) * 8;
+        const int gi = row0 + i, gk = kk0 + kc;
+        const bool ok = gi < m && gk < kpad;
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+          u16x8 val = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+          if (ok) val = *reinterpret_cast<const u16x8*>(Ab + p * a_plane + (int64_t)gi * sg.lda + gk);
+          ra[p][r] = val;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < NB; ++r) {
+        const int e = tid + r * 256;
+        const int j = e >> 2, kc = (e & 3) * 8;
+        const int gj = col0 + j, gk = kk0 + kc;
+        const bool ok = gj < n && gk < kpad;
+#pragma unroll
+        for (int p = 0; p < NP; ++p) {
+          u16x8 val = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+          if (ok) val = *reinterpret_cast<const u16x8*>(Bb + p * b_plane + (int64_t)gj * sg.ldb + gk);
+          rb[p][r] = val;
+        }
+      }
+    };
+    auto scale_a = [&]() {  // A <- alpha * A, re-rounded (re-split for SPLIT)
+#pragma unroll
+      for (int r = 0; r < NA; ++r)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          float x = __uint_as_float(((unsigned)ra[0][r][q]) << 16);
+          if (SPLIT) x += __uint_as_float(((unsigned)ra[NP - 1][r][q]) << 16);
+          x *= ralpha;
+          const __hip_bfloat16 h = __float2bfloat16(x);
+          ra[0][r][q] = __bfloat16_as_ushort(h);
+          if (SPLIT) ra[NP - 1][r][q] = __bfloat16_as_ushort(__float2bfloat16(x - __bfloat162float(h)));
+        }
+    };
+    auto store = [&](int buf) {
+      if (ralpha != 1.f) scale_a();
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+#pragma unroll
+        for (int r = 0; r < NA; ++r) {
+          const int e = tid + r * 256;
+          *reinterpret_cast<u16x8*>(&As[buf][p][e >> 2][(e & 3) * 8]) = ra[p][r];
+        }
+#pragma unroll
+        for (int r = 0; r < NB; ++r) {
+          const int e = tid + r * 256;
+          *reinterpret_cast<u16x8*>(&Bs[buf][p][e >> 2][(e & 3) * 8]) = rb[p][r];
+        }
+      }
+    };
+    auto advance = [&](int& si, int& kk) {
+      kk += BK;
+      if (kk >= segs[si].k) {
+        ++si;
+        while (si < send && segs[si].k <= 0) ++si;
+        kk = 0;
+      }
+    };
+    int buf = 0;
+    if (s < send) {
+      load(s, 0);
+      store(0);
+    }
+    __syncthreads();
+    while (s < send) {
+      int ns_ = s, nk = k0;
+      advance(ns_, nk);
+      const bool more = ns_ < send;
+      if (more) load(ns_, nk);
+      // MFMAs on the current buffer
+      bf8 bh[TN];
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+        bh[j] = *reinterpret_cast<const bf8*>(&Bs[buf][0][wn * WN + j * 16 + l16][lq * 8]);
+      if (SPLIT) {
+        bf8 bl[TN];
+#pragma unroll
+        for (int j = 0; j < TN; ++j)
+          bl[j] = *reinterpret_cast<const bf8*>(&Bs[buf][NP - 1][wn * WN + j * 16 + l16][lq * 8]);
+#pragma unroll
+        for (int i = 0; i < TM; ++i) {
+          const bf8 ah = *reinterpret_cast<const bf8*>(&As[buf][0][wm * WM + i * 16 + l16][lq * 8]);
+          const bf8 al = *reinterpret_cast<const bf8*>(&As[buf][NP - 1][wm * WM + i * 16 + l16][lq * 8]);
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], acc[i][j], 0, 0, 0);
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
+          }
+        }
+      } else {
+#pragma unroll
+        for (int i = 0; i < TM; ++i) {
+          const bf8 ah = *reinterpret_cast<const bf8*>(&As[buf][0][wm * WM + i * 16 + l16][lq * 8]);
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
+        }
+      }
+      if (more) store(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+      s = ns_;
+      k0 = nk;
+    }
+    TOut* Cb = C + (int64_t)v * c_vs + tk.c_off;
+    const int ldr = (n + 7) & ~7;
+#pragma unroll
+    for (int i = 0; i < TM; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j) {
+        const int col = col0 + wn * WN + j * 16 + l16;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = row0 + wm * WM + i * 16 + lq * 4 + r;
+          if (row < m && col < n) {
+            const int64_t idx = c_rowmajor ? ((int64_t)row * ldr + col) : (row + (int64_t)col * m);
+            store_out<TOut, SPLIT>(Cb, idx, c_plane, acc[i][j][r]);
+          }
+        }
+      }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // W mix: dst = sum_t coef_t * src_t over 1024-element chunks (4 per thread).
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -699,18 +880,26 @@ static hipError_t run16(int tr, const GTask* tasks, int ntasks, const GSeg* segs
   return hipGetLastError();
 }
 
+// BM == 256 selects the large-tile kernel (BN must be 128 then).
 template <typename TOut, int BM, int BN, bool SPLIT>
 static hipError_t runbf(const GTask* tasks, int ntasks, const GSeg* segs, int tpv, int nvec, const void* A,
                         int64_t a_vs, int64_t a_pl, const void* B, int64_t b_vs, int64_t b_pl, void* C, int64_t c_vs,
                         int64_t c_pl, int c_rowmajor, hipStream_t st) {
   const int total = tpv * nvec;
   if (total <= 0) return hipSuccess;
-  int grid = num_cu() * 2;
+  int grid = num_cu() * (BM == 256 ? 1 : 2);
   if (grid > total) grid = total;
   if (grid >= 8) grid &= ~7;
-  gemm_bf16_kernel<TOut, BM, BN, SPLIT><<<grid, 256, 0, st>>>(
-      tasks, ntasks, segs, tpv, total, (const __hip_bfloat16*)A, a_vs, a_pl, (const __hip_bfloat16*)B, b_vs, b_pl,
-      (TOut*)C, c_vs, c_pl, c_rowmajor);
+  if constexpr (BM == 256) {
+    static_assert(BN == 128, "large tile is 256x128");
+    gemm_bf16_big_kernel<TOut, SPLIT><<<grid, 256, 0, st>>>(
+        tasks, ntasks, segs, tpv, total, (const __hip_bfloat16*)A, a_vs, a_pl, (const __hip_bfloat16*)B, b_vs, b_pl,
+        (TOut*)C, c_vs, c_pl, c_rowmajor);
+  } else {
+    gemm_bf16_kernel<TOut, BM, BN, SPLIT><<<grid, 256, 0, st>>>(
+        tasks, ntasks, segs, tpv, total, (const __hip_bfloat16*)A, a_vs, a_pl, (const __hip_bfloat16*)B, b_vs, b_pl,
+        (TOut*)C, c_vs, c_pl, c_rowmajor);
+  }
   return hipGetLastError();
 }
 
@@ -731,20 +920,24 @@ hipError_t launch_gemm(int prec_in, int out_kind, int tr, int tile_class, const 
   }
   // BF16 / BF16X3: operands in the k-contiguous layout; out_kind 0 -> bf16
   // row-major (hi/lo planes for BF16X3), 1 -> double column-major.
+#define MPSA_BF_DISPATCH(TOUT, SPLIT_, APL, BPL, CPL, ROWM)                                                           \
+  do {                                                                                                               \
+    if (tile_class == 0)                                                                                             \
+      return runbf<TOUT, 256, 128, SPLIT_>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, APL, B, b_vs, BPL, C, \
+                                           c_vs, CPL, ROWM, st);                                                     \
+    if (tile_class == 2)                                                                                             \
+      return runbf<TOUT, 128, 128, SPLIT_>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, APL, B, b_vs, BPL, C, \
+                                           c_vs, CPL, ROWM, st);                                                     \
+    return runbf<TOUT, 64, 64, SPLIT_>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, APL, B, b_vs, BPL, C,     \
+                                       c_vs, CPL, ROWM, st);                                                         \
+  } while (0)
   if (prec_in == BF16) {
-    if (out_kind == 1) {
-      if (tile_class == 0) return runbf<double, 128, 128, false>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, 0, B, b_vs, 0, C, c_vs, 0, 0, st);
-      return runbf<double, 64, 64, false>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, 0, B, b_vs, 0, C, c_vs, 0, 0, st);
-    }
-    if (tile_class == 0) return runbf<__hip_bfloat16, 128, 128, false>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, 0, B, b_vs, 0, C, c_vs, 0, 1, st);
-    return runbf<__hip_bfloat16, 64, 64, false>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, 0, B, b_vs, 0, C, c_vs, 0, 1, st);
+    if (out_kind == 1) MPSA_BF_DISPATCH(double, false, 0, 0, 0, 0);
+    MPSA_BF_DISPATCH(__hip_bfloat16, false, 0, 0, 0, 1);
   }
-  if (out_kind == 1) {
-    if (tile_class == 0) return runbf<double, 128, 128, true>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, planes.a, B, b_vs, planes.b, C, c_vs, 0, 0, st);
-    return runbf<double, 64, 64, true>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, planes.a, B, b_vs, planes.b, C, c_vs, 0, 0, st);
-  }
-  if (tile_class == 0) return runbf<__hip_bfloat16, 128, 128, true>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, planes.a, B, b_vs, planes.b, C, c_vs, planes.c, 1, st);
-  return runbf<__hip_bfloat16, 64, 64, true>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, planes.a, B, b_vs, planes.b, C, c_vs, planes.c, 1, st);
+  if (out_kind == 1) MPSA_BF_DISPATCH(double, true, planes.a, planes.b, 0, 0);
+  MPSA_BF_DISPATCH(__hip_bfloat16, true, planes.a, planes.b, planes.c, 1);
+#undef MPSA_BF_DISPATCH
 }
 
 hipError_t launch_mix(int prec, const MixTask* tasks, int ntasks, const MixTerm* terms, int chunks_per_vec, int nvec,

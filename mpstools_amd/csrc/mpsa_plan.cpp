@@ -210,20 +210,31 @@ static T* upload(const std::vector<T>& v) {
 static int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
 static int64_t pad8(int64_t x) { return (x + 7) & ~int64_t(7); }
 
-// Tile class: 128x128 when it wastes little, 64x64 otherwise.
-static int pick_class(const std::vector<GTask>& t) {
-  double useful = 0, big = 0;
-  for (auto& x : t) {
-    useful += (double)x.m * x.n;
-    big += (double)((x.m + 127) / 128) * ((x.n + 127) / 128) * 128.0 * 128.0;
+// Tile class: the largest workgroup tile that wastes at most 20% of its
+// MFMA work on padding; the smallest tile otherwise.
+static int pick_class(const std::vector<GTask>& t, int prec) {
+  int best = 1;
+  int best_area = 0;
+  for (int cls = 0; cls < num_tile_classes(prec); ++cls) {
+    int bm, bn;
+    tile_dims(prec, cls, &bm, &bn);
+    double useful = 0, padded = 0;
+    for (auto& x : t) {
+      useful += (double)x.m * x.n;
+      padded += (double)((x.m + bm - 1) / bm) * ((x.n + bn - 1) / bn) * (double)bm * bn;
+    }
+    if (padded > 0 && useful / padded >= 0.8 && bm * bn > best_area) {
+      best = cls;
+      best_area = bm * bn;
+    }
   }
-  if (big <= 0) return 0;
-  return useful / big >= 0.7 ? 0 : 1;
+  return best;
 }
 
-static void finalize_stage(Stage& s) {
-  s.tile_class = pick_class(s.tasks);
-  const int BM = s.tile_class == 0 ? 128 : 64, BN = BM;
+static void finalize_stage(Stage& s, int prec) {
+  s.tile_class = pick_class(s.tasks, prec);
+  int BM, BN;
+  tile_dims(prec, s.tile_class, &BM, &BN);
   // longest tiles first (static round-robin over the persistent grid)
   std::vector<int64_t> cost(s.tasks.size());
   for (size_t i = 0; i < s.tasks.size(); ++i) {
@@ -685,8 +696,8 @@ static void build_plan(mpsa_plan* P, Built& b, const Tab& Ptab, const Tab& Qtab,
     if (!produced[(size_t)i] && Ytab.rows[(size_t)i] * Ytab.cols[(size_t)i] > 0)
       P->zero.push_back(Range{Ytab.off[(size_t)i], Ytab.rows[(size_t)i] * Ytab.cols[(size_t)i]});
 
-  finalize_stage(P->s1);
-  finalize_stage(P->s2);
+  finalize_stage(P->s1, P->prec);
+  finalize_stage(P->s2, P->prec);
   finalize_mix(P);
   P->rpL_chunks = finalize_repack(P->rpL);
   P->rpR_chunks = finalize_repack(P->rpR);
