@@ -129,6 +129,8 @@ def main():
     ap.add_argument("--lanczos-itmax", type=int, default=700)
     ap.add_argument("--refine-it", type=int, default=10)
     ap.add_argument("--no-lanczos", action="store_true")
+    ap.add_argument("--calib", action="store_true",
+                    help="run a known-size axpby (2 GiB read, 1 GiB written) for HBM counter calibration")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; multi-GPU sharding is described in "
@@ -145,6 +147,16 @@ def main():
     log(f"problem built in {time.perf_counter() - t0:.1f}s")
     x = prob["psi"]
     y = torch.zeros_like(x)
+    if args.calib:
+        from mpstools_amd import native
+        n = (1 << 30) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b = torch.ones(n, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        native.axpby(1.0, a, 1.0, b)
+        torch.cuda.synchronize()
+        log("calibration axpby: 2 GiB read, 1 GiB written")
+        del a, b
     res = {}
     for pr in precs:
         plan = prob["plan_" + pr]

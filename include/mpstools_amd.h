@@ -178,12 +178,63 @@ int mpsa_plan_vector_lengths(const mpsa_plan* plan, int64_t* x_len, int64_t* y_l
  * is that of the tridiagonal eigensolver (as with dstev, arbitrary). */
 int mpsa_lanczos(mpsa_plan* plan, const double* dpsi0, double* dstate, int itmax, double delta, int testenergy,
                  int target, double* energies, int cap, int* nit, int* nritz, mpsa_stream stream);
+/* nvec independent runs of mpsa_lanczos's recurrence (Ritz-value convergence
+ * test only) from the vectors dpsi0 + v*stride, v < nvec (the reference runs
+ * one TSLanzcosEngine per vector, src/lanzcos.hpp:165-278).  Every step is one
+ * batched matvec over all nvec vectors (mpsa_heff_apply with nvec) plus
+ * batched vector algebra whose coefficients stay on the device; with
+ * use_graph != 0 a block of 5 steps (the convergence-test period,
+ * src/lanzcos.hpp:107,210) is captured once into a hipGraph and replayed, and
+ * the plan's stage timing is switched off.  The host reads the 5 new
+ * (alpha, beta) pairs of every vector after each block and applies the
+ * reference's stopping rules to each run on its own; a run that has stopped
+ * idles (its later steps are computed and ignored) until all have stopped or
+ * itmax steps were made.  Per vector v: energies[v*cap ...] sorted Ritz
+ * values, nit[v], nritz[v]; dstates (may be NULL) receives Ritz state
+ * `target` of run v at dstates + v*stride.  stride >= the plan's vector
+ * length. */
+int mpsa_lanczos_batched(mpsa_plan* plan, const double* dpsi0, int64_t stride, int nvec, double* dstates, int itmax,
+                         double delta, int target, double* energies, int cap, int* nit, int* nritz, int use_graph,
+                         mpsa_stream stream);
 /* Host FP64 symmetric tridiagonal eigensolver used by mpsa_lanczos (replaces
  * dstev, blas/lapack_routines.cpp:435-446): d (n) diagonal -> ascending
  * eigenvalues, e (n-1) off-diagonal, z (n*n, column-major, may be NULL). */
 int mpsa_tridiag_eig(int n, double* d, const double* e, double* z);
 /* Free the device vector pool kept between Lanczos calls. */
 int mpsa_release_pools(void);
+
+/* ---- Energy and its adjoint over a whole chain ----
+ * Forward: MPO<T>::measure (src/MPOTypes.hpp:990-1017), E = <psi|H|psi>
+ * unnormalised, by the left environment sweep of addContractionLayer from
+ * getLMin (src/MPSfunctions.hpp:1917-1926) and the trace of the
+ * charge-diagonal blocks of the last environment.  Every `checkpoint`-th left
+ * environment is kept on the device.
+ * Backward (no counterpart in the reference): dE/dA[i] = 2 contractSingle(
+ * L[i-1], A[i], W[i], R[i+1]) on A[i]'s blocks (valid for a real symmetric H),
+ * and dE/dtheta_p = sum_i <A[i]| contractSingle(L[i-1], A[i], dW_p[i], R[i+1])>
+ * for an MPO affine in its parameters, W[i](theta) = W0[i] + sum_p theta_p
+ * dW_p[i].  The right environments are built from getRMax
+ * (src/MPSfunctions.hpp:1996-2022) during the backward sweep and the left
+ * ones recomputed segment by segment from the checkpoints.
+ * Chain description: nsites site tables (ns = 1, FP64), their MPO tensors,
+ * i_to_key arrays concatenated (d[i] charges for site i), and nparam*nsites
+ * derivative tensors dW[p*nsites + i] (entries with value dW_p[i]; n = 0 where
+ * W[i] does not depend on theta_p).  Site buffers dA[i] / gradients dgrad[i]
+ * are device FP64 buffers in the site tables' layouts.  energy_backward uses
+ * the checkpoints of the last energy_forward, which must have been run on the
+ * same tensors. */
+typedef struct mpsa_chain mpsa_chain;
+int mpsa_chain_create(mpsa_chain** chain, int nsites, const mpsa_blocks* sites, const mpsa_mpo* W,
+                      const uint64_t* itokey, const int* d, int nparam, const mpsa_mpo* dW, int checkpoint);
+int mpsa_energy_forward(mpsa_chain* chain, const double* const* dA, double* E, mpsa_stream stream);
+int mpsa_energy_backward(mpsa_chain* chain, const double* const* dA, double* const* dgrad, double* dE_dtheta,
+                         mpsa_stream stream);
+/* Key table of a chain environment: side > 0 gives L[index] (index -1 is
+ * getLMin), side <= 0 gives R[index] (index nsites is getRMax).  cap = 0
+ * queries the block count. */
+int mpsa_chain_env_table(const mpsa_chain* chain, int side, int index, int64_t cap, int64_t* n, uint64_t* q,
+                         uint32_t* p, int64_t* rows, int64_t* cols, int64_t* offsets);
+int mpsa_chain_destroy(mpsa_chain* chain);
 
 /* ---- Lanczos vector algebra over packed block vectors (BlockTensor.hpp:443-544) ----
  * X*Y = sum conj(Y) X (src/BlockTensor.hpp:520-544), computed in FP64 with a

@@ -2,7 +2,7 @@
 spin-1/2 Heisenberg chain, J = 1, L sites, MPS bond dimension D.
 
 Inputs are a seeded Gaussian MPS (torch's CUDA generator; random input
-generation is plumbing), left-normalised by QR on sites < c and
+generation is plumbing), left-normalised (CholeskyQR2) on sites < c and
 right-normalised on sites > c+1.  The environments L[c-1] and R[c+2] are then
 built layer by layer with this package's environment kernels
 (addContractionLayer on the GPU), exactly as DMRGengine builds them
@@ -32,6 +32,17 @@ def _lmin_table():
     return native.Table.from_keys(1, np.zeros((1, 2), np.uint64), np.zeros((1, 1), np.uint32), [1], [1])
 
 
+def _isometry(torch, G):
+    """Q with orthonormal columns spanning G (m x n, m >= n): CholeskyQR2
+    (Q = G R^-1 with R^T R = G^T G, applied twice).  A few large GEMM/Cholesky
+    calls instead of a Householder sweep's many small kernels."""
+    Q = G
+    for _ in range(2):
+        R = torch.linalg.cholesky(Q.T @ Q, upper=True)
+        Q = torch.linalg.solve_triangular(R, Q, upper=True, left=False)
+    return Q
+
+
 def dense_heisenberg_centre(D=2048, L=128, seed=20260003, build_envs="sweep", precisions=("f64",), verbose=False):
     import torch
     dev = "cuda"
@@ -54,12 +65,11 @@ def dense_heisenberg_centre(D=2048, L=128, seed=20260003, build_envs="sweep", pr
         # random site tensor -> isometry
         if direction > 0:
             G = torch.randn(2 * dl, dr, dtype=torch.float64, device=dev, generator=g)
-            Q, _ = torch.linalg.qr(G)
+            Q = _isometry(torch, G)
             blocks = [Q[i * dl:(i + 1) * dl, :] for i in range(2)]
         else:
             G = torch.randn(dl, 2 * dr, dtype=torch.float64, device=dev, generator=g)
-            Q, _ = torch.linalg.qr(G.T)
-            Qh = Q.T
+            Qh = _isometry(torch, G.T.contiguous()).T
             blocks = [Qh[:, i * dr:(i + 1) * dr] for i in range(2)]
         dA = torch.zeros(max(At.total, 1), dtype=torch.float64, device=dev)
         for i in range(2):

@@ -69,12 +69,25 @@ hipError_t launch_convert(int prec, const double* src, void* dst, int64_t n, hip
 hipError_t launch_repack(int prec, const Repack* tasks, int ntasks, int chunks_per_vec, int nvec, const double* src,
                          int64_t s_vs, void* dst, int64_t d_vs, hipStream_t st, int64_t plane = 0);
 hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t st);
+// p[i * stride] = v for i < n
+hipError_t launch_fill_strided(double* p, int64_t n, int64_t stride, double v, hipStream_t st);
 hipError_t launch_zero_ranges(const Range* r, int nr, double* base, int64_t vs, int nvec, hipStream_t st);
 hipError_t launch_dot(const double* x, const double* y, int64_t n, int cplx, double* partial, int nparts,
                       double* out2, hipStream_t st);
 hipError_t launch_axpby(double ar, double ai, const double* x, double br, double bi, double* y, int64_t n, int cplx,
                         hipStream_t st);
 int gemm_max_grid(int prec_in, int tile_class);
+// batched Lanczos vector algebra (device-resident coefficients and step counter)
+hipError_t launch_bdot(const double* x, const double* y, int64_t n, int64_t vs, int nvec, double* partial, int nparts,
+                       double* out, int ld, const int* step, int root, hipStream_t st);
+hipError_t launch_lz_normalize(const double* W, double* X, double* const* basis, const double* kbuf, int ld,
+                               const int* step, int64_t n, int64_t vs, int nvec, hipStream_t st);
+hipError_t launch_lz_update(const double* Y, const double* X, double* const* basis, const double* kbuf,
+                            const double* ebuf, int ld, const int* step, double* W, int64_t n, int64_t vs, int nvec,
+                            hipStream_t st);
+hipError_t launch_lz_step(int* step, hipStream_t st);
+hipError_t launch_lz_combine(double* const* basis, const double* c, int ldc, const int* m, double* out, int64_t n,
+                             int64_t vs, int nvec, hipStream_t st);
 
 // Workgroup tile of each (precision, tile class).  FP64/FP32: 0 = 128x128,
 // 1 = 64x64.  BF16/BF16X3: 0 = 256x128, 1 = 64x64, 2 = 128x128.

@@ -70,6 +70,36 @@ __device__ __forceinline__ int find_task(const Task* tasks, int ntasks, int t, i
   return lo;
 }
 
+// Position of a k-step iterator over the segments of one task, with the
+// segment's descriptor held in registers: memory is read only when the
+// iterator enters a new segment, never per k-step.
+struct SegCursor {
+  int si, kk, k, lda, ldb;
+  int64_t a_off, b_off;
+  double alpha;
+};
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+// Plans never emit segments with k <= 0, so entering a segment is one uniform
+// (scalar) descriptor read.
+__device__ __forceinline__ void cur_enter(SegCursor& c, const GSeg* __restrict__ segs, int si, int send) {
+  si = uni(si);
+  c.si = si;
+  c.kk = 0;
+  if (si < send) {
+    const GSeg g = *(segs + si);
+    c.k = uni(g.k);
+    c.lda = uni(g.lda);
+    c.ldb = uni(g.ldb);
+    c.a_off = g.a_off;
+    c.b_off = g.b_off;
+    c.alpha = g.alpha;
+  }
+}
+__device__ __forceinline__ void cur_next(SegCursor& c, const GSeg* __restrict__ segs, int send, int bk) {
+  c.kk += bk;
+  if (c.kk >= c.k) cur_enter(c, segs, uni(c.si) + 1, send);
+}
+
 // Persistent-grid mapping that keeps consecutive logical work items on one
 // XCD (workgroups are dispatched round-robin over the 8 XCDs).
 __device__ __forceinline__ int xcd_logical_id() {
@@ -129,16 +159,19 @@ __global__ __launch_bounds__(256, 2) void gemm16_kernel(const GTask* __restrict_
 #pragma unroll
       for (int j = 0; j < TN; ++j) acc[i][j] = V4{0, 0, 0, 0};
 
-    // k-step iterator over (segment, k0)
-    int s = tk.seg0, send = tk.seg0 + tk.nseg;
-    while (s < send && segs[s].k <= 0) ++s;
-    int k0 = 0;
+    const int send = tk.seg0 + tk.nseg;
+    SegCursor cl;  // position of the step held in registers
+    cur_enter(cl, segs, tk.seg0, send);
     T ra[NA], rb[NB];
-    auto load = [&](int si, int kk0) {
-      const GSeg sg = segs[si];
-      const T* Ab = A + (int64_t)v * a_vs + sg.a_off;
-      const T* Bb = B + (int64_t)v * b_vs + sg.b_off;
-      const T alpha = (T)sg.alpha;
+    unsigned oka = 0, okb = 0;  // in-bounds flags, applied at the LDS store so loads stay in flight
+    T ralpha = (T)1;
+    auto load = [&](const SegCursor& c) {
+      const T* Ab = A + (int64_t)v * a_vs + c.a_off;
+      const T* Bb = B + (int64_t)v * b_vs + c.b_off;
+      ralpha = (T)c.alpha;
+      const int kk0 = c.kk;
+      oka = 0;
+      okb = 0;
 #pragma unroll
       for (int r = 0; r < NA; ++r) {
         const int e = tid + r * 256;
@@ -146,9 +179,10 @@ __global__ __launch_bounds__(256, 2) void gemm16_kernel(const GTask* __restrict_
         if (!TA) { i = e % BM; kk = e / BM; }
         else { kk = e % BK; i = e / BK; }
         const int gi = row0 + i, gk = kk0 + kk;
-        T val = (T)0;
-        if (gi < m && gk < sg.k) val = TA ? Ab[gk + (int64_t)gi * sg.lda] : Ab[gi + (int64_t)gk * sg.lda];
-        ra[r] = val * alpha;
+        const bool ok = gi < m && gk < c.k;
+        const int gic = gi < m ? gi : m - 1, gkc = gk < c.k ? gk : c.k - 1;  // always-valid address
+        ra[r] = TA ? Ab[gkc + (int64_t)gic * c.lda] : Ab[gic + (int64_t)gkc * c.lda];
+        oka |= (ok ? 1u : 0u) << r;
       }
 #pragma unroll
       for (int r = 0; r < NB; ++r) {
@@ -157,9 +191,10 @@ __global__ __launch_bounds__(256, 2) void gemm16_kernel(const GTask* __restrict_
         if (!TB) { kk = e % BK; j = e / BK; }
         else { j = e % BN; kk = e / BN; }
         const int gj = col0 + j, gk = kk0 + kk;
-        T val = (T)0;
-        if (gj < n && gk < sg.k) val = TB ? Bb[gj + (int64_t)gk * sg.ldb] : Bb[gk + (int64_t)gj * sg.ldb];
-        rb[r] = val;
+        const bool ok = gj < n && gk < c.k;
+        const int gjc = gj < n ? gj : n - 1, gkc = gk < c.k ? gk : c.k - 1;
+        rb[r] = TB ? Bb[gjc + (int64_t)gkc * c.ldb] : Bb[gkc + (int64_t)gjc * c.ldb];
+        okb |= (ok ? 1u : 0u) << r;
       }
     };
     auto store = [&]() {
@@ -169,7 +204,7 @@ __global__ __launch_bounds__(256, 2) void gemm16_kernel(const GTask* __restrict_
         int i, kk;
         if (!TA) { i = e % BM; kk = e / BM; }
         else { kk = e % BK; i = e / BK; }
-        As[kk][i] = ra[r];
+        As[kk][i] = ((oka >> r) & 1u) ? ra[r] * ralpha : (T)0;
       }
 #pragma unroll
       for (int r = 0; r < NB; ++r) {
@@ -177,22 +212,16 @@ __global__ __launch_bounds__(256, 2) void gemm16_kernel(const GTask* __restrict_
         int j, kk;
         if (!TB) { kk = e % BK; j = e / BK; }
         else { j = e % BN; kk = e / BN; }
-        Bs[kk][j] = rb[r];
+        Bs[kk][j] = ((okb >> r) & 1u) ? rb[r] : (T)0;
       }
     };
-    if (s < send) load(s, 0);
-    while (s < send) {
+    if (cl.si < send) load(cl);
+    while (cl.si < send) {
       __syncthreads();  // previous MFMAs done reading LDS
       store();
       __syncthreads();
-      // advance and prefetch
-      int ns_ = s, nk = k0 + BK;
-      if (nk >= segs[ns_].k) {
-        ++ns_;
-        while (ns_ < send && segs[ns_].k <= 0) ++ns_;
-        nk = 0;
-      }
-      if (ns_ < send) load(ns_, nk);
+      cur_next(cl, segs, send, BK);
+      if (cl.si < send) load(cl);
 #pragma unroll
       for (int kq = 0; kq < BK / 4; ++kq) {
         T af[TM], bf[TN];
@@ -205,8 +234,6 @@ __global__ __launch_bounds__(256, 2) void gemm16_kernel(const GTask* __restrict_
 #pragma unroll
           for (int j = 0; j < TN; ++j) acc[i][j] = mfma16(af[i], bf[j], acc[i][j]);
       }
-      s = ns_;
-      k0 = nk;
     }
     // epilogue: see acc_row for the row each accumulator register holds
     TOut* Cb = C + (int64_t)v * c_vs + tk.c_off;
@@ -296,28 +323,32 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_kernel(const GTask* __restri
 #pragma unroll
       for (int j = 0; j < TN; ++j) acc[i][j] = f4{0, 0, 0, 0};
 
-    int s = tk.seg0, send = tk.seg0 + tk.nseg;
-    while (s < send && segs[s].k <= 0) ++s;
-    int k0 = 0;
+    const int send = tk.seg0 + tk.nseg;
+    SegCursor cl;  // step held in registers
+    cur_enter(cl, segs, tk.seg0, send);
     u16x8 ra[NP][NA], rb[NP][NB];
+    unsigned oka = 0, okb = 0;  // in-bounds flags, applied at the LDS store so loads stay in flight
     float alpha = 1.f;
-    auto load = [&](int si, int kk0) {
-      const GSeg sg = segs[si];
-      const unsigned short* Ab = Au + (int64_t)v * a_vs + sg.a_off;
-      const unsigned short* Bb = Bu + (int64_t)v * b_vs + sg.b_off;
-      const int kpad = (sg.k + 7) & ~7;
+    auto load = [&](const SegCursor& c) {
+      oka = 0;
+      okb = 0;
+      const unsigned short* Ab = Au + (int64_t)v * a_vs + c.a_off;
+      const unsigned short* Bb = Bu + (int64_t)v * b_vs + c.b_off;
+      const int kpad = (c.k + 7) & ~7;
+      const int kk0 = c.kk;
 #pragma unroll
       for (int r = 0; r < NA; ++r) {
         const int e = tid + r * 256;
         const int i = e / (BK / 8), kc = (e % (BK / 8)) * 8;
         const int gi = row0 + i, gk = kk0 + kc;
         const bool ok = gi < m && gk < kpad;
+        const int gic = gi < m ? gi : m - 1, gkc = gk < kpad ? gk : kpad - 8;  // always-valid address
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
-          u16x8 val = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
-          if (ok) val = *reinterpret_cast<const u16x8*>(Ab + p * a_plane + (int64_t)gi * sg.lda + gk);
+          u16x8 val = *reinterpret_cast<const u16x8*>(Ab + p * a_plane + (int64_t)gic * c.lda + gkc);
           ra[p][r] = val;
         }
+        oka |= (ok ? 1u : 0u) << r;
       }
 #pragma unroll
       for (int r = 0; r < NB; ++r) {
@@ -325,12 +356,13 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_kernel(const GTask* __restri
         const int j = e / (BK / 8), kc = (e % (BK / 8)) * 8;
         const int gj = col0 + j, gk = kk0 + kc;
         const bool ok = gj < n && gk < kpad;
+        const int gjc = gj < n ? gj : n - 1, gkc = gk < kpad ? gk : kpad - 8;
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
-          u16x8 val = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
-          if (ok) val = *reinterpret_cast<const u16x8*>(Bb + p * b_plane + (int64_t)gj * sg.ldb + gk);
+          u16x8 val = *reinterpret_cast<const u16x8*>(Bb + p * b_plane + (int64_t)gjc * c.ldb + gkc);
           rb[p][r] = val;
         }
+        okb |= (ok ? 1u : 0u) << r;
       }
     };
     auto store = [&]() {
@@ -339,29 +371,25 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_kernel(const GTask* __restri
 #pragma unroll
         for (int r = 0; r < NA; ++r) {
           const int e = tid + r * 256;
-          *reinterpret_cast<u16x8*>(&As[p][e / (BK / 8)][(e % (BK / 8)) * 8]) = ra[p][r];
+          *reinterpret_cast<u16x8*>(&As[p][e / (BK / 8)][(e % (BK / 8)) * 8]) =
+              ((oka >> r) & 1u) ? ra[p][r] : u16x8{0, 0, 0, 0, 0, 0, 0, 0};
         }
 #pragma unroll
         for (int r = 0; r < NB; ++r) {
           const int e = tid + r * 256;
-          *reinterpret_cast<u16x8*>(&Bs[p][e / (BK / 8)][(e % (BK / 8)) * 8]) = rb[p][r];
+          *reinterpret_cast<u16x8*>(&Bs[p][e / (BK / 8)][(e % (BK / 8)) * 8]) =
+              ((okb >> r) & 1u) ? rb[p][r] : u16x8{0, 0, 0, 0, 0, 0, 0, 0};
         }
       }
     };
-    if (s < send) load(s, 0);
-    int cur = s;
-    while (s < send) {
+    if (cl.si < send) load(cl);
+    while (cl.si < send) {
       __syncthreads();
       store();
       __syncthreads();
-      alpha = (float)segs[cur].alpha;
-      int ns_ = s, nk = k0 + BK;
-      if (nk >= segs[ns_].k) {
-        ++ns_;
-        while (ns_ < send && segs[ns_].k <= 0) ++ns_;
-        nk = 0;
-      }
-      if (ns_ < send) load(ns_, nk);
+      alpha = (float)cl.alpha;
+      cur_next(cl, segs, send, BK);
+      if (cl.si < send) load(cl);
       bf8 ah[TM], bh[TN];
 #pragma unroll
       for (int i = 0; i < TM; ++i) ah[i] = *reinterpret_cast<const bf8*>(&As[0][wm * WM + i * 16 + l16][lq * 8]);
@@ -381,14 +409,16 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_kernel(const GTask* __restri
 #pragma unroll
         for (int j = 0; j < TN; ++j)
           bl[j] = *reinterpret_cast<const bf8*>(&Bs[NP - 1][wn * WN + j * 16 + l16][lq * 8]);
-        // small cross terms first, then hi*hi
 #pragma unroll
         for (int i = 0; i < TM; ++i)
 #pragma unroll
-          for (int j = 0; j < TN; ++j) {
+          for (int j = 0; j < TN; ++j)
             part[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], part[i][j], 0, 0, 0);
+#pragma unroll
+        for (int i = 0; i < TM; ++i)
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
             part[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], part[i][j], 0, 0, 0);
-          }
       }
 #pragma unroll
       for (int i = 0; i < TM; ++i)
@@ -406,12 +436,9 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_kernel(const GTask* __restri
 #pragma unroll
           for (int j = 0; j < TN; ++j) acc[i][j] += alpha * part[i][j];
       }
-      cur = ns_;
-      s = ns_;
-      k0 = nk;
     }
     TOut* Cb = C + (int64_t)v * c_vs + tk.c_off;
-    const int ldr = (n + 7) & ~7;
+    const int ldr = (n + 31) & ~31;
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -431,29 +458,73 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_kernel(const GTask* __restri
 }
 
 // ---------------------------------------------------------------------------
-// Large-tile BF16 grouped GEMM: 256x128 workgroup tile, four waves of 128x64
-// (8x4 MFMA 16x16x32 tiles each), BK = 32, LDS double-buffered with one
-// barrier per k-step.  The wave tile halves the LDS bytes per MFMA relative to
-// 64x64 wave tiles.  Segment scales alpha != 1 are applied to the A tile on its
-// way into LDS (re-split into hi/lo for SPLIT).  Same operand layouts as
-// gemm_bf16_kernel.
+// Streaming BF16 grouped GEMM: operands go straight from global memory into
+// LDS with global_load_lds_dwordx4 (no register staging), NSTAGE k-steps in
+// flight, one barrier per k-step.  256x128 workgroup tile, four waves of
+// 128x64.  Requirements (met by the plan's bf16 layouts): k dimensions padded
+// with zeros to a multiple of 32, so every k-step is a full 32-wide slab.  Rows
+// past m (columns past n) are loaded from a clamped, valid address and their
+// garbage results are never stored.  LDS rows are 64 bytes with the 16-byte
+// chunk index XOR-ed by (row >> 2) & 3, which keeps every quarter-wave's
+// fragment reads on distinct banks.  Segment scales are applied by rescaling
+// the accumulators when the scale changes (exact for powers of two).
 // ---------------------------------------------------------------------------
-template <typename TOut, bool SPLIT>
-__global__ __launch_bounds__(256, 1) void gemm_bf16_big_kernel(
+typedef __attribute__((address_space(3))) void* lds_vptr;
+typedef __attribute__((address_space(1))) void* gbl_vptr;
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+// LDS fragment read issued as inline asm: the compiler cannot see that it reads
+// LDS, so it does not wait for every outstanding global->LDS load before it
+// (it would otherwise have to, lacking alias information).  Completion is
+// awaited explicitly with lds_wait() before the MFMAs use the registers.
+__device__ __forceinline__ bf8 lds_frag(const unsigned char* p) {
+  u32x4 r;
+  const unsigned addr = (unsigned)(uintptr_t)(const __attribute__((address_space(3))) unsigned char*)p;
+  asm volatile("ds_read_b128 %0, %1" : "=v"(r) : "v"(addr));
+  return __builtin_bit_cast(bf8, r);
+}
+__device__ __forceinline__ void lds_wait() { __builtin_amdgcn_s_waitcnt((15 & 15) | (7 << 4) | (0 << 8) | (3 << 14)); }
+
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  // gfx9 s_waitcnt: vmcnt[3:0] | expcnt[6:4] | lgkmcnt[11:8] | vmcnt_hi[15:14]
+  __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | (((N >> 4) & 3) << 14));
+}
+
+template <typename TOut, bool SPLIT, int NSTAGE>
+__global__ __launch_bounds__(256, 1) void gemm_bf16_stream_kernel(
     const GTask* __restrict__ tasks, int ntasks, const GSeg* __restrict__ segs, int tiles_per_vec, int total_tiles,
     const __hip_bfloat16* __restrict__ A, int64_t a_vs, int64_t a_plane, const __hip_bfloat16* __restrict__ B,
     int64_t b_vs, int64_t b_plane, TOut* __restrict__ C, int64_t c_vs, int64_t c_plane, int c_rowmajor) {
-  constexpr int BM = 256, BN = 128, BK = 32, PK = BK + 8;
+  constexpr int BM = 256, BN = 128, BK = 32;
   constexpr int NP = SPLIT ? 2 : 1;
   constexpr int WM = 128, WN = 64, TM = WM / 16, TN = WN / 16;
-  constexpr int NA = BM * BK / 8 / 256, NB = BN * BK / 8 / 256;  // 4 and 2 vectors per thread and plane
-  __shared__ __attribute__((aligned(16))) unsigned short As[2][NP][BM][PK];
-  __shared__ __attribute__((aligned(16))) unsigned short Bs[2][NP][BN][PK];
+  constexpr int ROWB = BK * 2;                        // bytes per LDS row (one plane)
+  constexpr int A_BYTES = BM * ROWB, B_BYTES = BN * ROWB;
+  constexpr int STAGE_BYTES = NP * (A_BYTES + B_BYTES);
+  constexpr int LA = BM / 64, LB = BN / 64;           // load instructions per wave and plane
+  constexpr int LOADS = NP * (LA + LB);               // per wave per stage
+  __shared__ __attribute__((aligned(1024))) unsigned char smem[NSTAGE * STAGE_BYTES];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
   const int l16 = lane & 15, lq = lane >> 4;
   const unsigned short* Au = reinterpret_cast<const unsigned short*>(A);
   const unsigned short* Bu = reinterpret_cast<const unsigned short*>(B);
+  // fragment byte offsets inside a plane (row-dependent swizzle)
+  int aoff[TM], boff[TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i) {
+    const int r = wm * WM + i * 16 + l16;
+    aoff[i] = r * ROWB + ((lq ^ ((r >> 2) & 3)) << 4);
+  }
+#pragma unroll
+  for (int j = 0; j < TN; ++j) {
+    const int r = wn * WN + j * 16 + l16;
+    boff[j] = r * ROWB + ((lq ^ ((r >> 2) & 3)) << 4);
+  }
+  // loader lane mapping: instruction q of this wave covers rows 16*(wave*L + q) + lane/4
+  const int lrow = lane >> 2, lslot = lane & 3;
 
   for (int g = xcd_logical_id(); g < total_tiles; g += gridDim.x) {
     const int v = g / tiles_per_vec;
@@ -463,137 +534,122 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16_big_kernel(
     const int tm = local % tk.tiles_m, tn = local / tk.tiles_m;
     const int row0 = tm * BM, col0 = tn * BN;
     const int m = tk.m, n = tk.n;
+    const int send = tk.seg0 + tk.nseg;
     f4 acc[TM][TN];
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
       for (int j = 0; j < TN; ++j) acc[i][j] = f4{0, 0, 0, 0};
 
-    int s = tk.seg0;
-    const int send = tk.seg0 + tk.nseg;
-    while (s < send && segs[s].k <= 0) ++s;
-    int k0 = 0;
-    u16x8 ra[NP][NA], rb[NP][NB];
-    float ralpha = 1.f;
-    auto load = [&](int si, int kk0) {
-      const GSeg sg = segs[si];
-      ralpha = (float)sg.alpha;
-      const unsigned short* Ab = Au + (int64_t)v * a_vs + sg.a_off;
-      const unsigned short* Bb = Bu + (int64_t)v * b_vs + sg.b_off;
-      const int kpad = (sg.k + 7) & ~7;
-#pragma unroll
-      for (int r = 0; r < NA; ++r) {
-        const int e = tid + r * 256;
-        const int i = e >> 2, kc = (e & 3) * 8;
-        const int gi = row0 + i, gk = kk0 + kc;
-        const bool ok = gi < m && gk < kpad;
-#pragma unroll
-        for (int p = 0; p < NP; ++p) {
-          u16x8 val = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
-          if (ok) val = *reinterpret_cast<const u16x8*>(Ab + p * a_plane + (int64_t)gi * sg.lda + gk);
-          ra[p][r] = val;
-        }
-      }
-#pragma unroll
-      for (int r = 0; r < NB; ++r) {
-        const int e = tid + r * 256;
-        const int j = e >> 2, kc = (e & 3) * 8;
-        const int gj = col0 + j, gk = kk0 + kc;
-        const bool ok = gj < n && gk < kpad;
-#pragma unroll
-        for (int p = 0; p < NP; ++p) {
-          u16x8 val = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
-          if (ok) val = *reinterpret_cast<const u16x8*>(Bb + p * b_plane + (int64_t)gj * sg.ldb + gk);
-          rb[p][r] = val;
-        }
-      }
-    };
-    auto scale_a = [&]() {  // A <- alpha * A, re-rounded (re-split for SPLIT)
-#pragma unroll
-      for (int r = 0; r < NA; ++r)
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          float x = __uint_as_float(((unsigned)ra[0][r][q]) << 16);
-          if (SPLIT) x += __uint_as_float(((unsigned)ra[NP - 1][r][q]) << 16);
-          x *= ralpha;
-          const __hip_bfloat16 h = __float2bfloat16(x);
-          ra[0][r][q] = __bfloat16_as_ushort(h);
-          if (SPLIT) ra[NP - 1][r][q] = __bfloat16_as_ushort(__float2bfloat16(x - __bfloat162float(h)));
-        }
-    };
-    auto store = [&](int buf) {
-      if (ralpha != 1.f) scale_a();
+    auto issue = [&](const SegCursor& c, int stage) {
+      unsigned char* base = smem + stage * STAGE_BYTES;
+      const unsigned short* Ab = Au + (int64_t)v * a_vs + c.a_off;
+      const unsigned short* Bb = Bu + (int64_t)v * b_vs + c.b_off;
 #pragma unroll
       for (int p = 0; p < NP; ++p) {
 #pragma unroll
-        for (int r = 0; r < NA; ++r) {
-          const int e = tid + r * 256;
-          *reinterpret_cast<u16x8*>(&As[buf][p][e >> 2][(e & 3) * 8]) = ra[p][r];
+        for (int q = 0; q < LA; ++q) {
+          const int rb = 16 * (wave * LA + q);
+          const int r = rb + lrow;
+          const int gi = row0 + r < m ? row0 + r : m - 1;
+          const int chunk = lslot ^ ((r >> 2) & 3);
+          const unsigned short* src = Ab + p * a_plane + (int64_t)gi * c.lda + c.kk + chunk * 8;
+          __builtin_amdgcn_global_load_lds((gbl_vptr)src, (lds_vptr)(base + p * A_BYTES + rb * ROWB), 16, 0, 0);
         }
 #pragma unroll
-        for (int r = 0; r < NB; ++r) {
-          const int e = tid + r * 256;
-          *reinterpret_cast<u16x8*>(&Bs[buf][p][e >> 2][(e & 3) * 8]) = rb[p][r];
+        for (int q = 0; q < LB; ++q) {
+          const int rb = 16 * (wave * LB + q);
+          const int r = rb + lrow;
+          const int gj = col0 + r < n ? col0 + r : n - 1;
+          const int chunk = lslot ^ ((r >> 2) & 3);
+          const unsigned short* src = Bb + p * b_plane + (int64_t)gj * c.ldb + c.kk + chunk * 8;
+          __builtin_amdgcn_global_load_lds((gbl_vptr)src,
+                                           (lds_vptr)(base + NP * A_BYTES + p * B_BYTES + rb * ROWB), 16, 0, 0);
         }
       }
     };
-    auto advance = [&](int& si, int& kk) {
-      kk += BK;
-      if (kk >= segs[si].k) {
-        ++si;
-        while (si < send && segs[si].k <= 0) ++si;
-        kk = 0;
+
+    // cursors of the NSTAGE-1 steps loaded ahead of the computed one
+    SegCursor cc;               // step being computed
+    cur_enter(cc, segs, tk.seg0, send);
+    SegCursor cl = cc;          // next step to issue
+    int issued = 0;             // steps issued ahead of cc (including cc)
+#pragma unroll
+    for (int st = 0; st < NSTAGE - 1; ++st) {
+      if (cl.si < send) {
+        issue(cl, st);
+        cur_next(cl, segs, send, BK);
+        ++issued;
       }
-    };
-    int buf = 0;
-    if (s < send) {
-      load(s, 0);
-      store(0);
     }
-    __syncthreads();
-    while (s < send) {
-      int ns_ = s, nk = k0;
-      advance(ns_, nk);
-      const bool more = ns_ < send;
-      if (more) load(ns_, nk);
-      // MFMAs on the current buffer
-      bf8 bh[TN];
+    float ascale = (cc.si < send) ? (float)cc.alpha : 1.f;
+    int stage = 0;
+    while (cc.si < send) {
+      // wait for this step's slab: allow the loads of the later issued steps to stay in flight
+      const int ahead = issued - 1;
+      if (ahead >= 2) wait_vmcnt<(2 * LOADS < 63 ? 2 * LOADS : 63)>();
+      else if (ahead == 1) wait_vmcnt<LOADS>();
+      else wait_vmcnt<0>();
+      // Every wave's slab is in LDS (each wave waited for its own loads above) and
+      // every wave is done with the previous step (its fragment reads completed
+      // before its MFMAs).  A bare barrier: __syncthreads' release fence would
+      // also wait for the loads still in flight for later steps.
+      asm volatile("s_barrier" ::: "memory");
+      // refill the buffer freed by the previous step
+      if (cl.si < send) {
+        issue(cl, (stage + NSTAGE - 1) % NSTAGE);
+        cur_next(cl, segs, send, BK);
+        ++issued;
+      }
+      const float a = (float)cc.alpha;
+      if (a != ascale && a != 0.f) {
+        const float f = ascale / a;
 #pragma unroll
-      for (int j = 0; j < TN; ++j)
-        bh[j] = *reinterpret_cast<const bf8*>(&Bs[buf][0][wn * WN + j * 16 + l16][lq * 8]);
-      if (SPLIT) {
-        bf8 bl[TN];
+        for (int i = 0; i < TM; ++i)
 #pragma unroll
-        for (int j = 0; j < TN; ++j)
-          bl[j] = *reinterpret_cast<const bf8*>(&Bs[buf][NP - 1][wn * WN + j * 16 + l16][lq * 8]);
+          for (int j = 0; j < TN; ++j) acc[i][j] *= f;
+        ascale = a;
+      }
+      if (a != 0.f) {
+        const unsigned char* base = smem + stage * STAGE_BYTES;
+        bf8 bh[TN], ah[TM];
 #pragma unroll
-        for (int i = 0; i < TM; ++i) {
-          const bf8 ah = *reinterpret_cast<const bf8*>(&As[buf][0][wm * WM + i * 16 + l16][lq * 8]);
-          const bf8 al = *reinterpret_cast<const bf8*>(&As[buf][NP - 1][wm * WM + i * 16 + l16][lq * 8]);
+        for (int j = 0; j < TN; ++j) bh[j] = lds_frag(base + NP * A_BYTES + boff[j]);
 #pragma unroll
-          for (int j = 0; j < TN; ++j) {
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], acc[i][j], 0, 0, 0);
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
-          }
+        for (int i = 0; i < TM; ++i) ah[i] = lds_frag(base + aoff[i]);
+        if (SPLIT) {
+          bf8 bl[TN], al[TM];
+#pragma unroll
+          for (int j = 0; j < TN; ++j) bl[j] = lds_frag(base + NP * A_BYTES + (NP - 1) * B_BYTES + boff[j]);
+#pragma unroll
+          for (int i = 0; i < TM; ++i) al[i] = lds_frag(base + (NP - 1) * A_BYTES + aoff[i]);
+          lds_wait();
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+        } else {
+          lds_wait();
         }
-      } else {
 #pragma unroll
-        for (int i = 0; i < TM; ++i) {
-          const bf8 ah = *reinterpret_cast<const bf8*>(&As[buf][0][wm * WM + i * 16 + l16][lq * 8]);
+        for (int i = 0; i < TM; ++i)
 #pragma unroll
           for (int j = 0; j < TN; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[i][j], 0, 0, 0);
-        }
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
       }
-      if (more) store(buf ^ 1);
-      __syncthreads();
-      buf ^= 1;
-      s = ns_;
-      k0 = nk;
+      cur_next(cc, segs, send, BK);
+      --issued;
+      stage = (stage + 1) % NSTAGE;
     }
+    __syncthreads();  // all LDS reads of this tile are done before the next tile's first loads
     TOut* Cb = C + (int64_t)v * c_vs + tk.c_off;
-    const int ldr = (n + 7) & ~7;
+    const int ldr = (n + 31) & ~31;
 #pragma unroll
     for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -604,7 +660,7 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16_big_kernel(
           const int row = row0 + wm * WM + i * 16 + lq * 4 + r;
           if (row < m && col < n) {
             const int64_t idx = c_rowmajor ? ((int64_t)row * ldr + col) : (row + (int64_t)col * m);
-            store_out<TOut, SPLIT>(Cb, idx, c_plane, acc[i][j][r]);
+            store_out<TOut, SPLIT>(Cb, idx, c_plane, acc[i][j][r] * ascale);
           }
         }
       }
@@ -718,6 +774,11 @@ __global__ void fill_kernel(double* __restrict__ p, int64_t n, double v) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
+__global__ void fill_strided_kernel(double* __restrict__ p, int64_t n, int64_t stride, double v) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i * stride] = v;
+}
+
 __global__ void zero_ranges_kernel(const Range* __restrict__ r, int nr, double* __restrict__ base, int64_t vs) {
   const Range rg = r[blockIdx.y];
   double* b = base + (int64_t)blockIdx.z * vs + rg.off;
@@ -781,6 +842,90 @@ __global__ void axpby_kernel(double ar, double ai, const double* __restrict__ x,
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// Batched Lanczos vector algebra.  nvec independent recurrences, vector v at
+// base + v * vs.  The step number j and the table of Krylov basis vectors are
+// read from device memory, so the launches of a block of steps are identical
+// from block to block and can be captured once into a graph and replayed.
+//   kbuf[v * ld + j] = ||W_v||            (beta of step j)
+//   X_v = W_v * (1 / kbuf),  basis[j]_v = X_v
+//   ebuf[v * ld + j] = <Y_v, X_v>         (alpha of step j, Y = H X)
+//   W_v = (-e X_v + Y_v) + (-k basis[j-1]_v)   (the second term from j = 1 on)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void bdot_partial_kernel(const double* __restrict__ x, const double* __restrict__ y,
+                                                           int64_t n, int64_t vs, double* __restrict__ partial) {
+  __shared__ double sr[256];
+  const int v = blockIdx.y;
+  const double* xv = x + (int64_t)v * vs;
+  const double* yv = y + (int64_t)v * vs;
+  double a = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) a += yv[i] * xv[i];
+  sr[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sr[threadIdx.x] += sr[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[(int64_t)v * gridDim.x + blockIdx.x] = sr[0];
+}
+
+__global__ void bdot_final_kernel(const double* __restrict__ partial, int nparts, double* __restrict__ out, int ld,
+                                  const int* __restrict__ step, int root) {
+  if (threadIdx.x != 0) return;
+  const int v = blockIdx.x;
+  double r = 0.0;
+  for (int p = 0; p < nparts; ++p) r += partial[(int64_t)v * nparts + p];
+  out[(int64_t)v * ld + *step] = root ? sqrt(r) : r;
+}
+
+__global__ void lz_normalize_kernel(const double* __restrict__ W, double* __restrict__ X, double* const* __restrict__ basis,
+                                    const double* __restrict__ kbuf, int ld, const int* __restrict__ step, int64_t n,
+                                    int64_t vs) {
+  const int v = blockIdx.y;
+  const int j = *step;
+  const double r = 1.0 / kbuf[(int64_t)v * ld + j];
+  const int64_t o = (int64_t)v * vs;
+  double* B = basis[j] + o;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double t = r * W[o + i];
+    X[o + i] = t;
+    B[i] = t;
+  }
+}
+
+__global__ void lz_update_kernel(const double* __restrict__ Y, const double* __restrict__ X,
+                                 double* const* __restrict__ basis, const double* __restrict__ kbuf,
+                                 const double* __restrict__ ebuf, int ld, const int* __restrict__ step,
+                                 double* __restrict__ W, int64_t n, int64_t vs) {
+  const int v = blockIdx.y;
+  const int j = *step;
+  const double e = ebuf[(int64_t)v * ld + j], k = kbuf[(int64_t)v * ld + j];
+  const int64_t o = (int64_t)v * vs;
+  const double* A0 = j > 0 ? basis[j - 1] + o : nullptr;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double t = -e * X[o + i] + Y[o + i];
+    if (A0) t = t + -k * A0[i];
+    W[o + i] = t;
+  }
+}
+
+__global__ void lz_step_kernel(int* step) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
+}
+
+// state_v = sum_{i < m_v} c[v * ldc + i] * basis[i]_v
+__global__ void lz_combine_kernel(double* const* __restrict__ basis, const double* __restrict__ c, int ldc,
+                                  const int* __restrict__ m, double* __restrict__ out, int64_t n, int64_t vs) {
+  const int v = blockIdx.y;
+  const int mv = m[v];
+  const int64_t o = (int64_t)v * vs;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    double t = 0.0;
+    for (int b = 0; b < mv; ++b) t += c[(int64_t)v * ldc + b] * basis[b][o + i];
+    out[o + i] = t;
+  }
+}
 
 // ---------------------------------------------------------------------------
 // Repack FP64 blocks into the compute layout (type conversion, optional
@@ -880,7 +1025,7 @@ static hipError_t run16(int tr, const GTask* tasks, int ntasks, const GSeg* segs
   return hipGetLastError();
 }
 
-// BM == 256 selects the large-tile kernel (BN must be 128 then).
+// BM == 256 selects the streaming large-tile kernel (BN must be 128 then).
 template <typename TOut, int BM, int BN, bool SPLIT>
 static hipError_t runbf(const GTask* tasks, int ntasks, const GSeg* segs, int tpv, int nvec, const void* A,
                         int64_t a_vs, int64_t a_pl, const void* B, int64_t b_vs, int64_t b_pl, void* C, int64_t c_vs,
@@ -892,7 +1037,7 @@ static hipError_t runbf(const GTask* tasks, int ntasks, const GSeg* segs, int tp
   if (grid >= 8) grid &= ~7;
   if constexpr (BM == 256) {
     static_assert(BN == 128, "large tile is 256x128");
-    gemm_bf16_big_kernel<TOut, SPLIT><<<grid, 256, 0, st>>>(
+    gemm_bf16_stream_kernel<TOut, SPLIT, SPLIT ? 3 : 4><<<grid, 256, 0, st>>>(
         tasks, ntasks, segs, tpv, total, (const __hip_bfloat16*)A, a_vs, a_pl, (const __hip_bfloat16*)B, b_vs, b_pl,
         (TOut*)C, c_vs, c_pl, c_rowmajor);
   } else {
@@ -996,6 +1141,14 @@ hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t st) {
   return hipGetLastError();
 }
 
+hipError_t launch_fill_strided(double* p, int64_t n, int64_t stride, double v, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  int64_t blocks = (n + 255) / 256;
+  int grid = (int)(blocks < 1024 ? blocks : 1024);
+  fill_strided_kernel<<<grid, 256, 0, st>>>(p, n, stride, v);
+  return hipGetLastError();
+}
+
 hipError_t launch_zero_ranges(const Range* r, int nr, double* base, int64_t vs, int nvec, hipStream_t st) {
   if (nr <= 0 || nvec <= 0) return hipSuccess;
   dim3 grid(64, nr, nvec);
@@ -1016,6 +1169,42 @@ hipError_t launch_axpby(double ar, double ai, const double* x, double br, double
   int64_t blocks = (n + 255) / 256;
   int grid = (int)(blocks < 16384 ? blocks : 16384);
   axpby_kernel<<<grid, 256, 0, st>>>(ar, ai, x, br, bi, y, n, cplx);
+  return hipGetLastError();
+}
+
+static int vec_grid(int64_t n) {
+  const int64_t blocks = (n + 255) / 256;
+  return (int)(blocks < 1024 ? (blocks > 0 ? blocks : 1) : 1024);
+}
+
+hipError_t launch_bdot(const double* x, const double* y, int64_t n, int64_t vs, int nvec, double* partial, int nparts,
+                       double* out, int ld, const int* step, int root, hipStream_t st) {
+  bdot_partial_kernel<<<dim3(nparts, nvec), 256, 0, st>>>(x, y, n, vs, partial);
+  bdot_final_kernel<<<nvec, 64, 0, st>>>(partial, nparts, out, ld, step, root);
+  return hipGetLastError();
+}
+
+hipError_t launch_lz_normalize(const double* W, double* X, double* const* basis, const double* kbuf, int ld,
+                               const int* step, int64_t n, int64_t vs, int nvec, hipStream_t st) {
+  lz_normalize_kernel<<<dim3(vec_grid(n), nvec), 256, 0, st>>>(W, X, basis, kbuf, ld, step, n, vs);
+  return hipGetLastError();
+}
+
+hipError_t launch_lz_update(const double* Y, const double* X, double* const* basis, const double* kbuf,
+                            const double* ebuf, int ld, const int* step, double* W, int64_t n, int64_t vs, int nvec,
+                            hipStream_t st) {
+  lz_update_kernel<<<dim3(vec_grid(n), nvec), 256, 0, st>>>(Y, X, basis, kbuf, ebuf, ld, step, W, n, vs);
+  return hipGetLastError();
+}
+
+hipError_t launch_lz_step(int* step, hipStream_t st) {
+  lz_step_kernel<<<1, 64, 0, st>>>(step);
+  return hipGetLastError();
+}
+
+hipError_t launch_lz_combine(double* const* basis, const double* c, int ldc, const int* m, double* out, int64_t n,
+                             int64_t vs, int nvec, hipStream_t st) {
+  lz_combine_kernel<<<dim3(vec_grid(n), nvec), 256, 0, st>>>(basis, c, ldc, m, out, n, vs);
   return hipGetLastError();
 }
 

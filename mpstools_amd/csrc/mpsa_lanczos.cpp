@@ -259,6 +259,185 @@ int mpsa_lanczos(mpsa_plan* plan, const double* dpsi0, double* dstate, int itmax
   }
 }
 
+int mpsa_lanczos_batched(mpsa_plan* plan, const double* dpsi0, int64_t stride, int nvec, double* dstates, int itmax,
+                         double delta, int target, double* energies, int cap, int* nit, int* nritz, int use_graph,
+                         mpsa_stream stream) {
+  hipStream_t cs = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<void*> small;  // device scalars and tables, freed on exit
+  auto cleanup = [&]() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (cs) (void)hipStreamSynchronize(cs);
+    for (void* p : small) (void)hipFree(p);
+    if (cs) (void)hipStreamDestroy(cs);
+  };
+  try {
+    if (!plan || !dpsi0 || nvec < 1 || itmax < 1 || !nit || !nritz || (cap > 0 && !energies))
+      throw LErr(MPSA_EINVAL, "lanczos_batched: bad arguments");
+    int64_t xl = 0, yl = 0;
+    int same = 0;
+    chk(mpsa_plan_vector_lengths(plan, &xl, &yl, &same));
+    if (!same) throw LErr(MPSA_EINVAL, "lanczos_batched: the plan's x and y tables must be the same vector space");
+    const int64_t n = xl, vs = stride;
+    if (vs < n) throw LErr(MPSA_EINVAL, "lanczos_batched: stride smaller than the vector length");
+    // Capture needs a stream other than the legacy default stream: all work
+    // runs on a private stream after the caller's stream has drained.
+    hchk(hipStreamSynchronize((hipStream_t)stream), "lanczos_batched: caller stream");
+    hchk(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "lanczos_batched: stream");
+    if (use_graph) chk(mpsa_plan_set_timing(plan, 0));
+    const int ld = itmax + 1;
+    const int nparts = (int)std::max<int64_t>(1, std::min<int64_t>(256, (n + 8191) / 8192));
+    auto dalloc = [&](size_t bytes) {
+      void* p = nullptr;
+      if (hipMalloc(&p, std::max<size_t>(bytes, 16)) != hipSuccess) throw LErr(MPSA_ENOMEM, "lanczos_batched alloc");
+      small.push_back(p);
+      return p;
+    };
+    int* dstep = (int*)dalloc(sizeof(int));
+    double* kbuf = (double*)dalloc(sizeof(double) * (size_t)nvec * ld);
+    double* ebuf = (double*)dalloc(sizeof(double) * (size_t)nvec * ld);
+    double* partial = (double*)dalloc(sizeof(double) * (size_t)nvec * nparts);
+    double** table = (double**)dalloc(sizeof(double*) * (size_t)ld);
+    const int zero = 0;
+    hchk(hipMemcpy(dstep, &zero, sizeof(int), hipMemcpyHostToDevice), "step counter");
+    const int64_t tot = vs * nvec;
+    Vecs V{{}, tot, cs};
+    double* W = V.get();
+    double* X = V.get();
+    double* Y = V.get();
+    hchk(hipMemcpyAsync(W, dpsi0, sizeof(double) * (size_t)tot, hipMemcpyDeviceToDevice, cs), "copy psi0");
+    std::vector<double*> basis;
+    basis.reserve((size_t)ld);
+    auto ensure_basis = [&](int upto) {
+      const size_t old = basis.size();
+      while ((int)basis.size() < upto) basis.push_back(V.get());
+      if (basis.size() > old)
+        hchk(hipMemcpy(table + old, basis.data() + old, sizeof(double*) * (basis.size() - old), hipMemcpyHostToDevice),
+             "basis table");
+    };
+    auto step_once = [&]() {
+      hchk(launch_bdot(W, W, n, vs, nvec, partial, nparts, kbuf, ld, dstep, 1, cs), "norm");
+      hchk(launch_lz_normalize(W, X, table, kbuf, ld, dstep, n, vs, nvec, cs), "normalise");
+      chk(mpsa_heff_apply(plan, X, vs, Y, vs, nvec, (mpsa_stream)cs));
+      hchk(launch_bdot(Y, X, n, vs, nvec, partial, nparts, ebuf, ld, dstep, 0, cs), "alpha");
+      hchk(launch_lz_update(Y, X, table, kbuf, ebuf, ld, dstep, W, n, vs, nvec, cs), "recurrence");
+      hchk(launch_lz_step(dstep, cs), "step");
+    };
+    const int Ndiag = 5;
+    // per-run state of the reference's control flow (src/lanzcos.hpp:184-257)
+    struct Run {
+      std::vector<double> es, ks;
+      double eold = 1e100;
+      int it = 0;
+      bool first = true, done = false;
+    };
+    std::vector<Run> runs((size_t)nvec);
+    std::vector<double> hk((size_t)nvec * ld), he((size_t)nvec * ld);
+    int launched = 0, consumed = 0;
+    while (true) {
+      // the first block ends at it = 5 (6 steps), later blocks are 5 steps
+      int cnt = launched == 0 ? 1 : Ndiag;
+      cnt = std::min(cnt, itmax - launched);
+      if (cnt <= 0) break;
+      ensure_basis(launched + cnt);
+      if (use_graph && cnt == Ndiag) {
+        if (!gexec) {
+          hchk(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "begin capture");
+          try {
+            for (int i = 0; i < Ndiag; ++i) step_once();
+          } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(cs, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+          }
+          hchk(hipStreamEndCapture(cs, &graph), "end capture");
+          hchk(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0), "instantiate");
+        }
+        hchk(hipGraphLaunch(gexec, cs), "graph launch");
+      } else {
+        for (int i = 0; i < cnt; ++i) step_once();
+      }
+      launched += cnt;
+      if (launched < 6 && launched < itmax) continue;  // complete the first test block before reading back
+      hchk(hipMemcpyAsync(hk.data(), kbuf, sizeof(double) * hk.size(), hipMemcpyDeviceToHost, cs), "read beta");
+      hchk(hipMemcpyAsync(he.data(), ebuf, sizeof(double) * he.size(), hipMemcpyDeviceToHost, cs), "read alpha");
+      hchk(hipStreamSynchronize(cs), "lanczos_batched sync");
+      bool all_done = true;
+      for (int v = 0; v < nvec; ++v) {
+        Run& r = runs[(size_t)v];
+        for (int j = consumed; j < launched && !r.done; ++j) {
+          const double k = hk[(size_t)v * ld + j];
+          if (!(std::fabs(k) >= delta)) {  // |k| < delta (a NaN stops the run too)
+            r.done = true;
+            break;
+          }
+          if (!r.first) r.ks.push_back(k);
+          const double e = he[(size_t)v * ld + j];
+          r.es.push_back(e);
+          if (r.it % Ndiag == 0 && r.it != 0) {
+            std::vector<double> d(r.es), sub(r.ks);
+            if (tridiag_ql((int)d.size(), d.data(), sub.data(), nullptr)) throw LErr(MPSA_EINVAL, "tridiag failed");
+            if (std::fabs(d[0] - r.eold) < delta) {
+              r.done = true;
+              break;
+            }
+            r.eold = d[0];
+          }
+          r.first = false;
+          ++r.it;
+          if (r.it >= itmax) r.done = true;
+        }
+        all_done = all_done && r.done;
+      }
+      consumed = launched;
+      if (all_done) break;
+    }
+    // Ritz values and states (tridiagonal eigensolve per run, src/lanzcos.hpp:258-268)
+    int mmax = 0;
+    for (auto& r : runs) mmax = std::max(mmax, (int)r.es.size());
+    std::vector<double> coef((size_t)nvec * std::max(mmax, 1), 0.0);
+    std::vector<int> ms((size_t)nvec, 0);
+    for (int v = 0; v < nvec; ++v) {
+      Run& r = runs[(size_t)v];
+      nit[v] = r.it;
+      const int m = (int)r.es.size();
+      nritz[v] = m;
+      ms[(size_t)v] = m;
+      if (m == 0) continue;
+      std::vector<double> d(r.es), sub(r.ks), z((size_t)m * m);
+      if (tridiag_ql(m, d.data(), sub.data(), z.data())) throw LErr(MPSA_EINVAL, "tridiag failed");
+      for (int i = 0; i < m && i < cap; ++i) energies[(size_t)v * cap + i] = d[(size_t)i];
+      if (dstates) {
+        if (target < 0 || target >= m)
+          throw LErr(MPSA_EINVAL, "LanzcosEngine::getState(const int j): j<_eigs.size2() failed");
+        for (int i = 0; i < m; ++i) coef[(size_t)v * std::max(mmax, 1) + i] = z[(size_t)target * m + i];
+      }
+    }
+    if (dstates && mmax > 0) {
+      double* dcoef = (double*)dalloc(sizeof(double) * coef.size());
+      int* dm = (int*)dalloc(sizeof(int) * ms.size());
+      hchk(hipMemcpy(dcoef, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice), "coefficients");
+      hchk(hipMemcpy(dm, ms.data(), sizeof(int) * ms.size(), hipMemcpyHostToDevice), "lengths");
+      hchk(launch_fill(dstates, tot, 0.0, cs), "zero states");
+      hchk(launch_lz_combine(table, dcoef, std::max(mmax, 1), dm, dstates, n, vs, nvec, cs), "states");
+    }
+    hchk(hipStreamSynchronize(cs), "lanczos_batched sync");
+    cleanup();
+    return MPSA_OK;
+  } catch (const LErr& e) {
+    cleanup();
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    cleanup();
+    set_last_error(e.what());
+    return MPSA_EINVAL;
+  }
+}
+
 int mpsa_release_pools(void) {
   g_pool.release();
   return MPSA_OK;

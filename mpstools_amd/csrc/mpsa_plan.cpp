@@ -208,7 +208,9 @@ static T* upload(const std::vector<T>& v) {
 }
 
 static int64_t align_up(int64_t x, int64_t a) { return (x + a - 1) / a * a; }
-static int64_t pad8(int64_t x) { return (x + 7) & ~int64_t(7); }
+// bf16 layouts pad every k dimension (and the row-major T leading dimension)
+// with zeros to a multiple of 32: the streaming kernel moves whole 32-wide slabs.
+static int64_t pad8(int64_t x) { return (x + 31) & ~int64_t(31); }
 
 // Tile class: the largest workgroup tile that wastes at most 20% of its
 // MFMA work on padding; the smallest tile otherwise.
@@ -232,6 +234,19 @@ static int pick_class(const std::vector<GTask>& t, int prec) {
 }
 
 static void finalize_stage(Stage& s, int prec) {
+  // drop empty-k segments (kernels assume k > 0); a task left without
+  // segments still writes its C block (as zeros)
+  {
+    std::vector<GSeg> segs;
+    for (auto& t : s.tasks) {
+      const int first = (int)segs.size();
+      for (int j = 0; j < t.nseg; ++j)
+        if (s.segs[(size_t)(t.seg0 + j)].k > 0) segs.push_back(s.segs[(size_t)(t.seg0 + j)]);
+      t.seg0 = first;
+      t.nseg = (int)segs.size() - first;
+    }
+    s.segs = segs;
+  }
   s.tile_class = pick_class(s.tasks, prec);
   int BM, BN;
   tile_dims(prec, s.tile_class, &BM, &BN);

@@ -142,6 +142,36 @@ def xxz_mpo(N, Jz=1.0, Jxy=1.0, hz=0.0):
     return heisenberg_mpo([Jz] * (N - 1), [Jxy] * (N - 1), [hz] * N)
 
 
+def mpo_difference(A, B, thresh=0.0):
+    """Entrywise A - B of two MPO chains with the same aux layout: the
+    derivative tensors dW/dtheta of an MPO affine in theta, as
+    mpo(theta = e_p) - mpo(theta = 0).  Entries that cancel are dropped."""
+    out = []
+    for Wa, Wb in zip(A, B):
+        M = MPOMat(Wa.Dl, Wa.Dr, Wa.d)
+        keys = set(Wa.m) | set(Wb.m)
+        for k in sorted(keys):
+            ea = Wa.m[k].e if k in Wa.m else {}
+            eb = Wb.m[k].e if k in Wb.m else {}
+            e = {}
+            for oi in set(ea) | set(eb):
+                v = ea.get(oi, 0.0) - eb.get(oi, 0.0)
+                if abs(v) > thresh:
+                    e[oi] = v
+            if e:
+                M[k] = LocalOp(Wa.d, e)
+        out.append(M)
+    return out
+
+
+def xxz_param_derivatives(N):
+    """dW[i]/dtheta for theta = (Jz, Jxy, hz) of xxz_mpo (uniform couplings)."""
+    base = xxz_mpo(N, 0.0, 0.0, 0.0)
+    return {"Jz": mpo_difference(xxz_mpo(N, 1.0, 0.0, 0.0), base),
+            "Jxy": mpo_difference(xxz_mpo(N, 0.0, 1.0, 0.0), base),
+            "hz": mpo_difference(xxz_mpo(N, 0.0, 0.0, 1.0), base)}
+
+
 def spinless_fermions_mpo(V, t, mu):
     """SpinlessFermionsMPO (src/Hamiltonians.hpp:521-569), Jordan-Wigner sign p."""
     N = len(mu)

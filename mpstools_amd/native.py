@@ -73,6 +73,13 @@ def lib():
         L.mpsa_plan_last_times.argtypes = [P, P]
         L.mpsa_plan_vector_lengths.argtypes = [P, P, P, P]
         L.mpsa_lanczos.argtypes = [P, P, P, C.c_int, C.c_double, C.c_int, C.c_int, P, C.c_int, P, P, P]
+        L.mpsa_lanczos_batched.argtypes = [P, P, C.c_int64, C.c_int, P, C.c_int, C.c_double, C.c_int, P, C.c_int,
+                                            P, P, C.c_int, P]
+        L.mpsa_chain_create.argtypes = [P, C.c_int, P, P, P, P, C.c_int, P, C.c_int]
+        L.mpsa_energy_forward.argtypes = [P, P, P, P]
+        L.mpsa_energy_backward.argtypes = [P, P, P, P, P]
+        L.mpsa_chain_env_table.argtypes = [P, C.c_int, C.c_int, C.c_int64, P, P, P, P, P, P]
+        L.mpsa_chain_destroy.argtypes = [P]
         L.mpsa_tridiag_eig.argtypes = [C.c_int, P, P, P]
         L.mpsa_dot.argtypes = [P, P, C.c_int64, C.c_int, P, P]
         L.mpsa_axpby.argtypes = [P, P, P, P, C.c_int64, C.c_int, P]
@@ -334,6 +341,78 @@ def lanczos(plan, dpsi0, itmax, delta, testenergy=True, target=0, want_state=Tru
     _check(lib().mpsa_lanczos(plan.h, _dptr(dpsi0), _dptr(state), int(itmax), float(delta), int(testenergy),
                               int(target), _ptr(E), cap, _ptr(nit), _ptr(nr), _stream()))
     return dict(it=int(nit[0]), energies=E[:int(nr[0])].copy(), state=state)
+
+
+def lanczos_batched(plan, dpsi0, nvec, stride, itmax, delta, target=0, want_state=True, graph=True):
+    """nvec independent Lanczos runs (energy test), one batched matvec per step;
+    with graph=True each block of 5 steps is replayed from a hipGraph."""
+    import torch
+    cap = itmax + 2
+    E = np.zeros(nvec * cap)
+    nit = np.zeros(nvec, np.int32)
+    nr = np.zeros(nvec, np.int32)
+    state = torch.zeros(nvec * stride, dtype=torch.float64, device=dpsi0.device) if want_state else None
+    _check(lib().mpsa_lanczos_batched(plan.h, _dptr(dpsi0), int(stride), int(nvec), _dptr(state), int(itmax),
+                                      float(delta), int(target), _ptr(E), cap, _ptr(nit), _ptr(nr), int(bool(graph)),
+                                      _stream()))
+    E = E.reshape(nvec, cap)
+    return dict(it=nit.astype(int), energies=[E[v, :int(nr[v])].copy() for v in range(nvec)], state=state)
+
+
+class Chain:
+    """A whole MPS chain under an MPO: energy (MPO::measure) and its adjoint
+    (dE/dA[i] for every site, dE/dtheta for the given derivative MPOs)."""
+
+    def __init__(self, site_tables, mpo, itk, dmpo=(), checkpoint=8):
+        self.N = len(site_tables)
+        self.tables = list(site_tables)
+        self._mpo = [MPOArg(W) for W in mpo]
+        self._dmpo = [MPOArg(W) for dW in dmpo for W in dW]
+        self.nparam = len(dmpo)
+        sites = (_Blocks * self.N)(*[t.s for t in self.tables])
+        mpos = (_MPO * self.N)(*[m.s for m in self._mpo])
+        dm = (_MPO * max(1, len(self._dmpo)))(*[m.s for m in self._dmpo])
+        self._keep = (sites, mpos, dm)
+        self._itk = np.ascontiguousarray(np.concatenate([_itk(x) for x in itk]), dtype=np.uint64)
+        self._d = np.array([len(x) for x in itk], dtype=np.int32)
+        h = C.c_void_p()
+        _check(lib().mpsa_chain_create(C.byref(h), self.N, C.cast(sites, C.c_void_p), C.cast(mpos, C.c_void_p),
+                                       _ptr(self._itk), _ptr(self._d), self.nparam,
+                                       C.cast(dm, C.c_void_p) if self._dmpo else None, int(checkpoint)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().mpsa_chain_destroy(self.h)
+            self.h = None
+
+    def _ptrs(self, bufs):
+        return (C.c_void_p * self.N)(*[_dptr(b) for b in bufs])
+
+    def forward(self, dA):
+        E = np.zeros(1)
+        arr = self._ptrs(dA)
+        _check(lib().mpsa_energy_forward(self.h, C.cast(arr, C.c_void_p), _ptr(E), _stream()))
+        return float(E[0])
+
+    def backward(self, dA):
+        import torch
+        grads = [torch.zeros_like(b) for b in dA]
+        dth = np.zeros(max(1, self.nparam))
+        a, g = self._ptrs(dA), self._ptrs(grads)
+        _check(lib().mpsa_energy_backward(self.h, C.cast(a, C.c_void_p), C.cast(g, C.c_void_p), _ptr(dth),
+                                          _stream()))
+        return grads, dth[:self.nparam].copy()
+
+    def env_table(self, side, index):
+        n = C.c_int64()
+        _check(lib().mpsa_chain_env_table(self.h, side, index, 0, C.byref(n), None, None, None, None, None))
+        k = n.value
+        q, p = np.zeros(2 * k, np.uint64), np.zeros(k, np.uint32)
+        r, c, o = np.zeros(k, np.int64), np.zeros(k, np.int64), np.zeros(k, np.int64)
+        _check(lib().mpsa_chain_env_table(self.h, side, index, k, C.byref(n), _ptr(q), _ptr(p), _ptr(r), _ptr(c),
+                                          _ptr(o)))
+        return Table(1, q.reshape(-1, 2), p.reshape(-1, 1), r, c, o)
 
 
 def tridiag_eig(d, e, vectors=True):

@@ -1,0 +1,132 @@
+"""Energy and its adjoint on the GPU (mpsa_chain_*): E = <psi|H|psi> by the
+left environment sweep of MPO::measure, dE/dA[i] for every site and
+dE/d(Jz, Jxy, hz) for the XXZ chain.
+
+Pins (none of them restates the gradient formula):
+  * E against the oracle's MPO::measure restatement;
+  * Euler's identity for a function quadratic in each A[i]:
+    <A[i], dE/dA[i]> = 2 E at every site;
+  * central finite differences, h = 1e-5, of E along seeded directions (E is
+    quadratic in each A[i], so the central difference is exact up to
+    rounding);
+  * dE/dtheta against Hellmann-Feynman expectation values, measured by the
+    oracle with the MPO of one coupling alone (H(theta) is linear in theta
+    with H(0) = 0), and against central differences in theta.
+"""
+import numpy as np
+import pytest
+
+from mpstools_amd import models, native, synth
+from mpstools_amd.blocks import Blocks
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+def _chain_problem(L, N, D, seed):
+    itk = [models.itokey_u1()] * L
+    mps, psi, c, dims = synth.centre_problem(L, itk, N, D, seed, None)
+    return mps, itk
+
+
+def _dev(torch, t, B):
+    return torch.from_numpy(t.pack(B)).cuda()
+
+
+def _perturbed(mps, i, delta, h):
+    out = list(mps)
+    A = mps[i].copy()
+    for k in A.keys():
+        A[k] = A[k] + h * delta[k]
+    out[i] = A
+    return out
+
+
+def _direction(A, rng):
+    d = Blocks(A.ns)
+    for k in A.keys():
+        d[k] = rng.standard_normal(A[k].shape)
+    return d
+
+
+@pytest.mark.parametrize("L,N,D,seed,K", [(10, 5, 12, 301, 3), (12, 6, 20, 302, 8)])
+def test_energy_gradients_small_chain(gpu, oracle, L, N, D, seed, K):
+    torch = gpu
+    _say("adjoint small", L, N, D, seed, "checkpoint", K)
+    mps, itk = _chain_problem(L, N, D, seed)
+    Jz, Jxy, hz = 0.8, 1.1, 0.3
+    mpo = models.xxz_mpo(L, Jz, Jxy, hz)
+    dm = models.xxz_param_derivatives(L)
+    tabs = [native.Table.from_blocks(A) for A in mps]
+    ch = native.Chain(tabs, mpo, itk, [dm["Jz"], dm["Jxy"], dm["hz"]], checkpoint=K)
+    dA = [_dev(torch, t, A) for t, A in zip(tabs, mps)]
+    E = ch.forward(dA)
+    E_ref = oracle.measure(mps, mpo, itk)
+    assert abs(E - E_ref) <= 1e-12 * max(1.0, abs(E_ref))
+    grads, dth = ch.backward(dA)
+    g = [t.unpack(x.cpu().numpy()) for t, x in zip(tabs, grads)]
+    for i in range(L):  # Euler: <A_i, dE/dA_i> = 2E
+        assert abs(mps[i].dot(g[i]) - 2 * E_ref) <= 1e-10 * max(1.0, abs(E_ref)), i
+    rng = np.random.default_rng(seed)
+    h = 1e-5
+    for i in range(L):
+        dlt = _direction(mps[i], rng)
+        ep = oracle.measure(_perturbed(mps, i, dlt, h), mpo, itk)
+        em = oracle.measure(_perturbed(mps, i, dlt, -h), mpo, itk)
+        fd = (ep - em) / (2 * h)
+        an = g[i].dot(dlt)
+        assert abs(fd - an) <= 1e-6 * max(1.0, abs(an)), (i, fd, an)
+    # Hellmann-Feynman: H(theta) is linear in theta, H(0) = 0
+    hf = [oracle.measure(mps, models.xxz_mpo(L, 1.0, 0.0, 0.0), itk),
+          oracle.measure(mps, models.xxz_mpo(L, 0.0, 1.0, 0.0), itk),
+          oracle.measure(mps, models.xxz_mpo(L, 0.0, 0.0, 1.0), itk)]
+    assert np.max(np.abs(dth - np.array(hf))) <= 1e-11 * max(1.0, np.max(np.abs(hf)))
+    th = np.array([Jz, Jxy, hz])
+    for p in range(3):
+        tp, tm = th.copy(), th.copy()
+        tp[p] += h
+        tm[p] -= h
+        fd = (oracle.measure(mps, models.xxz_mpo(L, *tp), itk) - oracle.measure(mps, models.xxz_mpo(L, *tm), itk)) / (2 * h)
+        assert abs(fd - dth[p]) <= 1e-6 * max(1.0, abs(dth[p])), p
+
+
+def test_energy_gradients_L64_D1024(gpu, oracle):
+    """XXZ L = 64, N_up = 32, D = 1024, FP64, checkpoints every 8 sites."""
+    torch = gpu
+    L, N, D = 64, 32, 1024
+    _say("adjoint L=64 D=1024")
+    mps, itk = _chain_problem(L, N, D, 20260004)
+    mpo = models.xxz_mpo(L, 1.0, 1.0, 0.0)
+    dm = models.xxz_param_derivatives(L)
+    tabs = [native.Table.from_blocks(A) for A in mps]
+    ch = native.Chain(tabs, mpo, itk, [dm["Jz"], dm["Jxy"], dm["hz"]], checkpoint=8)
+    dA = [_dev(torch, t, A) for t, A in zip(tabs, mps)]
+    E = ch.forward(dA)
+    E_ref = oracle.measure(mps, mpo, itk, nthreads=8)
+    _say("E", E, "oracle", E_ref)
+    assert abs(E - E_ref) <= 1e-12 * max(1.0, abs(E_ref))
+    grads, dth = ch.backward(dA)
+    for i in range(L):
+        s = float(torch.dot(grads[i], dA[i]))
+        assert abs(s - 2 * E) <= 1e-10 * max(1.0, abs(E)), i
+    # Hellmann-Feynman by GPU forward passes of each coupling's MPO alone
+    for p, args in enumerate([(1.0, 0.0, 0.0), (0.0, 1.0, 0.0), (0.0, 0.0, 1.0)]):
+        chp = native.Chain(tabs, models.xxz_mpo(L, *args), itk, checkpoint=8)
+        ep = chp.forward(dA)
+        assert abs(ep - dth[p]) <= 1e-11 * max(1.0, abs(ep)), p
+    # central difference at three sites through GPU forward passes
+    rng = np.random.default_rng(4)
+    h = 1e-5
+    for i in (0, 31, 63):
+        dlt = torch.from_numpy(tabs[i].pack(_direction(mps[i], rng))).cuda()
+        xs = list(dA)
+        xs[i] = dA[i] + h * dlt
+        ep = ch.forward(xs)
+        xs[i] = dA[i] - h * dlt
+        em = ch.forward(xs)
+        fd = (ep - em) / (2 * h)
+        an = float(torch.dot(grads[i], dlt))
+        assert abs(fd - an) <= 1e-6 * max(1.0, abs(an)), (i, fd, an)

@@ -146,3 +146,30 @@ def test_plain_schedule_reproduces_oracle_with_reference_flops(oracle, model, L,
     a = np.concatenate([yb[k].ravel() for k in ref.keys()])
     b = np.concatenate([ref[k].ravel() for k in ref.keys()])
     assert _rel(a, b) <= 1e-13
+
+
+def test_chain_environment_tables_match_oracle(oracle):
+    """mpsa_chain_create plans every environment of the chain on the host; its
+    key tables are the oracle's left/right environments (getLMin .. L[N-1],
+    getRMax .. R[0])."""
+    from mpstools_amd import models
+    L = 10
+    P = problems.build(oracle, "xxz", L, 5, 12, 3)
+    mps, mpo, itk = P["mps"], P["mpo"], P["itk"]
+    tabs = [native.Table.from_blocks(A) for A in mps]
+    d = models.xxz_param_derivatives(L)
+    ch = native.Chain(tabs, mpo, itk, [d["Jz"], d["Jxy"], d["hz"]], checkpoint=3)
+    Ls = oracle.left_envs(mps, mpo, itk, L - 1)
+    for i in range(-1, L):
+        ref = Ls[i] if i >= 0 else oracle.lmin()
+        assert ch.env_table(1, i).keys() == ref.keys(), i
+    Rs = oracle.right_envs(mps, mpo, itk, 0)
+    for i in range(0, L + 1):
+        ref = Rs[i] if i < L else oracle.rmax(mps[-1])
+        assert ch.env_table(0, i).keys() == ref.keys(), i
+    # derivative tensors: W(theta) = W(0) + sum_p theta_p dW_p entrywise
+    W = models.xxz_mpo(L, 0.7, 1.3, 0.2)
+    W0 = models.xxz_mpo(L, 0.0, 0.0, 0.0)
+    for i in range(L):
+        dense = W0[i].dense() + 0.7 * d["Jz"][i].dense() + 1.3 * d["Jxy"][i].dense() + 0.2 * d["hz"][i].dense()
+        assert np.allclose(dense, W[i].dense(), atol=1e-15)
