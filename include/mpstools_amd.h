@@ -203,6 +203,42 @@ int mpsa_tridiag_eig(int n, double* d, const double* e, double* z);
 /* Free the device vector pool kept between Lanczos calls. */
 int mpsa_release_pools(void);
 
+/* ---- split of a two-site tensor (split, src/MPSfunctions.hpp:1142-1278) ----
+ * psi (ns = 2 table, FP64 device buffer) -> left site tensor, singular values
+ * lambda, right site tensor.  Per centre charge qc = l + q(s1) the blocks are
+ * gathered into one dense matrix (rows by s1, columns by s2), decomposed by an
+ * FP64 SVD with the FixPhase sign rule (blas/lapack_routines.cpp:614-634,
+ * blas/helperfunctions.hpp:36-67) and truncated over all sectors together:
+ * do_trunc = 0 keeps at most chimax values above 1e-15
+ * (DiagBlockMatrix::truncate(chi), src/BlockTensor.hpp:234-276), do_trunc = 1
+ * keeps at most chimax values until the truncated weight falls below
+ * tr_weight (src/BlockTensor.hpp:85-136); chimax = 0 means 100 / 200 as in
+ * split :1257-1258.  normalize scales lambda to unit norm.  Left keys are
+ * (qc - q(s1), qc, s1), right keys (qc, qc + q(s2), s2) (split :1225-1238),
+ * blocks packed in key order; sectors with nothing kept are dropped
+ * (dropDangling, src/MPSfunctions.hpp:1444-1484).  direction > 0 multiplies
+ * lambda into the right tensor, direction <= 0 into the left one
+ * (DMRGengine::doStep, src/container.hpp:321-328).  The result owns its
+ * device buffers until mpsa_split_destroy. */
+typedef struct mpsa_split_result mpsa_split_result;
+int mpsa_split(const mpsa_blocks* psi, const double* dpsi, const uint64_t* itokey_l, int dl, const uint64_t* itokey_r,
+               int dr, int direction, unsigned chimax, int do_trunc, double tr_weight, int normalize,
+               mpsa_split_result** out, mpsa_stream stream);
+/* truncated weight and kept count (the pair split returns), block counts of
+ * the left/right tables, number of lambda sectors, packed buffer lengths */
+int mpsa_split_info(const mpsa_split_result* r, double* truncated_weight, int* chi, int64_t* n_left, int64_t* n_right,
+                    int64_t* n_lambda, int64_t* left_len, int64_t* right_len);
+/* which = 0: left table, 1: right table (arrays sized by mpsa_split_info) */
+int mpsa_split_table(const mpsa_split_result* r, int which, uint64_t* q, uint32_t* p, int64_t* rows, int64_t* cols,
+                     int64_t* offsets);
+/* device buffer of the left (0) or right (1) tensor, owned by the result */
+const double* mpsa_split_data(const mpsa_split_result* r, int which);
+/* copy the left (0) or right (1) packed buffer to caller device memory (synchronous) */
+int mpsa_split_copy(const mpsa_split_result* r, int which, double* dst, mpsa_stream stream);
+/* lambda: sector charges, sizes and the concatenated values (descending per sector) */
+int mpsa_split_lambda(const mpsa_split_result* r, uint64_t* q, int64_t* sizes, double* values);
+int mpsa_split_destroy(mpsa_split_result* r);
+
 /* ---- Energy and its adjoint over a whole chain ----
  * Forward: MPO<T>::measure (src/MPOTypes.hpp:990-1017), E = <psi|H|psi>
  * unnormalised, by the left environment sweep of addContractionLayer from

@@ -77,6 +77,18 @@ hipError_t launch_dot(const double* x, const double* y, int64_t n, int cplx, dou
 hipError_t launch_axpby(double ar, double ai, const double* x, double br, double bi, double* y, int64_t n, int cplx,
                         hipStream_t st);
 int gemm_max_grid(int prec_in, int tile_class);
+// dense block copies of the split (column-major; see copy2d_kernel)
+struct Copy2D {
+  int64_t src_off, dst_off;
+  int32_t rows, cols, lds, ldd;
+  int64_t rs_off, cs_off;  // offsets into the scale array, -1 for none
+  double scale;
+  int32_t trans, pad;
+};
+hipError_t launch_copy2d(const Copy2D* tasks, int ntasks, int64_t max_elems, const double* src, double* dst,
+                         const double* scales, hipStream_t st);
+hipError_t launch_fixphase(double* U, int64_t m, int64_t ldu, double* Vt, int64_t k, int64_t n, int64_t ldv,
+                           hipStream_t st);
 // batched Lanczos vector algebra (device-resident coefficients and step counter)
 hipError_t launch_bdot(const double* x, const double* y, int64_t n, int64_t vs, int nvec, double* partial, int nparts,
                        double* out, int ld, const int* step, int root, hipStream_t st);

@@ -473,17 +473,135 @@ typedef __attribute__((address_space(3))) void* lds_vptr;
 typedef __attribute__((address_space(1))) void* gbl_vptr;
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
-// LDS fragment read issued as inline asm: the compiler cannot see that it reads
-// LDS, so it does not wait for every outstanding global->LDS load before it
-// (it would otherwise have to, lacking alias information).  Completion is
-// awaited explicitly with lds_wait() before the MFMAs use the registers.
-__device__ __forceinline__ bf8 lds_frag(const unsigned char* p) {
-  u32x4 r;
-  const unsigned addr = (unsigned)(uintptr_t)(const __attribute__((address_space(3))) unsigned char*)p;
-  asm volatile("ds_read_b128 %0, %1" : "=v"(r) : "v"(addr));
-  return __builtin_bit_cast(bf8, r);
+// Fragment reads of one pipeline step from LDS.  The reads are issued from
+// inline asm (the compiler's own LDS loads would be made to wait for every
+// LDS-DMA load still in flight, since it cannot tell the stages apart), and
+// the same asm statement waits for them (lgkmcnt(0)) before it ends: an asm
+// statement's outputs must hold their values when the statement completes.
+// Fragment i of A lies 1024 B after fragment i-1 (16 rows of 64 B; the row
+// swizzle depends only on the lane), likewise for B; the lo planes follow at
+// A_BYTES = 16384 and B_BYTES = 8192.
+__device__ __forceinline__ void lds_frags_plain(unsigned aaddr, unsigned baddr, bf8 (&a)[8], bf8 (&b)[4]) {
+  u32x4 r0;
+  u32x4 r1;
+  u32x4 r2;
+  u32x4 r3;
+  u32x4 r4;
+  u32x4 r5;
+  u32x4 r6;
+  u32x4 r7;
+  u32x4 r8;
+  u32x4 r9;
+  u32x4 r10;
+  u32x4 r11;
+  asm volatile(
+      "ds_read_b128 %0, %12\n\t"
+      "ds_read_b128 %1, %12 offset:1024\n\t"
+      "ds_read_b128 %2, %12 offset:2048\n\t"
+      "ds_read_b128 %3, %12 offset:3072\n\t"
+      "ds_read_b128 %4, %12 offset:4096\n\t"
+      "ds_read_b128 %5, %12 offset:5120\n\t"
+      "ds_read_b128 %6, %12 offset:6144\n\t"
+      "ds_read_b128 %7, %12 offset:7168\n\t"
+      "ds_read_b128 %8, %13\n\t"
+      "ds_read_b128 %9, %13 offset:1024\n\t"
+      "ds_read_b128 %10, %13 offset:2048\n\t"
+      "ds_read_b128 %11, %13 offset:3072\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(r0), "=&v"(r1), "=&v"(r2), "=&v"(r3), "=&v"(r4), "=&v"(r5), "=&v"(r6), "=&v"(r7), "=&v"(r8), "=&v"(r9), "=&v"(r10), "=&v"(r11)
+      : "v"(aaddr), "v"(baddr));
+  a[0] = __builtin_bit_cast(bf8, r0);
+  a[1] = __builtin_bit_cast(bf8, r1);
+  a[2] = __builtin_bit_cast(bf8, r2);
+  a[3] = __builtin_bit_cast(bf8, r3);
+  a[4] = __builtin_bit_cast(bf8, r4);
+  a[5] = __builtin_bit_cast(bf8, r5);
+  a[6] = __builtin_bit_cast(bf8, r6);
+  a[7] = __builtin_bit_cast(bf8, r7);
+  b[0] = __builtin_bit_cast(bf8, r8);
+  b[1] = __builtin_bit_cast(bf8, r9);
+  b[2] = __builtin_bit_cast(bf8, r10);
+  b[3] = __builtin_bit_cast(bf8, r11);
 }
-__device__ __forceinline__ void lds_wait() { __builtin_amdgcn_s_waitcnt((15 & 15) | (7 << 4) | (0 << 8) | (3 << 14)); }
+
+__device__ __forceinline__ void lds_frags_split(unsigned aaddr, unsigned baddr, bf8 (&a)[8], bf8 (&b)[4], bf8 (&al)[8], bf8 (&bl)[4]) {
+  u32x4 r0;
+  u32x4 r1;
+  u32x4 r2;
+  u32x4 r3;
+  u32x4 r4;
+  u32x4 r5;
+  u32x4 r6;
+  u32x4 r7;
+  u32x4 r8;
+  u32x4 r9;
+  u32x4 r10;
+  u32x4 r11;
+  u32x4 r12;
+  u32x4 r13;
+  u32x4 r14;
+  u32x4 r15;
+  u32x4 r16;
+  u32x4 r17;
+  u32x4 r18;
+  u32x4 r19;
+  u32x4 r20;
+  u32x4 r21;
+  u32x4 r22;
+  u32x4 r23;
+  asm volatile(
+      "ds_read_b128 %0, %24\n\t"
+      "ds_read_b128 %1, %24 offset:1024\n\t"
+      "ds_read_b128 %2, %24 offset:2048\n\t"
+      "ds_read_b128 %3, %24 offset:3072\n\t"
+      "ds_read_b128 %4, %24 offset:4096\n\t"
+      "ds_read_b128 %5, %24 offset:5120\n\t"
+      "ds_read_b128 %6, %24 offset:6144\n\t"
+      "ds_read_b128 %7, %24 offset:7168\n\t"
+      "ds_read_b128 %8, %25\n\t"
+      "ds_read_b128 %9, %25 offset:1024\n\t"
+      "ds_read_b128 %10, %25 offset:2048\n\t"
+      "ds_read_b128 %11, %25 offset:3072\n\t"
+      "ds_read_b128 %12, %24 offset:16384\n\t"
+      "ds_read_b128 %13, %24 offset:17408\n\t"
+      "ds_read_b128 %14, %24 offset:18432\n\t"
+      "ds_read_b128 %15, %24 offset:19456\n\t"
+      "ds_read_b128 %16, %24 offset:20480\n\t"
+      "ds_read_b128 %17, %24 offset:21504\n\t"
+      "ds_read_b128 %18, %24 offset:22528\n\t"
+      "ds_read_b128 %19, %24 offset:23552\n\t"
+      "ds_read_b128 %20, %25 offset:8192\n\t"
+      "ds_read_b128 %21, %25 offset:9216\n\t"
+      "ds_read_b128 %22, %25 offset:10240\n\t"
+      "ds_read_b128 %23, %25 offset:11264\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(r0), "=&v"(r1), "=&v"(r2), "=&v"(r3), "=&v"(r4), "=&v"(r5), "=&v"(r6), "=&v"(r7), "=&v"(r8), "=&v"(r9), "=&v"(r10), "=&v"(r11), "=&v"(r12), "=&v"(r13), "=&v"(r14), "=&v"(r15), "=&v"(r16), "=&v"(r17), "=&v"(r18), "=&v"(r19), "=&v"(r20), "=&v"(r21), "=&v"(r22), "=&v"(r23)
+      : "v"(aaddr), "v"(baddr));
+  a[0] = __builtin_bit_cast(bf8, r0);
+  a[1] = __builtin_bit_cast(bf8, r1);
+  a[2] = __builtin_bit_cast(bf8, r2);
+  a[3] = __builtin_bit_cast(bf8, r3);
+  a[4] = __builtin_bit_cast(bf8, r4);
+  a[5] = __builtin_bit_cast(bf8, r5);
+  a[6] = __builtin_bit_cast(bf8, r6);
+  a[7] = __builtin_bit_cast(bf8, r7);
+  b[0] = __builtin_bit_cast(bf8, r8);
+  b[1] = __builtin_bit_cast(bf8, r9);
+  b[2] = __builtin_bit_cast(bf8, r10);
+  b[3] = __builtin_bit_cast(bf8, r11);
+  al[0] = __builtin_bit_cast(bf8, r12);
+  al[1] = __builtin_bit_cast(bf8, r13);
+  al[2] = __builtin_bit_cast(bf8, r14);
+  al[3] = __builtin_bit_cast(bf8, r15);
+  al[4] = __builtin_bit_cast(bf8, r16);
+  al[5] = __builtin_bit_cast(bf8, r17);
+  al[6] = __builtin_bit_cast(bf8, r18);
+  al[7] = __builtin_bit_cast(bf8, r19);
+  bl[0] = __builtin_bit_cast(bf8, r20);
+  bl[1] = __builtin_bit_cast(bf8, r21);
+  bl[2] = __builtin_bit_cast(bf8, r22);
+  bl[3] = __builtin_bit_cast(bf8, r23);
+}
 
 template <int N>
 __device__ __forceinline__ void wait_vmcnt() {
@@ -511,18 +629,11 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16_stream_kernel(
   const int l16 = lane & 15, lq = lane >> 4;
   const unsigned short* Au = reinterpret_cast<const unsigned short*>(A);
   const unsigned short* Bu = reinterpret_cast<const unsigned short*>(B);
-  // fragment byte offsets inside a plane (row-dependent swizzle)
-  int aoff[TM], boff[TN];
-#pragma unroll
-  for (int i = 0; i < TM; ++i) {
-    const int r = wm * WM + i * 16 + l16;
-    aoff[i] = r * ROWB + ((lq ^ ((r >> 2) & 3)) << 4);
-  }
-#pragma unroll
-  for (int j = 0; j < TN; ++j) {
-    const int r = wn * WN + j * 16 + l16;
-    boff[j] = r * ROWB + ((lq ^ ((r >> 2) & 3)) << 4);
-  }
+  static_assert(TM == 8 && TN == 4 && A_BYTES == 16384 && B_BYTES == 8192, "fragment reader layout");
+  // byte offsets of fragment 0 inside a plane; the swizzle (r >> 2) & 3 of row
+  // r = base + 16 i + l16 does not depend on i
+  const int aoff0 = (wm * WM + l16) * ROWB + ((lq ^ ((l16 >> 2) & 3)) << 4);
+  const int boff0 = (wn * WN + l16) * ROWB + ((lq ^ ((l16 >> 2) & 3)) << 4);
   // loader lane mapping: instruction q of this wave covers rows 16*(wave*L + q) + lane/4
   const int lrow = lane >> 2, lslot = lane & 3;
 
@@ -612,18 +723,13 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16_stream_kernel(
       }
       if (a != 0.f) {
         const unsigned char* base = smem + stage * STAGE_BYTES;
+        const unsigned aaddr = (unsigned)(uintptr_t)(const __attribute__((address_space(3))) unsigned char*)(base + aoff0);
+        const unsigned baddr =
+            (unsigned)(uintptr_t)(const __attribute__((address_space(3))) unsigned char*)(base + NP * A_BYTES + boff0);
         bf8 bh[TN], ah[TM];
-#pragma unroll
-        for (int j = 0; j < TN; ++j) bh[j] = lds_frag(base + NP * A_BYTES + boff[j]);
-#pragma unroll
-        for (int i = 0; i < TM; ++i) ah[i] = lds_frag(base + aoff[i]);
         if (SPLIT) {
           bf8 bl[TN], al[TM];
-#pragma unroll
-          for (int j = 0; j < TN; ++j) bl[j] = lds_frag(base + NP * A_BYTES + (NP - 1) * B_BYTES + boff[j]);
-#pragma unroll
-          for (int i = 0; i < TM; ++i) al[i] = lds_frag(base + (NP - 1) * A_BYTES + aoff[i]);
-          lds_wait();
+          lds_frags_split(aaddr, baddr, ah, bh, al, bl);
 #pragma unroll
           for (int i = 0; i < TM; ++i)
 #pragma unroll
@@ -635,7 +741,7 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16_stream_kernel(
             for (int j = 0; j < TN; ++j)
               acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
         } else {
-          lds_wait();
+          lds_frags_plain(aaddr, baddr, ah, bh);
         }
 #pragma unroll
         for (int i = 0; i < TM; ++i)
@@ -928,6 +1034,63 @@ __global__ void lz_combine_kernel(double* const* __restrict__ basis, const doubl
 }
 
 // ---------------------------------------------------------------------------
+// Split of a two-site tensor: block gather/scatter between packed block
+// buffers and per-sector dense matrices, and the FixPhase sign rule.
+// ---------------------------------------------------------------------------
+// dst(i, j) = scale * src(i, j) [* rs[i]] [* cs[j]]  (column-major; src
+// transposed when trans: src(j, i) with ld lds)
+__global__ __launch_bounds__(256) void copy2d_kernel(const Copy2D* __restrict__ tasks, const double* __restrict__ src,
+                                                     double* __restrict__ dst, const double* __restrict__ sc) {
+  const Copy2D t = tasks[blockIdx.y];
+  const int64_t total = (int64_t)t.rows * t.cols;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e % t.rows, j = e / t.rows;
+    double v = t.trans ? src[t.src_off + i * t.lds + j] : src[t.src_off + j * t.lds + i];
+    v *= t.scale;
+    if (t.rs_off >= 0) v *= sc[t.rs_off + i];
+    if (t.cs_off >= 0) v *= sc[t.cs_off + j];
+    dst[t.dst_off + j * t.ldd + i] = v;
+  }
+}
+
+// FixPhase (real case): vector i = row i of Vt (k x n, ld ldv).  Find the first
+// index of the largest |entry|, take phi = +1 if that entry is > 0 else -1,
+// and multiply row i of Vt and column i of U (m x k, ld ldu) by phi.
+__global__ __launch_bounds__(256) void fixphase_kernel(double* __restrict__ U, int64_t m, int64_t ldu,
+                                                       double* __restrict__ Vt, int64_t n, int64_t ldv) {
+  __shared__ double sv[256];
+  __shared__ int64_t si[256];
+  const int64_t i = blockIdx.x;
+  double best = 0.0;
+  int64_t bi = 0;
+  for (int64_t j = threadIdx.x; j < n; j += blockDim.x) {
+    const double a = fabs(Vt[i + j * ldv]);
+    if (a > best) {  // strided scan: the first index of each thread's maximum
+      best = a;
+      bi = j;
+    }
+  }
+  // threads that saw no entry above 0 keep (0, 0), which never beats a true entry
+  sv[threadIdx.x] = best;
+  si[threadIdx.x] = bi;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      const double a = sv[threadIdx.x + w];
+      const int64_t ai = si[threadIdx.x + w];
+      if (a > sv[threadIdx.x] || (a == sv[threadIdx.x] && ai < si[threadIdx.x])) {
+        sv[threadIdx.x] = a;
+        si[threadIdx.x] = ai;
+      }
+    }
+    __syncthreads();
+  }
+  const double phi = Vt[i + si[0] * ldv] > 0.0 ? 1.0 : -1.0;
+  for (int64_t j = threadIdx.x; j < n; j += blockDim.x) Vt[i + j * ldv] *= phi;
+  for (int64_t r = threadIdx.x; r < m; r += blockDim.x) U[r + i * ldu] *= phi;
+}
+
+// ---------------------------------------------------------------------------
 // Repack FP64 blocks into the compute layout (type conversion, optional
 // transpose, padded leading dimension).  One task per block, 1024-element
 // chunks of the source block.
@@ -1205,6 +1368,22 @@ hipError_t launch_lz_step(int* step, hipStream_t st) {
 hipError_t launch_lz_combine(double* const* basis, const double* c, int ldc, const int* m, double* out, int64_t n,
                              int64_t vs, int nvec, hipStream_t st) {
   lz_combine_kernel<<<dim3(vec_grid(n), nvec), 256, 0, st>>>(basis, c, ldc, m, out, n, vs);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy2d(const Copy2D* tasks, int ntasks, int64_t max_elems, const double* src, double* dst,
+                         const double* scales, hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  int64_t blocks = (max_elems + 255) / 256;
+  int gx = (int)(blocks < 256 ? (blocks > 0 ? blocks : 1) : 256);
+  copy2d_kernel<<<dim3(gx, ntasks), 256, 0, st>>>(tasks, src, dst, scales);
+  return hipGetLastError();
+}
+
+hipError_t launch_fixphase(double* U, int64_t m, int64_t ldu, double* Vt, int64_t k, int64_t n, int64_t ldv,
+                           hipStream_t st) {
+  if (k <= 0) return hipSuccess;
+  fixphase_kernel<<<(unsigned)k, 256, 0, st>>>(U, m, ldu, Vt, n, ldv);
   return hipGetLastError();
 }
 

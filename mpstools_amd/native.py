@@ -80,6 +80,14 @@ def lib():
         L.mpsa_energy_backward.argtypes = [P, P, P, P, P]
         L.mpsa_chain_env_table.argtypes = [P, C.c_int, C.c_int, C.c_int64, P, P, P, P, P, P]
         L.mpsa_chain_destroy.argtypes = [P]
+        L.mpsa_split.argtypes = [P, P, P, C.c_int, P, C.c_int, C.c_int, C.c_uint, C.c_int, C.c_double, C.c_int, P, P]
+        L.mpsa_split_info.argtypes = [P, P, P, P, P, P, P, P]
+        L.mpsa_split_table.argtypes = [P, C.c_int, P, P, P, P, P]
+        L.mpsa_split_data.argtypes = [P, C.c_int]
+        L.mpsa_split_data.restype = P
+        L.mpsa_split_copy.argtypes = [P, C.c_int, P, P]
+        L.mpsa_split_lambda.argtypes = [P, P, P, P]
+        L.mpsa_split_destroy.argtypes = [P]
         L.mpsa_tridiag_eig.argtypes = [C.c_int, P, P, P]
         L.mpsa_dot.argtypes = [P, P, C.c_int64, C.c_int, P, P]
         L.mpsa_axpby.argtypes = [P, P, P, P, C.c_int64, C.c_int, P]
@@ -413,6 +421,48 @@ class Chain:
         _check(lib().mpsa_chain_env_table(self.h, side, index, k, C.byref(n), _ptr(q), _ptr(p), _ptr(r), _ptr(c),
                                           _ptr(o)))
         return Table(1, q.reshape(-1, 2), p.reshape(-1, 1), r, c, o)
+
+
+def split(psi_t: Table, dpsi, itk_l, itk_r, direction=1, chimax=0, do_trunc=False, tr_weight=0.0, normalize=True):
+    """split of a two-site tensor on the device (src/MPSfunctions.hpp:1142-1278).
+    Returns dict(left=(Table, device tensor), right=(Table, device tensor),
+    lam={qc: values}, tw, chi); lambda is multiplied into the right tensor for
+    direction > 0, into the left one otherwise."""
+    import torch
+    L = lib()
+    itl, itr = _itk(itk_l), _itk(itk_r)
+    h = C.c_void_p()
+    _check(L.mpsa_split(C.byref(psi_t.s), _dptr(dpsi), _ptr(itl), len(itl), _ptr(itr), len(itr), int(direction),
+                        int(chimax), int(bool(do_trunc)), float(tr_weight), int(bool(normalize)), C.byref(h),
+                        _stream()))
+    try:
+        tw, chi = np.zeros(1), np.zeros(1, np.int32)
+        nl, nr, nlam, ll, rl = (np.zeros(1, np.int64) for _ in range(5))
+        _check(L.mpsa_split_info(h, _ptr(tw), _ptr(chi), _ptr(nl), _ptr(nr), _ptr(nlam), _ptr(ll), _ptr(rl)))
+        out = dict(tw=float(tw[0]), chi=int(chi[0]))
+        for which, n, ln, name in ((0, int(nl[0]), int(ll[0]), "left"), (1, int(nr[0]), int(rl[0]), "right")):
+            q, p = np.zeros(2 * n, np.uint64), np.zeros(n, np.uint32)
+            r, c, o = np.zeros(n, np.int64), np.zeros(n, np.int64), np.zeros(n, np.int64)
+            if n:
+                _check(L.mpsa_split_table(h, which, _ptr(q), _ptr(p), _ptr(r), _ptr(c), _ptr(o)))
+            t = Table(1, q.reshape(-1, 2), p.reshape(-1, 1), r, c, o)
+            buf = torch.zeros(max(ln, 1), dtype=torch.float64, device=dpsi.device)
+            if ln:
+                _check(L.mpsa_split_copy(h, which, _dptr(buf), _stream()))
+            out[name] = (t, buf)
+        k = int(nlam[0])
+        lq, ls = np.zeros(k, np.uint64), np.zeros(k, np.int64)
+        vals = np.zeros(max(1, out["chi"]))
+        if k:
+            _check(L.mpsa_split_lambda(h, _ptr(lq), _ptr(ls), _ptr(vals)))
+        lam, o = {}, 0
+        for i in range(k):
+            lam[int(lq[i])] = vals[o:o + int(ls[i])].copy()
+            o += int(ls[i])
+        out["lam"] = lam
+        return out
+    finally:
+        L.mpsa_split_destroy(h)
 
 
 def tridiag_eig(d, e, vectors=True):

@@ -17,7 +17,7 @@ import numpy as np
 import pytest
 
 from mpstools_amd import models, native, synth
-from mpstools_amd.blocks import Blocks
+from mpstools_amd.blocks import Blocks, twosite
 
 pytestmark = pytest.mark.gpu
 
@@ -27,9 +27,24 @@ def _say(*a):
 
 
 def _chain_problem(L, N, D, seed):
+    """Mixed-canonical random MPS with unit norm.  The QR sweeps leave the
+    norm in the two centre tensors, one of them growing and the other
+    shrinking with L; both are rescaled to unit Frobenius norm and then
+    together to a unit-norm state, so entries and E stay O(1)."""
     itk = [models.itokey_u1()] * L
     mps, psi, c, dims = synth.centre_problem(L, itk, N, D, seed, None)
+    for s in (c, c + 1):
+        mps[s] = mps[s].scaled(1.0 / mps[s].norm())
+    f = twosite(mps[c], mps[c + 1]).norm() ** 0.5
+    mps[c] = mps[c].scaled(1.0 / f)
+    mps[c + 1] = mps[c + 1].scaled(1.0 / f)
     return mps, itk
+
+
+def _fd_tol(an, E, h):
+    """Central-difference bar: 1e-6 relative, plus the rounding floor
+    64 eps |E| / h of a difference of two energies."""
+    return 1e-6 * max(1.0, abs(an)) + 64 * 2.2e-16 * abs(E) / h
 
 
 def _dev(torch, t, B):
@@ -78,7 +93,7 @@ def test_energy_gradients_small_chain(gpu, oracle, L, N, D, seed, K):
         em = oracle.measure(_perturbed(mps, i, dlt, -h), mpo, itk)
         fd = (ep - em) / (2 * h)
         an = g[i].dot(dlt)
-        assert abs(fd - an) <= 1e-6 * max(1.0, abs(an)), (i, fd, an)
+        assert abs(fd - an) <= _fd_tol(an, E_ref, h), (i, fd, an)
     # Hellmann-Feynman: H(theta) is linear in theta, H(0) = 0
     hf = [oracle.measure(mps, models.xxz_mpo(L, 1.0, 0.0, 0.0), itk),
           oracle.measure(mps, models.xxz_mpo(L, 0.0, 1.0, 0.0), itk),
@@ -90,7 +105,7 @@ def test_energy_gradients_small_chain(gpu, oracle, L, N, D, seed, K):
         tp[p] += h
         tm[p] -= h
         fd = (oracle.measure(mps, models.xxz_mpo(L, *tp), itk) - oracle.measure(mps, models.xxz_mpo(L, *tm), itk)) / (2 * h)
-        assert abs(fd - dth[p]) <= 1e-6 * max(1.0, abs(dth[p])), p
+        assert abs(fd - dth[p]) <= _fd_tol(dth[p], E_ref, h), p
 
 
 def test_energy_gradients_L64_D1024(gpu, oracle):
@@ -129,4 +144,4 @@ def test_energy_gradients_L64_D1024(gpu, oracle):
         em = ch.forward(xs)
         fd = (ep - em) / (2 * h)
         an = float(torch.dot(grads[i], dlt))
-        assert abs(fd - an) <= 1e-6 * max(1.0, abs(an)), (i, fd, an)
+        assert abs(fd - an) <= _fd_tol(an, E, h), (i, fd, an)

@@ -224,3 +224,80 @@ def test_models_match_dense_spin_chain(oracle):
     Href = sum(Jz[i] * op(sz, i) @ op(sz, i + 1) + Jxy[i] / 2 * (op(sp, i) @ op(sm, i + 1) + op(sm, i) @ op(sp, i + 1))
                for i in range(L - 1)) - sum(B[i] * op(sz, i) for i in range(L))
     assert np.max(np.abs(H - Href)) < 1e-14
+
+
+# ---------------------------------------------------------------- split oracle
+def _dense_twosite(psi, itk_l, itk_r):
+    """psi as one dense matrix over row labels (l, s1) and column labels (r, s2)."""
+    rows, cols = {}, {}
+    for (l, r, s1, s2) in psi.keys():
+        rows[(l, s1)] = psi[(l, r, s1, s2)].shape[0]
+        cols[(r, s2)] = psi[(l, r, s1, s2)].shape[1]
+    ro, o = {}, 0
+    for k in sorted(rows):
+        ro[k] = o
+        o += rows[k]
+    co, p = {}, 0
+    for k in sorted(cols):
+        co[k] = p
+        p += cols[k]
+    M = np.zeros((o, p))
+    for (l, r, s1, s2) in psi.keys():
+        b = psi[(l, r, s1, s2)]
+        M[ro[(l, s1)]:ro[(l, s1)] + b.shape[0], co[(r, s2)]:co[(r, s2)] + b.shape[1]] = b
+    return M
+
+
+@pytest.mark.parametrize("model,L,N,D,seed", [("xxz", 10, 5, 12, 401), ("tv", 12, 6, 20, 402)])
+def test_split_oracle_exact_decomposition(oracle, model, L, N, D, seed):
+    import pysplit
+    P = problems.build(oracle, model, L, N, D, seed)
+    c, itk, psi = P["c"], P["itk"], P["psi"]
+    left, lam, right, tw, chi = pysplit.split(psi, itk[c], itk[c + 1], chimax=10 ** 6, normalize=False, direction=1)
+    # singular values: the union over sectors is the spectrum of the dense matrix
+    sv = np.sort(np.concatenate([v for v in lam.values()]))[::-1]
+    ref = np.linalg.svd(_dense_twosite(psi, itk[c], itk[c + 1]), compute_uv=False)
+    ref = ref[ref > 1e-15]
+    assert len(sv) == len(ref) == chi
+    assert np.allclose(sv, ref, rtol=1e-12, atol=1e-14)
+    assert abs(tw - (1.0 - np.sum(ref ** 2))) <= 1e-12
+    # reconstruction psi(l, r, s1, s2) = U[(l, qc, s1)] (lambda V')[(qc, r, s2)]
+    for (l, r, s1, s2) in psi.keys():
+        qc = (l + itk[c][s1]) % (1 << 64)
+        got = left[(l, qc, s1)] @ right[(qc, r, s2)]
+        assert np.allclose(got, psi[(l, r, s1, s2)], atol=1e-12)
+    # left isometry per sector, FixPhase sign rule on V' = lambda^-1 (lambda V')
+    for qc, v in lam.items():
+        us = [left[k] for k in left.keys() if k[1] == qc]
+        G = sum(u.T @ u for u in us)
+        assert np.allclose(G, np.eye(len(v)), atol=1e-12)
+        vt = np.concatenate([right[k] for k in right.keys() if k[0] == qc], axis=1) / v[:, None]
+        for i in range(vt.shape[0]):
+            j = int(np.argmax(np.abs(vt[i])))
+            assert vt[i, j] > 0
+
+
+def test_split_oracle_truncation_rules(oracle):
+    import pysplit
+    P = problems.build(oracle, "xxz", 12, 6, 24, 403)
+    c, itk, psi = P["c"], P["itk"], P["psi"]
+    full = np.linalg.svd(_dense_twosite(psi, itk[c], itk[c + 1]), compute_uv=False)
+    for chi in (1, 5, 17):
+        left, lam, right, tw, cnt = pysplit.split(psi, itk[c], itk[c + 1], chimax=chi, direction=0)
+        kept = np.sort(np.concatenate([v for v in lam.values()]))[::-1]
+        assert cnt == chi == len(kept)
+        assert abs(np.sum(kept ** 2) - 1.0) <= 1e-12  # normalised
+        top = full[:chi]
+        assert np.allclose(kept, top / np.linalg.norm(top), rtol=1e-12)
+        assert abs(tw - (1.0 - np.sum(top ** 2))) <= 1e-12
+        # direction 0: lambda sits in the left tensor, the right one is row-orthonormal
+        for qc in lam:
+            vt = np.concatenate([right[k] for k in right.keys() if k[0] == qc], axis=1)
+            assert np.allclose(vt @ vt.T, np.eye(vt.shape[0]), atol=1e-12)
+    # weight mode: stop once the remaining weight drops below tr_weight * norm
+    nrm = np.sqrt(np.sum(full ** 2))
+    for w in (1e-1, 1e-3):
+        _, lam, _, tw, cnt = pysplit.split(psi, itk[c], itk[c + 1], chimax=1000, do_trunc=True, tr_weight=w)
+        rem = nrm - np.cumsum(full ** 2) / nrm
+        expect = int(np.argmax(rem <= nrm * w)) + 1
+        assert cnt == expect
