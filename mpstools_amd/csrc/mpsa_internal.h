@@ -44,6 +44,7 @@ struct Repack {
   int32_t rows, cols;
   int32_t dst_ld, transpose;
   int32_t chunk0, pad;
+  double scale = 1.0;  // dst = convert(scale * src)
 };
 
 struct Range {
@@ -102,9 +103,12 @@ hipError_t launch_lz_combine(double* const* basis, const double* c, int ldc, con
                              int64_t vs, int nvec, hipStream_t st);
 
 // Workgroup tile of each (precision, tile class).  FP64/FP32: 0 = 128x128,
-// 1 = 64x64.  BF16/BF16X3: 0 = 256x128, 1 = 64x64, 2 = 128x128.
+// 1 = 64x64.  BF16/BF16X3: 0 = 256x128, 1 = 64x64, 2 = 128x128, 3 = 256x256.
 inline void tile_dims(int prec, int cls, int* bm, int* bn) {
-  if (cls == 1) {
+  if ((prec == BF16 || prec == BF16X3) && cls == 3) {
+    *bm = 256;
+    *bn = 256;
+  } else if (cls == 1) {
     *bm = 64;
     *bn = 64;
   } else if ((prec == BF16 || prec == BF16X3) && cls == 0) {
@@ -115,7 +119,7 @@ inline void tile_dims(int prec, int cls, int* bm, int* bn) {
     *bn = 128;
   }
 }
-inline int num_tile_classes(int prec) { return (prec == BF16 || prec == BF16X3) ? 3 : 2; }
+inline int num_tile_classes(int prec) { return (prec == BF16 || prec == BF16X3) ? 4 : 2; }
 void set_last_error(const char* msg);
 
 }  // namespace mpsa

@@ -460,147 +460,196 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_kernel(const GTask* __restri
 // ---------------------------------------------------------------------------
 // Streaming BF16 grouped GEMM: operands go straight from global memory into
 // LDS with global_load_lds_dwordx4 (no register staging), NSTAGE k-steps in
-// flight, one barrier per k-step.  256x128 workgroup tile, four waves of
-// 128x64.  Requirements (met by the plan's bf16 layouts): k dimensions padded
-// with zeros to a multiple of 32, so every k-step is a full 32-wide slab.  Rows
-// past m (columns past n) are loaded from a clamped, valid address and their
-// garbage results are never stored.  LDS rows are 64 bytes with the 16-byte
-// chunk index XOR-ed by (row >> 2) & 3, which keeps every quarter-wave's
-// fragment reads on distinct banks.  Segment scales are applied by rescaling
-// the accumulators when the scale changes (exact for powers of two).
+// flight, one barrier per k-step.  Workgroup tile 256 x BN (BN = 128 or 256),
+// four waves of 128 x BN/2.  Requirements (met by the plan's bf16 layouts): k
+// dimensions padded with zeros to a multiple of 32, so every k-step is a full
+// 32-wide slab.  Rows past m (columns past n) are loaded from a clamped, valid
+// address and their garbage results are never stored.  LDS rows are 64 bytes
+// with the 16-byte chunk index XOR-ed by (row >> 2) & 3, which keeps every
+// quarter-wave's fragment reads on distinct banks.  Segment scales are applied
+// by rescaling the accumulators when the scale changes.
 // ---------------------------------------------------------------------------
 typedef __attribute__((address_space(3))) void* lds_vptr;
 typedef __attribute__((address_space(1))) void* gbl_vptr;
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
-// Fragment reads of one pipeline step from LDS.  The reads are issued from
-// inline asm (the compiler's own LDS loads would be made to wait for every
-// LDS-DMA load still in flight, since it cannot tell the stages apart), and
-// the same asm statement waits for them (lgkmcnt(0)) before it ends: an asm
-// statement's outputs must hold their values when the statement completes.
-// Fragment i of A lies 1024 B after fragment i-1 (16 rows of 64 B; the row
-// swizzle depends only on the lane), likewise for B; the lo planes follow at
-// A_BYTES = 16384 and B_BYTES = 8192.
-__device__ __forceinline__ void lds_frags_plain(unsigned aaddr, unsigned baddr, bf8 (&a)[8], bf8 (&b)[4]) {
-  u32x4 r0;
-  u32x4 r1;
-  u32x4 r2;
-  u32x4 r3;
-  u32x4 r4;
-  u32x4 r5;
-  u32x4 r6;
-  u32x4 r7;
-  u32x4 r8;
-  u32x4 r9;
-  u32x4 r10;
-  u32x4 r11;
+// Fragment reads from LDS.  The reads are issued from inline asm (the
+// compiler's own LDS loads would be made to wait for every LDS-DMA load still
+// in flight, since it cannot tell the pipeline stages apart), and the same asm
+// statement waits for them (lgkmcnt(0)) before it ends: an asm statement's
+// outputs must hold their values when the statement completes.  Each address
+// operand starts a group of fragments 1024 B apart (16 rows of 64 B; the row
+// swizzle depends only on the lane).
+__device__ __forceinline__ void lds_rd_4(unsigned a0, bf8 (&f)[4]) {
+  u32x4 r0, r1, r2, r3;
+  asm volatile(
+      "ds_read_b128 %0, %4\n\t"
+      "ds_read_b128 %1, %4 offset:1024\n\t"
+      "ds_read_b128 %2, %4 offset:2048\n\t"
+      "ds_read_b128 %3, %4 offset:3072\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(r0), "=&v"(r1), "=&v"(r2), "=&v"(r3)
+      : "v"(a0));
+  f[0] = __builtin_bit_cast(bf8, r0);
+  f[1] = __builtin_bit_cast(bf8, r1);
+  f[2] = __builtin_bit_cast(bf8, r2);
+  f[3] = __builtin_bit_cast(bf8, r3);
+}
+
+__device__ __forceinline__ void lds_rd_4_4(unsigned a0, unsigned a1, bf8 (&f)[8]) {
+  u32x4 r0, r1, r2, r3, r4, r5, r6, r7;
+  asm volatile(
+      "ds_read_b128 %0, %8\n\t"
+      "ds_read_b128 %1, %8 offset:1024\n\t"
+      "ds_read_b128 %2, %8 offset:2048\n\t"
+      "ds_read_b128 %3, %8 offset:3072\n\t"
+      "ds_read_b128 %4, %9\n\t"
+      "ds_read_b128 %5, %9 offset:1024\n\t"
+      "ds_read_b128 %6, %9 offset:2048\n\t"
+      "ds_read_b128 %7, %9 offset:3072\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(r0), "=&v"(r1), "=&v"(r2), "=&v"(r3), "=&v"(r4), "=&v"(r5), "=&v"(r6), "=&v"(r7)
+      : "v"(a0), "v"(a1));
+  f[0] = __builtin_bit_cast(bf8, r0);
+  f[1] = __builtin_bit_cast(bf8, r1);
+  f[2] = __builtin_bit_cast(bf8, r2);
+  f[3] = __builtin_bit_cast(bf8, r3);
+  f[4] = __builtin_bit_cast(bf8, r4);
+  f[5] = __builtin_bit_cast(bf8, r5);
+  f[6] = __builtin_bit_cast(bf8, r6);
+  f[7] = __builtin_bit_cast(bf8, r7);
+}
+
+__device__ __forceinline__ void lds_rd_4_8(unsigned a0, unsigned a1, bf8 (&f)[12]) {
+  u32x4 r0, r1, r2, r3, r4, r5, r6, r7, r8, r9, r10, r11;
   asm volatile(
       "ds_read_b128 %0, %12\n\t"
       "ds_read_b128 %1, %12 offset:1024\n\t"
       "ds_read_b128 %2, %12 offset:2048\n\t"
       "ds_read_b128 %3, %12 offset:3072\n\t"
-      "ds_read_b128 %4, %12 offset:4096\n\t"
-      "ds_read_b128 %5, %12 offset:5120\n\t"
-      "ds_read_b128 %6, %12 offset:6144\n\t"
-      "ds_read_b128 %7, %12 offset:7168\n\t"
-      "ds_read_b128 %8, %13\n\t"
-      "ds_read_b128 %9, %13 offset:1024\n\t"
-      "ds_read_b128 %10, %13 offset:2048\n\t"
-      "ds_read_b128 %11, %13 offset:3072\n\t"
+      "ds_read_b128 %4, %13\n\t"
+      "ds_read_b128 %5, %13 offset:1024\n\t"
+      "ds_read_b128 %6, %13 offset:2048\n\t"
+      "ds_read_b128 %7, %13 offset:3072\n\t"
+      "ds_read_b128 %8, %13 offset:4096\n\t"
+      "ds_read_b128 %9, %13 offset:5120\n\t"
+      "ds_read_b128 %10, %13 offset:6144\n\t"
+      "ds_read_b128 %11, %13 offset:7168\n\t"
       "s_waitcnt lgkmcnt(0)"
       : "=&v"(r0), "=&v"(r1), "=&v"(r2), "=&v"(r3), "=&v"(r4), "=&v"(r5), "=&v"(r6), "=&v"(r7), "=&v"(r8), "=&v"(r9), "=&v"(r10), "=&v"(r11)
-      : "v"(aaddr), "v"(baddr));
-  a[0] = __builtin_bit_cast(bf8, r0);
-  a[1] = __builtin_bit_cast(bf8, r1);
-  a[2] = __builtin_bit_cast(bf8, r2);
-  a[3] = __builtin_bit_cast(bf8, r3);
-  a[4] = __builtin_bit_cast(bf8, r4);
-  a[5] = __builtin_bit_cast(bf8, r5);
-  a[6] = __builtin_bit_cast(bf8, r6);
-  a[7] = __builtin_bit_cast(bf8, r7);
-  b[0] = __builtin_bit_cast(bf8, r8);
-  b[1] = __builtin_bit_cast(bf8, r9);
-  b[2] = __builtin_bit_cast(bf8, r10);
-  b[3] = __builtin_bit_cast(bf8, r11);
+      : "v"(a0), "v"(a1));
+  f[0] = __builtin_bit_cast(bf8, r0);
+  f[1] = __builtin_bit_cast(bf8, r1);
+  f[2] = __builtin_bit_cast(bf8, r2);
+  f[3] = __builtin_bit_cast(bf8, r3);
+  f[4] = __builtin_bit_cast(bf8, r4);
+  f[5] = __builtin_bit_cast(bf8, r5);
+  f[6] = __builtin_bit_cast(bf8, r6);
+  f[7] = __builtin_bit_cast(bf8, r7);
+  f[8] = __builtin_bit_cast(bf8, r8);
+  f[9] = __builtin_bit_cast(bf8, r9);
+  f[10] = __builtin_bit_cast(bf8, r10);
+  f[11] = __builtin_bit_cast(bf8, r11);
 }
 
-__device__ __forceinline__ void lds_frags_split(unsigned aaddr, unsigned baddr, bf8 (&a)[8], bf8 (&b)[4], bf8 (&al)[8], bf8 (&bl)[4]) {
-  u32x4 r0;
-  u32x4 r1;
-  u32x4 r2;
-  u32x4 r3;
-  u32x4 r4;
-  u32x4 r5;
-  u32x4 r6;
-  u32x4 r7;
-  u32x4 r8;
-  u32x4 r9;
-  u32x4 r10;
-  u32x4 r11;
-  u32x4 r12;
-  u32x4 r13;
-  u32x4 r14;
-  u32x4 r15;
-  u32x4 r16;
-  u32x4 r17;
-  u32x4 r18;
-  u32x4 r19;
-  u32x4 r20;
-  u32x4 r21;
-  u32x4 r22;
-  u32x4 r23;
+__device__ __forceinline__ void lds_rd_4_4_4_4(unsigned a0, unsigned a1, unsigned a2, unsigned a3, bf8 (&f)[16]) {
+  u32x4 r0, r1, r2, r3, r4, r5, r6, r7, r8, r9, r10, r11, r12, r13, r14, r15;
+  asm volatile(
+      "ds_read_b128 %0, %16\n\t"
+      "ds_read_b128 %1, %16 offset:1024\n\t"
+      "ds_read_b128 %2, %16 offset:2048\n\t"
+      "ds_read_b128 %3, %16 offset:3072\n\t"
+      "ds_read_b128 %4, %17\n\t"
+      "ds_read_b128 %5, %17 offset:1024\n\t"
+      "ds_read_b128 %6, %17 offset:2048\n\t"
+      "ds_read_b128 %7, %17 offset:3072\n\t"
+      "ds_read_b128 %8, %18\n\t"
+      "ds_read_b128 %9, %18 offset:1024\n\t"
+      "ds_read_b128 %10, %18 offset:2048\n\t"
+      "ds_read_b128 %11, %18 offset:3072\n\t"
+      "ds_read_b128 %12, %19\n\t"
+      "ds_read_b128 %13, %19 offset:1024\n\t"
+      "ds_read_b128 %14, %19 offset:2048\n\t"
+      "ds_read_b128 %15, %19 offset:3072\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(r0), "=&v"(r1), "=&v"(r2), "=&v"(r3), "=&v"(r4), "=&v"(r5), "=&v"(r6), "=&v"(r7), "=&v"(r8), "=&v"(r9), "=&v"(r10), "=&v"(r11), "=&v"(r12), "=&v"(r13), "=&v"(r14), "=&v"(r15)
+      : "v"(a0), "v"(a1), "v"(a2), "v"(a3));
+  f[0] = __builtin_bit_cast(bf8, r0);
+  f[1] = __builtin_bit_cast(bf8, r1);
+  f[2] = __builtin_bit_cast(bf8, r2);
+  f[3] = __builtin_bit_cast(bf8, r3);
+  f[4] = __builtin_bit_cast(bf8, r4);
+  f[5] = __builtin_bit_cast(bf8, r5);
+  f[6] = __builtin_bit_cast(bf8, r6);
+  f[7] = __builtin_bit_cast(bf8, r7);
+  f[8] = __builtin_bit_cast(bf8, r8);
+  f[9] = __builtin_bit_cast(bf8, r9);
+  f[10] = __builtin_bit_cast(bf8, r10);
+  f[11] = __builtin_bit_cast(bf8, r11);
+  f[12] = __builtin_bit_cast(bf8, r12);
+  f[13] = __builtin_bit_cast(bf8, r13);
+  f[14] = __builtin_bit_cast(bf8, r14);
+  f[15] = __builtin_bit_cast(bf8, r15);
+}
+
+__device__ __forceinline__ void lds_rd_4_4_8_8(unsigned a0, unsigned a1, unsigned a2, unsigned a3, bf8 (&f)[24]) {
+  u32x4 r0, r1, r2, r3, r4, r5, r6, r7, r8, r9, r10, r11, r12, r13, r14, r15, r16, r17, r18, r19, r20, r21, r22, r23;
   asm volatile(
       "ds_read_b128 %0, %24\n\t"
       "ds_read_b128 %1, %24 offset:1024\n\t"
       "ds_read_b128 %2, %24 offset:2048\n\t"
       "ds_read_b128 %3, %24 offset:3072\n\t"
-      "ds_read_b128 %4, %24 offset:4096\n\t"
-      "ds_read_b128 %5, %24 offset:5120\n\t"
-      "ds_read_b128 %6, %24 offset:6144\n\t"
-      "ds_read_b128 %7, %24 offset:7168\n\t"
-      "ds_read_b128 %8, %25\n\t"
-      "ds_read_b128 %9, %25 offset:1024\n\t"
-      "ds_read_b128 %10, %25 offset:2048\n\t"
-      "ds_read_b128 %11, %25 offset:3072\n\t"
-      "ds_read_b128 %12, %24 offset:16384\n\t"
-      "ds_read_b128 %13, %24 offset:17408\n\t"
-      "ds_read_b128 %14, %24 offset:18432\n\t"
-      "ds_read_b128 %15, %24 offset:19456\n\t"
-      "ds_read_b128 %16, %24 offset:20480\n\t"
-      "ds_read_b128 %17, %24 offset:21504\n\t"
-      "ds_read_b128 %18, %24 offset:22528\n\t"
-      "ds_read_b128 %19, %24 offset:23552\n\t"
-      "ds_read_b128 %20, %25 offset:8192\n\t"
-      "ds_read_b128 %21, %25 offset:9216\n\t"
-      "ds_read_b128 %22, %25 offset:10240\n\t"
-      "ds_read_b128 %23, %25 offset:11264\n\t"
+      "ds_read_b128 %4, %25\n\t"
+      "ds_read_b128 %5, %25 offset:1024\n\t"
+      "ds_read_b128 %6, %25 offset:2048\n\t"
+      "ds_read_b128 %7, %25 offset:3072\n\t"
+      "ds_read_b128 %8, %26\n\t"
+      "ds_read_b128 %9, %26 offset:1024\n\t"
+      "ds_read_b128 %10, %26 offset:2048\n\t"
+      "ds_read_b128 %11, %26 offset:3072\n\t"
+      "ds_read_b128 %12, %26 offset:4096\n\t"
+      "ds_read_b128 %13, %26 offset:5120\n\t"
+      "ds_read_b128 %14, %26 offset:6144\n\t"
+      "ds_read_b128 %15, %26 offset:7168\n\t"
+      "ds_read_b128 %16, %27\n\t"
+      "ds_read_b128 %17, %27 offset:1024\n\t"
+      "ds_read_b128 %18, %27 offset:2048\n\t"
+      "ds_read_b128 %19, %27 offset:3072\n\t"
+      "ds_read_b128 %20, %27 offset:4096\n\t"
+      "ds_read_b128 %21, %27 offset:5120\n\t"
+      "ds_read_b128 %22, %27 offset:6144\n\t"
+      "ds_read_b128 %23, %27 offset:7168\n\t"
       "s_waitcnt lgkmcnt(0)"
       : "=&v"(r0), "=&v"(r1), "=&v"(r2), "=&v"(r3), "=&v"(r4), "=&v"(r5), "=&v"(r6), "=&v"(r7), "=&v"(r8), "=&v"(r9), "=&v"(r10), "=&v"(r11), "=&v"(r12), "=&v"(r13), "=&v"(r14), "=&v"(r15), "=&v"(r16), "=&v"(r17), "=&v"(r18), "=&v"(r19), "=&v"(r20), "=&v"(r21), "=&v"(r22), "=&v"(r23)
-      : "v"(aaddr), "v"(baddr));
-  a[0] = __builtin_bit_cast(bf8, r0);
-  a[1] = __builtin_bit_cast(bf8, r1);
-  a[2] = __builtin_bit_cast(bf8, r2);
-  a[3] = __builtin_bit_cast(bf8, r3);
-  a[4] = __builtin_bit_cast(bf8, r4);
-  a[5] = __builtin_bit_cast(bf8, r5);
-  a[6] = __builtin_bit_cast(bf8, r6);
-  a[7] = __builtin_bit_cast(bf8, r7);
-  b[0] = __builtin_bit_cast(bf8, r8);
-  b[1] = __builtin_bit_cast(bf8, r9);
-  b[2] = __builtin_bit_cast(bf8, r10);
-  b[3] = __builtin_bit_cast(bf8, r11);
-  al[0] = __builtin_bit_cast(bf8, r12);
-  al[1] = __builtin_bit_cast(bf8, r13);
-  al[2] = __builtin_bit_cast(bf8, r14);
-  al[3] = __builtin_bit_cast(bf8, r15);
-  al[4] = __builtin_bit_cast(bf8, r16);
-  al[5] = __builtin_bit_cast(bf8, r17);
-  al[6] = __builtin_bit_cast(bf8, r18);
-  al[7] = __builtin_bit_cast(bf8, r19);
-  bl[0] = __builtin_bit_cast(bf8, r20);
-  bl[1] = __builtin_bit_cast(bf8, r21);
-  bl[2] = __builtin_bit_cast(bf8, r22);
-  bl[3] = __builtin_bit_cast(bf8, r23);
+      : "v"(a0), "v"(a1), "v"(a2), "v"(a3));
+  f[0] = __builtin_bit_cast(bf8, r0);
+  f[1] = __builtin_bit_cast(bf8, r1);
+  f[2] = __builtin_bit_cast(bf8, r2);
+  f[3] = __builtin_bit_cast(bf8, r3);
+  f[4] = __builtin_bit_cast(bf8, r4);
+  f[5] = __builtin_bit_cast(bf8, r5);
+  f[6] = __builtin_bit_cast(bf8, r6);
+  f[7] = __builtin_bit_cast(bf8, r7);
+  f[8] = __builtin_bit_cast(bf8, r8);
+  f[9] = __builtin_bit_cast(bf8, r9);
+  f[10] = __builtin_bit_cast(bf8, r10);
+  f[11] = __builtin_bit_cast(bf8, r11);
+  f[12] = __builtin_bit_cast(bf8, r12);
+  f[13] = __builtin_bit_cast(bf8, r13);
+  f[14] = __builtin_bit_cast(bf8, r14);
+  f[15] = __builtin_bit_cast(bf8, r15);
+  f[16] = __builtin_bit_cast(bf8, r16);
+  f[17] = __builtin_bit_cast(bf8, r17);
+  f[18] = __builtin_bit_cast(bf8, r18);
+  f[19] = __builtin_bit_cast(bf8, r19);
+  f[20] = __builtin_bit_cast(bf8, r20);
+  f[21] = __builtin_bit_cast(bf8, r21);
+  f[22] = __builtin_bit_cast(bf8, r22);
+  f[23] = __builtin_bit_cast(bf8, r23);
+}
+
+__device__ __forceinline__ unsigned lds_u32(const unsigned char* p) {
+  return (unsigned)(uintptr_t)(const __attribute__((address_space(3))) unsigned char*)p;
 }
 
 template <int N>
@@ -610,26 +659,50 @@ __device__ __forceinline__ void wait_vmcnt() {
   __builtin_amdgcn_s_waitcnt((N & 15) | (7 << 4) | (15 << 8) | (((N >> 4) & 3) << 14));
 }
 
-template <typename TOut, bool SPLIT, int NSTAGE>
+template <int TN, bool SPLIT>
+__device__ __forceinline__ void mfma_rows(f4 (&acc)[8][TN], const bf8* ah, const bf8* al, const bf8* bh,
+                                          const bf8* bl, int i0) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+      acc[i0 + i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i0 + i][j], 0, 0, 0);
+  if constexpr (SPLIT) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+        acc[i0 + i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i0 + i][j], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = 0; j < TN; ++j)
+        acc[i0 + i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i0 + i][j], 0, 0, 0);
+  }
+}
+
+// UNIT: every segment has scale 1 (no rescaling code in the k-loop).
+template <typename TOut, bool SPLIT, int NSTAGE, int BN, bool UNIT>
 __global__ __launch_bounds__(256, 1) void gemm_bf16_stream_kernel(
     const GTask* __restrict__ tasks, int ntasks, const GSeg* __restrict__ segs, int tiles_per_vec, int total_tiles,
     const __hip_bfloat16* __restrict__ A, int64_t a_vs, int64_t a_plane, const __hip_bfloat16* __restrict__ B,
     int64_t b_vs, int64_t b_plane, TOut* __restrict__ C, int64_t c_vs, int64_t c_plane, int c_rowmajor) {
-  constexpr int BM = 256, BN = 128, BK = 32;
+  constexpr int BM = 256, BK = 32;
   constexpr int NP = SPLIT ? 2 : 1;
-  constexpr int WM = 128, WN = 64, TM = WM / 16, TN = WN / 16;
+  constexpr int WM = 128, WN = BN / 2, TM = WM / 16, TN = WN / 16;
   constexpr int ROWB = BK * 2;                        // bytes per LDS row (one plane)
   constexpr int A_BYTES = BM * ROWB, B_BYTES = BN * ROWB;
   constexpr int STAGE_BYTES = NP * (A_BYTES + B_BYTES);
   constexpr int LA = BM / 64, LB = BN / 64;           // load instructions per wave and plane
   constexpr int LOADS = NP * (LA + LB);               // per wave per stage
+  static_assert(TM == 8 && (TN == 4 || TN == 8), "wave tile 128 x 64 or 128 x 128");
+  static_assert(NSTAGE * STAGE_BYTES <= 160 * 1024, "LDS budget");
   __shared__ __attribute__((aligned(1024))) unsigned char smem[NSTAGE * STAGE_BYTES];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
   const int l16 = lane & 15, lq = lane >> 4;
   const unsigned short* Au = reinterpret_cast<const unsigned short*>(A);
   const unsigned short* Bu = reinterpret_cast<const unsigned short*>(B);
-  static_assert(TM == 8 && TN == 4 && A_BYTES == 16384 && B_BYTES == 8192, "fragment reader layout");
   // byte offsets of fragment 0 inside a plane; the swizzle (r >> 2) & 3 of row
   // r = base + 16 i + l16 does not depend on i
   const int aoff0 = (wm * WM + l16) * ROWB + ((lq ^ ((l16 >> 2) & 3)) << 4);
@@ -698,13 +771,13 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16_stream_kernel(
     while (cc.si < send) {
       // wait for this step's slab: allow the loads of the later issued steps to stay in flight
       const int ahead = issued - 1;
-      if (ahead >= 2) wait_vmcnt<(2 * LOADS < 63 ? 2 * LOADS : 63)>();
+      if (NSTAGE > 2 && ahead >= 2) wait_vmcnt<(2 * LOADS < 63 ? 2 * LOADS : 63)>();
       else if (ahead == 1) wait_vmcnt<LOADS>();
       else wait_vmcnt<0>();
       // Every wave's slab is in LDS (each wave waited for its own loads above) and
       // every wave is done with the previous step (its fragment reads completed
-      // before its MFMAs).  A bare barrier: __syncthreads' release fence would
-      // also wait for the loads still in flight for later steps.
+      // inside the reader's asm).  A bare barrier: __syncthreads' release fence
+      // would also wait for the loads still in flight for later steps.
       asm volatile("s_barrier" ::: "memory");
       // refill the buffer freed by the previous step
       if (cl.si < send) {
@@ -712,8 +785,8 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16_stream_kernel(
         cur_next(cl, segs, send, BK);
         ++issued;
       }
-      const float a = (float)cc.alpha;
-      if (a != ascale && a != 0.f) {
+      const float a = UNIT ? 1.f : (float)cc.alpha;
+      if (!UNIT && a != ascale && a != 0.f) {
         const float f = ascale / a;
 #pragma unroll
         for (int i = 0; i < TM; ++i)
@@ -721,33 +794,52 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16_stream_kernel(
           for (int j = 0; j < TN; ++j) acc[i][j] *= f;
         ascale = a;
       }
-      if (a != 0.f) {
+      if (UNIT || a != 0.f) {
         const unsigned char* base = smem + stage * STAGE_BYTES;
-        const unsigned aaddr = (unsigned)(uintptr_t)(const __attribute__((address_space(3))) unsigned char*)(base + aoff0);
-        const unsigned baddr =
-            (unsigned)(uintptr_t)(const __attribute__((address_space(3))) unsigned char*)(base + NP * A_BYTES + boff0);
-        bf8 bh[TN], ah[TM];
-        if (SPLIT) {
-          bf8 bl[TN], al[TM];
-          lds_frags_split(aaddr, baddr, ah, bh, al, bl);
+        const unsigned aaddr = lds_u32(base + aoff0);
+        const unsigned baddr = lds_u32(base + NP * A_BYTES + boff0);
+        // first half: A fragments 0..3 (hi, lo) and every B fragment (hi, lo)
+        bf8 bh[TN], bl[TN], ah[4], al[4];
+        if constexpr (!SPLIT) {
+          bf8 f[4 + TN];
+          if constexpr (TN == 4) lds_rd_4_4(aaddr, baddr, f);
+          else lds_rd_4_8(aaddr, baddr, f);
 #pragma unroll
-          for (int i = 0; i < TM; ++i)
+          for (int i = 0; i < 4; ++i) ah[i] = f[i];
 #pragma unroll
-            for (int j = 0; j < TN; ++j)
-              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
-#pragma unroll
-          for (int i = 0; i < TM; ++i)
-#pragma unroll
-            for (int j = 0; j < TN; ++j)
-              acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+          for (int j = 0; j < TN; ++j) bh[j] = f[4 + j];
         } else {
-          lds_frags_plain(aaddr, baddr, ah, bh);
+          bf8 f[8 + 2 * TN];
+          if constexpr (TN == 4) lds_rd_4_4_4_4(aaddr, aaddr + A_BYTES, baddr, baddr + B_BYTES, f);
+          else lds_rd_4_4_8_8(aaddr, aaddr + A_BYTES, baddr, baddr + B_BYTES, f);
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            ah[i] = f[i];
+            al[i] = f[4 + i];
+          }
+#pragma unroll
+          for (int j = 0; j < TN; ++j) {
+            bh[j] = f[8 + j];
+            bl[j] = f[8 + TN + j];
+          }
         }
+        mfma_rows<TN, SPLIT>(acc, ah, al, bh, bl, 0);
+        // second half: A fragments 4..7; these reads overlap the MFMAs above
+        if constexpr (!SPLIT) {
+          bf8 f[4];
+          lds_rd_4(aaddr + 4096, f);
 #pragma unroll
-        for (int i = 0; i < TM; ++i)
+          for (int i = 0; i < 4; ++i) ah[i] = f[i];
+        } else {
+          bf8 f[8];
+          lds_rd_4_4(aaddr + 4096, aaddr + A_BYTES + 4096, f);
 #pragma unroll
-          for (int j = 0; j < TN; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+          for (int i = 0; i < 4; ++i) {
+            ah[i] = f[i];
+            al[i] = f[4 + i];
+          }
+        }
+        mfma_rows<TN, SPLIT>(acc, ah, al, bh, bl, 4);
       }
       cur_next(cc, segs, send, BK);
       --issued;
@@ -766,7 +858,7 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16_stream_kernel(
           const int row = row0 + wm * WM + i * 16 + lq * 4 + r;
           if (row < m && col < n) {
             const int64_t idx = c_rowmajor ? ((int64_t)row * ldr + col) : (row + (int64_t)col * m);
-            store_out<TOut, SPLIT>(Cb, idx, c_plane, acc[i][j][r] * ascale);
+            store_out<TOut, SPLIT>(Cb, idx, c_plane, UNIT ? acc[i][j][r] : acc[i][j][r] * ascale);
           }
         }
       }
@@ -1113,7 +1205,7 @@ __global__ __launch_bounds__(256) void repack_kernel(const Repack* __restrict__ 
       if (e < len) {
         const int64_t i = e % tk.rows, j = e / tk.rows;
         const int64_t di = tk.transpose ? (j + i * tk.dst_ld) : (i + j * tk.dst_ld);
-        d[di] = cvt<TO>(s[e]);
+        d[di] = cvt<TO>(tk.scale * s[e]);
       }
     }
   }
@@ -1138,7 +1230,7 @@ __global__ __launch_bounds__(256) void repack_split_kernel(const Repack* __restr
       if (e < len) {
         const int64_t i = e % tk.rows, j = e / tk.rows;
         const int64_t di = tk.transpose ? (j + i * tk.dst_ld) : (i + j * tk.dst_ld);
-        const double x = sp[e];
+        const double x = tk.scale * sp[e];
         const __hip_bfloat16 h = __float2bfloat16((float)x);
         d[di] = h;
         d[di + plane] = __float2bfloat16((float)(x - (double)__bfloat162float(h)));
@@ -1188,7 +1280,7 @@ static hipError_t run16(int tr, const GTask* tasks, int ntasks, const GSeg* segs
   return hipGetLastError();
 }
 
-// BM == 256 selects the streaming large-tile kernel (BN must be 128 then).
+// BM == 256 selects the streaming large-tile kernel (BN 128 or 256).
 template <typename TOut, int BM, int BN, bool SPLIT>
 static hipError_t runbf(const GTask* tasks, int ntasks, const GSeg* segs, int tpv, int nvec, const void* A,
                         int64_t a_vs, int64_t a_pl, const void* B, int64_t b_vs, int64_t b_pl, void* C, int64_t c_vs,
@@ -1199,8 +1291,9 @@ static hipError_t runbf(const GTask* tasks, int ntasks, const GSeg* segs, int tp
   if (grid > total) grid = total;
   if (grid >= 8) grid &= ~7;
   if constexpr (BM == 256) {
-    static_assert(BN == 128, "large tile is 256x128");
-    gemm_bf16_stream_kernel<TOut, SPLIT, SPLIT ? 3 : 4><<<grid, 256, 0, st>>>(
+    // stages that fit the 160 KB LDS: 256x128 -> 4 (3 split), 256x256 -> 4 (2 split)
+    constexpr int NST = BN == 128 ? (SPLIT ? 3 : 4) : (SPLIT ? 2 : 4);
+    gemm_bf16_stream_kernel<TOut, SPLIT, NST, BN, BN == 256><<<grid, 256, 0, st>>>(
         tasks, ntasks, segs, tpv, total, (const __hip_bfloat16*)A, a_vs, a_pl, (const __hip_bfloat16*)B, b_vs, b_pl,
         (TOut*)C, c_vs, c_pl, c_rowmajor);
   } else {
@@ -1230,6 +1323,9 @@ hipError_t launch_gemm(int prec_in, int out_kind, int tr, int tile_class, const 
   // row-major (hi/lo planes for BF16X3), 1 -> double column-major.
 #define MPSA_BF_DISPATCH(TOUT, SPLIT_, APL, BPL, CPL, ROWM)                                                           \
   do {                                                                                                               \
+    if (tile_class == 3)                                                                                             \
+      return runbf<TOUT, 256, 256, SPLIT_>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, APL, B, b_vs, BPL, C, \
+                                           c_vs, CPL, ROWM, st);                                                     \
     if (tile_class == 0)                                                                                             \
       return runbf<TOUT, 256, 128, SPLIT_>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, APL, B, b_vs, BPL, C, \
                                            c_vs, CPL, ROWM, st);                                                     \

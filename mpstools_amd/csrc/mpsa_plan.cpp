@@ -214,10 +214,11 @@ static int64_t pad8(int64_t x) { return (x + 31) & ~int64_t(31); }
 
 // Tile class: the largest workgroup tile that wastes at most 20% of its
 // MFMA work on padding; the smallest tile otherwise.
-static int pick_class(const std::vector<GTask>& t, int prec) {
+static int pick_class(const std::vector<GTask>& t, int prec, bool unit_alpha) {
   int best = 1;
   int best_area = 0;
   for (int cls = 0; cls < num_tile_classes(prec); ++cls) {
+    if (cls == 3 && !unit_alpha) continue;  // the 256x256 kernel has no segment scales
     int bm, bn;
     tile_dims(prec, cls, &bm, &bn);
     double useful = 0, padded = 0;
@@ -247,7 +248,9 @@ static void finalize_stage(Stage& s, int prec) {
     }
     s.segs = segs;
   }
-  s.tile_class = pick_class(s.tasks, prec);
+  bool unit = true;
+  for (auto& g : s.segs) unit = unit && g.alpha == 1.0;
+  s.tile_class = pick_class(s.tasks, prec, unit);
   int BM, BN;
   tile_dims(prec, s.tile_class, &BM, &BN);
   // longest tiles first (static round-robin over the persistent grid)
@@ -553,6 +556,10 @@ static int64_t write_keys(const std::vector<BKey>& keys, const std::map<BKey, st
 static void build_plan(mpsa_plan* P, Built& b, const Tab& Ptab, const Tab& Qtab, const Tab& Utab, const Tab& Ytab,
                        int kind, int direction) {
   const bool bf = P->prec == BF16 || P->prec == BF16X3;
+  // A single-term W mix is elided: stage 2 reads the T1 product directly and
+  // the coefficient rides on the segment (FP64/FP32), or, in BF16 plans, on a
+  // scaled copy of the R block made once per environment binding, so that
+  // every BF16 segment has unit scale.
   // workspace layout
   int64_t off = 0;
   for (auto& pr : b.prods) {
@@ -571,6 +578,7 @@ static void build_plan(mpsa_plan* P, Built& b, const Tab& Ptab, const Tab& Qtab,
 
   // bf16: converted operands with padded, k-contiguous layouts
   std::vector<int64_t> Loff(Ptab.n), Roff(Utab.n), xoff(Qtab.n);
+  std::map<std::pair<int, double>, int64_t> Rscaled;  // (R block, coefficient) -> scaled bf16 copy
   if (bf) {
     int64_t o = 0;
     for (int i = 0; i < (int)Ptab.n; ++i) {  // L^T: rows*pad8(cols) (element (i,k) at i*pad8(K)+k)
@@ -586,6 +594,20 @@ static void build_plan(mpsa_plan* P, Built& b, const Tab& Ptab, const Tab& Qtab,
       P->rpR.push_back(Repack{Utab.off[(size_t)i], o, (int32_t)Utab.rows[(size_t)i], (int32_t)Utab.cols[(size_t)i],
                               (int32_t)pad8(Utab.cols[(size_t)i]), 1, 0, 0});
       o = align_up(o + Utab.rows[(size_t)i] * pad8(Utab.cols[(size_t)i]), 64);
+    }
+    if (kind != 2) {  // scaled R copies for the coefficients of single-term slots
+      for (auto& sl : b.slots) {
+        if (sl.terms.size() != 1 || sl.terms[0].second == 1.0) continue;
+        const auto key = std::make_pair((int)sl.iu, sl.terms[0].second);
+        if (Rscaled.count(key)) continue;
+        const int i = sl.iu;
+        Rscaled[key] = o;
+        Repack r{Utab.off[(size_t)i], o, (int32_t)Utab.rows[(size_t)i], (int32_t)Utab.cols[(size_t)i],
+                 (int32_t)pad8(Utab.cols[(size_t)i]), 1, 0, 0};
+        r.scale = sl.terms[0].second;
+        P->rpR.push_back(r);
+        o = align_up(o + Utab.rows[(size_t)i] * pad8(Utab.cols[(size_t)i]), 64);
+      }
     }
     P->Rc_elems = std::max<int64_t>(o, 64);
     o = 0;
@@ -684,9 +706,14 @@ static void build_plan(mpsa_plan* P, Built& b, const Tab& Ptab, const Tab& Qtab,
         toff = sl.ws_off;
         alpha = 1.0;
       }
+      int64_t roff = Roff[(size_t)sl.iu];
+      if (bf && kind != 2 && alpha != 1.0) {  // the coefficient lives in a scaled R copy
+        roff = Rscaled.at(std::make_pair((int)sl.iu, alpha));
+        alpha = 1.0;
+      }
       if (kind != 2) {  // y = T * R^T: A = T (m x n_j), B = R (y.cols x n_j), k = n_j
         s.a_off = toff;
-        s.b_off = Roff[(size_t)sl.iu];
+        s.b_off = roff;
         s.k = (int32_t)sl.n;
         s.lda = (int32_t)(bf ? pad8(sl.n) : std::max<int64_t>(1, sl.m));
         s.ldb = (int32_t)(bf ? pad8(Utab.cols[(size_t)sl.iu]) : std::max<int64_t>(1, Utab.rows[(size_t)sl.iu]));
