@@ -465,8 +465,9 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_kernel(const GTask* __restri
 // dimensions padded with zeros to a multiple of 32, so every k-step is a full
 // 32-wide slab.  Rows past m (columns past n) are loaded from a clamped, valid
 // address and their garbage results are never stored.  LDS rows are 64 bytes
-// with the 16-byte chunk index XOR-ed by (row >> 2) & 3, which keeps every
-// quarter-wave's fragment reads on distinct banks.  Segment scales are applied
+// with the 16-byte chunk index XOR-ed by (row >> 1) & 3: a ds_read_b128 serves
+// 8 lanes (128 B) per cycle, and 8 consecutive rows at one logical chunk then
+// land on 8 distinct 16-byte bank groups (rows 2 apart are 128 B apart).  Segment scales are applied
 // by rescaling the accumulators when the scale changes.
 // ---------------------------------------------------------------------------
 typedef __attribute__((address_space(3))) void* lds_vptr;
@@ -703,10 +704,10 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16_stream_kernel(
   const int l16 = lane & 15, lq = lane >> 4;
   const unsigned short* Au = reinterpret_cast<const unsigned short*>(A);
   const unsigned short* Bu = reinterpret_cast<const unsigned short*>(B);
-  // byte offsets of fragment 0 inside a plane; the swizzle (r >> 2) & 3 of row
+  // byte offsets of fragment 0 inside a plane; the swizzle (r >> 1) & 3 of row
   // r = base + 16 i + l16 does not depend on i
-  const int aoff0 = (wm * WM + l16) * ROWB + ((lq ^ ((l16 >> 2) & 3)) << 4);
-  const int boff0 = (wn * WN + l16) * ROWB + ((lq ^ ((l16 >> 2) & 3)) << 4);
+  const int aoff0 = (wm * WM + l16) * ROWB + ((lq ^ ((l16 >> 1) & 3)) << 4);
+  const int boff0 = (wn * WN + l16) * ROWB + ((lq ^ ((l16 >> 1) & 3)) << 4);
   // loader lane mapping: instruction q of this wave covers rows 16*(wave*L + q) + lane/4
   const int lrow = lane >> 2, lslot = lane & 3;
 
@@ -736,7 +737,7 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16_stream_kernel(
           const int rb = 16 * (wave * LA + q);
           const int r = rb + lrow;
           const int gi = row0 + r < m ? row0 + r : m - 1;
-          const int chunk = lslot ^ ((r >> 2) & 3);
+          const int chunk = lslot ^ ((r >> 1) & 3);
           const unsigned short* src = Ab + p * a_plane + (int64_t)gi * c.lda + c.kk + chunk * 8;
           __builtin_amdgcn_global_load_lds((gbl_vptr)src, (lds_vptr)(base + p * A_BYTES + rb * ROWB), 16, 0, 0);
         }
@@ -745,7 +746,7 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16_stream_kernel(
           const int rb = 16 * (wave * LB + q);
           const int r = rb + lrow;
           const int gj = col0 + r < n ? col0 + r : n - 1;
-          const int chunk = lslot ^ ((r >> 2) & 3);
+          const int chunk = lslot ^ ((r >> 1) & 3);
           const unsigned short* src = Bb + p * b_plane + (int64_t)gj * c.ldb + c.kk + chunk * 8;
           __builtin_amdgcn_global_load_lds((gbl_vptr)src,
                                            (lds_vptr)(base + NP * A_BYTES + p * B_BYTES + rb * ROWB), 16, 0, 0);
@@ -932,32 +933,52 @@ __global__ __launch_bounds__(256) void mix_kernel<double, double>(const MixTask*
 }
 
 // W mix on hi/lo bf16 planes: dst = split(sum_t coef_t * (hi_t + lo_t)).
-__global__ __launch_bounds__(256) void mix_split_kernel(const MixTask* __restrict__ tasks, int ntasks,
-                                                        const MixTerm* __restrict__ terms, int chunks_per_vec,
-                                                        int total_chunks, const __hip_bfloat16* __restrict__ src,
-                                                        __hip_bfloat16* __restrict__ dst, int64_t vs, int64_t plane) {
+// BF16 mix, 8 elements (16 bytes) per thread and plane; a 1024-element chunk
+// per 128-thread block.  BF16 mix lengths are m * pad32(n) and every offset a
+// multiple of 64 elements, so all accesses are aligned 16-byte vectors.
+// SPLIT: sources and destination are hi/lo plane pairs (x = hi + lo).
+typedef unsigned short u16x8v __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ float bf_bits_to_f(unsigned short b) { return __uint_as_float((unsigned)b << 16); }
+template <bool SPLIT>
+__global__ __launch_bounds__(128) void mix_bf16_kernel(const MixTask* __restrict__ tasks, int ntasks,
+                                                       const MixTerm* __restrict__ terms, int chunks_per_vec,
+                                                       int total_chunks, const __hip_bfloat16* __restrict__ src,
+                                                       __hip_bfloat16* __restrict__ dst, int64_t vs, int64_t plane) {
+  const unsigned short* su = reinterpret_cast<const unsigned short*>(src);
+  unsigned short* du = reinterpret_cast<unsigned short*>(dst);
   for (int g = blockIdx.x; g < total_chunks; g += gridDim.x) {
     const int v = g / chunks_per_vec;
     const int c = g - v * chunks_per_vec;
     const MixTask tk = tasks[find_task(tasks, ntasks, c, &MixTask::chunk0)];
-    const int64_t base = (int64_t)(c - tk.chunk0) * 1024;
-    const __hip_bfloat16* sv = src + (int64_t)v * vs;
-    __hip_bfloat16* dv = dst + (int64_t)v * vs + tk.dst_off;
+    const int64_t e = (int64_t)(c - tk.chunk0) * 1024 + threadIdx.x * 8;
+    if (e >= tk.len) continue;
+    const unsigned short* sv = su + (int64_t)v * vs;
+    float acc[8];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int64_t e = base + threadIdx.x + r * 256;
-      if (e < tk.len) {
-        float acc = 0.f;
-        for (int q = 0; q < tk.nt; ++q) {
-          const MixTerm tm = terms[tk.t0 + q];
-          const float x = __bfloat162float(sv[tm.src_off + e]) + __bfloat162float(sv[tm.src_off + e + plane]);
-          acc += (float)tm.coef * x;
-        }
-        const __hip_bfloat16 h = __float2bfloat16(acc);
-        dv[e] = h;
-        dv[e + plane] = __float2bfloat16(acc - __bfloat162float(h));
+    for (int k = 0; k < 8; ++k) acc[k] = 0.f;
+    for (int q = 0; q < tk.nt; ++q) {
+      const MixTerm tm = terms[tk.t0 + q];
+      const float cf = (float)tm.coef;
+      const u16x8v h = *reinterpret_cast<const u16x8v*>(sv + tm.src_off + e);
+      if (SPLIT) {
+        const u16x8v l = *reinterpret_cast<const u16x8v*>(sv + tm.src_off + e + plane);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc[k] += cf * (bf_bits_to_f(h[k]) + bf_bits_to_f(l[k]));
+      } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc[k] += cf * bf_bits_to_f(h[k]);
       }
     }
+    u16x8v oh, ol;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const __hip_bfloat16 hb = __float2bfloat16(acc[k]);
+      oh[k] = __builtin_bit_cast(unsigned short, hb);
+      if (SPLIT) ol[k] = __builtin_bit_cast(unsigned short, __float2bfloat16(acc[k] - __bfloat162float(hb)));
+    }
+    unsigned short* dv = du + (int64_t)v * vs + tk.dst_off + e;
+    *reinterpret_cast<u16x8v*>(dv) = oh;
+    if (SPLIT) *reinterpret_cast<u16x8v*>(dv + plane) = ol;
   }
 }
 
@@ -1349,17 +1370,19 @@ hipError_t launch_mix(int prec, const MixTask* tasks, int ntasks, const MixTerm*
   const int total = chunks_per_vec * nvec;
   if (total <= 0 || ntasks <= 0) return hipSuccess;
   int grid = total < 8192 ? total : 8192;
-  if (prec == BF16X3) {
-    mix_split_kernel<<<grid, 256, 0, st>>>(tasks, ntasks, terms, chunks_per_vec, total, (const __hip_bfloat16*)src,
-                                           (__hip_bfloat16*)dst, vs, plane);
+  if (prec == BF16X3 || prec == BF16) {
+    if (prec == BF16X3)
+      mix_bf16_kernel<true><<<grid, 128, 0, st>>>(tasks, ntasks, terms, chunks_per_vec, total,
+                                                  (const __hip_bfloat16*)src, (__hip_bfloat16*)dst, vs, plane);
+    else
+      mix_bf16_kernel<false><<<grid, 128, 0, st>>>(tasks, ntasks, terms, chunks_per_vec, total,
+                                                   (const __hip_bfloat16*)src, (__hip_bfloat16*)dst, vs, plane);
     return hipGetLastError();
   }
   if (prec == F64)
     mix_kernel<double, double><<<grid, 256, 0, st>>>(tasks, ntasks, terms, chunks_per_vec, total, (const double*)src, (double*)dst, vs);
   else if (prec == F32)
     mix_kernel<float, float><<<grid, 256, 0, st>>>(tasks, ntasks, terms, chunks_per_vec, total, (const float*)src, (float*)dst, vs);
-  else
-    mix_kernel<__hip_bfloat16, float><<<grid, 256, 0, st>>>(tasks, ntasks, terms, chunks_per_vec, total, (const __hip_bfloat16*)src, (__hip_bfloat16*)dst, vs);
   return hipGetLastError();
 }
 
