@@ -460,8 +460,8 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_kernel(const GTask* __restri
 // ---------------------------------------------------------------------------
 // Streaming BF16 grouped GEMM: operands go straight from global memory into
 // LDS with global_load_lds_dwordx4 (no register staging), NSTAGE k-steps in
-// flight, one barrier per k-step.  Workgroup tile 256 x BN (BN = 128 or 256),
-// four waves of 128 x BN/2.  Requirements (met by the plan's bf16 layouts): k
+// flight, one barrier per k-step.  Workgroup tile 256 x BN: BN = 128 with four
+// waves of 128 x 64, BN = 256 with eight waves of 128 x 64.  Requirements (met by the plan's bf16 layouts): k
 // dimensions padded with zeros to a multiple of 32, so every k-step is a full
 // 32-wide slab.  Rows past m (columns past n) are loaded from a clamped, valid
 // address and their garbage results are never stored.  LDS rows are 64 bytes
@@ -683,18 +683,20 @@ __device__ __forceinline__ void mfma_rows(f4 (&acc)[8][TN], const bf8* ah, const
 }
 
 // UNIT: every segment has scale 1 (no rescaling code in the k-loop).
-template <typename TOut, bool SPLIT, int NSTAGE, int BN, bool UNIT>
-__global__ __launch_bounds__(256, 1) void gemm_bf16_stream_kernel(
+// NW waves: 4 (2 x 2 wave grid) or 8 (2 x 4, two waves per SIMD, so one
+// wave's LDS reads overlap the other's MFMAs).
+template <typename TOut, bool SPLIT, int NSTAGE, int BN, bool UNIT, int NW>
+__global__ __launch_bounds__(NW * 64, 1) void gemm_bf16_stream_kernel(
     const GTask* __restrict__ tasks, int ntasks, const GSeg* __restrict__ segs, int tiles_per_vec, int total_tiles,
     const __hip_bfloat16* __restrict__ A, int64_t a_vs, int64_t a_plane, const __hip_bfloat16* __restrict__ B,
     int64_t b_vs, int64_t b_plane, TOut* __restrict__ C, int64_t c_vs, int64_t c_plane, int c_rowmajor) {
   constexpr int BM = 256, BK = 32;
   constexpr int NP = SPLIT ? 2 : 1;
-  constexpr int WM = 128, WN = BN / 2, TM = WM / 16, TN = WN / 16;
+  constexpr int WM = 128, WN = BN / (NW / 2), TM = WM / 16, TN = WN / 16;
   constexpr int ROWB = BK * 2;                        // bytes per LDS row (one plane)
   constexpr int A_BYTES = BM * ROWB, B_BYTES = BN * ROWB;
   constexpr int STAGE_BYTES = NP * (A_BYTES + B_BYTES);
-  constexpr int LA = BM / 64, LB = BN / 64;           // load instructions per wave and plane
+  constexpr int LA = BM / (16 * NW), LB = BN / (16 * NW);  // load instructions per wave and plane
   constexpr int LOADS = NP * (LA + LB);               // per wave per stage
   static_assert(TM == 8 && (TN == 4 || TN == 8), "wave tile 128 x 64 or 128 x 128");
   static_assert(NSTAGE * STAGE_BYTES <= 160 * 1024, "LDS budget");
@@ -1314,7 +1316,8 @@ static hipError_t runbf(const GTask* tasks, int ntasks, const GSeg* segs, int tp
   if constexpr (BM == 256) {
     // stages that fit the 160 KB LDS: 256x128 -> 4 (3 split), 256x256 -> 4 (2 split)
     constexpr int NST = BN == 128 ? (SPLIT ? 3 : 4) : (SPLIT ? 2 : 4);
-    gemm_bf16_stream_kernel<TOut, SPLIT, NST, BN, BN == 256><<<grid, 256, 0, st>>>(
+    constexpr int NW = BN == 256 ? 8 : 4;
+    gemm_bf16_stream_kernel<TOut, SPLIT, NST, BN, BN == 256, NW><<<grid, NW * 64, 0, st>>>(
         tasks, ntasks, segs, tpv, total, (const __hip_bfloat16*)A, a_vs, a_pl, (const __hip_bfloat16*)B, b_vs, b_pl,
         (TOut*)C, c_vs, c_pl, c_rowmajor);
   } else {
