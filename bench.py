@@ -2,17 +2,18 @@
 
 Workload: the centre bond of a dense (no symmetry) spin-1/2 Heisenberg chain,
 J = 1, L = 128, MPO bond w = 5, d = 2, MPS bond D = 2048, seeded Gaussian MPS
-(left/right normalised by QR), environments built on the GPU with this
+(left/right normalised by CholeskyQR2), environments built on the GPU with this
 package's environment kernels.  A "step" is one two-site H_eff . psi over the
 four D x D psi blocks, inputs resident in HBM.
 
-Reported: the mixed-precision (bf16-in / fp32-accumulate MFMA) grouped matvec
-as `value` in TFLOP/s of algorithmic flops (F_alg = 64 D^3 for this MPO), the
-FP64 and FP32 matvecs beside it, the per-stage roofline of the dominant
-kernel, the Lanczos energy of the bond in mixed precision with FP64
-refinement against all-FP64 (|dE|/L), and a CPU baseline: the oracle's
-restatement of the reference tuple loop (one dgemm pair per tuple, MKL)
-timed on this host on a bounded sample (one matvec).
+Reported: the mixed-precision grouped matvec (default BF16X3: every operand
+split into hi + lo bf16, fp32 MFMA accumulation) as `value` in TFLOP/s of
+algorithmic flops (F_alg = 64 D^3 for this MPO), the FP64, FP64 plain-method,
+FP32 and BF16 matvecs beside it, the roofline of the dominant GEMM stage, the
+Lanczos energy of the bond in mixed precision with FP64 refinement against
+all-FP64 (|dE|/L), and a CPU baseline: the oracle's restatement of the
+reference tuple loop (one dgemm pair per tuple, MKL) timed on this host on a
+bounded sample (three matvecs).
 """
 from __future__ import annotations
 
@@ -34,29 +35,43 @@ def log(*a):
 
 
 def time_apply(torch, plan, x, y, steps, warmup):
+    """Wall time per matvec over `steps` applies after `warmup`, bracketed by
+    device synchronisation; the same applies carry HIP events on the plan's
+    stream, giving the average time of each stage inside the timed region."""
     for _ in range(warmup):
         plan.apply(x, y)
     torch.cuda.synchronize()
+    plan.set_timing(True)
     t0 = time.perf_counter()
     for _ in range(steps):
         plan.apply(x, y)
     torch.cuda.synchronize()
-    return (time.perf_counter() - t0) / steps * 1e3
-
-
-def stage_times(torch, plan, x, y, reps=5):
-    plan.set_timing(True)
-    acc = [0.0] * 4
-    for _ in range(reps):
-        plan.apply(x, y)
-        t = plan.last_times()
-        acc = [a + b for a, b in zip(acc, t)]
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    st, n = plan.timing_summary()
     plan.set_timing(False)
-    torch.cuda.synchronize()
-    return [a / reps for a in acc]
+    assert n == steps
+    return ms, st
 
 
-def cpu_baseline(prob, samples=1):
+TILE = {0: "256x128 (4 waves)", 1: "64x64", 2: "128x128", 3: "256x256 (8 waves)"}
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "gemm_bf16_stream_dense_D2048_hbm_traffic.txt")
+
+
+def profiled_traffic(prec, stage):
+    """HBM bytes per launch of one GEMM stage, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated; see the
+    file's header).  None when the profile is absent."""
+    try:
+        for line in open(TRAFFIC_PROFILE):
+            f = line.rstrip("\n").split("\t")
+            if len(f) >= 5 and f[0] == prec and f[1] == f"stage{stage}":
+                return int(float(f[4]))
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(prob, samples=3):
     """The oracle (restated reference tuple loop, MKL dgemm) on this host."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
@@ -162,8 +177,7 @@ def main():
         plan = prob["plan_" + pr]
         info = plan.info()
         si = plan.stage_info()
-        ms = time_apply(torch, plan, x, y, args.steps, args.warmup)
-        st = stage_times(torch, plan, x, y)
+        ms, st = time_apply(torch, plan, x, y, args.steps, args.warmup)
         res[pr] = dict(ms=ms, tflops=info["flops_alg"] / ms / 1e9, flops_alg=info["flops_alg"],
                        flops_ref=info["flops_ref"], stage_ms=st, stage_flops=[si["flops1"], si["flops2"]],
                        tile_class=[si["tile_class1"], si["tile_class2"]])
@@ -172,25 +186,31 @@ def main():
     r = res[mixed]
     # dominant kernel of the mixed matvec: the slower GEMM stage
     k = 1 if r["stage_ms"][1] >= r["stage_ms"][3] else 3
-    fl = r["stage_flops"][0 if k == 1 else 1]
+    stage = 1 if k == 1 else 2
+    fl = r["stage_flops"][stage - 1]
     achieved = fl / r["stage_ms"][k] / 1e9
     mf = MFMA_FLOPS_PER_ALG.get(mixed, 1.0)
     roof = {"bound": "mfma", "achieved": round(achieved * mf, 2), "peak": PEAK[mixed], "unit": "TFLOP/s",
-            "frac": round(achieved * mf / PEAK[mixed], 4), "traffic": None,
-            "kernel": "gemm_bf16_kernel stage %d (%s)" % (1 if k == 1 else 2, mixed),
-            "note": "achieved counts MFMA flops (%.0fx the algorithmic flops for %s)" % (mf, mixed)}
+            "frac": round(achieved * mf / PEAK[mixed], 4), "traffic": profiled_traffic(mixed, stage),
+            "kernel": "gemm_bf16_stream_kernel %s, stage %d (%s)" % (TILE.get(r["tile_class"][stage - 1], "?"),
+                                                                   stage, mixed),
+            "stage_ms": round(r["stage_ms"][k], 4), "stage_flops_alg": fl,
+            "note": "achieved = MFMA flops of the stage (%.0fx its algorithmic flops for %s) / its average "
+                    "HIP-event duration over the timed steps; traffic = HBM bytes per launch from "
+                    "profiles/gemm_bf16_stream_dense_D2048_hbm_traffic.txt" % (mf, mixed)}
     lan = None
     if not args.no_lanczos and "f64" in res:
         lan = lanczos_compare(torch, prob, mixed, args)
     cpu = None
     if not args.no_cpu:
-        log("cpu baseline: oracle contractTwosite on host cores (1 matvec)")
+        log("cpu baseline: oracle contractTwosite on host cores (3 matvecs)")
         dt, cores, blas = cpu_baseline(prob)
         cpu = {"value": round(r["flops_alg"] / dt / 1e12, 4), "unit": "TFLOP/s", "cores": cores, "kind": "port",
-               "sample": f"1 two-site matvec at D={args.D} (dense Heisenberg centre bond), {blas}, "
-                         f"{dt:.2f} s wall; F_alg/t"}
+               "sample": f"3 two-site matvecs at D={args.D} (dense Heisenberg centre bond, the reference "
+                         f"tuple loop: one {blas} dgemm pair per tuple, F_ref = 120 D^3 flops), "
+                         f"{dt:.2f} s wall per matvec; value = F_alg / t"}
     out = {
-        "metric": "H_eff·psi TFLOP/s at D=2048 (mixed bf16-in/fp32-acc grouped matvec)",
+        "metric": "H_eff\u00b7\u03c8 TFLOP/s + DMRG sweep s at D=2048; mixed vs FP64 at |\u0394E/L|\u22641e-9",
         "value": round(r["tflops"], 3),
         "unit": "TFLOP/s",
         "n_gpus": 1,
@@ -200,7 +220,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16-in/fp32-acc",
+        "dtype": "bf16x3 (split bf16 in, fp32 acc)" if mixed == "bf16x3" else mixed,
         "data": "synthetic",
         "config": {"workload": f"dense Heisenberg J=1 centre bond, L={args.L}, w=5, d=2, D={args.D}, seeded "
                                f"Gaussian MPS, QR-normalised, GPU-built environments",

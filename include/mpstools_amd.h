@@ -161,6 +161,10 @@ int mpsa_plan_set_timing(mpsa_plan* plan, int enable);
  * the stage's transpose code (tasks), tile class (segments) or precision (mix). */
 int mpsa_plan_describe(const mpsa_plan* plan, int what, void* buf, int64_t cap, int64_t* n, int* extra);
 int mpsa_plan_last_times(const mpsa_plan* plan, float* ms4);
+/* Average stage times over every apply timed since timing was enabled or the
+ * last summary (at most 512 applies are kept; later ones overwrite the last
+ * slot); *napplies receives their count and the count restarts at 0. */
+int mpsa_plan_timing_summary(mpsa_plan* plan, float* ms4, int* napplies);
 int mpsa_plan_destroy(mpsa_plan* plan);
 
 /* Extents (elements) of the x and y layouts and whether they are the same

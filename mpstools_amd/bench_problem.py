@@ -12,6 +12,8 @@ D x D blocks (s1, s2).
 """
 from __future__ import annotations
 
+import sys
+
 import numpy as np
 
 from . import models, native
@@ -84,14 +86,14 @@ def dense_heisenberg_centre(D=2048, L=128, seed=20260003, build_envs="sweep", pr
         for s in range(0, c):
             Et, dE = env_step(Et, dE, s, 1)
             if verbose and s % 16 == 0:
-                print(f"[bench_problem] left env site {s}", flush=True)
+                print(f"[bench_problem] left env site {s}", file=sys.stderr, flush=True)
         Lt, dL = Et, dE
         # R[L] = identity on the last bond (getRMax), aux 0
         Et, dE = _lmin_table(), torch.ones(1, dtype=torch.float64, device=dev)
         for s in range(L - 1, c + 1, -1):
             Et, dE = env_step(Et, dE, s, 0)
             if verbose and s % 16 == 0:
-                print(f"[bench_problem] right env site {s}", flush=True)
+                print(f"[bench_problem] right env site {s}", file=sys.stderr, flush=True)
         Rt, dR = Et, dE
     else:
         raise ValueError(build_envs)

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -186,10 +187,15 @@ struct mpsa_plan {
   int timing = 0;
   int on_device = 0;
   int xy_same = 0;
-  hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  // timing: one set of 5 events per timed apply (up to kTimedApplies kept)
+  static constexpr int kTimedApplies = 512;
+  std::vector<std::array<hipEvent_t, 5>> evs;
+  int nrec = 0;    // applies recorded since the last summary
+  int cur = -1;    // event set of the last timed apply
   ~mpsa_plan() {
-    for (auto& e : ev)
-      if (e) (void)hipEventDestroy(e);
+    for (auto& set : evs)
+      for (auto& e : set)
+        if (e) (void)hipEventDestroy(e);
     for (void* p : {(void*)s1.d_tasks, (void*)s1.d_segs, (void*)s2.d_tasks, (void*)s2.d_segs, (void*)d_mix,
                     (void*)d_terms, (void*)d_zero, (void*)d_rpL, (void*)d_rpR, (void*)d_rpx, ws, xc, Lc, Rc})
       if (p) (void)hipFree(p);
@@ -997,7 +1003,19 @@ int mpsa_heff_apply(mpsa_plan* P, const double* dx, int64_t x_stride, double* dy
       throw Error(MPSA_EINVAL, "apply: vector stride smaller than the table extent");
     hipStream_t st = (hipStream_t)stream;
     ensure_ws(P, nvec);
-    if (P->timing) HIPCHK(hipEventRecord(P->ev[0], st));
+    hipEvent_t* ev = nullptr;
+    if (P->timing) {
+      const int slot = std::min(P->nrec, mpsa_plan::kTimedApplies - 1);
+      while ((int)P->evs.size() <= slot) {
+        std::array<hipEvent_t, 5> set{};
+        for (auto& e : set) HIPCHK(hipEventCreate(&e));
+        P->evs.push_back(set);
+      }
+      ev = P->evs[(size_t)slot].data();
+      P->cur = slot;
+      P->nrec = slot + 1;
+      HIPCHK(hipEventRecord(ev[0], st));
+    }
     const void* A1 = P->prec == F64 ? (const void*)P->dL : P->Lc;
     const void* B1 = P->prec == F64 ? (const void*)dx : P->xc;
     const int64_t xcs = P->xc_per_vec * P->planes, wss = P->ws_per_vec * P->planes;
@@ -1005,25 +1023,25 @@ int mpsa_heff_apply(mpsa_plan* P, const double* dx, int64_t x_stride, double* dy
     if (P->prec != F64)
       HIPCHK(launch_repack(P->prec, P->d_rpx, (int)P->rpx.size(), P->rpx_chunks, nvec, dx, x_stride, P->xc, xcs, st,
                            P->xc_per_vec));
-    if (P->timing) HIPCHK(hipEventRecord(P->ev[1], st));
+    if (ev) HIPCHK(hipEventRecord(ev[1], st));
     Planes pl1;
     pl1.a = P->Lc_elems;
     pl1.b = P->xc_per_vec;
     pl1.c = P->ws_per_vec;
     HIPCHK(launch_gemm(P->prec, 0, P->s1.tr, P->s1.tile_class, P->s1.d_tasks, (int)P->s1.tasks.size(),
                        P->s1.d_segs, P->s1.tiles, nvec, A1, 0, B1, b1_vs, P->ws, wss, st, pl1));
-    if (P->timing) HIPCHK(hipEventRecord(P->ev[2], st));
+    if (ev) HIPCHK(hipEventRecord(ev[2], st));
     HIPCHK(launch_mix(P->prec, P->d_mix, (int)P->mix.size(), P->d_terms, P->mix_chunks, nvec, P->ws, P->ws, wss, st,
                       P->ws_per_vec));
     const void* B2 = P->prec == F64 ? (const void*)P->dR : P->Rc;
     HIPCHK(launch_zero_ranges(P->d_zero, (int)P->zero.size(), dy, y_stride, nvec, st));
-    if (P->timing) HIPCHK(hipEventRecord(P->ev[3], st));
+    if (ev) HIPCHK(hipEventRecord(ev[3], st));
     Planes pl2;
     pl2.a = P->ws_per_vec;
     pl2.b = P->Rc_elems;
     HIPCHK(launch_gemm(P->prec, 1, P->s2.tr, P->s2.tile_class, P->s2.d_tasks, (int)P->s2.tasks.size(),
                        P->s2.d_segs, P->s2.tiles, nvec, P->ws, wss, B2, 0, dy, y_stride, st, pl2));
-    if (P->timing) HIPCHK(hipEventRecord(P->ev[4], st));
+    if (ev) HIPCHK(hipEventRecord(ev[4], st));
   })
 }
 
@@ -1104,18 +1122,36 @@ int mpsa_plan_describe(const mpsa_plan* P, int what, void* buf, int64_t cap, int
 int mpsa_plan_set_timing(mpsa_plan* P, int enable) {
   MPSA_TRY({
     if (!P) throw Error(MPSA_EINVAL, "null plan");
-    if (enable && !P->ev[0])
-      for (auto& e : P->ev) HIPCHK(hipEventCreate(&e));
     P->timing = enable ? 1 : 0;
+    P->nrec = 0;
+    P->cur = -1;
   })
 }
 
 int mpsa_plan_last_times(const mpsa_plan* P, float* ms) {
   MPSA_TRY({
     if (!P || !ms) throw Error(MPSA_EINVAL, "null argument");
-    if (!P->timing || !P->ev[0]) throw Error(MPSA_EINVAL, "timing not enabled on this plan");
-    HIPCHK(hipEventSynchronize(P->ev[4]));
-    for (int i = 0; i < 4; ++i) HIPCHK(hipEventElapsedTime(&ms[i], P->ev[i], P->ev[i + 1]));
+    if (P->cur < 0) throw Error(MPSA_EINVAL, "no timed apply on this plan");
+    const auto& ev = P->evs[(size_t)P->cur];
+    HIPCHK(hipEventSynchronize(ev[4]));
+    for (int i = 0; i < 4; ++i) HIPCHK(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+  })
+}
+
+int mpsa_plan_timing_summary(mpsa_plan* P, float* ms, int* napplies) {
+  MPSA_TRY({
+    if (!P || !ms || !napplies) throw Error(MPSA_EINVAL, "null argument");
+    const int n = P->nrec;
+    for (int i = 0; i < 4; ++i) ms[i] = 0.f;
+    if (n > 0) HIPCHK(hipEventSynchronize(P->evs[(size_t)(n - 1)][4]));
+    for (int a = 0; a < n; ++a)
+      for (int i = 0; i < 4; ++i) {
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, P->evs[(size_t)a][(size_t)i], P->evs[(size_t)a][(size_t)i + 1]));
+        ms[i] += t / (float)n;
+      }
+    *napplies = n;
+    P->nrec = 0;
   })
 }
 

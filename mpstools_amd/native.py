@@ -71,6 +71,7 @@ def lib():
         L.mpsa_plan_set_timing.argtypes = [P, C.c_int]
         L.mpsa_plan_describe.argtypes = [P, C.c_int, P, C.c_int64, P, P]
         L.mpsa_plan_last_times.argtypes = [P, P]
+        L.mpsa_plan_timing_summary.argtypes = [P, P, P]
         L.mpsa_plan_vector_lengths.argtypes = [P, P, P, P]
         L.mpsa_lanczos.argtypes = [P, P, P, C.c_int, C.c_double, C.c_int, C.c_int, P, C.c_int, P, P, P]
         L.mpsa_lanczos_batched.argtypes = [P, P, C.c_int64, C.c_int, P, C.c_int, C.c_double, C.c_int, P, C.c_int,
@@ -269,6 +270,14 @@ class HeffPlan:
         ms = np.zeros(4, np.float32)
         _check(lib().mpsa_plan_last_times(self.h, _ptr(ms)))
         return [float(x) for x in ms]
+
+    def timing_summary(self):
+        """Average [convert, stage 1, mix + zero, stage 2] ms over the applies
+        timed since timing was enabled (HIP events on the apply stream)."""
+        ms = np.zeros(4, np.float32)
+        n = np.zeros(1, np.int32)
+        _check(lib().mpsa_plan_timing_summary(self.h, _ptr(ms), _ptr(n)))
+        return [float(x) for x in ms], int(n[0])
 
     def bind_env(self, dL, dR):
         self._env = (dL, dR)

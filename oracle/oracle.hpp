@@ -322,7 +322,9 @@ BlockMatrix<1, T> addContractionLayer(const BlockMatrix<1, T>& BL, const BlockMa
   if (maxthreads < 1) maxthreads = 1;
   std::vector<BlockMatrix<1, T>> bouts((size_t)maxthreads);
   const BlockMatrix<1, T>& Aup = Ap ? *Ap : A;
-#pragma omp parallel for num_threads(maxthreads) schedule(static)
+  // With one outgoing charge the region stays inactive, so the BLAS calls
+  // inside run multi-threaded instead of on the one thread of the loop.
+#pragma omp parallel for num_threads(maxthreads) schedule(static) if (n_out > 1)
   for (int i = 0; i < n_out; ++i) {
     int jthr = 0;
 #ifdef _OPENMP
@@ -439,7 +441,8 @@ void contractTwosite(BlockMatrix<2, T>& Aout, const BlockMatrix<1, T>& BL, const
   const int n = (int)lefts.size();
   if (maxthreads < 1) maxthreads = 1;
   std::vector<BlockMatrix<2, T>> aout((size_t)maxthreads);
-#pragma omp parallel for num_threads(maxthreads) schedule(static)
+  // one left charge (dense case): inactive region, multi-threaded BLAS inside
+#pragma omp parallel for num_threads(maxthreads) schedule(static) if (n > 1)
   for (int i = 0; i < n; ++i) {
     int j = 0;
 #ifdef _OPENMP
@@ -495,7 +498,7 @@ void contractSingle(BlockMatrix<1, T>& Aout, const BlockMatrix<1, T>& BL, const 
   const int n = (int)lefts.size();
   if (maxthreads < 1) maxthreads = 1;
   std::vector<BlockMatrix<1, T>> aout((size_t)maxthreads);
-#pragma omp parallel for num_threads(maxthreads) schedule(static)
+#pragma omp parallel for num_threads(maxthreads) schedule(static) if (n > 1)
   for (int i = 0; i < n; ++i) {
     int j = 0;
 #ifdef _OPENMP
