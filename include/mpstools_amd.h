@@ -243,6 +243,32 @@ int mpsa_split_copy(const mpsa_split_result* r, int which, double* dst, mpsa_str
 int mpsa_split_lambda(const mpsa_split_result* r, uint64_t* q, int64_t* sizes, double* values);
 int mpsa_split_destroy(mpsa_split_result* r);
 
+/* ---- two-site DMRG sweep (DMRGengine, src/container.hpp:293-337, 383-438) ----
+ * A chain of nsites site tensors (ns = 1 tables, FP64 device buffers copied
+ * at creation; the state must be right-normalised, as DMRGengine::simulate
+ * makes it with orthonormalize("rl")), MPO tensors and i_to_key arrays
+ * (concatenated, d[i] per site).  Creation builds R[N] = getRMax and the right
+ * environments R[N-1] .. R[2] on the device.  A sweep is the body of
+ * DMRGengine::simulate: doStep(site, +1) for site = 0 .. N-3, then
+ * doStep(site, -1) for site = N-2 .. 1, each step = two-site tensor,
+ * Lanczos (lanmax, delta, energy test; `precision` matvecs, then for a
+ * reduced precision at most refine_it FP64 Lanczos steps from the Ritz
+ * vector), getState(0), split (chimax, do_trunc, tr_weight as mpsa_split,
+ * normalised lambda into the site ahead) and the environment update.  Returns
+ * the energy of the last step, the largest truncated weight, the total number
+ * of Lanczos iterations and the wall time of the sweep. */
+typedef struct mpsa_dmrg mpsa_dmrg;
+int mpsa_dmrg_create(mpsa_dmrg** dmrg, int nsites, const mpsa_blocks* sites, const double* const* dA, const mpsa_mpo* W,
+                     const uint64_t* itokey, const int* d, mpsa_stream stream);
+int mpsa_dmrg_sweep(mpsa_dmrg* dmrg, int precision, int refine_it, unsigned chimax, int do_trunc, double tr_weight,
+                    int lanmax, double delta, double* energy, double* max_truncated_weight, int* lanczos_iterations,
+                    double* seconds);
+/* Site tensor `site`: block count and packed length; with non-NULL arrays the
+ * table, with non-NULL dst a device-to-device copy of the data. */
+int mpsa_dmrg_site(const mpsa_dmrg* dmrg, int site, int64_t* n_blocks, int64_t* len, uint64_t* q, uint32_t* p,
+                   int64_t* rows, int64_t* cols, int64_t* offsets, double* dst, mpsa_stream stream);
+int mpsa_dmrg_destroy(mpsa_dmrg* dmrg);
+
 /* ---- Energy and its adjoint over a whole chain ----
  * Forward: MPO<T>::measure (src/MPOTypes.hpp:990-1017), E = <psi|H|psi>
  * unnormalised, by the left environment sweep of addContractionLayer from

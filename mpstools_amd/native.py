@@ -89,6 +89,11 @@ def lib():
         L.mpsa_split_copy.argtypes = [P, C.c_int, P, P]
         L.mpsa_split_lambda.argtypes = [P, P, P, P]
         L.mpsa_split_destroy.argtypes = [P]
+        L.mpsa_dmrg_create.argtypes = [P, C.c_int, P, P, P, P, P, P]
+        L.mpsa_dmrg_sweep.argtypes = [P, C.c_int, C.c_int, C.c_uint, C.c_int, C.c_double, C.c_int, C.c_double, P, P,
+                                      P, P]
+        L.mpsa_dmrg_site.argtypes = [P, C.c_int, P, P, P, P, P, P, P, P, P]
+        L.mpsa_dmrg_destroy.argtypes = [P]
         L.mpsa_tridiag_eig.argtypes = [C.c_int, P, P, P]
         L.mpsa_dot.argtypes = [P, P, C.c_int64, C.c_int, P, P]
         L.mpsa_axpby.argtypes = [P, P, P, P, C.c_int64, C.c_int, P]
@@ -472,6 +477,51 @@ def split(psi_t: Table, dpsi, itk_l, itk_r, direction=1, chimax=0, do_trunc=Fals
         return out
     finally:
         L.mpsa_split_destroy(h)
+
+
+class Dmrg:
+    """Device-resident two-site DMRG (DMRGengine::doStep / simulate sweeps).
+    site_tables/dA: a right-normalised MPS (ns = 1 tables, device FP64)."""
+
+    def __init__(self, site_tables, dA, mpo, itk):
+        self.N = len(site_tables)
+        self._tabs = list(site_tables)
+        self._mpo = [MPOArg(W) for W in mpo]
+        sites = (_Blocks * self.N)(*[t.s for t in self._tabs])
+        mpos = (_MPO * self.N)(*[m.s for m in self._mpo])
+        ptrs = (C.c_void_p * self.N)(*[_dptr(b) for b in dA])
+        itk_all = np.ascontiguousarray(np.concatenate([_itk(x) for x in itk]), dtype=np.uint64)
+        d = np.array([len(x) for x in itk], dtype=np.int32)
+        h = C.c_void_p()
+        _check(lib().mpsa_dmrg_create(C.byref(h), self.N, C.cast(sites, C.c_void_p), C.cast(ptrs, C.c_void_p),
+                                      C.cast(mpos, C.c_void_p), _ptr(itk_all), _ptr(d), _stream()))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().mpsa_dmrg_destroy(self.h)
+            self.h = None
+
+    def sweep(self, chimax, do_trunc=False, tr_weight=0.0, lanmax=500, delta=1e-12, precision=F64, refine_it=10):
+        E, tw, sec = np.zeros(1), np.zeros(1), np.zeros(1)
+        it = np.zeros(1, np.int32)
+        _check(lib().mpsa_dmrg_sweep(self.h, int(precision), int(refine_it), int(chimax), int(bool(do_trunc)),
+                                     float(tr_weight), int(lanmax), float(delta), _ptr(E), _ptr(tw), _ptr(it),
+                                     _ptr(sec)))
+        return dict(E=float(E[0]), tw=float(tw[0]), it=int(it[0]), s=float(sec[0]))
+
+    def site(self, i):
+        """(Table, device tensor) of site tensor i."""
+        import torch
+        nb, ln = np.zeros(1, np.int64), np.zeros(1, np.int64)
+        _check(lib().mpsa_dmrg_site(self.h, int(i), _ptr(nb), _ptr(ln), None, None, None, None, None, None, None))
+        n = int(nb[0])
+        q, p = np.zeros(2 * n, np.uint64), np.zeros(n, np.uint32)
+        r, c, o = np.zeros(n, np.int64), np.zeros(n, np.int64), np.zeros(n, np.int64)
+        buf = torch.zeros(max(1, int(ln[0])), dtype=torch.float64, device="cuda")
+        _check(lib().mpsa_dmrg_site(self.h, int(i), _ptr(nb), _ptr(ln), _ptr(q), _ptr(p), _ptr(r), _ptr(c), _ptr(o),
+                                    _dptr(buf), _stream()))
+        return Table(1, q.reshape(-1, 2), p.reshape(-1, 1), r, c, o), buf
 
 
 def tridiag_eig(d, e, vectors=True):

@@ -13,6 +13,9 @@ import numpy as np
 
 def _op(buf, off, ld, rows, cols, trans):
     """op(X) of shape rows x cols where X is stored column-major at off with ld."""
+    assert off >= 0 and ld >= 1
+    last = off + (cols - 1 if not trans else rows - 1) * ld + (rows if not trans else cols)
+    assert last <= len(buf), ("operand read past the buffer", off, ld, rows, cols, len(buf))
     if not trans:  # X is rows x cols
         out = np.zeros((rows, cols))
         for j in range(cols):
@@ -37,6 +40,7 @@ def run_stage(tasks, segs, tr, A, B, C, c_vs=0):
             a = _op(A, int(s["a_off"]), int(s["lda"]), m, k, ta)
             b = _op(B, int(s["b_off"]), int(s["ldb"]), k, n, tb)
             acc += float(s["alpha"]) * (a @ b)
+        assert 0 <= int(t["c_off"]) and int(t["c_off"]) + m * n <= len(C), "output block past the buffer"
         C[int(t["c_off"]):int(t["c_off"]) + m * n] = acc.ravel(order="F")
 
 
@@ -50,6 +54,7 @@ def run_heff(desc, L, R, x, y, ws_len):
             acc += float(tm["coef"]) * ws[int(tm["src_off"]):int(tm["src_off"]) + ln]
         ws[d:d + ln] = acc
     for z in desc["zero"]:
+        assert 0 <= int(z["off"]) and int(z["off"]) + int(z["len"]) <= len(y), "zero range past the output"
         y[int(z["off"]):int(z["off"]) + int(z["len"])] = 0.0
     run_stage(desc["s2_tasks"], desc["s2_segs"], desc["s2_tasks_extra"], ws, R, y)
 
@@ -64,5 +69,6 @@ def run_env(desc, BL, A, Ap, out, ws_len):
             acc += float(tm["coef"]) * ws[int(tm["src_off"]):int(tm["src_off"]) + ln]
         ws[d:d + ln] = acc
     for z in desc["zero"]:
+        assert 0 <= int(z["off"]) and int(z["off"]) + int(z["len"]) <= len(out), "zero range past the output"
         out[int(z["off"]):int(z["off"]) + int(z["len"])] = 0.0
     run_stage(desc["s2_tasks"], desc["s2_segs"], desc["s2_tasks_extra"], Ap if Ap is not None else A, ws, out)
