@@ -869,6 +869,511 @@ __global__ __launch_bounds__(NW * 64, 1) void gemm_bf16_stream_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// Pipelined 256 x 256 BF16 tile for unit segment scales.
+//
+// Four waves, one per SIMD, each accumulate a 128 x 128 block in AGPRs (8 x 8
+// MFMA 16x16x32 tiles).  A wave issues at most one instruction per four
+// cycles and an MFMA occupies its SIMD's matrix pipe for 16, so with one wave
+// per SIMD every other instruction of the k-loop has to sit between MFMAs:
+//   * the MFMAs are issued from inline asm, eight per statement, and the same
+//     statements issue the LDS fragment reads for later MFMAs (the A
+//     fragments of row i + 2 while row i runs, from a ring of four A rows in
+//     VGPRs; the B fragment i of step s + 1, B double-buffered in VGPRs) and
+//     the global_load_lds DMA of the k-slab D steps ahead (with its address
+//     arithmetic);
+//   * a fragment read fills its destination registers after its statement
+//     has ended.  The registers are outputs of the issuing statement and
+//     in/out operands of a later statement that waits (s_waitcnt lgkmcnt);
+//     nothing else touches them in between (a test of the build checks the
+//     generated code for this);
+//   * k-slabs live in two LDS rings of RA and RB slots, A(s + 1) being read
+//     (rows 0 and 1) during step s.  Each step issues one group, the A and B
+//     slabs of step s + D; past the last slab the cursor stays on the last
+//     segment, so the surplus groups re-read valid memory into free slots and
+//     every step waits for the same vmcnt.  The step loop is unrolled over
+//     the ring period, so every slot address is a compile-time choice.
+// Per k-step and wave: 64 MFMAs (192 for the split form), 16 fragment reads
+// (32), 8 DMA loads (16).  Accumulators are read only after a statement that
+// waits out the MFMA result latency and names them as operands.
+// TR swaps the MFMA operands: the tile is computed transposed, so a lane holds
+// four consecutive columns of one row (row-major output is stored 8 bytes at
+// a time).
+// ---------------------------------------------------------------------------
+#define MPSA_MF(C, A, B)                                                                \
+  ".if %[tr]\n\tv_mfma_f32_16x16x32_bf16 %[" C "], %[" B "], %[" A "], %[" C "]\n\t" \
+  ".else\n\tv_mfma_f32_16x16x32_bf16 %[" C "], %[" A "], %[" B "], %[" C "]\n\t.endif\n\t"
+#define MPSA_ACC8_OUT(c)                                                                                   \
+  [c0] "+a"(c[0]), [c1] "+a"(c[1]), [c2] "+a"(c[2]), [c3] "+a"(c[3]), [c4] "+a"(c[4]), [c5] "+a"(c[5]), \
+      [c6] "+a"(c[6]), [c7] "+a"(c[7])
+#define MPSA_B8_IN(b)                                                                                       \
+  [b0] "v"(b[0]), [b1] "v"(b[1]), [b2] "v"(b[2]), [b3] "v"(b[3]), [b4] "v"(b[4]), [b5] "v"(b[5]),        \
+      [b6] "v"(b[6]), [b7] "v"(b[7])
+// LDS DMA of 16 bytes per lane from g + v + kk into LDS m0 + mo + OFF + 16 lane:
+// the address add and the M0 write come first, an MFMA sits between them and
+// the load
+#define MPSA_DA(T, V, OFF) "v_add_u32 %[" T "], %[" V "], %[kk]\n\ts_add_u32 m0, %[m0], %[mo]+" OFF "\n\t"
+#define MPSA_LD(T) "global_load_lds_dwordx4 %[" T "], %[g]\n\t"
+
+// c[j] += a * b[j] for j = 0..7; two fragment reads (o0 <- LDS[ad0 + O0],
+// o1 <- LDS[ad1 + O1]) are issued after the first and second MFMA and land
+// asynchronously.
+template <bool TR, int O0, int O1>
+__device__ __forceinline__ void mm8_rd2(f4 (&c)[8], const bf8& a, const bf8 (&b)[8], bf8& o0, unsigned ad0,
+                                        bf8& o1, unsigned ad1) {
+  asm volatile(MPSA_MF("c0", "a", "b0")
+               "ds_read_b128 %[o0], %[ad0] offset:%[f0]\n\t"
+               MPSA_MF("c1", "a", "b1")
+               "ds_read_b128 %[o1], %[ad1] offset:%[f1]\n\t"
+               MPSA_MF("c2", "a", "b2") MPSA_MF("c3", "a", "b3") MPSA_MF("c4", "a", "b4")
+               MPSA_MF("c5", "a", "b5") MPSA_MF("c6", "a", "b6") MPSA_MF("c7", "a", "b7")
+               : MPSA_ACC8_OUT(c), [o0] "=&v"(o0), [o1] "=&v"(o1)
+               : [a] "v"(a), MPSA_B8_IN(b), [ad0] "v"(ad0), [ad1] "v"(ad1), [f0] "i"(O0), [f1] "i"(O1),
+                 [tr] "i"(TR ? 1 : 0));
+}
+
+// mm8_rd2 plus one DMA load (LDS m0b + MO, global g + v + kk)
+template <bool TR, int O0, int O1, int MO>
+__device__ __forceinline__ void mm8_rd2_ld1(f4 (&c)[8], const bf8& a, const bf8 (&b)[8], bf8& o0, unsigned ad0,
+                                            bf8& o1, unsigned ad1, unsigned m0b, const char* g, unsigned v,
+                                            unsigned kk) {
+  unsigned t0;
+  asm volatile(MPSA_MF("c0", "a", "b0")
+               "ds_read_b128 %[o0], %[ad0] offset:%[f0]\n\t"
+               MPSA_MF("c1", "a", "b1")
+               "ds_read_b128 %[o1], %[ad1] offset:%[f1]\n\t"
+               MPSA_DA("t0", "v0", "0")
+               MPSA_MF("c2", "a", "b2")
+               MPSA_LD("t0")
+               MPSA_MF("c3", "a", "b3") MPSA_MF("c4", "a", "b4")
+               MPSA_MF("c5", "a", "b5") MPSA_MF("c6", "a", "b6") MPSA_MF("c7", "a", "b7")
+               : MPSA_ACC8_OUT(c), [o0] "=&v"(o0), [o1] "=&v"(o1), [t0] "=&v"(t0)
+               : [a] "v"(a), MPSA_B8_IN(b), [ad0] "v"(ad0), [ad1] "v"(ad1), [f0] "i"(O0), [f1] "i"(O1),
+                 [m0] "s"(m0b), [mo] "i"(MO), [g] "s"(g), [v0] "v"(v), [kk] "s"(kk), [tr] "i"(TR ? 1 : 0)
+               : "m0", "scc");
+}
+
+// mm8_rd2 plus two DMA loads (LDS m0b + MO + 1024 q, global g + v_q + kk, q = 0, 1)
+template <bool TR, int O0, int O1, int MO>
+__device__ __forceinline__ void mm8_rd2_ld2(f4 (&c)[8], const bf8& a, const bf8 (&b)[8], bf8& o0, unsigned ad0,
+                                            bf8& o1, unsigned ad1, unsigned m0b, const char* g, unsigned v0,
+                                            unsigned v1, unsigned kk) {
+  unsigned t0, t1;
+  asm volatile(MPSA_MF("c0", "a", "b0")
+               "ds_read_b128 %[o0], %[ad0] offset:%[f0]\n\t"
+               MPSA_MF("c1", "a", "b1")
+               "ds_read_b128 %[o1], %[ad1] offset:%[f1]\n\t"
+               MPSA_DA("t0", "v0", "0")
+               MPSA_MF("c2", "a", "b2") MPSA_MF("c3", "a", "b3")
+               MPSA_LD("t0") MPSA_DA("t1", "v1", "1024")
+               MPSA_MF("c4", "a", "b4") MPSA_MF("c5", "a", "b5")
+               MPSA_LD("t1")
+               MPSA_MF("c6", "a", "b6") MPSA_MF("c7", "a", "b7")
+               : MPSA_ACC8_OUT(c), [o0] "=&v"(o0), [o1] "=&v"(o1), [t0] "=&v"(t0), [t1] "=&v"(t1)
+               : [a] "v"(a), MPSA_B8_IN(b), [ad0] "v"(ad0), [ad1] "v"(ad1), [f0] "i"(O0), [f1] "i"(O1),
+                 [m0] "s"(m0b), [mo] "i"(MO), [g] "s"(g), [v0] "v"(v0), [v1] "v"(v1), [kk] "s"(kk),
+                 [tr] "i"(TR ? 1 : 0)
+               : "m0", "scc");
+}
+
+// mm8_rd2 plus four DMA loads (LDS m0b + MO + 1024 q, global g + v[q] + kk)
+template <bool TR, int O0, int O1, int MO>
+__device__ __forceinline__ void mm8_rd2_ld4(f4 (&c)[8], const bf8& a, const bf8 (&b)[8], bf8& o0, unsigned ad0,
+                                            bf8& o1, unsigned ad1, unsigned m0b, const char* g,
+                                            const unsigned (&v)[4], unsigned kk) {
+  unsigned t0, t1, t2, t3;
+  asm volatile(MPSA_MF("c0", "a", "b0")
+               "ds_read_b128 %[o0], %[ad0] offset:%[f0]\n\t"
+               MPSA_MF("c1", "a", "b1")
+               "ds_read_b128 %[o1], %[ad1] offset:%[f1]\n\t"
+               MPSA_DA("t0", "v0", "0")
+               MPSA_MF("c2", "a", "b2")
+               MPSA_LD("t0") MPSA_DA("t1", "v1", "1024")
+               MPSA_MF("c3", "a", "b3")
+               MPSA_LD("t1") MPSA_DA("t2", "v2", "2048")
+               MPSA_MF("c4", "a", "b4")
+               MPSA_LD("t2") MPSA_DA("t3", "v3", "3072")
+               MPSA_MF("c5", "a", "b5")
+               MPSA_LD("t3")
+               MPSA_MF("c6", "a", "b6") MPSA_MF("c7", "a", "b7")
+               : MPSA_ACC8_OUT(c), [o0] "=&v"(o0), [o1] "=&v"(o1), [t0] "=&v"(t0), [t1] "=&v"(t1),
+                 [t2] "=&v"(t2), [t3] "=&v"(t3)
+               : [a] "v"(a), MPSA_B8_IN(b), [ad0] "v"(ad0), [ad1] "v"(ad1), [f0] "i"(O0), [f1] "i"(O1),
+                 [m0] "s"(m0b), [mo] "i"(MO), [g] "s"(g), [v0] "v"(v[0]), [v1] "v"(v[1]), [v2] "v"(v[2]),
+                 [v3] "v"(v[3]), [kk] "s"(kk), [tr] "i"(TR ? 1 : 0)
+               : "m0", "scc");
+}
+
+// c[j] += a * b[j] for j = 0..7
+template <bool TR>
+__device__ __forceinline__ void mm8(f4 (&c)[8], const bf8& a, const bf8 (&b)[8]) {
+  asm volatile(MPSA_MF("c0", "a", "b0") MPSA_MF("c1", "a", "b1") MPSA_MF("c2", "a", "b2") MPSA_MF("c3", "a", "b3")
+               MPSA_MF("c4", "a", "b4") MPSA_MF("c5", "a", "b5") MPSA_MF("c6", "a", "b6") MPSA_MF("c7", "a", "b7")
+               : MPSA_ACC8_OUT(c)
+               : [a] "v"(a), MPSA_B8_IN(b), [tr] "i"(TR ? 1 : 0));
+}
+
+// four DMA loads outside the MFMA stream (tile prologue)
+template <int MO>
+__device__ __forceinline__ void dma4(unsigned m0b, const char* g, const unsigned (&v)[4], unsigned kk) {
+  unsigned t0, t1, t2, t3;
+  asm volatile(MPSA_DA("t0", "v0", "0") "s_nop 0\n\t" MPSA_LD("t0") MPSA_DA("t1", "v1", "1024") "s_nop 0\n\t"
+               MPSA_LD("t1") MPSA_DA("t2", "v2", "2048") "s_nop 0\n\t" MPSA_LD("t2")
+               MPSA_DA("t3", "v3", "3072") "s_nop 0\n\t" MPSA_LD("t3")
+               : [t0] "=&v"(t0), [t1] "=&v"(t1), [t2] "=&v"(t2), [t3] "=&v"(t3)
+               : [m0] "s"(m0b), [mo] "i"(MO), [g] "s"(g), [v0] "v"(v[0]), [v1] "v"(v[1]), [v2] "v"(v[2]),
+                 [v3] "v"(v[3]), [kk] "s"(kk)
+               : "m0", "scc");
+}
+#undef MPSA_MF
+#undef MPSA_DA
+#undef MPSA_LD
+
+// Waits until at most N LDS reads are outstanding; x and y hold their data after it.
+template <int N>
+__device__ __forceinline__ void lgkm_wait(bf8& x, bf8& y) {
+  asm volatile("s_waitcnt lgkmcnt(%[n])" : "+v"(x), "+v"(y) : [n] "i"(N));
+}
+template <int N>
+__device__ __forceinline__ void lgkm_wait(bf8& x) {
+  asm volatile("s_waitcnt lgkmcnt(%[n])" : "+v"(x) : [n] "i"(N));
+}
+// Waits for every LDS read; the eight B fragments of one plane and two A rows land.
+__device__ __forceinline__ void lgkm_wait_all(bf8 (&b)[8], bf8& x, bf8& y) {
+  asm volatile("s_waitcnt lgkmcnt(0)"
+               : "+v"(b[0]), "+v"(b[1]), "+v"(b[2]), "+v"(b[3]), "+v"(b[4]), "+v"(b[5]), "+v"(b[6]), "+v"(b[7]),
+                 "+v"(x), "+v"(y));
+}
+// Orders later accumulator reads after the MFMA result latency (nops are issued
+// by the first call only).
+template <bool NOPS>
+__device__ __forceinline__ void acc_fence(f4 (&c)[8]) {
+  if constexpr (NOPS)
+    asm volatile("s_nop 7\n\ts_nop 7\n\ts_nop 7" : MPSA_ACC8_OUT(c));
+  else
+    asm volatile("" : MPSA_ACC8_OUT(c));
+}
+#undef MPSA_ACC8_OUT
+#undef MPSA_B8_IN
+
+// Synchronous reads (the statement waits for them): two fragments.
+__device__ __forceinline__ void lds_rd_2(unsigned a0, unsigned a1, bf8& f0, bf8& f1) {
+  asm volatile(
+      "ds_read_b128 %0, %2\n\t"
+      "ds_read_b128 %1, %3\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&v"(f0), "=&v"(f1)
+      : "v"(a0), "v"(a1));
+}
+
+template <int V>
+struct IC {
+  static constexpr int value = V;
+};
+
+// a wave-uniform pointer, held in SGPRs
+__device__ __forceinline__ const char* uni_ptr(const char* p) {
+  const uint64_t x = (uint64_t)(uintptr_t)p;
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(x >> 32));
+  return (const char*)(uintptr_t)(((uint64_t)hi << 32) | lo);
+}
+
+// four consecutive outputs of one row (row-major, 8-byte aligned)
+template <bool SPLIT>
+__device__ __forceinline__ void store4_rowmajor(__hip_bfloat16* p, int64_t plane, f4 x) {
+  __hip_bfloat16 h[4], l[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    h[r] = __float2bfloat16(x[r]);
+    if constexpr (SPLIT) l[r] = __float2bfloat16(x[r] - __bfloat162float(h[r]));
+  }
+  uint2 u;
+  u.x = (uint32_t)__builtin_bit_cast(unsigned short, h[0]) | ((uint32_t)__builtin_bit_cast(unsigned short, h[1]) << 16);
+  u.y = (uint32_t)__builtin_bit_cast(unsigned short, h[2]) | ((uint32_t)__builtin_bit_cast(unsigned short, h[3]) << 16);
+  *reinterpret_cast<uint2*>(p) = u;
+  if constexpr (SPLIT) {
+    u.x = (uint32_t)__builtin_bit_cast(unsigned short, l[0]) | ((uint32_t)__builtin_bit_cast(unsigned short, l[1]) << 16);
+    u.y = (uint32_t)__builtin_bit_cast(unsigned short, l[2]) | ((uint32_t)__builtin_bit_cast(unsigned short, l[3]) << 16);
+    *reinterpret_cast<uint2*>(p + plane) = u;
+  }
+}
+
+template <typename TOut, bool SPLIT, bool ROWM, int D, int RA, int RB>
+__global__ __launch_bounds__(256, 1) void gemm_bf16_pipe_kernel(
+    const GTask* __restrict__ tasks, int ntasks, const GSeg* __restrict__ segs, int tiles_per_vec, int total_tiles,
+    const __hip_bfloat16* __restrict__ A, int64_t a_vs, int64_t a_plane, const __hip_bfloat16* __restrict__ B,
+    int64_t b_vs, int64_t b_plane, TOut* __restrict__ C, int64_t c_vs, int64_t c_plane) {
+  constexpr int BM = 256, BN = 256, BK = 32, NP = SPLIT ? 2 : 1;
+  constexpr int ROWB = BK * 2;        // bytes per LDS row (one plane)
+  constexpr int PLANE = BM * ROWB;    // one plane of one operand slab
+  constexpr int SLOT = NP * PLANE;
+  constexpr int G = 8 * NP;           // DMA loads per wave and step: A and B, NP planes, 4 each
+  // the ring period: every slot index below is a function of the step's phase
+  constexpr int P = (RA == RB && RA % 2 == 0) ? RA : (RA == 3 && RB == 2 ? 6 : 0);
+  static_assert(P > 0, "ring period");
+  static_assert(D >= 2 && RA >= D + 1 && RB >= D && (RA + RB) * SLOT <= 160 * 1024, "LDS rings");
+  static_assert((D - 2) * G < 64, "vmcnt range");
+  __shared__ __attribute__((aligned(1024))) unsigned char smem[(RA + RB) * SLOT];
+  const int wave = uni(threadIdx.x >> 6);
+  const int wm = wave & 1, wn = wave >> 1;
+  // M0 of this wave's first DMA row (16 * 4 * wave) in slot 0 of each ring
+  const unsigned m0a0 = (unsigned)uni((int)(lds_u32(smem) + 64 * wave * ROWB));
+  const unsigned m0b0 = (unsigned)uni((int)(lds_u32(smem) + RA * SLOT + 64 * wave * ROWB));
+  const char* const Ac = reinterpret_cast<const char*>(A);
+  const char* const Bc = reinterpret_cast<const char*>(B);
+
+  for (int g = xcd_logical_id(); g < total_tiles; g += gridDim.x) {
+    // lane-dependent values are re-derived per tile (an opaque copy of the
+    // lane id): nothing lane-dependent stays live across the tile loop
+    int lane = threadIdx.x & 63;
+    asm volatile("" : "+v"(lane));
+    const int l16 = lane & 15, lq = lane >> 4;
+    // fragment f of a 16-row group: row l16, 16-byte chunk lq, stored at chunk lq ^ ((row >> 1) & 3)
+    const int foff = l16 * ROWB + ((lq ^ ((l16 >> 1) & 3)) << 4);
+    // DMA lane mapping: load q of this wave covers rows 16 * (4 * wave + q) + lane / 4 of a plane;
+    // lane writes LDS chunk lane & 3 with global chunk (lane & 3) ^ ((row >> 1) & 3)
+    const int lrow = lane >> 2;
+    const int lchunk = (lane & 3) ^ ((lrow >> 1) & 3);
+
+    const int v = g / tiles_per_vec;
+    const int t = g - v * tiles_per_vec;
+    const GTask tk = tasks[find_task(tasks, ntasks, t, &GTask::tile0)];
+    const int local = t - tk.tile0;
+    const int tm = local % tk.tiles_m, tn = local / tk.tiles_m;
+    const int row0 = tm * BM, col0 = tn * BN;
+    const int m = tk.m, n = tk.n;
+    const int send = tk.seg0 + tk.nseg;
+    int nsteps = 0;
+    for (int si = tk.seg0; si < send; ++si) nsteps += (uni(segs[si].k) + BK - 1) / BK;
+
+    f4 acc[8][8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc[i][j] = f4{0, 0, 0, 0};
+
+    if (nsteps > 0) {
+      // fragment addresses of every ring slot
+      unsigned aS[RA], bS[RB];
+#pragma unroll
+      for (int q = 0; q < RA; ++q) aS[q] = lds_u32(smem) + q * SLOT + wm * 128 * ROWB + foff;
+#pragma unroll
+      for (int q = 0; q < RB; ++q) bS[q] = lds_u32(smem + RA * SLOT) + q * SLOT + wn * 128 * ROWB + foff;
+      // DMA cursor (group = A slab + B slab of one k-step): the plane bases of
+      // its segment, the per-lane byte offsets of its rows and 2 kk.  Past the
+      // last segment the cursor keeps the last segment's fields and restarts at k = 0.
+      SegCursor cl;
+      cur_enter(cl, segs, tk.seg0, send);
+      unsigned offA[4], offB[4], kk2 = 0;
+      const char *sa0, *sa1, *sb0, *sb1;
+      auto seg_setup = [&]() {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int r = 16 * (4 * wave + q) + lrow;
+          const int gi = row0 + r < m ? row0 + r : m - 1;
+          const int gj = col0 + r < n ? col0 + r : n - 1;
+          offA[q] = ((unsigned)gi * (unsigned)cl.lda + (unsigned)(lchunk * 8)) * 2u;
+          offB[q] = ((unsigned)gj * (unsigned)cl.ldb + (unsigned)(lchunk * 8)) * 2u;
+        }
+        sa0 = uni_ptr(Ac + 2 * ((int64_t)v * a_vs + cl.a_off));
+        sb0 = uni_ptr(Bc + 2 * ((int64_t)v * b_vs + cl.b_off));
+        sa1 = uni_ptr(sa0 + 2 * a_plane);
+        sb1 = uni_ptr(sb0 + 2 * b_plane);
+        kk2 = 0;
+        // an SGPR written by a VALU instruction is read by VMEM only 5 wait states later
+        asm volatile("s_nop 4");
+      };
+      seg_setup();
+      auto advance = [&]() {
+        cl.kk += BK;
+        kk2 += 2 * BK;
+        if (cl.kk >= cl.k) {
+          cur_enter(cl, segs, uni(cl.si) + 1, send);
+          // unconditional: the descriptor load is waited for here and not at every later use
+          seg_setup();
+        }
+      };
+
+      // prologue: groups 0 .. D-1 into slots 0 .. D-1
+#pragma unroll
+      for (int s = 0; s < D; ++s) {
+        dma4<0>(m0a0 + s * SLOT, sa0, offA, kk2);
+        if constexpr (SPLIT) dma4<PLANE>(m0a0 + s * SLOT, sa1, offA, kk2);
+        dma4<0>(m0b0 + s * SLOT, sb0, offB, kk2);
+        if constexpr (SPLIT) dma4<PLANE>(m0b0 + s * SLOT, sb1, offB, kk2);
+        advance();
+      }
+      bf8 bh0[8], bl0[8], bh1[8], bl1[8], ah[4], al[4];
+      // groups 0 and 1 present: groups 2 .. D-1 may stay in flight
+      wait_vmcnt<(D - 2) * G>();
+      asm volatile("s_barrier" ::: "memory");
+      {
+        const unsigned b0 = bS[0], a0 = aS[0];
+        lds_rd_2(b0, b0 + 1024, bh0[0], bh0[1]);
+        lds_rd_2(b0 + 2048, b0 + 3072, bh0[2], bh0[3]);
+        lds_rd_2(b0 + 4096, b0 + 5120, bh0[4], bh0[5]);
+        lds_rd_2(b0 + 6144, b0 + 7168, bh0[6], bh0[7]);
+        lds_rd_2(a0, a0 + 1024, ah[0], ah[1]);
+        if constexpr (SPLIT) {
+          lds_rd_2(b0 + PLANE, b0 + PLANE + 1024, bl0[0], bl0[1]);
+          lds_rd_2(b0 + PLANE + 2048, b0 + PLANE + 3072, bl0[2], bl0[3]);
+          lds_rd_2(b0 + PLANE + 4096, b0 + PLANE + 5120, bl0[4], bl0[5]);
+          lds_rd_2(b0 + PLANE + 6144, b0 + PLANE + 7168, bl0[6], bl0[7]);
+          lds_rd_2(a0 + PLANE, a0 + PLANE + 1024, al[0], al[1]);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 8; ++i) acc_fence<false>(acc[i]);
+      asm volatile("s_nop 4");
+
+      // step of phase PH (= s mod P): A(s) in slot PH % RA, A(s + 1) in (PH + 1) % RA,
+      // B(s + 1) in (PH + 1) % RB; the group of step s + D goes to A slot
+      // (PH + D) % RA and B slot (PH + D) % RB
+      auto step = [&](auto PH_, bf8 (&bh)[8], bf8 (&bl)[8], bf8 (&nh)[8], bf8 (&nl)[8]) {
+        constexpr int PH = decltype(PH_)::value;
+        constexpr int KA = PH % RA, KA1 = (PH + 1) % RA, KB1 = (PH + 1) % RB;
+        constexpr int IA = (PH + D) % RA, IB = (PH + D) % RB;
+        // A(s + 1) and B(s + 1) have landed: groups s + 2 .. s + D - 1 may stay in flight
+        wait_vmcnt<(D - 2) * G>();
+        // every wave's DMA for step s + 1 has landed and every wave is done
+        // with the slots the new group overwrites
+        asm volatile("s_barrier" ::: "memory");
+        const unsigned a_cur = aS[KA], a_nxt = aS[KA1], b_nxt = bS[KB1];
+        const unsigned kk = kk2;
+        auto row = [&](auto I_) {
+          constexpr int I = decltype(I_)::value;
+          constexpr int R = I % 4, RN = (I + 2) % 4;
+          constexpr int AO = ((I + 2) & 7) * 1024;
+          const unsigned aad = I + 2 < 8 ? a_cur : a_nxt;
+          if constexpr (!SPLIT) {
+            // load I of the group: A rows (q = I) for I < 4, B rows (q = I - 4) after
+            if constexpr (I < 4)
+              mm8_rd2_ld1<ROWM, AO, I * 1024, IA * SLOT + I * 1024>(acc[I], ah[R], bh, ah[RN], aad, nh[I], b_nxt,
+                                                                  m0a0, sa0, offA[I], kk);
+            else
+              mm8_rd2_ld1<ROWM, AO, I * 1024, IB * SLOT + (I - 4) * 1024>(acc[I], ah[R], bh, ah[RN], aad, nh[I],
+                                                                        b_nxt, m0b0, sb0, offB[I - 4], kk);
+            // row I + 1's A fragment came from row I - 1; the reads of row I and
+            // the B read of row I - 1 may stay in flight
+            if constexpr (I < 7) lgkm_wait<3>(ah[(I + 1) % 4]);
+          } else {
+            // rows 0 .. 3 carry the group (one plane of A or B per row, half of
+            // it in each of the first two MFMA statements): the DMA engine moves
+            // about 64 bytes a clock per CU, so a burst of the whole group would
+            // stall the MFMA stream
+            if constexpr (I < 4) {
+              constexpr bool LB = I >= 2;
+              constexpr int MO = (LB ? IB : IA) * SLOT + (I & 1) * PLANE;
+              const unsigned m0x = LB ? m0b0 : m0a0;
+              const char* const gx = LB ? ((I & 1) ? sb1 : sb0) : ((I & 1) ? sa1 : sa0);
+              const unsigned(&ox)[4] = LB ? offB : offA;
+              mm8_rd2_ld2<ROWM, AO, AO + PLANE, MO>(acc[I], ah[R], bh, ah[RN], aad, al[RN], aad, m0x, gx, ox[0],
+                                                    ox[1], kk);
+              mm8_rd2_ld2<ROWM, I * 1024, I * 1024 + PLANE, MO + 2048>(acc[I], al[R], bh, nh[I], b_nxt, nl[I], b_nxt,
+                                                                      m0x, gx, ox[2], ox[3], kk);
+            } else {
+              mm8_rd2<ROWM, AO, AO + PLANE>(acc[I], ah[R], bh, ah[RN], aad, al[RN], aad);
+              mm8_rd2<ROWM, I * 1024, I * 1024 + PLANE>(acc[I], al[R], bh, nh[I], b_nxt, nl[I], b_nxt);
+            }
+            mm8<ROWM>(acc[I], ah[R], bl);
+            // row I + 1's A fragments came from row I - 1; the reads of row I and
+            // the B reads of row I - 1 may stay in flight
+            if constexpr (I < 7) lgkm_wait<6>(ah[(I + 1) % 4], al[(I + 1) % 4]);
+          }
+        };
+        row(IC<0>{});
+        row(IC<1>{});
+        row(IC<2>{});
+        row(IC<3>{});
+        row(IC<4>{});
+        row(IC<5>{});
+        row(IC<6>{});
+        row(IC<7>{});
+        lgkm_wait_all(nh, ah[0], ah[1]);
+        if constexpr (SPLIT) lgkm_wait_all(nl, al[0], al[1]);
+        advance();
+      };
+      // The loop runs whole ring periods and has its only exit at the bottom
+      // (an exit between steps defeats register coalescing of the
+      // accumulators); the remaining steps follow it.  Even phases read B
+      // from bh0/bl0, odd phases from bh1/bl1.
+      int s = 0;
+      for (; s + P <= nsteps; s += P) {
+        step(IC<0>{}, bh0, bl0, bh1, bl1);
+        step(IC<1>{}, bh1, bl1, bh0, bl0);
+        if constexpr (P > 2) {
+          step(IC<2>{}, bh0, bl0, bh1, bl1);
+          step(IC<3>{}, bh1, bl1, bh0, bl0);
+        }
+        if constexpr (P > 4) {
+          step(IC<4>{}, bh0, bl0, bh1, bl1);
+          step(IC<5>{}, bh1, bl1, bh0, bl0);
+        }
+      }
+      if (s < nsteps) step(IC<0>{}, bh0, bl0, bh1, bl1);
+      if (s + 1 < nsteps) step(IC<1>{}, bh1, bl1, bh0, bl0);
+      if constexpr (P > 2) {
+        if (s + 2 < nsteps) step(IC<2>{}, bh0, bl0, bh1, bl1);
+        if (s + 3 < nsteps) step(IC<3>{}, bh1, bl1, bh0, bl0);
+      }
+      if constexpr (P > 4) {
+        if (s + 4 < nsteps) step(IC<4>{}, bh0, bl0, bh1, bl1);
+      }
+      // the surplus groups are still in flight: drain them before the rings are reused
+      wait_vmcnt<0>();
+    }
+    __syncthreads();  // every wave is done with the LDS rings before the next tile's loads
+
+    TOut* Cb = C + (int64_t)v * c_vs + tk.c_off;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      // row i of the accumulators is read only after this point (the first
+      // fence also waits out the MFMA latency)
+      if (i == 0) acc_fence<true>(acc[0]);
+      else acc_fence<false>(acc[i]);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        if constexpr (ROWM) {
+          // transposed tile: row row0 + 128 wm + 16 i + l16, columns col0 + 128 wn + 16 j + 4 lq + r
+          const int ldr = (n + 31) & ~31;
+          const int row = row0 + wm * 128 + i * 16 + l16;
+          const int col = col0 + wn * 128 + j * 16 + lq * 4;
+          if (row < m) {
+            TOut* p = Cb + (int64_t)row * ldr + col;
+            if (col + 3 < n) {
+              if constexpr (sizeof(TOut) == 2) store4_rowmajor<SPLIT>(reinterpret_cast<__hip_bfloat16*>(p), c_plane,
+                                                                      acc[i][j]);
+              else
+#pragma unroll
+                for (int r = 0; r < 4; ++r) store_out<TOut, SPLIT>(p, r, c_plane, acc[i][j][r]);
+            } else {
+#pragma unroll
+              for (int r = 0; r < 4; ++r)
+                if (col + r < n) store_out<TOut, SPLIT>(p, r, c_plane, acc[i][j][r]);
+            }
+          }
+        } else {
+          // column col0 + 128 wn + 16 j + l16, rows row0 + 128 wm + 16 i + 4 lq + r (contiguous)
+          const int col = col0 + wn * 128 + j * 16 + l16;
+          const int row = row0 + wm * 128 + i * 16 + lq * 4;
+          if (col < n) {
+            TOut* p = Cb + row + (int64_t)col * m;
+            if (row + 3 < m) {
+#pragma unroll
+              for (int r = 0; r < 4; ++r) store_out<TOut, SPLIT>(p, r, c_plane, acc[i][j][r]);
+            } else {
+#pragma unroll
+              for (int r = 0; r < 4; ++r)
+                if (row + r < m) store_out<TOut, SPLIT>(p, r, c_plane, acc[i][j][r]);
+            }
+          }
+        }
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // W mix: dst = sum_t coef_t * src_t over 1024-element chunks (4 per thread).
 // ---------------------------------------------------------------------------
 template <typename T>
@@ -1314,12 +1819,24 @@ static hipError_t runbf(const GTask* tasks, int ntasks, const GSeg* segs, int tp
   if (grid > total) grid = total;
   if (grid >= 8) grid &= ~7;
   if constexpr (BM == 256) {
-    // stages that fit the 160 KB LDS: 256x128 -> 4 (3 split), 256x256 -> 4 (2 split)
-    constexpr int NST = BN == 128 ? (SPLIT ? 3 : 4) : (SPLIT ? 2 : 4);
-    constexpr int NW = BN == 256 ? 8 : 4;
-    gemm_bf16_stream_kernel<TOut, SPLIT, NST, BN, BN == 256, NW><<<grid, NW * 64, 0, st>>>(
-        tasks, ntasks, segs, tpv, total, (const __hip_bfloat16*)A, a_vs, a_pl, (const __hip_bfloat16*)B, b_vs, b_pl,
-        (TOut*)C, c_vs, c_pl, c_rowmajor);
+    // 256x128: stream kernel, 4 stages (3 split) in the 160 KB LDS
+    if constexpr (BN == 256) {
+      // DMA distance 2 steps, rings 3 + 2 slots of 32 KB (split); 3 steps, 4 + 4 slots of 16 KB (plain)
+      constexpr int D = SPLIT ? 2 : 3, RA = SPLIT ? 3 : 4, RB = SPLIT ? 2 : 4;
+      if (c_rowmajor)
+        gemm_bf16_pipe_kernel<TOut, SPLIT, true, D, RA, RB><<<grid, 256, 0, st>>>(
+            tasks, ntasks, segs, tpv, total, (const __hip_bfloat16*)A, a_vs, a_pl, (const __hip_bfloat16*)B, b_vs,
+            b_pl, (TOut*)C, c_vs, c_pl);
+      else
+        gemm_bf16_pipe_kernel<TOut, SPLIT, false, D, RA, RB><<<grid, 256, 0, st>>>(
+            tasks, ntasks, segs, tpv, total, (const __hip_bfloat16*)A, a_vs, a_pl, (const __hip_bfloat16*)B, b_vs,
+            b_pl, (TOut*)C, c_vs, c_pl);
+    } else {
+      constexpr int NST = SPLIT ? 3 : 4;
+      gemm_bf16_stream_kernel<TOut, SPLIT, NST, BN, false, 4><<<grid, 256, 0, st>>>(
+          tasks, ntasks, segs, tpv, total, (const __hip_bfloat16*)A, a_vs, a_pl, (const __hip_bfloat16*)B, b_vs,
+          b_pl, (TOut*)C, c_vs, c_pl, c_rowmajor);
+    }
   } else {
     gemm_bf16_kernel<TOut, BM, BN, SPLIT><<<grid, 256, 0, st>>>(
         tasks, ntasks, segs, tpv, total, (const __hip_bfloat16*)A, a_vs, a_pl, (const __hip_bfloat16*)B, b_vs, b_pl,

@@ -220,11 +220,25 @@ static int64_t pad8(int64_t x) { return (x + 31) & ~int64_t(31); }
 
 // Tile class: the largest workgroup tile that wastes at most 20% of its
 // MFMA work on padding; the smallest tile otherwise.
-static int pick_class(const std::vector<GTask>& t, int prec, bool unit_alpha) {
+// The 256x256 class needs unit scales and per-lane operand offsets
+// (row * ld * 2 bytes inside one block) below 2^31.
+static bool fits_pipe_tile(const std::vector<GTask>& t, const std::vector<GSeg>& segs) {
+  for (auto& x : t)
+    for (int j = 0; j < x.nseg; ++j) {
+      const GSeg& g = segs[(size_t)(x.seg0 + j)];
+      if (g.alpha != 1.0) return false;
+      if ((int64_t)x.m * g.lda * 2 >= (int64_t(1) << 31) || (int64_t)x.n * g.ldb * 2 >= (int64_t(1) << 31))
+        return false;
+    }
+  return true;
+}
+
+static int pick_class(const std::vector<GTask>& t, const std::vector<GSeg>& segs, int prec) {
   int best = 1;
   int best_area = 0;
+  const bool pipe_ok = fits_pipe_tile(t, segs);
   for (int cls = 0; cls < num_tile_classes(prec); ++cls) {
-    if (cls == 3 && !unit_alpha) continue;  // the 256x256 kernel has no segment scales
+    if (cls == 3 && !pipe_ok) continue;
     int bm, bn;
     tile_dims(prec, cls, &bm, &bn);
     double useful = 0, padded = 0;
@@ -254,9 +268,7 @@ static void finalize_stage(Stage& s, int prec) {
     }
     s.segs = segs;
   }
-  bool unit = true;
-  for (auto& g : s.segs) unit = unit && g.alpha == 1.0;
-  s.tile_class = pick_class(s.tasks, prec, unit);
+  s.tile_class = pick_class(s.tasks, s.segs, prec);
   int BM, BN;
   tile_dims(prec, s.tile_class, &BM, &BN);
   // longest tiles first (static round-robin over the persistent grid)

@@ -96,7 +96,8 @@ def test_env_mixed_variant_matches_oracle(gpu, oracle):
 
 @pytest.mark.parametrize("prec,tol", [(native.F32, 2e-6), (native.BF16, 3e-2), (native.BF16X3, 1e-4),
                                       (native.F64 | native.PLAIN, 1e-12)])
-@pytest.mark.parametrize("model,L,N,D,seed,dense", [CASES[2], CASES[3], CASES[4], ("xxz", 20, 0, 512, 106, True)])
+@pytest.mark.parametrize("model,L,N,D,seed,dense", [CASES[2], CASES[3], CASES[4], ("xxz", 20, 0, 512, 106, True),
+                                                     ("xxz", 24, 0, 720, 107, True)])
 def test_heff2_reduced_precision_bounds(gpu, oracle, model, L, N, D, seed, dense, prec, tol):
     """FP32 inputs/accumulate: relative error <= 2e-6; BF16 inputs with FP32
     accumulate: <= 3e-2 (bf16 has an 8-bit significand, unit roundoff 2^-9);
