@@ -1078,26 +1078,6 @@ __device__ __forceinline__ const char* uni_ptr(const char* p) {
   return (const char*)(uintptr_t)(((uint64_t)hi << 32) | lo);
 }
 
-// four consecutive outputs of one row (row-major, 8-byte aligned)
-template <bool SPLIT>
-__device__ __forceinline__ void store4_rowmajor(__hip_bfloat16* p, int64_t plane, f4 x) {
-  __hip_bfloat16 h[4], l[4];
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    h[r] = __float2bfloat16(x[r]);
-    if constexpr (SPLIT) l[r] = __float2bfloat16(x[r] - __bfloat162float(h[r]));
-  }
-  uint2 u;
-  u.x = (uint32_t)__builtin_bit_cast(unsigned short, h[0]) | ((uint32_t)__builtin_bit_cast(unsigned short, h[1]) << 16);
-  u.y = (uint32_t)__builtin_bit_cast(unsigned short, h[2]) | ((uint32_t)__builtin_bit_cast(unsigned short, h[3]) << 16);
-  *reinterpret_cast<uint2*>(p) = u;
-  if constexpr (SPLIT) {
-    u.x = (uint32_t)__builtin_bit_cast(unsigned short, l[0]) | ((uint32_t)__builtin_bit_cast(unsigned short, l[1]) << 16);
-    u.y = (uint32_t)__builtin_bit_cast(unsigned short, l[2]) | ((uint32_t)__builtin_bit_cast(unsigned short, l[3]) << 16);
-    *reinterpret_cast<uint2*>(p + plane) = u;
-  }
-}
-
 template <typename TOut, bool SPLIT, bool ROWM, int D, int RA, int RB>
 __global__ __launch_bounds__(256, 1) void gemm_bf16_pipe_kernel(
     const GTask* __restrict__ tasks, int ntasks, const GSeg* __restrict__ segs, int tiles_per_vec, int total_tiles,
@@ -1113,7 +1093,11 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16_pipe_kernel(
   static_assert(P > 0, "ring period");
   static_assert(D >= 2 && RA >= D + 1 && RB >= D && (RA + RB) * SLOT <= 160 * 1024, "LDS rings");
   static_assert((D - 2) * G < 64, "vmcnt range");
-  __shared__ __attribute__((aligned(1024))) unsigned char smem[(RA + RB) * SLOT];
+  // the rings, and after the k-loop the staging area of a row-major epilogue:
+  // 128 rows of 128 bf16 per wave, 16 bytes of padding per row
+  constexpr int EP = 272;
+  constexpr int SMEM = (RA + RB) * SLOT > 4 * 128 * EP ? (RA + RB) * SLOT : 4 * 128 * EP;
+  __shared__ __attribute__((aligned(1024))) unsigned char smem[SMEM];
   const int wave = uni(threadIdx.x >> 6);
   const int wm = wave & 1, wn = wave >> 1;
   // M0 of this wave's first DMA row (16 * 4 * wave) in slot 0 of each ring
@@ -1325,46 +1309,93 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16_pipe_kernel(
     __syncthreads();  // every wave is done with the LDS rings before the next tile's loads
 
     TOut* Cb = C + (int64_t)v * c_vs + tk.c_off;
+    if constexpr (ROWM && sizeof(TOut) == 2) {
+      // Row-major bf16 output staged through LDS, one plane at a time: each
+      // wave writes its 128 x 128 block (8 bytes per lane and tile) and reads
+      // it back as 4 rows x 256 bytes per instruction, so every global store
+      // writes whole 256-byte row segments.  A wave reads back only what it
+      // wrote itself; LDS operations of one wave complete in order.
+      unsigned char* const wst = smem + wave * 128 * EP;
+      const int ldr = (n + 31) & ~31;
+      // the accumulators are read only after this point (the first fence also
+      // waits out the MFMA latency)
+      acc_fence<true>(acc[0]);
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      // row i of the accumulators is read only after this point (the first
-      // fence also waits out the MFMA latency)
-      if (i == 0) acc_fence<true>(acc[0]);
-      else acc_fence<false>(acc[i]);
+      for (int i = 1; i < 8; ++i) acc_fence<false>(acc[i]);
+#pragma unroll 1
+      for (int p = 0; p < NP; ++p) {
 #pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        if constexpr (ROWM) {
-          // transposed tile: row row0 + 128 wm + 16 i + l16, columns col0 + 128 wn + 16 j + 4 lq + r
-          const int ldr = (n + 31) & ~31;
-          const int row = row0 + wm * 128 + i * 16 + l16;
-          const int col = col0 + wn * 128 + j * 16 + lq * 4;
-          if (row < m) {
-            TOut* p = Cb + (int64_t)row * ldr + col;
-            if (col + 3 < n) {
-              if constexpr (sizeof(TOut) == 2) store4_rowmajor<SPLIT>(reinterpret_cast<__hip_bfloat16*>(p), c_plane,
-                                                                      acc[i][j]);
-              else
+        for (int i = 0; i < 8; ++i) {
+          // keeps row i's conversions after row i - 1's LDS writes (few live registers)
+          acc_fence<false>(acc[i]);
 #pragma unroll
-                for (int r = 0; r < 4; ++r) store_out<TOut, SPLIT>(p, r, c_plane, acc[i][j][r]);
+          for (int j = 0; j < 8; ++j) {
+            unsigned short h[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              const __hip_bfloat16 hi = __float2bfloat16(acc[i][j][r]);
+              h[r] = __builtin_bit_cast(unsigned short,
+                                        p == 0 ? hi : __float2bfloat16(acc[i][j][r] - __bfloat162float(hi)));
+            }
+            const uint2 u = make_uint2((uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16));
+            *reinterpret_cast<uint2*>(wst + (16 * i + l16) * EP + (16 * j + 4 * lq) * 2) = u;
+          }
+        }
+        unsigned short* const Cp = reinterpret_cast<unsigned short*>(Cb) + p * c_plane;
+#pragma unroll 4
+        for (int it = 0; it < 32; ++it) {
+          const int r = it * 4 + (lane >> 4), c8 = (lane & 15) * 8;
+          const uint4 val = *reinterpret_cast<const uint4*>(wst + r * EP + c8 * 2);
+          const int grow = row0 + wm * 128 + r, gcol = col0 + wn * 128 + c8;
+          if (grow < m) {
+            unsigned short* q = Cp + (int64_t)grow * ldr + gcol;
+            if (gcol + 7 < n) {
+              *reinterpret_cast<uint4*>(q) = val;
             } else {
+              const uint32_t w[4] = {val.x, val.y, val.z, val.w};
+#pragma unroll
+              for (int e = 0; e < 8; ++e)
+                if (gcol + e < n) q[e] = (unsigned short)(w[e >> 1] >> (16 * (e & 1)));
+            }
+          }
+        }
+      }
+      // every wave is done with the staging area before the next tile's DMA
+      asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    } else {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) {
+        // row i of the accumulators is read only after this point (the first
+        // fence also waits out the MFMA latency)
+        if (i == 0) acc_fence<true>(acc[0]);
+        else acc_fence<false>(acc[i]);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          if constexpr (ROWM) {
+            // transposed tile: row row0 + 128 wm + 16 i + l16, columns col0 + 128 wn + 16 j + 4 lq + r
+            const int ldr = (n + 31) & ~31;
+            const int row = row0 + wm * 128 + i * 16 + l16;
+            const int col = col0 + wn * 128 + j * 16 + lq * 4;
+            if (row < m) {
+              TOut* p = Cb + (int64_t)row * ldr + col;
 #pragma unroll
               for (int r = 0; r < 4; ++r)
                 if (col + r < n) store_out<TOut, SPLIT>(p, r, c_plane, acc[i][j][r]);
             }
-          }
-        } else {
-          // column col0 + 128 wn + 16 j + l16, rows row0 + 128 wm + 16 i + 4 lq + r (contiguous)
-          const int col = col0 + wn * 128 + j * 16 + l16;
-          const int row = row0 + wm * 128 + i * 16 + lq * 4;
-          if (col < n) {
-            TOut* p = Cb + row + (int64_t)col * m;
-            if (row + 3 < m) {
+          } else {
+            // column col0 + 128 wn + 16 j + l16, rows row0 + 128 wm + 16 i + 4 lq + r (contiguous)
+            const int col = col0 + wn * 128 + j * 16 + l16;
+            const int row = row0 + wm * 128 + i * 16 + lq * 4;
+            if (col < n) {
+              TOut* p = Cb + row + (int64_t)col * m;
+              if (row + 3 < m) {
 #pragma unroll
-              for (int r = 0; r < 4; ++r) store_out<TOut, SPLIT>(p, r, c_plane, acc[i][j][r]);
-            } else {
+                for (int r = 0; r < 4; ++r) store_out<TOut, SPLIT>(p, r, c_plane, acc[i][j][r]);
+              } else {
 #pragma unroll
-              for (int r = 0; r < 4; ++r)
-                if (row + r < m) store_out<TOut, SPLIT>(p, r, c_plane, acc[i][j][r]);
+                for (int r = 0; r < 4; ++r)
+                  if (row + r < m) store_out<TOut, SPLIT>(p, r, c_plane, acc[i][j][r]);
+              }
             }
           }
         }
