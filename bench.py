@@ -53,8 +53,8 @@ def time_apply(torch, plan, x, y, steps, warmup):
     return ms, st
 
 
-TILE = {0: "256x128 (4 waves)", 1: "64x64", 2: "128x128", 3: "256x256 (8 waves)"}
-TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "gemm_bf16_stream_dense_D2048_hbm_traffic.txt")
+TILE = {0: "256x128 stream, 4 waves", 1: "64x64", 2: "128x128", 3: "256x256 pipelined, 4 waves"}
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "gemm_bf16_pipe_dense_D2048_hbm_traffic.txt")
 
 
 def profiled_traffic(prec, stage):
@@ -192,12 +192,12 @@ def main():
     mf = MFMA_FLOPS_PER_ALG.get(mixed, 1.0)
     roof = {"bound": "mfma", "achieved": round(achieved * mf, 2), "peak": PEAK[mixed], "unit": "TFLOP/s",
             "frac": round(achieved * mf / PEAK[mixed], 4), "traffic": profiled_traffic(mixed, stage),
-            "kernel": "gemm_bf16_stream_kernel %s, stage %d (%s)" % (TILE.get(r["tile_class"][stage - 1], "?"),
-                                                                   stage, mixed),
+            "kernel": "gemm_bf16_%s_kernel %s, stage %d (%s)" % ("pipe" if r["tile_class"][stage - 1] == 3 else "stream",
+                                                           TILE.get(r["tile_class"][stage - 1], "?"), stage, mixed),
             "stage_ms": round(r["stage_ms"][k], 4), "stage_flops_alg": fl,
             "note": "achieved = MFMA flops of the stage (%.0fx its algorithmic flops for %s) / its average "
                     "HIP-event duration over the timed steps; traffic = HBM bytes per launch from "
-                    "profiles/gemm_bf16_stream_dense_D2048_hbm_traffic.txt" % (mf, mixed)}
+                    "profiles/gemm_bf16_pipe_dense_D2048_hbm_traffic.txt" % (mf, mixed)}
     lan = None
     if not args.no_lanczos and "f64" in res:
         lan = lanczos_compare(torch, prob, mixed, args)
