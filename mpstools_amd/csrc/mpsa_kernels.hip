@@ -1143,9 +1143,15 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16_pipe_kernel(
       for (int q = 0; q < RA; ++q) aS[q] = lds_u32(smem) + q * SLOT + wm * 128 * ROWB + foff;
 #pragma unroll
       for (int q = 0; q < RB; ++q) bS[q] = lds_u32(smem + RA * SLOT) + q * SLOT + wn * 128 * ROWB + foff;
-      // DMA cursor (group = A slab + B slab of one k-step): the plane bases of
-      // its segment, the per-lane byte offsets of its rows and 2 kk.  Past the
-      // last segment the cursor keeps the last segment's fields and restarts at k = 0.
+      // DMA cursor (group = A slab + B slab of one k-step): the tile it loads
+      // for, the plane bases of its segment, the per-lane byte offsets of its
+      // rows and 2 kk.  Past the last slab of this tile the cursor moves on to
+      // the first slabs of this workgroup's next tile: the surplus groups of the
+      // last steps then bring those into L2 (their LDS copies land in free slots
+      // and are dropped), ahead of the next tile's prologue.  Past the last
+      // segment of a tile the cursor keeps that segment's fields and restarts at k = 0.
+      int d_row0 = row0, d_col0 = col0, d_m = m, d_n = n, d_v = v, d_send = send;
+      bool moved_on = false;
       SegCursor cl;
       cur_enter(cl, segs, tk.seg0, send);
       unsigned offA[4], offB[4], kk2 = 0;
@@ -1154,13 +1160,13 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16_pipe_kernel(
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int r = 16 * (4 * wave + q) + lrow;
-          const int gi = row0 + r < m ? row0 + r : m - 1;
-          const int gj = col0 + r < n ? col0 + r : n - 1;
+          const int gi = d_row0 + r < d_m ? d_row0 + r : d_m - 1;
+          const int gj = d_col0 + r < d_n ? d_col0 + r : d_n - 1;
           offA[q] = ((unsigned)gi * (unsigned)cl.lda + (unsigned)(lchunk * 8)) * 2u;
           offB[q] = ((unsigned)gj * (unsigned)cl.ldb + (unsigned)(lchunk * 8)) * 2u;
         }
-        sa0 = uni_ptr(Ac + 2 * ((int64_t)v * a_vs + cl.a_off));
-        sb0 = uni_ptr(Bc + 2 * ((int64_t)v * b_vs + cl.b_off));
+        sa0 = uni_ptr(Ac + 2 * ((int64_t)d_v * a_vs + cl.a_off));
+        sb0 = uni_ptr(Bc + 2 * ((int64_t)d_v * b_vs + cl.b_off));
         sa1 = uni_ptr(sa0 + 2 * a_plane);
         sb1 = uni_ptr(sb0 + 2 * b_plane);
         kk2 = 0;
@@ -1172,7 +1178,26 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16_pipe_kernel(
         cl.kk += BK;
         kk2 += 2 * BK;
         if (cl.kk >= cl.k) {
-          cur_enter(cl, segs, uni(cl.si) + 1, send);
+          cur_enter(cl, segs, uni(cl.si) + 1, d_send);
+          if (cl.si >= d_send && !moved_on) {
+            moved_on = true;
+            const int g2 = g + (int)gridDim.x;
+            if (g2 < total_tiles) {
+              const int v2 = g2 / tiles_per_vec;
+              const int t2 = g2 - v2 * tiles_per_vec;
+              const GTask tk2 = tasks[find_task(tasks, ntasks, t2, &GTask::tile0)];
+              if (tk2.nseg > 0) {
+                const int local2 = t2 - tk2.tile0;
+                d_row0 = (local2 % tk2.tiles_m) * BM;
+                d_col0 = (local2 / tk2.tiles_m) * BN;
+                d_m = tk2.m;
+                d_n = tk2.n;
+                d_v = v2;
+                d_send = tk2.seg0 + tk2.nseg;
+                cur_enter(cl, segs, tk2.seg0, d_send);
+              }
+            }
+          }
           // unconditional: the descriptor load is waited for here and not at every later use
           seg_setup();
         }
@@ -1762,8 +1787,10 @@ __global__ __launch_bounds__(256) void repack_kernel(const Repack* __restrict__ 
     for (int r = 0; r < 4; ++r) {
       const int64_t e = base + threadIdx.x + r * 256;
       if (e < len) {
-        const int64_t i = e % tk.rows, j = e / tk.rows;
-        const int64_t di = tk.transpose ? (j + i * tk.dst_ld) : (i + j * tk.dst_ld);
+        // blocks hold fewer than 2^31 elements (checked when the plan is built): 32-bit index math
+        const uint32_t e32 = (uint32_t)e, rows = (uint32_t)tk.rows;
+        const uint32_t j = e32 / rows, i = e32 - j * rows;
+        const int64_t di = tk.transpose ? ((int64_t)j + (int64_t)i * tk.dst_ld) : ((int64_t)i + (int64_t)j * tk.dst_ld);
         d[di] = cvt<TO>(tk.scale * s[e]);
       }
     }
@@ -1787,8 +1814,9 @@ __global__ __launch_bounds__(256) void repack_split_kernel(const Repack* __restr
     for (int r = 0; r < 4; ++r) {
       const int64_t e = base + threadIdx.x + r * 256;
       if (e < len) {
-        const int64_t i = e % tk.rows, j = e / tk.rows;
-        const int64_t di = tk.transpose ? (j + i * tk.dst_ld) : (i + j * tk.dst_ld);
+        const uint32_t e32 = (uint32_t)e, rows = (uint32_t)tk.rows;
+        const uint32_t j = e32 / rows, i = e32 - j * rows;
+        const int64_t di = tk.transpose ? ((int64_t)j + (int64_t)i * tk.dst_ld) : ((int64_t)i + (int64_t)j * tk.dst_ld);
         const double x = tk.scale * sp[e];
         const __hip_bfloat16 h = __float2bfloat16((float)x);
         d[di] = h;

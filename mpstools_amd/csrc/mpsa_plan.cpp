@@ -314,6 +314,8 @@ static int finalize_repack(std::vector<Repack>& r) {
   std::vector<Repack> r2;
   for (auto t : r) {
     if ((int64_t)t.rows * t.cols <= 0) continue;
+    // the conversion kernels index a block with 32-bit arithmetic
+    if ((int64_t)t.rows * t.cols >= (int64_t(1) << 31)) throw Error(MPSA_EINVAL, "block too large for conversion");
     t.chunk0 = chunk;
     chunk += (int)(((int64_t)t.rows * t.cols + 1023) / 1024);
     r2.push_back(t);
