@@ -1387,6 +1387,54 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16_pipe_kernel(
       }
       // every wave is done with the staging area before the next tile's DMA
       asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    } else if constexpr (!ROWM && sizeof(TOut) == 8) {
+      // Column-major double output staged through LDS, 32 columns at a time:
+      // each wave writes 32 columns x 128 rows (32 bytes per lane and tile;
+      // columns 1040 bytes apart) and reads a whole column (1 KB) back per
+      // instruction, so every global store writes a 1 KB run of one column.
+      constexpr int CP = 1040;
+      unsigned char* const wst = smem + wave * 32 * CP;
+      acc_fence<true>(acc[0]);
+#pragma unroll
+      for (int i = 1; i < 8; ++i) acc_fence<false>(acc[i]);
+      double* const Cd = reinterpret_cast<double*>(Cb);
+#pragma unroll
+      for (int cg = 0; cg < 4; ++cg) {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+          // keeps row i's conversions after the previous LDS writes (few live registers)
+          acc_fence<false>(acc[i]);
+#pragma unroll
+          for (int jj = 0; jj < 2; ++jj) {
+            const f4 x = acc[i][2 * cg + jj];
+            double* q = reinterpret_cast<double*>(wst + (16 * jj + l16) * CP + (16 * i + 4 * lq) * 8);
+            q[0] = x[0];
+            q[1] = x[1];
+            q[2] = x[2];
+            q[3] = x[3];
+          }
+        }
+        const int grow = row0 + wm * 128 + 2 * lane;
+#pragma unroll 4
+        for (int c = 0; c < 32; ++c) {
+          const double2 val = *reinterpret_cast<const double2*>(wst + c * CP + lane * 16);
+          const int gcol = col0 + wn * 128 + 32 * cg + c;
+          if (gcol < n) {
+            double* q = Cd + grow + (int64_t)gcol * m;
+            if (grow + 1 < m) {
+              if ((((uintptr_t)q) & 15) == 0) {
+                *reinterpret_cast<double2*>(q) = val;
+              } else {
+                q[0] = val.x;
+                q[1] = val.y;
+              }
+            } else if (grow < m) {
+              q[0] = val.x;
+            }
+          }
+        }
+      }
+      asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
     } else {
 #pragma unroll
       for (int i = 0; i < 8; ++i) {
