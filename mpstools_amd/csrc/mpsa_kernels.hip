@@ -16,6 +16,8 @@
 #include <hip/hip_bf16.h>
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include "mpsa_internal.h"
 
 namespace mpsa {
@@ -1012,6 +1014,23 @@ __device__ __forceinline__ void mm8(f4 (&c)[8], const bf8& a, const bf8 (&b)[8])
                : [a] "v"(a), MPSA_B8_IN(b), [tr] "i"(TR ? 1 : 0));
 }
 
+// mm8 plus one DMA load (LDS m0b + MO, global g + v + kk)
+template <bool TR, int MO>
+__device__ __forceinline__ void mm8_ld1(f4 (&c)[8], const bf8& a, const bf8 (&b)[8], unsigned m0b, const char* g,
+                                        unsigned v, unsigned kk) {
+  unsigned t0;
+  asm volatile(MPSA_MF("c0", "a", "b0")
+               MPSA_DA("t0", "v0", "0")
+               MPSA_MF("c1", "a", "b1")
+               MPSA_LD("t0")
+               MPSA_MF("c2", "a", "b2") MPSA_MF("c3", "a", "b3") MPSA_MF("c4", "a", "b4")
+               MPSA_MF("c5", "a", "b5") MPSA_MF("c6", "a", "b6") MPSA_MF("c7", "a", "b7")
+               : MPSA_ACC8_OUT(c), [t0] "=&v"(t0)
+               : [a] "v"(a), MPSA_B8_IN(b), [m0] "s"(m0b), [mo] "i"(MO), [g] "s"(g), [v0] "v"(v), [kk] "s"(kk),
+                 [tr] "i"(TR ? 1 : 0)
+               : "m0", "scc");
+}
+
 // four DMA loads outside the MFMA stream (tile prologue)
 template <int MO>
 __device__ __forceinline__ void dma4(unsigned m0b, const char* g, const unsigned (&v)[4], unsigned kk) {
@@ -1266,25 +1285,43 @@ __global__ __launch_bounds__(256, 1) void gemm_bf16_pipe_kernel(
             // the B read of row I - 1 may stay in flight
             if constexpr (I < 7) lgkm_wait<3>(ah[(I + 1) % 4]);
           } else {
-            // rows 0 .. 3 carry the group (one plane of A or B per row, half of
-            // it in each of the first two MFMA statements): the DMA engine moves
-            // about 64 bytes a clock per CU, so a burst of the whole group would
-            // stall the MFMA stream
-            if constexpr (I < 4) {
-              constexpr bool LB = I >= 2;
-              constexpr int MO = (LB ? IB : IA) * SLOT + (I & 1) * PLANE;
-              const unsigned m0x = LB ? m0b0 : m0a0;
-              const char* const gx = LB ? ((I & 1) ? sb1 : sb0) : ((I & 1) ? sa1 : sa0);
-              const unsigned(&ox)[4] = LB ? offB : offA;
-              mm8_rd2_ld2<ROWM, AO, AO + PLANE, MO>(acc[I], ah[R], bh, ah[RN], aad, al[RN], aad, m0x, gx, ox[0],
-                                                    ox[1], kk);
-              mm8_rd2_ld2<ROWM, I * 1024, I * 1024 + PLANE, MO + 2048>(acc[I], al[R], bh, nh[I], b_nxt, nl[I], b_nxt,
-                                                                      m0x, gx, ox[2], ox[3], kk);
+            // rows 0 .. 5 carry the group, one load per MFMA statement (load E of
+            // the group in chunk E % 3 of row E / 3): the DMA path costs the
+            // issuing wave cycles per load, so the loads are spread thin
+            constexpr int E0 = 3 * I, E1 = 3 * I + 1, E2 = 3 * I + 2;
+            auto ld = [&](auto EC) {
+              constexpr int E = decltype(EC)::value;
+              constexpr bool LB = E >= 8;
+              constexpr int p = (E / 4) % 2, q = E % 4;
+              constexpr int MO = (LB ? IB : IA) * SLOT + p * PLANE + q * 1024;
+              struct L {
+                unsigned m0;
+                const char* g;
+                unsigned v;
+              };
+              return std::pair<IC<MO>, L>{IC<MO>{}, L{LB ? m0b0 : m0a0, LB ? (p ? sb1 : sb0) : (p ? sa1 : sa0),
+                                                        LB ? offB[q] : offA[q]}};
+            };
+            if constexpr (E0 < 16) {
+              const auto l0 = ld(IC<E0>{});
+              mm8_rd2_ld1<ROWM, AO, AO + PLANE, decltype(l0.first)::value>(acc[I], ah[R], bh, ah[RN], aad, al[RN], aad,
+                                                                           l0.second.m0, l0.second.g, l0.second.v, kk);
             } else {
               mm8_rd2<ROWM, AO, AO + PLANE>(acc[I], ah[R], bh, ah[RN], aad, al[RN], aad);
+            }
+            if constexpr (E1 < 16) {
+              const auto l1 = ld(IC<E1>{});
+              mm8_rd2_ld1<ROWM, I * 1024, I * 1024 + PLANE, decltype(l1.first)::value>(
+                  acc[I], al[R], bh, nh[I], b_nxt, nl[I], b_nxt, l1.second.m0, l1.second.g, l1.second.v, kk);
+            } else {
               mm8_rd2<ROWM, I * 1024, I * 1024 + PLANE>(acc[I], al[R], bh, nh[I], b_nxt, nl[I], b_nxt);
             }
-            mm8<ROWM>(acc[I], ah[R], bl);
+            if constexpr (E2 < 16) {
+              const auto l2 = ld(IC<E2>{});
+              mm8_ld1<ROWM, decltype(l2.first)::value>(acc[I], ah[R], bl, l2.second.m0, l2.second.g, l2.second.v, kk);
+            } else {
+              mm8<ROWM>(acc[I], ah[R], bl);
+            }
             // row I + 1's A fragments came from row I - 1; the reads of row I and
             // the B reads of row I - 1 may stay in flight
             if constexpr (I < 7) lgkm_wait<6>(ah[(I + 1) % 4], al[(I + 1) % 4]);
