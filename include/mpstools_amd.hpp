@@ -603,11 +603,12 @@ class TSLanzcosEngine {
     _es.resize((size_t)nr);
     _plan_keep.reset(new PlanHolder{plan});
     _dx.reset(new DevBuf((size_t)_space.total));
-    hipMemcpy(_dx->p, dx.p, sizeof(double) * (size_t)_space.total, hipMemcpyDeviceToDevice);
     _keepL.reset(new DevBuf((size_t)tl.total));
     _keepR.reset(new DevBuf((size_t)tr.total));
-    hipMemcpy(_keepL->p, dL.p, sizeof(double) * (size_t)tl.total, hipMemcpyDeviceToDevice);
-    hipMemcpy(_keepR->p, dR.p, sizeof(double) * (size_t)tr.total, hipMemcpyDeviceToDevice);
+    if (hipMemcpy(_dx->p, dx.p, sizeof(double) * (size_t)_space.total, hipMemcpyDeviceToDevice) != hipSuccess ||
+        hipMemcpy(_keepL->p, dL.p, sizeof(double) * (size_t)tl.total, hipMemcpyDeviceToDevice) != hipSuccess ||
+        hipMemcpy(_keepR->p, dR.p, sizeof(double) * (size_t)tr.total, hipMemcpyDeviceToDevice) != hipSuccess)
+      mps_exception("TSLanzcosEngine: device copy failed");
     mpsa_check(mpsa_plan_bind_env(plan, _keepL->p, _keepR->p, nullptr), "TSLanzcosEngine");
     if (_verbose == 1 && !_es.empty()) std::cout << "E=" << _es[0] << " after " << nit << " iterations; ";
     return nit;

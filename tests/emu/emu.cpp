@@ -1,0 +1,711 @@
+// emu.cpp -- CPU emulation of the device layer under the host drivers of
+// libmpstools_amd (test infrastructure only; never part of the gfx950 library).
+//
+// Linked with the library's own host sources (mpsa_plan.cpp, mpsa_dmrg.cpp,
+// mpsa_split.cpp, mpsa_lanczos.cpp, mpsa_adjoint.cpp) in place of the HIP
+// runtime, the gfx950 kernels (mpsa_kernels.hip) and rocSOLVER, so that the
+// host logic -- plan building, descriptor upload, buffer lifetimes, the
+// sweep's control flow -- runs unchanged on the CPU:
+//   * device memory is host memory registered per allocation, filled with a
+//     NaN byte pattern (reads of never-written memory poison the results);
+//     freed allocations are kept, poisoned, so later accesses are reported as
+//     use after free;
+//   * each launcher computes what its kernel computes and checks every address
+//     the kernel touches (the FP64/FP32 GEMM kernel clamps its loads into the
+//     logical block, so its physical and logical extents agree; a segment with
+//     k <= 0 or a tile of an empty task, whose clamped address would be -1, is
+//     reported);
+//   * streams run synchronously; stream capture records the launches into a
+//     graph that hipGraphLaunch replays;
+//   * rocsolver_dgesvd checks its documented array extents and calls LAPACK
+//     dgesvd (loaded at run time, like the oracle's BLAS).
+// The BF16 kernels are not emulated (their launch is reported).
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+
+#include <dlfcn.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "mpsa_internal.h"
+
+namespace {
+struct Alloc {
+  size_t bytes;
+  bool live;
+  long serial;
+};
+std::map<uintptr_t, Alloc> g_alloc;
+long g_serial = 0, g_viol = 0, g_launch = 0;
+std::string g_ctx = "-";
+std::vector<std::string> g_msgs;
+
+void report(const std::string& m) {
+  ++g_viol;
+  if (g_msgs.size() < 200) g_msgs.push_back("[launch " + std::to_string(g_launch) + " " + g_ctx + "] " + m);
+  if (g_viol <= 20) fprintf(stderr, "EMU VIOLATION [launch %ld %s] %s\n", g_launch, g_ctx.c_str(), m.c_str());
+}
+
+// [p, p + bytes) inside one live allocation?
+bool check(const void* p, size_t bytes, const char* what) {
+  if (bytes == 0) return true;
+  const uintptr_t a = (uintptr_t)p;
+  auto it = g_alloc.upper_bound(a);
+  if (it != g_alloc.begin()) {
+    --it;
+    const uintptr_t b = it->first;
+    if (a >= b && a + bytes <= b + it->second.bytes) {
+      if (!it->second.live) {
+        report(std::string(what) + ": use after free (alloc #" + std::to_string(it->second.serial) + ")");
+        return false;
+      }
+      return true;
+    }
+    if (a >= b && a < b + it->second.bytes + 4096) {
+      char buf[256];
+      snprintf(buf, sizeof buf, "%s: out of bounds, offset %ld bytes %zu in alloc #%ld of %zu bytes%s", what,
+               (long)(a - b), bytes, it->second.serial, it->second.bytes, it->second.live ? "" : " (freed)");
+      report(buf);
+      return false;
+    }
+  }
+  auto nx = g_alloc.lower_bound(a);
+  if (nx != g_alloc.end() && nx->first < a + 4096 + bytes) {
+    char buf[256];
+    snprintf(buf, sizeof buf, "%s: before the start of alloc #%ld by %ld bytes", what, nx->second.serial,
+             (long)(nx->first - a));
+    report(buf);
+    return false;
+  }
+  report(std::string(what) + ": not device memory");
+  return false;
+}
+template <typename T>
+bool chk_range(const T* base, int64_t lo, int64_t hi_incl, const char* what) {  // elements [lo, hi]
+  if (hi_incl < lo) return true;
+  return check(base + lo, sizeof(T) * (size_t)(hi_incl - lo + 1), what);
+}
+bool is_dev(const void* p) {
+  const uintptr_t a = (uintptr_t)p;
+  auto it = g_alloc.upper_bound(a);
+  if (it == g_alloc.begin()) return false;
+  --it;
+  return a >= it->first && a < it->first + it->second.bytes;
+}
+
+// streams, capture, graphs
+struct Stream {
+  bool capturing = false;
+  std::vector<std::function<void()>> cap;
+};
+struct Graph {
+  std::vector<std::function<void()>> ops;
+};
+Stream g_null;
+Stream* S(hipStream_t s) { return s ? reinterpret_cast<Stream*>(s) : &g_null; }
+void submit(hipStream_t st, const char* name, std::function<void()> f) {
+  Stream* s = S(st);
+  std::string nm = name;
+  auto wrapped = [f, nm]() {
+    ++g_launch;
+    g_ctx = nm;
+    f();
+    g_ctx = "-";
+  };
+  if (s->capturing) s->cap.push_back(wrapped);
+  else wrapped();
+}
+}  // namespace
+
+extern "C" {
+long emu_violations() { return g_viol; }
+const char* emu_message(int i) { return i < (int)g_msgs.size() ? g_msgs[(size_t)i].c_str() : nullptr; }
+long emu_launches() { return g_launch; }
+long emu_live_bytes() {
+  long b = 0;
+  for (auto& kv : g_alloc)
+    if (kv.second.live) b += (long)kv.second.bytes;
+  return b;
+}
+}
+
+// ---------------------------------------------------------------------------
+// HIP runtime
+// ---------------------------------------------------------------------------
+hipError_t hipMalloc(void** p, size_t bytes) {
+  if (!p) return hipErrorInvalidValue;
+  const size_t b = bytes ? bytes : 1;
+  void* m = aligned_alloc(256, (b + 255) / 256 * 256);
+  memset(m, 0xff, (b + 255) / 256 * 256);  // NaN pattern: reads of never-written memory show up
+  g_alloc[(uintptr_t)m] = Alloc{b, true, ++g_serial};
+  *p = m;
+  return hipSuccess;
+}
+hipError_t hipFree(void* p) {
+  if (!p) return hipSuccess;
+  auto it = g_alloc.find((uintptr_t)p);
+  if (it == g_alloc.end()) {
+    report("hipFree of a pointer that is not an allocation");
+    return hipErrorInvalidValue;
+  }
+  if (!it->second.live) {
+    report("double hipFree");
+    return hipErrorInvalidValue;
+  }
+  it->second.live = false;
+  memset(p, 0xfe, it->second.bytes);
+  return hipSuccess;
+}
+static hipError_t emu_copy(void* dst, const void* src, size_t n, hipMemcpyKind k) {
+  const bool dd = k == hipMemcpyDeviceToDevice || k == hipMemcpyHostToDevice || (k == hipMemcpyDefault && is_dev(dst));
+  const bool sd = k == hipMemcpyDeviceToDevice || k == hipMemcpyDeviceToHost || (k == hipMemcpyDefault && is_dev(src));
+  bool ok = true;
+  if (dd) ok = check(dst, n, "memcpy dst") && ok;
+  if (sd) ok = check(src, n, "memcpy src") && ok;
+  if (ok && n) memmove(dst, src, n);
+  return hipSuccess;
+}
+hipError_t hipMemcpy(void* dst, const void* src, size_t n, hipMemcpyKind k) {
+  if (g_null.capturing) return hipErrorStreamCaptureUnsupported;
+  return emu_copy(dst, src, n, k);
+}
+hipError_t hipMemcpyAsync(void* dst, const void* src, size_t n, hipMemcpyKind k, hipStream_t st) {
+  submit(st, "memcpyAsync", [=]() { (void)emu_copy(dst, src, n, k); });
+  return hipSuccess;
+}
+hipError_t hipMemset(void* p, int v, size_t n) {
+  if (check(p, n, "memset")) memset(p, v, n);
+  return hipSuccess;
+}
+hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) {
+  *s = reinterpret_cast<hipStream_t>(new Stream());
+  return hipSuccess;
+}
+hipError_t hipStreamDestroy(hipStream_t s) {
+  if (S(s) != &g_null) delete S(s);
+  return hipSuccess;
+}
+hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode) {
+  S(s)->capturing = true;
+  S(s)->cap.clear();
+  return hipSuccess;
+}
+hipError_t hipStreamEndCapture(hipStream_t s, hipGraph_t* g) {
+  Graph* G = new Graph();
+  G->ops = S(s)->cap;
+  S(s)->cap.clear();
+  S(s)->capturing = false;
+  *g = reinterpret_cast<hipGraph_t>(G);
+  return hipSuccess;
+}
+hipError_t hipGraphInstantiate(hipGraphExec_t* e, hipGraph_t g, hipGraphNode_t*, char*, size_t) {
+  Graph* G = new Graph(*reinterpret_cast<Graph*>(g));
+  *e = reinterpret_cast<hipGraphExec_t>(G);
+  return hipSuccess;
+}
+hipError_t hipGraphLaunch(hipGraphExec_t e, hipStream_t st) {
+  for (auto& f : reinterpret_cast<Graph*>(e)->ops) {
+    if (S(st)->capturing) S(st)->cap.push_back(f);
+    else f();
+  }
+  return hipSuccess;
+}
+hipError_t hipGraphExecDestroy(hipGraphExec_t e) {
+  delete reinterpret_cast<Graph*>(e);
+  return hipSuccess;
+}
+hipError_t hipGraphDestroy(hipGraph_t g) {
+  delete reinterpret_cast<Graph*>(g);
+  return hipSuccess;
+}
+hipError_t hipEventCreate(hipEvent_t* e) {
+  *e = reinterpret_cast<hipEvent_t>(new int(0));
+  return hipSuccess;
+}
+hipError_t hipEventDestroy(hipEvent_t e) {
+  delete reinterpret_cast<int*>(e);
+  return hipSuccess;
+}
+hipError_t hipEventRecord(hipEvent_t, hipStream_t) { return hipSuccess; }
+hipError_t hipEventSynchronize(hipEvent_t) { return hipSuccess; }
+hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) {
+  *ms = 0.f;
+  return hipSuccess;
+}
+hipError_t hipGetDevice(int* d) {
+  *d = 0;
+  return hipSuccess;
+}
+const char* hipGetErrorString(hipError_t e) { return e == hipSuccess ? "no error" : "emulated HIP error"; }
+
+// ---------------------------------------------------------------------------
+// rocBLAS handle and rocSOLVER dgesvd (LAPACK dgesvd underneath)
+// ---------------------------------------------------------------------------
+typedef void (*dgesvd_fn)(const char*, const char*, const int*, const int*, double*, const int*, double*, double*,
+                          const int*, double*, const int*, double*, const int*, int*);
+// LAPACK dgesvd_ from the first library found (MKL or the system LAPACK)
+static dgesvd_fn lapack_dgesvd() {
+  static dgesvd_fn f = nullptr;
+  static bool tried = false;
+  if (!tried) {
+    tried = true;
+    const char* cands[] = {getenv("EMU_LAPACK"), "libmkl_rt.so", "/opt/conda/lib/libmkl_rt.so", "libmkl_rt.so.1",
+                           "liblapack.so.3", "libopenblas.so.0"};
+    for (const char* c : cands) {
+      if (!c) continue;
+      void* h = dlopen(c, RTLD_NOW | RTLD_LOCAL);
+      if (h && (f = (dgesvd_fn)dlsym(h, "dgesvd_"))) break;
+    }
+  }
+  return f;
+}
+rocblas_status rocblas_create_handle(rocblas_handle* h) {
+  *h = reinterpret_cast<rocblas_handle>(new int(1));
+  return rocblas_status_success;
+}
+rocblas_status rocblas_set_stream(rocblas_handle, hipStream_t) { return rocblas_status_success; }
+rocblas_status rocsolver_dgesvd(rocblas_handle, const rocblas_svect ls, const rocblas_svect rs, const rocblas_int m,
+                                const rocblas_int n, double* A, const rocblas_int lda, double* Sg, double* U,
+                                const rocblas_int ldu, double* V, const rocblas_int ldv, double* E,
+                                const rocblas_workmode, rocblas_int* info) {
+  submit(nullptr, "rocsolver_dgesvd", [=]() {
+    const int k = std::min(m, n);
+    if (ls != rocblas_svect_singular || rs != rocblas_svect_singular || m < 0 || n < 0 || lda < std::max(1, m) ||
+        ldu < std::max(1, m) || ldv < std::max(1, k)) {
+      report("gesvd: unsupported or inconsistent arguments");
+      return;
+    }
+    bool ok = chk_range(A, 0, (int64_t)lda * (n - 1) + m - 1, "gesvd A");
+    ok = chk_range(Sg, 0, k - 1, "gesvd S") && ok;
+    ok = chk_range(U, 0, (int64_t)ldu * (k - 1) + m - 1, "gesvd U") && ok;
+    ok = chk_range(V, 0, (int64_t)ldv * (n - 1) + k - 1, "gesvd V") && ok;
+    if (k > 1) ok = chk_range(E, 0, k - 2, "gesvd E") && ok;
+    ok = check(info, sizeof(rocblas_int), "gesvd info") && ok;
+    if (!ok || k == 0) return;
+    dgesvd_fn dgesvd_ = lapack_dgesvd();
+    if (!dgesvd_) {
+      report("gesvd: no LAPACK library found");
+      return;
+    }
+    int M = m, N = n, LDA = lda, LDU = ldu, LDV = ldv, lwork = -1, inf = 0;
+    double wq = 0;
+    dgesvd_("S", "S", &M, &N, A, &LDA, Sg, U, &LDU, V, &LDV, &wq, &lwork, &inf);
+    lwork = (int)wq;
+    std::vector<double> work((size_t)std::max(lwork, 1));
+    dgesvd_("S", "S", &M, &N, A, &LDA, Sg, U, &LDU, V, &LDV, work.data(), &lwork, &inf);
+    *info = inf;
+  });
+  return rocblas_status_success;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+namespace mpsa {
+
+// find_task of the kernels, with its descriptor reads checked
+template <typename Task>
+static int find_task(const Task* tasks, int ntasks, int t, int Task::*field) {
+  int lo = 0, hi = ntasks - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (!check(tasks + mid, sizeof(Task), "task table")) return 0;
+    if (tasks[mid].*field <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <typename T, typename TOut>
+static void gemm16(int BM, int BN, bool TA, bool TB, const GTask* tasks, int ntasks, const GSeg* segs, int tpv,
+                   int nvec, const T* A, int64_t a_vs, const T* B, int64_t b_vs, TOut* C, int64_t c_vs) {
+  const int total = tpv * nvec;
+  for (int g = 0; g < total; ++g) {
+    const int v = g / tpv, t = g - v * tpv;
+    if (ntasks < 1) {
+      report("gemm: tiles but no tasks");
+      return;
+    }
+    const int ti = find_task(tasks, ntasks, t, &GTask::tile0);
+    if (!check(tasks + ti, sizeof(GTask), "gemm task")) return;
+    const GTask tk = tasks[ti];
+    if (tk.tiles_m <= 0) {
+      report("gemm: task with tiles_m <= 0 reached");
+      return;
+    }
+    const int local = t - tk.tile0, tm = local % tk.tiles_m, tn = local / tk.tiles_m;
+    const int row0 = tm * BM, col0 = tn * BN, m = tk.m, n = tk.n;
+    if (m <= 0 || n <= 0) {
+      report("gemm: tile of an empty task (clamped address -1)");
+      return;
+    }
+    const int r1 = std::min(row0 + BM, m), c1 = std::min(col0 + BN, n);
+    std::vector<double> acc((size_t)BM * BN, 0.0);
+    const T* Av = A + (int64_t)v * a_vs;
+    const T* Bv = B + (int64_t)v * b_vs;
+    for (int s = tk.seg0; s < tk.seg0 + tk.nseg; ++s) {
+      if (!check(segs + s, sizeof(GSeg), "gemm segment")) return;
+      const GSeg sg = segs[s];
+      if (sg.k <= 0) {
+        report("gemm: segment with k <= 0 (the kernel reads at clamped index -1)");
+        return;
+      }
+      // physical reads: rows clamp(row0 .. row0+BM-1) in [row0, m-1] (row0 < m), k in [0, k-1]
+      const int rlo = std::min(row0, m - 1), rhi = m - 1 < row0 + BM - 1 ? m - 1 : row0 + BM - 1;
+      const int clo = std::min(col0, n - 1), chi = n - 1 < col0 + BN - 1 ? n - 1 : col0 + BN - 1;
+      const T* Ab = Av + sg.a_off;
+      const T* Bb = Bv + sg.b_off;
+      bool ok = TA ? chk_range(Ab, (int64_t)rlo * sg.lda, (int64_t)rhi * sg.lda + sg.k - 1, "gemm A")
+                   : chk_range(Ab, rlo, (int64_t)(sg.k - 1) * sg.lda + rhi, "gemm A");
+      ok = (TB ? chk_range(Bb, clo, (int64_t)(sg.k - 1) * sg.ldb + chi, "gemm B")
+               : chk_range(Bb, (int64_t)clo * sg.ldb, (int64_t)chi * sg.ldb + sg.k - 1, "gemm B")) &&
+           ok;
+      if (!ok) return;
+      for (int j = col0; j < c1; ++j)
+        for (int i = row0; i < r1; ++i) {
+          double a = 0;
+          for (int kk = 0; kk < sg.k; ++kk) {
+            const double x = TA ? (double)Ab[kk + (int64_t)i * sg.lda] : (double)Ab[i + (int64_t)kk * sg.lda];
+            const double y = TB ? (double)Bb[j + (int64_t)kk * sg.ldb] : (double)Bb[kk + (int64_t)j * sg.ldb];
+            a += x * y;
+          }
+          acc[(size_t)(j - col0) * BM + (i - row0)] += sg.alpha * a;
+        }
+    }
+    TOut* Cb = C + (int64_t)v * c_vs + tk.c_off;
+    if (!chk_range(Cb, row0 + (int64_t)col0 * m, (r1 - 1) + (int64_t)(c1 - 1) * m, "gemm C")) return;
+    for (int j = col0; j < c1; ++j)
+      for (int i = row0; i < r1; ++i) Cb[i + (int64_t)j * m] = (TOut)acc[(size_t)(j - col0) * BM + (i - row0)];
+  }
+}
+
+hipError_t launch_gemm(int prec_in, int out_kind, int tr, int tile_class, const GTask* tasks, int ntasks,
+                       const GSeg* segs, int tiles_per_vec, int nvec, const void* A, int64_t a_vs, const void* B,
+                       int64_t b_vs, void* C, int64_t c_vs, hipStream_t st, Planes) {
+  if (tiles_per_vec * nvec <= 0) return hipSuccess;
+  const int bm = tile_class == 0 ? 128 : 64;
+  const bool TA = (tr >> 1) & 1, TB = tr & 1;
+  if (prec_in == F64) {
+    submit(st, "gemm f64", [=]() {
+      gemm16<double, double>(bm, bm, TA, TB, tasks, ntasks, segs, tiles_per_vec, nvec, (const double*)A, a_vs,
+                             (const double*)B, b_vs, (double*)C, c_vs);
+    });
+    return hipSuccess;
+  }
+  if (prec_in == F32) {
+    submit(st, "gemm f32", [=]() {
+      if (out_kind == 1)
+        gemm16<float, double>(bm, bm, TA, TB, tasks, ntasks, segs, tiles_per_vec, nvec, (const float*)A, a_vs,
+                              (const float*)B, b_vs, (double*)C, c_vs);
+      else
+        gemm16<float, float>(bm, bm, TA, TB, tasks, ntasks, segs, tiles_per_vec, nvec, (const float*)A, a_vs,
+                             (const float*)B, b_vs, (float*)C, c_vs);
+    });
+    return hipSuccess;
+  }
+  submit(st, "gemm bf16", []() { report("bf16 GEMM is not emulated"); });
+  return hipSuccess;
+}
+
+hipError_t launch_mix(int prec, const MixTask* tasks, int ntasks, const MixTerm* terms, int chunks_per_vec, int nvec,
+                      const void* src, void* dst, int64_t vs, hipStream_t st, int64_t) {
+  const int total = chunks_per_vec * nvec;
+  if (total <= 0 || ntasks <= 0) return hipSuccess;
+  if (prec != F64 && prec != F32) {
+    submit(st, "mix bf16", []() { report("bf16 mix is not emulated"); });
+    return hipSuccess;
+  }
+  submit(st, "mix", [=]() {
+    const size_t es = prec == F64 ? 8 : 4;
+    for (int g = 0; g < total; ++g) {
+      const int v = g / chunks_per_vec, c = g - v * chunks_per_vec;
+      const int ti = find_task(tasks, ntasks, c, &MixTask::chunk0);
+      if (!check(tasks + ti, sizeof(MixTask), "mix task")) return;
+      const MixTask tk = tasks[ti];
+      const int64_t base = (int64_t)(c - tk.chunk0) * 1024;
+      const int64_t e1 = std::min<int64_t>(base + 1024, tk.len);
+      if (e1 <= base) continue;
+      if (!chk_range(terms, tk.t0, (int64_t)tk.t0 + tk.nt - 1, "mix terms")) return;
+      const char* sv = (const char*)src + es * (size_t)(v * vs);
+      char* dv = (char*)dst + es * (size_t)(v * vs + tk.dst_off);
+      if (!check(dv + es * base, es * (size_t)(e1 - base), "mix dst")) return;
+      for (int q = 0; q < tk.nt; ++q)
+        if (!check(sv + es * (size_t)(terms[tk.t0 + q].src_off + base), es * (size_t)(e1 - base), "mix src")) return;
+      for (int64_t e = base; e < e1; ++e) {
+        double acc = 0;
+        for (int q = 0; q < tk.nt; ++q) {
+          const MixTerm tm = terms[tk.t0 + q];
+          acc += tm.coef * (prec == F64 ? ((const double*)sv)[tm.src_off + e] : (double)((const float*)sv)[tm.src_off + e]);
+        }
+        if (prec == F64) ((double*)dv)[e] = acc;
+        else ((float*)dv)[e] = (float)acc;
+      }
+    }
+  });
+  return hipSuccess;
+}
+
+hipError_t launch_convert(int prec, const double* src, void* dst, int64_t n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  submit(st, "convert", [=]() {
+    const size_t es = prec == F32 ? 4 : prec == BF16 ? 2 : 8;
+    if (!check(src, 8 * (size_t)n, "convert src") || !check(dst, es * (size_t)n, "convert dst")) return;
+    for (int64_t i = 0; i < n; ++i) {
+      if (prec == F32) ((float*)dst)[i] = (float)src[i];
+      else if (prec == F64) ((double*)dst)[i] = src[i];
+    }
+  });
+  return hipSuccess;
+}
+
+hipError_t launch_repack(int prec, const Repack* tasks, int ntasks, int chunks_per_vec, int nvec, const double* src,
+                         int64_t s_vs, void* dst, int64_t d_vs, hipStream_t st, int64_t plane) {
+  const int total = chunks_per_vec * nvec;
+  if (total <= 0 || ntasks <= 0) return hipSuccess;
+  submit(st, "repack", [=]() {
+    const size_t es = prec == F32 ? 4 : (prec == BF16 || prec == BF16X3) ? 2 : 8;
+    for (int g = 0; g < total; ++g) {
+      const int v = g / chunks_per_vec, c = g - v * chunks_per_vec;
+      const int ti = find_task(tasks, ntasks, c, &Repack::chunk0);
+      if (!check(tasks + ti, sizeof(Repack), "repack task")) return;
+      const Repack tk = tasks[ti];
+      const int64_t len = (int64_t)tk.rows * tk.cols, base = (int64_t)(c - tk.chunk0) * 1024;
+      const int64_t e1 = std::min<int64_t>(base + 1024, len);
+      if (e1 <= base) continue;
+      const double* s = src + v * s_vs + tk.src_off;
+      char* d = (char*)dst + es * (size_t)(v * d_vs + tk.dst_off);
+      if (!chk_range(s, base, e1 - 1, "repack src")) return;
+      for (int64_t e = base; e < e1; ++e) {
+        const int64_t j = e / tk.rows, i = e - j * tk.rows;
+        const int64_t di = tk.transpose ? (j + i * tk.dst_ld) : (i + j * tk.dst_ld);
+        if (!check(d + es * (size_t)di, es, "repack dst")) return;
+        if (prec == BF16X3 && !check(d + es * (size_t)(di + plane), es, "repack dst lo plane")) return;
+        const double x = tk.scale * s[e];
+        if (prec == F32) *(float*)(d + 4 * di) = (float)x;
+        else if (prec == F64) *(double*)(d + 8 * di) = x;
+      }
+    }
+  });
+  return hipSuccess;
+}
+
+hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  submit(st, "fill", [=]() {
+    if (chk_range(p, 0, n - 1, "fill"))
+      for (int64_t i = 0; i < n; ++i) p[i] = v;
+  });
+  return hipSuccess;
+}
+hipError_t launch_fill_strided(double* p, int64_t n, int64_t stride, double v, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  submit(st, "fill_strided", [=]() {
+    for (int64_t i = 0; i < n; ++i)
+      if (check(p + i * stride, 8, "fill_strided")) p[i * stride] = v;
+  });
+  return hipSuccess;
+}
+hipError_t launch_zero_ranges(const Range* r, int nr, double* base, int64_t vs, int nvec, hipStream_t st) {
+  if (nr <= 0 || nvec <= 0) return hipSuccess;
+  submit(st, "zero_ranges", [=]() {
+    if (!chk_range(r, 0, nr - 1, "zero_ranges table")) return;
+    for (int z = 0; z < nvec; ++z)
+      for (int y = 0; y < nr; ++y) {
+        double* b = base + z * vs + r[y].off;
+        if (chk_range(b, 0, r[y].len - 1, "zero_ranges"))
+          for (int64_t i = 0; i < r[y].len; ++i) b[i] = 0.0;
+      }
+  });
+  return hipSuccess;
+}
+hipError_t launch_dot(const double* x, const double* y, int64_t n, int cplx, double* partial, int nparts, double* out2,
+                      hipStream_t st) {
+  submit(st, "dot", [=]() {
+    const int64_t len = cplx ? 2 * n : n;
+    if (!chk_range(x, 0, len - 1, "dot x") || !chk_range(y, 0, len - 1, "dot y") ||
+        !chk_range(partial, 0, 2 * (int64_t)nparts - 1, "dot partial") || !chk_range(out2, 0, 1, "dot out"))
+      return;
+    double ar = 0, ai = 0;
+    for (int64_t i = 0; i < n; ++i) {
+      if (!cplx) ar += y[i] * x[i];
+      else {
+        ar += y[2 * i] * x[2 * i] + y[2 * i + 1] * x[2 * i + 1];
+        ai += y[2 * i] * x[2 * i + 1] - y[2 * i + 1] * x[2 * i];
+      }
+    }
+    for (int p = 0; p < 2 * nparts; ++p) partial[p] = 0;
+    partial[0] = ar;
+    partial[1] = ai;
+    out2[0] = ar;
+    out2[1] = ai;
+  });
+  return hipSuccess;
+}
+hipError_t launch_axpby(double ar, double ai, const double* x, double br, double bi, double* y, int64_t n, int cplx,
+                        hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  submit(st, "axpby", [=]() {
+    const int64_t len = cplx ? 2 * n : n;
+    if (!chk_range(x, 0, len - 1, "axpby x") || !chk_range(y, 0, len - 1, "axpby y")) return;
+    for (int64_t i = 0; i < n; ++i) {
+      if (!cplx) y[i] = ar * x[i] + br * y[i];
+      else {
+        const double xr = x[2 * i], xi = x[2 * i + 1], yr = y[2 * i], yi = y[2 * i + 1];
+        y[2 * i] = ar * xr - ai * xi + br * yr - bi * yi;
+        y[2 * i + 1] = ar * xi + ai * xr + br * yi + bi * yr;
+      }
+    }
+  });
+  return hipSuccess;
+}
+hipError_t launch_bdot(const double* x, const double* y, int64_t n, int64_t vs, int nvec, double* partial, int nparts,
+                       double* out, int ld, const int* step, int root, hipStream_t st) {
+  submit(st, "bdot", [=]() {
+    if (!check(step, 4, "bdot step") || !chk_range(partial, 0, (int64_t)nvec * nparts - 1, "bdot partial")) return;
+    const int j = *step;
+    for (int v = 0; v < nvec; ++v) {
+      if (!chk_range(x + v * vs, 0, n - 1, "bdot x") || !chk_range(y + v * vs, 0, n - 1, "bdot y")) return;
+      double a = 0;
+      for (int64_t i = 0; i < n; ++i) a += x[v * vs + i] * y[v * vs + i];
+      if (!check(out + (int64_t)v * ld + j, 8, "bdot out")) return;
+      out[(int64_t)v * ld + j] = root ? std::sqrt(a) : a;
+    }
+  });
+  return hipSuccess;
+}
+hipError_t launch_lz_normalize(const double* W, double* X, double* const* basis, const double* kbuf, int ld,
+                               const int* step, int64_t n, int64_t vs, int nvec, hipStream_t st) {
+  submit(st, "lz_normalize", [=]() {
+    if (!check(step, 4, "lz step")) return;
+    const int j = *step;
+    if (!check(basis + j, 8, "lz basis table")) return;
+    for (int v = 0; v < nvec; ++v) {
+      if (!check(kbuf + (int64_t)v * ld + j, 8, "lz kbuf")) return;
+      const double r = 1.0 / kbuf[(int64_t)v * ld + j];
+      const int64_t o = (int64_t)v * vs;
+      double* Bv = basis[j] + o;
+      if (!chk_range(W + o, 0, n - 1, "lz W") || !chk_range(X + o, 0, n - 1, "lz X") ||
+          !chk_range(Bv, 0, n - 1, "lz basis"))
+        return;
+      for (int64_t i = 0; i < n; ++i) X[o + i] = Bv[i] = r * W[o + i];
+    }
+  });
+  return hipSuccess;
+}
+hipError_t launch_lz_update(const double* Y, const double* X, double* const* basis, const double* kbuf,
+                            const double* ebuf, int ld, const int* step, double* W, int64_t n, int64_t vs, int nvec,
+                            hipStream_t st) {
+  submit(st, "lz_update", [=]() {
+    if (!check(step, 4, "lz step")) return;
+    const int j = *step;
+    if (j > 0 && !check(basis + j - 1, 8, "lz basis table")) return;
+    for (int v = 0; v < nvec; ++v) {
+      const int64_t o = (int64_t)v * vs;
+      if (!check(ebuf + (int64_t)v * ld + j, 8, "lz ebuf") || !check(kbuf + (int64_t)v * ld + j, 8, "lz kbuf")) return;
+      const double e = ebuf[(int64_t)v * ld + j], k = kbuf[(int64_t)v * ld + j];
+      const double* A0 = j > 0 ? basis[j - 1] + o : nullptr;
+      if (!chk_range(Y + o, 0, n - 1, "lz Y") || !chk_range(X + o, 0, n - 1, "lz X") ||
+          !chk_range(W + o, 0, n - 1, "lz W") || (A0 && !chk_range(A0, 0, n - 1, "lz A0")))
+        return;
+      for (int64_t i = 0; i < n; ++i) {
+        double t = -e * X[o + i] + Y[o + i];
+        if (A0) t = t + -k * A0[i];
+        W[o + i] = t;
+      }
+    }
+  });
+  return hipSuccess;
+}
+hipError_t launch_lz_step(int* step, hipStream_t st) {
+  submit(st, "lz_step", [=]() {
+    if (check(step, 4, "lz step")) *step += 1;
+  });
+  return hipSuccess;
+}
+hipError_t launch_lz_combine(double* const* basis, const double* c, int ldc, const int* m, double* out, int64_t n,
+                             int64_t vs, int nvec, hipStream_t st) {
+  submit(st, "lz_combine", [=]() {
+    if (!chk_range(m, 0, nvec - 1, "lz m")) return;
+    for (int v = 0; v < nvec; ++v) {
+      const int mv = m[v];
+      const int64_t o = (int64_t)v * vs;
+      if (!chk_range(out + o, 0, n - 1, "lz out") || !chk_range(c + (int64_t)v * ldc, 0, mv - 1, "lz coef") ||
+          !chk_range(basis, 0, mv - 1, "lz basis table"))
+        return;
+      for (int b = 0; b < mv; ++b)
+        if (!chk_range(basis[b] + o, 0, n - 1, "lz basis")) return;
+      for (int64_t i = 0; i < n; ++i) {
+        double t = 0;
+        for (int b = 0; b < mv; ++b) t += c[(int64_t)v * ldc + b] * basis[b][o + i];
+        out[o + i] = t;
+      }
+    }
+  });
+  return hipSuccess;
+}
+hipError_t launch_copy2d(const Copy2D* tasks, int ntasks, int64_t max_elems, const double* src, double* dst,
+                         const double* scales, hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  submit(st, "copy2d", [=]() {
+    if (!chk_range(tasks, 0, ntasks - 1, "copy2d tasks")) return;
+    for (int y = 0; y < ntasks; ++y) {
+      const Copy2D t = tasks[y];
+      const int64_t total = (int64_t)t.rows * t.cols;
+      if (total > max_elems) report("copy2d: task larger than max_elems (part of it is never copied)");
+      for (int64_t e = 0; e < total; ++e) {
+        const int64_t i = e % t.rows, j = e / t.rows;
+        const int64_t si = t.src_off + (t.trans ? i * t.lds + j : j * t.lds + i);
+        if (!check(src + si, 8, "copy2d src")) return;
+        double v = src[si] * t.scale;
+        if (t.rs_off >= 0) {
+          if (!check(scales + t.rs_off + i, 8, "copy2d row scale")) return;
+          v *= scales[t.rs_off + i];
+        }
+        if (t.cs_off >= 0) {
+          if (!check(scales + t.cs_off + j, 8, "copy2d col scale")) return;
+          v *= scales[t.cs_off + j];
+        }
+        if (!check(dst + t.dst_off + j * t.ldd + i, 8, "copy2d dst")) return;
+        dst[t.dst_off + j * t.ldd + i] = v;
+      }
+    }
+  });
+  return hipSuccess;
+}
+hipError_t launch_fixphase(double* U, int64_t m, int64_t ldu, double* Vt, int64_t k, int64_t n, int64_t ldv,
+                           hipStream_t st) {
+  if (k <= 0) return hipSuccess;
+  submit(st, "fixphase", [=]() {
+    for (int64_t i = 0; i < k; ++i) {
+      double best = 0;
+      int64_t bi = 0;
+      for (int64_t j = 0; j < n; ++j) {
+        if (!check(Vt + i + j * ldv, 8, "fixphase Vt")) return;
+        if (std::fabs(Vt[i + j * ldv]) > best) {
+          best = std::fabs(Vt[i + j * ldv]);
+          bi = j;
+        }
+      }
+      if (!check(Vt + i + bi * ldv, 8, "fixphase Vt")) return;
+      const double phi = Vt[i + bi * ldv] > 0.0 ? 1.0 : -1.0;
+      for (int64_t j = 0; j < n; ++j) Vt[i + j * ldv] *= phi;
+      for (int64_t r = 0; r < m; ++r) {
+        if (!check(U + r + i * ldu, 8, "fixphase U")) return;
+        U[r + i * ldu] *= phi;
+      }
+    }
+  });
+  return hipSuccess;
+}
+int gemm_max_grid(int, int) { return 512; }
+
+}  // namespace mpsa

@@ -1,0 +1,153 @@
+"""The DMRG sweep's host logic (mpsa_dmrg.cpp over the plan builder, the
+split and the Lanczos driver, unchanged) on the CPU, against an emulation of
+the device layer (tests/emu/emu.cpp): kernels computed on the host with every
+device address they touch checked against the live allocations, never-written
+memory NaN-filled, freed memory kept and poisoned.
+
+Bars, as for the device sweep (DMRGengine::simulate, src/container.hpp:383-438):
+untruncated sweeps reach the exact ground-state energy within 1e-9 (free
+fermions; exact diagonalisation of the Sz sector); a truncated sweep from the
+same right-normalised start gives the oracle's energy and truncated weight, its
+key sets and a final state of the same energy (oracle/pydmrg.py); and the
+emulation reports no out-of-bounds access, use after free or bad free.
+
+CPU only.  The emulated library is built by tests/emu/Makefile with g++.
+"""
+import ctypes as C
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+import scipy.sparse.linalg
+
+import dense
+import pydmrg
+from mpstools_amd import models, native
+from test_oracle import _sector_H, right_normalised_mps
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+EMU_LIB = os.path.join(HERE, "emu", "_build", "libmpsa_emu.so")
+
+
+@pytest.fixture(scope="module")
+def emu():
+    if not shutil.which("g++") or not os.path.exists("/opt/rocm/include/hip/hip_runtime.h"):
+        pytest.skip("g++ or the HIP headers are not available")
+    subprocess.run(["make", "-s", "-C", os.path.join(HERE, "emu")], check=True, capture_output=True)
+    L = C.CDLL(EMU_LIB)
+    L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    L.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    L.emu_violations.restype = C.c_long
+    L.emu_message.restype = C.c_char_p
+    return L
+
+
+@pytest.fixture
+def lib(emu, monkeypatch):
+    """native's ABI wrappers bound to the emulated library for one test."""
+    monkeypatch.setattr(native, "LIBPATH", EMU_LIB)
+    monkeypatch.setattr(native, "_lib", None)
+    monkeypatch.setattr(native, "_stream", lambda: C.c_void_p(0))
+    L = native.lib()
+    v0 = emu.emu_violations()
+    yield emu
+    n = emu.emu_violations() - v0
+    msgs = [emu.emu_message(i).decode() for i in range(min(emu.emu_violations(), 10)) if emu.emu_message(i)]
+    assert n == 0, msgs
+    assert L is native._lib
+
+
+class DevBuf:
+    """A device buffer of the emulated runtime, filled from a host array."""
+
+    def __init__(self, L, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        self.L, self.n = L, a.size
+        self.p = C.c_void_p()
+        assert L.hipMalloc(C.byref(self.p), max(a.nbytes, 8)) == 0
+        if a.nbytes:
+            L.hipMemcpy(self.p, a.ctypes.data, a.nbytes, 1)
+
+    def data_ptr(self):
+        return self.p.value
+
+    def host(self):
+        out = np.zeros(self.n)
+        if self.n:
+            self.L.hipMemcpy(out.ctypes.data, self.p, out.nbytes, 2)
+        return out
+
+
+def _dmrg(L, mps, mpo, itk):
+    tabs = [native.Table.from_blocks(A) for A in mps]
+    bufs = [DevBuf(L, t.pack(A)) for t, A in zip(tabs, mps)]
+    return native.Dmrg(tabs, bufs, mpo, itk), bufs
+
+
+def _download(L, d):
+    out = []
+    for i in range(d.N):
+        nb, ln = np.zeros(1, np.int64), np.zeros(1, np.int64)
+        native._check(L.mpsa_dmrg_site(d.h, i, native._ptr(nb), native._ptr(ln), None, None, None, None, None, None,
+                                       None))
+        n = int(nb[0])
+        q, p = np.zeros(2 * n, np.uint64), np.zeros(n, np.uint32)
+        r, c, o = np.zeros(n, np.int64), np.zeros(n, np.int64), np.zeros(n, np.int64)
+        buf = DevBuf(L, np.zeros(max(1, int(ln[0]))))
+        native._check(L.mpsa_dmrg_site(d.h, i, native._ptr(nb), native._ptr(ln), native._ptr(q), native._ptr(p),
+                                       native._ptr(r), native._ptr(c), native._ptr(o), buf.p, C.c_void_p(0)))
+        out.append(native.Table(1, q.reshape(-1, 2), p.reshape(-1, 1), r, c, o).unpack(buf.host()))
+    return out
+
+
+@pytest.mark.parametrize("L,N,Jz,hz", [(10, 5, 0.0, 0.0), (8, 4, 1.0, 0.1)])
+def test_emulated_sweeps_reach_exact_energy(lib, oracle, L, N, Jz, hz):
+    mps, itk = right_normalised_mps(L, N, 64, 31)
+    mpo = models.xxz_mpo(L, Jz, 1.0, hz)
+    d, keep = _dmrg(lib, mps, mpo, itk)
+    for _ in range(3):
+        r = d.sweep(chimax=64, lanmax=300, delta=1e-12)
+    if Jz == 0.0:
+        eps = np.sort(np.cos(np.pi * np.arange(1, L + 1) / (L + 1)))
+        e0 = eps[:N].sum()
+    else:
+        e0 = scipy.sparse.linalg.eigsh(_sector_H(dense.dense_H(mpo), L, N), k=1, which="SA")[0][0]
+    assert abs(r["E"] - e0) < 1e-9
+    assert r["tw"] < 1e-12
+    assert abs(oracle.measure(_download(lib, d), mpo, itk) - r["E"]) < 1e-9
+    del d
+
+
+@pytest.mark.parametrize("model,L,N,chi", [("xxz", 12, 6, 8), ("tv", 14, 7, 12)])
+def test_emulated_truncated_sweep_matches_oracle(lib, oracle, model, L, N, chi):
+    mps, itk = right_normalised_mps(L, N, 24, 41)
+    mpo = models.xxz_mpo(L, 1.0, 1.0, 0.0) if model == "xxz" else models.tv_mpo(L, 1.0, 2.0, 0.0)
+    ref = pydmrg.Dmrg(mps, mpo, itk)
+    rr = ref.sweep(chimax=chi, lanmax=300, delta=1e-12)
+    d, keep = _dmrg(lib, mps, mpo, itk)
+    r = d.sweep(chimax=chi, lanmax=300, delta=1e-12)
+    assert abs(r["E"] - rr["E"]) < 1e-9
+    assert abs(r["tw"] - rr["tw"]) < 1e-10
+    got = _download(lib, d)
+    assert [A.keys() for A in got] == [A.keys() for A in ref.mps]
+    assert abs(oracle.measure(got, mpo, itk) - oracle.measure(ref.mps, mpo, itk)) < 1e-9
+    del d
+
+
+def test_emulation_reports_planted_mistakes(emu):
+    """The emulation itself is pinned: an out-of-bounds copy, a use after free
+    and a free of a foreign pointer are each reported."""
+    v0 = emu.emu_violations()
+    buf = DevBuf(emu, np.zeros(16))
+    host = np.zeros(32)
+    emu.hipMemcpy(host.ctypes.data, buf.p, 17 * 8, 2)  # one element past the end
+    assert emu.emu_violations() == v0 + 1
+    emu.hipFree.argtypes = [C.c_void_p]
+    assert emu.hipFree(buf.p) == 0
+    emu.hipMemcpy(host.ctypes.data, buf.p, 8, 2)  # read after free
+    assert emu.emu_violations() == v0 + 2
+    assert "use after free" in emu.emu_message(int(emu.emu_violations()) - 1).decode()
+    assert emu.hipFree(C.c_void_p(host.ctypes.data)) != 0
+    assert emu.emu_violations() == v0 + 3
