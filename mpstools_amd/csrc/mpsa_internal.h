@@ -70,6 +70,8 @@ hipError_t launch_convert(int prec, const double* src, void* dst, int64_t n, hip
 hipError_t launch_repack(int prec, const Repack* tasks, int ntasks, int chunks_per_vec, int nvec, const double* src,
                          int64_t s_vs, void* dst, int64_t d_vs, hipStream_t st, int64_t plane = 0);
 hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t st);
+// p[i] = uniform in [-1, 1) from SplitMix64 of seed + i * golden gamma
+hipError_t launch_fill_hash(double* p, int64_t n, uint64_t seed, hipStream_t st);
 // p[i * stride] = v for i < n
 hipError_t launch_fill_strided(double* p, int64_t n, int64_t stride, double v, hipStream_t st);
 hipError_t launch_zero_ranges(const Range* r, int nr, double* base, int64_t vs, int nvec, hipStream_t st);

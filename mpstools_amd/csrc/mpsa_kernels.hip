@@ -1641,6 +1641,21 @@ __global__ void fill_kernel(double* __restrict__ p, int64_t n, double v) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
 }
 
+// SplitMix64 finaliser (Steele, Lea & Flood, OOPSLA 2014; constants as
+// published with Vigna's splitmix64.c) of seed + i * 0x9e3779b97f4a7c15,
+// top 53 bits mapped to [-1, 1).
+__device__ __host__ __forceinline__ double hash_unit(uint64_t seed, uint64_t i) {
+  uint64_t z = seed + (i + 1) * 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+__global__ void fill_hash_kernel(double* __restrict__ p, int64_t n, uint64_t seed) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = hash_unit(seed, (uint64_t)i);
+}
+
 __global__ void fill_strided_kernel(double* __restrict__ p, int64_t n, int64_t stride, double v) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     p[i * stride] = v;
@@ -2084,6 +2099,14 @@ hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t st) {
   int64_t blocks = (n + 255) / 256;
   int grid = (int)(blocks < 16384 ? blocks : 16384);
   fill_kernel<<<grid, 256, 0, st>>>(p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_hash(double* p, int64_t n, uint64_t seed, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  int64_t blocks = (n + 255) / 256;
+  int grid = (int)(blocks < 16384 ? blocks : 16384);
+  fill_hash_kernel<<<grid, 256, 0, st>>>(p, n, seed);
   return hipGetLastError();
 }
 

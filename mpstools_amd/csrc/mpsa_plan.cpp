@@ -233,6 +233,13 @@ static bool fits_pipe_tile(const std::vector<GTask>& t, const std::vector<GSeg>&
   return true;
 }
 
+// Tile classes a stage may launch with: every class of the precision, the
+// 256x256 class only where fits_pipe_tile holds.
+static bool class_admissible(const std::vector<GTask>& t, const std::vector<GSeg>& segs, int prec, int cls) {
+  if (cls < 0 || cls >= num_tile_classes(prec)) return false;
+  return cls != 3 || fits_pipe_tile(t, segs);
+}
+
 static int pick_class(const std::vector<GTask>& t, const std::vector<GSeg>& segs, int prec) {
   int best = 1;
   int best_area = 0;
@@ -254,6 +261,21 @@ static int pick_class(const std::vector<GTask>& t, const std::vector<GSeg>& segs
   return best;
 }
 
+// Number the workgroup tiles of a stage for tile class cls, in task order
+// (tiles m-fastest inside a task).
+static void assign_tiles(Stage& s, int prec, int cls) {
+  int BM, BN;
+  tile_dims(prec, cls, &BM, &BN);
+  int tile = 0;
+  for (auto& t : s.tasks) {
+    t.tiles_m = (t.m + BM - 1) / BM;
+    t.tile0 = tile;
+    tile += t.tiles_m * ((t.n + BN - 1) / BN);
+  }
+  s.tiles = tile;
+  s.tile_class = cls;
+}
+
 static void finalize_stage(Stage& s, int prec) {
   // drop empty-k segments (kernels assume k > 0); a task left without
   // segments still writes its C block (as zeros)
@@ -268,9 +290,6 @@ static void finalize_stage(Stage& s, int prec) {
     }
     s.segs = segs;
   }
-  s.tile_class = pick_class(s.tasks, s.segs, prec);
-  int BM, BN;
-  tile_dims(prec, s.tile_class, &BM, &BN);
   // longest tiles first (static round-robin over the persistent grid)
   std::vector<int64_t> cost(s.tasks.size());
   for (size_t i = 0; i < s.tasks.size(); ++i) {
@@ -282,18 +301,10 @@ static void finalize_stage(Stage& s, int prec) {
   for (size_t i = 0; i < ord.size(); ++i) ord[i] = i;
   std::stable_sort(ord.begin(), ord.end(), [&](size_t a, size_t b) { return cost[a] > cost[b]; });
   std::vector<GTask> t2;
-  int tile = 0;
-  for (size_t i : ord) {
-    GTask t = s.tasks[i];
-    if (t.m <= 0 || t.n <= 0) continue;
-    t.tiles_m = (t.m + BM - 1) / BM;
-    const int tiles_n = (t.n + BN - 1) / BN;
-    t.tile0 = tile;
-    tile += t.tiles_m * tiles_n;
-    t2.push_back(t);
-  }
+  for (size_t i : ord)
+    if (s.tasks[i].m > 0 && s.tasks[i].n > 0) t2.push_back(s.tasks[i]);
   s.tasks = t2;
-  s.tiles = tile;
+  assign_tiles(s, prec, pick_class(s.tasks, s.segs, prec));
 }
 
 static void finalize_mix(mpsa_plan* P) {
@@ -813,6 +824,40 @@ static void ensure_ws(mpsa_plan* P, int nvec) {
 }  // namespace mpsa
 
 // =================================================================== C ABI
+// Re-number a stage's tiles for tile class cls and refresh its device task
+// list (same number of tasks, so the allocation is reused).
+static void set_stage_class(mpsa_plan* P, Stage& S, int prec, int cls) {
+  if (S.tile_class == cls) return;
+  assign_tiles(S, prec, cls);
+  if (P->on_device && S.d_tasks && !S.tasks.empty())
+    HIPCHK(hipMemcpy(S.d_tasks, S.tasks.data(), sizeof(GTask) * S.tasks.size(), hipMemcpyHostToDevice));
+}
+
+// The grouped GEMM launch of stage 1 (T = L x) or stage 2 (y = T R^T) of an
+// H_eff plan, with the operands mpsa_heff_apply gives it.
+static void launch_stage(mpsa_plan* P, int which, const double* dx, int64_t x_stride, double* dy, int64_t y_stride,
+                         int nvec, hipStream_t st) {
+  const int64_t xcs = P->xc_per_vec * P->planes, wss = P->ws_per_vec * P->planes;
+  if (which == 1) {
+    const void* A1 = P->prec == F64 ? (const void*)P->dL : P->Lc;
+    const void* B1 = P->prec == F64 ? (const void*)dx : P->xc;
+    Planes pl1;
+    pl1.a = P->Lc_elems;
+    pl1.b = P->xc_per_vec;
+    pl1.c = P->ws_per_vec;
+    HIPCHK(launch_gemm(P->prec, 0, P->s1.tr, P->s1.tile_class, P->s1.d_tasks, (int)P->s1.tasks.size(),
+                       P->s1.d_segs, P->s1.tiles, nvec, A1, 0, B1, P->prec == F64 ? x_stride : xcs, P->ws, wss, st,
+                       pl1));
+  } else {
+    const void* B2 = P->prec == F64 ? (const void*)P->dR : P->Rc;
+    Planes pl2;
+    pl2.a = P->ws_per_vec;
+    pl2.b = P->Rc_elems;
+    HIPCHK(launch_gemm(P->prec, 1, P->s2.tr, P->s2.tile_class, P->s2.d_tasks, (int)P->s2.tasks.size(),
+                       P->s2.d_segs, P->s2.tiles, nvec, P->ws, wss, B2, 0, dy, y_stride, st, pl2));
+  }
+}
+
 #define MPSA_TRY(...)                          \
   try {                                        \
     __VA_ARGS__;                               \
@@ -1030,33 +1075,100 @@ int mpsa_heff_apply(mpsa_plan* P, const double* dx, int64_t x_stride, double* dy
       P->nrec = slot + 1;
       HIPCHK(hipEventRecord(ev[0], st));
     }
-    const void* A1 = P->prec == F64 ? (const void*)P->dL : P->Lc;
-    const void* B1 = P->prec == F64 ? (const void*)dx : P->xc;
     const int64_t xcs = P->xc_per_vec * P->planes, wss = P->ws_per_vec * P->planes;
-    int64_t b1_vs = P->prec == F64 ? x_stride : xcs;
     if (P->prec != F64)
       HIPCHK(launch_repack(P->prec, P->d_rpx, (int)P->rpx.size(), P->rpx_chunks, nvec, dx, x_stride, P->xc, xcs, st,
                            P->xc_per_vec));
     if (ev) HIPCHK(hipEventRecord(ev[1], st));
-    Planes pl1;
-    pl1.a = P->Lc_elems;
-    pl1.b = P->xc_per_vec;
-    pl1.c = P->ws_per_vec;
-    HIPCHK(launch_gemm(P->prec, 0, P->s1.tr, P->s1.tile_class, P->s1.d_tasks, (int)P->s1.tasks.size(),
-                       P->s1.d_segs, P->s1.tiles, nvec, A1, 0, B1, b1_vs, P->ws, wss, st, pl1));
+    launch_stage(P, 1, dx, x_stride, dy, y_stride, nvec, st);
     if (ev) HIPCHK(hipEventRecord(ev[2], st));
     HIPCHK(launch_mix(P->prec, P->d_mix, (int)P->mix.size(), P->d_terms, P->mix_chunks, nvec, P->ws, P->ws, wss, st,
                       P->ws_per_vec));
-    const void* B2 = P->prec == F64 ? (const void*)P->dR : P->Rc;
     HIPCHK(launch_zero_ranges(P->d_zero, (int)P->zero.size(), dy, y_stride, nvec, st));
     if (ev) HIPCHK(hipEventRecord(ev[3], st));
-    Planes pl2;
-    pl2.a = P->ws_per_vec;
-    pl2.b = P->Rc_elems;
-    HIPCHK(launch_gemm(P->prec, 1, P->s2.tr, P->s2.tile_class, P->s2.d_tasks, (int)P->s2.tasks.size(),
-                       P->s2.d_segs, P->s2.tiles, nvec, P->ws, wss, B2, 0, dy, y_stride, st, pl2));
+    launch_stage(P, 2, dx, x_stride, dy, y_stride, nvec, st);
     if (ev) HIPCHK(hipEventRecord(ev[4], st));
   })
+}
+
+int mpsa_plan_set_tile_class(mpsa_plan* P, int stage, int tile_class) {
+  MPSA_TRY({
+    if (!P || (stage != 1 && stage != 2)) throw Error(MPSA_EINVAL, "set_tile_class: bad arguments");
+    Stage& S = stage == 1 ? P->s1 : P->s2;
+    const int prec = P->kind == 2 ? F64 : P->prec;
+    if (!class_admissible(S.tasks, S.segs, prec, tile_class))
+      throw Error(MPSA_EINVAL, "set_tile_class: tile class not available for this stage");
+    set_stage_class(P, S, prec, tile_class);
+  })
+}
+
+int mpsa_plan_autotune(mpsa_plan* P, int nvec, int reps, double* ms, mpsa_stream stream) {
+  double* x = nullptr;
+  double* y = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  auto release = [&]() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (x) (void)hipFree(x);
+    if (y) (void)hipFree(y);
+  };
+  try {
+    if (!P || P->kind != 0) throw Error(MPSA_EINVAL, "autotune: not an H_eff plan");
+    if (nvec < 1 || reps < 1) throw Error(MPSA_EINVAL, "autotune: nvec and reps must be >= 1");
+    if (!P->dL && P->L_total > 0) throw Error(MPSA_EINVAL, "autotune: environments not bound");
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t xs = std::max<int64_t>(P->x_total, 1), ys = std::max<int64_t>(P->y_total, 1);
+    if (hipMalloc((void**)&x, sizeof(double) * (size_t)(xs * nvec)) != hipSuccess ||
+        hipMalloc((void**)&y, sizeof(double) * (size_t)(ys * nvec)) != hipSuccess)
+      throw Error(MPSA_ENOMEM, "autotune: vector allocation failed");
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(launch_fill_hash(x, xs * nvec, 0x9e3779b97f4a7c15ull, st));
+    const int timing = P->timing;
+    P->timing = 0;
+    // one full apply leaves converted operands and stage-1 products in place
+    // for the stage launches timed below
+    const int rc = mpsa_heff_apply(P, x, xs, y, ys, nvec, stream);
+    P->timing = timing;
+    if (rc != MPSA_OK) throw Error(rc, mpsa_last_error());
+    for (int stage = 1; stage <= 2; ++stage) {
+      Stage& S = stage == 1 ? P->s1 : P->s2;
+      const int cur = S.tile_class;
+      int best = -1;
+      double best_ms = 0.0;
+      for (int cls = 0; cls < 4; ++cls) {
+        if (ms) ms[(stage - 1) * 4 + cls] = -1.0;
+        if (!class_admissible(S.tasks, S.segs, P->prec, cls)) continue;
+        set_stage_class(P, S, P->prec, cls);
+        launch_stage(P, stage, x, xs, y, ys, nvec, st);  // warm-up
+        HIPCHK(hipEventRecord(e0, st));
+        for (int r = 0; r < reps; ++r) launch_stage(P, stage, x, xs, y, ys, nvec, st);
+        HIPCHK(hipEventRecord(e1, st));
+        HIPCHK(hipEventSynchronize(e1));
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, e0, e1));
+        const double per = (double)t / reps;
+        if (ms) ms[(stage - 1) * 4 + cls] = per;
+        // the strictly fastest class wins; on a tie the plan's own class stays
+        if (best < 0 || per < best_ms || (per == best_ms && cls == cur)) {
+          best = cls;
+          best_ms = per;
+        }
+      }
+      set_stage_class(P, S, P->prec, best);
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    release();
+    return MPSA_OK;
+  } catch (const Error& e) {
+    release();
+    g_err = e.what();
+    return e.code;
+  } catch (const std::exception& e) {
+    release();
+    g_err = e.what();
+    return MPSA_EINVAL;
+  }
 }
 
 int mpsa_env_apply(mpsa_plan* P, const double* dBL, const double* dA, const double* dAp, double* dout,

@@ -229,6 +229,7 @@ class HeffPlan:
     def __init__(self, Lt: Table, Rt: Table, xt: Table, yt: Table, W, itk, precision=F64):
         self.Lt, self.Rt, self.xt, self.yt = Lt, Rt, xt, yt
         self.h = C.c_void_p()
+        self._L = lib()
         self.precision = PREC.get(precision, precision) if isinstance(precision, str) else precision
         if xt.ns == 2:
             Wl, Wr = W
@@ -246,8 +247,10 @@ class HeffPlan:
         self._env = None
 
     def __del__(self):
-        if getattr(self, "h", None) and self.h.value and _lib is not None:
-            _lib.mpsa_plan_destroy(self.h)
+        # destroyed by the library that created it
+        L = getattr(self, "_L", None)
+        if getattr(self, "h", None) and self.h.value and L is not None:
+            L.mpsa_plan_destroy(self.h)
             self.h = None
 
     def info(self):
@@ -269,6 +272,17 @@ class HeffPlan:
 
     def set_timing(self, on=True):
         _check(lib().mpsa_plan_set_timing(self.h, int(on)))
+
+    def set_tile_class(self, stage, tile_class):
+        """Launch GEMM stage 1 or 2 with the given tile class from now on."""
+        _check(lib().mpsa_plan_set_tile_class(self.h, int(stage), int(tile_class)))
+
+    def autotune(self, nvec=1, reps=3):
+        """Time every admissible tile class of both stages (environments bound)
+        and keep the fastest; returns {(stage, class): ms per launch}."""
+        ms = np.zeros(8)
+        _check(lib().mpsa_plan_autotune(self.h, int(nvec), int(reps), _ptr(ms), _stream()))
+        return {(st + 1, c): float(ms[st * 4 + c]) for st in range(2) for c in range(4) if ms[st * 4 + c] >= 0}
 
     def last_times(self):
         """ms of [convert x, stage-1 GEMM, mix + zero fill, stage-2 GEMM] of the last apply."""
@@ -327,6 +341,7 @@ class EnvPlan:
     def __init__(self, BLt: Table, At: Table, W, itk, direction, outt: Table, Apt: Table | None = None):
         self.BLt, self.At, self.Apt, self.outt = BLt, At, Apt, outt
         self.h = C.c_void_p()
+        self._L = lib()
         a = _itk(itk)
         m = MPOArg(W)
         _check(lib().mpsa_env_plan(C.byref(self.h), C.byref(BLt.s), C.byref(At.s),
@@ -334,8 +349,10 @@ class EnvPlan:
                                    int(direction), C.byref(outt.s), F64))
 
     def __del__(self):
-        if getattr(self, "h", None) and self.h.value and _lib is not None:
-            _lib.mpsa_plan_destroy(self.h)
+        # destroyed by the library that created it
+        L = getattr(self, "_L", None)
+        if getattr(self, "h", None) and self.h.value and L is not None:
+            L.mpsa_plan_destroy(self.h)
             self.h = None
 
     def info(self):
@@ -397,6 +414,7 @@ class Chain:
         self._keep = (sites, mpos, dm)
         self._itk = np.ascontiguousarray(np.concatenate([_itk(x) for x in itk]), dtype=np.uint64)
         self._d = np.array([len(x) for x in itk], dtype=np.int32)
+        self._L = lib()
         h = C.c_void_p()
         _check(lib().mpsa_chain_create(C.byref(h), self.N, C.cast(sites, C.c_void_p), C.cast(mpos, C.c_void_p),
                                        _ptr(self._itk), _ptr(self._d), self.nparam,
@@ -404,8 +422,9 @@ class Chain:
         self.h = h
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().mpsa_chain_destroy(self.h)
+        L = getattr(self, "_L", None)
+        if getattr(self, "h", None) and L is not None:
+            L.mpsa_chain_destroy(self.h)
             self.h = None
 
     def _ptrs(self, bufs):
@@ -492,14 +511,16 @@ class Dmrg:
         ptrs = (C.c_void_p * self.N)(*[_dptr(b) for b in dA])
         itk_all = np.ascontiguousarray(np.concatenate([_itk(x) for x in itk]), dtype=np.uint64)
         d = np.array([len(x) for x in itk], dtype=np.int32)
+        self._L = lib()
         h = C.c_void_p()
         _check(lib().mpsa_dmrg_create(C.byref(h), self.N, C.cast(sites, C.c_void_p), C.cast(ptrs, C.c_void_p),
                                       C.cast(mpos, C.c_void_p), _ptr(itk_all), _ptr(d), _stream()))
         self.h = h
 
     def __del__(self):
-        if getattr(self, "h", None):
-            lib().mpsa_dmrg_destroy(self.h)
+        L = getattr(self, "_L", None)
+        if getattr(self, "h", None) and L is not None:
+            L.mpsa_dmrg_destroy(self.h)
             self.h = None
 
     def sweep(self, chimax, do_trunc=False, tr_weight=0.0, lanmax=500, delta=1e-12, precision=F64, refine_it=10):

@@ -41,7 +41,16 @@ Blas& blas() {
     // OpenMP-over-charges loops share one runtime (libgomp).
     typedef int (*set_layer_fn)(int);
     set_layer_fn sl = (set_layer_fn)dlsym(h, "MKL_Set_Threading_Layer");
-    if (sl) sl(3 /* MKL_THREADING_GNU */);
+    const int layer = sl ? sl(3 /* MKL_THREADING_GNU */) : 3;
+    // A process that initialised MKL before us keeps its layer.  The Intel
+    // layer (libiomp5) beside the GNU runtime of the charge loops gives racy
+    // results; MKL then runs on one thread and the charge loops keep theirs.
+    if (layer != 3 && layer != 1 /* MKL_THREADING_SEQUENTIAL */) {
+      typedef void (*set_threads_fn)(int);
+      set_threads_fn st = (set_threads_fn)dlsym(h, "MKL_Set_Num_Threads");
+      if (st) st(1);
+      b->name = std::string(b->name) + " (one thread: MKL threading layer already chosen)";
+    }
     b->dgemm = (dgemm_fn)dlsym(h, "dgemm_");
     b->zgemm = (zgemm_fn)dlsym(h, "zgemm_");
     b->dstev = (dstev_fn)dlsym(h, "dstev_");

@@ -263,7 +263,13 @@ static dgesvd_fn lapack_dgesvd() {
     for (const char* c : cands) {
       if (!c) continue;
       void* h = dlopen(c, RTLD_NOW | RTLD_LOCAL);
-      if (h && (f = (dgesvd_fn)dlsym(h, "dgesvd_"))) break;
+      if (h && (f = (dgesvd_fn)dlsym(h, "dgesvd_"))) {
+        // MKL: the GNU threading layer, as the oracle asks for (the Intel
+        // layer beside the GNU OpenMP runtime makes the oracle's loops racy)
+        typedef int (*set_layer_fn)(int);
+        if (set_layer_fn sl = (set_layer_fn)dlsym(h, "MKL_Set_Threading_Layer")) sl(3);
+        break;
+      }
     }
   }
   return f;
@@ -503,6 +509,20 @@ hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t st) {
   submit(st, "fill", [=]() {
     if (chk_range(p, 0, n - 1, "fill"))
       for (int64_t i = 0; i < n; ++i) p[i] = v;
+  });
+  return hipSuccess;
+}
+hipError_t launch_fill_hash(double* p, int64_t n, uint64_t seed, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  submit(st, "fill_hash", [=]() {
+    if (!chk_range(p, 0, n - 1, "fill_hash")) return;
+    for (int64_t i = 0; i < n; ++i) {
+      uint64_t z = seed + ((uint64_t)i + 1) * 0x9e3779b97f4a7c15ull;
+      z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+      z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+      z ^= z >> 31;
+      p[i] = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+    }
   });
   return hipSuccess;
 }

@@ -1,8 +1,9 @@
-"""The DMRG sweep's host logic (mpsa_dmrg.cpp over the plan builder, the
-split and the Lanczos driver, unchanged) on the CPU, against an emulation of
-the device layer (tests/emu/emu.cpp): kernels computed on the host with every
-device address they touch checked against the live allocations, never-written
-memory NaN-filled, freed memory kept and poisoned.
+"""The library's host logic on the CPU, against an emulation of the device
+layer (tests/emu/emu.cpp): kernels computed on the host with every device
+address they touch checked against the live allocations, never-written memory
+NaN-filled, freed memory kept and poisoned.  Covered: the DMRG sweep
+(mpsa_dmrg.cpp over the plan builder, the split and the Lanczos driver,
+unchanged) and the tile-class selection of H_eff plans.
 
 Bars, as for the device sweep (DMRGengine::simulate, src/container.hpp:383-438):
 untruncated sweeps reach the exact ground-state energy within 1e-9 (free
@@ -23,6 +24,7 @@ import pytest
 import scipy.sparse.linalg
 
 import dense
+import problems
 import pydmrg
 from mpstools_amd import models, native
 from test_oracle import _sector_H, right_normalised_mps
@@ -151,3 +153,47 @@ def test_emulation_reports_planted_mistakes(emu):
     assert "use after free" in emu.emu_message(int(emu.emu_violations()) - 1).decode()
     assert emu.hipFree(C.c_void_p(host.ctypes.data)) != 0
     assert emu.emu_violations() == v0 + 3
+
+
+def test_emulated_tile_classes_agree_and_autotune_keeps_ties(lib, oracle):
+    """Every FP64 tile class of both GEMM stages gives the same bits and the
+    oracle's contraction (doContraction, src/MPSfunctions.hpp:726-804); bad
+    classes are refused; the autotuner times every admissible class and, with
+    the emulation's zero-duration events, keeps the plan's own classes."""
+    P = problems.build(oracle, "tv", 20, 10, 60, 108)
+    c, mpo, itk = P["c"], P["mpo"], P["itk"]
+    L, R, psi = P["Ls"][c - 1], P["Rs"][c + 2], P["psi"]
+    Lt, Rt, xt = native.Table.from_blocks(L), native.Table.from_blocks(R), native.Table.from_blocks(psi)
+    vt = xt.union(native.heff2_outkeys(Lt, Rt, xt, mpo[c], mpo[c + 1], itk[c], itk[c + 1]))
+    plan = native.HeffPlan(Lt, Rt, vt, vt, (mpo[c], mpo[c + 1]), (itk[c], itk[c + 1]), native.F64)
+    dL, dR = DevBuf(lib, Lt.pack(L)), DevBuf(lib, Rt.pack(R))
+    plan.bind_env(dL, dR)
+    x = DevBuf(lib, vt.pack(psi))
+    outs = {}
+    for c1 in (0, 1):
+        for c2 in (0, 1):
+            plan.set_tile_class(1, c1)
+            plan.set_tile_class(2, c2)
+            assert (plan.stage_info()["tile_class1"], plan.stage_info()["tile_class2"]) == (c1, c2)
+            y = DevBuf(lib, np.zeros(vt.total))
+            plan.apply(x, y)
+            outs[(c1, c2)] = y.host()
+    ref = outs[(0, 0)]
+    assert all(np.array_equal(v, ref) for v in outs.values())
+    y_ref = oracle.contract_twosite(L, R, psi, mpo[c], mpo[c + 1], itk[c], itk[c + 1])
+    y0 = vt.unpack(ref)
+    num = sum(np.sum((y0[k] - y_ref[k]) ** 2) for k in y_ref.keys())
+    den = sum(np.sum(y_ref[k] ** 2) for k in y_ref.keys())
+    assert np.sqrt(num / den) <= 1e-12
+    for bad in ((1, 2), (2, 3), (1, -1), (3, 0)):
+        with pytest.raises(native.MpsaError):
+            plan.set_tile_class(*bad)
+    plan.set_tile_class(1, 1)
+    plan.set_tile_class(2, 0)
+    ms = plan.autotune(nvec=2, reps=1)
+    assert set(ms) == {(1, 0), (1, 1), (2, 0), (2, 1)}
+    assert (plan.stage_info()["tile_class1"], plan.stage_info()["tile_class2"]) == (1, 0)
+    y = DevBuf(lib, np.zeros(vt.total))
+    plan.apply(x, y)
+    assert np.array_equal(y.host(), ref)
+    del plan
