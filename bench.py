@@ -144,6 +144,8 @@ def main():
     ap.add_argument("--lanczos-itmax", type=int, default=700)
     ap.add_argument("--refine-it", type=int, default=10)
     ap.add_argument("--no-lanczos", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the plans' heuristic tile classes instead of autotuning them")
     ap.add_argument("--calib", action="store_true",
                     help="run a known-size axpby (2 GiB read, 1 GiB written) for HBM counter calibration")
     args = ap.parse_args()
@@ -176,11 +178,17 @@ def main():
     for pr in precs:
         plan = prob["plan_" + pr]
         info = plan.info()
+        tuning = None
+        if not args.no_tune:
+            # time every admissible tile class of both GEMM stages, keep the fastest
+            t = plan.autotune(nvec=1, reps=3)
+            tuning = {"stage%d" % s_: {str(c): round(ms_, 4) for (s2, c), ms_ in sorted(t.items()) if s2 == s_}
+                      for s_ in (1, 2)}
         si = plan.stage_info()
         ms, st = time_apply(torch, plan, x, y, args.steps, args.warmup)
         res[pr] = dict(ms=ms, tflops=info["flops_alg"] / ms / 1e9, flops_alg=info["flops_alg"],
                        flops_ref=info["flops_ref"], stage_ms=st, stage_flops=[si["flops1"], si["flops2"]],
-                       tile_class=[si["tile_class1"], si["tile_class2"]])
+                       tile_class=[si["tile_class1"], si["tile_class2"]], tuning=tuning)
         log(pr, f"{ms:.3f} ms/matvec", f"{res[pr]['tflops']:.1f} TFLOP/s", "stages(ms)", [round(s, 3) for s in st])
     mixed = args.mixed if args.mixed in res else precs[-1]
     r = res[mixed]
@@ -229,7 +237,9 @@ def main():
         "cpu_baseline": cpu,
         "by_precision": {pr: {"ms": round(v["ms"], 4), "tflops": round(v["tflops"], 3),
                               "frac_of_peak": round(v["tflops"] / PEAK[pr], 4),
-                              "stage_ms": [round(s, 4) for s in v["stage_ms"]]} for pr, v in res.items()},
+                              "stage_ms": [round(s, 4) for s in v["stage_ms"]],
+                              "tile_class": v["tile_class"],
+                              "tile_tuning_ms": v["tuning"]} for pr, v in res.items()},
         "flops_alg": r["flops_alg"],
         "flops_ref": r["flops_ref"],
     }
