@@ -34,17 +34,18 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def time_apply(torch, plan, x, y, steps, warmup):
-    """Wall time per matvec over `steps` applies after `warmup`, bracketed by
-    device synchronisation; the same applies carry HIP events on the plan's
-    stream, giving the average time of each stage inside the timed region."""
+def time_apply(torch, plan, x, y, steps, warmup, nvec=1, stride=0):
+    """Wall time per (batched) matvec over `steps` applies after `warmup`,
+    bracketed by device synchronisation; the same applies carry HIP events on
+    the plan's stream, giving the average time of each stage inside the timed
+    region."""
     for _ in range(warmup):
-        plan.apply(x, y)
+        plan.apply(x, y, nvec, stride, stride)
     torch.cuda.synchronize()
     plan.set_timing(True)
     t0 = time.perf_counter()
     for _ in range(steps):
-        plan.apply(x, y)
+        plan.apply(x, y, nvec, stride, stride)
     torch.cuda.synchronize()
     ms = (time.perf_counter() - t0) / steps * 1e3
     st, n = plan.timing_summary()
@@ -54,6 +55,11 @@ def time_apply(torch, plan, x, y, steps, warmup):
 
 
 TILE = {0: "256x128 stream, 4 waves", 1: "64x64", 2: "128x128", 3: "256x256 pipelined, 4 waves"}
+KERNEL = {0: "gemm_bf16_stream_kernel", 1: "gemm_bf16_kernel", 2: "gemm_bf16_kernel", 3: "gemm_bf16_pipe_kernel"}
+# U(1) bonds: (model, L, N, D, seed) -- the t-V chain at half filling, the XXZ
+# chain of the gradient workload, the small XXZ chain
+WORKLOADS = {"dense": None, "tv512": ("tv", 100, 50, 512, 20260002), "xxz1024": ("xxz", 64, 32, 1024, 20260004),
+             "xxz64": ("xxz", 32, 16, 64, 20260001)}
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "gemm_bf16_pipe_dense_D2048_hbm_traffic.txt")
 
 
@@ -144,6 +150,9 @@ def main():
     ap.add_argument("--lanczos-itmax", type=int, default=700)
     ap.add_argument("--refine-it", type=int, default=10)
     ap.add_argument("--no-lanczos", action="store_true")
+    ap.add_argument("--workload", default="dense", choices=sorted(WORKLOADS),
+                    help="dense: the D=2048 dense bond (the headline); the others are U(1) bonds")
+    ap.add_argument("--nvec", type=int, default=1, help="vectors per batched apply")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the plans' heuristic tile classes instead of autotuning them")
     ap.add_argument("--calib", action="store_true",
@@ -157,12 +166,25 @@ def main():
     from mpstools_amd import bench_problem
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     precs = args.precisions.split(",")
-    log(f"building dense Heisenberg centre problem L={args.L} D={args.D}")
     t0 = time.perf_counter()
-    prob = bench_problem.dense_heisenberg_centre(D=args.D, L=args.L, seed=args.seed, precisions=precs, verbose=True)
+    if args.workload == "dense":
+        log(f"building dense Heisenberg centre problem L={args.L} D={args.D}")
+        prob = bench_problem.dense_heisenberg_centre(D=args.D, L=args.L, seed=args.seed, precisions=precs,
+                                                     verbose=True)
+        workload = (f"dense Heisenberg J=1 centre bond, L={args.L}, w=5, d=2, D={args.D}, seeded Gaussian MPS, "
+                    f"QR-normalised, GPU-built environments")
+    else:
+        model, L, N, D, seed = WORKLOADS[args.workload]
+        log(f"building U(1) {model} centre problem L={L} N={N} D={D}")
+        prob = bench_problem.u1_centre(model, L, N, D, seed, precisions=precs, verbose=True)
+        workload = (f"U(1) {'t-V t=1 V=2' if model == 'tv' else 'XXZ Jz=Jxy=1'} centre bond, L={L}, N={N}, w=5, d=2, "
+                    f"D={D} over charge sectors ({prob['psi_blocks']} psi blocks, dims "
+                    f"{prob['psi_block_dims'][0]}-{prob['psi_block_dims'][1]}), seeded MPS, GPU-built environments")
     torch.cuda.synchronize()
     log(f"problem built in {time.perf_counter() - t0:.1f}s")
-    x = prob["psi"]
+    nvec = max(1, args.nvec)
+    stride = prob["vt"].total if nvec > 1 else 0
+    x = prob["psi"].repeat(nvec) if nvec > 1 else prob["psi"]
     y = torch.zeros_like(x)
     if args.calib:
         from mpstools_amd import native
@@ -181,12 +203,12 @@ def main():
         tuning = None
         if not args.no_tune:
             # time every admissible tile class of both GEMM stages, keep the fastest
-            t = plan.autotune(nvec=1, reps=3)
+            t = plan.autotune(nvec=nvec, reps=3)
             tuning = {"stage%d" % s_: {str(c): round(ms_, 4) for (s2, c), ms_ in sorted(t.items()) if s2 == s_}
                       for s_ in (1, 2)}
         si = plan.stage_info()
-        ms, st = time_apply(torch, plan, x, y, args.steps, args.warmup)
-        res[pr] = dict(ms=ms, tflops=info["flops_alg"] / ms / 1e9, flops_alg=info["flops_alg"],
+        ms, st = time_apply(torch, plan, x, y, args.steps, args.warmup, nvec, stride)
+        res[pr] = dict(ms=ms, tflops=nvec * info["flops_alg"] / ms / 1e9, flops_alg=info["flops_alg"],
                        flops_ref=info["flops_ref"], stage_ms=st, stage_flops=[si["flops1"], si["flops2"]],
                        tile_class=[si["tile_class1"], si["tile_class2"]], tuning=tuning)
         log(pr, f"{ms:.3f} ms/matvec", f"{res[pr]['tflops']:.1f} TFLOP/s", "stages(ms)", [round(s, 3) for s in st])
@@ -195,13 +217,14 @@ def main():
     # dominant kernel of the mixed matvec: the slower GEMM stage
     k = 1 if r["stage_ms"][1] >= r["stage_ms"][3] else 3
     stage = 1 if k == 1 else 2
-    fl = r["stage_flops"][stage - 1]
+    fl = nvec * r["stage_flops"][stage - 1]
     achieved = fl / r["stage_ms"][k] / 1e9
     mf = MFMA_FLOPS_PER_ALG.get(mixed, 1.0)
     roof = {"bound": "mfma", "achieved": round(achieved * mf, 2), "peak": PEAK[mixed], "unit": "TFLOP/s",
-            "frac": round(achieved * mf / PEAK[mixed], 4), "traffic": profiled_traffic(mixed, stage),
-            "kernel": "gemm_bf16_%s_kernel %s, stage %d (%s)" % ("pipe" if r["tile_class"][stage - 1] == 3 else "stream",
-                                                           TILE.get(r["tile_class"][stage - 1], "?"), stage, mixed),
+            "frac": round(achieved * mf / PEAK[mixed], 4),
+            "traffic": profiled_traffic(mixed, stage) if args.workload == "dense" and nvec == 1 else None,
+            "kernel": "%s %s, stage %d (%s)" % (KERNEL.get(r["tile_class"][stage - 1], "gemm_bf16_kernel"),
+                                                 TILE.get(r["tile_class"][stage - 1], "?"), stage, mixed),
             "stage_ms": round(r["stage_ms"][k], 4), "stage_flops_alg": fl,
             "note": "achieved = MFMA flops of the stage (%.0fx its algorithmic flops for %s) / its average "
                     "HIP-event duration over the timed steps; traffic = HBM bytes per launch from "
@@ -214,9 +237,8 @@ def main():
         log("cpu baseline: oracle contractTwosite on host cores (3 matvecs)")
         dt, cores, blas = cpu_baseline(prob)
         cpu = {"value": round(r["flops_alg"] / dt / 1e12, 4), "unit": "TFLOP/s", "cores": cores, "kind": "port",
-               "sample": f"3 two-site matvecs at D={args.D} (dense Heisenberg centre bond, the reference "
-                         f"tuple loop: one {blas} dgemm pair per tuple, F_ref = 120 D^3 flops), "
-                         f"{dt:.2f} s wall per matvec; value = F_alg / t"}
+               "sample": f"3 two-site matvecs of the same bond ({args.workload}; the reference tuple loop: one "
+                         f"{blas} dgemm pair per tuple), {dt:.2f} s wall per matvec; value = F_alg / t"}
     out = {
         "metric": "H_eff\u00b7\u03c8 TFLOP/s + DMRG sweep s at D=2048; mixed vs FP64 at |\u0394E/L|\u22641e-9",
         "value": round(r["tflops"], 3),
@@ -230,9 +252,7 @@ def main():
         "vs_baseline": None,
         "dtype": "bf16x3 (split bf16 in, fp32 acc)" if mixed == "bf16x3" else mixed,
         "data": "synthetic",
-        "config": {"workload": f"dense Heisenberg J=1 centre bond, L={args.L}, w=5, d=2, D={args.D}, seeded "
-                               f"Gaussian MPS, QR-normalised, GPU-built environments",
-                   "global_batch": 1, "parallelism": "single GPU"},
+        "config": {"workload": workload, "global_batch": nvec, "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "by_precision": {pr: {"ms": round(v["ms"], 4), "tflops": round(v["tflops"], 3),

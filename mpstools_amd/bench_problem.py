@@ -121,3 +121,68 @@ def dense_heisenberg_centre(D=2048, L=128, seed=20260003, build_envs="sweep", pr
         plan.bind_env(dL, dR)
         out["plan_" + pr] = plan
     return out
+
+
+def u1_centre(model="tv", L=100, N=50, D=512, seed=20260002, precisions=("f64",), verbose=False):
+    """Centre bond of a U(1)-symmetric chain: t-V (t = 1, V = 2, mu = 0) or XXZ
+    (Jz = Jxy = 1, hz = 0), N particles (up spins), bond dimension D split over
+    charge sectors by the package's binomial profile (synth.bond_dims).
+
+    The MPS is this package's seeded generator (synth.random_mps, host numpy),
+    brought to mixed-canonical form around (c, c+1) on the host; the
+    environments L[c-1] and R[c+2] are built on the GPU layer by layer with the
+    environment plans (addContractionLayer, src/MPSfunctions.hpp:442-493), from
+    getLMin / getRMax (:1917-1926); psi is BlockMatrix<2>(A_c, A_{c+1})
+    (src/BlockTensor.hpp:285-301), normalised."""
+    import torch
+    from . import synth
+    from .blocks import twosite
+    dev = "cuda"
+    if model == "tv":
+        mpo = models.tv_mpo(L, 1.0, 2.0, 0.0)
+    elif model == "xxz":
+        mpo = models.xxz_mpo(L, 1.0, 1.0, 0.0)
+    else:
+        raise ValueError(model)
+    itk = [models.itokey_u1()] * L
+    c = L // 2 - 1
+    mps, dims = synth.random_mps(L, itk, N, D, seed)
+    mps = synth.mixed_canonical(mps, c)
+
+    def run(Et, dE, s, direction):
+        At = native.Table.from_blocks(mps[s])
+        outt = native.env_outkeys(Et, At, mpo[s], itk[s], direction)
+        plan = native.EnvPlan(Et, At, mpo[s], itk[s], direction, outt)
+        dA = torch.from_numpy(At.pack(mps[s])).to(dev)
+        dout = torch.zeros(max(outt.total, 1), dtype=torch.float64, device=dev)
+        plan.apply(dE, dA, dout)
+        return outt, dout
+
+    Et, dE = _lmin_table(), torch.ones(1, dtype=torch.float64, device=dev)
+    for s in range(0, c):
+        Et, dE = run(Et, dE, s, 1)
+        if verbose and s % 16 == 0:
+            print(f"[bench_problem] left env site {s}", file=sys.stderr, flush=True)
+    Lt, dL = Et, dE
+    # getRMax: identity on the last bond's charge (N), aux 0
+    Et = native.Table.from_keys(1, np.array([[N, N]], np.uint64), np.zeros((1, 1), np.uint32), [1], [1])
+    dE = torch.ones(1, dtype=torch.float64, device=dev)
+    for s in range(L - 1, c + 1, -1):
+        Et, dE = run(Et, dE, s, 0)
+        if verbose and s % 16 == 0:
+            print(f"[bench_problem] right env site {s}", file=sys.stderr, flush=True)
+    Rt, dR = Et, dE
+    psi_b = twosite(mps[c], mps[c + 1])
+    psi_b = psi_b.scaled(1.0 / psi_b.norm())
+    xt = native.Table.from_blocks(psi_b)
+    yt = native.heff2_outkeys(Lt, Rt, xt, mpo[c], mpo[c + 1], itk[c], itk[c + 1])
+    vt = xt.union(yt)
+    psi = torch.from_numpy(vt.pack(psi_b)).to(dev)
+    out = dict(mpo=mpo, itk=itk[c], c=c, dims=dims, Lt=Lt, Rt=Rt, dL=dL, dR=dR, xt=xt, yt=yt, vt=vt, psi=psi, L=L,
+               D=D, psi_blocks=len(xt.keys()), psi_block_dims=(int(min(xt.rows.min(), xt.cols.min())),
+                                                             int(max(xt.rows.max(), xt.cols.max()))))
+    for pr in precisions:
+        plan = native.HeffPlan(Lt, Rt, vt, vt, (mpo[c], mpo[c + 1]), (itk[c], itk[c + 1]), native.PREC[pr])
+        plan.bind_env(dL, dR)
+        out["plan_" + pr] = plan
+    return out
