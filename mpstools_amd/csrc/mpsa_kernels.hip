@@ -1952,7 +1952,10 @@ static hipError_t run16(int tr, const GTask* tasks, int ntasks, const GSeg* segs
                         int64_t a_vs, const void* B, int64_t b_vs, void* C, int64_t c_vs, hipStream_t st) {
   const int total = tpv * nvec;
   if (total <= 0) return hipSuccess;
-  int grid = num_cu() * 2;
+  // persistent workgroups per CU: 64x64 tiles (92 VGPRs in FP64, 64 in FP32)
+  // run several per CU so that the short-k tiles of small sectors overlap
+  // their load latency; 128x128 keeps two
+  int grid = num_cu() * (BM == 64 ? (sizeof(T) == 4 ? 8 : 5) : 2);
   if (grid > total) grid = total;
   if (grid >= 8) grid &= ~7;
   const T* a = (const T*)A;
@@ -1974,7 +1977,7 @@ static hipError_t runbf(const GTask* tasks, int ntasks, const GSeg* segs, int tp
                         int64_t c_pl, int c_rowmajor, hipStream_t st) {
   const int total = tpv * nvec;
   if (total <= 0) return hipSuccess;
-  int grid = num_cu() * (BM == 256 ? 1 : 2);
+  int grid = num_cu() * (BM == 256 ? 1 : (BM == 64 ? 5 : 2));  // 64x64: at most 86 VGPRs
   if (grid > total) grid = total;
   if (grid >= 8) grid &= ~7;
   if constexpr (BM == 256) {
