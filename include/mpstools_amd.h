@@ -197,7 +197,11 @@ int mpsa_plan_vector_lengths(const mpsa_plan* plan, int64_t* x_len, int64_t* y_l
  * tridiagonal eigensolve runs on the host in FP64.  energies receives the
  * sorted Ritz values (at most cap), *nit the iterations, *nritz their count;
  * dstate (may be NULL) the Ritz vector `target`.  The sign of a Ritz vector
- * is that of the tridiagonal eigensolver (as with dstev, arbitrary). */
+ * is that of the tridiagonal eigensolver (as with dstev, arbitrary).
+ * With testenergy the run is mpsa_lanczos_batched with one vector and graph
+ * replay (coefficients on the device, one host read-back per 5-step test;
+ * the plan's stage timing is switched off); the residual test is driven from
+ * the host step by step. */
 int mpsa_lanczos(mpsa_plan* plan, const double* dpsi0, double* dstate, int itmax, double delta, int testenergy,
                  int target, double* energies, int cap, int* nit, int* nritz, mpsa_stream stream);
 /* nvec independent runs of mpsa_lanczos's recurrence (Ritz-value convergence

@@ -184,6 +184,13 @@ int mpsa_lanczos(mpsa_plan* plan, const double* dpsi0, double* dstate, int itmax
     chk(mpsa_plan_vector_lengths(plan, &xl, &yl, &same));
     if (!same) throw LErr(MPSA_EINVAL, "lanczos: the plan's x and y tables must be the same vector space");
     const int64_t n = xl;
+    // The energy test runs as one vector of the batched recurrence: norms and
+    // alphas stay on the device, five steps per graph replay, one read-back
+    // per test -- the same steps, stops and Ritz vector as the loop below,
+    // without a host round trip per step.
+    if (testenergy)
+      return mpsa_lanczos_batched(plan, dpsi0, std::max<int64_t>(n, 1), 1, dstate, itmax, delta, target, energies,
+                                  cap, nit, nritz, 1, stream);
     hipStream_t st = (hipStream_t)stream;
     Vecs V{{}, n, st};
     std::vector<double*> As;
