@@ -41,6 +41,13 @@ class LocalOp:
 
     __rmul__ = __mul__
 
+    def __add__(self, other):
+        # SparseOperator addition: entries of both, summed on common keys
+        out = LocalOp(self.d, self.e)
+        for k, v in other.e.items():
+            out.e[k] = out.e.get(k, 0.0) + v
+        return out
+
     def dense(self):
         m = np.zeros((self.d, self.d), dtype=np.result_type(*[type(v) for v in self.e.values()], float)
                      if self.e else float)
@@ -225,3 +232,98 @@ def itokey_u1(d=2):
 def itokey_dense(d=2):
     """No symmetry: every local state carries charge 0 (dense blocks)."""
     return [0] * d
+
+
+def pack_charge(q1, q2):
+    """AbKeyType(q1, q2, 2): the two U(1) charges packed as (q2 << 16) + q1
+    (src/keyTypes.hpp:25-30); sums and differences act on the packed integer."""
+    return (int(q2) << 16) + int(q1)
+
+
+def itokey_u1xu1():
+    """i_to_key of the Hubbard site (main/TO_solver_new.cpp:407-415): states
+    0 (empty), 1 (down), 2 (up), 3 (up and down) carry (N_up, N_down) =
+    (0,0), (0,1), (1,0), (1,1)."""
+    return [pack_charge(0, 0), pack_charge(0, 1), pack_charge(1, 0), pack_charge(1, 1)]
+
+
+def _hubbard_ops():
+    # src/Hamiltonians.hpp:829-843: the down fermion is created after the up
+    # one inside a site (|3> = c+_up c+_dn |0>), p is the site's fermion parity
+    nu = LocalOp(4, {(2, 2): 1.0, (3, 3): 1.0})
+    nd = LocalOp(4, {(1, 1): 1.0, (3, 3): 1.0})
+    cuD = LocalOp(4, {(2, 0): 1.0, (3, 1): 1.0})
+    cu = LocalOp(4, {(0, 2): 1.0, (1, 3): 1.0})
+    cdD = LocalOp(4, {(1, 0): 1.0, (3, 2): -1.0})
+    cd = LocalOp(4, {(0, 1): 1.0, (2, 3): -1.0})
+    Id = LocalOp(4, {(k, k): 1.0 for k in range(4)})
+    p = LocalOp(4, {(0, 0): 1.0, (1, 1): -1.0, (2, 2): -1.0, (3, 3): 1.0})
+    return nu, nd, cuD, cu, cdD, cd, Id, p
+
+
+def fermi_hubbard_mpo(hop, U, ch_up, ch_down, V=None):
+    """FermiHubbardMPO (src/Hamiltonians.hpp:823-889; with the doublon-doublon
+    term V, :891-962), w = 6 (7 with V), d = 4:
+      H = sum_i -hop_i sum_s (c+_{i s} c_{i+1 s} + h.c.)
+          + sum_i (ch_up_i n_{i up} + ch_down_i n_{i dn} + U_i n_{i up} n_{i dn})
+          [+ sum_i V_i (n_up n_dn)_i (n_up n_dn)_{i+1}]
+    with the reference's aux layout, operator products and squeeze(1e-15)."""
+    N = len(ch_up)
+    assert len(ch_down) == N
+    nu, nd, cuD, cu, cdD, cd, Id, p = _hubbard_ops()
+    n = nu * nd
+
+    def onsite(s):
+        return ch_up[s] * nu + ch_down[s] * nd + U[s] * (nu * nd)
+
+    w = 6 if V is None else 7
+    last = w - 1
+    mpo = []
+    M = MPOMat(1, w, 4)
+    M[(0, 0)] = onsite(0)
+    M[(0, 1)] = -hop[0] * (cuD * p)
+    M[(0, 2)] = -hop[0] * (cdD * p)
+    M[(0, 3)] = hop[0] * (cu * p)
+    M[(0, 4)] = hop[0] * (cd * p)
+    if V is not None:
+        M[(0, 5)] = V[0] * n
+    M[(0, last)] = Id
+    M.squeeze(1e-15)
+    mpo.append(M)
+    for site in range(1, N - 1):
+        Mb = MPOMat(w, w, 4)
+        Mb[(0, 0)] = Id
+        Mb[(1, 0)] = cu
+        Mb[(2, 0)] = cd
+        Mb[(3, 0)] = cuD
+        Mb[(4, 0)] = cdD
+        if V is not None:
+            Mb[(5, 0)] = n
+        Mb[(last, 0)] = onsite(site)
+        Mb[(last, 1)] = -hop[site] * (cuD * p)
+        Mb[(last, 2)] = -hop[site] * (cdD * p)
+        Mb[(last, 3)] = hop[site] * (cu * p)
+        Mb[(last, 4)] = hop[site] * (cd * p)
+        if V is not None:
+            Mb[(last, 5)] = V[site] * n
+        Mb[(last, last)] = Id
+        Mb.squeeze(1e-15)
+        mpo.append(Mb)
+    M = MPOMat(w, 1, 4)
+    M[(0, 0)] = Id
+    M[(1, 0)] = cu
+    M[(2, 0)] = cd
+    M[(3, 0)] = cuD
+    M[(4, 0)] = cdD
+    if V is not None:
+        M[(5, 0)] = n
+    M[(last, 0)] = onsite(N - 1)
+    M.squeeze(1e-15)
+    mpo.append(M)
+    return mpo
+
+
+def hubbard_mpo(N, t=1.0, U=4.0, ch_up=0.0, ch_down=0.0, V=None):
+    """Uniform Fermi-Hubbard chain of N sites (optional uniform V)."""
+    return fermi_hubbard_mpo([t] * (N - 1), [U] * N, [ch_up] * N, [ch_down] * N,
+                             None if V is None else [V] * (N - 1))

@@ -51,10 +51,19 @@ def build(oracle, model, L, N, D, seed, c=None, dense=False, **kw):
         mpo = models.tv_mpo(L, kw.get("t", 1.0), kw.get("V", 2.0), kw.get("mu", 0.0))
     elif model == "asym":
         mpo = asym_mpo(L)
+    elif model == "hubbard":
+        mpo = models.hubbard_mpo(L, kw.get("t", 1.0), kw.get("U", 4.0), kw.get("ch_up", 0.0), kw.get("ch_down", 0.0),
+                                 kw.get("V", None))
     else:
         raise ValueError(model)
-    itk = [models.itokey_dense() if dense else models.itokey_u1()] * L
-    mps, psi, c, dims = synth.centre_problem(L, itk, 0 if dense else N, D, seed, c)
+    if model == "hubbard":
+        # N electrons, N // 2 up (U(1) x U(1) charges packed as AbKeyType)
+        itk = [models.itokey_dense(4) if dense else models.itokey_u1xu1()] * L
+        Ntot = 0 if dense else models.pack_charge(kw.get("Nup", N // 2), N - kw.get("Nup", N // 2))
+    else:
+        itk = [models.itokey_dense() if dense else models.itokey_u1()] * L
+        Ntot = 0 if dense else N
+    mps, psi, c, dims = synth.centre_problem(L, itk, Ntot, D, seed, c)
     Ls = oracle.left_envs(mps, mpo, itk, c - 1)
     Rs = oracle.right_envs(mps, mpo, itk, c + 1)
     return dict(mpo=mpo, itk=itk, mps=mps, psi=psi, c=c, dims=dims, Ls=Ls, Rs=Rs)

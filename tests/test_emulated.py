@@ -197,3 +197,38 @@ def test_emulated_tile_classes_agree_and_autotune_keeps_ties(lib, oracle):
     plan.apply(x, y)
     assert np.array_equal(y.host(), ref)
     del plan
+
+
+@pytest.mark.parametrize("V", [None, 0.7])
+def test_emulated_hubbard_plans_match_oracle(lib, oracle, V):
+    """Plans over U(1) x U(1) packed charges and d = 4 (FermiHubbardMPO): the
+    two-site H_eff and both environment updates against the oracle, with every
+    device access in bounds."""
+    P = problems.build(oracle, "hubbard", 8, 8, 24, 33, U=4.0, V=V)
+    c, mpo, itk = P["c"], P["mpo"], P["itk"]
+    L, R, psi = P["Ls"][c - 1], P["Rs"][c + 2], P["psi"]
+    Lt, Rt, xt = native.Table.from_blocks(L), native.Table.from_blocks(R), native.Table.from_blocks(psi)
+    vt = xt.union(native.heff2_outkeys(Lt, Rt, xt, mpo[c], mpo[c + 1], itk[c], itk[c + 1]))
+    plan = native.HeffPlan(Lt, Rt, vt, vt, (mpo[c], mpo[c + 1]), (itk[c], itk[c + 1]), native.F64)
+    dL, dR = DevBuf(lib, Lt.pack(L)), DevBuf(lib, Rt.pack(R))
+    plan.bind_env(dL, dR)
+    x, y = DevBuf(lib, vt.pack(psi)), DevBuf(lib, np.zeros(vt.total))
+    plan.apply(x, y)
+    ref = oracle.contract_twosite(L, R, psi, mpo[c], mpo[c + 1], itk[c], itk[c + 1])
+    y0 = vt.unpack(y.host())
+    num = sum(np.sum((y0[k] - ref[k]) ** 2) for k in ref.keys())
+    den = sum(np.sum(ref[k] ** 2) for k in ref.keys())
+    assert np.sqrt(num / den) <= 1e-12
+    del plan
+    for direction, E, A, Wm in ((1, L, P["mps"][c], mpo[c]), (0, R, P["mps"][c + 1], mpo[c + 1])):
+        Et, At = native.Table.from_blocks(E), native.Table.from_blocks(A)
+        outt = native.env_outkeys(Et, At, Wm, itk[c], direction)
+        ep = native.EnvPlan(Et, At, Wm, itk[c], direction, outt)
+        out = DevBuf(lib, np.zeros(max(outt.total, 1)))
+        ep.apply(DevBuf(lib, Et.pack(E)), DevBuf(lib, At.pack(A)), out)
+        got = outt.unpack(out.host()[:outt.total] if outt.total else np.zeros(0))
+        want = oracle.add_layer(E, A, Wm, itk[c], direction)
+        assert got.keys() == want.keys()
+        for k in want.keys():
+            assert np.max(np.abs(got[k] - want[k])) <= 1e-12 * max(1.0, np.max(np.abs(want[k])))
+        del ep

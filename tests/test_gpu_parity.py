@@ -33,9 +33,11 @@ CASES = [
     ("xxz", 12, 0, 48, 104, True),
     ("tv", 24, 12, 150, 105, False),
 ]
+# Fermi-Hubbard (d = 4, U(1) x U(1) packed charges), U = 4, 6 up + 6 down
+HUB = [("hubbard", 12, 12, 64, 108, False)]
 
 
-@pytest.mark.parametrize("model,L,N,D,seed,dense", CASES)
+@pytest.mark.parametrize("model,L,N,D,seed,dense", CASES + HUB)
 def test_heff2_f64_matches_oracle(gpu, oracle, model, L, N, D, seed, dense):
     _say("heff2 f64", model, L, N, D, seed)
     P = problems.build(oracle, model, L, N, D, seed, dense=dense)
@@ -49,7 +51,7 @@ def test_heff2_f64_matches_oracle(gpu, oracle, model, L, N, D, seed, dense):
     assert _rel(y, y_ref) <= 1e-12
 
 
-@pytest.mark.parametrize("model,L,N,D,seed,dense", CASES[:4])
+@pytest.mark.parametrize("model,L,N,D,seed,dense", CASES[:4] + HUB)
 def test_heff1_f64_matches_oracle(gpu, oracle, model, L, N, D, seed, dense):
     _say("heff1 f64", model, L, N, D, seed)
     P = problems.build(oracle, model, L, N, D, seed, dense=dense)
@@ -61,7 +63,7 @@ def test_heff1_f64_matches_oracle(gpu, oracle, model, L, N, D, seed, dense):
 
 
 @pytest.mark.parametrize("direction", [1, 0])
-@pytest.mark.parametrize("model,L,N,D,seed,dense", CASES)
+@pytest.mark.parametrize("model,L,N,D,seed,dense", CASES + HUB)
 def test_env_f64_matches_oracle(gpu, oracle, model, L, N, D, seed, dense, direction):
     _say("env f64", direction, model, L, N, D, seed)
     P = problems.build(oracle, model, L, N, D, seed, dense=dense)
@@ -97,7 +99,7 @@ def test_env_mixed_variant_matches_oracle(gpu, oracle):
 @pytest.mark.parametrize("prec,tol", [(native.F32, 2e-6), (native.BF16, 3e-2), (native.BF16X3, 1e-4),
                                       (native.F64 | native.PLAIN, 1e-12)])
 @pytest.mark.parametrize("model,L,N,D,seed,dense", [CASES[2], CASES[3], CASES[4], ("xxz", 20, 0, 512, 106, True),
-                                                     ("xxz", 24, 0, 720, 107, True)])
+                                                     ("xxz", 24, 0, 720, 107, True)] + HUB)
 def test_heff2_reduced_precision_bounds(gpu, oracle, model, L, N, D, seed, dense, prec, tol):
     """FP32 inputs/accumulate: relative error <= 2e-6; BF16 inputs with FP32
     accumulate: <= 3e-2 (bf16 has an 8-bit significand, unit roundoff 2^-9);
@@ -200,7 +202,8 @@ def _lanczos_setup(gpu, oracle, model, L, N, D, seed):
     return P, (Lt, Rt, vt, dL, dR, x0)
 
 
-@pytest.mark.parametrize("model,L,N,D,seed", [("xxz", 12, 6, 24, 111), ("tv", 20, 10, 60, 112)])
+@pytest.mark.parametrize("model,L,N,D,seed", [("xxz", 12, 6, 24, 111), ("tv", 20, 10, 60, 112),
+                                              ("hubbard", 10, 10, 48, 113)])
 def test_lanczos_matches_oracle(gpu, oracle, model, L, N, D, seed):
     """Device Lanczos (same recurrence and stopping rules as TSLanzcosEngine)
     against the oracle's: 10 fixed steps give the same Ritz values; run to
