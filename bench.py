@@ -59,7 +59,7 @@ KERNEL = {0: "gemm_bf16_stream_kernel", 1: "gemm_bf16_kernel", 2: "gemm_bf16_ker
 # U(1) bonds: (model, L, N, D, seed) -- the t-V chain at half filling, the XXZ
 # chain of the gradient workload, the small XXZ chain
 WORKLOADS = {"dense": None, "tv512": ("tv", 100, 50, 512, 20260002), "xxz1024": ("xxz", 64, 32, 1024, 20260004),
-             "xxz64": ("xxz", 32, 16, 64, 20260001)}
+             "xxz64": ("xxz", 32, 16, 64, 20260001), "hub1024": ("hubbard", 80, 80, 1024, 20260005)}
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "gemm_bf16_pipe_dense_D2048_hbm_traffic.txt")
 
 
@@ -177,7 +177,10 @@ def main():
         model, L, N, D, seed = WORKLOADS[args.workload]
         log(f"building U(1) {model} centre problem L={L} N={N} D={D}")
         prob = bench_problem.u1_centre(model, L, N, D, seed, precisions=precs, verbose=True)
-        workload = (f"U(1) {'t-V t=1 V=2' if model == 'tv' else 'XXZ Jz=Jxy=1'} centre bond, L={L}, N={N}, w=5, d=2, "
+        name = {"tv": "U(1) t-V t=1 V=2", "xxz": "U(1) XXZ Jz=Jxy=1",
+                "hubbard": "U(1)xU(1) Fermi-Hubbard t=1 U=4 (real FP64, N_up=N_dn)"}[model]
+        workload = (f"{name} centre bond, L={L}, N={N}, w={6 if model == 'hubbard' else 5}, "
+                    f"d={4 if model == 'hubbard' else 2}, "
                     f"D={D} over charge sectors ({prob['psi_blocks']} psi blocks, dims "
                     f"{prob['psi_block_dims'][0]}-{prob['psi_block_dims'][1]}), seeded MPS, GPU-built environments")
     torch.cuda.synchronize()

@@ -124,9 +124,11 @@ def dense_heisenberg_centre(D=2048, L=128, seed=20260003, build_envs="sweep", pr
 
 
 def u1_centre(model="tv", L=100, N=50, D=512, seed=20260002, precisions=("f64",), verbose=False):
-    """Centre bond of a U(1)-symmetric chain: t-V (t = 1, V = 2, mu = 0) or XXZ
-    (Jz = Jxy = 1, hz = 0), N particles (up spins), bond dimension D split over
-    charge sectors by the package's binomial profile (synth.bond_dims).
+    """Centre bond of a U(1)-symmetric chain: t-V (t = 1, V = 2, mu = 0), XXZ
+    (Jz = Jxy = 1, hz = 0) or Fermi-Hubbard (t = 1, U = 4, U(1) x U(1)), N
+    particles (up spins; electrons for Hubbard, half of them up), bond
+    dimension D split over charge sectors by the package's profile
+    (synth.bond_dims).
 
     The MPS is this package's seeded generator (synth.random_mps, host numpy),
     brought to mixed-canonical form around (c, c+1) on the host; the
@@ -142,9 +144,15 @@ def u1_centre(model="tv", L=100, N=50, D=512, seed=20260002, precisions=("f64",)
         mpo = models.tv_mpo(L, 1.0, 2.0, 0.0)
     elif model == "xxz":
         mpo = models.xxz_mpo(L, 1.0, 1.0, 0.0)
+    elif model == "hubbard":  # t = 1, U = 4; N electrons, N // 2 of them up
+        mpo = models.hubbard_mpo(L, 1.0, 4.0)
     else:
         raise ValueError(model)
-    itk = [models.itokey_u1()] * L
+    if model == "hubbard":
+        itk = [models.itokey_u1xu1()] * L
+        N = models.pack_charge(N // 2, N - N // 2)
+    else:
+        itk = [models.itokey_u1()] * L
     c = L // 2 - 1
     mps, dims = synth.random_mps(L, itk, N, D, seed)
     mps = synth.mixed_canonical(mps, c)

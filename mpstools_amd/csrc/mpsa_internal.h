@@ -105,23 +105,17 @@ hipError_t launch_lz_combine(double* const* basis, const double* c, int ldc, con
                              int64_t vs, int nvec, hipStream_t st);
 
 // Workgroup tile of each (precision, tile class).  FP64/FP32: 0 = 128x128,
-// 1 = 64x64.  BF16/BF16X3: 0 = 256x128, 1 = 64x64, 2 = 128x128, 3 = 256x256.
+// 1 = 64x64, 2 = 32x32.  BF16/BF16X3: 0 = 256x128, 1 = 64x64, 2 = 128x128,
+// 3 = 256x256.
 inline void tile_dims(int prec, int cls, int* bm, int* bn) {
-  if ((prec == BF16 || prec == BF16X3) && cls == 3) {
-    *bm = 256;
-    *bn = 256;
-  } else if (cls == 1) {
-    *bm = 64;
-    *bn = 64;
-  } else if ((prec == BF16 || prec == BF16X3) && cls == 0) {
-    *bm = 256;
-    *bn = 128;
-  } else {
-    *bm = 128;
-    *bn = 128;
-  }
+  static const int bf[4][2] = {{256, 128}, {64, 64}, {128, 128}, {256, 256}};
+  static const int fp[3][2] = {{128, 128}, {64, 64}, {32, 32}};
+  const bool b = prec == BF16 || prec == BF16X3;
+  const int c = cls < 0 ? 0 : (b ? (cls > 3 ? 3 : cls) : (cls > 2 ? 2 : cls));
+  *bm = b ? bf[c][0] : fp[c][0];
+  *bn = b ? bf[c][1] : fp[c][1];
 }
-inline int num_tile_classes(int prec) { return (prec == BF16 || prec == BF16X3) ? 4 : 2; }
+inline int num_tile_classes(int prec) { return (prec == BF16 || prec == BF16X3) ? 4 : 3; }
 void set_last_error(const char* msg);
 
 }  // namespace mpsa

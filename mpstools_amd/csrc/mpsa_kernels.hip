@@ -1952,10 +1952,10 @@ static hipError_t run16(int tr, const GTask* tasks, int ntasks, const GSeg* segs
                         int64_t a_vs, const void* B, int64_t b_vs, void* C, int64_t c_vs, hipStream_t st) {
   const int total = tpv * nvec;
   if (total <= 0) return hipSuccess;
-  // persistent workgroups per CU: 64x64 tiles (92 VGPRs in FP64, 64 in FP32)
-  // run several per CU so that the short-k tiles of small sectors overlap
-  // their load latency; 128x128 keeps two
-  int grid = num_cu() * (BM == 64 ? (sizeof(T) == 4 ? 8 : 5) : 2);
+  // persistent workgroups per CU: the small tiles (64x64: 92 VGPRs in FP64,
+  // 64 in FP32; 32x32 fewer) run several per CU so that the short-k tiles of
+  // small sectors overlap their load latency; 128x128 keeps two
+  int grid = num_cu() * (BM == 32 ? 8 : (BM == 64 ? (sizeof(T) == 4 ? 8 : 5) : 2));
   if (grid > total) grid = total;
   if (grid >= 8) grid &= ~7;
   const T* a = (const T*)A;
@@ -2010,18 +2010,20 @@ static hipError_t runbf(const GTask* tasks, int ntasks, const GSeg* segs, int tp
 hipError_t launch_gemm(int prec_in, int out_kind, int tr, int tile_class, const GTask* tasks, int ntasks,
                        const GSeg* segs, int tiles_per_vec, int nvec, const void* A, int64_t a_vs, const void* B,
                        int64_t b_vs, void* C, int64_t c_vs, hipStream_t st, Planes planes) {
-  if (prec_in == F64) {
-    if (tile_class == 0) return run16<double, double, 128, 128>(tr, tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, st);
-    return run16<double, double, 64, 64>(tr, tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, st);
-  }
+#define MPSA_RUN16(T_, TO_)                                                                                    \
+  do {                                                                                                       \
+    if (tile_class == 0)                                                                                     \
+      return run16<T_, TO_, 128, 128>(tr, tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, st); \
+    if (tile_class == 2)                                                                                     \
+      return run16<T_, TO_, 32, 32>(tr, tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, st);   \
+    return run16<T_, TO_, 64, 64>(tr, tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, st);     \
+  } while (0)
+  if (prec_in == F64) MPSA_RUN16(double, double);
   if (prec_in == F32) {
-    if (out_kind == 1) {
-      if (tile_class == 0) return run16<float, double, 128, 128>(tr, tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, st);
-      return run16<float, double, 64, 64>(tr, tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, st);
-    }
-    if (tile_class == 0) return run16<float, float, 128, 128>(tr, tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, st);
-    return run16<float, float, 64, 64>(tr, tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, st);
+    if (out_kind == 1) MPSA_RUN16(float, double);
+    MPSA_RUN16(float, float);
   }
+#undef MPSA_RUN16
   // BF16 / BF16X3: operands in the k-contiguous layout; out_kind 0 -> bf16
   // row-major (hi/lo planes for BF16X3), 1 -> double column-major.
 #define MPSA_BF_DISPATCH(TOUT, SPLIT_, APL, BPL, CPL, ROWM)                                                           \

@@ -241,8 +241,9 @@ static bool class_admissible(const std::vector<GTask>& t, const std::vector<GSeg
 }
 
 static int pick_class(const std::vector<GTask>& t, const std::vector<GSeg>& segs, int prec) {
-  int best = 1;
+  int best = -1, fallback = 1;
   int best_area = 0;
+  double fallback_fill = -1.0;
   const bool pipe_ok = fits_pipe_tile(t, segs);
   for (int cls = 0; cls < num_tile_classes(prec); ++cls) {
     if (cls == 3 && !pipe_ok) continue;
@@ -253,12 +254,17 @@ static int pick_class(const std::vector<GTask>& t, const std::vector<GSeg>& segs
       useful += (double)x.m * x.n;
       padded += (double)((x.m + bm - 1) / bm) * ((x.n + bn - 1) / bn) * (double)bm * bn;
     }
-    if (padded > 0 && useful / padded >= 0.8 && bm * bn > best_area) {
+    const double fill = padded > 0 ? useful / padded : 1.0;
+    if (fill >= 0.8 && bm * bn > best_area) {
       best = cls;
       best_area = bm * bn;
     }
+    if (fill > fallback_fill) {  // no tile fills 80 %: the least padded one
+      fallback = cls;
+      fallback_fill = fill;
+    }
   }
-  return best;
+  return best >= 0 ? best : fallback;
 }
 
 // Number the workgroup tiles of a stage for tile class cls, in task order

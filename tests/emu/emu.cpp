@@ -398,7 +398,7 @@ hipError_t launch_gemm(int prec_in, int out_kind, int tr, int tile_class, const 
                        const GSeg* segs, int tiles_per_vec, int nvec, const void* A, int64_t a_vs, const void* B,
                        int64_t b_vs, void* C, int64_t c_vs, hipStream_t st, Planes) {
   if (tiles_per_vec * nvec <= 0) return hipSuccess;
-  const int bm = tile_class == 0 ? 128 : 64;
+  const int bm = tile_class == 0 ? 128 : (tile_class == 2 ? 32 : 64);
   const bool TA = (tr >> 1) & 1, TB = tr & 1;
   if (prec_in == F64) {
     submit(st, "gemm f64", [=]() {

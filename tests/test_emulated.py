@@ -170,8 +170,8 @@ def test_emulated_tile_classes_agree_and_autotune_keeps_ties(lib, oracle):
     plan.bind_env(dL, dR)
     x = DevBuf(lib, vt.pack(psi))
     outs = {}
-    for c1 in (0, 1):
-        for c2 in (0, 1):
+    for c1 in (0, 1, 2):
+        for c2 in (0, 1, 2):
             plan.set_tile_class(1, c1)
             plan.set_tile_class(2, c2)
             assert (plan.stage_info()["tile_class1"], plan.stage_info()["tile_class2"]) == (c1, c2)
@@ -185,13 +185,13 @@ def test_emulated_tile_classes_agree_and_autotune_keeps_ties(lib, oracle):
     num = sum(np.sum((y0[k] - y_ref[k]) ** 2) for k in y_ref.keys())
     den = sum(np.sum(y_ref[k] ** 2) for k in y_ref.keys())
     assert np.sqrt(num / den) <= 1e-12
-    for bad in ((1, 2), (2, 3), (1, -1), (3, 0)):
+    for bad in ((1, 3), (2, 4), (1, -1), (3, 0)):
         with pytest.raises(native.MpsaError):
             plan.set_tile_class(*bad)
     plan.set_tile_class(1, 1)
     plan.set_tile_class(2, 0)
     ms = plan.autotune(nvec=2, reps=1)
-    assert set(ms) == {(1, 0), (1, 1), (2, 0), (2, 1)}
+    assert set(ms) == {(1, 0), (1, 1), (1, 2), (2, 0), (2, 1), (2, 2)}
     assert (plan.stage_info()["tile_class1"], plan.stage_info()["tile_class2"]) == (1, 0)
     y = DevBuf(lib, np.zeros(vt.total))
     plan.apply(x, y)
