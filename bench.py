@@ -54,8 +54,9 @@ def time_apply(torch, plan, x, y, steps, warmup, nvec=1, stride=0):
     return ms, st
 
 
-TILE = {0: "256x128 stream, 4 waves", 1: "64x64", 2: "128x128", 3: "256x256 pipelined, 4 waves"}
-KERNEL = {0: "gemm_bf16_stream_kernel", 1: "gemm_bf16_kernel", 2: "gemm_bf16_kernel", 3: "gemm_bf16_pipe_kernel"}
+TILE = {0: "256x128 stream, 4 waves", 1: "64x64", 2: "128x128", 3: "256x256 pipelined, 4 waves", 4: "32x32"}
+KERNEL = {0: "gemm_bf16_stream_kernel", 1: "gemm_bf16_kernel", 2: "gemm_bf16_kernel", 3: "gemm_bf16_pipe_kernel",
+          4: "gemm_bf16_kernel"}
 # U(1) bonds: (model, L, N, D, seed) -- the t-V chain at half filling, the XXZ
 # chain of the gradient workload, the small XXZ chain
 WORKLOADS = {"dense": None, "tv512": ("tv", 100, 50, 512, 20260002), "xxz1024": ("xxz", 64, 32, 1024, 20260004),

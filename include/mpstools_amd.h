@@ -143,7 +143,8 @@ int mpsa_plan_info(const mpsa_plan* plan, double* flops_alg, double* flops_ref, 
                    int64_t* workspace_bytes);
 /* Algorithmic flops of the two grouped GEMM stages (one vector) and the tile
  * class each stage launches with.  FP64/FP32: 0 = 128x128, 1 = 64x64,
- * 2 = 32x32.  BF16/BF16X3: 0 = 256x128, 1 = 64x64, 2 = 128x128, 3 = 256x256. */
+ * 2 = 32x32.  BF16/BF16X3: 0 = 256x128, 1 = 64x64, 2 = 128x128, 3 = 256x256,
+ * 4 = 32x32. */
 int mpsa_plan_stage_info(const mpsa_plan* plan, double* flops_stage1, double* flops_stage2, int* tile_class1,
                          int* tile_class2);
 /* Per-stage HIP-event timing of mpsa_heff_apply on the call's stream.  When
@@ -163,9 +164,9 @@ int mpsa_plan_set_tile_class(mpsa_plan* plan, int stage, int tile_class);
  * tile class of each grouped GEMM stage on seeded pseudo-random vectors
  * (nvec vectors per launch, one warm-up and `reps` timed launches, HIP events
  * on `stream`) and keeps the fastest per stage (a tie keeps the plan's own
- * class).  ms (optional, 8 doubles): average milliseconds per launch of stage
- * 1 classes 0..3 then stage 2 classes 0..3, -1 for classes not admitted or
- * not defined for the precision.
+ * class).  ms (optional, 10 doubles): average milliseconds per launch of
+ * stage 1 classes 0..4 then stage 2 classes 0..4, -1 for classes not admitted
+ * or not defined for the precision.
  * Allocates and frees two vectors; synchronizes the stream. */
 int mpsa_plan_autotune(mpsa_plan* plan, int nvec, int reps, double* ms, mpsa_stream stream);
 /* Copy out a plan's device schedule for inspection: what = 0/1 stage-1 tasks /

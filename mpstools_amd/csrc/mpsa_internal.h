@@ -106,16 +106,17 @@ hipError_t launch_lz_combine(double* const* basis, const double* c, int ldc, con
 
 // Workgroup tile of each (precision, tile class).  FP64/FP32: 0 = 128x128,
 // 1 = 64x64, 2 = 32x32.  BF16/BF16X3: 0 = 256x128, 1 = 64x64, 2 = 128x128,
-// 3 = 256x256.
+// 3 = 256x256, 4 = 32x32.
+constexpr int kMaxTileClasses = 5;
 inline void tile_dims(int prec, int cls, int* bm, int* bn) {
-  static const int bf[4][2] = {{256, 128}, {64, 64}, {128, 128}, {256, 256}};
+  static const int bf[5][2] = {{256, 128}, {64, 64}, {128, 128}, {256, 256}, {32, 32}};
   static const int fp[3][2] = {{128, 128}, {64, 64}, {32, 32}};
   const bool b = prec == BF16 || prec == BF16X3;
-  const int c = cls < 0 ? 0 : (b ? (cls > 3 ? 3 : cls) : (cls > 2 ? 2 : cls));
+  const int c = cls < 0 ? 0 : (b ? (cls > 4 ? 4 : cls) : (cls > 2 ? 2 : cls));
   *bm = b ? bf[c][0] : fp[c][0];
   *bn = b ? bf[c][1] : fp[c][1];
 }
-inline int num_tile_classes(int prec) { return (prec == BF16 || prec == BF16X3) ? 4 : 3; }
+inline int num_tile_classes(int prec) { return (prec == BF16 || prec == BF16X3) ? 5 : 3; }
 void set_last_error(const char* msg);
 
 }  // namespace mpsa

@@ -302,7 +302,10 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_kernel(const GTask* __restri
   constexpr int BK = 32, PK = BK + 8;
   constexpr int NP = SPLIT ? 2 : 1;
   constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
-  constexpr int NA = BM * BK / 8 / 256, NB = BN * BK / 8 / 256;  // 16-byte vectors per thread and plane
+  // 16-byte vectors per plane of a tile slab, and per thread (a 32-row tile
+  // has 128: half the threads load it)
+  constexpr int NVA = BM * BK / 8, NVB = BN * BK / 8;
+  constexpr int NA = (NVA + 255) / 256, NB = (NVB + 255) / 256;
   __shared__ __attribute__((aligned(16))) unsigned short As[NP][BM][PK];
   __shared__ __attribute__((aligned(16))) unsigned short Bs[NP][BN][PK];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -341,6 +344,7 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_kernel(const GTask* __restri
 #pragma unroll
       for (int r = 0; r < NA; ++r) {
         const int e = tid + r * 256;
+        if (NVA % 256 != 0 && e >= NVA) continue;
         const int i = e / (BK / 8), kc = (e % (BK / 8)) * 8;
         const int gi = row0 + i, gk = kk0 + kc;
         const bool ok = gi < m && gk < kpad;
@@ -355,6 +359,7 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_kernel(const GTask* __restri
 #pragma unroll
       for (int r = 0; r < NB; ++r) {
         const int e = tid + r * 256;
+        if (NVB % 256 != 0 && e >= NVB) continue;
         const int j = e / (BK / 8), kc = (e % (BK / 8)) * 8;
         const int gj = col0 + j, gk = kk0 + kc;
         const bool ok = gj < n && gk < kpad;
@@ -373,12 +378,14 @@ __global__ __launch_bounds__(256, 2) void gemm_bf16_kernel(const GTask* __restri
 #pragma unroll
         for (int r = 0; r < NA; ++r) {
           const int e = tid + r * 256;
+          if (NVA % 256 != 0 && e >= NVA) continue;
           *reinterpret_cast<u16x8*>(&As[p][e / (BK / 8)][(e % (BK / 8)) * 8]) =
               ((oka >> r) & 1u) ? ra[p][r] : u16x8{0, 0, 0, 0, 0, 0, 0, 0};
         }
 #pragma unroll
         for (int r = 0; r < NB; ++r) {
           const int e = tid + r * 256;
+          if (NVB % 256 != 0 && e >= NVB) continue;
           *reinterpret_cast<u16x8*>(&Bs[p][e / (BK / 8)][(e % (BK / 8)) * 8]) =
               ((okb >> r) & 1u) ? rb[p][r] : u16x8{0, 0, 0, 0, 0, 0, 0, 0};
         }
@@ -1977,7 +1984,7 @@ static hipError_t runbf(const GTask* tasks, int ntasks, const GSeg* segs, int tp
                         int64_t c_pl, int c_rowmajor, hipStream_t st) {
   const int total = tpv * nvec;
   if (total <= 0) return hipSuccess;
-  int grid = num_cu() * (BM == 256 ? 1 : (BM == 64 ? 5 : 2));  // 64x64: at most 86 VGPRs
+  int grid = num_cu() * (BM == 256 ? 1 : (BM == 64 ? 5 : (BM == 32 ? 8 : 2)));  // 64x64: at most 86 VGPRs
   if (grid > total) grid = total;
   if (grid >= 8) grid &= ~7;
   if constexpr (BM == 256) {
@@ -2037,6 +2044,9 @@ hipError_t launch_gemm(int prec_in, int out_kind, int tr, int tile_class, const 
     if (tile_class == 2)                                                                                             \
       return runbf<TOUT, 128, 128, SPLIT_>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, APL, B, b_vs, BPL, C, \
                                            c_vs, CPL, ROWM, st);                                                     \
+    if (tile_class == 4)                                                                                             \
+      return runbf<TOUT, 32, 32, SPLIT_>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, APL, B, b_vs, BPL, C,   \
+                                         c_vs, CPL, ROWM, st);                                                       \
     return runbf<TOUT, 64, 64, SPLIT_>(tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, APL, B, b_vs, BPL, C,     \
                                        c_vs, CPL, ROWM, st);                                                         \
   } while (0)

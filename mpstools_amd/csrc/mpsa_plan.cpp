@@ -1142,8 +1142,8 @@ int mpsa_plan_autotune(mpsa_plan* P, int nvec, int reps, double* ms, mpsa_stream
       const int cur = S.tile_class;
       int best = -1;
       double best_ms = 0.0;
-      for (int cls = 0; cls < 4; ++cls) {
-        if (ms) ms[(stage - 1) * 4 + cls] = -1.0;
+      for (int cls = 0; cls < kMaxTileClasses; ++cls) {
+        if (ms) ms[(stage - 1) * kMaxTileClasses + cls] = -1.0;
         if (!class_admissible(S.tasks, S.segs, P->prec, cls)) continue;
         set_stage_class(P, S, P->prec, cls);
         launch_stage(P, stage, x, xs, y, ys, nvec, st);  // warm-up
@@ -1154,7 +1154,7 @@ int mpsa_plan_autotune(mpsa_plan* P, int nvec, int reps, double* ms, mpsa_stream
         float t = 0.f;
         HIPCHK(hipEventElapsedTime(&t, e0, e1));
         const double per = (double)t / reps;
-        if (ms) ms[(stage - 1) * 4 + cls] = per;
+        if (ms) ms[(stage - 1) * kMaxTileClasses + cls] = per;
         // the strictly fastest class wins; on a tie the plan's own class stays
         if (best < 0 || per < best_ms || (per == best_ms && cls == cur)) {
           best = cls;

@@ -23,6 +23,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.path.join(HERE, "libmpstools_amd.so")
 
 F64, F32, BF16, BF16X3 = 0, 1, 2, 3
+TILE_CLASSES = 5  # tile classes per GEMM stage in mpsa_plan_autotune's table
 PLAIN = 0x100
 PREC = {"f64": F64, "fp64": F64, "f32": F32, "fp32": F32, "bf16": BF16, "bf16x3": BF16X3, "f64plain": F64 | PLAIN}
 
@@ -280,9 +281,10 @@ class HeffPlan:
     def autotune(self, nvec=1, reps=3):
         """Time every admissible tile class of both stages (environments bound)
         and keep the fastest; returns {(stage, class): ms per launch}."""
-        ms = np.zeros(8)
+        ms = np.zeros(2 * TILE_CLASSES)
         _check(lib().mpsa_plan_autotune(self.h, int(nvec), int(reps), _ptr(ms), _stream()))
-        return {(st + 1, c): float(ms[st * 4 + c]) for st in range(2) for c in range(4) if ms[st * 4 + c] >= 0}
+        return {(st + 1, c): float(ms[st * TILE_CLASSES + c]) for st in range(2) for c in range(TILE_CLASSES)
+                if ms[st * TILE_CLASSES + c] >= 0}
 
     def last_times(self):
         """ms of [convert x, stage-1 GEMM, mix + zero fill, stage-2 GEMM] of the last apply."""
