@@ -137,6 +137,47 @@ def lanczos_compare(torch, prob, mixed, args):
             "solve_speedup": round(t64 / (tm + tr), 3)}
 
 
+def sweep_estimate(torch, prob, lan, L, D):
+    """Cost of one full-D bond step of a two-site sweep (DMRGengine::doStep,
+    src/container.hpp:293-337) from its measured parts on the bench bond: the
+    mixed Lanczos with FP64 refinement (from lanczos_compare), the split of
+    the two-site tensor keeping chi = D (split, src/MPSfunctions.hpp:1142-1278:
+    one 2D x 2D FP64 SVD here), and the environment update with the new left
+    tensor (addContractionLayer, :442-493).  A sweep is 2 (L - 2) such steps;
+    the bonds within log2(D) sites of the ends are cheaper, so the product is
+    an upper estimate."""
+    from mpstools_amd import native
+    vt, itk, c, mpo = prob["vt"], prob["itk"], prob["c"], prob["mpo"]
+    x = prob["psi"]
+    native.split(vt, x, itk, itk, direction=1, chimax=D)  # warm-up (library set-up)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    out = native.split(vt, x, itk, itk, direction=1, chimax=D)
+    torch.cuda.synchronize()
+    t_split = time.perf_counter() - t0
+    At, dA = out["left"]
+    outt = native.env_outkeys(prob["Lt"], At, mpo[c], itk, 1)
+    ep = native.EnvPlan(prob["Lt"], At, mpo[c], itk, 1, outt)
+    dout = torch.zeros(max(outt.total, 1), dtype=torch.float64, device="cuda")
+    ep.apply(prob["dL"], dA, dout)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        ep.apply(prob["dL"], dA, dout)
+    torch.cuda.synchronize()
+    t_env = (time.perf_counter() - t0) / 3
+    t_lan = lan["s_mixed"] + lan["s_refine"]
+    step = t_lan + t_split + t_env
+    log(f"bond step: lanczos {t_lan:.3f} s + split {t_split:.3f} s + env {t_env * 1e3:.1f} ms; "
+        f"sweep estimate {2 * (L - 2) * step:.1f} s")
+    return {"lanczos_s": round(t_lan, 4), "split_s": round(t_split, 4), "env_update_s": round(t_env, 5),
+            "bond_step_s": round(step, 4), "bond_steps": 2 * (L - 2), "sweep_s_estimate": round(2 * (L - 2) * step, 2),
+            "kept": out["chi"], "truncated_weight": out["tw"],
+            "note": "estimate from the measured parts of one full-D bond step (mixed Lanczos + FP64 "
+                    "refinement, split with chi = D, left environment update); the device sweep driver is "
+                    "switched off in this build"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,6 +195,8 @@ def main():
     ap.add_argument("--workload", default="dense", choices=sorted(WORKLOADS),
                     help="dense: the D=2048 dense bond (the headline); the others are U(1) bonds")
     ap.add_argument("--nvec", type=int, default=1, help="vectors per batched apply")
+    ap.add_argument("--sweep-estimate", action="store_true",
+                    help="time the split and environment update of the bond and estimate a sweep")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the plans' heuristic tile classes instead of autotuning them")
     ap.add_argument("--calib", action="store_true",
@@ -236,6 +279,9 @@ def main():
     lan = None
     if not args.no_lanczos and "f64" in res:
         lan = lanczos_compare(torch, prob, mixed, args)
+    sweep = None
+    if args.sweep_estimate and lan is not None and args.workload == "dense":
+        sweep = sweep_estimate(torch, prob, lan, args.L, args.D)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline: oracle contractTwosite on host cores (3 matvecs)")
@@ -273,6 +319,8 @@ def main():
         out["speedup_mixed_vs_f64_plain_method"] = round(res["f64plain"]["ms"] / r["ms"], 3)
     if lan is not None:
         out["lanczos"] = lan
+    if sweep is not None:
+        out["sweep_estimate"] = sweep
     print(json.dumps(out), flush=True)
 
 

@@ -17,8 +17,8 @@
 //     reported);
 //   * streams run synchronously; stream capture records the launches into a
 //     graph that hipGraphLaunch replays;
-//   * rocsolver_dgesvd checks its documented array extents and calls LAPACK
-//     dgesvd (loaded at run time, like the oracle's BLAS).
+//   * rocsolver_dgesvd / dgesvdj check their documented array extents and
+//     call LAPACK dgesvd (loaded at run time, like the oracle's BLAS).
 // The BF16 kernels are not emulated (their launch is reported).
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
@@ -282,6 +282,23 @@ rocblas_status rocblas_set_stream(rocblas_handle, hipStream_t) { return rocblas_
 rocblas_status rocsolver_dgesvd(rocblas_handle, const rocblas_svect ls, const rocblas_svect rs, const rocblas_int m,
                                 const rocblas_int n, double* A, const rocblas_int lda, double* Sg, double* U,
                                 const rocblas_int ldu, double* V, const rocblas_int ldv, double* E,
+                                const rocblas_workmode, rocblas_int* info);
+// gesvdj: the same arrays as gesvd without E, plus a residual and a sweep
+// count on the device (computed here by LAPACK dgesvd)
+rocblas_status rocsolver_dgesvdj(rocblas_handle h, const rocblas_svect ls, const rocblas_svect rs,
+                                 const rocblas_int m, const rocblas_int n, double* A, const rocblas_int lda,
+                                 const double, double* residual, const rocblas_int, rocblas_int* n_sweeps,
+                                 double* Sg, double* U, const rocblas_int ldu, double* V, const rocblas_int ldv,
+                                 rocblas_int* info) {
+  submit(nullptr, "rocsolver_dgesvdj outputs", [=]() {
+    if (check(residual, 8, "gesvdj residual")) *residual = 0.0;
+    if (check(n_sweeps, sizeof(rocblas_int), "gesvdj sweeps")) *n_sweeps = 1;
+  });
+  return rocsolver_dgesvd(h, ls, rs, m, n, A, lda, Sg, U, ldu, V, ldv, nullptr, rocblas_outofplace, info);
+}
+rocblas_status rocsolver_dgesvd(rocblas_handle, const rocblas_svect ls, const rocblas_svect rs, const rocblas_int m,
+                                const rocblas_int n, double* A, const rocblas_int lda, double* Sg, double* U,
+                                const rocblas_int ldu, double* V, const rocblas_int ldv, double* E,
                                 const rocblas_workmode, rocblas_int* info) {
   submit(nullptr, "rocsolver_dgesvd", [=]() {
     const int k = std::min(m, n);
@@ -294,7 +311,7 @@ rocblas_status rocsolver_dgesvd(rocblas_handle, const rocblas_svect ls, const ro
     ok = chk_range(Sg, 0, k - 1, "gesvd S") && ok;
     ok = chk_range(U, 0, (int64_t)ldu * (k - 1) + m - 1, "gesvd U") && ok;
     ok = chk_range(V, 0, (int64_t)ldv * (n - 1) + k - 1, "gesvd V") && ok;
-    if (k > 1) ok = chk_range(E, 0, k - 2, "gesvd E") && ok;
+    if (k > 1 && E) ok = chk_range(E, 0, k - 2, "gesvd E") && ok;
     ok = check(info, sizeof(rocblas_int), "gesvd info") && ok;
     if (!ok || k == 0) return;
     dgesvd_fn dgesvd_ = lapack_dgesvd();
