@@ -53,7 +53,7 @@ enum {
 };
 
 /* Arithmetic of the sector GEMMs.  Inputs and outputs at the ABI are always
- * FP64 (or complex128 interleaved when is_complex is set); F32 and BF16 plans
+ * FP64 (or planar complex128, see mpsa_blocks); F32 and BF16 plans
  * keep reduced-precision copies of the environments and convert psi on the fly.
  * BF16 means bf16 inputs with fp32 accumulation on MFMA; BF16X3 splits every
  * operand into hi + lo bf16 parts and accumulates hi*hi + hi*lo + lo*hi in fp32
@@ -67,7 +67,13 @@ enum { MPSA_PLAIN = 0x100 };
 typedef struct mpsa_blocks {
   int64_t n;              /* number of blocks */
   int32_t ns;             /* physical indices per key (1 or 2) */
-  int32_t is_complex;     /* 0: real double, 1: complex128 interleaved */
+  int32_t is_complex;     /* 0: real double.  1: complex128, planar: the real
+                             parts of the blocks at their offsets, then the
+                             imaginary parts at offset + extent, extent =
+                             max(offset + rows*cols); the buffer holds
+                             2*extent doubles.  H_eff and environment plans
+                             (FP64 only) take complex tables with a real MPO:
+                             every table of a plan complex, or none. */
   const uint64_t* q;      /* [2n] charges: left/upper, right/lower */
   const uint32_t* p;      /* [ns*n] physical indices or MPO aux index */
   const int64_t* rows;    /* [n] */
@@ -78,7 +84,7 @@ typedef struct mpsa_blocks {
 typedef struct mpsa_mpo {
   int64_t n;              /* number of nonzero entries */
   uint32_t Dl, Dr;        /* aux dimensions (MPOMat::Dl/Dr) */
-  int32_t is_complex;     /* values: n doubles, or 2n doubles (re, im) */
+  int32_t is_complex;     /* must be 0: MPO values are real doubles */
   const uint32_t* wl;
   const uint32_t* wr;
   const uint32_t* out;
@@ -188,7 +194,8 @@ int mpsa_plan_timing_summary(mpsa_plan* plan, float* ms4, int* napplies);
 int mpsa_plan_destroy(mpsa_plan* plan);
 
 /* Extents (elements) of the x and y layouts and whether they are the same
- * vector space (same keys, shapes and offsets). */
+ * vector space (same keys, shapes and offsets).  Complex plans report both
+ * planes (2 * extent doubles). */
 int mpsa_plan_vector_lengths(const mpsa_plan* plan, int64_t* x_len, int64_t* y_len, int* same_layout);
 
 /* ---- Lanczos (TSLanzcosEngine::simulate + getState, src/lanzcos.hpp:165-278, 81-92) ----
@@ -203,7 +210,12 @@ int mpsa_plan_vector_lengths(const mpsa_plan* plan, int64_t* x_len, int64_t* y_l
  * With testenergy the run is mpsa_lanczos_batched with one vector and graph
  * replay (coefficients on the device, one host read-back per 5-step test;
  * the plan's stage timing is switched off); the residual test is driven from
- * the host step by step. */
+ * the host step by step.
+ * On a complex plan the recurrence runs on the planar vector of 2 * extent
+ * doubles.  For a Hermitian H_eff the Lanczos vectors are real polynomials in
+ * H_eff applied to psi0, and Re<x|y> is the real dot product of the planar
+ * vectors, so this is the complex recurrence (same alpha, beta, Ritz values
+ * and states). */
 int mpsa_lanczos(mpsa_plan* plan, const double* dpsi0, double* dstate, int itmax, double delta, int testenergy,
                  int target, double* energies, int cap, int* nit, int* nritz, mpsa_stream stream);
 /* nvec independent runs of mpsa_lanczos's recurrence (Ritz-value convergence

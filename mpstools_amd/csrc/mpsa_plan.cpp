@@ -16,6 +16,7 @@
 #include <array>
 #include <cmath>
 #include <cstring>
+#include <initializer_list>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -60,7 +61,8 @@ struct Tab {
   std::vector<BKey> key;
   std::vector<int64_t> rows, cols, off;
   std::map<BKey, int> idx;
-  int64_t total = 0;
+  int64_t total = 0;  // extent of one plane: max(offset + rows * cols)
+  int cplx = 0;       // complex128, planar: real parts, then imaginary parts at + total
   int find(const BKey& k) const {
     auto it = idx.find(k);
     return it == idx.end() ? -1 : it->second;
@@ -72,12 +74,13 @@ static Tab parse(const mpsa_blocks* b, int ns_expect, const char* name) {
   if (b->n < 0) throw Error(MPSA_EINVAL, std::string(name) + ": negative block count");
   if (b->ns != ns_expect)
     throw Error(MPSA_EINVAL, std::string(name) + ": expected ns=" + std::to_string(ns_expect));
-  if (b->is_complex) throw Error(MPSA_EINVAL, std::string(name) + ": complex tables are not supported by this build");
+  if (b->is_complex != 0 && b->is_complex != 1) throw Error(MPSA_EINVAL, std::string(name) + ": bad is_complex");
   if (b->n > 0 && (!b->q || !b->p || !b->rows || !b->cols || !b->offset))
     throw Error(MPSA_EINVAL, std::string(name) + ": null array");
   Tab t;
   t.n = b->n;
   t.ns = b->ns;
+  t.cplx = b->is_complex;
   for (int64_t i = 0; i < b->n; ++i) {
     BKey k{b->q[2 * i], b->q[2 * i + 1], b->p[b->ns * i], b->ns == 2 ? b->p[b->ns * i + 1] : 0u};
     if (b->rows[i] < 0 || b->cols[i] < 0 || b->offset[i] < 0 || b->rows[i] > (1 << 30) || b->cols[i] > (1 << 30))
@@ -169,6 +172,7 @@ struct mpsa_plan {
   int rpL_chunks = 0, rpR_chunks = 0, rpx_chunks = 0;
   int64_t Lc_elems = 0, Rc_elems = 0;
   int planes = 1;  // 2: hi/lo bf16 planes (BF16X3)
+  int cplx = 0;    // complex128 operands in planar layout (FP64 only)
   // device
   MixTask* d_mix = nullptr;
   MixTerm* d_terms = nullptr;
@@ -590,6 +594,81 @@ static int64_t write_keys(const std::vector<BKey>& keys, const std::map<BKey, st
 //   heff: stage 1 T1 = P * Q  (P = L env, Q = x),  stage 2 y = T2 * U^T (U = R env)
 //   env>0: T1 = P * Q (P = BL, Q = A), out = U^T * T2 (U = Ap)
 //   env<=0: T1 = P * Q^T (P = BR, Q = A), out = U * T2 (U = Ap)
+// Complex128 plans keep every operand in planar layout: the real parts of its
+// blocks at the table's offsets, then the imaginary parts one plane extent
+// further on.  The MPO values are real, so each product a * op(A) * op(B) of
+// the real schedule becomes four real products on the planes,
+//   Re C += a (Ar Br - s Ai Bi),   Im C += a (Ar Bi + s Ai Br),
+// with s = -1 where A enters conjugated (the upper tensor of an environment
+// update: the reference's conj(A_in), src/MPSfunctions.hpp:335-341) and +1
+// otherwise.  Transposes act on both planes alike.  The real and the
+// imaginary half of an output block are separate GEMM tasks whose segments
+// accumulate in registers, so every output plane is written once.
+static void complexify_stage(Stage& S, int64_t pa, int64_t pb, int64_t pc, double s) {
+  std::vector<GTask> t2;
+  std::vector<GSeg> s2;
+  t2.reserve(2 * S.tasks.size());
+  s2.reserve(4 * S.segs.size());
+  for (int im = 0; im < 2; ++im)
+    for (const GTask& t : S.tasks) {
+      GTask u = t;
+      u.c_off = t.c_off + (im ? pc : 0);
+      u.seg0 = (int32_t)s2.size();
+      u.nseg = 2 * t.nseg;
+      for (int j = 0; j < t.nseg; ++j) {
+        const GSeg& g = S.segs[(size_t)(t.seg0 + j)];
+        GSeg x = g, y = g;
+        y.a_off += pa;  // the second product always takes A's imaginary plane
+        if (!im) {      // Ar Br, -s Ai Bi
+          y.b_off += pb;
+          y.alpha = -s * g.alpha;
+        } else {        // Ar Bi, s Ai Br
+          x.b_off += pb;
+          y.alpha = s * g.alpha;
+        }
+        s2.push_back(x);
+        s2.push_back(y);
+      }
+      t2.push_back(u);
+    }
+  S.tasks.swap(t2);
+  S.segs.swap(s2);
+}
+
+// Plane extents: P (stage-1 A: L or the old environment), Q (stage-1 B: x or
+// the lower tensor), U (R, or the upper tensor of an environment update), Y
+// (the output).  Doubles the plan's buffer extents and its flop counts
+// (a complex multiply-add is four real ones).
+static void complexify(mpsa_plan* P, int64_t pP, int64_t pQ, int64_t pU, int64_t pY, int kind) {
+  const int64_t wsr = P->ws_per_vec;
+  complexify_stage(P->s1, pP, pQ, wsr, 1.0);
+  if (kind != 2) complexify_stage(P->s2, wsr, pU, pY, 1.0);  // y = T R^T
+  else complexify_stage(P->s2, pU, wsr, pY, -1.0);           // out = op(conj(Aup)) T
+  const size_t nm = P->mix.size();
+  for (size_t i = 0; i < nm; ++i) {  // W mixes are real: the same sums on the imaginary plane
+    MixTask m = P->mix[i];
+    m.dst_off += wsr;
+    const int32_t t0 = (int32_t)P->terms.size();
+    for (int j = 0; j < m.nt; ++j) {
+      MixTerm x = P->terms[(size_t)(m.t0 + j)];
+      x.src_off += wsr;
+      P->terms.push_back(x);
+    }
+    m.t0 = t0;
+    P->mix.push_back(m);
+  }
+  const size_t nz = P->zero.size();
+  for (size_t i = 0; i < nz; ++i) P->zero.push_back(Range{P->zero[i].off + pY, P->zero[i].len});
+  P->ws_per_vec = 2 * wsr;
+  P->L_total = 2 * pP;
+  P->x_total = 2 * pQ;
+  P->R_total = 2 * pU;
+  P->y_total = 2 * pY;
+  P->flops_alg *= 4;
+  P->flops1 *= 4;
+  P->flops2 *= 4;
+}
+
 static void build_plan(mpsa_plan* P, Built& b, const Tab& Ptab, const Tab& Qtab, const Tab& Utab, const Tab& Ytab,
                        int kind, int direction) {
   const bool bf = P->prec == BF16 || P->prec == BF16X3;
@@ -775,6 +854,7 @@ static void build_plan(mpsa_plan* P, Built& b, const Tab& Ptab, const Tab& Qtab,
     if (!produced[(size_t)i] && Ytab.rows[(size_t)i] * Ytab.cols[(size_t)i] > 0)
       P->zero.push_back(Range{Ytab.off[(size_t)i], Ytab.rows[(size_t)i] * Ytab.cols[(size_t)i]});
 
+  if (P->cplx) complexify(P, Ptab.total, Qtab.total, Utab.total, Ytab.total, kind);
   finalize_stage(P->s1, P->prec);
   finalize_stage(P->s2, P->prec);
   finalize_mix(P);
@@ -934,6 +1014,16 @@ int mpsa_env_outkeys(const mpsa_blocks* BL, const mpsa_blocks* A, const mpsa_blo
   })
 }
 
+// 1 when every table of a plan is complex128, 0 when none is.
+static int complex_tables(std::initializer_list<const Tab*> t, int precision) {
+  int nc = 0;
+  for (const Tab* x : t) nc += x->cplx;
+  if (nc == 0) return 0;
+  if (nc != (int)t.size()) throw Error(MPSA_EINVAL, "complex plans need every table complex128");
+  if (precision != F64) throw Error(MPSA_EINVAL, "complex plans run in FP64 only");
+  return 1;
+}
+
 static int plan_heff(mpsa_plan** plan, const mpsa_blocks* L, const mpsa_blocks* R, const mpsa_blocks* x,
                      const mpsa_blocks* y, const std::vector<Tuple>& tup, double fref, int nsx, int precision) {
   (void)nsx;
@@ -946,6 +1036,7 @@ static int plan_heff(mpsa_plan** plan, const mpsa_blocks* L, const mpsa_blocks* 
   P->kind = 0;
   P->prec = precision;
   Tab tl = parse(L, 1, "L"), tr = parse(R, 1, "R"), tx = parse(x, x->ns, "x"), ty = parse(y, y->ns, "y");
+  P->cplx = complex_tables({&tl, &tr, &tx, &ty}, precision);
   P->L_total = tl.total;
   P->R_total = tr.total;
   P->x_total = tx.total;
@@ -959,7 +1050,7 @@ static int plan_heff(mpsa_plan** plan, const mpsa_blocks* L, const mpsa_blocks* 
       throw Error(MPSA_EINVAL, "y table block shape differs from the produced block");
   }
   build_plan(P.get(), b, tl, tx, tr, ty, 0, 1);
-  P->flops_ref = fref;
+  P->flops_ref = P->cplx ? 4 * fref : fref;
   P->xy_same = (tx.n == ty.n && tx.ns == ty.ns && tx.rows == ty.rows && tx.cols == ty.cols && tx.off == ty.off &&
                 tx.key == ty.key) ? 1 : 0;
   *plan = P.release();
@@ -1015,6 +1106,7 @@ int mpsa_env_plan(mpsa_plan** plan, const mpsa_blocks* BL, const mpsa_blocks* A,
     P->kind = 2;
     P->prec = F64;
     P->direction = direction;
+    P->cplx = complex_tables({&tb, &ta, &tap, &to}, F64);
     P->L_total = tb.total;
     P->x_total = ta.total;
     P->R_total = tap.total;
@@ -1028,7 +1120,7 @@ int mpsa_env_plan(mpsa_plan** plan, const mpsa_blocks* BL, const mpsa_blocks* A,
         throw Error(MPSA_EINVAL, "out table block shape differs from the produced block");
     }
     build_plan(P.get(), b, tb, ta, tap, to, 2, direction);
-    P->flops_ref = fref;
+    P->flops_ref = P->cplx ? 4 * fref : fref;
     *plan = P.release();
   })
 }

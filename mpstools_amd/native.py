@@ -114,8 +114,9 @@ def _ptr(a):
 class Table:
     """A key table (mpsa_blocks) with its arrays kept alive."""
 
-    def __init__(self, ns, q, p, rows, cols, offsets):
+    def __init__(self, ns, q, p, rows, cols, offsets, is_complex=0):
         self.ns = ns
+        self.is_complex = int(is_complex)
         self.q = np.ascontiguousarray(q, dtype=np.uint64).reshape(-1, 2)
         self.p = np.ascontiguousarray(p, dtype=np.uint32).reshape(-1, ns)
         self.rows = np.ascontiguousarray(rows, dtype=np.int64)
@@ -123,13 +124,23 @@ class Table:
         self.offsets = np.ascontiguousarray(offsets, dtype=np.int64)
         self.n = len(self.rows)
         self.total = int(max([0] + [int(o + r * c) for o, r, c in zip(self.offsets, self.rows, self.cols)]))
-        self.s = _Blocks(self.n, ns, 0, self.q.ctypes.data, self.p.ctypes.data, self.rows.ctypes.data,
+        self.s = _Blocks(self.n, ns, self.is_complex, self.q.ctypes.data, self.p.ctypes.data, self.rows.ctypes.data,
                          self.cols.ctypes.data, self.offsets.ctypes.data)
 
     @staticmethod
     def from_blocks(B: Blocks, align=64):
         t = B.table(align)
-        return Table(B.ns, t["q"], t["p"], t["rows"], t["cols"], t["offsets"])
+        return Table(B.ns, t["q"], t["p"], t["rows"], t["cols"], t["offsets"], B.dtype.kind == "c")
+
+    def as_complex(self) -> "Table":
+        """The same keys and offsets as a complex128 table (planar layout: the
+        real parts, then the imaginary parts one extent further on)."""
+        return Table(self.ns, self.q, self.p, self.rows, self.cols, self.offsets, 1)
+
+    @property
+    def length(self):
+        """Doubles of a packed buffer: the extent, twice for complex tables."""
+        return self.total * (2 if self.is_complex else 1)
 
     @staticmethod
     def from_keys(ns, q, p, rows, cols, align=64):
@@ -156,13 +167,20 @@ class Table:
         ks = sorted(d)
         q = np.array([[k[0], k[1]] for k in ks], np.uint64).reshape(-1, 2)
         p = np.array([k[2:] for k in ks], np.uint32).reshape(-1, self.ns)
-        return Table.from_keys(self.ns, q, p, [d[k][0] for k in ks], [d[k][1] for k in ks], align)
+        t = Table.from_keys(self.ns, q, p, [d[k][0] for k in ks], [d[k][1] for k in ks], align)
+        return t.as_complex() if self.is_complex else t
 
     def pack(self, B: Blocks, dtype=np.float64):
+        if self.is_complex:
+            z = B.pack(table=self.as_dict()).astype(np.complex128)
+            return np.concatenate([z.real, z.imag]).astype(dtype)
         return B.pack(table=self.as_dict()).astype(dtype)
 
     def unpack(self, buf, keep=None) -> Blocks:
-        return Blocks.unpack(self.ns, self.as_dict(), np.asarray(buf), keep=keep)
+        buf = np.asarray(buf)
+        if self.is_complex:
+            buf = buf[:self.total] + 1j * buf[self.total:2 * self.total]
+        return Blocks.unpack(self.ns, self.as_dict(), buf, keep=keep)
 
 
 class MPOArg:
@@ -579,15 +597,38 @@ def _to_dev(t: Table, B: Blocks):
     return torch.from_numpy(t.pack(B)).cuda()
 
 
+def _promote(*bs):
+    """Complex128 copies of all tables when any of them is complex (a complex
+    plan takes complex tables only), else the tables themselves."""
+    if not any(b is not None and b.dtype.kind == "c" for b in bs):
+        return bs
+    out = []
+    for b in bs:
+        if b is None or b.dtype.kind == "c":
+            out.append(b)
+            continue
+        z = Blocks(b.ns, np.complex128)
+        for k, v in b.b.items():
+            z[k] = v
+        out.append(z)
+    return tuple(out)
+
+
+def _out_table(t: Table, cplx: bool) -> Table:
+    return t.as_complex() if cplx else t
+
+
 def contract_twosite(L: Blocks, R: Blocks, psi: Blocks, Wl, Wr, itl, itr, precision=F64) -> Blocks:
     """contractTwosite (src/MPSfunctions.hpp:949-981): returns H_eff psi with the
-    reference's output key set."""
+    reference's output key set.  Complex128 blocks (with the real MPO) run a
+    complex plan."""
     import torch
+    L, R, psi = _promote(L, R, psi)
     Lt, Rt, xt = Table.from_blocks(L), Table.from_blocks(R), Table.from_blocks(psi)
-    yt = heff2_outkeys(Lt, Rt, xt, Wl, Wr, itl, itr)
+    yt = _out_table(heff2_outkeys(Lt, Rt, xt, Wl, Wr, itl, itr), xt.is_complex)
     plan = HeffPlan(Lt, Rt, xt, yt, (Wl, Wr), (itl, itr), precision)
     dL, dR, dx = _to_dev(Lt, L), _to_dev(Rt, R), _to_dev(xt, psi)
-    dy = torch.zeros(max(yt.total, 1), dtype=torch.float64, device="cuda")
+    dy = torch.zeros(max(yt.length, 1), dtype=torch.float64, device="cuda")
     plan.bind_env(dL, dR)
     plan.apply(dx, dy)
     torch.cuda.synchronize()
@@ -597,11 +638,12 @@ def contract_twosite(L: Blocks, R: Blocks, psi: Blocks, Wl, Wr, itl, itr, precis
 def contract_single(L: Blocks, A: Blocks, W, R: Blocks, itk, precision=F64) -> Blocks:
     """contractSingle (src/MPSfunctions.hpp:857-885)."""
     import torch
+    L, A, R = _promote(L, A, R)
     Lt, Rt, xt = Table.from_blocks(L), Table.from_blocks(R), Table.from_blocks(A)
-    yt = heff1_outkeys(Lt, Rt, xt, W, itk)
+    yt = _out_table(heff1_outkeys(Lt, Rt, xt, W, itk), xt.is_complex)
     plan = HeffPlan(Lt, Rt, xt, yt, W, itk, precision)
     dL, dR, dx = _to_dev(Lt, L), _to_dev(Rt, R), _to_dev(xt, A)
-    dy = torch.zeros(max(yt.total, 1), dtype=torch.float64, device="cuda")
+    dy = torch.zeros(max(yt.length, 1), dtype=torch.float64, device="cuda")
     plan.bind_env(dL, dR)
     plan.apply(dx, dy)
     torch.cuda.synchronize()
@@ -613,13 +655,14 @@ def add_contraction_layer(BL: Blocks, A: Blocks, W, itk, direction, Ap: Blocks |
     import torch
     if len(A) == 0:
         raise MpsaError(1, "Error in addContractionLayer: A is empty.")
+    BL, A, Ap = _promote(BL, A, Ap)
     BLt, At = Table.from_blocks(BL), Table.from_blocks(A)
     Apt = Table.from_blocks(Ap) if Ap is not None else None
-    outt = env_outkeys(BLt, At, W, itk, direction, Apt)
+    outt = _out_table(env_outkeys(BLt, At, W, itk, direction, Apt), At.is_complex)
     plan = EnvPlan(BLt, At, W, itk, direction, outt, Apt)
     dBL, dA = _to_dev(BLt, BL), _to_dev(At, A)
     dAp = _to_dev(Apt, Ap) if Ap is not None else None
-    dout = torch.zeros(max(outt.total, 1), dtype=torch.float64, device="cuda")
+    dout = torch.zeros(max(outt.length, 1), dtype=torch.float64, device="cuda")
     plan.apply(dBL, dA, dout, dAp)
     torch.cuda.synchronize()
     return outt.unpack(dout.cpu().numpy())

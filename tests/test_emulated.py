@@ -232,3 +232,63 @@ def test_emulated_hubbard_plans_match_oracle(lib, oracle, V):
         for k in want.keys():
             assert np.max(np.abs(got[k] - want[k])) <= 1e-12 * max(1.0, np.max(np.abs(want[k])))
         del ep
+
+
+@pytest.mark.parametrize("model,L,N,D,seed", [("tv", 10, 5, 12, 41), ("hubbard", 6, 6, 12, 42)])
+def test_emulated_complex_plans_match_oracle(lib, oracle, model, L, N, D, seed):
+    """Complex128 plans (planar re/im storage, real MPO): two-site and one-site
+    H_eff and both environment updates, plain and mixed, against the complex
+    oracle (itself pinned against real contractions in test_complex.py), with
+    every device access in bounds."""
+    import complexcase as cc
+    P = problems.build(oracle, model, L, N, D, seed)
+    c, mpo, itk = P["c"], P["mpo"], P["itk"]
+    Lz, Rz = cc.with_imag(P["Ls"][c - 1], seed + 1), cc.with_imag(P["Rs"][c + 2], seed + 2)
+    xz = cc.with_imag(P["psi"], seed + 3)
+    Lt, Rt, xt = native.Table.from_blocks(Lz), native.Table.from_blocks(Rz), native.Table.from_blocks(xz)
+    assert Lt.is_complex and Lt.length == 2 * Lt.total
+    vt = native.heff2_outkeys(Lt, Rt, xt, mpo[c], mpo[c + 1], itk[c], itk[c + 1]).as_complex()
+    plan = native.HeffPlan(Lt, Rt, xt, vt, (mpo[c], mpo[c + 1]), (itk[c], itk[c + 1]), native.F64)
+    dL, dR = DevBuf(lib, Lt.pack(Lz)), DevBuf(lib, Rt.pack(Rz))
+    plan.bind_env(dL, dR)
+    x, y = DevBuf(lib, xt.pack(xz)), DevBuf(lib, np.zeros(vt.length))
+    plan.apply(x, y)
+    want = oracle.contract_twosite(Lz, Rz, xz, mpo[c], mpo[c + 1], itk[c], itk[c + 1], cplx=True)
+    got = vt.unpack(y.host())
+    assert cc.rel(got, {k: np.asarray(want[k]) for k in want.keys()}) <= 1e-12
+    with pytest.raises(native.MpsaError):  # complex plans are FP64 only
+        native.HeffPlan(Lt, Rt, xt, vt, (mpo[c], mpo[c + 1]), (itk[c], itk[c + 1]), native.BF16X3)
+    with pytest.raises(native.MpsaError):  # every table complex, or none
+        native.HeffPlan(native.Table.from_blocks(P["Ls"][c - 1]), Rt, xt, vt, (mpo[c], mpo[c + 1]),
+                        (itk[c], itk[c + 1]), native.F64)
+    del plan
+    # one site
+    R1 = cc.with_imag(P["Rs"][c + 1], seed + 4)
+    Az = cc.with_imag(P["mps"][c], seed + 5)
+    R1t, At = native.Table.from_blocks(R1), native.Table.from_blocks(Az)
+    yt = native.heff1_outkeys(Lt, R1t, At, mpo[c], itk[c]).as_complex()
+    p1 = native.HeffPlan(Lt, R1t, At, yt, mpo[c], itk[c], native.F64)
+    p1.bind_env(dL, DevBuf(lib, R1t.pack(R1)))
+    y1 = DevBuf(lib, np.zeros(yt.length))
+    p1.apply(DevBuf(lib, At.pack(Az)), y1)
+    want = oracle.contract_single(Lz, Az, mpo[c], R1, itk[c], cplx=True)
+    got = yt.unpack(y1.host())
+    assert cc.rel(got, {k: np.asarray(want[k]) for k in want.keys()}) <= 1e-12
+    del p1
+    # environments
+    for direction, E, A in ((1, Lz, P["mps"][c]), (0, Rz, P["mps"][c + 1])):
+        site = c if direction > 0 else c + 1
+        Ez, Azz, Uz = E, cc.with_imag(A, seed + 6), cc.with_imag(A, seed + 7)
+        for up in (None, Uz):
+            Et, At2 = native.Table.from_blocks(Ez), native.Table.from_blocks(Azz)
+            Ut = native.Table.from_blocks(up) if up is not None else None
+            outt = native.env_outkeys(Et, At2, mpo[site], itk[site], direction, Ut).as_complex()
+            ep = native.EnvPlan(Et, At2, mpo[site], itk[site], direction, outt, Ut)
+            out = DevBuf(lib, np.zeros(max(outt.length, 1)))
+            ep.apply(DevBuf(lib, Et.pack(Ez)), DevBuf(lib, At2.pack(Azz)), out,
+                     DevBuf(lib, Ut.pack(up)) if up is not None else None)
+            got = outt.unpack(out.host())
+            want = oracle.add_layer(Ez, Azz, mpo[site], itk[site], direction, Ap=up, cplx=True)
+            assert got.keys() == want.keys()
+            assert cc.rel(got, {k: np.asarray(want[k]) for k in want.keys()}) <= 1e-12, (direction, up is None)
+            del ep
