@@ -26,7 +26,8 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK = {"f64": 78.6, "f64plain": 78.6, "f32": 157.3, "bf16": 2500.0, "bf16x3": 2500.0}  # MI355X dense TFLOP/s
+PEAK = {"f64": 78.6, "f64plain": 78.6, "f32": 157.3, "bf16": 2500.0, "bf16x3": 2500.0,
+        "c128": 78.6}  # MI355X dense TFLOP/s (complex128 runs on the FP64 MFMA path)
 MFMA_FLOPS_PER_ALG = {"bf16x3": 3.0}  # MFMA work per algorithmic flop
 
 
@@ -230,9 +231,12 @@ def main():
     torch.cuda.synchronize()
     log(f"problem built in {time.perf_counter() - t0:.1f}s")
     nvec = max(1, args.nvec)
-    stride = prob["vt"].total if nvec > 1 else 0
-    x = prob["psi"].repeat(nvec) if nvec > 1 else prob["psi"]
-    y = torch.zeros_like(x)
+
+    def vectors(pr):
+        """Start vector(s) of precision pr's plan (complex plans: planar re/im)."""
+        v = prob.get("psi_" + pr, prob["psi"])
+        x_ = v.repeat(nvec) if nvec > 1 else v
+        return x_, torch.zeros_like(x_), (v.numel() if nvec > 1 else 0)
     if args.calib:
         from mpstools_amd import native
         n = (1 << 30) // 8
@@ -254,6 +258,7 @@ def main():
             tuning = {"stage%d" % s_: {str(c): round(ms_, 4) for (s2, c), ms_ in sorted(t.items()) if s2 == s_}
                       for s_ in (1, 2)}
         si = plan.stage_info()
+        x, y, stride = vectors(pr)
         ms, st = time_apply(torch, plan, x, y, args.steps, args.warmup, nvec, stride)
         res[pr] = dict(ms=ms, tflops=nvec * info["flops_alg"] / ms / 1e9, flops_alg=info["flops_alg"],
                        flops_ref=info["flops_ref"], stage_ms=st, stage_flops=[si["flops1"], si["flops2"]],

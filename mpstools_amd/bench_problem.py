@@ -45,6 +45,33 @@ def _isometry(torch, G):
     return Q
 
 
+def _add_plan(torch, out, pr, W, itk, seed):
+    """out["plan_" + pr]: the H_eff plan of the bond at precision pr.  "c128"
+    is the complex128 plan (FP64, planar re/im) over the same keys, bound to
+    complex environments made of the real ones plus seeded imaginary parts of
+    the same magnitude, with a complex start vector in out["psi_c128"]."""
+    Lt, Rt, vt, dL, dR = out["Lt"], out["Rt"], out["vt"], out["dL"], out["dR"]
+    if pr != "c128":
+        plan = native.HeffPlan(Lt, Rt, vt, vt, W, itk, native.PREC[pr])
+        plan.bind_env(dL, dR)
+        out["plan_" + pr] = plan
+        return
+    g = torch.Generator(device=dL.device)
+    g.manual_seed(seed + 128)
+
+    def cplx(v):
+        im = torch.randn(v.numel(), dtype=torch.float64, device=v.device, generator=g) * v.abs().mean()
+        return torch.cat([v, im])
+
+    out["dL_c128"], out["dR_c128"] = cplx(dL), cplx(dR)
+    x = cplx(out["psi"])
+    out["psi_c128"] = x / x.norm()
+    vz = vt.as_complex()
+    plan = native.HeffPlan(Lt.as_complex(), Rt.as_complex(), vz, vz, W, itk, native.F64)
+    plan.bind_env(out["dL_c128"], out["dR_c128"])
+    out["plan_c128"] = plan
+
+
 def dense_heisenberg_centre(D=2048, L=128, seed=20260003, build_envs="sweep", precisions=("f64",), verbose=False):
     import torch
     dev = "cuda"
@@ -117,9 +144,7 @@ def dense_heisenberg_centre(D=2048, L=128, seed=20260003, build_envs="sweep", pr
     psi /= psi.norm()
     out = dict(mpo=mpo, itk=itk, c=c, dims=dims, Lt=Lt, Rt=Rt, dL=dL, dR=dR, xt=xt, yt=yt, vt=vt, psi=psi, L=L, D=D)
     for pr in precisions:
-        plan = native.HeffPlan(Lt, Rt, vt, vt, (mpo[c], mpo[c + 1]), (itk, itk), native.PREC[pr])
-        plan.bind_env(dL, dR)
-        out["plan_" + pr] = plan
+        _add_plan(torch, out, pr, (mpo[c], mpo[c + 1]), (itk, itk), seed)
     return out
 
 
@@ -190,7 +215,5 @@ def u1_centre(model="tv", L=100, N=50, D=512, seed=20260002, precisions=("f64",)
                D=D, psi_blocks=len(xt.keys()), psi_block_dims=(int(min(xt.rows.min(), xt.cols.min())),
                                                              int(max(xt.rows.max(), xt.cols.max()))))
     for pr in precisions:
-        plan = native.HeffPlan(Lt, Rt, vt, vt, (mpo[c], mpo[c + 1]), (itk[c], itk[c + 1]), native.PREC[pr])
-        plan.bind_env(dL, dR)
-        out["plan_" + pr] = plan
+        _add_plan(torch, out, pr, (mpo[c], mpo[c + 1]), (itk[c], itk[c + 1]), seed)
     return out
