@@ -252,9 +252,9 @@ const char* hipGetErrorString(hipError_t e) { return e == hipSuccess ? "no error
 // ---------------------------------------------------------------------------
 typedef void (*dgesvd_fn)(const char*, const char*, const int*, const int*, double*, const int*, double*, double*,
                           const int*, double*, const int*, double*, const int*, int*);
-// LAPACK dgesvd_ from the first library found (MKL or the system LAPACK)
-static dgesvd_fn lapack_dgesvd() {
-  static dgesvd_fn f = nullptr;
+// The first LAPACK library found (MKL or the system LAPACK)
+static void* lapack_lib() {
+  static void* lib = nullptr;
   static bool tried = false;
   if (!tried) {
     tried = true;
@@ -263,16 +263,21 @@ static dgesvd_fn lapack_dgesvd() {
     for (const char* c : cands) {
       if (!c) continue;
       void* h = dlopen(c, RTLD_NOW | RTLD_LOCAL);
-      if (h && (f = (dgesvd_fn)dlsym(h, "dgesvd_"))) {
+      if (h && dlsym(h, "dgesvd_")) {
         // MKL: the GNU threading layer, as the oracle asks for (the Intel
         // layer beside the GNU OpenMP runtime makes the oracle's loops racy)
         typedef int (*set_layer_fn)(int);
         if (set_layer_fn sl = (set_layer_fn)dlsym(h, "MKL_Set_Threading_Layer")) sl(3);
+        lib = h;
         break;
       }
     }
   }
-  return f;
+  return lib;
+}
+static dgesvd_fn lapack_dgesvd() {
+  void* h = lapack_lib();
+  return h ? (dgesvd_fn)dlsym(h, "dgesvd_") : nullptr;
 }
 rocblas_status rocblas_create_handle(rocblas_handle* h) {
   *h = reinterpret_cast<rocblas_handle>(new int(1));
@@ -325,6 +330,78 @@ rocblas_status rocsolver_dgesvd(rocblas_handle, const rocblas_svect ls, const ro
     lwork = (int)wq;
     std::vector<double> work((size_t)std::max(lwork, 1));
     dgesvd_("S", "S", &M, &N, A, &LDA, Sg, U, &LDU, V, &LDV, work.data(), &lwork, &inf);
+    *info = inf;
+  });
+  return rocblas_status_success;
+}
+
+// rocBLAS dgemm (host pointer mode), computed with plain loops
+rocblas_status rocblas_dgemm(rocblas_handle, rocblas_operation ta, rocblas_operation tb, rocblas_int m,
+                             rocblas_int n, rocblas_int k, const double* alpha, const double* A, rocblas_int lda,
+                             const double* B, rocblas_int ldb, const double* beta, double* C, rocblas_int ldc) {
+  const double al = *alpha, be = *beta;
+  submit(nullptr, "rocblas_dgemm", [=]() {
+    const bool tA = ta != rocblas_operation_none, tB = tb != rocblas_operation_none;
+    const int ar = tA ? k : m, ac = tA ? m : k, br = tB ? n : k, bc = tB ? k : n;
+    if (m < 0 || n < 0 || k < 0 || lda < std::max(1, ar) || ldb < std::max(1, br) || ldc < std::max(1, m)) {
+      report("dgemm: inconsistent arguments");
+      return;
+    }
+    if (m == 0 || n == 0) return;
+    bool ok = chk_range(C, 0, (int64_t)ldc * (n - 1) + m - 1, "dgemm C");
+    if (k > 0) {
+      ok = chk_range(A, 0, (int64_t)lda * (ac - 1) + ar - 1, "dgemm A") && ok;
+      ok = chk_range(B, 0, (int64_t)ldb * (bc - 1) + br - 1, "dgemm B") && ok;
+    }
+    if (!ok) return;
+    std::vector<double> out((size_t)m * n);
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < m; ++i) {
+        double acc = 0;
+        for (int l = 0; l < k; ++l)
+          acc += (tA ? A[l + (int64_t)i * lda] : A[i + (int64_t)l * lda]) *
+                 (tB ? B[j + (int64_t)l * ldb] : B[l + (int64_t)j * ldb]);
+        out[i + (size_t)j * m] = al * acc + (be == 0.0 ? 0.0 : be * C[i + (int64_t)j * ldc]);
+      }
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < m; ++i) C[i + (int64_t)j * ldc] = out[i + (size_t)j * m];
+  });
+  return rocblas_status_success;
+}
+
+// rocSOLVER dsyevd (LAPACK dsyev underneath): eigenvalues ascending in D,
+// eigenvectors over A; E is the documented n-element workspace
+rocblas_status rocsolver_dsyevd(rocblas_handle, const rocblas_evect evect, const rocblas_fill uplo,
+                                const rocblas_int n, double* A, const rocblas_int lda, double* D, double* E,
+                                rocblas_int* info) {
+  submit(nullptr, "rocsolver_dsyevd", [=]() {
+    if (evect != rocblas_evect_original || n < 0 || lda < std::max(1, n)) {
+      report("syevd: unsupported or inconsistent arguments");
+      return;
+    }
+    bool ok = check(info, sizeof(rocblas_int), "syevd info");
+    if (n > 0) {
+      ok = chk_range(A, 0, (int64_t)lda * (n - 1) + n - 1, "syevd A") && ok;
+      ok = chk_range(D, 0, n - 1, "syevd D") && ok;
+      ok = chk_range(E, 0, n - 1, "syevd E") && ok;
+    }
+    if (!ok || n == 0) return;
+    typedef void (*dsyev_fn)(const char*, const char*, const int*, double*, const int*, double*, double*,
+                             const int*, int*);
+    void* h = lapack_lib();
+    dsyev_fn dsyev_ = h ? (dsyev_fn)dlsym(h, "dsyev_") : nullptr;
+    if (!dsyev_) {
+      report("syevd: no LAPACK library found");
+      return;
+    }
+    const char* ul = uplo == rocblas_fill_upper ? "U" : "L";
+    int N = n, LDA = lda, lwork = -1, inf = 0;
+    double wq = 0;
+    dsyev_("V", ul, &N, A, &LDA, D, &wq, &lwork, &inf);
+    lwork = (int)wq;
+    std::vector<double> work((size_t)std::max(lwork, 1));
+    dsyev_("V", ul, &N, A, &LDA, D, work.data(), &lwork, &inf);
+    E[0] = 0.0;
     *info = inf;
   });
   return rocblas_status_success;

@@ -292,3 +292,56 @@ def test_emulated_complex_plans_match_oracle(lib, oracle, model, L, N, D, seed):
             assert got.keys() == want.keys()
             assert cc.rel(got, {k: np.asarray(want[k]) for k in want.keys()}) <= 1e-12, (direction, up is None)
             del ep
+
+
+@pytest.mark.parametrize("model,L,N,D,seed,dense", [("xxz", 12, 0, 48, 503, True), ("tv", 16, 8, 80, 507, False)])
+def test_emulated_split_matches_oracle(lib, oracle, model, L, N, D, seed, dense):
+    """The split's sector SVDs, the Gram-preconditioned Jacobi path included
+    (sectors with m >= n >= 64; the dense bond is one 96 x 96 sector), against
+    the numpy restatement: singular values within 1e-12 relative (LAPACK on
+    both sides here), the lambda-carrying tensor within 1e-10, isometries."""
+    import pysplit
+    P = problems.build(oracle, model, L, N, D, seed, dense=dense)
+    c, itk, psi = P["c"], P["itk"], P["psi"]
+    t = native.Table.from_blocks(psi)
+    dpsi = DevBuf(lib, t.pack(psi))
+    Lb = native.lib()
+    itl, itr = native._itk(itk[c]), native._itk(itk[c + 1])
+    h = C.c_void_p()
+    native._check(Lb.mpsa_split(C.byref(t.s), dpsi.p, native._ptr(itl), len(itl), native._ptr(itr), len(itr), 1,
+                                10 ** 6, 0, 0.0, 1, C.byref(h), C.c_void_p(0)))
+    tw, chi = np.zeros(1), np.zeros(1, np.int32)
+    nl, nr, nlam, ll, rl = (np.zeros(1, np.int64) for _ in range(5))
+    native._check(Lb.mpsa_split_info(h, native._ptr(tw), native._ptr(chi), native._ptr(nl), native._ptr(nr),
+                                     native._ptr(nlam), native._ptr(ll), native._ptr(rl)))
+    got = {}
+    for which, n, ln, name in ((0, int(nl[0]), int(ll[0]), "left"), (1, int(nr[0]), int(rl[0]), "right")):
+        q, p = np.zeros(2 * n, np.uint64), np.zeros(n, np.uint32)
+        r, cc_, o = np.zeros(n, np.int64), np.zeros(n, np.int64), np.zeros(n, np.int64)
+        native._check(Lb.mpsa_split_table(h, which, native._ptr(q), native._ptr(p), native._ptr(r), native._ptr(cc_),
+                                          native._ptr(o)))
+        tb = native.Table(1, q.reshape(-1, 2), p.reshape(-1, 1), r, cc_, o)
+        buf = DevBuf(lib, np.zeros(max(ln, 1)))
+        native._check(Lb.mpsa_split_copy(h, which, buf.p, C.c_void_p(0)))
+        got[name] = tb.unpack(buf.host()[:tb.total])
+    k = int(nlam[0])
+    lq, ls, vals = np.zeros(k, np.uint64), np.zeros(k, np.int64), np.zeros(max(1, int(chi[0])))
+    native._check(Lb.mpsa_split_lambda(h, native._ptr(lq), native._ptr(ls), native._ptr(vals)))
+    Lb.mpsa_split_destroy(h)
+    left, lam, right, twr, chir = pysplit.split(psi, itk[c], itk[c + 1], chimax=10 ** 6, direction=1)
+    assert int(chi[0]) == chir
+    if dense:  # one (2D) x (2D) sector of rank D: the preconditioned path ran
+        assert 2 * int(t.rows.max()) >= 64 and 2 * int(t.cols.max()) >= 64
+    o = 0
+    lmax = max(float(np.max(v)) for v in lam.values())
+    for i in range(k):
+        want = np.asarray(lam[int(lq[i])])
+        d = np.abs(vals[o:o + int(ls[i])] - want)
+        assert np.all(d <= np.maximum(1e-12 * want, 100 * np.finfo(float).eps * lmax))
+        o += int(ls[i])
+    assert got["right"].keys() == right.keys()
+    for key in right.keys():  # lambda rides on the right tensor
+        assert np.max(np.abs(got["right"][key] - right[key]), initial=0.0) <= 1e-10, key
+    for qc in lam:  # U^T U = 1 per sector
+        X = np.vstack([got["left"][kk] for kk in sorted(got["left"].keys()) if kk[1] == qc])
+        assert np.max(np.abs(X.T @ X - np.eye(X.shape[1]))) <= 1e-12

@@ -75,7 +75,10 @@ def _check_isometry(tb, lam, side, tol):
         assert np.max(np.abs(G - np.eye(G.shape[0]))) <= tol, qc
 
 
-CASES = [("xxz", 10, 5, 12, 501, False), ("tv", 14, 7, 30, 502, False), ("xxz", 12, 0, 48, 503, True)]
+# the dense bonds are one (2D) x (2D) sector: 96 x 96 and 256 x 256 take the
+# Gram-preconditioned Jacobi path (sectors with m >= n >= 64)
+CASES = [("xxz", 10, 5, 12, 501, False), ("tv", 14, 7, 30, 502, False), ("xxz", 12, 0, 48, 503, True),
+         ("xxz", 16, 0, 128, 505, True)]
 
 
 @pytest.mark.parametrize("model,L,N,D,seed,dense", CASES)
