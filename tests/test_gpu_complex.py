@@ -99,3 +99,31 @@ def test_complex_gauge_covariance_and_lanczos(gpu, oracle, model, L, N, D, seed)
     sw = cc.gauge_left(sr, G)
     ov = sum(complex(np.vdot(np.asarray(sw[k]), np.asarray(sz[k]))) for k in sw.keys())
     assert abs(abs(ov) - 1.0) <= 1e-6
+
+
+def test_complex_batched_lanczos_matches_single_runs(gpu, oracle):
+    """Several complex start vectors through mpsa_lanczos_batched (one batched
+    complex matvec per step, graph replay) against one mpsa_lanczos run per
+    vector: the same iteration counts and Ritz values (1e-10)."""
+    _say("complex batched lanczos")
+    P = problems.build(oracle, "tv", 20, 10, 60, 116)
+    c, mps, mpo, itk = P["c"], P["mps"], P["mpo"], P["itk"]
+    # a Hermitian complex H_eff: the left environment of the gauged MPS
+    G = cc.bond_unitaries(mps[c - 1], 116)
+    Lz = oracle.add_layer(cc.with_imag(P["Ls"][c - 2], 0, 0.0), cc.gauge_right(mps[c - 1], G), mpo[c - 1],
+                          itk[c - 1], 1, cplx=True)
+    Rz = cc.with_imag(P["Rs"][c + 2], 0, 0.0)
+    Lt, Rt = native.Table.from_blocks(Lz), native.Table.from_blocks(Rz)
+    xs = [cc.with_imag(cc.gauge_left(P["psi"], G), 200 + v) for v in range(4)]
+    xt = native.Table.from_blocks(xs[0])
+    vt = xt.union(native.heff2_outkeys(Lt, Rt, xt, mpo[c], mpo[c + 1], itk[c], itk[c + 1]))
+    plan = native.HeffPlan(Lt, Rt, vt, vt, (mpo[c], mpo[c + 1]), (itk[c], itk[c + 1]), native.F64)
+    dL, dR = torch.from_numpy(Lt.pack(Lz)).cuda(), torch.from_numpy(Rt.pack(Rz)).cuda()
+    plan.bind_env(dL, dR)
+    n = vt.length
+    x0 = torch.cat([torch.from_numpy(vt.pack(x)) for x in xs]).cuda()
+    b = native.lanczos_batched(plan, x0, 4, n, 40, 1e-10, want_state=False)
+    for v in range(4):
+        s = native.lanczos(plan, x0[v * n:(v + 1) * n].clone(), 40, 1e-10, True, want_state=False)
+        assert b["it"][v] == s["it"]
+        assert np.max(np.abs(b["energies"][v] - s["energies"])) <= 1e-10 * max(1.0, np.max(np.abs(s["energies"])))
