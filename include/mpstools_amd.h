@@ -236,6 +236,19 @@ int mpsa_lanczos(mpsa_plan* plan, const double* dpsi0, double* dstate, int itmax
 int mpsa_lanczos_batched(mpsa_plan* plan, const double* dpsi0, int64_t stride, int nvec, double* dstates, int itmax,
                          double delta, int target, double* energies, int cap, int* nit, int* nritz, int use_graph,
                          mpsa_stream stream);
+/* ---- Chebyshev moments on one bond (the recurrence of
+ * ChebychevEngine::simulateSSCompress, src/container.hpp:2343-2470, with the
+ * plan's exact H_eff in place of the compressed MPO application) ----
+ * t_0 = dt0, t_1 = Ht t_0, t_{k+1} = 2 Ht t_k - t_{k-1}, Ht = (H_eff - shift)
+ * / scale (|scale| above the spectral half-width keeps |T_k| <= 1).
+ * moments[k] = <t_0|t_k> for k < nmom (the real part: the real dot product of
+ * the vectors, planar for complex plans).  recons (may be NULL, 2*nmom - 1
+ * doubles): the reference's reconstructed moments (:2390-2394, 2433-2436),
+ * recons[2k] = 2<t_k|t_k> - mu_0 and recons[2k-1] = 2<t_k|t_{k-1}> - mu_1,
+ * which equal mu_2k and mu_{2k-1}.  The plan's x and y tables must coincide;
+ * allocates four vectors from the Lanczos pool; synchronizes per moment. */
+int mpsa_chebyshev_moments(mpsa_plan* plan, const double* dt0, int nmom, double scale, double shift,
+                           double* moments, double* recons, mpsa_stream stream);
 /* Host FP64 symmetric tridiagonal eigensolver used by mpsa_lanczos (replaces
  * dstev, blas/lapack_routines.cpp:435-446): d (n) diagonal -> ascending
  * eigenvalues, e (n-1) off-diagonal, z (n*n, column-major, may be NULL). */

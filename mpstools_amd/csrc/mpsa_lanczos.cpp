@@ -450,4 +450,66 @@ int mpsa_release_pools(void) {
   return MPSA_OK;
 }
 
+// Chebyshev recurrence of ChebychevEngine::simulateSSCompress
+// (src/container.hpp:2343-2470) on one bond, with the exact H_eff of the plan
+// in place of the compressed MPO application: t_1 = Ht t_0,
+// t_{k+1} = 2 Ht t_k - t_{k-1} (:2423-2428), Ht = (H_eff - shift) / scale;
+// moments mu_k = <t_0|t_k> (:2368, 2388, 2431), and with recons the
+// reconstruction of the doubled moments (:2390-2394, 2433-2436):
+// 2<t_k|t_k> - mu_0 = mu_2k and 2<t_k|t_{k-1}> - mu_1 = mu_{2k-1}.
+int mpsa_chebyshev_moments(mpsa_plan* plan, const double* dt0, int nmom, double scale, double shift,
+                           double* moments, double* recons, mpsa_stream stream) {
+  try {
+    if (!plan || !dt0 || nmom < 1 || !moments || !(std::fabs(scale) > 0.0) || !std::isfinite(scale) ||
+        !std::isfinite(shift))
+      throw LErr(MPSA_EINVAL, "chebyshev: bad arguments");
+    int64_t xl = 0, yl = 0;
+    int same = 0;
+    chk(mpsa_plan_vector_lengths(plan, &xl, &yl, &same));
+    if (!same) throw LErr(MPSA_EINVAL, "chebyshev: the plan's x and y tables must be the same vector space");
+    const int64_t n = xl;
+    hipStream_t st = (hipStream_t)stream;
+    Vecs V{{}, n, st};
+    double* t0 = V.get();
+    double* a = V.get();  // t_{k-2}
+    double* b = V.get();  // t_{k-1}
+    double* y = V.get();  // t_k
+    hchk(hipMemcpyAsync(t0, dt0, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, st), "copy t0");
+    hchk(hipMemcpyAsync(a, dt0, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, st), "copy t0");
+    const double mu0 = ddot(t0, t0, n, st);
+    moments[0] = mu0;
+    if (recons) recons[0] = mu0;
+    if (nmom == 1) return MPSA_OK;
+    chk(mpsa_heff_apply(plan, a, n, b, n, 1, stream));
+    axpby(-shift / scale, a, 1.0 / scale, b, n, st);  // t_1 = (H t_0 - shift t_0) / scale
+    const double mu1 = ddot(t0, b, n, st);
+    moments[1] = mu1;
+    if (recons) {
+      recons[1] = 2.0 * ddot(b, a, n, st) - mu1;
+      recons[2] = 2.0 * ddot(b, b, n, st) - mu0;
+    }
+    for (int k = 2; k < nmom; ++k) {
+      chk(mpsa_heff_apply(plan, b, n, y, n, 1, stream));
+      axpby(-2.0 * shift / scale, b, 2.0 / scale, y, n, st);  // 2 Ht t_{k-1}
+      axpby(-1.0, a, 1.0, y, n, st);                          // - t_{k-2}
+      moments[k] = ddot(t0, y, n, st);
+      if (recons) {
+        recons[2 * k - 1] = 2.0 * ddot(y, b, n, st) - mu1;
+        recons[2 * k] = 2.0 * ddot(y, y, n, st) - mu0;
+      }
+      double* r = a;
+      a = b;
+      b = y;
+      y = r;
+    }
+    return MPSA_OK;
+  } catch (const LErr& e) {
+    set_last_error(e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    set_last_error(e.what());
+    return MPSA_EINVAL;
+  }
+}
+
 }  // extern "C"

@@ -77,6 +77,7 @@ def lib():
         L.mpsa_lanczos.argtypes = [P, P, P, C.c_int, C.c_double, C.c_int, C.c_int, P, C.c_int, P, P, P]
         L.mpsa_lanczos_batched.argtypes = [P, P, C.c_int64, C.c_int, P, C.c_int, C.c_double, C.c_int, P, C.c_int,
                                             P, P, C.c_int, P]
+        L.mpsa_chebyshev_moments.argtypes = [P, P, C.c_int, C.c_double, C.c_double, P, P, P]
         L.mpsa_chain_create.argtypes = [P, C.c_int, P, P, P, P, C.c_int, P, C.c_int]
         L.mpsa_energy_forward.argtypes = [P, P, P, P]
         L.mpsa_energy_backward.argtypes = [P, P, P, P, P]
@@ -400,6 +401,18 @@ def lanczos(plan, dpsi0, itmax, delta, testenergy=True, target=0, want_state=Tru
     _check(lib().mpsa_lanczos(plan.h, _dptr(dpsi0), _dptr(state), int(itmax), float(delta), int(testenergy),
                               int(target), _ptr(E), cap, _ptr(nit), _ptr(nr), _stream()))
     return dict(it=int(nit[0]), energies=E[:int(nr[0])].copy(), state=state)
+
+
+def chebyshev_moments(plan, dt0, nmom, scale, shift=0.0, recons=True):
+    """Chebyshev moments mu_k = <t_0|T_k((H_eff - shift)/scale) t_0> on one bond
+    (the recurrence of ChebychevEngine::simulateSSCompress,
+    src/container.hpp:2343-2470, with the exact H_eff).  Returns (moments,
+    reconstructed doubled moments or None)."""
+    mu = np.zeros(nmom)
+    rc = np.zeros(2 * nmom - 1) if recons else None
+    _check(lib().mpsa_chebyshev_moments(plan.h, _dptr(dt0), int(nmom), float(scale), float(shift), _ptr(mu),
+                                        _ptr(rc) if recons else None, _stream()))
+    return mu, rc
 
 
 def lanczos_batched(plan, dpsi0, nvec, stride, itmax, delta, target=0, want_state=True, graph=True):

@@ -1,6 +1,8 @@
 """Seeded problem builders shared by the tests (inputs only; no expectations)."""
 from __future__ import annotations
 
+import numpy as np
+
 from mpstools_amd import models, synth
 from mpstools_amd.models import LocalOp, MPOMat
 
@@ -67,3 +69,18 @@ def build(oracle, model, L, N, D, seed, c=None, dense=False, **kw):
     Ls = oracle.left_envs(mps, mpo, itk, c - 1)
     Rs = oracle.right_envs(mps, mpo, itk, c + 1)
     return dict(mpo=mpo, itk=itk, mps=mps, psi=psi, c=c, dims=dims, Ls=Ls, Rs=Rs)
+
+
+def dense_heff(oracle, P, vt):
+    """The two-site H_eff of the problem's centre bond as a dense matrix over
+    the layout of table vt, one oracle contraction per unit vector (alignment
+    gaps give zero rows and columns)."""
+    c, mpo, itk = P["c"], P["mpo"], P["itk"]
+    H = np.zeros((vt.total, vt.total))
+    for j in range(vt.total):
+        e = np.zeros(vt.total)
+        e[j] = 1.0
+        y = oracle.contract_twosite(P["Ls"][c - 1], P["Rs"][c + 2], vt.unpack(e), mpo[c], mpo[c + 1], itk[c],
+                                    itk[c + 1])
+        H[:, j] = vt.pack(y)
+    return H

@@ -345,3 +345,31 @@ def test_emulated_split_matches_oracle(lib, oracle, model, L, N, D, seed, dense)
     for qc in lam:  # U^T U = 1 per sector
         X = np.vstack([got["left"][kk] for kk in sorted(got["left"].keys()) if kk[1] == qc])
         assert np.max(np.abs(X.T @ X - np.eye(X.shape[1]))) <= 1e-12
+
+
+def test_emulated_chebyshev_moments_match_spectral_sum(lib, oracle):
+    """mpsa_chebyshev_moments against the closed form
+    mu_k = sum_n |<n|t_0>|^2 T_k((E_n - shift) / scale) from the dense H_eff's
+    eigendecomposition, and the reconstructed doubled moments against the same
+    closed form at 2k and 2k-1."""
+    P = problems.build(oracle, "tv", 10, 5, 8, 61)
+    c, mpo, itk = P["c"], P["mpo"], P["itk"]
+    Lb, Rb, psi = P["Ls"][c - 1], P["Rs"][c + 2], P["psi"]
+    Lt, Rt, xt = native.Table.from_blocks(Lb), native.Table.from_blocks(Rb), native.Table.from_blocks(psi)
+    vt = xt.union(native.heff2_outkeys(Lt, Rt, xt, mpo[c], mpo[c + 1], itk[c], itk[c + 1]))
+    H = problems.dense_heff(oracle, P, vt)
+    assert np.max(np.abs(H - H.T)) <= 1e-12 * np.max(np.abs(H))
+    E, U = np.linalg.eigh(H)
+    shift, scale = 0.5 * (E[-1] + E[0]), 0.55 * (E[-1] - E[0])
+    x0 = vt.pack(psi)
+    w = (U.T @ x0) ** 2
+    th = np.arccos(np.clip((E - shift) / scale, -1, 1))
+    nmom = 40
+    want = np.array([np.sum(w * np.cos(k * th)) for k in range(2 * nmom - 1)])
+    plan = native.HeffPlan(Lt, Rt, vt, vt, (mpo[c], mpo[c + 1]), (itk[c], itk[c + 1]), native.F64)
+    plan.bind_env(DevBuf(lib, Lt.pack(Lb)), DevBuf(lib, Rt.pack(Rb)))
+    mu, rc = native.chebyshev_moments(plan, DevBuf(lib, x0), nmom, scale, shift)
+    assert np.max(np.abs(mu - want[:nmom])) <= 1e-12 * want[0]
+    assert np.max(np.abs(rc - want)) <= 1e-11 * want[0]
+    with pytest.raises(native.MpsaError):
+        native.chebyshev_moments(plan, DevBuf(lib, x0), 4, 0.0, shift)
