@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <complex>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
@@ -151,6 +152,9 @@ class ukey : public TensorKeyType<Nb, 1, uint> {  // keyTypes.hpp:424-483
 };
 
 // ------------------------------------------------------------------ containers
+inline double conj_value(double x) { return x; }
+inline std::complex<double> conj_value(const std::complex<double>& x) { return std::conj(x); }
+
 template <typename T>
 class matrix {  // column-major, the subset of ublas::matrix<T,column_major> the path uses
  public:
@@ -195,7 +199,7 @@ class BlockMatrix : public std::map<TensorKeyType<2, Ns, KeyType>, matrix<T>> { 
       if (n != it->second.size1() * it->second.size2())
         mps_exception("in BlockMatrix::operator *(BlockMatrix<Ns,KeyType,T>): size1==size2 failed");
       T s = T(0);
-      for (size_t i = 0; i < n; ++i) s += it->second.data()[i] * kv.second.data()[i];
+      for (size_t i = 0; i < n; ++i) s += conj_value(it->second.data()[i]) * kv.second.data()[i];
       val = val + s;
     }
     return val;
@@ -226,16 +230,29 @@ class BlockMatrix : public std::map<TensorKeyType<2, Ns, KeyType>, matrix<T>> { 
 // ------------------------------------------------------------------ ABI glue
 namespace detail {
 
+// Complex128 blocks travel in the ABI's planar layout: the real parts at the
+// table's offsets, then the imaginary parts one extent further on.
+template <typename T>
+struct planes_of {
+  static constexpr int value = 1;
+};
+template <>
+struct planes_of<std::complex<double>> {
+  static constexpr int value = 2;
+};
+
 struct Tab {
   std::vector<uint64_t> q;
   std::vector<uint32_t> p;
   std::vector<int64_t> rows, cols, off;
-  int64_t total = 0;
+  int64_t total = 0;  // extent of one plane
+  int planes = 1;
   mpsa_blocks b{};
+  int64_t len() const { return total * planes; }  // doubles of a packed buffer
   void finish(int ns) {
     b.n = (int64_t)rows.size();
     b.ns = ns;
-    b.is_complex = 0;
+    b.is_complex = planes == 2 ? 1 : 0;
     b.q = q.data();
     b.p = p.data();
     b.rows = rows.data();
@@ -259,13 +276,15 @@ Tab table_of(const BlockMatrix<Ns, K, T>& B) {
     o += (int64_t)(kv.second.size1() * kv.second.size2());
   }
   t.total = o;
+  t.planes = planes_of<T>::value;
   t.finish(Ns);
   return t;
 }
 
 inline Tab table_of_keys(int ns, int64_t n, const std::vector<uint64_t>& q, const std::vector<uint32_t>& p,
-                         const std::vector<int64_t>& r, const std::vector<int64_t>& c) {
+                         const std::vector<int64_t>& r, const std::vector<int64_t>& c, int planes = 1) {
   Tab t;
+  t.planes = planes;
   t.q = q;
   t.p = p;
   t.rows = r;
@@ -279,6 +298,12 @@ inline Tab table_of_keys(int ns, int64_t n, const std::vector<uint64_t>& q, cons
   t.total = o;
   t.finish(ns);
   return t;
+}
+
+inline double real_value(double x) { return x; }
+inline double real_value(const std::complex<double>& x) {
+  if (x.imag() != 0.0) mps_exception("mpstools_amd: complex MPO values are not supported (the MPO must be real)");
+  return x.real();
 }
 
 struct Mpo {
@@ -295,7 +320,7 @@ Mpo mpo_of(const MPOMat<1, T>& W) {
       r.wr.push_back(o.first[1]);
       r.out.push_back(e.first.out());
       r.in.push_back(e.first.in());
-      r.val.push_back((double)e.second);
+      r.val.push_back(real_value(e.second));
     }
   r.m.n = (int64_t)r.val.size();
   r.m.Dl = W.Dl;
@@ -335,22 +360,38 @@ struct DevBuf {
   DevBuf& operator=(const DevBuf&) = delete;
 };
 
+inline void put_block(double* h, int64_t, int64_t off, const double* src, size_t n) {
+  std::memcpy(h + off, src, sizeof(double) * n);
+}
+inline void put_block(double* h, int64_t total, int64_t off, const std::complex<double>* src, size_t n) {
+  for (size_t i = 0; i < n; ++i) {
+    h[off + (int64_t)i] = src[i].real();
+    h[total + off + (int64_t)i] = src[i].imag();
+  }
+}
+inline void get_block(const double* h, int64_t, int64_t off, double* dst, size_t n) {
+  std::memcpy(dst, h + off, sizeof(double) * n);
+}
+inline void get_block(const double* h, int64_t total, int64_t off, std::complex<double>* dst, size_t n) {
+  for (size_t i = 0; i < n; ++i) dst[i] = std::complex<double>(h[off + (int64_t)i], h[total + off + (int64_t)i]);
+}
+
 template <int Ns, class K, typename T>
 void upload(const BlockMatrix<Ns, K, T>& B, const Tab& t, DevBuf& d) {
-  std::vector<double> h((size_t)std::max<int64_t>(t.total, 1), 0.0);
+  std::vector<double> h((size_t)std::max<int64_t>(t.len(), 1), 0.0);
   size_t i = 0;
   for (auto& kv : B) {
-    std::memcpy(h.data() + t.off[i], kv.second.data(), sizeof(double) * kv.second.size1() * kv.second.size2());
+    put_block(h.data(), t.total, t.off[i], kv.second.data(), kv.second.size1() * kv.second.size2());
     ++i;
   }
-  if (hipMemcpy(d.p, h.data(), sizeof(double) * (size_t)t.total, hipMemcpyHostToDevice) != hipSuccess)
+  if (hipMemcpy(d.p, h.data(), sizeof(double) * (size_t)t.len(), hipMemcpyHostToDevice) != hipSuccess)
     mps_exception("mpstools_amd: upload failed");
 }
 
 template <int Ns, class K, typename T>
 void download(BlockMatrix<Ns, K, T>& B, const Tab& t, const DevBuf& d) {
-  std::vector<double> h((size_t)std::max<int64_t>(t.total, 1));
-  if (hipMemcpy(h.data(), d.p, sizeof(double) * (size_t)t.total, hipMemcpyDeviceToHost) != hipSuccess)
+  std::vector<double> h((size_t)std::max<int64_t>(t.len(), 1));
+  if (hipMemcpy(h.data(), d.p, sizeof(double) * (size_t)t.len(), hipMemcpyDeviceToHost) != hipSuccess)
     mps_exception("mpstools_amd: download failed");
   B.clear();
   for (size_t i = 0; i < t.rows.size(); ++i) {
@@ -358,7 +399,7 @@ void download(BlockMatrix<Ns, K, T>& B, const Tab& t, const DevBuf& d) {
     for (int s = 0; s < Ns; ++s) p[(size_t)s] = t.p[i * Ns + (size_t)s];
     TensorKeyType<2, Ns, K> key(K(t.q[2 * i], 1), K(t.q[2 * i + 1], 1), p);
     matrix<T> m((size_t)t.rows[i], (size_t)t.cols[i]);
-    std::memcpy(m.data(), h.data() + t.off[i], sizeof(double) * m.size1() * m.size2());
+    get_block(h.data(), t.total, t.off[i], m.data(), m.size1() * m.size2());
     B[key] = m;
   }
 }
@@ -395,12 +436,12 @@ void contractTwosite(BlockMatrix<2, KeyType, T>& Aout, const BlockMatrix<1, KeyT
   p.resize((size_t)(2 * n));
   r.resize((size_t)n);
   c.resize((size_t)n);
-  Tab ty = table_of_keys(2, n, q, p, r, c);
+  Tab ty = table_of_keys(2, n, q, p, r, c, planes_of<T>::value);
   mpsa_plan* plan = nullptr;
   mpsa_check(mpsa_heff2_plan(&plan, &tl.b, &tr.b, &tx.b, &ty.b, &ml.m, &mr.m, il.data(), (int)il.size(), ir.data(),
                              (int)ir.size(), MPSA_F64),
              "contractTwosite");
-  DevBuf dL((size_t)tl.total), dR((size_t)tr.total), dx((size_t)tx.total), dy((size_t)ty.total);
+  DevBuf dL((size_t)tl.len()), dR((size_t)tr.len()), dx((size_t)tx.len()), dy((size_t)ty.len());
   upload(BL, tl, dL);
   upload(BR, tr, dR);
   upload(A, tx, dx);
@@ -435,11 +476,11 @@ void contractSingle(BlockMatrix<1, KeyType, T>& Aout, const BlockMatrix<1, KeyTy
   p.resize((size_t)n);
   r.resize((size_t)n);
   c.resize((size_t)n);
-  Tab ty = table_of_keys(1, n, q, p, r, c);
+  Tab ty = table_of_keys(1, n, q, p, r, c, planes_of<T>::value);
   mpsa_plan* plan = nullptr;
   mpsa_check(mpsa_heff1_plan(&plan, &tl.b, &tr.b, &tx.b, &ty.b, &m.m, it.data(), (int)it.size(), MPSA_F64),
              "contractSingle");
-  DevBuf dL((size_t)tl.total), dR((size_t)tr.total), dx((size_t)tx.total), dy((size_t)ty.total);
+  DevBuf dL((size_t)tl.len()), dR((size_t)tr.len()), dx((size_t)tx.len()), dy((size_t)ty.len());
   upload(BL, tl, dL);
   upload(BR, tr, dR);
   upload(A, tx, dx);
@@ -476,13 +517,13 @@ BlockMatrix<1, KeyType, T> env_layer(const BlockMatrix<1, KeyType, T>& BL, const
   p.resize((size_t)n);
   r.resize((size_t)n);
   c.resize((size_t)n);
-  Tab to = table_of_keys(1, n, q, p, r, c);
+  Tab to = table_of_keys(1, n, q, p, r, c, planes_of<T>::value);
   mpsa_plan* plan = nullptr;
   mpsa_check(mpsa_env_plan(&plan, &tb.b, &ta.b, Ap ? &tap.b : nullptr, &m.m, it.data(), (int)it.size(), direction,
                            &to.b, MPSA_F64),
              "addContractionLayer");
-  DevBuf dB((size_t)tb.total), dA((size_t)ta.total), dO((size_t)to.total);
-  DevBuf dAp(Ap ? (size_t)tap.total : 1);
+  DevBuf dB((size_t)tb.len()), dA((size_t)ta.len()), dO((size_t)to.len());
+  DevBuf dAp(Ap ? (size_t)tap.len() : 1);
   upload(BL, tb, dB);
   upload(A, ta, dA);
   if (Ap) upload(*Ap, tap, dAp);
@@ -579,12 +620,12 @@ class TSLanzcosEngine {
       vr.push_back(kv.second.first);
       vc.push_back(kv.second.second);
     }
-    _space = table_of_keys(2, (int64_t)vr.size(), vq, vp, vr, vc);
+    _space = table_of_keys(2, (int64_t)vr.size(), vq, vp, vr, vc, planes_of<T>::value);
     mpsa_plan* plan = nullptr;
     mpsa_check(mpsa_heff2_plan(&plan, &tl.b, &tr.b, &_space.b, &_space.b, &ml.m, &mr.m, il.data(), (int)il.size(),
                                ir.data(), (int)ir.size(), _prec),
                "TSLanzcosEngine");
-    DevBuf dL((size_t)tl.total), dR((size_t)tr.total), dx((size_t)_space.total);
+    DevBuf dL((size_t)tl.len()), dR((size_t)tr.len()), dx((size_t)_space.len());
     upload(_L, tl, dL);
     upload(_R, tr, dR);
     {
@@ -596,25 +637,25 @@ class TSLanzcosEngine {
     mpsa_check(mpsa_plan_bind_env(plan, dL.p, dR.p, nullptr), "TSLanzcosEngine");
     _es.assign((size_t)itmax + 2, 0.0);
     int nit = 0, nr = 0;
-    _gs.reset(new DevBuf((size_t)_space.total));
+    _gs.reset(new DevBuf((size_t)_space.len()));
     mpsa_check(mpsa_lanczos(plan, dx.p, _gs->p, itmax, delta, testenergy ? 1 : 0, 0, _es.data(), itmax + 2, &nit, &nr,
                             nullptr),
                "TSLanzcosEngine::simulate");
     _es.resize((size_t)nr);
     _plan_keep.reset(new PlanHolder{plan});
-    _dx.reset(new DevBuf((size_t)_space.total));
-    _keepL.reset(new DevBuf((size_t)tl.total));
-    _keepR.reset(new DevBuf((size_t)tr.total));
-    if (hipMemcpy(_dx->p, dx.p, sizeof(double) * (size_t)_space.total, hipMemcpyDeviceToDevice) != hipSuccess ||
-        hipMemcpy(_keepL->p, dL.p, sizeof(double) * (size_t)tl.total, hipMemcpyDeviceToDevice) != hipSuccess ||
-        hipMemcpy(_keepR->p, dR.p, sizeof(double) * (size_t)tr.total, hipMemcpyDeviceToDevice) != hipSuccess)
+    _dx.reset(new DevBuf((size_t)_space.len()));
+    _keepL.reset(new DevBuf((size_t)tl.len()));
+    _keepR.reset(new DevBuf((size_t)tr.len()));
+    if (hipMemcpy(_dx->p, dx.p, sizeof(double) * (size_t)_space.len(), hipMemcpyDeviceToDevice) != hipSuccess ||
+        hipMemcpy(_keepL->p, dL.p, sizeof(double) * (size_t)tl.len(), hipMemcpyDeviceToDevice) != hipSuccess ||
+        hipMemcpy(_keepR->p, dR.p, sizeof(double) * (size_t)tr.len(), hipMemcpyDeviceToDevice) != hipSuccess)
       mps_exception("TSLanzcosEngine: device copy failed");
     mpsa_check(mpsa_plan_bind_env(plan, _keepL->p, _keepR->p, nullptr), "TSLanzcosEngine");
     if (_verbose == 1 && !_es.empty()) std::cout << "E=" << _es[0] << " after " << nit << " iterations; ";
     return nit;
   }
 
-  T E(const int i) const { return _es.at((size_t)i); }
+  T E(const int i) const { return T(_es.at((size_t)i)); }  // Ritz values are real (Hermitian H_eff)
 
   // getState(j) (src/lanzcos.hpp:81-92): the Ritz vector j of the last run.
   BlockMatrix<2, KeyType, T> getState(const int j) {
@@ -626,7 +667,7 @@ class TSLanzcosEngine {
       return out;
     }
     // rerun the recurrence for another target: same plan, same start vector
-    DevBuf st((size_t)_space.total);
+    DevBuf st((size_t)_space.len());
     std::vector<double> es(_es.size() + 2);
     int nit = 0, nr = 0;
     mpsa_check(mpsa_lanczos(_plan_keep->p, _dx->p, st.p, (int)_es.size(), 0.0, 1, j, es.data(), (int)es.size(), &nit,
@@ -650,7 +691,7 @@ class TSLanzcosEngine {
   std::vector<i_to_key<KeyType>> _itokey;
   uint _verbose;
   int _prec;
-  std::vector<T> _es;
+  std::vector<double> _es;
   detail::Tab _space;
   std::unique_ptr<detail::DevBuf> _gs, _dx, _keepL, _keepR;
   std::unique_ptr<PlanHolder> _plan_keep;

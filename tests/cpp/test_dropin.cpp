@@ -4,6 +4,8 @@
 // PASS/FAIL line per check; exit status 0 only if all pass.
 #define MPSTOOLS_AMD_THROW
 #include "../../include/mpstools_amd.hpp"
+#include <complex>
+
 #include "../../oracle/oracle.hpp"
 
 #include <cstdio>
@@ -77,6 +79,69 @@ static double rel(const BlockMatrix<Ns, AbKeyType, Real>& a, const oracle::Block
       const double d = it->second.data()[i] - kv.second.a[i];
       num += d * d;
       den += kv.second.a[i] * kv.second.a[i];
+    }
+  }
+  return std::sqrt(num / std::max(den, 1e-300));
+}
+
+// complex128 counterparts: the same blocks with seeded imaginary parts, the
+// oracle's complex template as the reference (its complex path is pinned
+// against real contractions by tests/test_complex.py)
+typedef std::complex<double> Cplx;
+static MPOMat<1, Cplx> cplx_mpo(const MPOMat<1, Real>& W) {
+  MPOMat<1, Cplx> o;
+  o.Dl = W.Dl;
+  o.Dr = W.Dr;
+  for (auto& g : W)
+    for (auto& e : g.second) o[g.first][e.first] = Cplx(e.second, 0.0);
+  return o;
+}
+static oracle::MPOMat<Cplx> to_oracle_c(const MPOMat<1, Real>& W) {
+  oracle::MPOMat<Cplx> o;
+  o.Dl = W.Dl;
+  o.Dr = W.Dr;
+  for (auto& g : W)
+    for (auto& e : g.second) o.m[oracle::UKey{g.first[0], g.first[1]}][oracle::OpKey{e.first.in(), e.first.out()}] = e.second;
+  return o;
+}
+template <int Ns>
+static BlockMatrix<Ns, AbKeyType, Cplx> with_imag(const BlockMatrix<Ns, AbKeyType, Real>& B, std::mt19937_64& rng) {
+  std::normal_distribution<double> nd(0.0, 0.5);
+  BlockMatrix<Ns, AbKeyType, Cplx> o;
+  for (auto& kv : B) {
+    matrix<Cplx> m(kv.second.size1(), kv.second.size2());
+    for (size_t i = 0; i < m.size1() * m.size2(); ++i)
+      m.data()[i] = Cplx(kv.second.data()[i], nd(rng) * std::fabs(kv.second.data()[i]));
+    o[kv.first] = m;
+  }
+  return o;
+}
+template <int Ns>
+static oracle::BlockMatrix<Ns, Cplx> to_oracle_c(const BlockMatrix<Ns, AbKeyType, Cplx>& B) {
+  oracle::BlockMatrix<Ns, Cplx> o;
+  for (auto& kv : B) {
+    uint32_t p[Ns];
+    for (int s = 0; s < Ns; ++s) p[s] = kv.first(s);
+    oracle::Mat<Cplx> m((int64_t)kv.second.size1(), (int64_t)kv.second.size2());
+    for (size_t i = 0; i < m.a.size(); ++i) m.a[i] = kv.second.data()[i];
+    o[oracle::mk<Ns>(kv.first[0].getQ(), kv.first[1].getQ(), p)] = m;
+  }
+  return o;
+}
+template <int Ns>
+static double rel_c(const BlockMatrix<Ns, AbKeyType, Cplx>& a, const oracle::BlockMatrix<Ns, Cplx>& b) {
+  if (a.size() != b.size()) return 1e300;
+  double num = 0, den = 0;
+  for (auto& kv : b) {
+    TensorKeyType<2, Ns, AbKeyType> k;
+    k[0] = AbKeyType(kv.first.q[0], 1);
+    k[1] = AbKeyType(kv.first.q[1], 1);
+    for (int s = 0; s < Ns; ++s) k(s) = kv.first.p[s];
+    auto it = a.find(k);
+    if (it == a.end()) return 1e300;
+    for (size_t i = 0; i < kv.second.a.size(); ++i) {
+      num += std::norm(it->second.data()[i] - kv.second.a[i]);
+      den += std::norm(kv.second.a[i]);
     }
   }
   return std::sqrt(num / std::max(den, 1e-300));
@@ -192,6 +257,53 @@ int main() {
       for (size_t i = 0; i < kv.second.a.size(); ++i) ov += f->second.data()[i] * kv.second.a[i];
     }
     report("TSLanzcosEngine state overlap", std::fabs(std::fabs(ov) - 1.0), 1e-6);
+  }
+  // complex128 blocks (planar on the device) against the oracle's complex template
+  {
+    std::mt19937_64 crng(20260002);
+    auto Lc = with_imag<1>(L, crng), Rc = with_imag<1>(R, crng);
+    auto psic = with_imag<2>(psi, crng);
+    auto oLc = to_oracle_c<1>(Lc), oRc = to_oracle_c<1>(Rc);
+    std::vector<i_to_key<AbKeyType>> itks2 = {itk, itk};
+    BlockMatrix<2, AbKeyType, Cplx> yc;
+    contractTwosite(yc, Lc, Rc, psic, cplx_mpo(heis(c, N)), cplx_mpo(heis(c + 1, N)), itks2);
+    oracle::BlockMatrix<2, Cplx> oyc;
+    oracle::contractTwosite(oyc, oLc, oRc, to_oracle_c<2>(psic), to_oracle_c(heis(c, N)), to_oracle_c(heis(c + 1, N)),
+                            oitk, oitk, 1);
+    report("contractTwosite complex128", rel_c<2>(yc, oyc), 1e-12);
+    auto Ac = with_imag<1>(mps[(size_t)c], crng), Apc = with_imag<1>(mps[(size_t)c], crng);
+    auto Lpc = addContractionLayer(Lc, Ac, cplx_mpo(heis(c, N)), itk, 1);
+    auto oLpc = oracle::addContractionLayer(oLc, to_oracle_c<1>(Ac), to_oracle_c(heis(c, N)), oitk, 1, 1);
+    report("addContractionLayer complex128", rel_c<1>(Lpc, oLpc), 1e-12);
+    auto Lmc = addContractionLayer(Lc, Ac, cplx_mpo(heis(c, N)), Apc, itk, 1);
+    const auto oApc = to_oracle_c<1>(Apc);
+    auto oLmc = oracle::addContractionLayer(oLc, to_oracle_c<1>(Ac), to_oracle_c(heis(c, N)), oitk, 1, 1, &oApc);
+    report("addContractionLayer Ap complex128", rel_c<1>(Lmc, oLmc), 1e-12);
+    auto R1c = with_imag<1>(addContractionLayer(R, mps[(size_t)c + 1], heis(c + 1, N), itk, -1), crng);
+    BlockMatrix<1, AbKeyType, Cplx> y1c;
+    contractSingle(y1c, Lc, Ac, cplx_mpo(heis(c, N)), R1c, itk);
+    oracle::BlockMatrix<1, Cplx> oy1c;
+    oracle::contractSingle(oy1c, oLc, to_oracle_c<1>(Ac), to_oracle_c(heis(c, N)), to_oracle_c<1>(R1c), oitk, 1);
+    report("contractSingle complex128", rel_c<1>(y1c, oy1c), 1e-12);
+    // Lanczos from a complex start under the real (Hermitian) H_eff: the real ground state energy
+    BlockMatrix<1, AbKeyType, Cplx> Lr, Rr;
+    for (auto& kv : L) { matrix<Cplx> m(kv.second.size1(), kv.second.size2()); for (size_t i = 0; i < m.size1() * m.size2(); ++i) m.data()[i] = kv.second.data()[i]; Lr[kv.first] = m; }
+    for (auto& kv : R) { matrix<Cplx> m(kv.second.size1(), kv.second.size2()); for (size_t i = 0; i < m.size1() * m.size2(); ++i) m.data()[i] = kv.second.data()[i]; Rr[kv.first] = m; }
+    TSLanzcosEngine<AbKeyType, Cplx> lanc(Lr, Rr, with_imag<1>(mps[(size_t)c], crng), with_imag<1>(mps[(size_t)c + 1], crng),
+                                          cplx_mpo(heis(c, N)), cplx_mpo(heis(c + 1, N)), itk, itk, 0);
+    lanc.simulate(300, 1e-11, true);
+    TSLanzcosEngine<AbKeyType, Real> lanr(L, R, mps[(size_t)c], mps[(size_t)c + 1], heis(c, N), heis(c + 1, N), itk, itk, 0);
+    lanr.simulate(300, 1e-11, true);
+    report("TSLanzcosEngine complex128 E0", std::fabs(lanc.E(0).real() - lanr.E(0)), 1e-9);
+    auto gsc = lanc.getState(0);
+    auto gsr = lanr.getState(0);
+    Cplx ovc(0.0, 0.0);
+    for (auto& kv : gsr) {
+      auto f = gsc.find(kv.first);
+      if (f == gsc.end()) continue;
+      for (size_t i = 0; i < kv.second.size1() * kv.second.size2(); ++i) ovc += kv.second.data()[i] * f->second.data()[i];
+    }
+    report("TSLanzcosEngine complex128 state overlap", std::fabs(std::abs(ovc) - 1.0), 1e-6);
   }
   // error path: empty A
   try {

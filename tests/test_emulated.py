@@ -373,3 +373,17 @@ def test_emulated_chebyshev_moments_match_spectral_sum(lib, oracle):
     assert np.max(np.abs(rc - want)) <= 1e-11 * want[0]
     with pytest.raises(native.MpsaError):
         native.chebyshev_moments(plan, DevBuf(lib, x0), 4, 0.0, shift)
+
+
+def test_emulated_cpp_dropin(lib):
+    """The C++ drop-in front end (include/mpstools_amd.hpp: contractTwosite,
+    contractSingle, addContractionLayer with and without Ap, TSLanzcosEngine,
+    real and complex128) against the oracle's templates, over the emulated
+    device layer (tests/cpp/test_dropin.cpp; the same program runs on the GPU
+    in tests/test_gpu_dropin.py)."""
+    exe = os.path.join(HERE, "emu", "_build", "test_dropin_emu")
+    assert os.path.exists(exe)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "ALL PASS" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
+    assert "complex128" in r.stdout
+    assert "EMU VIOLATION" not in r.stderr, r.stderr[-2000:]
