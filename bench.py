@@ -27,8 +27,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"f64": 78.6, "f64plain": 78.6, "f32": 157.3, "bf16": 2500.0, "bf16x3": 2500.0,
-        "c128": 78.6}  # MI355X dense TFLOP/s (complex128 runs on the FP64 MFMA path)
-MFMA_FLOPS_PER_ALG = {"bf16x3": 3.0}  # MFMA work per algorithmic flop
+        "c128": 78.6, "c128_f32": 157.3, "c128_bf16x3": 2500.0}  # MI355X dense TFLOP/s (complex: its GEMM path)
+MFMA_FLOPS_PER_ALG = {"bf16x3": 3.0, "c128_bf16x3": 3.0}  # MFMA work per algorithmic flop
 
 
 def log(*a):

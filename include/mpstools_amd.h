@@ -71,9 +71,10 @@ typedef struct mpsa_blocks {
                              parts of the blocks at their offsets, then the
                              imaginary parts at offset + extent, extent =
                              max(offset + rows*cols); the buffer holds
-                             2*extent doubles.  H_eff and environment plans
-                             (FP64 only) take complex tables with a real MPO:
-                             every table of a plan complex, or none. */
+                             2*extent doubles.  H_eff plans (any precision)
+                             and environment plans (FP64) take complex tables
+                             with a real MPO: every table of a plan complex,
+                             or none. */
   const uint64_t* q;      /* [2n] charges: left/upper, right/lower */
   const uint32_t* p;      /* [ns*n] physical indices or MPO aux index */
   const int64_t* rows;    /* [n] */

@@ -47,11 +47,12 @@ def _isometry(torch, G):
 
 def _add_plan(torch, out, pr, W, itk, seed):
     """out["plan_" + pr]: the H_eff plan of the bond at precision pr.  "c128"
-    is the complex128 plan (FP64, planar re/im) over the same keys, bound to
+    is the complex128 plan (FP64, planar re/im; "c128_<prec>" the same at a
+    reduced GEMM precision) over the same keys, bound to
     complex environments made of the real ones plus seeded imaginary parts of
     the same magnitude, with a complex start vector in out["psi_c128"]."""
     Lt, Rt, vt, dL, dR = out["Lt"], out["Rt"], out["vt"], out["dL"], out["dR"]
-    if pr != "c128":
+    if not pr.startswith("c128"):
         plan = native.HeffPlan(Lt, Rt, vt, vt, W, itk, native.PREC[pr])
         plan.bind_env(dL, dR)
         out["plan_" + pr] = plan
@@ -63,13 +64,16 @@ def _add_plan(torch, out, pr, W, itk, seed):
         im = torch.randn(v.numel(), dtype=torch.float64, device=v.device, generator=g) * v.abs().mean()
         return torch.cat([v, im])
 
-    out["dL_c128"], out["dR_c128"] = cplx(dL), cplx(dR)
-    x = cplx(out["psi"])
-    out["psi_c128"] = x / x.norm()
+    if "dL_c128" not in out:
+        out["dL_c128"], out["dR_c128"] = cplx(dL), cplx(dR)
+        x = cplx(out["psi"])
+        out["psi_c128"] = x / x.norm()
+    out["psi_" + pr] = out["psi_c128"]
     vz = vt.as_complex()
-    plan = native.HeffPlan(Lt.as_complex(), Rt.as_complex(), vz, vz, W, itk, native.F64)
+    sub = pr[len("c128"):].lstrip("_") or "f64"  # c128 (FP64), c128_f32, c128_bf16x3, ...
+    plan = native.HeffPlan(Lt.as_complex(), Rt.as_complex(), vz, vz, W, itk, native.PREC[sub])
     plan.bind_env(out["dL_c128"], out["dR_c128"])
-    out["plan_c128"] = plan
+    out["plan_" + pr] = plan
 
 
 def dense_heisenberg_centre(D=2048, L=128, seed=20260003, build_envs="sweep", precisions=("f64",), verbose=False):

@@ -604,7 +604,11 @@ static int64_t write_keys(const std::vector<BKey>& keys, const std::map<BKey, st
 // otherwise.  Transposes act on both planes alike.  The real and the
 // imaginary half of an output block are separate GEMM tasks whose segments
 // accumulate in registers, so every output plane is written once.
-static void complexify_stage(Stage& S, int64_t pa, int64_t pb, int64_t pc, double s) {
+// nega / negb >= 0: offset of a negated copy of A's / B's imaginary plane
+// (reduced-precision plans keep one), which then carries the minus sign of
+// Ar Br - Ai Bi so that every segment keeps its scale.
+static void complexify_stage(Stage& S, int64_t pa, int64_t pb, int64_t pc, double s, int64_t nega = -1,
+                             int64_t negb = -1) {
   std::vector<GTask> t2;
   std::vector<GSeg> s2;
   t2.reserve(2 * S.tasks.size());
@@ -622,6 +626,13 @@ static void complexify_stage(Stage& S, int64_t pa, int64_t pb, int64_t pc, doubl
         if (!im) {      // Ar Br, -s Ai Bi
           y.b_off += pb;
           y.alpha = -s * g.alpha;
+          if (s > 0 && nega >= 0) {
+            y.a_off = g.a_off + nega;
+            y.alpha = g.alpha;
+          } else if (s > 0 && negb >= 0) {
+            y.b_off = g.b_off + negb;
+            y.alpha = g.alpha;
+          }
         } else {        // Ar Bi, s Ai Br
           x.b_off += pb;
           y.alpha = s * g.alpha;
@@ -639,11 +650,41 @@ static void complexify_stage(Stage& S, int64_t pa, int64_t pb, int64_t pc, doubl
 // the lower tensor), U (R, or the upper tensor of an environment update), Y
 // (the output).  Doubles the plan's buffer extents and its flop counts
 // (a complex multiply-add is four real ones).
+// Reduced-precision H_eff plans convert both planes: L and R as [re | im |
+// -im], x as [re | im], each plane in the real plan's converted layout.
+static void dup_repack(std::vector<Repack>& r, int64_t src_plane, int64_t dst_plane, bool neg) {
+  const size_t n = r.size();
+  for (size_t i = 0; i < n; ++i) {
+    Repack im = r[i];
+    im.src_off += src_plane;
+    im.dst_off += dst_plane;
+    r.push_back(im);
+    if (neg) {
+      Repack ng = im;
+      ng.dst_off += dst_plane;
+      ng.scale = -im.scale;
+      r.push_back(ng);
+    }
+  }
+}
+
 static void complexify(mpsa_plan* P, int64_t pP, int64_t pQ, int64_t pU, int64_t pY, int kind) {
   const int64_t wsr = P->ws_per_vec;
-  complexify_stage(P->s1, pP, pQ, wsr, 1.0);
-  if (kind != 2) complexify_stage(P->s2, wsr, pU, pY, 1.0);  // y = T R^T
-  else complexify_stage(P->s2, pU, wsr, pY, -1.0);           // out = op(conj(Aup)) T
+  if (P->prec == F64) {
+    complexify_stage(P->s1, pP, pQ, wsr, 1.0);
+    if (kind != 2) complexify_stage(P->s2, wsr, pU, pY, 1.0);  // y = T R^T
+    else complexify_stage(P->s2, pU, wsr, pY, -1.0);           // out = op(conj(Aup)) T
+  } else {  // H_eff only (environment plans are FP64)
+    const int64_t Lr = P->Lc_elems, Rr = P->Rc_elems, xr = P->xc_per_vec;
+    dup_repack(P->rpL, pP, Lr, true);
+    dup_repack(P->rpR, pU, Rr, true);
+    dup_repack(P->rpx, pQ, xr, false);
+    complexify_stage(P->s1, Lr, xr, wsr, 1.0, 2 * Lr, -1);
+    complexify_stage(P->s2, wsr, Rr, pY, 1.0, -1, 2 * Rr);
+    P->Lc_elems = 3 * Lr;
+    P->Rc_elems = 3 * Rr;
+    P->xc_per_vec = 2 * xr;
+  }
   const size_t nm = P->mix.size();
   for (size_t i = 0; i < nm; ++i) {  // W mixes are real: the same sums on the imaginary plane
     MixTask m = P->mix[i];
@@ -1015,12 +1056,11 @@ int mpsa_env_outkeys(const mpsa_blocks* BL, const mpsa_blocks* A, const mpsa_blo
 }
 
 // 1 when every table of a plan is complex128, 0 when none is.
-static int complex_tables(std::initializer_list<const Tab*> t, int precision) {
+static int complex_tables(std::initializer_list<const Tab*> t) {
   int nc = 0;
   for (const Tab* x : t) nc += x->cplx;
   if (nc == 0) return 0;
   if (nc != (int)t.size()) throw Error(MPSA_EINVAL, "complex plans need every table complex128");
-  if (precision != F64) throw Error(MPSA_EINVAL, "complex plans run in FP64 only");
   return 1;
 }
 
@@ -1036,7 +1076,7 @@ static int plan_heff(mpsa_plan** plan, const mpsa_blocks* L, const mpsa_blocks* 
   P->kind = 0;
   P->prec = precision;
   Tab tl = parse(L, 1, "L"), tr = parse(R, 1, "R"), tx = parse(x, x->ns, "x"), ty = parse(y, y->ns, "y");
-  P->cplx = complex_tables({&tl, &tr, &tx, &ty}, precision);
+  P->cplx = complex_tables({&tl, &tr, &tx, &ty});
   P->L_total = tl.total;
   P->R_total = tr.total;
   P->x_total = tx.total;
@@ -1106,7 +1146,7 @@ int mpsa_env_plan(mpsa_plan** plan, const mpsa_blocks* BL, const mpsa_blocks* A,
     P->kind = 2;
     P->prec = F64;
     P->direction = direction;
-    P->cplx = complex_tables({&tb, &ta, &tap, &to}, F64);
+    P->cplx = complex_tables({&tb, &ta, &tap, &to});
     P->L_total = tb.total;
     P->x_total = ta.total;
     P->R_total = tap.total;

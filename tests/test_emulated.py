@@ -256,8 +256,23 @@ def test_emulated_complex_plans_match_oracle(lib, oracle, model, L, N, D, seed):
     want = oracle.contract_twosite(Lz, Rz, xz, mpo[c], mpo[c + 1], itk[c], itk[c + 1], cplx=True)
     got = vt.unpack(y.host())
     assert cc.rel(got, {k: np.asarray(want[k]) for k in want.keys()}) <= 1e-12
-    with pytest.raises(native.MpsaError):  # complex plans are FP64 only
-        native.HeffPlan(Lt, Rt, xt, vt, (mpo[c], mpo[c + 1]), (itk[c], itk[c + 1]), native.BF16X3)
+    # FP32 complex plan: L and R converted as [re | im | -im] planes, every segment at unit scale
+    p32 = native.HeffPlan(Lt, Rt, xt, vt, (mpo[c], mpo[c + 1]), (itk[c], itk[c + 1]), native.F32)
+    p32.bind_env(dL, dR)
+    y32 = DevBuf(lib, np.zeros(vt.length))
+    p32.apply(x, y32)
+    assert cc.rel(vt.unpack(y32.host()), {k: np.asarray(want[k]) for k in want.keys()}) <= 1e-5
+    del p32
+    # BF16X3 complex plan (its GEMMs are not emulated): the schedule keeps unit
+    # segment scales, so the 256x256 pipelined class stays admissible, and the
+    # conversion of both planes at binding stays inside the buffers
+    pb = native.HeffPlan(Lt, Rt, xt, vt, (mpo[c], mpo[c + 1]), (itk[c], itk[c + 1]), native.BF16X3)
+    sched = native.describe(pb.h)
+    assert np.all(sched["s1_segs"]["alpha"] == 1.0) and np.all(sched["s2_segs"]["alpha"] == 1.0)
+    pb.set_tile_class(1, 3)
+    pb.set_tile_class(2, 3)
+    pb.bind_env(dL, dR)
+    del pb
     with pytest.raises(native.MpsaError):  # every table complex, or none
         native.HeffPlan(native.Table.from_blocks(P["Ls"][c - 1]), Rt, xt, vt, (mpo[c], mpo[c + 1]),
                         (itk[c], itk[c + 1]), native.F64)

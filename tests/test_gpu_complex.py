@@ -127,3 +127,38 @@ def test_complex_batched_lanczos_matches_single_runs(gpu, oracle):
         s = native.lanczos(plan, x0[v * n:(v + 1) * n].clone(), 40, 1e-10, True, want_state=False)
         assert b["it"][v] == s["it"]
         assert np.max(np.abs(b["energies"][v] - s["energies"])) <= 1e-10 * max(1.0, np.max(np.abs(s["energies"])))
+
+
+@pytest.mark.parametrize("prec,tol", [(native.F32, 1e-5), (native.BF16X3, 1e-4)])
+@pytest.mark.parametrize("model,L,N,D,seed,dense", [("tv", 20, 10, 60, 117, False), ("xxz", 16, 0, 256, 118, True),
+                                                    ("hubbard", 12, 12, 64, 119, False)])
+def test_complex_reduced_precision_within_bound(gpu, oracle, model, L, N, D, seed, dense, prec, tol):
+    """Reduced-precision complex plans (L and R converted as [re | im | -im]
+    planes, x as [re | im]) against the complex oracle, within the
+    precision's bound for real plans, with every admissible tile class."""
+    _say("complex reduced precision", model, L, N, D, prec)
+    P, Lz, Rz, xz = _complex_case(oracle, model, L, N, D, seed, dense)
+    c, mpo, itk = P["c"], P["mpo"], P["itk"]
+    Lt, Rt, xt = native.Table.from_blocks(Lz), native.Table.from_blocks(Rz), native.Table.from_blocks(xz)
+    yt = native.heff2_outkeys(Lt, Rt, xt, mpo[c], mpo[c + 1], itk[c], itk[c + 1]).as_complex()
+    plan = native.HeffPlan(Lt, Rt, xt, yt, (mpo[c], mpo[c + 1]), (itk[c], itk[c + 1]), prec)
+    dL, dR = torch.from_numpy(Lt.pack(Lz)).cuda(), torch.from_numpy(Rt.pack(Rz)).cuda()
+    plan.bind_env(dL, dR)
+    x = torch.from_numpy(xt.pack(xz)).cuda()
+    want = oracle.contract_twosite(Lz, Rz, xz, mpo[c], mpo[c + 1], itk[c], itk[c + 1], cplx=True)
+    want = {k: np.asarray(want[k]) for k in want.keys()}
+    ncls = native.TILE_CLASSES if prec == native.BF16X3 else 3
+    for c1 in range(ncls):
+        for c2 in range(ncls):
+            try:
+                plan.set_tile_class(1, c1)
+                plan.set_tile_class(2, c2)
+            except native.MpsaError:
+                continue
+            y = torch.zeros(yt.length, dtype=torch.float64, device="cuda")
+            plan.apply(x, y)
+            torch.cuda.synchronize()
+            assert cc.rel(yt.unpack(y.cpu().numpy()), want) <= tol, (c1, c2)
+    if prec == native.BF16X3 and dense:  # unit segment scales: the 256x256 class is admitted
+        plan.set_tile_class(1, 3)
+        plan.set_tile_class(2, 3)
