@@ -15,7 +15,10 @@
 // Sectors with m >= n >= kPrecondMin are preconditioned: with V_e the
 // eigenvectors of the Gram matrix M^T M (rocSOLVER syevd), the columns of
 // B = M V_e are orthogonal up to rounding, so Jacobi on B stops after one or
-// two sweeps instead of tens; then M = U S (V_e V_B)^T.  The Jacobi pass on B
+// two sweeps instead of tens; then M = U S (V_e V_B)^T.  Wide sectors (m < n)
+// take gesvdj directly: the same construction on M^T (U_e from M M^T,
+// B = M^T U_e) measured 7e-11 relative singular-value error on a 64 x 128
+// sector where the tall and square sectors reach 1e-15.  The Jacobi pass on B
 // keeps the accuracy of a direct Jacobi SVD of M (the eigensolver only
 // supplies a rotation, whose rounding is part of B).  On a 4096 x 4096 sector
 // with singular values over ten decades: 1.3 s against 9.8 s, backward error
@@ -207,7 +210,9 @@ int mpsa_split(const mpsa_blocks* psi, const double* dpsi, const uint64_t* itoke
         pre_n = std::max(pre_n, S.n);
       }
     }
-    DevArr<double> Ge, Bw, Vb, ev, ework;  // V_e (in place of M^T M), B = M V_e, V_B^T, eigenvalues, syevd's E
+    // Ge: V_e (in place of M^T M), Bw: B = M V_e, Vb: V_B^T, ev/ework:
+    // syevd's eigenvalues and workspace
+    DevArr<double> Ge, Bw, Vb, ev, ework;
     if (pre_n) {
       Ge.alloc((size_t)pre_nn);
       Vb.alloc((size_t)pre_nn);
@@ -229,7 +234,7 @@ int mpsa_split(const mpsa_blocks* psi, const double* dpsi, const uint64_t* itoke
       double* Ms = M.p + S.off_M;
       // abstol 0: converge to machine precision; 100 sweeps is far above what
       // Jacobi needs (quadratic convergence), info reports a miss
-      if (S.m >= S.n && S.n >= kPrecondMin) {
+      if (S.m >= S.n && S.n >= kPrecondMin) {  // tall or square
         rchk(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, n, n, m, &one, Ms, m, Ms, m, &zero,
                            Ge.p, n),
              "split: Gram");

@@ -309,14 +309,18 @@ def test_emulated_complex_plans_match_oracle(lib, oracle, model, L, N, D, seed):
             del ep
 
 
-@pytest.mark.parametrize("model,L,N,D,seed,dense", [("xxz", 12, 0, 48, 503, True), ("tv", 16, 8, 80, 507, False)])
-def test_emulated_split_matches_oracle(lib, oracle, model, L, N, D, seed, dense):
-    """The split's sector SVDs, the Gram-preconditioned Jacobi path included
-    (sectors with m >= n >= 64; the dense bond is one 96 x 96 sector), against
+@pytest.mark.parametrize("model,L,N,D,seed,dense,c", [("xxz", 12, 0, 48, 503, True, None),
+                                                      ("tv", 16, 8, 80, 507, False, None),
+                                                      ("xxz", 16, 0, 64, 508, True, 5)])
+def test_emulated_split_matches_oracle(lib, oracle, model, L, N, D, seed, dense, c):
+    """The split's sector SVDs, the Gram-preconditioned Jacobi paths included
+    (m >= n >= 64: the dense centre bond is one 96 x 96 sector; the dense
+    bond at sites 5, 6 of L = 16 is one wide 64 x 128 sector, taken by gesvdj
+    directly), against
     the numpy restatement: singular values within 1e-12 relative (LAPACK on
     both sides here), the lambda-carrying tensor within 1e-10, isometries."""
     import pysplit
-    P = problems.build(oracle, model, L, N, D, seed, dense=dense)
+    P = problems.build(oracle, model, L, N, D, seed, c=c, dense=dense)
     c, itk, psi = P["c"], P["itk"], P["psi"]
     t = native.Table.from_blocks(psi)
     dpsi = DevBuf(lib, t.pack(psi))
@@ -345,7 +349,7 @@ def test_emulated_split_matches_oracle(lib, oracle, model, L, N, D, seed, dense)
     Lb.mpsa_split_destroy(h)
     left, lam, right, twr, chir = pysplit.split(psi, itk[c], itk[c + 1], chimax=10 ** 6, direction=1)
     assert int(chi[0]) == chir
-    if dense:  # one (2D) x (2D) sector of rank D: the preconditioned path ran
+    if dense:  # one sector of 2 m(l) x 2 m(r), both >= 64: a preconditioned path ran
         assert 2 * int(t.rows.max()) >= 64 and 2 * int(t.cols.max()) >= 64
     o = 0
     lmax = max(float(np.max(v)) for v in lam.values())
@@ -360,6 +364,11 @@ def test_emulated_split_matches_oracle(lib, oracle, model, L, N, D, seed, dense)
     for qc in lam:  # U^T U = 1 per sector
         X = np.vstack([got["left"][kk] for kk in sorted(got["left"].keys()) if kk[1] == qc])
         assert np.max(np.abs(X.T @ X - np.eye(X.shape[1]))) <= 1e-12
+    # the bare left vectors, each weighted by lambda / lambda_max (fixed by the SVD where lambda matters)
+    assert got["left"].keys() == left.keys()
+    for key in left.keys():
+        wgt = np.asarray(lam[key[1]]) / lmax
+        assert np.max(np.abs((got["left"][key] - left[key]) * wgt[None, :]), initial=0.0) <= 1e-10, key
 
 
 def test_emulated_chebyshev_moments_match_spectral_sum(lib, oracle):

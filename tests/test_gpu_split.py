@@ -75,19 +75,20 @@ def _check_isometry(tb, lam, side, tol):
         assert np.max(np.abs(G - np.eye(G.shape[0]))) <= tol, qc
 
 
-# the dense bonds are one (2D) x (2D) sector: 96 x 96 and 256 x 256 take the
-# Gram-preconditioned Jacobi path (sectors with m >= n >= 64)
-CASES = [("xxz", 10, 5, 12, 501, False), ("tv", 14, 7, 30, 502, False), ("xxz", 12, 0, 48, 503, True),
-         ("xxz", 16, 0, 128, 505, True)]
+# the dense centre bonds are one (2D) x (2D) sector: 96 x 96 and 256 x 256 take
+# the Gram-preconditioned Jacobi path (m >= n >= 64); the dense bond at sites
+# 5, 6 of L = 16 is one wide 64 x 128 sector (gesvdj directly)
+CASES = [("xxz", 10, 5, 12, 501, False, None), ("tv", 14, 7, 30, 502, False, None),
+         ("xxz", 12, 0, 48, 503, True, None), ("xxz", 16, 0, 128, 505, True, None), ("xxz", 16, 0, 64, 508, True, 5)]
 
 
-@pytest.mark.parametrize("model,L,N,D,seed,dense", CASES)
+@pytest.mark.parametrize("model,L,N,D,seed,dense,c", CASES)
 @pytest.mark.parametrize("direction,chimax,do_trunc,tw", [(1, 10 ** 6, False, 0.0), (0, 7, False, 0.0),
                                                           (1, 1000, True, 1e-2)])
-def test_split_matches_oracle(gpu, oracle, model, L, N, D, seed, dense, direction, chimax, do_trunc, tw):
+def test_split_matches_oracle(gpu, oracle, model, L, N, D, seed, dense, c, direction, chimax, do_trunc, tw):
     torch = gpu
-    _say("split", model, L, N, D, seed, "dir", direction, "chimax", chimax, "trunc", do_trunc)
-    P = problems.build(oracle, model, L, N, D, seed, dense=dense)
+    _say("split", model, L, N, D, seed, "c", c, "dir", direction, "chimax", chimax, "trunc", do_trunc)
+    P = problems.build(oracle, model, L, N, D, seed, c=c, dense=dense)
     c, itk, psi = P["c"], P["itk"], P["psi"]
     t = native.Table.from_blocks(psi)
     dpsi = torch.from_numpy(t.pack(psi)).cuda()
