@@ -188,7 +188,8 @@ def main():
     ap.add_argument("--L", type=int, default=128)
     ap.add_argument("--seed", type=int, default=20260003)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--precisions", default="f64,f64plain,f32,bf16,bf16x3")
+    ap.add_argument("--precisions", default="f64,f64plain,f32,bf16,bf16x3,c128,c128_bf16x3",
+                    help="plans to time; c128 and c128_<prec> are complex128 plans of the same bond")
     ap.add_argument("--mixed", default="bf16x3", help="precision of the mixed-precision matvec")
     ap.add_argument("--lanczos-itmax", type=int, default=700)
     ap.add_argument("--refine-it", type=int, default=10)
