@@ -3,7 +3,9 @@ layer (tests/emu/emu.cpp): kernels computed on the host with every device
 address they touch checked against the live allocations, never-written memory
 NaN-filled, freed memory kept and poisoned.  Covered: the DMRG sweep
 (mpsa_dmrg.cpp over the plan builder, the split and the Lanczos driver,
-unchanged) and the tile-class selection of H_eff plans.
+unchanged), the tile-class selection of H_eff plans, Hubbard (U(1) x U(1))
+plans, complex128 plans (FP64 and FP32; the BF16X3 schedule), the split's
+sector SVD paths, the Chebyshev moments driver and the C++ drop-in header.
 
 Bars, as for the device sweep (DMRGengine::simulate, src/container.hpp:383-438):
 untruncated sweeps reach the exact ground-state energy within 1e-9 (free
