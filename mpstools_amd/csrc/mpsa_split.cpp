@@ -15,10 +15,16 @@
 // Sectors with m >= n >= kPrecondMin are preconditioned: with V_e the
 // eigenvectors of the Gram matrix M^T M (rocSOLVER syevd), the columns of
 // B = M V_e are orthogonal up to rounding, so Jacobi on B stops after one or
-// two sweeps instead of tens; then M = U S (V_e V_B)^T.  Wide sectors (m < n)
-// take gesvdj directly: the same construction on M^T (U_e from M M^T,
-// B = M^T U_e) measured 7e-11 relative singular-value error on a 64 x 128
-// sector where the tall and square sectors reach 1e-15.  The Jacobi pass on B
+// two sweeps instead of tens; then M = U S (V_e V_B)^T.  The accuracy is then
+// that of gesvdj's stopping test, an absolute bound on the off-diagonal of
+// B^T B: it does not resolve singular values below about sqrt(eps) * sigma_max
+// once the large ones are converged.  Measured: a 128 x 64 sector with
+// singular values down to 3e-10 sigma_max, 6.8e-11 sigma_max error after one
+// sweep against 7e-15 for direct gesvdj (nine sweeps); a 4096 x 4096 sector,
+// backward error 1.6e-9 against direct gesvdj's 2.2e-9 (38 sweeps, 7.6x the
+// time).  So only sectors of at least kPrecondMin columns, where the direct
+// Jacobi SVD is itself tolerance-limited and slow, are preconditioned; wide
+// sectors (m < n) always take gesvdj directly.  The Jacobi pass on B
 // keeps the accuracy of a direct Jacobi SVD of M (the eigensolver only
 // supplies a rotation, whose rounding is part of B).  On a 4096 x 4096 sector
 // with singular values over ten decades: 1.3 s against 9.8 s, backward error
@@ -54,8 +60,14 @@ void rchk(rocblas_status s, const char* what) {
     throw SErr(MPSA_EHIP, std::string(what) + ": rocBLAS/rocSOLVER status " + std::to_string((int)s));
 }
 int64_t align64(int64_t x) { return (x + 63) / 64 * 64; }
-// smallest column count of a sector whose SVD is preconditioned
-constexpr int64_t kPrecondMin = 64;
+// smallest column count of a sector whose SVD is preconditioned (below it the
+// direct Jacobi SVD is cheap and converges to rounding level); the CPU
+// emulation of the device layer builds with a smaller value so that its tests
+// reach this path at small sizes
+#ifndef MPSA_PRECOND_MIN
+#define MPSA_PRECOND_MIN 1024
+#endif
+constexpr int64_t kPrecondMin = MPSA_PRECOND_MIN;
 
 // one rocBLAS handle per device, created on first use
 rocblas_handle handle_for_current_device() {

@@ -316,9 +316,9 @@ def test_emulated_complex_plans_match_oracle(lib, oracle, model, L, N, D, seed):
                                                       ("xxz", 16, 0, 64, 508, True, 5)])
 def test_emulated_split_matches_oracle(lib, oracle, model, L, N, D, seed, dense, c):
     """The split's sector SVDs, the Gram-preconditioned Jacobi paths included
-    (m >= n >= 64: the dense centre bond is one 96 x 96 sector; the dense
-    bond at sites 5, 6 of L = 16 is one wide 64 x 128 sector, taken by gesvdj
-    directly), against
+    (built here with the threshold at 64 columns, m >= n >= 64: the dense
+    centre bond is one 96 x 96 sector; the dense bond at sites 5, 6 of L = 16
+    is one wide 64 x 128 sector, taken by gesvdj directly), against
     the numpy restatement: singular values within 1e-12 relative (LAPACK on
     both sides here), the lambda-carrying tensor within 1e-10, isometries."""
     import pysplit

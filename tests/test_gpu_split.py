@@ -58,7 +58,8 @@ def _cmp_weighted(tb, ref, lam_got, lam_ref, side, tol):
     for k in ref.keys():
         qc = k[1] if side == "left" else k[0]
         w = np.asarray(lam_ref[qc]) / lmax
-        assert np.allclose(np.asarray(lam_got[qc]) / lmax, w, rtol=0, atol=1e-14)
+        # the lambda bar of _cmp_lambda (1e-12 relative or 100 eps lambda_max), in lambda_max units
+        assert np.all(np.abs(np.asarray(lam_got[qc]) / lmax - w) <= np.maximum(1e-12 * w, 100 * np.finfo(float).eps))
         d = got[k] - ref[k]
         d = d * w[None, :] if side == "left" else d * w[:, None]
         assert np.max(np.abs(d), initial=0.0) <= tol, k
@@ -75,9 +76,9 @@ def _check_isometry(tb, lam, side, tol):
         assert np.max(np.abs(G - np.eye(G.shape[0]))) <= tol, qc
 
 
-# the dense centre bonds are one (2D) x (2D) sector: 96 x 96 and 256 x 256 take
-# the Gram-preconditioned Jacobi path (m >= n >= 64); the dense bond at sites
-# 5, 6 of L = 16 is one wide 64 x 128 sector (gesvdj directly)
+# the dense centre bonds are one (2D) x (2D) sector (96 x 96, 256 x 256); the
+# dense bond at sites 5, 6 of L = 16 is one wide 64 x 128 sector; all of them
+# below the preconditioning threshold (1024 columns): gesvdj directly
 CASES = [("xxz", 10, 5, 12, 501, False, None), ("tv", 14, 7, 30, 502, False, None),
          ("xxz", 12, 0, 48, 503, True, None), ("xxz", 16, 0, 128, 505, True, None), ("xxz", 16, 0, 64, 508, True, 5)]
 
@@ -143,3 +144,35 @@ def test_split_of_a_ground_state_truncated(gpu, oracle, model, L, N, D, seed, de
     for qc, v in out["lam"].items():
         X = np.hstack([gr[k] for k in sorted(gr.keys()) if k[0] == qc])
         assert np.max(np.abs(X @ X.T - np.diag(np.asarray(v) ** 2))) <= 1e-12, qc
+
+
+
+def test_split_preconditioned_sector_within_tolerance(gpu, oracle):
+    """A 1024 x 1024 sector (the dense D = 512 bond) through the
+    Gram-preconditioned Jacobi SVD.  Its accuracy is that of gesvdj's stopping
+    test, an absolute bound on the off-diagonal of B^T B, not rounding level, so
+    the bars are: lambda within 1e-9 lambda_max, the lambda-carrying tensor and
+    the lambda-weighted bare vectors within 1e-9, both isometries within 1e-12,
+    and the rebuilt psi within 1e-9 of its norm."""
+    torch = gpu
+    _say("split preconditioned 1024 x 1024")
+    P = problems.build(oracle, "xxz", 22, 0, 512, 509, dense=True)
+    c, itk, psi = P["c"], P["itk"], P["psi"]
+    t = native.Table.from_blocks(psi)
+    assert 2 * int(t.cols.max()) >= 1024
+    dpsi = torch.from_numpy(t.pack(psi)).cuda()
+    out = native.split(t, dpsi, itk[c], itk[c + 1], direction=1, chimax=10 ** 6)
+    left, lam, right, twr, chi = pysplit.split(psi, itk[c], itk[c + 1], chimax=10 ** 6, direction=1)
+    assert out["chi"] == chi
+    lmax = max(float(np.max(v)) for v in lam.values())
+    for qc in lam:
+        assert np.max(np.abs(np.asarray(out["lam"][qc]) - np.asarray(lam[qc]))) <= 1e-9 * lmax
+    _cmp_blocks(*out["right"], right, 1e-9)
+    _cmp_weighted(out["left"], left, out["lam"], lam, "left", 1e-9)
+    _check_isometry(out["left"], out["lam"], "left", 1e-12)
+    gl = out["left"][0].unpack(out["left"][1].cpu().numpy())
+    gr = out["right"][0].unpack(out["right"][1].cpu().numpy())
+    rebuilt = twosite(gl, gr)
+    num = sum(float(np.sum((rebuilt[k] - psi[k]) ** 2)) for k in psi.keys())
+    den = sum(float(np.sum(psi[k] ** 2)) for k in psi.keys())
+    assert np.sqrt(num / den) <= 1e-9  # psi is normalised, so lambda is not rescaled
