@@ -256,6 +256,12 @@ int mpsa_chebyshev_moments(mpsa_plan* plan, const double* dt0, int nmom, double 
 int mpsa_tridiag_eig(int n, double* d, const double* e, double* z);
 /* Free the device vector pool kept between Lanczos calls. */
 int mpsa_release_pools(void);
+/* Idle bytes held by that pool and its cap (default 8 GiB).  Vectors handed
+ * back beyond the cap are freed, vectors of other lengths first; a failed
+ * device allocation releases every idle vector and is tried once more.
+ * (No reference counterpart: the reference allocates per Lanczos run.) */
+int mpsa_pool_info(int64_t* idle_bytes, int64_t* cap_bytes);
+int mpsa_pool_set_cap(int64_t cap_bytes);
 
 /* ---- split of a two-site tensor (split, src/MPSfunctions.hpp:1142-1278) ----
  * psi (ns = 2 table, FP64 device buffer) -> left site tensor, singular values

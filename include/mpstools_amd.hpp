@@ -25,8 +25,10 @@
 #include <cstdlib>
 #include <cstring>
 #include <iostream>
+#include <list>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -404,12 +406,163 @@ void download(BlockMatrix<Ns, K, T>& B, const Tab& t, const DevBuf& d) {
   }
 }
 
+// ---------------------------------------------------------------- plan cache
+// The reference recomputes contractTwosite from scratch on every Lanczos step
+// (src/lanzcos.hpp:197) with the same BL, BR and MPO.  Here a plan is built
+// once per block structure: the cache key is the key tables (charges, physical
+// or aux indices, shapes) of every operand, the MPO entries and i_to_key.  An
+// entry keeps the plan, its device buffers and the output table; the
+// environment buffers are uploaded again only when a 64-bit hash of the host
+// blocks changes.  Least recently used entries are dropped beyond kCap.  The
+// cache is process-wide, guarded by a mutex, and never destroyed at exit (the
+// HIP runtime may be gone by then); release_plan_cache() frees it explicitly.
+inline void sig_append(std::string& s, const void* p, size_t n) { s.append((const char*)p, n); }
+inline std::string sig_of(const Tab& t) {
+  std::string s;
+  sig_append(s, t.q.data(), t.q.size() * 8);
+  sig_append(s, t.p.data(), t.p.size() * 4);
+  sig_append(s, t.rows.data(), t.rows.size() * 8);
+  sig_append(s, t.cols.data(), t.cols.size() * 8);
+  s += "|" + std::to_string(t.planes) + "|" + std::to_string(t.rows.size()) + "|";
+  return s;
+}
+inline std::string sig_of(const Mpo& m) {
+  std::string s = std::to_string(m.m.Dl) + "," + std::to_string(m.m.Dr) + ":";
+  sig_append(s, m.wl.data(), m.wl.size() * 4);
+  sig_append(s, m.wr.data(), m.wr.size() * 4);
+  sig_append(s, m.out.data(), m.out.size() * 4);
+  sig_append(s, m.in.data(), m.in.size() * 4);
+  sig_append(s, m.val.data(), m.val.size() * 8);
+  return s + "|" + std::to_string(m.val.size()) + "|";
+}
+inline std::string sig_of(const std::vector<uint64_t>& v) {
+  std::string s;
+  sig_append(s, v.data(), v.size() * 8);
+  return s + "|" + std::to_string(v.size()) + "|";
+}
+
+// Word-wise multiply-xor hash of the host blocks (shapes are in the key).
+template <int Ns, class K, typename T>
+uint64_t hash_blocks(const BlockMatrix<Ns, K, T>& B) {
+  uint64_t h = 0x243f6a8885a308d3ull;
+  for (auto& kv : B) {
+    const size_t nb = kv.second.size1() * kv.second.size2() * sizeof(T);
+    const unsigned char* d = reinterpret_cast<const unsigned char*>(kv.second.data());
+    size_t i = 0;
+    for (; i + 8 <= nb; i += 8) {
+      uint64_t w;
+      std::memcpy(&w, d + i, 8);
+      h = (h ^ w) * 0x100000001b3ull;
+      h ^= h >> 29;
+    }
+    for (; i < nb; ++i) h = (h ^ d[i]) * 0x100000001b3ull;
+    h = (h ^ nb) * 0x9e3779b97f4a7c15ull;
+  }
+  return h;
+}
+
+struct CachedPlan {
+  std::string key;
+  mpsa_plan* plan = nullptr;
+  Tab ty;  // output table
+  std::unique_ptr<DevBuf> d0, d1, dx, dxp, dy;  // env 0 (L / BL), env 1 (R), x (A), Ap, y
+  uint64_t h0 = 0, h1 = 0;
+  bool bound = false;
+  ~CachedPlan() {
+    if (plan) mpsa_plan_destroy(plan);
+  }
+};
+
+struct PlanCacheStats {
+  long builds = 0;       // plans built
+  long hits = 0;         // calls served by a cached plan
+  long env_uploads = 0;  // environment blocks (L, R) copied to the device
+};
+
+struct PlanCache {
+  static constexpr size_t kCap = 8;
+  std::mutex mu;
+  std::list<std::unique_ptr<CachedPlan>> items;  // most recently used first
+  PlanCacheStats stats;
+  CachedPlan* find(const std::string& k) {
+    for (auto it = items.begin(); it != items.end(); ++it)
+      if ((*it)->key == k) {
+        items.splice(items.begin(), items, it);
+        ++stats.hits;
+        return items.front().get();
+      }
+    return nullptr;
+  }
+  CachedPlan* insert(std::unique_ptr<CachedPlan> e) {
+    ++stats.builds;
+    items.push_front(std::move(e));
+    while (items.size() > kCap) items.pop_back();
+    return items.front().get();
+  }
+};
+inline PlanCache& plan_cache() {
+  static PlanCache* c = new PlanCache();  // leaked on purpose: no device frees after runtime teardown
+  return *c;
+}
+
 }  // namespace detail
 
+// Counters of the drop-in plan cache (plans built, calls served from it,
+// environment uploads) and its explicit release.
+inline detail::PlanCacheStats plan_cache_stats() {
+  std::lock_guard<std::mutex> g(detail::plan_cache().mu);
+  return detail::plan_cache().stats;
+}
+inline void release_plan_cache() {
+  std::lock_guard<std::mutex> g(detail::plan_cache().mu);
+  detail::plan_cache().items.clear();
+  detail::plan_cache().stats = detail::PlanCacheStats();
+}
+
 // ------------------------------------------------------------------ contractions
+namespace detail {
+// Output table from an *_outkeys query: fn(cap, &n, q, p, rows, cols).
+template <typename F>
+Tab out_table(int ns, int planes, F fn, const char* where) {
+  int64_t n = 0;
+  mpsa_check(fn(0, &n, nullptr, nullptr, nullptr, nullptr), where);
+  std::vector<uint64_t> q((size_t)(2 * n + 2));
+  std::vector<uint32_t> p((size_t)(ns * n + ns));
+  std::vector<int64_t> r((size_t)n + 1), c((size_t)n + 1);
+  if (n) mpsa_check(fn(n, &n, q.data(), p.data(), r.data(), c.data()), where);
+  q.resize((size_t)(2 * n));
+  p.resize((size_t)(ns * n));
+  r.resize((size_t)n);
+  c.resize((size_t)n);
+  return table_of_keys(ns, n, q, p, r, c, planes);
+}
+
+// Upload the environments of a cached H_eff entry when their content changed,
+// then the vector, apply, and download the result.
+template <int NsX, class K, typename T>
+void heff_run(CachedPlan& e, const BlockMatrix<1, K, T>& BL, const Tab& tl, const BlockMatrix<1, K, T>& BR,
+              const Tab& tr, const BlockMatrix<NsX, K, T>& A, const Tab& tx, BlockMatrix<NsX, K, T>& Aout,
+              const char* where) {
+  const uint64_t hl = hash_blocks(BL), hr = hash_blocks(BR);
+  if (!e.bound || hl != e.h0 || hr != e.h1) {
+    upload(BL, tl, *e.d0);
+    upload(BR, tr, *e.d1);
+    mpsa_check(mpsa_plan_bind_env(e.plan, e.d0->p, e.d1->p, nullptr), where);
+    e.h0 = hl;
+    e.h1 = hr;
+    e.bound = true;
+    ++plan_cache().stats.env_uploads;
+  }
+  upload(A, tx, *e.dx);
+  mpsa_check(mpsa_heff_apply(e.plan, e.dx->p, 0, e.dy->p, 0, 1, nullptr), where);
+  download(Aout, e.ty, *e.dy);
+}
+}  // namespace detail
+
 // contractTwosite (src/MPSfunctions.hpp:949-981).  maxthreads and debug are
 // accepted for signature compatibility; the work is one grouped GPU launch per
-// stage.
+// stage.  Repeated calls on the same block structure reuse a cached plan and,
+// while BL and BR are unchanged, their device copies.
 template <class KeyType, typename T>
 void contractTwosite(BlockMatrix<2, KeyType, T>& Aout, const BlockMatrix<1, KeyType, T>& BL,
                      const BlockMatrix<1, KeyType, T>& BR, const BlockMatrix<2, KeyType, T>& A,
@@ -421,37 +574,34 @@ void contractTwosite(BlockMatrix<2, KeyType, T>& Aout, const BlockMatrix<1, KeyT
   Tab tl = table_of(BL), tr = table_of(BR), tx = table_of(A);
   Mpo ml = mpo_of(mpol), mr = mpo_of(mpor);
   auto il = itk_of(itokey.at(0)), ir = itk_of(itokey.at(1));
-  int64_t n = 0;
-  mpsa_check(mpsa_heff2_outkeys(&tl.b, &tr.b, &tx.b, &ml.m, &mr.m, il.data(), (int)il.size(), ir.data(),
-                                (int)ir.size(), 0, &n, nullptr, nullptr, nullptr, nullptr),
-             "contractTwosite");
-  std::vector<uint64_t> q((size_t)(2 * n + 2));
-  std::vector<uint32_t> p((size_t)(2 * n + 2));
-  std::vector<int64_t> r((size_t)n + 1), c((size_t)n + 1);
-  if (n)
-    mpsa_check(mpsa_heff2_outkeys(&tl.b, &tr.b, &tx.b, &ml.m, &mr.m, il.data(), (int)il.size(), ir.data(),
-                                  (int)ir.size(), n, &n, q.data(), p.data(), r.data(), c.data()),
+  const std::string key = "heff2|" + sig_of(tl) + sig_of(tr) + sig_of(tx) + sig_of(ml) + sig_of(mr) + sig_of(il) +
+                          sig_of(ir);
+  PlanCache& C = plan_cache();
+  std::lock_guard<std::mutex> g(C.mu);
+  CachedPlan* e = C.find(key);
+  if (!e) {
+    std::unique_ptr<CachedPlan> ne(new CachedPlan());
+    ne->key = key;
+    ne->ty = out_table(2, planes_of<T>::value,
+                       [&](int64_t cap, int64_t* n, uint64_t* q, uint32_t* p, int64_t* r, int64_t* c) {
+                         return mpsa_heff2_outkeys(&tl.b, &tr.b, &tx.b, &ml.m, &mr.m, il.data(), (int)il.size(),
+                                                   ir.data(), (int)ir.size(), cap, n, q, p, r, c);
+                       },
+                       "contractTwosite");
+    ne->ty.finish(2);
+    mpsa_check(mpsa_heff2_plan(&ne->plan, &tl.b, &tr.b, &tx.b, &ne->ty.b, &ml.m, &mr.m, il.data(), (int)il.size(),
+                               ir.data(), (int)ir.size(), MPSA_F64),
                "contractTwosite");
-  q.resize((size_t)(2 * n));
-  p.resize((size_t)(2 * n));
-  r.resize((size_t)n);
-  c.resize((size_t)n);
-  Tab ty = table_of_keys(2, n, q, p, r, c, planes_of<T>::value);
-  mpsa_plan* plan = nullptr;
-  mpsa_check(mpsa_heff2_plan(&plan, &tl.b, &tr.b, &tx.b, &ty.b, &ml.m, &mr.m, il.data(), (int)il.size(), ir.data(),
-                             (int)ir.size(), MPSA_F64),
-             "contractTwosite");
-  DevBuf dL((size_t)tl.len()), dR((size_t)tr.len()), dx((size_t)tx.len()), dy((size_t)ty.len());
-  upload(BL, tl, dL);
-  upload(BR, tr, dR);
-  upload(A, tx, dx);
-  mpsa_check(mpsa_plan_bind_env(plan, dL.p, dR.p, nullptr), "contractTwosite");
-  mpsa_check(mpsa_heff_apply(plan, dx.p, 0, dy.p, 0, 1, nullptr), "contractTwosite");
-  mpsa_plan_destroy(plan);
-  download(Aout, ty, dy);
+    ne->d0.reset(new DevBuf((size_t)tl.len()));
+    ne->d1.reset(new DevBuf((size_t)tr.len()));
+    ne->dx.reset(new DevBuf((size_t)tx.len()));
+    ne->dy.reset(new DevBuf((size_t)ne->ty.len()));
+    e = C.insert(std::move(ne));
+  }
+  heff_run(*e, BL, tl, BR, tr, A, tx, Aout, "contractTwosite");
 }
 
-// contractSingle (src/MPSfunctions.hpp:857-885).
+// contractSingle (src/MPSfunctions.hpp:857-885), cached like contractTwosite.
 template <class KeyType, typename T>
 void contractSingle(BlockMatrix<1, KeyType, T>& Aout, const BlockMatrix<1, KeyType, T>& BL,
                     const BlockMatrix<1, KeyType, T>& A, const MPOMat<1, T>& mpo, const BlockMatrix<1, KeyType, T>& BR,
@@ -461,33 +611,29 @@ void contractSingle(BlockMatrix<1, KeyType, T>& Aout, const BlockMatrix<1, KeyTy
   Tab tl = table_of(BL), tr = table_of(BR), tx = table_of(A);
   Mpo m = mpo_of(mpo);
   auto it = itk_of(itokey);
-  int64_t n = 0;
-  mpsa_check(mpsa_heff1_outkeys(&tl.b, &tr.b, &tx.b, &m.m, it.data(), (int)it.size(), 0, &n, nullptr, nullptr,
-                                nullptr, nullptr),
-             "contractSingle");
-  std::vector<uint64_t> q((size_t)(2 * n + 2));
-  std::vector<uint32_t> p((size_t)n + 1);
-  std::vector<int64_t> r((size_t)n + 1), c((size_t)n + 1);
-  if (n)
-    mpsa_check(mpsa_heff1_outkeys(&tl.b, &tr.b, &tx.b, &m.m, it.data(), (int)it.size(), n, &n, q.data(), p.data(),
-                                  r.data(), c.data()),
+  const std::string key = "heff1|" + sig_of(tl) + sig_of(tr) + sig_of(tx) + sig_of(m) + sig_of(it);
+  PlanCache& C = plan_cache();
+  std::lock_guard<std::mutex> g(C.mu);
+  CachedPlan* e = C.find(key);
+  if (!e) {
+    std::unique_ptr<CachedPlan> ne(new CachedPlan());
+    ne->key = key;
+    ne->ty = out_table(1, planes_of<T>::value,
+                       [&](int64_t cap, int64_t* n, uint64_t* q, uint32_t* p, int64_t* r, int64_t* c) {
+                         return mpsa_heff1_outkeys(&tl.b, &tr.b, &tx.b, &m.m, it.data(), (int)it.size(), cap, n, q,
+                                                   p, r, c);
+                       },
+                       "contractSingle");
+    ne->ty.finish(1);
+    mpsa_check(mpsa_heff1_plan(&ne->plan, &tl.b, &tr.b, &tx.b, &ne->ty.b, &m.m, it.data(), (int)it.size(), MPSA_F64),
                "contractSingle");
-  q.resize((size_t)(2 * n));
-  p.resize((size_t)n);
-  r.resize((size_t)n);
-  c.resize((size_t)n);
-  Tab ty = table_of_keys(1, n, q, p, r, c, planes_of<T>::value);
-  mpsa_plan* plan = nullptr;
-  mpsa_check(mpsa_heff1_plan(&plan, &tl.b, &tr.b, &tx.b, &ty.b, &m.m, it.data(), (int)it.size(), MPSA_F64),
-             "contractSingle");
-  DevBuf dL((size_t)tl.len()), dR((size_t)tr.len()), dx((size_t)tx.len()), dy((size_t)ty.len());
-  upload(BL, tl, dL);
-  upload(BR, tr, dR);
-  upload(A, tx, dx);
-  mpsa_check(mpsa_plan_bind_env(plan, dL.p, dR.p, nullptr), "contractSingle");
-  mpsa_check(mpsa_heff_apply(plan, dx.p, 0, dy.p, 0, 1, nullptr), "contractSingle");
-  mpsa_plan_destroy(plan);
-  download(Aout, ty, dy);
+    ne->d0.reset(new DevBuf((size_t)tl.len()));
+    ne->d1.reset(new DevBuf((size_t)tr.len()));
+    ne->dx.reset(new DevBuf((size_t)tx.len()));
+    ne->dy.reset(new DevBuf((size_t)ne->ty.len()));
+    e = C.insert(std::move(ne));
+  }
+  heff_run(*e, BL, tl, BR, tr, A, tx, Aout, "contractSingle");
 }
 
 namespace detail {
@@ -502,35 +648,37 @@ BlockMatrix<1, KeyType, T> env_layer(const BlockMatrix<1, KeyType, T>& BL, const
   if (Ap) tap = table_of(*Ap);
   Mpo m = mpo_of(mpo);
   auto it = itk_of(itokey);
-  int64_t n = 0;
-  mpsa_check(mpsa_env_outkeys(&tb.b, &ta.b, Ap ? &tap.b : nullptr, &m.m, it.data(), (int)it.size(), direction, 0,
-                              &n, nullptr, nullptr, nullptr, nullptr),
-             "addContractionLayer");
-  std::vector<uint64_t> q((size_t)(2 * n + 2));
-  std::vector<uint32_t> p((size_t)n + 1);
-  std::vector<int64_t> r((size_t)n + 1), c((size_t)n + 1);
-  if (n)
-    mpsa_check(mpsa_env_outkeys(&tb.b, &ta.b, Ap ? &tap.b : nullptr, &m.m, it.data(), (int)it.size(), direction, n,
-                                &n, q.data(), p.data(), r.data(), c.data()),
+  const std::string key = "env|" + std::to_string(direction > 0 ? 1 : 0) + (Ap ? "|ap|" + sig_of(tap) : "|") +
+                          sig_of(tb) + sig_of(ta) + sig_of(m) + sig_of(it);
+  PlanCache& C = plan_cache();
+  std::lock_guard<std::mutex> g(C.mu);
+  CachedPlan* e = C.find(key);
+  if (!e) {
+    std::unique_ptr<CachedPlan> ne(new CachedPlan());
+    ne->key = key;
+    ne->ty = out_table(1, planes_of<T>::value,
+                       [&](int64_t cap, int64_t* n, uint64_t* q, uint32_t* p, int64_t* r, int64_t* c) {
+                         return mpsa_env_outkeys(&tb.b, &ta.b, Ap ? &tap.b : nullptr, &m.m, it.data(),
+                                                 (int)it.size(), direction, cap, n, q, p, r, c);
+                       },
+                       "addContractionLayer");
+    ne->ty.finish(1);
+    mpsa_check(mpsa_env_plan(&ne->plan, &tb.b, &ta.b, Ap ? &tap.b : nullptr, &m.m, it.data(), (int)it.size(),
+                             direction, &ne->ty.b, MPSA_F64),
                "addContractionLayer");
-  q.resize((size_t)(2 * n));
-  p.resize((size_t)n);
-  r.resize((size_t)n);
-  c.resize((size_t)n);
-  Tab to = table_of_keys(1, n, q, p, r, c, planes_of<T>::value);
-  mpsa_plan* plan = nullptr;
-  mpsa_check(mpsa_env_plan(&plan, &tb.b, &ta.b, Ap ? &tap.b : nullptr, &m.m, it.data(), (int)it.size(), direction,
-                           &to.b, MPSA_F64),
+    ne->d0.reset(new DevBuf((size_t)tb.len()));
+    ne->dx.reset(new DevBuf((size_t)ta.len()));
+    ne->dxp.reset(new DevBuf(Ap ? (size_t)tap.len() : 1));
+    ne->dy.reset(new DevBuf((size_t)ne->ty.len()));
+    e = C.insert(std::move(ne));
+  }
+  upload(BL, tb, *e->d0);
+  upload(A, ta, *e->dx);
+  if (Ap) upload(*Ap, tap, *e->dxp);
+  mpsa_check(mpsa_env_apply(e->plan, e->d0->p, e->dx->p, Ap ? e->dxp->p : nullptr, e->dy->p, nullptr),
              "addContractionLayer");
-  DevBuf dB((size_t)tb.len()), dA((size_t)ta.len()), dO((size_t)to.len());
-  DevBuf dAp(Ap ? (size_t)tap.len() : 1);
-  upload(BL, tb, dB);
-  upload(A, ta, dA);
-  if (Ap) upload(*Ap, tap, dAp);
-  mpsa_check(mpsa_env_apply(plan, dB.p, dA.p, Ap ? dAp.p : nullptr, dO.p, nullptr), "addContractionLayer");
-  mpsa_plan_destroy(plan);
   BlockMatrix<1, KeyType, T> out;
-  download(out, to, dO);
+  download(out, e->ty, *e->dy);
   return out;
 }
 }  // namespace detail
@@ -622,9 +770,11 @@ class TSLanzcosEngine {
     }
     _space = table_of_keys(2, (int64_t)vr.size(), vq, vp, vr, vc, planes_of<T>::value);
     mpsa_plan* plan = nullptr;
+    _plan_keep.reset();
     mpsa_check(mpsa_heff2_plan(&plan, &tl.b, &tr.b, &_space.b, &_space.b, &ml.m, &mr.m, il.data(), (int)il.size(),
                                ir.data(), (int)ir.size(), _prec),
                "TSLanzcosEngine");
+    _plan_keep.reset(new PlanHolder{plan});  // owns the plan from here on, also when a later call fails
     DevBuf dL((size_t)tl.len()), dR((size_t)tr.len()), dx((size_t)_space.len());
     upload(_L, tl, dL);
     upload(_R, tr, dR);
@@ -642,7 +792,6 @@ class TSLanzcosEngine {
                             nullptr),
                "TSLanzcosEngine::simulate");
     _es.resize((size_t)nr);
-    _plan_keep.reset(new PlanHolder{plan});
     _dx.reset(new DevBuf((size_t)_space.len()));
     _keepL.reset(new DevBuf((size_t)tl.len()));
     _keepR.reset(new DevBuf((size_t)tr.len()));

@@ -368,6 +368,11 @@ int mpsa_dmrg_create(mpsa_dmrg** out, int nsites, const mpsa_blocks* sites, cons
   mpsa_dmrg* D = nullptr;
   try {
     if (!out || nsites < 3 || !sites || !dA || !W || !itokey || !d) throw HErr(MPSA_EINVAL, "dmrg_create: bad arguments");
+#ifndef MPSA_DMRG_SWEEP
+    // Switched off together with mpsa_dmrg_sweep (see there): create builds
+    // the right environments through the same plan cache and buffer code.
+    throw HErr(MPSA_EINVAL, "dmrg_create: disabled in this build (unexplained GPU memory fault under investigation)");
+#endif
     D = new mpsa_dmrg();
     D->N = nsites;
     D->st = (hipStream_t)stream;

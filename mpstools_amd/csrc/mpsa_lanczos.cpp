@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -98,26 +99,79 @@ int tridiag_ql(int n, double* d, const double* e_in, double* z) {
 }
 
 // Pool of device vectors (Krylov bases are reused across Lanczos calls).
+// Idle vectors are kept up to a byte cap; a vector handed back beyond the cap
+// frees idle vectors of other lengths first (the ones a sweep over bonds of
+// other sizes left behind), then itself.  A failed allocation releases every
+// idle vector and tries once more.  All members lock one mutex, so Lanczos
+// runs on different host threads share the pool safely.
 struct VecPool {
+  std::mutex mu;
   std::map<int64_t, std::vector<double*>> free_;
+  int64_t idle_bytes = 0;
+  int64_t cap_bytes = int64_t(8) << 30;  // 8 GiB of idle vectors
+  static int64_t bytes_of(int64_t n) { return (int64_t)sizeof(double) * std::max<int64_t>(n, 1); }
+  void release_locked() {
+    for (auto& kv : free_)
+      for (double* p : kv.second) (void)hipFree(p);
+    free_.clear();
+    idle_bytes = 0;
+  }
+  void trim_locked(int64_t keep_len) {  // free idle vectors of other lengths, then of keep_len, down to the cap
+    for (auto it = free_.begin(); it != free_.end() && idle_bytes > cap_bytes;) {
+      if (it->first == keep_len) {
+        ++it;
+        continue;
+      }
+      for (double* p : it->second) {
+        (void)hipFree(p);
+        idle_bytes -= bytes_of(it->first);
+      }
+      it = free_.erase(it);
+    }
+    auto k = free_.find(keep_len);
+    while (k != free_.end() && idle_bytes > cap_bytes && !k->second.empty()) {
+      (void)hipFree(k->second.back());
+      k->second.pop_back();
+      idle_bytes -= bytes_of(keep_len);
+    }
+  }
   double* get(int64_t n) {
+    std::lock_guard<std::mutex> g(mu);
     auto& f = free_[n];
     if (!f.empty()) {
       double* p = f.back();
       f.pop_back();
+      idle_bytes -= bytes_of(n);
       return p;
     }
     double* p = nullptr;
-    if (hipMalloc((void**)&p, sizeof(double) * (size_t)std::max<int64_t>(n, 1)) != hipSuccess) return nullptr;
-    return p;
+    if (hipMalloc((void**)&p, (size_t)bytes_of(n)) == hipSuccess) return p;
+    (void)hipGetLastError();
+    release_locked();
+    p = nullptr;
+    if (hipMalloc((void**)&p, (size_t)bytes_of(n)) == hipSuccess) return p;
+    (void)hipGetLastError();
+    return nullptr;
   }
   void put(int64_t n, double* p) {
-    if (p) free_[n].push_back(p);
+    if (!p) return;
+    std::lock_guard<std::mutex> g(mu);
+    free_[n].push_back(p);
+    idle_bytes += bytes_of(n);
+    if (idle_bytes > cap_bytes) trim_locked(n);
   }
   void release() {
-    for (auto& kv : free_)
-      for (double* p : kv.second) (void)hipFree(p);
-    free_.clear();
+    std::lock_guard<std::mutex> g(mu);
+    release_locked();
+  }
+  void set_cap(int64_t bytes) {
+    std::lock_guard<std::mutex> g(mu);
+    cap_bytes = std::max<int64_t>(bytes, 0);
+    trim_locked(-1);
+  }
+  int64_t idle() {
+    std::lock_guard<std::mutex> g(mu);
+    return idle_bytes;
   }
 };
 static VecPool g_pool;
@@ -447,6 +501,24 @@ int mpsa_lanczos_batched(mpsa_plan* plan, const double* dpsi0, int64_t stride, i
 
 int mpsa_release_pools(void) {
   g_pool.release();
+  return MPSA_OK;
+}
+
+int mpsa_pool_info(int64_t* idle_bytes, int64_t* cap_bytes) {
+  if (idle_bytes) *idle_bytes = g_pool.idle();
+  if (cap_bytes) {
+    std::lock_guard<std::mutex> g(g_pool.mu);
+    *cap_bytes = g_pool.cap_bytes;
+  }
+  return MPSA_OK;
+}
+
+int mpsa_pool_set_cap(int64_t cap_bytes) {
+  if (cap_bytes < 0) {
+    set_last_error("pool_set_cap: negative cap");
+    return MPSA_EINVAL;
+  }
+  g_pool.set_cap(cap_bytes);
   return MPSA_OK;
 }
 

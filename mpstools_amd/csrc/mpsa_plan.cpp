@@ -922,28 +922,39 @@ static void ensure_device(mpsa_plan* P) {
   P->on_device = 1;
 }
 
-static void ensure_ws(mpsa_plan* P, int nvec) {
+// Zero n bytes of a fresh device buffer with a kernel on the stream the plan is
+// used with (allocations are rounded up to whole doubles), so that the zeroing
+// is ordered before the first kernel that reads the buffer on that stream, also
+// a non-blocking one, and no memset node can end up in a captured graph.
+static void zero_on_stream(void* p, size_t bytes, hipStream_t st) {
+  HIPCHK(launch_fill((double*)p, (int64_t)((bytes + 7) / 8), 0.0, st));
+}
+static size_t round8(size_t bytes) { return (bytes + 7) / 8 * 8; }
+
+static void ensure_ws(mpsa_plan* P, int nvec, hipStream_t st) {
   ensure_device(P);
   if (P->ws_nvec < nvec) {
     if (P->ws) HIPCHK(hipFree(P->ws));
     P->ws = nullptr;
-    const size_t bytes = (size_t)P->ws_per_vec * P->planes * P->esize * (size_t)nvec;
-    if (hipMalloc(&P->ws, bytes) != hipSuccess) {
-      P->ws_nvec = 0;
+    P->ws_nvec = 0;
+    const size_t bytes = round8((size_t)P->ws_per_vec * P->planes * P->esize * (size_t)nvec);
+    if (hipMalloc(&P->ws, std::max<size_t>(bytes, 8)) != hipSuccess) {
+      P->ws = nullptr;
       throw Error(MPSA_ENOMEM, "workspace allocation failed");
     }
-    HIPCHK(hipMemset(P->ws, 0, bytes));  // padding lanes must read as zero
+    zero_on_stream(P->ws, bytes, st);  // padding lanes must read as zero
     P->ws_nvec = nvec;
   }
   if (P->prec != F64 && P->xc_nvec < nvec) {
     if (P->xc) HIPCHK(hipFree(P->xc));
     P->xc = nullptr;
-    const size_t bytes = (size_t)P->xc_per_vec * P->planes * P->esize * (size_t)nvec;
-    if (hipMalloc(&P->xc, bytes) != hipSuccess) {
-      P->xc_nvec = 0;
+    P->xc_nvec = 0;
+    const size_t bytes = round8((size_t)P->xc_per_vec * P->planes * P->esize * (size_t)nvec);
+    if (hipMalloc(&P->xc, std::max<size_t>(bytes, 8)) != hipSuccess) {
+      P->xc = nullptr;
       throw Error(MPSA_ENOMEM, "converted-vector allocation failed");
     }
-    HIPCHK(hipMemset(P->xc, 0, bytes));
+    zero_on_stream(P->xc, bytes, st);
     P->xc_nvec = nvec;
   }
 }
@@ -952,12 +963,17 @@ static void ensure_ws(mpsa_plan* P, int nvec) {
 
 // =================================================================== C ABI
 // Re-number a stage's tiles for tile class cls and refresh its device task
-// list (same number of tasks, so the allocation is reused).
-static void set_stage_class(mpsa_plan* P, Stage& S, int prec, int cls) {
+// list (same number of tasks, so the allocation is reused).  Kernels already
+// queued on `st` read the old list, so the stream drains before the copy, and
+// the copy itself is queued on `st` ahead of every later launch.
+static void set_stage_class(mpsa_plan* P, Stage& S, int prec, int cls, hipStream_t st) {
   if (S.tile_class == cls) return;
+  if (P->on_device && S.d_tasks && !S.tasks.empty()) HIPCHK(hipStreamSynchronize(st));
   assign_tiles(S, prec, cls);
-  if (P->on_device && S.d_tasks && !S.tasks.empty())
-    HIPCHK(hipMemcpy(S.d_tasks, S.tasks.data(), sizeof(GTask) * S.tasks.size(), hipMemcpyHostToDevice));
+  if (P->on_device && S.d_tasks && !S.tasks.empty()) {
+    HIPCHK(hipMemcpyAsync(S.d_tasks, S.tasks.data(), sizeof(GTask) * S.tasks.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+  }
 }
 
 // The grouped GEMM launch of stage 1 (T = L x) or stage 2 (y = T R^T) of an
@@ -1176,12 +1192,18 @@ int mpsa_plan_bind_env(mpsa_plan* P, const double* dL, const double* dR, mpsa_st
     if (P->prec != F64) {
       const size_t lb = (size_t)P->Lc_elems * P->planes * P->esize, rb = (size_t)P->Rc_elems * P->planes * P->esize;
       if (!P->Lc) {
-        if (hipMalloc(&P->Lc, lb) != hipSuccess) throw Error(MPSA_ENOMEM, "env copy");
-        HIPCHK(hipMemset(P->Lc, 0, lb));
+        if (hipMalloc(&P->Lc, round8(lb) + 8) != hipSuccess) {
+          P->Lc = nullptr;
+          throw Error(MPSA_ENOMEM, "env copy");
+        }
+        zero_on_stream(P->Lc, round8(lb), st);
       }
       if (!P->Rc) {
-        if (hipMalloc(&P->Rc, rb) != hipSuccess) throw Error(MPSA_ENOMEM, "env copy");
-        HIPCHK(hipMemset(P->Rc, 0, rb));
+        if (hipMalloc(&P->Rc, round8(rb) + 8) != hipSuccess) {
+          P->Rc = nullptr;
+          throw Error(MPSA_ENOMEM, "env copy");
+        }
+        zero_on_stream(P->Rc, round8(rb), st);
       }
       HIPCHK(launch_repack(P->prec, P->d_rpL, (int)P->rpL.size(), P->rpL_chunks, 1, dL, 0, P->Lc, 0, st, P->Lc_elems));
       HIPCHK(launch_repack(P->prec, P->d_rpR, (int)P->rpR.size(), P->rpR_chunks, 1, dR, 0, P->Rc, 0, st, P->Rc_elems));
@@ -1199,7 +1221,7 @@ int mpsa_heff_apply(mpsa_plan* P, const double* dx, int64_t x_stride, double* dy
     if (nvec > 1 && (x_stride < P->x_total || y_stride < P->y_total))
       throw Error(MPSA_EINVAL, "apply: vector stride smaller than the table extent");
     hipStream_t st = (hipStream_t)stream;
-    ensure_ws(P, nvec);
+    ensure_ws(P, nvec, st);
     hipEvent_t* ev = nullptr;
     if (P->timing) {
       const int slot = std::min(P->nrec, mpsa_plan::kTimedApplies - 1);
@@ -1236,7 +1258,10 @@ int mpsa_plan_set_tile_class(mpsa_plan* P, int stage, int tile_class) {
     const int prec = P->kind == 2 ? F64 : P->prec;
     if (!class_admissible(S.tasks, S.segs, prec, tile_class))
       throw Error(MPSA_EINVAL, "set_tile_class: tile class not available for this stage");
-    set_stage_class(P, S, prec, tile_class);
+    // no stream argument: every queued kernel on the device may read the old
+    // task list, so the whole device drains first
+    if (P->on_device && S.tile_class != tile_class) HIPCHK(hipDeviceSynchronize());
+    set_stage_class(P, S, prec, tile_class, nullptr);
   })
 }
 
@@ -1277,7 +1302,7 @@ int mpsa_plan_autotune(mpsa_plan* P, int nvec, int reps, double* ms, mpsa_stream
       for (int cls = 0; cls < kMaxTileClasses; ++cls) {
         if (ms) ms[(stage - 1) * kMaxTileClasses + cls] = -1.0;
         if (!class_admissible(S.tasks, S.segs, P->prec, cls)) continue;
-        set_stage_class(P, S, P->prec, cls);
+        set_stage_class(P, S, P->prec, cls, st);
         launch_stage(P, stage, x, xs, y, ys, nvec, st);  // warm-up
         HIPCHK(hipEventRecord(e0, st));
         for (int r = 0; r < reps; ++r) launch_stage(P, stage, x, xs, y, ys, nvec, st);
@@ -1293,7 +1318,7 @@ int mpsa_plan_autotune(mpsa_plan* P, int nvec, int reps, double* ms, mpsa_stream
           best_ms = per;
         }
       }
-      set_stage_class(P, S, P->prec, best);
+      set_stage_class(P, S, P->prec, best, st);
     }
     HIPCHK(hipStreamSynchronize(st));
     release();
@@ -1316,7 +1341,7 @@ int mpsa_env_apply(mpsa_plan* P, const double* dBL, const double* dA, const doub
     if (!dBL || !dA || !dout) throw Error(MPSA_EINVAL, "env_apply: null buffer");
     const double* up = dAp ? dAp : dA;
     hipStream_t st = (hipStream_t)stream;
-    ensure_ws(P, 1);
+    ensure_ws(P, 1, st);
     HIPCHK(launch_gemm(F64, 0, P->s1.tr, P->s1.tile_class, P->s1.d_tasks, (int)P->s1.tasks.size(), P->s1.d_segs,
                        P->s1.tiles, 1, dBL, 0, dA, 0, P->ws, 0, st));
     HIPCHK(launch_mix(F64, P->d_mix, (int)P->mix.size(), P->d_terms, P->mix_chunks, 1, P->ws, P->ws, 0, st));

@@ -69,6 +69,8 @@ def lib():
         L.mpsa_plan_info.argtypes = [P, P, P, P, P, P]
         L.mpsa_plan_destroy.argtypes = [P]
         L.mpsa_plan_stage_info.argtypes = [P, P, P, P, P]
+        L.mpsa_pool_info.argtypes = [P, P]
+        L.mpsa_pool_set_cap.argtypes = [C.c_int64]
         L.mpsa_plan_set_timing.argtypes = [P, C.c_int]
         L.mpsa_plan_describe.argtypes = [P, C.c_int, P, C.c_int64, P, P]
         L.mpsa_plan_last_times.argtypes = [P, P]

@@ -51,6 +51,8 @@ Blas& blas() {
       if (st) st(1);
       b->name = std::string(b->name) + " (one thread: MKL threading layer already chosen)";
     }
+    b->set_threads = (set_threads_fn)dlsym(h, "MKL_Set_Num_Threads");
+    if (!b->set_threads) b->set_threads = (set_threads_fn)dlsym(h, "openblas_set_num_threads");
     b->dgemm = (dgemm_fn)dlsym(h, "dgemm_");
     b->zgemm = (zgemm_fn)dlsym(h, "zgemm_");
     b->dstev = (dstev_fn)dlsym(h, "dstev_");
@@ -198,6 +200,21 @@ extern "C" {
 
 const char* orc_last_error() { return g_err.c_str(); }
 const char* orc_blas_name() { return blas().name.c_str(); }
+// Threads of the BLAS library (the CPU baseline's 1-thread and all-core runs);
+// the charge loops take their thread count per call.
+int orc_set_blas_threads(int n) {
+  Blas& b = blas();
+  if (!b.set_threads) return 1;
+  b.set_threads(n < 1 ? 1 : n);
+  return 0;
+}
+// C (m x n) = A (m x k) * B (k x n), column-major, through the loaded BLAS:
+// the product the reference issues per tuple (blas/blasroutines.cpp:837-848).
+int orc_dgemm(int m, int n, int k, const double* A, const double* B, double* C) {
+  ORC_TRY({
+    gemm_raw('n', 'n', m, n, k, 1.0, A, m, B, k, C, m);
+  })
+}
 
 orc_bm* orc_bm_new(int ns, int is_complex) {
   orc_bm* b = new orc_bm();

@@ -117,10 +117,12 @@ typedef void (*zgemm_fn)(const char*, const char*, const int*, const int*, const
                          const int*);
 typedef void (*dstev_fn)(const char*, const int*, double*, double*, double*, const int*, double*,
                          int*);
+typedef void (*set_threads_fn)(int);
 struct Blas {
   dgemm_fn dgemm = nullptr;
   zgemm_fn zgemm = nullptr;
   dstev_fn dstev = nullptr;
+  set_threads_fn set_threads = nullptr;  // MKL_Set_Num_Threads / openblas_set_num_threads
   std::string name = "loops";
 };
 Blas& blas();

@@ -53,6 +53,8 @@ def lib():
         L.orc_lanczos_twosite.argtypes = [P, P, P, P, P, P, C.c_int, P, C.c_int, C.c_int, C.c_double,
                                           C.c_int, C.c_int, C.c_int, P, C.c_int, P, P, P]
         L.orc_tridiag.argtypes = [C.c_int, P, P, P]
+        L.orc_set_blas_threads.argtypes = [C.c_int]
+        L.orc_dgemm.argtypes = [C.c_int, C.c_int, C.c_int, P, P, P]
         _lib = L
     return _lib
 
@@ -68,6 +70,22 @@ def _check(rc):
 
 def blas_name() -> str:
     return lib().orc_blas_name().decode()
+
+
+def set_blas_threads(n: int) -> bool:
+    """Threads of the loaded BLAS (MKL or OpenBLAS); False when it has no such control."""
+    return lib().orc_set_blas_threads(int(n)) == 0
+
+
+def dgemm(A, B):
+    """C = A B (column-major float64) through the loaded BLAS."""
+    A = np.asfortranarray(A, dtype=np.float64)
+    B = np.asfortranarray(B, dtype=np.float64)
+    m, k = A.shape
+    n = B.shape[1]
+    Cm = np.zeros((m, n), order="F")
+    _check(lib().orc_dgemm(m, n, k, _p(A), _p(B), _p(Cm)))
+    return Cm
 
 
 class BM:

@@ -226,6 +226,38 @@ int main() {
   oracle::BlockMatrix<2, double> oy;
   oracle::contractTwosite(oy, oL, oR, to_oracle<2>(psi), to_oracle(heis(c, N)), to_oracle(heis(c + 1, N)), oitk, oitk, 1);
   report("contractTwosite", rel<2>(y, oy), 1e-12);
+  // plan cache: a reference-style Lanczos loop (src/lanzcos.hpp:197) calls
+  // contractTwosite on fixed BL, BR with a new vector each step; the plan and
+  // the device environments are built once, and a changed R is re-uploaded
+  {
+    const auto s0 = plan_cache_stats();
+    BlockMatrix<2, AbKeyType, Real> x = psi, yy;
+    double worst_c = 0;
+    for (int it = 0; it < 5; ++it) {
+      for (auto& kv : x)
+        for (size_t i = 0; i < kv.second.size1() * kv.second.size2(); ++i) kv.second.data()[i] *= -1.25;
+      contractTwosite(yy, L, R, x, heis(c, N), heis(c + 1, N), itks);
+      oracle::BlockMatrix<2, double> oyy;
+      oracle::contractTwosite(oyy, oL, oR, to_oracle<2>(x), to_oracle(heis(c, N)), to_oracle(heis(c + 1, N)), oitk,
+                              oitk, 1);
+      worst_c = std::max(worst_c, rel<2>(yy, oyy));
+    }
+    const auto s1 = plan_cache_stats();
+    report("contractTwosite cached loop", worst_c, 1e-12);
+    report("plan cache: no plan built in the loop", (double)(s1.builds - s0.builds), 0.0);
+    report("plan cache: every loop call served by the cache", s1.hits - s0.hits >= 5 ? 0.0 : 1.0, 0.0);
+    report("plan cache: environments not re-uploaded", (double)(s1.env_uploads - s0.env_uploads), 0.0);
+    auto R2 = R;
+    R2.begin()->second.data()[0] += 0.5;
+    contractTwosite(yy, L, R2, x, heis(c, N), heis(c + 1, N), itks);
+    oracle::BlockMatrix<2, double> oyy;
+    auto oR2 = to_oracle<1>(R2);
+    oracle::contractTwosite(oyy, oL, oR2, to_oracle<2>(x), to_oracle(heis(c, N)), to_oracle(heis(c + 1, N)), oitk, oitk,
+                            1);
+    const auto s2 = plan_cache_stats();
+    report("plan cache: changed R uploaded once", (double)(s2.env_uploads - s1.env_uploads - 1), 0.0);
+    report("plan cache: changed R result", rel<2>(yy, oyy), 1e-12);
+  }
   // one-site
   {
     auto Rc1 = addContractionLayer(R, mps[(size_t)c + 1], heis(c + 1, N), itk, -1);

@@ -186,6 +186,8 @@ hipError_t hipMemset(void* p, int v, size_t n) {
   return hipSuccess;
 }
 hipError_t hipStreamSynchronize(hipStream_t) { return hipSuccess; }
+hipError_t hipDeviceSynchronize(void) { return hipSuccess; }
+hipError_t hipGetLastError(void) { return hipSuccess; }
 hipError_t hipStreamCreateWithFlags(hipStream_t* s, unsigned) {
   *s = reinterpret_cast<hipStream_t>(new Stream());
   return hipSuccess;

@@ -413,3 +413,65 @@ def test_emulated_cpp_dropin(lib):
     assert r.returncode == 0 and "ALL PASS" in r.stdout, r.stdout[-2000:] + r.stderr[-2000:]
     assert "complex128" in r.stdout
     assert "EMU VIOLATION" not in r.stderr, r.stderr[-2000:]
+
+
+def test_emulated_vector_pool_stays_under_its_cap(lib, oracle):
+    """Lanczos runs on bonds of three vector lengths with a small pool cap: the
+    idle bytes the device vector pool keeps stay under the cap after every run
+    (vectors of other lengths are freed first), and each run's lowest Ritz
+    value is the dense H_eff's ground-state energy."""
+    L = native.lib()
+    L.mpsa_pool_info.argtypes = [C.c_void_p, C.c_void_p]
+    L.mpsa_pool_set_cap.argtypes = [C.c_int64]
+    native._check(L.mpsa_release_pools())
+    cap = 48 * 1024
+    native._check(L.mpsa_pool_set_cap(cap))
+    try:
+        for n, seed in [(8, 71), (10, 72), (12, 73), (8, 74)]:
+            P = problems.build(oracle, "xxz", n, n // 2, 16, seed)
+            c, mpo, itk = P["c"], P["mpo"], P["itk"]
+            Lb, Rb, psi = P["Ls"][c - 1], P["Rs"][c + 2], P["psi"]
+            Lt, Rt, xt = native.Table.from_blocks(Lb), native.Table.from_blocks(Rb), native.Table.from_blocks(psi)
+            vt = xt.union(native.heff2_outkeys(Lt, Rt, xt, mpo[c], mpo[c + 1], itk[c], itk[c + 1]))
+            plan = native.HeffPlan(Lt, Rt, vt, vt, (mpo[c], mpo[c + 1]), (itk[c], itk[c + 1]), native.F64)
+            plan.bind_env(DevBuf(lib, Lt.pack(Lb)), DevBuf(lib, Rt.pack(Rb)))
+            x0, st = DevBuf(lib, vt.pack(psi)), DevBuf(lib, np.zeros(vt.total))
+            E = np.zeros(302)
+            nit, nr = np.zeros(1, np.int32), np.zeros(1, np.int32)
+            native._check(L.mpsa_lanczos(plan.h, x0.p, st.p, 300, 1e-12, 1, 0, native._ptr(E), 302, native._ptr(nit),
+                                         native._ptr(nr), None))
+            idle = np.zeros(1, np.int64)
+            native._check(L.mpsa_pool_info(native._ptr(idle), None))
+            assert 0 < idle[0] <= cap
+            H = problems.dense_heff(oracle, P, vt)
+            assert abs(E[0] - np.linalg.eigvalsh(H)[0]) <= 1e-9 * max(1.0, abs(E[0]))
+            del plan
+    finally:
+        native._check(L.mpsa_pool_set_cap(8 << 30))
+
+
+def test_emulated_autotune_on_a_non_default_stream(lib, oracle):
+    """Tile-class tuning and applies issued on a stream of the caller's own
+    (non-blocking, as torch's side streams are): the task-list rewrites and the
+    workspace zeroing go on that stream; results equal the default-stream apply."""
+    P = problems.build(oracle, "xxz", 12, 6, 20, 77)
+    c, mpo, itk = P["c"], P["mpo"], P["itk"]
+    Lb, Rb, psi = P["Ls"][c - 1], P["Rs"][c + 2], P["psi"]
+    Lt, Rt, xt = native.Table.from_blocks(Lb), native.Table.from_blocks(Rb), native.Table.from_blocks(psi)
+    vt = xt.union(native.heff2_outkeys(Lt, Rt, xt, mpo[c], mpo[c + 1], itk[c], itk[c + 1]))
+    L = native.lib()
+    s = C.c_void_p()
+    assert lib.hipStreamCreateWithFlags(C.byref(s), 1) == 0
+    ref = native.HeffPlan(Lt, Rt, vt, vt, (mpo[c], mpo[c + 1]), (itk[c], itk[c + 1]), native.F64)
+    dL, dR, x = DevBuf(lib, Lt.pack(Lb)), DevBuf(lib, Rt.pack(Rb)), DevBuf(lib, vt.pack(psi))
+    ref.bind_env(dL, dR)
+    y0 = DevBuf(lib, np.zeros(vt.total))
+    ref.apply(x, y0)
+    plan = native.HeffPlan(Lt, Rt, vt, vt, (mpo[c], mpo[c + 1]), (itk[c], itk[c + 1]), native.F64)
+    native._check(L.mpsa_plan_bind_env(plan.h, dL.p, dR.p, s))
+    ms = np.zeros(2 * native.TILE_CLASSES)
+    native._check(L.mpsa_plan_autotune(plan.h, 2, 1, native._ptr(ms), s))
+    y1 = DevBuf(lib, np.zeros(vt.total))
+    native._check(L.mpsa_heff_apply(plan.h, x.p, 0, y1.p, 0, 1, s))
+    assert np.array_equal(y1.host(), y0.host())
+    del plan, ref
