@@ -358,6 +358,50 @@ int mpsa_chain_env_table(const mpsa_chain* chain, int side, int index, int64_t c
                          uint32_t* p, int64_t* rows, int64_t* cols, int64_t* offsets);
 int mpsa_chain_destroy(mpsa_chain* chain);
 
+/* ---- Chebyshev moments with variational MPS compression ----
+ * ChebychevEngine::simulateSSCompress (src/container.hpp:2343-2470) and the
+ * MPS functions under it, for nvec start vectors at once.  The engine holds
+ * eight MPS slots; slot 0 is the start state given at creation (nvec vectors,
+ * each nsites ns = 1 tables and device buffers: sites[v * nsites + i],
+ * dA[v * nsites + i]; all real FP64 or all complex128 planar), sharing the
+ * MPO W (nsites real tensors) and i_to_key (concatenated, d[i] per site).
+ * h1_precision is the arithmetic of the one-site contraction inside the
+ * compression sweeps (MPSA_F64, or MPSA_F32 for complex64 / float); every
+ * other step is FP64.  Results are per vector; complex results are (re, im).
+ *   apply_mpo:  slot dst = applyMPO(W, slot src)        (MPSfunctions.hpp:2088-2345)
+ *   overlap:    computeOverlap(a, b) = <b|a>            (MPSfunctions.hpp:2066-2084)
+ *   axpby:      slot dst = xa a + xb b, the MPS sum of MPSBase::operator+=
+ *               with the scale on the first site        (MPSTypes.hpp:223-300)
+ *   compress_single_site: slot dst ~ W |slot src>, conv per vector
+ *                                                       (MPSfunctions.hpp:2670-2789)
+ *   compress:   slot compressed in place, conv per vector (MPSfunctions.hpp:2353-2510)
+ *   moments:    mu_n = <t_n|t_0> for n < nmom (nvec x nmom x 2), the
+ *               reconstructed doubled moments (nvec x (2 nmom - 1) x 2), and
+ *               the two convergence measures per moment (nvec x nmom); slot 1
+ *               holds the last compressed t afterwards.  opi = opf = identity.
+ *   site:       one vector's site tensor of a slot (its own charges, 64-element
+ *               aligned offsets, planar when complex); NULL arrays query sizes.
+ * The pair sweeps, canonicalisation (prepareSite with a batched Jacobi SVD per
+ * charge sector), truncate and dropDangling follow MPSfunctions.hpp:21-269,
+ * 1444-1484, 1705-1727.  One engine is one stream; calls on it are serial. */
+typedef struct mpsa_cheb mpsa_cheb;
+int mpsa_cheb_create(mpsa_cheb** out, int nsites, int nvec, const mpsa_blocks* sites, const double* const* dA,
+                     const mpsa_mpo* W, const uint64_t* itokey, const int* d, int h1_precision, mpsa_stream stream);
+int mpsa_cheb_copy(mpsa_cheb* c, int dst, int src);
+int mpsa_cheb_apply_mpo(mpsa_cheb* c, int dst, int src);
+int mpsa_cheb_overlap(mpsa_cheb* c, int a, int b, double* out);
+int mpsa_cheb_axpby(mpsa_cheb* c, int dst, double xa, int a, double xb, int b);
+int mpsa_cheb_compress_single_site(mpsa_cheb* c, int dst, int src, unsigned chimax, int maxstep, double delta,
+                                   double* conv);
+int mpsa_cheb_compress(mpsa_cheb* c, int slot, unsigned chimax, int maxstep, double delta, double* conv);
+int mpsa_cheb_moments(mpsa_cheb* c, int nmom, unsigned chimax, int ncmax, double delta, double* moments,
+                      double* recons, double* conv1, double* conv2, double* seconds);
+int mpsa_cheb_site(mpsa_cheb* c, int slot, int vec, int site, int64_t* n_blocks, int64_t* len, uint64_t* q,
+                   uint32_t* p, int64_t* rows, int64_t* cols, int64_t* offsets, double* dst);
+/* Plans built so far and device bytes held by the engine's buffer pool. */
+int mpsa_cheb_stats(const mpsa_cheb* c, int64_t* plans_built, int64_t* pool_bytes);
+int mpsa_cheb_destroy(mpsa_cheb* c);
+
 /* ---- Lanczos vector algebra over packed block vectors (BlockTensor.hpp:443-544) ----
  * X*Y = sum conj(Y) X (src/BlockTensor.hpp:520-544), computed in FP64 with a
  * fixed-order two-level reduction; result written to host *out (re, im). */

@@ -1,0 +1,1368 @@
+// mpsa_cheb.cpp -- Chebyshev moments with variational MPS compression on the
+// device: the reference's ChebychevEngine::simulateSSCompress
+// (src/container.hpp:2343-2470) and the MPS functions under it,
+//   applyMPO            src/MPSfunctions.hpp:2088-2345
+//   computeOverlap      src/MPSfunctions.hpp:2066-2084
+//   prepareSite         src/MPSTypes.hpp:586-620 (prepareMatrix/prepareBlock,
+//                       src/MPSfunctions.hpp:21-269)
+//   truncate            src/MPSfunctions.hpp:1705-1727 with
+//                       DiagBlockMatrix::truncate src/BlockTensor.hpp:85-136
+//   dropDangling        src/MPSfunctions.hpp:1444-1484
+//   compress            src/MPSfunctions.hpp:2353-2510
+//   compressSingleSite  src/MPSfunctions.hpp:2670-2789
+//   MPS *=, +=, -=      src/MPSTypes.hpp:223-300 (outersum, fixSizes)
+//
+// Batching: the nvec start vectors share one set of block tables.  Vector v's
+// charges carry v in bits 40..55 of the packed AbKeyType word (charges and
+// i_to_key values of the models stay far below 2^40), so blocks of different
+// vectors never meet in a contraction: every H_eff, environment, SVD and copy
+// of a step covers all vectors in one grouped launch, while truncation,
+// overlaps and the convergence tests are evaluated per vector.
+//
+// Device work per step: the contractions go through the plans of
+// mpsa_plan.cpp (one-site H_eff and environment updates with a distinct upper
+// tensor, cached by block structure); canonicalisation gathers each charge
+// sector into a dense matrix (copy2d), factors all sectors with the batched
+// Jacobi SVD (mpsa_factor.hip) and multiplies S V^dagger or U S into the
+// neighbour with one grouped GEMM.  Host work is table bookkeeping only.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <list>
+#include <map>
+#include <memory>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "../../include/mpstools_amd.h"
+#include "mpsa_host.h"
+#include "mpsa_internal.h"
+
+using namespace mpsa;
+using namespace mpsa::host;
+
+namespace {
+
+constexpr int kNsShift = 40;
+inline uint64_t ns_of(uint64_t q) { return q >> kNsShift; }
+typedef std::complex<double> cplx;
+typedef std::array<uint64_t, 3> Key;  // (left charge, right charge, physical index)
+
+// ------------------------------------------------------------ device memory
+// Buffers are recycled within one stream: a buffer handed back is reused only
+// by work queued later on the same stream.
+struct Pool {
+  std::multimap<size_t, void*> free_;
+  std::vector<std::pair<void*, size_t>> all;
+  size_t idle = 0;
+  void* get(size_t bytes) {
+    bytes = std::max<size_t>((bytes + 255) / 256 * 256, 256);
+    auto it = free_.lower_bound(bytes);
+    if (it != free_.end() && it->first <= 2 * bytes) {
+      void* p = it->second;
+      idle -= it->first;
+      free_.erase(it);
+      return p;
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      release_idle();
+      if (hipMalloc(&p, bytes) != hipSuccess) throw HErr(MPSA_ENOMEM, "cheb: device allocation failed");
+    }
+    all.push_back({p, bytes});
+    return p;
+  }
+  size_t size_of(void* p) const {
+    for (auto& a : all)
+      if (a.first == p) return a.second;
+    return 0;
+  }
+  void put(void* p) {
+    const size_t b = size_of(p);
+    free_.insert({b, p});
+    idle += b;
+  }
+  void release_idle() {
+    for (auto& kv : free_) {
+      (void)hipFree(kv.second);
+      for (size_t i = 0; i < all.size(); ++i)
+        if (all[i].first == kv.second) {
+          all.erase(all.begin() + (long)i);
+          break;
+        }
+    }
+    free_.clear();
+    idle = 0;
+  }
+  ~Pool() {
+    for (auto& a : all) (void)hipFree(a.first);
+  }
+};
+
+struct Buf {
+  Pool* pool = nullptr;
+  double* p = nullptr;
+  int64_t n = 0;
+  Buf() {}
+  Buf(Pool& pl, int64_t nd, hipStream_t st, bool zero = true) : pool(&pl), n(std::max<int64_t>(nd, 1)) {
+    p = (double*)pl.get(sizeof(double) * (size_t)n);
+    if (zero) hchk(launch_fill(p, n, 0.0, st), "cheb: zero");
+  }
+  Buf(Buf&& o) noexcept : pool(o.pool), p(o.p), n(o.n) { o.p = nullptr; }
+  Buf& operator=(Buf&& o) noexcept {
+    if (this != &o) {
+      reset();
+      pool = o.pool;
+      p = o.p;
+      n = o.n;
+      o.p = nullptr;
+    }
+    return *this;
+  }
+  Buf(const Buf&) = delete;
+  Buf& operator=(const Buf&) = delete;
+  void reset() {
+    if (p && pool) pool->put(p);
+    p = nullptr;
+  }
+  ~Buf() { reset(); }
+};
+
+// An MPS of nvec packed vectors: tables, device buffers, lambdas per bond.
+struct DMps {
+  std::vector<HTab> A;
+  std::vector<Buf> d;
+  std::vector<std::map<uint64_t, std::vector<double>>> lam;
+  bool valid() const { return !A.empty(); }
+};
+
+HTab table_from(const std::map<Key, std::pair<int64_t, int64_t>>& m, int cplx_) {
+  HTab t;
+  t.ns = 1;
+  t.cplx = cplx_;
+  for (auto& kv : m) {
+    t.q.push_back(kv.first[0]);
+    t.q.push_back(kv.first[1]);
+    t.p.push_back((uint32_t)kv.first[2]);
+    t.rows.push_back(kv.second.first);
+    t.cols.push_back(kv.second.second);
+  }
+  t.pack();
+  return t;
+}
+
+std::string sig_tab(const HTab& t) {
+  std::string s;
+  s.append((const char*)t.q.data(), t.q.size() * 8);
+  s.append((const char*)t.p.data(), t.p.size() * 4);
+  s.append((const char*)t.rows.data(), t.rows.size() * 8);
+  s.append((const char*)t.cols.data(), t.cols.size() * 8);
+  s += "|" + std::to_string(t.n()) + "," + std::to_string(t.cplx) + "|";
+  return s;
+}
+std::string sig_mpo(const HMpo& w) {
+  std::string s = std::to_string(w.Dl) + "," + std::to_string(w.Dr) + ":";
+  s.append((const char*)w.wl.data(), w.wl.size() * 4);
+  s.append((const char*)w.wr.data(), w.wr.size() * 4);
+  s.append((const char*)w.out.data(), w.out.size() * 4);
+  s.append((const char*)w.in.data(), w.in.size() * 4);
+  s.append((const char*)w.val.data(), w.val.size() * 8);
+  return s + "|" + std::to_string(w.val.size()) + "|";
+}
+
+}  // namespace
+
+struct mpsa_cheb {
+  hipStream_t st = nullptr;
+  int N = 0, nvec = 0, planes = 1, h1prec = MPSA_F64;
+  std::vector<HMpo> W, Id;
+  std::vector<std::vector<uint64_t>> itk;
+  Pool pool;
+  std::vector<std::shared_ptr<void>> keep;  // host copies of uploaded descriptors, until the next sync
+  std::list<std::pair<std::string, mpsa_plan*>> plans;
+  static constexpr size_t kPlanCap = 96;
+  std::vector<DMps> slot;
+  int64_t plans_built = 0;
+  ~mpsa_cheb() {
+    if (st) (void)hipStreamSynchronize(st);
+    for (auto& kv : plans) mpsa_plan_destroy(kv.second);
+    slot.clear();
+  }
+  void sync() {
+    hchk(hipStreamSynchronize(st), "cheb: sync");
+    keep.clear();
+  }
+  template <typename T>
+  Buf upload(const std::vector<T>& v) {
+    Buf b(pool, (int64_t)((sizeof(T) * std::max<size_t>(v.size(), 1) + 7) / 8), st, false);
+    if (!v.empty()) {
+      auto h = std::make_shared<std::vector<T>>(v);
+      hchk(hipMemcpyAsync(b.p, h->data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st), "cheb: upload");
+      keep.push_back(h);
+    }
+    return b;
+  }
+  mpsa_plan* find_plan(const std::string& k) {
+    for (auto it = plans.begin(); it != plans.end(); ++it)
+      if (it->first == k) {
+        plans.splice(plans.begin(), plans, it);
+        return it->second;
+      }
+    return nullptr;
+  }
+  void put_plan(const std::string& k, mpsa_plan* p) {
+    plans.emplace_front(k, p);
+    ++plans_built;
+    while (plans.size() > kPlanCap) {
+      sync();  // an evicted plan's last launch must have finished
+      mpsa_plan_destroy(plans.back().second);
+      plans.pop_back();
+    }
+  }
+};
+
+namespace {
+
+typedef mpsa_cheb Eng;
+
+// ------------------------------------------------------------ copies, GEMMs
+void add_copy(std::vector<Copy2D>& v, const Eng& E, int64_t src_off, int64_t src_plane, int64_t dst_off,
+              int64_t dst_plane, int64_t rows, int64_t cols, int64_t lds, int64_t ldd, double scale = 1.0,
+              int64_t rs = -1, int64_t cs = -1, int trans = 0) {
+  if (rows <= 0 || cols <= 0) return;
+  for (int pl = 0; pl < E.planes; ++pl) {
+    Copy2D t{};
+    t.src_off = src_off + pl * src_plane;
+    t.dst_off = dst_off + pl * dst_plane;
+    t.rows = (int32_t)rows;
+    t.cols = (int32_t)cols;
+    t.lds = (int32_t)std::max<int64_t>(lds, 1);
+    t.ldd = (int32_t)std::max<int64_t>(ldd, 1);
+    t.rs_off = rs;
+    t.cs_off = cs;
+    t.scale = scale;
+    t.trans = trans;
+    v.push_back(t);
+  }
+}
+
+void run_copies(Eng& E, const std::vector<Copy2D>& v, const double* src, double* dst, const double* scales) {
+  if (v.empty()) return;
+  int64_t mx = 0;
+  for (auto& t : v) mx = std::max<int64_t>(mx, (int64_t)t.rows * t.cols);
+  Buf d = E.upload(v);
+  const int64_t kMaxY = 65535;  // one task per grid row of copy2d_kernel
+  for (int64_t first = 0; first < (int64_t)v.size(); first += kMaxY) {
+    const int cnt = (int)std::min<int64_t>(kMaxY, (int64_t)v.size() - first);
+    hchk(launch_copy2d((const Copy2D*)d.p + first, cnt, mx, src, dst, scales, E.st), "cheb: copy");
+  }
+}
+
+// C (m x n, ld m) = A (m x k, ld lda) * B (k x n, ld ldb), planar complex when
+// the engine is complex: Re C = Ar Br - Ai Bi, Im C = Ar Bi + Ai Br.
+struct GemmList {
+  std::vector<GTask> tasks;
+  std::vector<GSeg> segs;
+  void add(const Eng& E, int64_t c_off, int64_t c_plane, int64_t m, int64_t n, int64_t k, int64_t a_off,
+           int64_t a_plane, int64_t lda, int64_t b_off, int64_t b_plane, int64_t ldb) {
+    if (m <= 0 || n <= 0) return;
+    auto seg = [&](int64_t ao, int64_t bo, double al) {
+      GSeg g{};
+      g.a_off = ao;
+      g.b_off = bo;
+      g.k = (int32_t)k;
+      g.lda = (int32_t)std::max<int64_t>(lda, 1);
+      g.ldb = (int32_t)std::max<int64_t>(ldb, 1);
+      g.alpha = al;
+      segs.push_back(g);
+    };
+    for (int pl = 0; pl < E.planes; ++pl) {
+      GTask t{};
+      t.c_off = c_off + pl * c_plane;
+      t.m = (int32_t)m;
+      t.n = (int32_t)n;
+      t.seg0 = (int32_t)segs.size();
+      if (E.planes == 1) {
+        seg(a_off, b_off, 1.0);
+      } else if (pl == 0) {
+        seg(a_off, b_off, 1.0);
+        seg(a_off + a_plane, b_off + b_plane, -1.0);
+      } else {
+        seg(a_off, b_off + b_plane, 1.0);
+        seg(a_off + a_plane, b_off, 1.0);
+      }
+      t.nseg = (int32_t)segs.size() - t.seg0;
+      tasks.push_back(t);
+    }
+  }
+  void run(Eng& E, const double* A, const double* B, double* C) {
+    if (tasks.empty()) return;
+    int tiles = 0, cls = 0;
+    finalize_gemm_list(tasks, segs, F64, &tiles, &cls);
+    if (tasks.empty() || tiles <= 0) return;
+    Buf dt = E.upload(tasks), ds = E.upload(segs);
+    hchk(launch_gemm(F64, 0, 0, cls, (const GTask*)dt.p, (int)tasks.size(), (const GSeg*)ds.p, tiles, 1, A, 0, B, 0,
+                     (void*)C, 0, E.st),
+         "cheb: grouped GEMM");
+  }
+};
+
+// ------------------------------------------------------------ basic tensors
+DMps new_mps(Eng& E) {
+  DMps M;
+  M.A.resize((size_t)E.N);
+  M.d.resize((size_t)E.N);
+  M.lam.resize((size_t)std::max(E.N - 1, 0));
+  return M;
+}
+
+DMps copy_mps(Eng& E, const DMps& a) {
+  DMps M = new_mps(E);
+  for (int s = 0; s < E.N; ++s) {
+    M.A[(size_t)s] = a.A[(size_t)s];
+    M.d[(size_t)s] = Buf(E.pool, a.A[(size_t)s].len(), E.st, false);
+    if (a.A[(size_t)s].len() > 0)
+      hchk(hipMemcpyAsync(M.d[(size_t)s].p, a.d[(size_t)s].p, sizeof(double) * (size_t)a.A[(size_t)s].len(),
+                          hipMemcpyDeviceToDevice, E.st),
+           "cheb: copy");
+  }
+  M.lam = a.lam;
+  return M;
+}
+
+Key key_of(const HTab& t, int64_t i) { return Key{t.q[2 * i], t.q[2 * i + 1], (uint64_t)t.p[(size_t)i]}; }
+
+// getLMin per vector: a 1 x 1 identity at each vector's left boundary charge
+HTab lmin_tab_of(const DMps& M, int cplx_) {
+  std::map<Key, std::pair<int64_t, int64_t>> m;
+  for (int64_t i = 0; i < M.A[0].n(); ++i) m[Key{M.A[0].q[2 * i], M.A[0].q[2 * i], 0}] = {1, 1};
+  return table_from(m, cplx_);
+}
+// getRMax: identity per right charge of the last site
+HTab rmax_tab_of(const DMps& M, int cplx_) {
+  const HTab& a = M.A.back();
+  std::map<Key, std::pair<int64_t, int64_t>> m;
+  for (int64_t i = 0; i < a.n(); ++i) m[Key{a.q[2 * i + 1], a.q[2 * i + 1], 0}] = {a.cols[(size_t)i], a.cols[(size_t)i]};
+  return table_from(m, cplx_);
+}
+Buf identity_buf(Eng& E, const HTab& t) {
+  Buf b(E.pool, t.len(), E.st);
+  for (int64_t k = 0; k < t.n(); ++k)
+    hchk(launch_fill_strided(b.p + t.off[(size_t)k], t.rows[(size_t)k], t.rows[(size_t)k] + 1, 1.0, E.st),
+         "cheb: identity");
+  return b;
+}
+
+// ------------------------------------------------------------ contractions
+// E' = addContractionLayer(E, A, W, dir) with the upper (conjugated) tensor Ap
+void env_layer(Eng& E, const HTab& Et, const double* dE, const HTab& At, const double* dA, const HTab& Apt,
+               const double* dAp, const HMpo& W, const std::vector<uint64_t>& itk, int dir, HTab& out_t, Buf& out_b) {
+  mpsa_blocks be = Et.view(), ba = At.view(), bp = Apt.view();
+  mpsa_mpo mw = W.view();
+  int64_t n = 0;
+  chk(mpsa_env_outkeys(&be, &ba, &bp, &mw, itk.data(), (int)itk.size(), dir, 0, &n, nullptr, nullptr, nullptr,
+                       nullptr));
+  std::vector<uint64_t> q((size_t)2 * n + 2);
+  std::vector<uint32_t> p((size_t)n + 1);
+  std::vector<int64_t> r((size_t)n + 1), c((size_t)n + 1);
+  if (n)
+    chk(mpsa_env_outkeys(&be, &ba, &bp, &mw, itk.data(), (int)itk.size(), dir, n, &n, q.data(), p.data(), r.data(),
+                         c.data()));
+  out_t = from_query(n, 1, q, p, r, c);
+  out_t.cplx = Et.cplx;
+  const std::string key = "E" + std::to_string(dir > 0) + sig_tab(Et) + sig_tab(At) + sig_tab(Apt) + sig_mpo(W);
+  mpsa_plan* pl = E.find_plan(key);
+  if (!pl) {
+    mpsa_blocks bo = out_t.view();
+    chk(mpsa_env_plan(&pl, &be, &ba, &bp, &mw, itk.data(), (int)itk.size(), dir, &bo, MPSA_F64));
+    E.put_plan(key, pl);
+  }
+  out_b = Buf(E.pool, out_t.len(), E.st);
+  if (n) chk(mpsa_env_apply(pl, dE, dA, dAp, out_b.p, (mpsa_stream)E.st));
+}
+
+// contractSingle(L, A, W, R): the one-site H_eff applied to A; the output table
+// is the produced key set in key order
+void heff1(Eng& E, const HTab& Lt, const double* dL, const HTab& At, const double* dA, const HTab& Rt,
+           const double* dR, const HMpo& W, const std::vector<uint64_t>& itk, HTab& out_t, Buf& out_b) {
+  mpsa_blocks bl = Lt.view(), br = Rt.view(), ba = At.view();
+  mpsa_mpo mw = W.view();
+  int64_t n = 0;
+  chk(mpsa_heff1_outkeys(&bl, &br, &ba, &mw, itk.data(), (int)itk.size(), 0, &n, nullptr, nullptr, nullptr,
+                         nullptr));
+  std::vector<uint64_t> q((size_t)2 * n + 2);
+  std::vector<uint32_t> p((size_t)n + 1);
+  std::vector<int64_t> r((size_t)n + 1), c((size_t)n + 1);
+  if (n)
+    chk(mpsa_heff1_outkeys(&bl, &br, &ba, &mw, itk.data(), (int)itk.size(), n, &n, q.data(), p.data(), r.data(),
+                           c.data()));
+  out_t = from_query(n, 1, q, p, r, c);
+  out_t.cplx = At.cplx;
+  out_b = Buf(E.pool, out_t.len(), E.st);
+  if (!n) return;
+  const std::string key = "H" + std::to_string(E.h1prec) + sig_tab(Lt) + sig_tab(Rt) + sig_tab(At) + sig_mpo(W);
+  mpsa_plan* pl = E.find_plan(key);
+  if (!pl) {
+    mpsa_blocks by = out_t.view();
+    chk(mpsa_heff1_plan(&pl, &bl, &br, &ba, &by, &mw, itk.data(), (int)itk.size(), E.h1prec));
+    E.put_plan(key, pl);
+  }
+  chk(mpsa_plan_bind_env(pl, dL, dR, (mpsa_stream)E.st));
+  chk(mpsa_heff_apply(pl, dA, 0, out_b.p, 0, 1, (mpsa_stream)E.st));
+}
+
+// computeOverlap(a, b) per vector: <b|a> (the upper tensor b is conjugated)
+std::vector<cplx> overlap(Eng& E, const DMps& a, const DMps& b) {
+  const int cp = E.planes == 2;
+  HTab Lt = lmin_tab_of(a, cp);
+  Buf Lb = identity_buf(E, Lt);
+  for (int s = 0; s < E.N; ++s) {
+    HTab nt;
+    Buf nb;
+    env_layer(E, Lt, Lb.p, a.A[(size_t)s], a.d[(size_t)s].p, b.A[(size_t)s], b.d[(size_t)s].p, E.Id[(size_t)s],
+              E.itk[(size_t)s], 1, nt, nb);
+    Lt = nt;
+    Lb = std::move(nb);
+  }
+  std::vector<double> h((size_t)std::max<int64_t>(Lt.len(), 1));
+  if (Lt.len() > 0)
+    hchk(hipMemcpyAsync(h.data(), Lb.p, sizeof(double) * (size_t)Lt.len(), hipMemcpyDeviceToHost, E.st),
+         "cheb: overlap");
+  E.sync();
+  std::vector<cplx> out((size_t)E.nvec, cplx(0.0, 0.0));
+  for (int64_t i = 0; i < Lt.n(); ++i) {
+    const uint64_t v = ns_of(Lt.q[2 * i]);
+    if (v >= (uint64_t)E.nvec || Lt.rows[(size_t)i] < 1 || Lt.cols[(size_t)i] < 1) continue;
+    const int64_t o = Lt.off[(size_t)i];
+    out[(size_t)v] = cplx(h[(size_t)o], cp ? h[(size_t)(o + Lt.total)] : 0.0);
+  }
+  return out;
+}
+
+// ------------------------------------------------------------ prepareSite
+// prepareSite(mps, site, "l" or "r", multiply = true) for every sector of the
+// site at once; lambdas kept on the host when asked (for a later truncate).
+void prepare_site(Eng& E, DMps& M, int site, bool left, bool keep_lambda) {
+  const int sa = left ? site : site - 1, sb = left ? site + 1 : site;  // (A, B) = (site, site+1) or (site-1, site)
+  const HTab &At = M.A[(size_t)sa], &Bt = M.A[(size_t)sb];
+  const HTab& X = left ? At : Bt;  // the tensor factorised
+  struct Sec {
+    std::vector<int64_t> blk;  // blocks of X in key order
+    std::vector<int64_t> off;  // their row (l) or column (r) offsets in the sector matrix
+    int64_t m = 0, n = 0, k = 0, mo = 0, uo = 0, vo = 0, so = 0, wo = 0;
+  };
+  std::map<uint64_t, Sec> secs;
+  for (int64_t i = 0; i < X.n(); ++i) {
+    const uint64_t q = left ? X.q[2 * i + 1] : X.q[2 * i];
+    Sec& S = secs[q];
+    S.blk.push_back(i);
+  }
+  int64_t tm = 0, tu = 0, tv = 0, ts = 0, tw = 0;
+  for (auto& kv : secs) {
+    Sec& S = kv.second;
+    for (int64_t i : S.blk) {
+      if (left) {
+        S.off.push_back(S.m);
+        S.m += X.rows[(size_t)i];
+        S.n = X.cols[(size_t)i];
+      } else {
+        S.off.push_back(S.n);
+        S.n += X.cols[(size_t)i];
+        S.m = X.rows[(size_t)i];
+      }
+    }
+    S.k = std::min(S.m, S.n);
+    if (std::min(S.m, S.n) > kSvdMaxR) throw HErr(MPSA_EINVAL, "cheb: charge sector too large for the batched SVD");
+    S.mo = tm;
+    tm += align64(S.m * S.n);
+    S.uo = tu;
+    tu += align64(S.m * S.k);
+    S.vo = tv;
+    tv += align64(S.k * S.n);
+    S.so = ts;
+    ts += S.k;
+    S.wo = tw;
+    tw += align64(std::max(S.m, S.n) * S.k + S.k * S.k);
+  }
+  const int pl = E.planes;
+  Buf Mb(E.pool, pl * tm, E.st), Ub(E.pool, pl * tu, E.st), Vb(E.pool, pl * tv, E.st), Sb(E.pool, ts, E.st),
+      Wb(E.pool, pl * tw, E.st, false);
+  // gather
+  std::vector<Copy2D> g;
+  const int64_t xoff_plane = X.total;
+  const double* dX = left ? M.d[(size_t)sa].p : M.d[(size_t)sb].p;
+  for (auto& kv : secs) {
+    Sec& S = kv.second;
+    for (size_t j = 0; j < S.blk.size(); ++j) {
+      const int64_t i = S.blk[j];
+      const int64_t dst = left ? S.mo + S.off[j] : S.mo + S.off[j] * S.m;
+      add_copy(g, E, X.off[(size_t)i], xoff_plane, dst, tm, X.rows[(size_t)i], X.cols[(size_t)i], X.rows[(size_t)i],
+               S.m);
+    }
+  }
+  run_copies(E, g, dX, Mb.p, nullptr);
+  std::vector<SvdTask> tasks;
+  for (auto& kv : secs) {
+    const Sec& S = kv.second;
+    SvdTask t{};
+    t.a_off = S.mo;
+    t.u_off = S.uo;
+    t.vt_off = S.vo;
+    t.s_off = S.so;
+    t.w_off = S.wo;
+    t.m = (int32_t)S.m;
+    t.n = (int32_t)S.n;
+    t.lda = (int32_t)std::max<int64_t>(S.m, 1);
+    tasks.push_back(t);
+  }
+  {
+    Buf dt = E.upload(tasks);
+    hchk(launch_svd_batched(pl == 2, (const SvdTask*)dt.p, (int)tasks.size(), Mb.p, tm, Ub.p, tu, Vb.p, tv, Sb.p,
+                            Wb.p, tw, E.st),
+         "cheb: sector SVD");
+  }
+  // new tables: the factorised tensor keeps U (l) or V^dagger (r) with k
+  // columns (rows); the neighbour's blocks of the sector charge take k rows
+  // (columns)
+  std::map<Key, std::pair<int64_t, int64_t>> nx, ny;
+  for (int64_t i = 0; i < X.n(); ++i) {
+    const Key kk = key_of(X, i);
+    const Sec& S = secs[left ? kk[1] : kk[0]];
+    nx[kk] = left ? std::make_pair(X.rows[(size_t)i], S.k) : std::make_pair(S.k, X.cols[(size_t)i]);
+  }
+  const HTab& Y = left ? Bt : At;  // the neighbour
+  for (int64_t i = 0; i < Y.n(); ++i) {
+    const Key kk = key_of(Y, i);
+    const uint64_t q = left ? kk[0] : kk[1];
+    auto it = secs.find(q);
+    if (it == secs.end()) ny[kk] = {Y.rows[(size_t)i], Y.cols[(size_t)i]};
+    else ny[kk] = left ? std::make_pair(it->second.k, Y.cols[(size_t)i]) : std::make_pair(Y.rows[(size_t)i], it->second.k);
+  }
+  HTab NX = table_from(nx, pl == 2), NY = table_from(ny, pl == 2);
+  Buf dNX(E.pool, NX.len(), E.st), dNY(E.pool, NY.len(), E.st);
+  std::map<Key, int64_t> nx_index, y_index;
+  for (int64_t t = 0; t < NX.n(); ++t) nx_index[key_of(NX, t)] = t;
+  for (int64_t t = 0; t < Y.n(); ++t) y_index[key_of(Y, t)] = t;
+  // factorised tensor from U / V^dagger
+  std::vector<Copy2D> c1;
+  for (auto& kv : secs) {
+    const Sec& S = kv.second;
+    for (size_t j = 0; j < S.blk.size(); ++j) {
+      const int64_t i = S.blk[j];
+      const int64_t b = nx_index.at(key_of(X, i));
+      if (left)
+        add_copy(c1, E, S.uo + S.off[j], tu, NX.off[(size_t)b], NX.total, X.rows[(size_t)i], S.k, S.m,
+                 X.rows[(size_t)i]);
+      else
+        add_copy(c1, E, S.vo + S.off[j] * S.k, tv, NX.off[(size_t)b], NX.total, S.k, X.cols[(size_t)i], S.k, S.k);
+    }
+  }
+  run_copies(E, c1, left ? Ub.p : Vb.p, dNX.p, nullptr);
+  // S V^dagger (l) or U S (r), then the neighbour's product
+  Buf F(E.pool, left ? pl * tv : pl * tu, E.st);
+  std::vector<Copy2D> c2;
+  for (auto& kv : secs) {
+    const Sec& S = kv.second;
+    if (left) add_copy(c2, E, S.vo, tv, S.vo, tv, S.k, S.n, S.k, S.k, 1.0, S.so, -1);
+    else add_copy(c2, E, S.uo, tu, S.uo, tu, S.m, S.k, S.m, S.m, 1.0, -1, S.so);
+  }
+  run_copies(E, c2, left ? Vb.p : Ub.p, F.p, Sb.p);
+  GemmList gl;
+  std::vector<Copy2D> c3;  // neighbour blocks outside every sector are copied
+  const double* dY = left ? M.d[(size_t)sb].p : M.d[(size_t)sa].p;
+  for (int64_t t = 0; t < NY.n(); ++t) {
+    const Key kk = key_of(NY, t);
+    const int64_t i = y_index.at(kk);
+    const uint64_t q = left ? kk[0] : kk[1];
+    auto it = secs.find(q);
+    if (it == secs.end()) {
+      add_copy(c3, E, Y.off[(size_t)i], Y.total, NY.off[(size_t)t], NY.total, Y.rows[(size_t)i], Y.cols[(size_t)i],
+               Y.rows[(size_t)i], Y.rows[(size_t)i]);
+      continue;
+    }
+    const Sec& S = it->second;
+    if (left)  // (k x n) (n x cols)
+      gl.add(E, NY.off[(size_t)t], NY.total, S.k, Y.cols[(size_t)i], S.n, S.vo, tv, S.k, Y.off[(size_t)i], Y.total,
+             Y.rows[(size_t)i]);
+    else  // (rows x m) (m x k)
+      gl.add(E, NY.off[(size_t)t], NY.total, Y.rows[(size_t)i], S.k, S.m, Y.off[(size_t)i], Y.total,
+             Y.rows[(size_t)i], S.uo, tu, S.m);
+  }
+  if (left) gl.run(E, F.p, dY, dNY.p);
+  else gl.run(E, dY, F.p, dNY.p);
+  run_copies(E, c3, dY, dNY.p, nullptr);
+  if (keep_lambda) {
+    std::vector<double> h((size_t)std::max<int64_t>(ts, 1));
+    if (ts) hchk(hipMemcpyAsync(h.data(), Sb.p, sizeof(double) * (size_t)ts, hipMemcpyDeviceToHost, E.st), "cheb: S");
+    E.sync();
+    auto& L = M.lam[(size_t)(left ? site : site - 1)];
+    L.clear();
+    for (auto& kv : secs) L[kv.first].assign(h.begin() + kv.second.so, h.begin() + kv.second.so + kv.second.k);
+  }
+  if (left) {
+    M.A[(size_t)sa] = NX;
+    M.d[(size_t)sa] = std::move(dNX);
+    M.A[(size_t)sb] = NY;
+    M.d[(size_t)sb] = std::move(dNY);
+  } else {
+    M.A[(size_t)sb] = NX;
+    M.d[(size_t)sb] = std::move(dNX);
+    M.A[(size_t)sa] = NY;
+    M.d[(size_t)sa] = std::move(dNY);
+  }
+}
+
+// ------------------------------------------------------------ truncate
+// DiagBlockMatrix::truncate(tr_weight, chimax, false) per vector: the largest
+// values over the vector's sectors while |tw| > norm * tr_weight and fewer
+// than chimax are kept (multimap<value, q> visited from the end: among equal
+// values the last inserted first).  Returns the kept count per charge.
+std::map<uint64_t, int64_t> lambda_keep(const std::map<uint64_t, std::vector<double>>& lam, unsigned chimax,
+                                        double tr_weight) {
+  std::map<uint64_t, std::vector<std::pair<double, uint64_t>>> by_vec;
+  for (auto& kv : lam)
+    for (double x : kv.second) by_vec[ns_of(kv.first)].push_back({x, kv.first});
+  std::map<uint64_t, int64_t> keep;
+  for (auto& vv : by_vec) {
+    auto& all = vv.second;
+    double nrm = 0.0;
+    for (auto& a : all) nrm += a.first * a.first;
+    nrm = std::sqrt(nrm);
+    std::stable_sort(all.begin(), all.end(),
+                     [](const std::pair<double, uint64_t>& a, const std::pair<double, uint64_t>& b) {
+                       return a.first < b.first;
+                     });
+    double tw = nrm;
+    unsigned chi = 0;
+    for (auto r = all.rbegin(); r != all.rend() && std::fabs(tw) > nrm * tr_weight && chi < chimax; ++r) {
+      tw -= r->first * r->first / nrm;
+      ++keep[r->second];
+      ++chi;
+    }
+  }
+  return keep;
+}
+
+// truncate(mps, chimax, delta) + dropDangling on every bond
+void truncate_mps(Eng& E, DMps& M, unsigned chimax, double delta) {
+  if (!chimax) return;
+  std::vector<std::map<uint64_t, int64_t>> keep((size_t)E.N - 1);
+  for (int b = 0; b + 1 < E.N; ++b) keep[(size_t)b] = lambda_keep(M.lam[(size_t)b], chimax, delta);
+  for (int s = 0; s < E.N; ++s) {
+    const HTab& T = M.A[(size_t)s];
+    std::map<Key, std::pair<int64_t, int64_t>> nm;
+    std::vector<int64_t> src;
+    for (int64_t i = 0; i < T.n(); ++i) {
+      int64_t r = T.rows[(size_t)i], c = T.cols[(size_t)i];
+      if (s > 0) {
+        auto it = keep[(size_t)s - 1].find(T.q[2 * i]);
+        if (it == keep[(size_t)s - 1].end()) continue;
+        r = std::min(r, it->second);
+      }
+      if (s + 1 < E.N) {
+        auto it = keep[(size_t)s].find(T.q[2 * i + 1]);
+        if (it == keep[(size_t)s].end()) continue;
+        c = std::min(c, it->second);
+      }
+      nm[key_of(T, i)] = {r, c};
+    }
+    HTab NT = table_from(nm, T.cplx);
+    Buf nb(E.pool, NT.len(), E.st);
+    std::vector<Copy2D> cp;
+    std::map<Key, int64_t> t_index;
+    for (int64_t i = 0; i < T.n(); ++i) t_index[key_of(T, i)] = i;
+    for (int64_t t = 0; t < NT.n(); ++t) {
+      const int64_t i = t_index.at(key_of(NT, t));
+      add_copy(cp, E, T.off[(size_t)i], T.total, NT.off[(size_t)t], NT.total, NT.rows[(size_t)t], NT.cols[(size_t)t],
+               T.rows[(size_t)i], NT.rows[(size_t)t]);
+    }
+    run_copies(E, cp, M.d[(size_t)s].p, nb.p, nullptr);
+    M.A[(size_t)s] = NT;
+    M.d[(size_t)s] = std::move(nb);
+  }
+  for (int b = 0; b + 1 < E.N; ++b) {
+    auto& L = M.lam[(size_t)b];
+    std::map<uint64_t, std::vector<double>> nl;
+    for (auto& kv : keep[(size_t)b]) {
+      auto it = L.find(kv.first);
+      if (it != L.end()) nl[kv.first].assign(it->second.begin(), it->second.begin() + kv.second);
+    }
+    L.swap(nl);
+  }
+}
+
+// ------------------------------------------------------------ applyMPO
+DMps apply_mpo(Eng& E, const DMps& M, const std::vector<HMpo>& W) {
+  DMps O = new_mps(E);
+  std::map<std::pair<uint64_t, uint32_t>, uint64_t> new_left;  // (a, wl) -> new charge
+  std::map<uint64_t, std::vector<int64_t>> left_sizes;
+  for (int64_t i = 0; i < M.A[0].n(); ++i) {  // site 0: each vector's boundary charge keeps its value
+    new_left[{M.A[0].q[2 * i], 0u}] = M.A[0].q[2 * i];
+    left_sizes[M.A[0].q[2 * i]] = std::vector<int64_t>(W[0].Dl, 1);
+  }
+  for (int s = 0; s < E.N; ++s) {
+    const HTab& T = M.A[(size_t)s];
+    const HMpo& w = W[(size_t)s];
+    const auto& lmap = E.itk[(size_t)s];
+    // five-leg pieces: ((a, wl), (b, wr), out) -> (block, value)
+    typedef std::pair<uint64_t, uint32_t> DK;
+    struct P {
+      int64_t blk;
+      double v;
+    };
+    std::map<std::tuple<DK, DK, uint32_t>, P> F;
+    for (size_t e = 0; e < w.val.size(); ++e)
+      for (int64_t i = 0; i < T.n(); ++i) {
+        if (T.p[(size_t)i] != w.in[e]) continue;
+        auto k = std::make_tuple(DK{T.q[2 * i], w.wl[e]}, DK{T.q[2 * i + 1], w.wr[e]}, w.out[e]);
+        if (F.count(k)) throw HErr(MPSA_EINVAL, "cheb: applyMPO with two MPO entries for one block (MPO not supported)");
+        F[k] = P{i, w.val[e]};
+      }
+    std::map<DK, uint64_t> new_right;
+    for (auto& kv : F) {
+      const DK& kl = std::get<0>(kv.first);
+      auto it = new_left.find(kl);
+      if (it == new_left.end()) continue;
+      const DK& kr = std::get<1>(kv.first);
+      if (!new_right.count(kr)) new_right[kr] = it->second + lmap[std::get<2>(kv.first)];
+    }
+    std::map<uint64_t, std::vector<DK>> inv_left, inv_right;
+    for (auto& kv : new_left) inv_left[kv.second].push_back(kv.first);
+    for (auto& kv : new_right) inv_right[kv.second].push_back(kv.first);
+    std::map<uint64_t, std::vector<int64_t>> right_sizes;
+    for (auto& il : inv_left)
+      for (uint32_t sp = 0; sp < lmap.size(); ++sp) {
+        const uint64_t qr = il.first + lmap[sp];
+        auto ir = inv_right.find(qr);
+        if (ir == inv_right.end()) continue;
+        for (auto& kl : il.second)
+          for (auto& kr : ir->second) {
+            auto f = F.find(std::make_tuple(kl, kr, sp));
+            if (f == F.end()) continue;
+            auto& v = right_sizes[qr];
+            if (v.empty()) v.assign(w.Dr, 0);
+            v[kr.second] = std::max(v[kr.second], T.cols[(size_t)f->second.blk]);
+          }
+      }
+    // blocks (ql, qr, s) of the coarse grid
+    struct Piece {
+      Key key;
+      int64_t blk, r0, c0;
+      double v;
+    };
+    std::vector<Piece> pieces;
+    std::map<Key, std::pair<int64_t, int64_t>> nm;
+    for (auto& il : inv_left) {
+      const uint64_t ql = il.first;
+      auto lsit = left_sizes.find(ql);
+      std::vector<int64_t> ls = lsit != left_sizes.end() ? lsit->second : std::vector<int64_t>(w.Dl, 0);
+      ls.resize(w.Dl, 0);
+      for (uint32_t sp = 0; sp < lmap.size(); ++sp) {
+        const uint64_t qr = ql + lmap[sp];
+        auto ir = inv_right.find(qr);
+        if (ir == inv_right.end()) continue;
+        std::vector<std::pair<DK, DK>> found;
+        for (auto& kl : il.second)
+          for (auto& kr : ir->second)
+            if (F.count(std::make_tuple(kl, kr, sp))) found.push_back({kl, kr});
+        if (found.empty()) continue;
+        const auto& rs = right_sizes[qr];
+        std::vector<int64_t> sx(w.Dl + 1, 0), sy(w.Dr + 1, 0);
+        for (uint32_t a = 0; a < w.Dl; ++a) sx[a + 1] = sx[a] + ls[a];
+        for (uint32_t b = 0; b < w.Dr; ++b) sy[b + 1] = sy[b] + rs[b];
+        const Key nk{ql, qr, sp};
+        nm[nk] = {sx[w.Dl], sy[w.Dr]};
+        for (auto& fr : found) {
+          const P& pc = F[std::make_tuple(fr.first, fr.second, sp)];
+          pieces.push_back(Piece{nk, pc.blk, sx[fr.first.second], sy[fr.second.second], pc.v});
+        }
+      }
+    }
+    HTab NT = table_from(nm, T.cplx);
+    Buf nb(E.pool, NT.len(), E.st);
+    std::map<Key, int64_t> where;
+    for (int64_t t = 0; t < NT.n(); ++t) where[key_of(NT, t)] = t;
+    std::vector<Copy2D> cp;
+    for (auto& pc : pieces) {
+      const int64_t t = where[pc.key];
+      const int64_t rows = T.rows[(size_t)pc.blk], cols = T.cols[(size_t)pc.blk];
+      add_copy(cp, E, T.off[(size_t)pc.blk], T.total, NT.off[(size_t)t] + pc.r0 + pc.c0 * NT.rows[(size_t)t], NT.total,
+               rows, cols, rows, NT.rows[(size_t)t], pc.v);
+    }
+    run_copies(E, cp, M.d[(size_t)s].p, nb.p, nullptr);
+    O.A[(size_t)s] = NT;
+    O.d[(size_t)s] = std::move(nb);
+    left_sizes = right_sizes;
+    new_left = new_right;
+  }
+  return O;
+}
+
+// ------------------------------------------------------------ MPS sums
+// xa a + xb b as the reference's += (first site side by side, last site
+// stacked, bulk outersum, sizes fixed): a direct sum of the bond spaces with
+// a's part first; the scale of each summand sits on its first site.
+DMps axpby_mps(Eng& E, double xa, const DMps& a, double xb, const DMps& b) {
+  const int N = E.N;
+  // bond dims per charge of each summand (bond s between sites s and s+1)
+  auto dims = [&](const DMps& M, int bond) {
+    std::map<uint64_t, int64_t> d;
+    const HTab& L = M.A[(size_t)bond];
+    for (int64_t i = 0; i < L.n(); ++i) d[L.q[2 * i + 1]] = std::max(d[L.q[2 * i + 1]], L.cols[(size_t)i]);
+    const HTab& R = M.A[(size_t)bond + 1];
+    for (int64_t i = 0; i < R.n(); ++i) d[R.q[2 * i]] = std::max(d[R.q[2 * i]], R.rows[(size_t)i]);
+    return d;
+  };
+  std::vector<std::map<uint64_t, int64_t>> da, db;
+  for (int bnd = 0; bnd + 1 < N; ++bnd) {
+    da.push_back(dims(a, bnd));
+    db.push_back(dims(b, bnd));
+  }
+  auto get = [](const std::map<uint64_t, int64_t>& m, uint64_t q) {
+    auto it = m.find(q);
+    return it == m.end() ? (int64_t)0 : it->second;
+  };
+  DMps O = new_mps(E);
+  for (int s = 0; s < N; ++s) {
+    const HTab &Ta = a.A[(size_t)s], &Tb = b.A[(size_t)s];
+    std::map<Key, std::pair<int64_t, int64_t>> nm;
+    auto shape = [&](const Key& k) {
+      const int64_t r = s == 0 ? 1 : get(da[(size_t)s - 1], k[0]) + get(db[(size_t)s - 1], k[0]);
+      const int64_t c = s == N - 1 ? 1 : get(da[(size_t)s], k[1]) + get(db[(size_t)s], k[1]);
+      return std::make_pair(r, c);
+    };
+    for (int64_t i = 0; i < Ta.n(); ++i) nm[key_of(Ta, i)] = shape(key_of(Ta, i));
+    for (int64_t i = 0; i < Tb.n(); ++i) nm[key_of(Tb, i)] = shape(key_of(Tb, i));
+    HTab NT = table_from(nm, Ta.cplx);
+    Buf nb(E.pool, NT.len(), E.st);
+    std::map<Key, int64_t> where;
+    for (int64_t t = 0; t < NT.n(); ++t) where[key_of(NT, t)] = t;
+    std::vector<Copy2D> ca, cb;
+    for (int64_t i = 0; i < Ta.n(); ++i) {
+      const int64_t t = where[key_of(Ta, i)];
+      add_copy(ca, E, Ta.off[(size_t)i], Ta.total, NT.off[(size_t)t], NT.total, Ta.rows[(size_t)i], Ta.cols[(size_t)i],
+               Ta.rows[(size_t)i], NT.rows[(size_t)t], s == 0 ? xa : 1.0);
+    }
+    for (int64_t i = 0; i < Tb.n(); ++i) {
+      const Key k = key_of(Tb, i);
+      const int64_t t = where[k];
+      const int64_t r0 = s == 0 ? 0 : get(da[(size_t)s - 1], k[0]);
+      const int64_t c0 = s == N - 1 ? 0 : get(da[(size_t)s], k[1]);
+      add_copy(cb, E, Tb.off[(size_t)i], Tb.total, NT.off[(size_t)t] + r0 + c0 * NT.rows[(size_t)t], NT.total,
+               Tb.rows[(size_t)i], Tb.cols[(size_t)i], Tb.rows[(size_t)i], NT.rows[(size_t)t], s == 0 ? xb : 1.0);
+    }
+    run_copies(E, ca, a.d[(size_t)s].p, nb.p, nullptr);
+    run_copies(E, cb, b.d[(size_t)s].p, nb.p, nullptr);
+    O.A[(size_t)s] = NT;
+    O.d[(size_t)s] = std::move(nb);
+  }
+  return O;
+}
+
+// ------------------------------------------------------------ compression
+double conv_of(double n1sq, double n2sq, cplx ov) { return (n1sq + n2sq - 2.0 * std::abs(ov)) / n1sq; }
+
+// Copy the blocks of the vectors marked in `mask` from `from` to `to` (both
+// in layout T).
+void copy_vector_blocks(Eng& E, const HTab& T, const double* from, double* to, const std::vector<char>& mask) {
+  std::vector<Copy2D> cp;
+  for (int64_t i = 0; i < T.n(); ++i) {
+    if (!mask[(size_t)ns_of(T.q[2 * i])]) continue;
+    add_copy(cp, E, T.off[(size_t)i], T.total, T.off[(size_t)i], T.total, T.rows[(size_t)i], T.cols[(size_t)i],
+             T.rows[(size_t)i], T.rows[(size_t)i]);
+  }
+  run_copies(E, cp, from, to, nullptr);
+}
+
+bool same_tables(const HTab& a, const HTab& b) {
+  return a.q == b.q && a.p == b.p && a.rows == b.rows && a.cols == b.cols;
+}
+bool same_tables(const DMps& a, const DMps& b) {
+  for (size_t s = 0; s < a.A.size(); ++s)
+    if (!same_tables(a.A[s], b.A[s])) return false;
+  return true;
+}
+
+// The variational sweeps shared by compressSingleSite (MPO W) and compress
+// (identity MPO): comp is optimised against `mps` with mixed environments
+// (mps below, comp above).  Every vector follows the reference's loop: stop
+// when conv grows, converge when step >= maxstep or |conv| < delta; a vector
+// that has stopped keeps its state (and its BR[1]) while the others sweep on.
+// With final_site0 the left-most tensor is solved once more from BR[1] (the
+// end of compress).
+void sweeps(Eng& E, DMps& comp, const DMps& mps, const std::vector<HMpo>& Wv, int maxstep, double delta,
+            const std::vector<double>& n1sq, bool final_site0) {
+  const int N = E.N, cp = E.planes == 2;
+  if (maxstep == 0) return;
+  std::vector<HTab> BLt((size_t)N + 1), BRt((size_t)N + 1);  // BLt[s+1] = BL[s]; BRt[s] = BR[s]
+  std::vector<Buf> BLb((size_t)N + 1), BRb((size_t)N + 1);
+  BLt[0] = lmin_tab_of(mps, cp);
+  BLb[0] = identity_buf(E, BLt[0]);
+  BRt[(size_t)N] = rmax_tab_of(mps, cp);
+  BRb[(size_t)N] = identity_buf(E, BRt[(size_t)N]);
+  for (int s = N - 1; s > 0; --s)
+    env_layer(E, BRt[(size_t)s + 1], BRb[(size_t)s + 1].p, mps.A[(size_t)s], mps.d[(size_t)s].p, comp.A[(size_t)s],
+              comp.d[(size_t)s].p, Wv[(size_t)s], E.itk[(size_t)s], -1, BRt[(size_t)s], BRb[(size_t)s]);
+  std::vector<char> done((size_t)E.nvec, 0);
+  std::vector<double> conv_old((size_t)E.nvec, 1.0);
+  HTab br1_t;   // BR[1] of every vector as of its last sweep
+  Buf br1_b;
+  int step = 1;
+  while (true) {
+    bool any = false, frozen_any = false;
+    for (char d : done) {
+      any = any || !d;
+      frozen_any = frozen_any || d;
+    }
+    if (!any) break;
+    DMps before;
+    if (frozen_any) before = copy_mps(E, comp);
+    for (int s = 0; s < N - 1; ++s) {
+      HTab t;
+      Buf b;
+      heff1(E, BLt[(size_t)s], BLb[(size_t)s].p, mps.A[(size_t)s], mps.d[(size_t)s].p, BRt[(size_t)s + 1],
+            BRb[(size_t)s + 1].p, Wv[(size_t)s], E.itk[(size_t)s], t, b);
+      comp.A[(size_t)s] = t;
+      comp.d[(size_t)s] = std::move(b);
+      prepare_site(E, comp, s, true, false);
+      env_layer(E, BLt[(size_t)s], BLb[(size_t)s].p, mps.A[(size_t)s], mps.d[(size_t)s].p, comp.A[(size_t)s],
+                comp.d[(size_t)s].p, Wv[(size_t)s], E.itk[(size_t)s], 1, BLt[(size_t)s + 1], BLb[(size_t)s + 1]);
+    }
+    for (int s = N - 1; s > 0; --s) {
+      HTab t;
+      Buf b;
+      heff1(E, BLt[(size_t)s], BLb[(size_t)s].p, mps.A[(size_t)s], mps.d[(size_t)s].p, BRt[(size_t)s + 1],
+            BRb[(size_t)s + 1].p, Wv[(size_t)s], E.itk[(size_t)s], t, b);
+      comp.A[(size_t)s] = t;
+      comp.d[(size_t)s] = std::move(b);
+      prepare_site(E, comp, s, false, false);
+      env_layer(E, BRt[(size_t)s + 1], BRb[(size_t)s + 1].p, mps.A[(size_t)s], mps.d[(size_t)s].p, comp.A[(size_t)s],
+                comp.d[(size_t)s].p, Wv[(size_t)s], E.itk[(size_t)s], -1, BRt[(size_t)s], BRb[(size_t)s]);
+    }
+    if (frozen_any) {
+      if (!same_tables(before, comp) || (final_site0 && br1_b.p && !same_tables(br1_t, BRt[1])))
+        throw HErr(MPSA_EINVAL, "cheb: block structure changed between compression sweeps");
+      for (int s = 0; s < N; ++s)
+        copy_vector_blocks(E, comp.A[(size_t)s], before.d[(size_t)s].p, comp.d[(size_t)s].p, done);
+    }
+    const std::vector<cplx> n2 = overlap(E, comp, comp), ov = overlap(E, comp, mps);
+    std::vector<char> newly((size_t)E.nvec, 0);
+    for (int v = 0; v < E.nvec; ++v) {
+      if (done[(size_t)v]) continue;
+      const double conv = conv_of(n1sq[(size_t)v], n2[(size_t)v].real(), ov[(size_t)v]);
+      if (conv_old[(size_t)v] < conv) {
+        done[(size_t)v] = newly[(size_t)v] = 1;
+        continue;
+      }
+      conv_old[(size_t)v] = conv;
+      if (step >= maxstep || std::fabs(conv) < delta) done[(size_t)v] = newly[(size_t)v] = 1;
+    }
+    if (final_site0) {
+      if (!br1_b.p) {
+        br1_t = BRt[1];
+        br1_b = Buf(E.pool, br1_t.len(), E.st);
+      }
+      copy_vector_blocks(E, br1_t, BRb[1].p, br1_b.p, newly);
+    }
+    ++step;
+  }
+  if (final_site0) {
+    HTab t;
+    Buf b;
+    heff1(E, BLt[0], BLb[0].p, mps.A[0], mps.d[0].p, br1_t, br1_b.p, Wv[0], E.itk[0], t, b);
+    comp.A[0] = t;
+    comp.d[0] = std::move(b);
+  }
+}
+
+std::vector<double> real_parts(const std::vector<cplx>& v) {
+  std::vector<double> r;
+  for (auto& x : v) r.push_back(x.real());
+  return r;
+}
+
+// compressSingleSite(mps, W, chimax, maxstep, delta): returns comp ~ H mps
+DMps compress_single_site(Eng& E, const DMps& mps, unsigned chimax, int maxstep, double delta,
+                          std::vector<double>& conv) {
+  const int N = E.N;
+  std::vector<double> n1sq;
+  {
+    DMps t = apply_mpo(E, mps, E.W);
+    n1sq = real_parts(overlap(E, t, t));
+  }
+  DMps tmps = copy_mps(E, mps);
+  for (int s = 0; s < N - 1; ++s) prepare_site(E, tmps, s, true, true);
+  for (int s = N - 1; s > 0; --s) prepare_site(E, tmps, s, false, true);
+  const unsigned D = N > 1 ? E.W[1].Dr : 1;
+  truncate_mps(E, tmps, (unsigned)std::floor((double)chimax / (double)D), delta);
+  DMps comp = apply_mpo(E, tmps, E.W);
+  tmps = DMps();
+  for (int s = 0; s < N - 1; ++s) prepare_site(E, comp, s, true, false);
+  for (int s = N - 1; s > 0; --s) prepare_site(E, comp, s, false, false);
+  sweeps(E, comp, mps, E.W, maxstep, delta, n1sq, false);
+  const std::vector<cplx> n2 = overlap(E, comp, comp), ov = overlap(E, comp, mps);
+  conv.assign((size_t)E.nvec, 0.0);
+  for (int v = 0; v < E.nvec; ++v) conv[(size_t)v] = conv_of(n1sq[(size_t)v], n2[(size_t)v].real(), ov[(size_t)v]);
+  return comp;
+}
+
+// compress(mps, chimax, maxstep, delta, normalize = false): mps is
+// canonicalised in place, the compressed state is returned
+DMps compress(Eng& E, DMps& mps, unsigned chimax, int maxstep, double delta, std::vector<double>& conv) {
+  const int N = E.N;
+  for (int s = 0; s < N - 1; ++s) prepare_site(E, mps, s, true, true);
+  for (int s = N - 2; s > 0; --s) prepare_site(E, mps, s, false, true);
+  DMps comp = copy_mps(E, mps);
+  truncate_mps(E, comp, chimax, delta);
+  for (int s = 0; s < N - 1; ++s) prepare_site(E, comp, s, true, false);
+  for (int s = N - 1; s > 0; --s) prepare_site(E, comp, s, false, false);
+  const std::vector<double> n1sq = real_parts(overlap(E, mps, mps));
+  sweeps(E, comp, mps, E.Id, maxstep, delta, n1sq, true);  // ends with the site-0 solve
+  for (int s = 0; s < N - 1; ++s) prepare_site(E, comp, s, true, false);
+  for (int s = N - 1; s > 0; --s) prepare_site(E, comp, s, false, false);
+  const std::vector<cplx> n2 = overlap(E, comp, comp), ov = overlap(E, comp, mps);
+  conv.assign((size_t)E.nvec, 0.0);
+  for (int v = 0; v < E.nvec; ++v) conv[(size_t)v] = conv_of(n1sq[(size_t)v], n2[(size_t)v].real(), ov[(size_t)v]);
+  return comp;
+}
+
+DMps& slot_at(Eng& E, int i, bool must_exist) {
+  if (i < 0 || i >= (int)E.slot.size()) throw HErr(MPSA_EINVAL, "cheb: slot index out of range");
+  if (must_exist && !E.slot[(size_t)i].valid()) throw HErr(MPSA_EINVAL, "cheb: empty slot");
+  return E.slot[(size_t)i];
+}
+
+int fail(const std::exception& e, int code) {
+  set_last_error(e.what());
+  return code;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpsa_cheb_create(mpsa_cheb** out, int nsites, int nvec, const mpsa_blocks* sites, const double* const* dA,
+                     const mpsa_mpo* W, const uint64_t* itokey, const int* d, int h1_precision, mpsa_stream stream) {
+  mpsa_cheb* E = nullptr;
+  try {
+    if (!out || nsites < 2 || nvec < 1 || nvec > 4096 || !sites || !dA || !W || !itokey || !d)
+      throw HErr(MPSA_EINVAL, "cheb_create: bad arguments");
+    if (h1_precision != MPSA_F64 && h1_precision != MPSA_F32)
+      throw HErr(MPSA_EINVAL, "cheb_create: the one-site contraction runs in FP64 or FP32");
+    E = new mpsa_cheb();
+    E->st = (hipStream_t)stream;
+    E->N = nsites;
+    E->nvec = nvec;
+    E->h1prec = h1_precision;
+    E->planes = sites[0].is_complex ? 2 : 1;
+    const uint64_t* ik = itokey;
+    for (int i = 0; i < nsites; ++i) {
+      if (d[i] < 1) throw HErr(MPSA_EINVAL, "cheb_create: d < 1");
+      E->itk.emplace_back(ik, ik + d[i]);
+      ik += d[i];
+      for (auto x : E->itk.back())
+        if (x >> kNsShift) throw HErr(MPSA_EINVAL, "cheb_create: i_to_key value above 2^40");
+      E->W.push_back(copy_mpo(&W[i]));
+      HMpo id;
+      id.Dl = id.Dr = 1;
+      for (int s = 0; s < d[i]; ++s) {
+        id.wl.push_back(0);
+        id.wr.push_back(0);
+        id.out.push_back((uint32_t)s);
+        id.in.push_back((uint32_t)s);
+        id.val.push_back(1.0);
+      }
+      E->Id.push_back(id);
+    }
+    E->slot.resize(8);
+    DMps M = new_mps(*E);
+    for (int s = 0; s < nsites; ++s) {
+      std::map<Key, std::pair<int64_t, int64_t>> nm;
+      struct Src {
+        int v;
+        int64_t i;
+      };
+      std::map<Key, Src> src;
+      for (int v = 0; v < nvec; ++v) {
+        const mpsa_blocks& b = sites[(size_t)v * nsites + s];
+        if (b.ns != 1 || b.n < 0 || (b.is_complex != 0) != (E->planes == 2))
+          throw HErr(MPSA_EINVAL, "cheb_create: site tables must be ns = 1 and all real or all complex");
+        for (int64_t i = 0; i < b.n; ++i) {
+          if ((b.q[2 * i] >> kNsShift) || (b.q[2 * i + 1] >> kNsShift))
+            throw HErr(MPSA_EINVAL, "cheb_create: charge above 2^40");
+          const Key k{b.q[2 * i] + ((uint64_t)v << kNsShift), b.q[2 * i + 1] + ((uint64_t)v << kNsShift),
+                      (uint64_t)b.p[i]};
+          nm[k] = {b.rows[i], b.cols[i]};
+          src[k] = Src{v, i};
+        }
+      }
+      HTab T = table_from(nm, E->planes == 2);
+      Buf buf(E->pool, T.len(), E->st);
+      std::vector<Copy2D> dummy;
+      for (int64_t t = 0; t < T.n(); ++t) {
+        const Src& sr = src[key_of(T, t)];
+        const mpsa_blocks& b = sites[(size_t)sr.v * nsites + s];
+        int64_t tot = 0;
+        for (int64_t i = 0; i < b.n; ++i) tot = std::max<int64_t>(tot, b.offset[i] + b.rows[i] * b.cols[i]);
+        const double* base = dA[(size_t)sr.v * nsites + s];
+        if (!base) throw HErr(MPSA_EINVAL, "cheb_create: null site buffer");
+        const int64_t len = b.rows[sr.i] * b.cols[sr.i];
+        for (int pl = 0; pl < E->planes; ++pl)
+          if (len)
+            hchk(hipMemcpyAsync(buf.p + T.off[(size_t)t] + pl * T.total, base + b.offset[sr.i] + pl * tot,
+                                sizeof(double) * (size_t)len, hipMemcpyDeviceToDevice, E->st),
+                 "cheb_create: copy");
+      }
+      M.A[(size_t)s] = T;
+      M.d[(size_t)s] = std::move(buf);
+    }
+    E->slot[0] = std::move(M);
+    E->sync();
+    *out = E;
+    return MPSA_OK;
+  } catch (const HErr& e) {
+    delete E;
+    return fail(e, e.code);
+  } catch (const std::exception& e) {
+    delete E;
+    return fail(e, MPSA_EINVAL);
+  }
+}
+
+int mpsa_cheb_copy(mpsa_cheb* E, int dst, int src) {
+  try {
+    if (!E) throw HErr(MPSA_EINVAL, "cheb: null engine");
+    DMps c = copy_mps(*E, slot_at(*E, src, true));
+    slot_at(*E, dst, false) = std::move(c);
+    E->sync();
+    return MPSA_OK;
+  } catch (const HErr& e) {
+    return fail(e, e.code);
+  } catch (const std::exception& e) {
+    return fail(e, MPSA_EINVAL);
+  }
+}
+
+int mpsa_cheb_apply_mpo(mpsa_cheb* E, int dst, int src) {
+  try {
+    if (!E) throw HErr(MPSA_EINVAL, "cheb: null engine");
+    DMps o = apply_mpo(*E, slot_at(*E, src, true), E->W);
+    slot_at(*E, dst, false) = std::move(o);
+    E->sync();
+    return MPSA_OK;
+  } catch (const HErr& e) {
+    return fail(e, e.code);
+  } catch (const std::exception& e) {
+    return fail(e, MPSA_EINVAL);
+  }
+}
+
+int mpsa_cheb_overlap(mpsa_cheb* E, int a, int b, double* out) {
+  try {
+    if (!E || !out) throw HErr(MPSA_EINVAL, "cheb: bad arguments");
+    const auto v = overlap(*E, slot_at(*E, a, true), slot_at(*E, b, true));
+    for (size_t i = 0; i < v.size(); ++i) {
+      out[2 * i] = v[i].real();
+      out[2 * i + 1] = v[i].imag();
+    }
+    return MPSA_OK;
+  } catch (const HErr& e) {
+    return fail(e, e.code);
+  } catch (const std::exception& e) {
+    return fail(e, MPSA_EINVAL);
+  }
+}
+
+int mpsa_cheb_axpby(mpsa_cheb* E, int dst, double xa, int a, double xb, int b) {
+  try {
+    if (!E) throw HErr(MPSA_EINVAL, "cheb: null engine");
+    DMps o = axpby_mps(*E, xa, slot_at(*E, a, true), xb, slot_at(*E, b, true));
+    slot_at(*E, dst, false) = std::move(o);
+    E->sync();
+    return MPSA_OK;
+  } catch (const HErr& e) {
+    return fail(e, e.code);
+  } catch (const std::exception& e) {
+    return fail(e, MPSA_EINVAL);
+  }
+}
+
+int mpsa_cheb_compress_single_site(mpsa_cheb* E, int dst, int src, unsigned chimax, int maxstep, double delta,
+                                   double* conv) {
+  try {
+    if (!E || maxstep < 0) throw HErr(MPSA_EINVAL, "cheb: bad arguments");
+    std::vector<double> c;
+    DMps o = compress_single_site(*E, slot_at(*E, src, true), chimax, maxstep, delta, c);
+    slot_at(*E, dst, false) = std::move(o);
+    E->sync();
+    if (conv) std::copy(c.begin(), c.end(), conv);
+    return MPSA_OK;
+  } catch (const HErr& e) {
+    return fail(e, e.code);
+  } catch (const std::exception& e) {
+    return fail(e, MPSA_EINVAL);
+  }
+}
+
+int mpsa_cheb_compress(mpsa_cheb* E, int slot, unsigned chimax, int maxstep, double delta, double* conv) {
+  try {
+    if (!E || maxstep < 0) throw HErr(MPSA_EINVAL, "cheb: bad arguments");
+    std::vector<double> c;
+    DMps o = compress(*E, slot_at(*E, slot, true), chimax, maxstep, delta, c);
+    slot_at(*E, slot, false) = std::move(o);
+    E->sync();
+    if (conv) std::copy(c.begin(), c.end(), conv);
+    return MPSA_OK;
+  } catch (const HErr& e) {
+    return fail(e, e.code);
+  } catch (const std::exception& e) {
+    return fail(e, MPSA_EINVAL);
+  }
+}
+
+// simulateSSCompress with the start vectors of slot 0 as t0 and as the final
+// state (opi = opf = identity): mu_n = <t_n|t_0> per vector (re, im),
+// reconstructed doubled moments, and both convergence measures per moment.
+int mpsa_cheb_moments(mpsa_cheb* E, int nmom, unsigned chimax, int ncmax, double delta, double* moments,
+                      double* recons, double* conv1, double* conv2, double* seconds) {
+  try {
+    if (!E || nmom < 1 || ncmax < 0 || !moments) throw HErr(MPSA_EINVAL, "cheb_moments: bad arguments");
+    const auto t0c = std::chrono::steady_clock::now();
+    const int nv = E->nvec;
+    auto put = [&](double* arr, int ld, int idx, int v, cplx x) {
+      if (!arr) return;
+      arr[((size_t)v * ld + idx) * 2] = x.real();
+      arr[((size_t)v * ld + idx) * 2 + 1] = x.imag();
+    };
+    auto put_r = [&](double* arr, int idx, int v, double x) {
+      if (arr) arr[(size_t)v * nmom + idx] = x;
+    };
+    const int nrec = 2 * nmom - 1;
+    DMps& f = slot_at(*E, 0, true);
+    DMps t0 = copy_mps(*E, f);
+    auto mu0 = overlap(*E, f, t0);
+    for (int v = 0; v < nv; ++v) {
+      put(moments, nmom, 0, v, mu0[(size_t)v]);
+      put_r(conv1, 0, v, 0.0);
+      put_r(conv2, 0, v, 0.0);
+    }
+    if (nmom > 1) {
+      std::vector<double> c1;
+      DMps t1 = compress_single_site(*E, t0, chimax, ncmax, delta, c1);
+      auto mu1 = overlap(*E, f, t1);
+      auto a = overlap(*E, t0, t0), b = overlap(*E, t1, t0), c = overlap(*E, t1, t1);
+      for (int v = 0; v < nv; ++v) {
+        put(moments, nmom, 1, v, mu1[(size_t)v]);
+        put(recons, nrec, 0, v, 2.0 * a[(size_t)v] - mu0[(size_t)v]);
+        put(recons, nrec, 1, v, 2.0 * b[(size_t)v] - mu1[(size_t)v]);
+        put(recons, nrec, 2, v, 2.0 * c[(size_t)v] - mu0[(size_t)v]);
+        put_r(conv1, 1, v, c1[(size_t)v]);
+        put_r(conv2, 1, v, 0.0);
+      }
+      for (int n = 2; n < nmom; ++n) {
+        DMps tm = compress_single_site(*E, t1, chimax, ncmax, delta, c1);
+        DMps t2 = axpby_mps(*E, 2.0, tm, -1.0, t0);
+        tm = DMps();
+        auto nsq = overlap(*E, t2, t2);
+        auto mun = overlap(*E, f, t2);
+        auto x = overlap(*E, t2, t1);
+        std::vector<double> c2;
+        DMps t2c = compress(*E, t2, chimax, ncmax, delta, c2);
+        for (int v = 0; v < nv; ++v) {
+          put(moments, nmom, n, v, mun[(size_t)v]);
+          put(recons, nrec, 2 * n - 1, v, 2.0 * x[(size_t)v] - mu1[(size_t)v]);
+          put(recons, nrec, 2 * n, v, 2.0 * nsq[(size_t)v] - mu0[(size_t)v]);
+          put_r(conv1, n, v, c1[(size_t)v]);
+          put_r(conv2, n, v, c2[(size_t)v]);
+        }
+        t0 = std::move(t1);
+        t1 = std::move(t2c);
+      }
+      slot_at(*E, 1, false) = std::move(t1);  // the last moment's state
+    }
+    E->sync();
+    if (seconds) *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0c).count();
+    return MPSA_OK;
+  } catch (const HErr& e) {
+    return fail(e, e.code);
+  } catch (const std::exception& e) {
+    return fail(e, MPSA_EINVAL);
+  }
+}
+
+// Site tensor `site` of vector `vec` in slot `slot`: its own charges (the
+// vector's bits removed), packed at 64-element offsets; planar when complex.
+int mpsa_cheb_site(mpsa_cheb* E, int slot, int vec, int site, int64_t* n_blocks, int64_t* len, uint64_t* q,
+                   uint32_t* p, int64_t* rows, int64_t* cols, int64_t* offsets, double* dst) {
+  try {
+    if (!E || vec < 0 || vec >= E->nvec || site < 0 || site >= E->N || !n_blocks || !len)
+      throw HErr(MPSA_EINVAL, "cheb_site: bad arguments");
+    const DMps& M = slot_at(*E, slot, true);
+    const HTab& T = M.A[(size_t)site];
+    std::vector<int64_t> idx;
+    for (int64_t i = 0; i < T.n(); ++i)
+      if (ns_of(T.q[2 * i]) == (uint64_t)vec) idx.push_back(i);
+    int64_t o = 0;
+    std::vector<int64_t> offs;
+    for (int64_t i : idx) {
+      o = align64(o);
+      offs.push_back(o);
+      o += T.rows[(size_t)i] * T.cols[(size_t)i];
+    }
+    *n_blocks = (int64_t)idx.size();
+    *len = o * E->planes;
+    const uint64_t base = (uint64_t)vec << kNsShift;
+    if (q && p && rows && cols && offsets)
+      for (size_t j = 0; j < idx.size(); ++j) {
+        q[2 * j] = T.q[2 * idx[j]] - base;
+        q[2 * j + 1] = T.q[2 * idx[j] + 1] - base;
+        p[j] = T.p[(size_t)idx[j]];
+        rows[j] = T.rows[(size_t)idx[j]];
+        cols[j] = T.cols[(size_t)idx[j]];
+        offsets[j] = offs[j];
+      }
+    if (dst && o > 0) {
+      std::vector<Copy2D> cp;
+      for (size_t j = 0; j < idx.size(); ++j) {
+        const int64_t i = idx[j];
+        add_copy(cp, *E, T.off[(size_t)i], T.total, offs[j], o, T.rows[(size_t)i], T.cols[(size_t)i],
+                 T.rows[(size_t)i], T.rows[(size_t)i]);
+      }
+      run_copies(*E, cp, M.d[(size_t)site].p, dst, nullptr);
+      E->sync();
+    }
+    return MPSA_OK;
+  } catch (const HErr& e) {
+    return fail(e, e.code);
+  } catch (const std::exception& e) {
+    return fail(e, MPSA_EINVAL);
+  }
+}
+
+int mpsa_cheb_stats(const mpsa_cheb* E, int64_t* plans_built, int64_t* pool_bytes) {
+  if (!E) {
+    set_last_error("cheb_stats: null engine");
+    return MPSA_EINVAL;
+  }
+  if (plans_built) *plans_built = E->plans_built;
+  if (pool_bytes) {
+    int64_t b = 0;
+    for (auto& a : E->pool.all) b += (int64_t)a.second;
+    *pool_bytes = b;
+  }
+  return MPSA_OK;
+}
+
+int mpsa_cheb_destroy(mpsa_cheb* E) {
+  delete E;
+  return MPSA_OK;
+}
+
+}  // extern "C"

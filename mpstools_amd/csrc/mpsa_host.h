@@ -32,16 +32,18 @@ inline int64_t align64(int64_t x) { return (x + 63) / 64 * 64; }
 // A key table owned on the host.
 struct HTab {
   int ns = 1;
+  int cplx = 0;  // complex128, planar: real parts, then imaginary parts at + total
   std::vector<uint64_t> q;
   std::vector<uint32_t> p;
   std::vector<int64_t> rows, cols, off;
   int64_t total = 0;
   int64_t n() const { return (int64_t)rows.size(); }
+  int64_t len() const { return total * (cplx ? 2 : 1); }  // doubles of a packed buffer
   mpsa_blocks view() const {
     mpsa_blocks b;
     b.n = n();
     b.ns = ns;
-    b.is_complex = 0;
+    b.is_complex = cplx;
     b.q = q.data();
     b.p = p.data();
     b.rows = rows.data();

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <limits>
+#include <vector>
 
 namespace mpsa {
 
@@ -80,6 +81,10 @@ hipError_t launch_dot(const double* x, const double* y, int64_t n, int cplx, dou
 hipError_t launch_axpby(double ar, double ai, const double* x, double br, double bi, double* y, int64_t n, int cplx,
                         hipStream_t st);
 int gemm_max_grid(int prec_in, int tile_class);
+// Finalise a grouped GEMM task list built outside a plan (mpsa_plan.cpp): the
+// plans' own invariants (segments with k > 0, non-empty tasks, longest first,
+// tiles numbered in task order for the chosen class).
+void finalize_gemm_list(std::vector<GTask>& tasks, std::vector<GSeg>& segs, int prec, int* tiles, int* tile_class);
 // dense block copies of the split (column-major; see copy2d_kernel)
 struct Copy2D {
   int64_t src_off, dst_off;
@@ -92,6 +97,18 @@ hipError_t launch_copy2d(const Copy2D* tasks, int ntasks, int64_t max_elems, con
                          const double* scales, hipStream_t st);
 hipError_t launch_fixphase(double* U, int64_t m, int64_t ldu, double* Vt, int64_t k, int64_t n, int64_t ldv,
                            hipStream_t st);
+// Batched thin SVD of small sector matrices (mpsa_factor.hip): M (m x n, ld
+// lda) at a_off -> U (m x k, ld m) at u_off, S (k) at s_off, V^dagger (k x n,
+// ld k) at vt_off, k = min(m, n); workspace of p * r + r * r elements at
+// w_off (p = max(m, n), r = min(m, n) <= kSvdMaxR).  Complex data planar.
+constexpr int kSvdMaxR = 1024;
+struct SvdTask {
+  int64_t a_off, u_off, vt_off, s_off, w_off;
+  int32_t m, n, lda, pad;
+};
+hipError_t launch_svd_batched(int cplx, const SvdTask* tasks, int ntasks, const double* M, int64_t m_plane,
+                              double* U, int64_t u_plane, double* Vt, int64_t vt_plane, double* S, double* W,
+                              int64_t w_plane, hipStream_t st);
 // batched Lanczos vector algebra (device-resident coefficients and step counter)
 hipError_t launch_bdot(const double* x, const double* y, int64_t n, int64_t vs, int nvec, double* partial, int nparts,
                        double* out, int ld, const int* step, int root, hipStream_t st);

@@ -317,6 +317,21 @@ static void finalize_stage(Stage& s, int prec) {
   assign_tiles(s, prec, pick_class(s.tasks, s.segs, prec));
 }
 
+// A grouped GEMM assembled outside a plan (block products of the compression
+// and the two-site product) goes through the same finalisation: empty-k
+// segments dropped, empty tasks dropped, longest tasks first, tile class by
+// the same heuristic, tiles numbered per task.
+void finalize_gemm_list(std::vector<GTask>& tasks, std::vector<GSeg>& segs, int prec, int* tiles, int* tile_class) {
+  Stage s;
+  s.tasks.swap(tasks);
+  s.segs.swap(segs);
+  finalize_stage(s, prec);
+  tasks.swap(s.tasks);
+  segs.swap(s.segs);
+  *tiles = s.tiles;
+  *tile_class = s.tile_class;
+}
+
 static void finalize_mix(mpsa_plan* P) {
   int chunk = 0;
   std::vector<MixTask> m2;

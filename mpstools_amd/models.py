@@ -327,3 +327,31 @@ def hubbard_mpo(N, t=1.0, U=4.0, ch_up=0.0, ch_down=0.0, V=None):
     """Uniform Fermi-Hubbard chain of N sites (optional uniform V)."""
     return fermi_hubbard_mpo([t] * (N - 1), [U] * N, [ch_up] * N, [ch_down] * N,
                              None if V is None else [V] * (N - 1))
+
+
+def rescale_mpo(mpo, scale, shift):
+    """The MPO of (H - shift) / scale, for the Chebyshev recurrence (the
+    reference rescales its Hamiltonian's couplings before simulateSSCompress,
+    src/container.hpp:2343).  For MPOs in the layout of the constructors above
+    (aux channel Dl-1 of a site is the 'term not started' channel, 0 the 'term
+    done' one): every entry leaving the start channel other than its
+    pass-through identity begins exactly one term, so those entries are
+    divided by `scale`; -shift / (scale N) identity is added to every site's
+    on-site entry (start -> done)."""
+    N = len(mpo)
+    out = []
+    for site, W in enumerate(mpo):
+        M = MPOMat(W.Dl, W.Dr, W.d)
+        start = W.Dl - 1 if site > 0 else 0
+        for k, op in W.m.items():
+            wl, wr = k
+            if wl == start and not (site < N - 1 and wr == W.Dr - 1):
+                M[k] = op * (1.0 / scale)
+            else:
+                M[k] = op
+        onsite = (start, 0)
+        base = M.m.get(onsite, LocalOp(W.d))
+        M[onsite] = base + LocalOp(W.d, {(i, i): -shift / (scale * N) for i in range(W.d)})
+        M.squeeze(1e-16)
+        out.append(M)
+    return out

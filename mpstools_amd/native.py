@@ -99,6 +99,17 @@ def lib():
         L.mpsa_dmrg_site.argtypes = [P, C.c_int, P, P, P, P, P, P, P, P, P]
         L.mpsa_dmrg_destroy.argtypes = [P]
         L.mpsa_tridiag_eig.argtypes = [C.c_int, P, P, P]
+        L.mpsa_cheb_create.argtypes = [P, C.c_int, C.c_int, P, P, P, P, P, C.c_int, P]
+        L.mpsa_cheb_copy.argtypes = [P, C.c_int, C.c_int]
+        L.mpsa_cheb_apply_mpo.argtypes = [P, C.c_int, C.c_int]
+        L.mpsa_cheb_overlap.argtypes = [P, C.c_int, C.c_int, P]
+        L.mpsa_cheb_axpby.argtypes = [P, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int]
+        L.mpsa_cheb_compress_single_site.argtypes = [P, C.c_int, C.c_int, C.c_uint, C.c_int, C.c_double, P]
+        L.mpsa_cheb_compress.argtypes = [P, C.c_int, C.c_uint, C.c_int, C.c_double, P]
+        L.mpsa_cheb_moments.argtypes = [P, C.c_int, C.c_uint, C.c_int, C.c_double, P, P, P, P, P]
+        L.mpsa_cheb_site.argtypes = [P, C.c_int, C.c_int, C.c_int, P, P, P, P, P, P, P, P]
+        L.mpsa_cheb_stats.argtypes = [P, P, P]
+        L.mpsa_cheb_destroy.argtypes = [P]
         L.mpsa_dot.argtypes = [P, P, C.c_int64, C.c_int, P, P]
         L.mpsa_axpby.argtypes = [P, P, P, P, C.c_int64, C.c_int, P]
         _lib = L
@@ -578,6 +589,113 @@ class Dmrg:
         _check(lib().mpsa_dmrg_site(self.h, int(i), _ptr(nb), _ptr(ln), _ptr(q), _ptr(p), _ptr(r), _ptr(c), _ptr(o),
                                     _dptr(buf), _stream()))
         return Table(1, q.reshape(-1, 2), p.reshape(-1, 1), r, c, o), buf
+
+
+class Cheb:
+    """Chebyshev moments with variational MPS compression on the device
+    (ChebychevEngine::simulateSSCompress, src/container.hpp:2343-2470, and
+    applyMPO / compressSingleSite / compress / computeOverlap of
+    src/MPSfunctions.hpp) for nvec start vectors in one set of grouped
+    launches.  `states`: per vector a list of site Blocks (ns = 1; all real or
+    all complex); `alloc(np_array)` returns a device buffer with data_ptr()
+    (default: a torch tensor on the GPU) and `to_host(buf)` reads one back."""
+
+    def __init__(self, states, mpo, itk, h1_precision=F64, alloc=None, to_host=None):
+        import ctypes as C
+        if alloc is None:
+            import torch
+            alloc = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+            to_host = lambda b: b.cpu().numpy()  # noqa: E731
+        self._alloc, self._to_host = alloc, to_host
+        self.nvec, self.N = len(states), len(states[0])
+        self.cplx = any(B.dtype.kind == "c" for st in states for B in st)
+        tabs, bufs = [], []
+        for st in states:
+            for B in st:
+                if self.cplx and B.dtype.kind != "c":
+                    B = Blocks.from_table(B.ns, *[B.table()[k] for k in ("q", "p", "rows", "cols")],
+                                          B.pack().astype(np.complex128), np.complex128)
+                t = Table.from_blocks(B)
+                tabs.append(t)
+                bufs.append(alloc(t.pack(B)))
+        self._tabs, self._bufs = tabs, bufs
+        self._mpo = [MPOArg(W) for W in mpo]
+        self._itk = [list(x) for x in itk]
+        sites = (_Blocks * len(tabs))(*[t.s for t in tabs])
+        mpos = (_MPO * self.N)(*[m.s for m in self._mpo])
+        ptrs = (C.c_void_p * len(bufs))(*[_dptr(b) for b in bufs])
+        itk_all = np.ascontiguousarray(np.concatenate([_itk(x) for x in itk]), dtype=np.uint64)
+        d = np.array([len(x) for x in itk], dtype=np.int32)
+        self._L = lib()
+        h = C.c_void_p()
+        _check(lib().mpsa_cheb_create(C.byref(h), self.N, self.nvec, C.cast(sites, C.c_void_p),
+                                      C.cast(ptrs, C.c_void_p), C.cast(mpos, C.c_void_p), _ptr(itk_all), _ptr(d),
+                                      int(h1_precision), _stream()))
+        self.h = h
+
+    def __del__(self):
+        L = getattr(self, "_L", None)
+        if getattr(self, "h", None) and L is not None:
+            L.mpsa_cheb_destroy(self.h)
+            self.h = None
+
+    def _cv(self, a):
+        return a[0::2] + 1j * a[1::2] if self.cplx else a[0::2]
+
+    def copy(self, dst, src):
+        _check(lib().mpsa_cheb_copy(self.h, dst, src))
+
+    def apply_mpo(self, dst, src):
+        _check(lib().mpsa_cheb_apply_mpo(self.h, dst, src))
+
+    def overlap(self, a, b):
+        """computeOverlap(a, b) = <b|a> per vector."""
+        out = np.zeros(2 * self.nvec)
+        _check(lib().mpsa_cheb_overlap(self.h, a, b, _ptr(out)))
+        return self._cv(out)
+
+    def axpby(self, dst, xa, a, xb, b):
+        _check(lib().mpsa_cheb_axpby(self.h, dst, float(xa), a, float(xb), b))
+
+    def compress_single_site(self, dst, src, chimax, maxstep=5, delta=1e-10):
+        conv = np.zeros(self.nvec)
+        _check(lib().mpsa_cheb_compress_single_site(self.h, dst, src, int(chimax), int(maxstep), float(delta),
+                                                    _ptr(conv)))
+        return conv
+
+    def compress(self, slot, chimax, maxstep=5, delta=1e-10):
+        conv = np.zeros(self.nvec)
+        _check(lib().mpsa_cheb_compress(self.h, slot, int(chimax), int(maxstep), float(delta), _ptr(conv)))
+        return conv
+
+    def moments(self, nmom, chimax, ncmax=5, delta=1e-10):
+        mu = np.zeros(2 * self.nvec * nmom)
+        rc = np.zeros(2 * self.nvec * (2 * nmom - 1))
+        c1, c2, sec = np.zeros(self.nvec * nmom), np.zeros(self.nvec * nmom), np.zeros(1)
+        _check(lib().mpsa_cheb_moments(self.h, int(nmom), int(chimax), int(ncmax), float(delta), _ptr(mu), _ptr(rc),
+                                       _ptr(c1), _ptr(c2), _ptr(sec)))
+        return dict(moments=self._cv(mu).reshape(self.nvec, nmom), recons=self._cv(rc).reshape(self.nvec, -1),
+                    conv1=c1.reshape(self.nvec, nmom), conv2=c2.reshape(self.nvec, nmom), s=float(sec[0]))
+
+    def site(self, slot, vec, site):
+        nb, ln = np.zeros(1, np.int64), np.zeros(1, np.int64)
+        _check(lib().mpsa_cheb_site(self.h, slot, vec, site, _ptr(nb), _ptr(ln), None, None, None, None, None, None))
+        n = int(nb[0])
+        q, p = np.zeros(2 * max(n, 1), np.uint64), np.zeros(max(n, 1), np.uint32)
+        r, c, o = np.zeros(max(n, 1), np.int64), np.zeros(max(n, 1), np.int64), np.zeros(max(n, 1), np.int64)
+        buf = self._alloc(np.zeros(max(1, int(ln[0]))))
+        _check(lib().mpsa_cheb_site(self.h, slot, vec, site, _ptr(nb), _ptr(ln), _ptr(q), _ptr(p), _ptr(r), _ptr(c),
+                                    _ptr(o), _dptr(buf)))
+        t = Table(1, q[:2 * n].reshape(-1, 2), p[:n].reshape(-1, 1), r[:n], c[:n], o[:n], int(self.cplx))
+        return t.unpack(self._to_host(buf))
+
+    def state(self, slot, vec):
+        return [self.site(slot, vec, i) for i in range(self.N)]
+
+    def stats(self):
+        a, b = np.zeros(1, np.int64), np.zeros(1, np.int64)
+        _check(lib().mpsa_cheb_stats(self.h, _ptr(a), _ptr(b)))
+        return dict(plans_built=int(a[0]), pool_bytes=int(b[0]))
 
 
 def tridiag_eig(d, e, vectors=True):

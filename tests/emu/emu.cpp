@@ -822,6 +822,98 @@ hipError_t launch_fixphase(double* U, int64_t m, int64_t ldu, double* Vt, int64_
   });
   return hipSuccess;
 }
+// batched thin SVD (mpsa_factor.hip's contract): LAPACK dgesvd / zgesvd per
+// task, singular values descending, FixPhase on real data
+hipError_t launch_svd_batched(int cplx, const SvdTask* tasks, int ntasks, const double* M, int64_t m_plane, double* U,
+                              int64_t u_plane, double* Vt, int64_t vt_plane, double* S, double* W, int64_t w_plane,
+                              hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  submit(st, "svd batched", [=]() {
+    if (!chk_range(tasks, 0, ntasks - 1, "svd tasks")) return;
+    void* h = lapack_lib();
+    typedef void (*zgesvd_fn)(const char*, const char*, const int*, const int*, void*, const int*, double*, void*,
+                              const int*, void*, const int*, void*, const int*, double*, int*);
+    dgesvd_fn dg = lapack_dgesvd();
+    zgesvd_fn zg = h ? (zgesvd_fn)dlsym(h, "zgesvd_") : nullptr;
+    if (!dg || (cplx && !zg)) {
+      report("svd: no LAPACK library found");
+      return;
+    }
+    const int pl = cplx ? 2 : 1;
+    for (int ti = 0; ti < ntasks; ++ti) {
+      const SvdTask t = tasks[ti];
+      const int m = t.m, n = t.n, k = std::min(m, n), p = std::max(m, n);
+      if (m <= 0 || n <= 0) continue;
+      if (k > kSvdMaxR || t.lda < m) {
+        report("svd: task outside the kernel's limits");
+        return;
+      }
+      bool ok = true;
+      for (int q = 0; q < pl; ++q) {
+        ok = chk_range(M, t.a_off + q * m_plane, t.a_off + q * m_plane + (int64_t)t.lda * (n - 1) + m - 1, "svd M") && ok;
+        ok = chk_range(U, t.u_off + q * u_plane, t.u_off + q * u_plane + (int64_t)m * k - 1, "svd U") && ok;
+        ok = chk_range(Vt, t.vt_off + q * vt_plane, t.vt_off + q * vt_plane + (int64_t)k * n - 1, "svd Vt") && ok;
+        ok = chk_range(W, t.w_off + q * w_plane, t.w_off + q * w_plane + (int64_t)p * k + (int64_t)k * k - 1, "svd W") && ok;
+      }
+      ok = chk_range(S, t.s_off, t.s_off + k - 1, "svd S") && ok;
+      if (!ok) return;
+      int mm = m, nn = n, lda = m, ldu = m, ldvt = k, lwork = -1, info = 0;
+      if (!cplx) {
+        std::vector<double> a((size_t)m * n), u((size_t)m * k), vt((size_t)k * n), s((size_t)k);
+        for (int j = 0; j < n; ++j)
+          for (int i = 0; i < m; ++i) a[(size_t)i + (size_t)j * m] = M[t.a_off + i + (int64_t)j * t.lda];
+        double wq = 0;
+        dg("S", "S", &mm, &nn, a.data(), &lda, s.data(), u.data(), &ldu, vt.data(), &ldvt, &wq, &lwork, &info);
+        lwork = (int)wq;
+        std::vector<double> work((size_t)std::max(lwork, 1));
+        dg("S", "S", &mm, &nn, a.data(), &lda, s.data(), u.data(), &ldu, vt.data(), &ldvt, work.data(), &lwork,
+           &info);
+        for (int i = 0; i < k; ++i) {  // FixPhase
+          double best = 0.0;
+          int bi = 0;
+          for (int j = 0; j < n; ++j)
+            if (std::fabs(vt[(size_t)i + (size_t)j * k]) > best) {
+              best = std::fabs(vt[(size_t)i + (size_t)j * k]);
+              bi = j;
+            }
+          if (!(vt[(size_t)i + (size_t)bi * k] > 0.0)) {
+            for (int j = 0; j < n; ++j) vt[(size_t)i + (size_t)j * k] = -vt[(size_t)i + (size_t)j * k];
+            for (int r = 0; r < m; ++r) u[(size_t)r + (size_t)i * m] = -u[(size_t)r + (size_t)i * m];
+          }
+        }
+        for (size_t e = 0; e < u.size(); ++e) U[t.u_off + (int64_t)e] = u[e];
+        for (size_t e = 0; e < vt.size(); ++e) Vt[t.vt_off + (int64_t)e] = vt[e];
+        for (int i = 0; i < k; ++i) S[t.s_off + i] = s[(size_t)i];
+      } else {
+        std::vector<double> a((size_t)2 * m * n), u((size_t)2 * m * k), vt((size_t)2 * k * n), s((size_t)k),
+            rwork((size_t)5 * k + 8);
+        for (int j = 0; j < n; ++j)
+          for (int i = 0; i < m; ++i) {
+            a[2 * ((size_t)i + (size_t)j * m)] = M[t.a_off + i + (int64_t)j * t.lda];
+            a[2 * ((size_t)i + (size_t)j * m) + 1] = M[t.a_off + m_plane + i + (int64_t)j * t.lda];
+          }
+        double wq[2] = {0, 0};
+        zg("S", "S", &mm, &nn, a.data(), &lda, s.data(), u.data(), &ldu, vt.data(), &ldvt, wq, &lwork, rwork.data(),
+           &info);
+        lwork = (int)wq[0];
+        std::vector<double> work((size_t)2 * std::max(lwork, 1));
+        zg("S", "S", &mm, &nn, a.data(), &lda, s.data(), u.data(), &ldu, vt.data(), &ldvt, work.data(), &lwork,
+           rwork.data(), &info);
+        for (size_t e = 0; e < (size_t)m * k; ++e) {
+          U[t.u_off + (int64_t)e] = u[2 * e];
+          U[t.u_off + u_plane + (int64_t)e] = u[2 * e + 1];
+        }
+        for (size_t e = 0; e < (size_t)k * n; ++e) {
+          Vt[t.vt_off + (int64_t)e] = vt[2 * e];
+          Vt[t.vt_off + vt_plane + (int64_t)e] = vt[2 * e + 1];
+        }
+        for (int i = 0; i < k; ++i) S[t.s_off + i] = s[(size_t)i];
+      }
+      if (info != 0) report("svd: LAPACK did not converge");
+    }
+  });
+  return hipSuccess;
+}
 int gemm_max_grid(int, int) { return 512; }
 
 }  // namespace mpsa
