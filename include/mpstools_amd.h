@@ -388,6 +388,13 @@ typedef struct mpsa_cheb mpsa_cheb;
 int mpsa_cheb_create(mpsa_cheb** out, int nsites, int nvec, const mpsa_blocks* sites, const double* const* dA,
                      const mpsa_mpo* W, const uint64_t* itokey, const int* d, int h1_precision, mpsa_stream stream);
 int mpsa_cheb_copy(mpsa_cheb* c, int dst, int src);
+/* orthonormalize(mps, "rl") (src/MPSfunctions.hpp:1757-1793) of a slot:
+ * prepareSite "r" from the last site to site 1 (batched sector SVDs), then
+ * each vector's first site divided by its norm; the state is unchanged up to
+ * its norm (and sign).  With mpsa_fill_gauss this makes seeded, normalised
+ * inputs on the device (in place of prepareMatrix's host SVDs,
+ * src/MPSfunctions.hpp:21-269). */
+int mpsa_cheb_orthonormalize(mpsa_cheb* c, int slot);
 int mpsa_cheb_apply_mpo(mpsa_cheb* c, int dst, int src);
 int mpsa_cheb_overlap(mpsa_cheb* c, int a, int b, double* out);
 int mpsa_cheb_axpby(mpsa_cheb* c, int dst, double xa, int a, double xb, int b);
@@ -406,6 +413,13 @@ int mpsa_cheb_destroy(mpsa_cheb* c);
  * X*Y = sum conj(Y) X (src/BlockTensor.hpp:520-544), computed in FP64 with a
  * fixed-order two-level reduction; result written to host *out (re, im). */
 int mpsa_dot(const double* dx, const double* dy, int64_t n, int is_complex, double* out, mpsa_stream stream);
+/* Seeded N(0, 1) entries (no reference counterpart: the reference's
+ * initialisers are time-seeded, src/BlockTensor.hpp:394-404):
+ * x[i] = sqrt(-2 ln u_{2i}) cos(2 pi u_{2i+1}) with u_j = ((SplitMix64(seed +
+ * (j + 1) * 0x9e3779b97f4a7c15) >> 11) + 1) * 2^-53, ln and cos evaluated by
+ * fixed fma series, so a host restatement of the same steps reproduces the
+ * bits. */
+int mpsa_fill_gauss(double* dx, int64_t n, uint64_t seed, mpsa_stream stream);
 /* y = a*x + b*y (FP64; complex when is_complex, a and b complex as 2 doubles). */
 int mpsa_axpby(const double* a, const double* dx, const double* b, double* dy, int64_t n, int is_complex,
                mpsa_stream stream);

@@ -1033,6 +1033,24 @@ DMps compress(Eng& E, DMps& mps, unsigned chimax, int maxstep, double delta, std
   return comp;
 }
 
+// orthonormalize(mps, "rl") (src/MPSfunctions.hpp:1757-1793): prepareSite
+// "r" from the last site to site 1, then the first site divided by each
+// vector's norm (the reference's prepareMatrix into the left dummy keeps
+// V^dagger there; this differs at most by the sign of the whole state).
+void orthonormalize_rl(Eng& E, DMps& M) {
+  for (int s = E.N - 1; s > 0; --s) prepare_site(E, M, s, false, false);
+  const std::vector<cplx> nrm = overlap(E, M, M);
+  const HTab& T = M.A[0];
+  std::vector<Copy2D> cp;
+  for (int64_t i = 0; i < T.n(); ++i) {
+    const double n2 = nrm[(size_t)ns_of(T.q[2 * i])].real();
+    if (!(n2 > 0.0)) throw HErr(MPSA_EINVAL, "cheb: orthonormalize of a vector with zero norm");
+    add_copy(cp, E, T.off[(size_t)i], T.total, T.off[(size_t)i], T.total, T.rows[(size_t)i], T.cols[(size_t)i],
+             T.rows[(size_t)i], T.rows[(size_t)i], 1.0 / std::sqrt(n2));
+  }
+  run_copies(E, cp, M.d[0].p, M.d[0].p, nullptr);
+}
+
 DMps& slot_at(Eng& E, int i, bool must_exist) {
   if (i < 0 || i >= (int)E.slot.size()) throw HErr(MPSA_EINVAL, "cheb: slot index out of range");
   if (must_exist && !E.slot[(size_t)i].valid()) throw HErr(MPSA_EINVAL, "cheb: empty slot");
@@ -1141,6 +1159,19 @@ int mpsa_cheb_copy(mpsa_cheb* E, int dst, int src) {
     if (!E) throw HErr(MPSA_EINVAL, "cheb: null engine");
     DMps c = copy_mps(*E, slot_at(*E, src, true));
     slot_at(*E, dst, false) = std::move(c);
+    E->sync();
+    return MPSA_OK;
+  } catch (const HErr& e) {
+    return fail(e, e.code);
+  } catch (const std::exception& e) {
+    return fail(e, MPSA_EINVAL);
+  }
+}
+
+int mpsa_cheb_orthonormalize(mpsa_cheb* E, int slot) {
+  try {
+    if (!E) throw HErr(MPSA_EINVAL, "cheb: null engine");
+    orthonormalize_rl(*E, slot_at(*E, slot, true));
     E->sync();
     return MPSA_OK;
   } catch (const HErr& e) {

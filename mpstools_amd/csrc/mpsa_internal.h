@@ -73,6 +73,9 @@ hipError_t launch_repack(int prec, const Repack* tasks, int ntasks, int chunks_p
 hipError_t launch_fill(double* p, int64_t n, double v, hipStream_t st);
 // p[i] = uniform in [-1, 1) from SplitMix64 of seed + i * golden gamma
 hipError_t launch_fill_hash(double* p, int64_t n, uint64_t seed, hipStream_t st);
+// p[i] = sqrt(-2 ln u_{2i}) cos(2 pi u_{2i+1}), u_j = ((SplitMix64(seed + (j + 1)
+// * golden gamma) >> 11) + 1) 2^-53: seeded N(0, 1) entries, host-reproducible
+hipError_t launch_fill_gauss(double* p, int64_t n, uint64_t seed, hipStream_t st);
 // p[i * stride] = v for i < n
 hipError_t launch_fill_strided(double* p, int64_t n, int64_t stride, double v, hipStream_t st);
 hipError_t launch_zero_ranges(const Range* r, int nr, double* base, int64_t vs, int nvec, hipStream_t st);

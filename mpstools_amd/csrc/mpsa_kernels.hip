@@ -1663,6 +1663,56 @@ __global__ void fill_hash_kernel(double* __restrict__ p, int64_t n, uint64_t see
     p[i] = hash_unit(seed, (uint64_t)i);
 }
 
+// Gaussian entries for seeded inputs: SplitMix64 (as hash_unit) -> two
+// uniforms in (0, 1] -> Box-Muller, z_i = sqrt(-2 ln u_{2i}) cos(2 pi u_{2i+1}).
+// ln and cos are this library's own series (atanh series of ln m on
+// [sqrt(1/2), sqrt(2)); Taylor series of sin / cos on [-pi/4, pi/4] after an
+// exact octant reduction), every multiply-add an explicit fma and every other
+// step a correctly rounded IEEE operation, so a host build of the same steps
+// gives the same bits.
+__device__ __forceinline__ double unit_open(uint64_t seed, uint64_t i) {
+  uint64_t z = seed + (i + 1) * 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  z ^= z >> 31;
+  return ((double)(z >> 11) + 1.0) * 0x1.0p-53;
+}
+__device__ __forceinline__ double ln_unit(double u) {
+  int e = 0;
+  double m = frexp(u, &e);
+  if (m < 0x1.6a09e667f3bcdp-1) {
+    m = 2.0 * m;
+    e -= 1;
+  }
+  const double s = (m - 1.0) / (m + 1.0), z = s * s;
+  double p = 1.0 / 23.0;
+  for (int k = 10; k >= 0; --k) p = fma(p, z, 1.0 / (double)(2 * k + 1));
+  return fma((double)e, 0x1.62e42fefa39efp-1, 2.0 * s * p);
+}
+__device__ __forceinline__ double cos_2pi(double u) {
+  const double x = u > 0.5 ? u - 1.0 : u;
+  const double a4 = 4.0 * fabs(x);
+  const double q = a4 < 0.5 ? 0.0 : (a4 < 1.5 ? 1.0 : 2.0);
+  const double t = (a4 - q) * 0x1.921fb54442d18p+0, t2 = t * t;
+  double f[23];
+  f[0] = 1.0;
+  for (int j = 1; j < 23; ++j) f[j] = f[j - 1] * (double)j;
+  if (q == 1.0) {
+    double sn = -1.0 / f[21];
+    for (int k = 9; k >= 0; --k) sn = fma(sn, t2, ((k & 1) ? -1.0 : 1.0) / f[2 * k + 1]);
+    return -(t * sn);
+  }
+  double c = -1.0 / f[22];
+  for (int k = 10; k >= 0; --k) c = fma(c, t2, ((k & 1) ? -1.0 : 1.0) / f[2 * k]);
+  return q == 0.0 ? c : -c;
+}
+__global__ void fill_gauss_kernel(double* __restrict__ p, int64_t n, uint64_t seed) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double u1 = unit_open(seed, 2 * (uint64_t)i), u2 = unit_open(seed, 2 * (uint64_t)i + 1);
+    p[i] = sqrt(-2.0 * ln_unit(u1)) * cos_2pi(u2);
+  }
+}
+
 __global__ void fill_strided_kernel(double* __restrict__ p, int64_t n, int64_t stride, double v) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
     p[i * stride] = v;
@@ -2122,6 +2172,14 @@ hipError_t launch_fill_hash(double* p, int64_t n, uint64_t seed, hipStream_t st)
   int64_t blocks = (n + 255) / 256;
   int grid = (int)(blocks < 16384 ? blocks : 16384);
   fill_hash_kernel<<<grid, 256, 0, st>>>(p, n, seed);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_gauss(double* p, int64_t n, uint64_t seed, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  int64_t blocks = (n + 255) / 256;
+  int grid = (int)(blocks < 16384 ? blocks : 16384);
+  fill_gauss_kernel<<<grid, 256, 0, st>>>(p, n, seed);
   return hipGetLastError();
 }
 

@@ -1481,6 +1481,13 @@ int mpsa_dot(const double* dx, const double* dy, int64_t n, int is_complex, doub
   })
 }
 
+int mpsa_fill_gauss(double* dx, int64_t n, uint64_t seed, mpsa_stream stream) {
+  MPSA_TRY({
+    if (n < 0 || (n > 0 && !dx)) throw Error(MPSA_EINVAL, "fill_gauss: bad arguments");
+    HIPCHK(launch_fill_gauss(dx, n, seed, (hipStream_t)stream));
+  })
+}
+
 int mpsa_axpby(const double* a, const double* dx, const double* b, double* dy, int64_t n, int is_complex,
                mpsa_stream stream) {
   MPSA_TRY({

@@ -101,6 +101,8 @@ def lib():
         L.mpsa_tridiag_eig.argtypes = [C.c_int, P, P, P]
         L.mpsa_cheb_create.argtypes = [P, C.c_int, C.c_int, P, P, P, P, P, C.c_int, P]
         L.mpsa_cheb_copy.argtypes = [P, C.c_int, C.c_int]
+        L.mpsa_cheb_orthonormalize.argtypes = [P, C.c_int]
+        L.mpsa_fill_gauss.argtypes = [P, C.c_int64, C.c_uint64, P]
         L.mpsa_cheb_apply_mpo.argtypes = [P, C.c_int, C.c_int]
         L.mpsa_cheb_overlap.argtypes = [P, C.c_int, C.c_int, P]
         L.mpsa_cheb_axpby.argtypes = [P, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int]
@@ -648,6 +650,10 @@ class Cheb:
     def apply_mpo(self, dst, src):
         _check(lib().mpsa_cheb_apply_mpo(self.h, dst, src))
 
+    def orthonormalize(self, slot=0):
+        """orthonormalize(mps, "rl") on the device (right-normalised, unit norm)."""
+        _check(lib().mpsa_cheb_orthonormalize(self.h, slot))
+
     def overlap(self, a, b):
         """computeOverlap(a, b) = <b|a> per vector."""
         out = np.zeros(2 * self.nvec)
@@ -696,6 +702,73 @@ class Cheb:
         a, b = np.zeros(1, np.int64), np.zeros(1, np.int64)
         _check(lib().mpsa_cheb_stats(self.h, _ptr(a), _ptr(b)))
         return dict(plans_built=int(a[0]), pool_bytes=int(b[0]))
+
+
+def fill_gauss(buf, seed, n=None):
+    """Seeded N(0, 1) entries on the device (mpsa_fill_gauss)."""
+    n = buf.numel() if n is None else n
+    _check(lib().mpsa_fill_gauss(_dptr(buf), int(n), int(seed) & ((1 << 64) - 1), _stream()))
+
+
+def random_mps_tables(L, itk, Ntot, D, cplx=False):
+    """Block tables of a seeded random MPS with the sector profile of
+    synth.bond_dims (no data): one Table per site."""
+    from .synth import bond_dims
+    from .blocks import qadd
+    dims = bond_dims(L, itk, Ntot, D)
+    tabs = []
+    for i in range(L):
+        q, p, r, c = [], [], [], []
+        for a in sorted(dims[i]):
+            for s, qs in enumerate(itk[i]):
+                b = qadd(a, qs)
+                if b in dims[i + 1]:
+                    q.append((a, b))
+                    p.append(s)
+                    r.append(dims[i][a])
+                    c.append(dims[i + 1][b])
+        order = sorted(range(len(q)), key=lambda k: (q[k][0], q[k][1], p[k]))
+        t = Table.from_keys(1, np.array([q[k] for k in order], np.uint64).reshape(-1, 2),
+                            np.array([p[k] for k in order], np.uint32).reshape(-1, 1),
+                            [r[k] for k in order], [c[k] for k in order])
+        tabs.append(t.as_complex() if cplx else t)
+    return tabs
+
+
+class ChebRandom(Cheb):
+    """A Cheb engine whose nvec start vectors are seeded random MPS generated
+    on the device: block tables from the sector profile, entries from
+    mpsa_fill_gauss (seed + 1000 * vector + site; both planes for complex),
+    then orthonormalize("rl") on the device."""
+
+    def __init__(self, L, itk, Ntot, D, nvec, mpo, seed, cplx=False, h1_precision=F64):
+        import ctypes as C
+        import torch
+        self._alloc = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()  # noqa: E731
+        self._to_host = lambda b: b.cpu().numpy()  # noqa: E731
+        self.nvec, self.N, self.cplx = nvec, L, bool(cplx)
+        base = random_mps_tables(L, itk, Ntot, D, cplx)
+        tabs, bufs = [], []
+        for v in range(nvec):
+            for i, t in enumerate(base):
+                b = torch.empty(max(1, t.length), dtype=torch.float64, device="cuda")
+                fill_gauss(b, seed + 1000 * v + i, t.length)
+                tabs.append(t)
+                bufs.append(b)
+        self._tabs, self._bufs = tabs, bufs
+        self._mpo = [MPOArg(W) for W in mpo]
+        sites = (_Blocks * len(tabs))(*[t.s for t in tabs])
+        mpos = (_MPO * L)(*[m.s for m in self._mpo])
+        ptrs = (C.c_void_p * len(bufs))(*[_dptr(b) for b in bufs])
+        itk_all = np.ascontiguousarray(np.concatenate([_itk(x) for x in itk]), dtype=np.uint64)
+        d = np.array([len(x) for x in itk], dtype=np.int32)
+        self._L = lib()
+        h = C.c_void_p()
+        _check(lib().mpsa_cheb_create(C.byref(h), L, nvec, C.cast(sites, C.c_void_p), C.cast(ptrs, C.c_void_p),
+                                      C.cast(mpos, C.c_void_p), _ptr(itk_all), _ptr(d), int(h1_precision), _stream()))
+        self.h = h
+        self._bufs = None  # the engine holds its own copies
+        self.orthonormalize(0)
 
 
 def tridiag_eig(d, e, vectors=True):

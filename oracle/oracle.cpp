@@ -200,6 +200,53 @@ extern "C" {
 
 const char* orc_last_error() { return g_err.c_str(); }
 const char* orc_blas_name() { return blas().name.c_str(); }
+// Seeded Gaussian entries as the library's generator defines them (SplitMix64
+// of seed + (j + 1) * golden gamma, top 53 bits + 1 over 2^53 -> u_j in (0, 1];
+// x_i = sqrt(-2 ln u_{2i}) cos(2 pi u_{2i+1})), with ln from the atanh series
+// ln m = 2 s sum_k s^2k / (2k + 1), s = (m - 1) / (m + 1), m in [sqrt(1/2),
+// sqrt(2)), and cos(2 pi u) from the Taylor series of cos / sin on
+// [-pi/4, pi/4] after folding u into an octant; every multiply-add an fma.
+// Restated here for the host so the device bits can be checked.
+static double orc_unit(uint64_t seed, uint64_t j) {
+  uint64_t z = seed + (j + 1) * 0x9e3779b97f4a7c15ull;
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  z ^= z >> 31;
+  return ((double)(z >> 11) + 1.0) * 0x1.0p-53;
+}
+static double orc_ln(double u) {
+  int e = 0;
+  double m = std::frexp(u, &e);
+  if (m < 0x1.6a09e667f3bcdp-1) {
+    m = 2.0 * m;
+    e -= 1;
+  }
+  const double s = (m - 1.0) / (m + 1.0), z = s * s;
+  double acc = 1.0 / 23.0;
+  for (int k = 10; k >= 0; --k) acc = std::fma(acc, z, 1.0 / (double)(2 * k + 1));
+  return std::fma((double)e, 0x1.62e42fefa39efp-1, 2.0 * s * acc);
+}
+static double orc_cos2pi(double u) {
+  const double x = u > 0.5 ? u - 1.0 : u;
+  const double a4 = 4.0 * std::fabs(x);
+  const double q = a4 < 0.5 ? 0.0 : (a4 < 1.5 ? 1.0 : 2.0);
+  const double t = (a4 - q) * 0x1.921fb54442d18p+0, t2 = t * t;
+  double fact[23];
+  fact[0] = 1.0;
+  for (int j = 1; j < 23; ++j) fact[j] = fact[j - 1] * (double)j;
+  if (q == 1.0) {  // cos(pi/2 + t) = -sin t
+    double sn = -1.0 / fact[21];
+    for (int k = 9; k >= 0; --k) sn = std::fma(sn, t2, ((k & 1) ? -1.0 : 1.0) / fact[2 * k + 1]);
+    return -(t * sn);
+  }
+  double c = -1.0 / fact[22];
+  for (int k = 10; k >= 0; --k) c = std::fma(c, t2, ((k & 1) ? -1.0 : 1.0) / fact[2 * k]);
+  return q == 0.0 ? c : -c;  // cos(pi + t) = -cos t
+}
+void orc_fill_gauss(double* p, int64_t n, uint64_t seed) {
+  for (int64_t i = 0; i < n; ++i)
+    p[i] = std::sqrt(-2.0 * orc_ln(orc_unit(seed, 2 * (uint64_t)i))) * orc_cos2pi(orc_unit(seed, 2 * (uint64_t)i + 1));
+}
 // Threads of the BLAS library (the CPU baseline's 1-thread and all-core runs);
 // the charge loops take their thread count per call.
 int orc_set_blas_threads(int n) {

@@ -54,6 +54,7 @@ def lib():
                                           C.c_int, C.c_int, C.c_int, P, C.c_int, P, P, P]
         L.orc_tridiag.argtypes = [C.c_int, P, P, P]
         L.orc_set_blas_threads.argtypes = [C.c_int]
+        L.orc_fill_gauss.argtypes = [P, C.c_int64, C.c_uint64]
         L.orc_dgemm.argtypes = [C.c_int, C.c_int, C.c_int, P, P, P]
         _lib = L
     return _lib
@@ -75,6 +76,13 @@ def blas_name() -> str:
 def set_blas_threads(n: int) -> bool:
     """Threads of the loaded BLAS (MKL or OpenBLAS); False when it has no such control."""
     return lib().orc_set_blas_threads(int(n)) == 0
+
+
+def gauss(n, seed):
+    """The library's seeded N(0, 1) sequence (mpsa_fill_gauss), computed on the host."""
+    out = np.zeros(int(n))
+    lib().orc_fill_gauss(_p(out), int(n), int(seed) & ((1 << 64) - 1))
+    return out
 
 
 def dgemm(A, B):

@@ -1,7 +1,9 @@
 """Shared checks of the device Chebyshev engine (native.Cheb over the C ABI)
 against the oracle's restatement (oracle/pycheb.py), run on the CPU emulation
 of the device layer (tests/test_emulated.py) and on the GPU
-(tests/test_gpu_cheb_engine.py).  Inputs only here; each check states its bar.
+(tests/test_gpu_cheb_engine.py).  Inputs only here; each check states its bar.  Cases are chosen so that no
+charge sector of a compression start is rank deficient (see the GPU test's
+docstring for why the reference's result is not reproducible there).
 """
 from __future__ import annotations
 

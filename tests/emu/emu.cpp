@@ -622,6 +622,17 @@ hipError_t launch_fill_hash(double* p, int64_t n, uint64_t seed, hipStream_t st)
   });
   return hipSuccess;
 }
+// seeded Gaussian fill: the oracle's host restatement (oracle/oracle.cpp,
+// compiled into this emulation library)
+extern "C" void orc_fill_gauss(double* p, int64_t n, uint64_t seed);
+hipError_t launch_fill_gauss(double* p, int64_t n, uint64_t seed, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  submit(st, "fill_gauss", [=]() {
+    if (!chk_range(p, 0, n - 1, "fill_gauss")) return;
+    orc_fill_gauss(p, n, seed);
+  });
+  return hipSuccess;
+}
 hipError_t launch_fill_strided(double* p, int64_t n, int64_t stride, double v, hipStream_t st) {
   if (n <= 0) return hipSuccess;
   submit(st, "fill_strided", [=]() {

@@ -503,3 +503,40 @@ def test_emulated_chebyshev_engine_matches_oracle(lib, oracle, model, L, nvec, c
     r = eng3.moments(6, 10000, 2)
     assert np.abs(r["moments"] - chebcase.dense_moments(sts, mpo, itk, 6)).max() <= 1e-10
     del eng, eng2, eng3
+
+
+def test_emulated_device_canonicalisation(lib, oracle):
+    """A seeded random MPS with the sector profile's tables and entries from
+    mpsa_fill_gauss (the device bits equal the host generator's), brought to
+    right-normalised form by orthonormalize("rl") on the device: the state is
+    unchanged up to its norm and sign, has unit norm, and every site after the
+    first is right-normalised (B B^H = 1 per left charge)."""
+    import chebcase
+    import pycheb
+    from mpstools_amd import native
+    L = 6
+    itk = [models.itokey_u1xu1()] * L
+    tabs = native.random_mps_tables(L, itk, models.pack_charge(3, 3), 12)
+    blocks = []
+    for i, t in enumerate(tabs):
+        b = DevBuf(lib, np.zeros(t.length))
+        native.fill_gauss(b, 7 + i, t.length)
+        data = b.host()
+        assert np.array_equal(data, oracle.gauss(t.length, 7 + i))
+        blocks.append(t.unpack(data))
+    al, th = _emu_alloc(lib)
+    eng = native.Cheb([blocks], models.hubbard_mpo(L), itk, alloc=al, to_host=th)
+    v0 = pycheb.to_dense(pycheb.MPS(blocks, itk))
+    eng.orthonormalize(0)
+    got = eng.state(0, 0)
+    v1 = pycheb.to_dense(pycheb.MPS(got, itk))
+    assert abs(np.linalg.norm(v1) - 1.0) <= 1e-12
+    assert np.abs(np.abs(np.vdot(v1, v0)) / np.linalg.norm(v0) - 1.0) <= 1e-12
+    for s in range(1, L):
+        secs = {}
+        for k, blk in got[s].items():
+            secs.setdefault(k[0], []).append(blk)
+        for q, bl in secs.items():
+            X = np.concatenate(bl, axis=1)
+            assert np.abs(X @ X.T - np.eye(X.shape[0])).max() <= 1e-12
+    del eng

@@ -10,8 +10,16 @@ batched Jacobi kernel here and LAPACK in the oracle, so agreement is to
 rounding); untruncated moments equal the dense Chebyshev moments within
 1e-10; with the one-site contraction in FP32 (complex64) the moments agree
 with FP64 within 1e-5.  A mid-size chain (Hubbard L = 16, D = 48, 4 vectors)
-checks |mu_n| <= mu_0 (spectrum in [-1, 1]) and the reconstructed doubled
-moments against the directly computed ones.
+checks |mu_n| <= mu_0 (spectrum in [-1, 1]) and the bond dimensions.
+
+Parity is unpinned where the compression start (applyMPO of the truncated
+state, canonicalised) has charge sectors with singular values at rounding
+level: prepareSite keeps all min(m, n) directions of a sector
+(src/MPSfunctions.hpp:60-75), so the variational space then contains basis
+vectors fixed only by how the SVD completes a rank-deficient basis, and the
+reference's own result depends on its LAPACK build (seen on XXZ L = 8 at
+chimax = 20: singular values 2e-36 and 5e-19 at two bonds, moments differing
+by 1e-5 between two LAPACK-based runs).  The cases here are well conditioned.
 """
 import numpy as np
 import pytest
@@ -27,7 +35,7 @@ def _say(*a):
 
 
 @pytest.mark.parametrize("model,L,nvec,chimax,cplx", [("hubbard", 4, 2, 24, False), ("xxz", 6, 3, 15, False),
-                                                      ("hubbard", 4, 2, 24, True), ("xxz", 8, 2, 20, True)])
+                                                      ("hubbard", 4, 2, 24, True), ("xxz", 8, 2, 40, True)])
 def test_cheb_engine_steps_and_moments_match_oracle(gpu, oracle, model, L, nvec, chimax, cplx):
     _say("cheb engine", model, L, nvec, chimax, cplx)
     sts, itk = chebcase.states(model, L, nvec, 40 + L, cplx=cplx)
