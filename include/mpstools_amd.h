@@ -237,6 +237,33 @@ int mpsa_lanczos(mpsa_plan* plan, const double* dpsi0, double* dstate, int itmax
 int mpsa_lanczos_batched(mpsa_plan* plan, const double* dpsi0, int64_t stride, int nvec, double* dstates, int itmax,
                          double delta, int target, double* energies, int cap, int* nit, int* nritz, int use_graph,
                          mpsa_stream stream);
+/* mpsa_lanczos_batched with the options of the reference's projected run
+ * (TSLanzcosEngine::simulate(itmax, delta, testenergy, state, delta_ortho),
+ * src/lanzcos.hpp:281-414, called by DMRGengine::doStepExcited,
+ * src/container.hpp:340-347):
+ *   nproj > 0: the start vector and every new recurrence vector lose their
+ *     component along nproj states, state p at dproj + p*proj_stride (and,
+ *     with proj_vstride > 0, vector v of the batch has its own set at
+ *     + v*proj_vstride; proj_vstride = 0 shares one set).  The states need
+ *     not be normalised or orthogonal: they are orthonormalised once on the
+ *     device (FP64, classical Gram-Schmidt twice); for one state this is the
+ *     reference's A - s (s*A) / (s*s).  On a complex plan the complex span of
+ *     each state is projected out (s and i s).  A state that is zero or
+ *     linearly dependent on the earlier ones (relative norm below 1e-12
+ *     after the projection) is an error.  The reference squeezes its state
+ *     (drops blocks whose entries are all <= 1e-15); such blocks change the
+ *     projection by at most 1e-15 relative and are kept here.
+ *   reortho = 1: after every recurrence step the new vector is
+ *     reorthogonalised against all Krylov vectors of its run (FP64 classical
+ *     Gram-Schmidt, two passes, coefficients on the device, inside the
+ *     replayed graph).  The reference's delta_ortho Gram-Schmidt is commented
+ *     out (src/lanzcos.hpp:342-366); this is the option in its place, off by
+ *     default (plain three-term recurrence, as the reference runs).
+ * nproj = 0 and reortho = 0 is mpsa_lanczos_batched. */
+int mpsa_lanczos_batched_ex(mpsa_plan* plan, const double* dpsi0, int64_t stride, int nvec, double* dstates, int itmax,
+                            double delta, int target, double* energies, int cap, int* nit, int* nritz, int use_graph,
+                            const double* dproj, int nproj, int64_t proj_stride, int64_t proj_vstride, int reortho,
+                            mpsa_stream stream);
 /* ---- Chebyshev moments on one bond (the recurrence of
  * ChebychevEngine::simulateSSCompress, src/container.hpp:2343-2470, with the
  * plan's exact H_eff in place of the compressed MPO application) ----

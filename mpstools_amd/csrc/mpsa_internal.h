@@ -7,6 +7,8 @@
 #include <limits>
 #include <vector>
 
+struct mpsa_plan;
+
 namespace mpsa {
 
 // One output block of a grouped GEMM stage: C (m x n, ld = m) at c_off.
@@ -88,6 +90,8 @@ int gemm_max_grid(int prec_in, int tile_class);
 // plans' own invariants (segments with k > 0, non-empty tasks, longest first,
 // tiles numbered in task order for the chosen class).
 void finalize_gemm_list(std::vector<GTask>& tasks, std::vector<GSeg>& segs, int prec, int* tiles, int* tile_class);
+// whether the plan's operands are complex128 (planar)
+bool plan_complex(const mpsa_plan* P);
 // dense block copies of the split (column-major; see copy2d_kernel)
 struct Copy2D {
   int64_t src_off, dst_off;
@@ -121,6 +125,13 @@ hipError_t launch_lz_update(const double* Y, const double* X, double* const* bas
                             const double* ebuf, int ld, const int* step, double* W, int64_t n, int64_t vs, int nvec,
                             hipStream_t st);
 hipError_t launch_lz_step(int* step, hipStream_t st);
+// W_v -= sum_{i < nb} <Q_i(v), W_v> Q_i(v) with Q_i(v) = Q[i] + v * qvs,
+// nb = (cnt ? *cnt : 0) + add (at most nbmax); partial holds
+// nvec * nbmax * nparts doubles, c nvec * nbmax
+hipError_t launch_project_out(double* W, double* const* Q, int64_t qvs, const int* cnt, int add, int nbmax, int64_t n,
+                              int64_t vs, int nvec, double* partial, int nparts, double* c, hipStream_t st);
+// X_v *= 1 / sqrt(nrm2[v])
+hipError_t launch_scale_rsqrt(double* X, const double* nrm2, int64_t n, int64_t vs, int nvec, hipStream_t st);
 hipError_t launch_lz_combine(double* const* basis, const double* c, int ldc, const int* m, double* out, int64_t n,
                              int64_t vs, int nvec, hipStream_t st);
 

@@ -1849,6 +1849,75 @@ __global__ void lz_update_kernel(const double* __restrict__ Y, const double* __r
   }
 }
 
+// Projection of the recurrence vector against a set of orthonormal vectors
+// Q_0 .. Q_{nb-1}; vector v of Q_i lies at Q[i] + v * qvs (qvs = 0: one set
+// shared by all vectors).  nb = (*cnt) + add when cnt is given (the Krylov
+// basis grows with the device step counter), else add; at most nbmax.
+//   c_{v,i} = <Q_i(v), W_v>   (fixed partition and fixed-order sums)
+//   W_v    -= sum_i c_{v,i} Q_i(v)
+// The dot products run eight basis vectors per pass over W.
+__global__ __launch_bounds__(256) void multidot_partial_kernel(const double* __restrict__ W,
+                                                               double* const* __restrict__ Q, int64_t qvs,
+                                                               const int* __restrict__ cnt, int add, int nbmax,
+                                                               int64_t n, int64_t vs, double* __restrict__ partial) {
+  __shared__ double sr[8][256];
+  const int v = blockIdx.y;
+  const int nb = min((cnt ? *cnt : 0) + add, nbmax);
+  const double* wv = W + (int64_t)v * vs;
+  for (int i0 = 0; i0 < nb; i0 += 8) {
+    const int g = min(8, nb - i0);
+    const double* q[8];
+    for (int t = 0; t < 8; ++t) q[t] = Q[i0 + min(t, g - 1)] + (int64_t)v * qvs;
+    double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x; k < n; k += (int64_t)gridDim.x * 256) {
+      const double w = wv[k];
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+        if (t < g) a[t] += q[t][k] * w;
+    }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) sr[t][threadIdx.x] = a[t];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+      if ((int)threadIdx.x < w)
+        for (int t = 0; t < g; ++t) sr[t][threadIdx.x] += sr[t][threadIdx.x + w];
+      __syncthreads();
+    }
+    if ((int)threadIdx.x < g) partial[((int64_t)v * nbmax + i0 + threadIdx.x) * gridDim.x + blockIdx.x] = sr[threadIdx.x][0];
+    __syncthreads();
+  }
+}
+__global__ void multidot_final_kernel(const double* __restrict__ partial, int nparts, const int* __restrict__ cnt,
+                                      int add, int nbmax, double* __restrict__ c) {
+  const int v = blockIdx.x;
+  const int nb = min((cnt ? *cnt : 0) + add, nbmax);
+  for (int i = threadIdx.x; i < nb; i += blockDim.x) {
+    double r = 0.0;
+    for (int p = 0; p < nparts; ++p) r += partial[((int64_t)v * nbmax + i) * nparts + p];
+    c[(int64_t)v * nbmax + i] = r;
+  }
+}
+__global__ void multisub_kernel(double* __restrict__ W, double* const* __restrict__ Q, int64_t qvs,
+                                const int* __restrict__ cnt, int add, int nbmax, const double* __restrict__ c, int64_t n,
+                                int64_t vs) {
+  const int v = blockIdx.y;
+  const int nb = min((cnt ? *cnt : 0) + add, nbmax);
+  double* wv = W + (int64_t)v * vs;
+  const double* cv = c + (int64_t)v * nbmax;
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
+    double t = wv[k];
+    for (int i = 0; i < nb; ++i) t -= cv[i] * Q[i][(int64_t)v * qvs + k];
+    wv[k] = t;
+  }
+}
+// X_v *= 1 / sqrt(nrm2[v])
+__global__ void scale_rsqrt_kernel(double* __restrict__ X, const double* __restrict__ nrm2, int64_t n, int64_t vs) {
+  const int v = blockIdx.y;
+  const double r = 1.0 / sqrt(nrm2[v]);
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+    X[(int64_t)v * vs + k] *= r;
+}
+
 __global__ void lz_step_kernel(int* step) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
 }
@@ -2236,6 +2305,20 @@ hipError_t launch_lz_update(const double* Y, const double* X, double* const* bas
                             const double* ebuf, int ld, const int* step, double* W, int64_t n, int64_t vs, int nvec,
                             hipStream_t st) {
   lz_update_kernel<<<dim3(vec_grid(n), nvec), 256, 0, st>>>(Y, X, basis, kbuf, ebuf, ld, step, W, n, vs);
+  return hipGetLastError();
+}
+
+hipError_t launch_project_out(double* W, double* const* Q, int64_t qvs, const int* cnt, int add, int nbmax, int64_t n,
+                              int64_t vs, int nvec, double* partial, int nparts, double* c, hipStream_t st) {
+  if (nbmax <= 0 || nvec <= 0) return hipSuccess;
+  multidot_partial_kernel<<<dim3(nparts, nvec), 256, 0, st>>>(W, Q, qvs, cnt, add, nbmax, n, vs, partial);
+  multidot_final_kernel<<<nvec, 256, 0, st>>>(partial, nparts, cnt, add, nbmax, c);
+  multisub_kernel<<<dim3(vec_grid(n), nvec), 256, 0, st>>>(W, Q, qvs, cnt, add, nbmax, c, n, vs);
+  return hipGetLastError();
+}
+
+hipError_t launch_scale_rsqrt(double* X, const double* nrm2, int64_t n, int64_t vs, int nvec, hipStream_t st) {
+  scale_rsqrt_kernel<<<dim3(vec_grid(n), nvec), 256, 0, st>>>(X, nrm2, n, vs);
   return hipGetLastError();
 }
 

@@ -200,6 +200,8 @@ struct Vecs {
   int64_t n;
   hipStream_t st;
   ~Vecs() {
+    // work queued on st may still use the vectors (an exception path)
+    if (!v.empty()) (void)hipStreamSynchronize(st);
     for (double* p : v) g_pool.put(n, p);
   }
   double* get() {
@@ -323,6 +325,14 @@ int mpsa_lanczos(mpsa_plan* plan, const double* dpsi0, double* dstate, int itmax
 int mpsa_lanczos_batched(mpsa_plan* plan, const double* dpsi0, int64_t stride, int nvec, double* dstates, int itmax,
                          double delta, int target, double* energies, int cap, int* nit, int* nritz, int use_graph,
                          mpsa_stream stream) {
+  return mpsa_lanczos_batched_ex(plan, dpsi0, stride, nvec, dstates, itmax, delta, target, energies, cap, nit, nritz,
+                                 use_graph, nullptr, 0, 0, 0, 0, stream);
+}
+
+int mpsa_lanczos_batched_ex(mpsa_plan* plan, const double* dpsi0, int64_t stride, int nvec, double* dstates, int itmax,
+                            double delta, int target, double* energies, int cap, int* nit, int* nritz, int use_graph,
+                            const double* dproj, int nproj, int64_t proj_stride, int64_t proj_vstride, int reortho,
+                            mpsa_stream stream) {
   hipStream_t cs = nullptr;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
@@ -337,12 +347,16 @@ int mpsa_lanczos_batched(mpsa_plan* plan, const double* dpsi0, int64_t stride, i
   try {
     if (!plan || !dpsi0 || nvec < 1 || itmax < 1 || !nit || !nritz || (cap > 0 && !energies))
       throw LErr(MPSA_EINVAL, "lanczos_batched: bad arguments");
+    if (nproj < 0 || (nproj > 0 && !dproj) || proj_vstride < 0 || (reortho != 0 && reortho != 1))
+      throw LErr(MPSA_EINVAL, "lanczos_batched: bad projection arguments");
     int64_t xl = 0, yl = 0;
     int same = 0;
     chk(mpsa_plan_vector_lengths(plan, &xl, &yl, &same));
     if (!same) throw LErr(MPSA_EINVAL, "lanczos_batched: the plan's x and y tables must be the same vector space");
     const int64_t n = xl, vs = stride;
     if (vs < n) throw LErr(MPSA_EINVAL, "lanczos_batched: stride smaller than the vector length");
+    if (nproj > 0 && (proj_stride < n || (proj_vstride > 0 && proj_vstride < n)))
+      throw LErr(MPSA_EINVAL, "lanczos_batched: projection strides smaller than the vector length");
     // Capture needs a stream other than the legacy default stream: all work
     // runs on a private stream after the caller's stream has drained.
     hchk(hipStreamSynchronize((hipStream_t)stream), "lanczos_batched: caller stream");
@@ -378,12 +392,69 @@ int mpsa_lanczos_batched(mpsa_plan* plan, const double* dpsi0, int64_t stride, i
         hchk(hipMemcpy(table + old, basis.data() + old, sizeof(double*) * (basis.size() - old), hipMemcpyHostToDevice),
              "basis table");
     };
+    // Projection states (src/lanzcos.hpp:281-414): copied once, then
+    // orthonormalised in FP64 (classical Gram-Schmidt, twice); for one state
+    // W - q <q, W> with q = s / |s| is the reference's W - s (s*W) / (s*s).
+    // With proj_vstride = 0 one set serves every vector (qvs = 0).
+    // On a complex plan the recurrence runs on planar vectors with the real
+    // dot product; the complex span of a state s is the real span of s and
+    // i s, so both are projected out (nq = 2 nproj directions).
+    const int nvp = proj_vstride > 0 ? nvec : 1;
+    const int64_t qvs = proj_vstride > 0 ? vs : 0;
+    const bool cplx = plan_complex(plan);
+    const int nq = cplx ? 2 * nproj : nproj;
+    const int nbmax = std::max(nq, reortho ? ld : 0);
+    double* ppart = nbmax > 0 ? (double*)dalloc(sizeof(double) * (size_t)nvec * nbmax * nparts) : nullptr;
+    double* pcoef = nbmax > 0 ? (double*)dalloc(sizeof(double) * (size_t)nvec * nbmax) : nullptr;
+    double** ptab = nullptr;
+    Vecs P{{}, vs * nvp, cs};
+    if (nproj > 0) {
+      ptab = (double**)dalloc(sizeof(double*) * (size_t)nq);
+      std::vector<double*> hp((size_t)nq);
+      double* nrm = (double*)dalloc(sizeof(double) * 2 * (size_t)nvp);
+      std::vector<double> hn(2 * (size_t)nvp);
+      const int64_t e = n / 2;  // complex extent
+      for (int p = 0; p < nproj; ++p) {
+        double* Q = hp[(size_t)(cplx ? 2 * p : p)] = P.get();
+        for (int v = 0; v < nvp; ++v)
+          hchk(hipMemcpyAsync(Q + (int64_t)v * vs, dproj + p * proj_stride + v * proj_vstride,
+                              sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, cs),
+               "copy projection state");
+        if (cplx) {  // i s: (re, im) -> (-im, re)
+          double* R = hp[(size_t)(2 * p + 1)] = P.get();
+          for (int v = 0; v < nvp; ++v) {
+            hchk(launch_axpby(-1.0, 0.0, Q + (int64_t)v * vs + e, 0.0, 0.0, R + (int64_t)v * vs, e, 0, cs), "i s");
+            hchk(launch_axpby(1.0, 0.0, Q + (int64_t)v * vs, 0.0, 0.0, R + (int64_t)v * vs + e, e, 0, cs), "i s");
+          }
+        }
+      }
+      hchk(hipMemcpyAsync(ptab, hp.data(), sizeof(double*) * (size_t)nq, hipMemcpyHostToDevice, cs), "state table");
+      for (int p = 0; p < nq; ++p) {
+        double* Q = hp[(size_t)p];
+        hchk(launch_bdot(Q, Q, n, vs, nvp, partial, nparts, nrm, 1, dstep, 0, cs), "state norm");
+        for (int pass = 0; pass < 2 && p > 0; ++pass)
+          hchk(launch_project_out(Q, ptab, vs, nullptr, p, nq, n, vs, nvp, ppart, nparts, pcoef, cs), "state GS");
+        hchk(launch_bdot(Q, Q, n, vs, nvp, partial, nparts, nrm + nvp, 1, dstep, 0, cs), "state norm");
+        hchk(hipMemcpyAsync(hn.data(), nrm, sizeof(double) * hn.size(), hipMemcpyDeviceToHost, cs), "read norms");
+        hchk(hipStreamSynchronize(cs), "state norms");
+        for (int v = 0; v < nvp; ++v)
+          if (!(hn[(size_t)v] > 0.0) || !(hn[(size_t)nvp + v] > 1e-24 * hn[(size_t)v]))
+            throw LErr(MPSA_EINVAL, "lanczos_batched: projection state " + std::to_string(cplx ? p / 2 : p) +
+                                        " is zero or linearly dependent on the earlier ones");
+        hchk(launch_scale_rsqrt(Q, nrm + nvp, n, vs, nvp, cs), "state normalise");
+      }
+    }
     auto step_once = [&]() {
+      if (nproj > 0)
+        hchk(launch_project_out(W, ptab, qvs, nullptr, nq, nbmax, n, vs, nvec, ppart, nparts, pcoef, cs), "project");
       hchk(launch_bdot(W, W, n, vs, nvec, partial, nparts, kbuf, ld, dstep, 1, cs), "norm");
       hchk(launch_lz_normalize(W, X, table, kbuf, ld, dstep, n, vs, nvec, cs), "normalise");
       chk(mpsa_heff_apply(plan, X, vs, Y, vs, nvec, (mpsa_stream)cs));
       hchk(launch_bdot(Y, X, n, vs, nvec, partial, nparts, ebuf, ld, dstep, 0, cs), "alpha");
       hchk(launch_lz_update(Y, X, table, kbuf, ebuf, ld, dstep, W, n, vs, nvec, cs), "recurrence");
+      // full reorthogonalisation against the basis 0..step, twice
+      for (int pass = 0; reortho && pass < 2; ++pass)
+        hchk(launch_project_out(W, table, vs, dstep, 1, nbmax, n, vs, nvec, ppart, nparts, pcoef, cs), "reorthogonalise");
       hchk(launch_lz_step(dstep, cs), "step");
     };
     const int Ndiag = 5;

@@ -974,6 +974,8 @@ static void ensure_ws(mpsa_plan* P, int nvec, hipStream_t st) {
   }
 }
 
+bool plan_complex(const mpsa_plan* P) { return P && P->cplx; }
+
 }  // namespace mpsa
 
 // =================================================================== C ABI

@@ -79,6 +79,8 @@ def lib():
         L.mpsa_lanczos.argtypes = [P, P, P, C.c_int, C.c_double, C.c_int, C.c_int, P, C.c_int, P, P, P]
         L.mpsa_lanczos_batched.argtypes = [P, P, C.c_int64, C.c_int, P, C.c_int, C.c_double, C.c_int, P, C.c_int,
                                             P, P, C.c_int, P]
+        L.mpsa_lanczos_batched_ex.argtypes = [P, P, C.c_int64, C.c_int, P, C.c_int, C.c_double, C.c_int, P, C.c_int,
+                                               P, P, C.c_int, P, C.c_int, C.c_int64, C.c_int64, C.c_int, P]
         L.mpsa_chebyshev_moments.argtypes = [P, P, C.c_int, C.c_double, C.c_double, P, P, P]
         L.mpsa_chain_create.argtypes = [P, C.c_int, P, P, P, P, C.c_int, P, C.c_int]
         L.mpsa_energy_forward.argtypes = [P, P, P, P]
@@ -430,18 +432,24 @@ def chebyshev_moments(plan, dt0, nmom, scale, shift=0.0, recons=True):
     return mu, rc
 
 
-def lanczos_batched(plan, dpsi0, nvec, stride, itmax, delta, target=0, want_state=True, graph=True):
+def lanczos_batched(plan, dpsi0, nvec, stride, itmax, delta, target=0, want_state=True, graph=True, proj=None,
+                    nproj=0, proj_stride=0, proj_vstride=0, reortho=False):
     """nvec independent Lanczos runs (energy test), one batched matvec per step;
-    with graph=True each block of 5 steps is replayed from a hipGraph."""
+    with graph=True each block of 5 steps is replayed from a hipGraph.  With
+    nproj > 0 every run is projected against the states in `proj` (state p at
+    offset p * proj_stride, per-vector sets at v * proj_vstride when nonzero;
+    the reference's projected run, src/lanzcos.hpp:281-414); reortho adds full
+    FP64 reorthogonalisation against each run's Krylov basis."""
     import torch
     cap = itmax + 2
     E = np.zeros(nvec * cap)
     nit = np.zeros(nvec, np.int32)
     nr = np.zeros(nvec, np.int32)
     state = torch.zeros(nvec * stride, dtype=torch.float64, device=dpsi0.device) if want_state else None
-    _check(lib().mpsa_lanczos_batched(plan.h, _dptr(dpsi0), int(stride), int(nvec), _dptr(state), int(itmax),
-                                      float(delta), int(target), _ptr(E), cap, _ptr(nit), _ptr(nr), int(bool(graph)),
-                                      _stream()))
+    _check(lib().mpsa_lanczos_batched_ex(plan.h, _dptr(dpsi0), int(stride), int(nvec), _dptr(state), int(itmax),
+                                         float(delta), int(target), _ptr(E), cap, _ptr(nit), _ptr(nr),
+                                         int(bool(graph)), _dptr(proj), int(nproj), int(proj_stride),
+                                         int(proj_vstride), int(bool(reortho)), _stream()))
     E = E.reshape(nvec, cap)
     return dict(it=nit.astype(int), energies=[E[v, :int(nr[v])].copy() for v in range(nvec)], state=state)
 

@@ -389,16 +389,30 @@ int orc_measure(int n, const orc_bm* const* mps, const orc_mpo* const* mpo, cons
 // Two-site Lanczos (TSLanzcosEngine::simulate) on real data.  energies gets
 // the final (sorted) Ritz values, at most cap of them; *nit the iteration count,
 // *nritz the number of Ritz values; gs receives getState(target).
+int orc_lanczos_twosite_state(const orc_bm* L, const orc_bm* R, const orc_bm* psi0, const orc_mpo* wl,
+                              const orc_mpo* wr, const uint64_t* itl, int dl, const uint64_t* itr, int dr, int itmax,
+                              double delta, int testenergy, int nthreads, int target, double* energies, int cap,
+                              int* nit, int* nritz, orc_bm* gs, const orc_bm* state);
 int orc_lanczos_twosite(const orc_bm* L, const orc_bm* R, const orc_bm* psi0, const orc_mpo* wl, const orc_mpo* wr,
                         const uint64_t* itl, int dl, const uint64_t* itr, int dr, int itmax, double delta,
                         int testenergy, int nthreads, int target, double* energies, int cap, int* nit, int* nritz,
                         orc_bm* gs) {
+  return orc_lanczos_twosite_state(L, R, psi0, wl, wr, itl, dl, itr, dr, itmax, delta, testenergy, nthreads, target,
+                                   energies, cap, nit, nritz, gs, nullptr);
+}
+
+// The projected run (src/lanzcos.hpp:281-414) when state is given.
+int orc_lanczos_twosite_state(const orc_bm* L, const orc_bm* R, const orc_bm* psi0, const orc_mpo* wl,
+                              const orc_mpo* wr, const uint64_t* itl, int dl, const uint64_t* itr, int dr, int itmax,
+                              double delta, int testenergy, int nthreads, int target, double* energies, int cap,
+                              int* nit, int* nritz, orc_bm* gs, const orc_bm* state) {
   ORC_TRY({
     ItoKey il = mk_itk(itl, dl), ir = mk_itk(itr, dr);
     auto mv = [&](BlockMatrix<2, double>& y, const BlockMatrix<2, double>& x) {
       contractTwosite(y, L->d1, R->d1, x, wl->d, wr->d, il, ir, nthreads);
     };
-    LanczosResult<double> r = lanczos<double>(psi0->d2, mv, itmax, delta, testenergy != 0);
+    LanczosResult<double> r =
+        lanczos<double>(psi0->d2, mv, itmax, delta, testenergy != 0, state ? &state->d2 : nullptr);
     *nit = r.it;
     *nritz = (int)r.es.size();
     for (int i = 0; i < (int)r.es.size() && i < cap; ++i) energies[i] = r.es[(size_t)i];

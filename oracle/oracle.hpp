@@ -530,8 +530,30 @@ struct LanczosResult {
   std::vector<BlockMatrix<2, T>> As;
 };
 
+// BlockMatrix::squeeze (BlockTensor.hpp:417-441): drop every block whose
+// entries are all at most thresh in magnitude.
+template <int Ns, typename T>
+void squeeze(BlockMatrix<Ns, T>& X, double thresh) {
+  for (auto it = X.begin(); it != X.end();) {
+    bool er = true;
+    for (auto& x : it->second.a)
+      if (std::abs(x) > thresh) {
+        er = false;
+        break;
+      }
+    it = er ? X.erase(it) : std::next(it);
+  }
+}
+
+// With `state` the run is the projected one of
+// TSLanzcosEngine::simulate(itmax, delta, testenergy, state, delta_ortho)
+// (src/lanzcos.hpp:281-414): state is squeezed at 1e-15, the start vector
+// and every new recurrence vector lose their component along state,
+// A <- A - state ((state*A) / (state*state)); delta_ortho is unused there
+// (its Gram-Schmidt block is commented out).
 template <typename T, typename MV>
-LanczosResult<T> lanczos(const BlockMatrix<2, T>& Ainit, MV matvec, int itmax, double delta, bool testenergy) {
+LanczosResult<T> lanczos(const BlockMatrix<2, T>& Ainit, MV matvec, int itmax, double delta, bool testenergy,
+                         const BlockMatrix<2, T>* state_in = nullptr) {
   LanczosResult<T> res;
   int it = 0;
   T k = 0, e, eold = T(1e100);
@@ -540,6 +562,19 @@ LanczosResult<T> lanczos(const BlockMatrix<2, T>& Ainit, MV matvec, int itmax, d
   BlockMatrix<2, T> A0, A1 = Ainit, A2, A3;
   double resi = 1e5;
   const int Ndiag = 5;  // lanzcos.hpp:107
+  BlockMatrix<2, T> state;
+  T state_norm = T(0);
+  if (state_in) {
+    state = *state_in;
+    squeeze(state, 1e-15);     // :296
+    state_norm = dot(state, state);
+  }
+  auto project = [&](BlockMatrix<2, T>& A) {  // :298, :340, :372
+    if (!state_in) return;
+    const T c = dot(state, A) / state_norm;
+    axpy(A, -c, state);
+  };
+  project(A1);
   while (!converged) {
     k = std::sqrt(dot(A1, A1));                     // :187
     if (std::abs(k) < delta) { converged = true; break; }
@@ -584,6 +619,7 @@ LanczosResult<T> lanczos(const BlockMatrix<2, T>& Ainit, MV matvec, int itmax, d
       BlockMatrix<2, T> t0 = A0;
       scale(t0, -k);
       axpy(A3, T(1.0), t0);
+      project(A3);
       A0 = A1;
       A1 = A3;
     } else {                                        // :243-248
@@ -591,6 +627,7 @@ LanczosResult<T> lanczos(const BlockMatrix<2, T>& Ainit, MV matvec, int itmax, d
       A3 = A1;
       scale(A3, -e);
       axpy(A3, T(1.0), A2);
+      project(A3);
       A1 = A3;
       first = false;
     }

@@ -52,6 +52,8 @@ def lib():
         L.orc_measure.argtypes = [C.c_int, P, P, P, C.c_int, C.c_int, P]
         L.orc_lanczos_twosite.argtypes = [P, P, P, P, P, P, C.c_int, P, C.c_int, C.c_int, C.c_double,
                                           C.c_int, C.c_int, C.c_int, P, C.c_int, P, P, P]
+        L.orc_lanczos_twosite_state.argtypes = [P, P, P, P, P, P, C.c_int, P, C.c_int, C.c_int, C.c_double,
+                                                C.c_int, C.c_int, C.c_int, P, C.c_int, P, P, P, P]
         L.orc_tridiag.argtypes = [C.c_int, P, P, P]
         L.orc_set_blas_threads.argtypes = [C.c_int]
         L.orc_fill_gauss.argtypes = [P, C.c_int64, C.c_uint64]
@@ -234,19 +236,21 @@ def dot(X, Y):
     return complex(re[0], im[0]) if X.dtype.kind == "c" else float(re[0])
 
 
-def lanczos_twosite(L, R, psi0, Wl, Wr, itl, itr, itmax, delta, testenergy=True, nthreads=1, target=0):
-    """TSLanzcosEngine::simulate + getState (src/lanzcos.hpp:165-278, 81-92)."""
+def lanczos_twosite(L, R, psi0, Wl, Wr, itl, itr, itmax, delta, testenergy=True, nthreads=1, target=0, state=None):
+    """TSLanzcosEngine::simulate + getState (src/lanzcos.hpp:165-278, 81-92);
+    with `state` (two-site blocks) the projected run of src/lanzcos.hpp:281-414."""
     a, b = _itk(itl), _itk(itr)
     hl, hr, hp = BM.from_blocks(L), BM.from_blocks(R), BM.from_blocks(psi0)
     ml, mr = MPO(Wl), MPO(Wr)
+    hs = BM.from_blocks(state) if state is not None else None
     cap = max(itmax + 2, 8)
     E = np.zeros(cap)
     nit = np.zeros(1, np.int32)
     nr = np.zeros(1, np.int32)
     gs = BM(2)
-    _check(lib().orc_lanczos_twosite(hl.h, hr.h, hp.h, ml.h, mr.h, _p(a), len(itl), _p(b), len(itr), itmax,
-                                      delta, int(testenergy), nthreads, target, _p(E), cap, _p(nit), _p(nr),
-                                      gs.h))
+    _check(lib().orc_lanczos_twosite_state(hl.h, hr.h, hp.h, ml.h, mr.h, _p(a), len(itl), _p(b), len(itr), itmax,
+                                            delta, int(testenergy), nthreads, target, _p(E), cap, _p(nit), _p(nr),
+                                            gs.h, hs.h if hs is not None else None))
     return dict(it=int(nit[0]), energies=E[:int(nr[0])].copy(), gs=gs.to_blocks())
 
 

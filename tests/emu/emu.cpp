@@ -752,6 +752,46 @@ hipError_t launch_lz_update(const double* Y, const double* X, double* const* bas
   });
   return hipSuccess;
 }
+hipError_t launch_project_out(double* W, double* const* Q, int64_t qvs, const int* cnt, int add, int nbmax, int64_t n,
+                              int64_t vs, int nvec, double* partial, int nparts, double* c, hipStream_t st) {
+  if (nbmax <= 0 || nvec <= 0) return hipSuccess;
+  submit(st, "project_out", [=]() {
+    if (cnt && !check(cnt, 4, "project count")) return;
+    const int nb = std::min((cnt ? *cnt : 0) + add, nbmax);
+    if (nb > 0 && !chk_range(Q, 0, nb - 1, "project table")) return;
+    if (!chk_range(partial, 0, (int64_t)nvec * nbmax * nparts - 1, "project partial") ||
+        !chk_range(c, 0, (int64_t)nvec * nbmax - 1, "project coefficients"))
+      return;
+    for (int v = 0; v < nvec; ++v) {
+      double* wv = W + (int64_t)v * vs;
+      if (!chk_range(wv, 0, n - 1, "project W")) return;
+      for (int i = 0; i < nb; ++i) {
+        const double* q = Q[i] + (int64_t)v * qvs;
+        if (!chk_range(q, 0, n - 1, "project Q")) return;
+        double a = 0;
+        for (int64_t k = 0; k < n; ++k) a += q[k] * wv[k];
+        c[(int64_t)v * nbmax + i] = a;
+      }
+      for (int64_t k = 0; k < n; ++k) {
+        double t = wv[k];
+        for (int i = 0; i < nb; ++i) t -= c[(int64_t)v * nbmax + i] * Q[i][(int64_t)v * qvs + k];
+        wv[k] = t;
+      }
+    }
+  });
+  return hipSuccess;
+}
+hipError_t launch_scale_rsqrt(double* X, const double* nrm2, int64_t n, int64_t vs, int nvec, hipStream_t st) {
+  submit(st, "scale_rsqrt", [=]() {
+    if (!chk_range(nrm2, 0, nvec - 1, "rsqrt norms")) return;
+    for (int v = 0; v < nvec; ++v) {
+      if (!chk_range(X + (int64_t)v * vs, 0, n - 1, "rsqrt X")) return;
+      const double r = 1.0 / std::sqrt(nrm2[v]);
+      for (int64_t k = 0; k < n; ++k) X[(int64_t)v * vs + k] *= r;
+    }
+  });
+  return hipSuccess;
+}
 hipError_t launch_lz_step(int* step, hipStream_t st) {
   submit(st, "lz_step", [=]() {
     if (check(step, 4, "lz step")) *step += 1;

@@ -540,3 +540,30 @@ def test_emulated_device_canonicalisation(lib, oracle):
             X = np.concatenate(bl, axis=1)
             assert np.abs(X @ X.T - np.eye(X.shape[0])).max() <= 1e-12
     del eng
+
+
+def _emu_io(lib):
+    return (lambda a: DevBuf(lib, a)), (lambda b: b.host())
+
+
+def test_emulated_lanczos_projection(lib, oracle):
+    """Projection against given states (one, two non-orthonormal, per-vector
+    sets, degenerate sets rejected) against the dense spectrum and the oracle's
+    projected run (tests/lzcase.py)."""
+    import lzcase
+    dev, host = _emu_io(lib)
+    lzcase.check_projection(oracle, lzcase.case(oracle, "xxz", 10, 5, 12, 15), dev, host)
+
+
+def test_emulated_lanczos_reorthogonalisation(lib, oracle):
+    """Full FP64 reorthogonalisation: dim steps give the whole spectrum (the
+    plain recurrence does not); short runs keep the oracle's Ritz values."""
+    import lzcase
+    dev, host = _emu_io(lib)
+    lzcase.check_reortho(oracle, lzcase.case(oracle, "xxz", 10, 5, 12, 15), dev, host)
+
+
+def test_emulated_lanczos_complex_projection(lib, oracle):
+    import lzcase
+    dev, host = _emu_io(lib)
+    lzcase.check_complex_projection(oracle, lzcase.case(oracle, "tv", 12, 6, 16, 116), dev, host, 116)

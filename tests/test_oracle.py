@@ -175,6 +175,31 @@ def test_lanczos_residual_criterion_and_itmax(oracle):
     assert abs(r["energies"][0] - w[0]) < 1e-8
 
 
+def test_lanczos_projected_run_reaches_the_first_excited_state(oracle):
+    """The projected run (src/lanzcos.hpp:281-414) against the exact ground
+    state of the dense H_eff converges to the second eigenvalue, its Ritz
+    vector is that eigenvector and orthogonal to the ground state, for the
+    energy test and the residual test; given the state scaled by 3, the same.
+    A run that skipped the projection of the start vector or of a recurrence
+    vector would find E_0 again (rounding-level components grow)."""
+    import lzcase
+    K = lzcase.case(oracle, "xxz", 10, 5, 12, 15)
+    E, U, vt, c = K["E"], K["U"], K["vt"], K["c"]
+    args = (K["Lb"], K["Rb"], K["psi"], K["mpo"][c], K["mpo"][c + 1], K["itk"][c], K["itk"][c + 1])
+    g0 = U[:, 0]
+    assert E[1] - E[0] > 1e-3 and E[2] - E[1] > 1e-3
+    plain = oracle.lanczos_twosite(*args, itmax=300, delta=1e-12, testenergy=True)
+    assert abs(plain["energies"][0] - E[0]) <= 1e-9
+    for testenergy, tol in [(True, 1e-9), (False, 1e-8)]:
+        for f in (1.0, 3.0):
+            r = oracle.lanczos_twosite(*args, itmax=300, delta=1e-12 if testenergy else 1e-9, testenergy=testenergy,
+                                       state=vt.unpack(f * g0))
+            assert abs(r["energies"][0] - E[1]) <= tol
+            s = vt.pack(r["gs"])
+            assert abs(np.dot(g0, s)) <= 1e-8
+            assert abs(abs(np.dot(U[:, 1], s)) / np.linalg.norm(s) - 1.0) <= 1e-6
+
+
 def test_tridiag_matches_scipy(oracle):
     rng = np.random.default_rng(0)
     d, e = rng.standard_normal(30), rng.standard_normal(29)
