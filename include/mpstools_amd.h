@@ -434,6 +434,10 @@ int mpsa_cheb_site(mpsa_cheb* c, int slot, int vec, int site, int64_t* n_blocks,
                    uint32_t* p, int64_t* rows, int64_t* cols, int64_t* offsets, double* dst);
 /* Plans built so far and device bytes held by the engine's buffer pool. */
 int mpsa_cheb_stats(const mpsa_cheb* c, int64_t* plans_built, int64_t* pool_bytes);
+/* Host-side time of the engine since its creation: out5 = {seconds building
+ * plans, seconds in output-key queries, seconds waiting on the stream, number
+ * of waits, plans cached now}. */
+int mpsa_cheb_profile(const mpsa_cheb* engine, double* out5);
 int mpsa_cheb_destroy(mpsa_cheb* c);
 
 /* ---- Lanczos vector algebra over packed block vectors (BlockTensor.hpp:443-544) ----

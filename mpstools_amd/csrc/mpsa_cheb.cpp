@@ -34,6 +34,7 @@
 #include <complex>
 #include <cstring>
 #include <list>
+#include <unordered_map>
 #include <map>
 #include <memory>
 #include <set>
@@ -186,17 +187,35 @@ struct mpsa_cheb {
   std::vector<std::vector<uint64_t>> itk;
   Pool pool;
   std::vector<std::shared_ptr<void>> keep;  // host copies of uploaded descriptors, until the next sync
-  std::list<std::pair<std::string, mpsa_plan*>> plans;
-  static constexpr size_t kPlanCap = 96;
+  // plans by structure key (tables and MPO, not data): a moment loop meets the
+  // same few hundred structures again and again once the bond dimensions
+  // have settled.  Bounded by count and by workspace bytes; eviction waits
+  // for the stream once and then drops the least recently used quarter.
+  struct PlanEntry {
+    std::string key;
+    mpsa_plan* plan;
+    int64_t bytes;
+  };
+  std::list<PlanEntry> plans;
+  std::unordered_map<std::string, std::list<PlanEntry>::iterator> plan_index;
+  static constexpr size_t kPlanCap = 4096;
+  static constexpr int64_t kPlanBytesCap = int64_t(24) << 30;
+  int64_t plan_bytes = 0;
   std::vector<DMps> slot;
   int64_t plans_built = 0;
+  // host time (seconds): plan builds, output-key queries, waits on the stream
+  double t_plan = 0.0, t_keys = 0.0, t_sync = 0.0;
+  int64_t n_sync = 0;
   ~mpsa_cheb() {
     if (st) (void)hipStreamSynchronize(st);
-    for (auto& kv : plans) mpsa_plan_destroy(kv.second);
+    for (auto& e : plans) mpsa_plan_destroy(e.plan);
     slot.clear();
   }
   void sync() {
+    const auto t0 = std::chrono::steady_clock::now();
     hchk(hipStreamSynchronize(st), "cheb: sync");
+    t_sync += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    ++n_sync;
     keep.clear();
   }
   template <typename T>
@@ -210,20 +229,26 @@ struct mpsa_cheb {
     return b;
   }
   mpsa_plan* find_plan(const std::string& k) {
-    for (auto it = plans.begin(); it != plans.end(); ++it)
-      if (it->first == k) {
-        plans.splice(plans.begin(), plans, it);
-        return it->second;
-      }
-    return nullptr;
+    auto f = plan_index.find(k);
+    if (f == plan_index.end()) return nullptr;
+    plans.splice(plans.begin(), plans, f->second);
+    return f->second->plan;
   }
   void put_plan(const std::string& k, mpsa_plan* p) {
-    plans.emplace_front(k, p);
+    int64_t ws = 0;
+    (void)mpsa_plan_info(p, nullptr, nullptr, nullptr, nullptr, &ws);
+    plans.push_front(PlanEntry{k, p, ws});
+    plan_index[k] = plans.begin();
+    plan_bytes += ws;
     ++plans_built;
-    while (plans.size() > kPlanCap) {
-      sync();  // an evicted plan's last launch must have finished
-      mpsa_plan_destroy(plans.back().second);
-      plans.pop_back();
+    if (plans.size() > kPlanCap || plan_bytes > kPlanBytesCap) {
+      sync();  // the evicted plans' last launches must have finished
+      while (plans.size() > 1 && (plans.size() > kPlanCap * 3 / 4 || plan_bytes > kPlanBytesCap * 3 / 4)) {
+        plan_bytes -= plans.back().bytes;
+        plan_index.erase(plans.back().key);
+        mpsa_plan_destroy(plans.back().plan);
+        plans.pop_back();
+      }
     }
   }
 };
@@ -231,6 +256,14 @@ struct mpsa_cheb {
 namespace {
 
 typedef mpsa_cheb Eng;
+
+// adds the wall time of its scope to an accumulator
+struct ScopeTimer {
+  double& acc;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  explicit ScopeTimer(double& a) : acc(a) {}
+  ~ScopeTimer() { acc += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
+};
 
 // ------------------------------------------------------------ copies, GEMMs
 void add_copy(std::vector<Copy2D>& v, const Eng& E, int64_t src_off, int64_t src_plane, int64_t dst_off,
@@ -367,20 +400,28 @@ void env_layer(Eng& E, const HTab& Et, const double* dE, const HTab& At, const d
   mpsa_blocks be = Et.view(), ba = At.view(), bp = Apt.view();
   mpsa_mpo mw = W.view();
   int64_t n = 0;
-  chk(mpsa_env_outkeys(&be, &ba, &bp, &mw, itk.data(), (int)itk.size(), dir, 0, &n, nullptr, nullptr, nullptr,
-                       nullptr));
-  std::vector<uint64_t> q((size_t)2 * n + 2);
-  std::vector<uint32_t> p((size_t)n + 1);
-  std::vector<int64_t> r((size_t)n + 1), c((size_t)n + 1);
-  if (n)
-    chk(mpsa_env_outkeys(&be, &ba, &bp, &mw, itk.data(), (int)itk.size(), dir, n, &n, q.data(), p.data(), r.data(),
-                         c.data()));
+  std::vector<uint64_t> q;
+  std::vector<uint32_t> p;
+  std::vector<int64_t> r, c;
+  {
+    ScopeTimer tk(E.t_keys);
+    chk(mpsa_env_outkeys(&be, &ba, &bp, &mw, itk.data(), (int)itk.size(), dir, 0, &n, nullptr, nullptr, nullptr,
+                         nullptr));
+    q.resize((size_t)2 * n + 2);
+    p.resize((size_t)n + 1);
+    r.resize((size_t)n + 1);
+    c.resize((size_t)n + 1);
+    if (n)
+      chk(mpsa_env_outkeys(&be, &ba, &bp, &mw, itk.data(), (int)itk.size(), dir, n, &n, q.data(), p.data(),
+                           r.data(), c.data()));
+  }
   out_t = from_query(n, 1, q, p, r, c);
   out_t.cplx = Et.cplx;
   const std::string key = "E" + std::to_string(dir > 0) + sig_tab(Et) + sig_tab(At) + sig_tab(Apt) + sig_mpo(W);
   mpsa_plan* pl = E.find_plan(key);
   if (!pl) {
     mpsa_blocks bo = out_t.view();
+    ScopeTimer tp(E.t_plan);
     chk(mpsa_env_plan(&pl, &be, &ba, &bp, &mw, itk.data(), (int)itk.size(), dir, &bo, MPSA_F64));
     E.put_plan(key, pl);
   }
@@ -395,14 +436,21 @@ void heff1(Eng& E, const HTab& Lt, const double* dL, const HTab& At, const doubl
   mpsa_blocks bl = Lt.view(), br = Rt.view(), ba = At.view();
   mpsa_mpo mw = W.view();
   int64_t n = 0;
-  chk(mpsa_heff1_outkeys(&bl, &br, &ba, &mw, itk.data(), (int)itk.size(), 0, &n, nullptr, nullptr, nullptr,
-                         nullptr));
-  std::vector<uint64_t> q((size_t)2 * n + 2);
-  std::vector<uint32_t> p((size_t)n + 1);
-  std::vector<int64_t> r((size_t)n + 1), c((size_t)n + 1);
-  if (n)
-    chk(mpsa_heff1_outkeys(&bl, &br, &ba, &mw, itk.data(), (int)itk.size(), n, &n, q.data(), p.data(), r.data(),
-                           c.data()));
+  std::vector<uint64_t> q;
+  std::vector<uint32_t> p;
+  std::vector<int64_t> r, c;
+  {
+    ScopeTimer tk(E.t_keys);
+    chk(mpsa_heff1_outkeys(&bl, &br, &ba, &mw, itk.data(), (int)itk.size(), 0, &n, nullptr, nullptr, nullptr,
+                           nullptr));
+    q.resize((size_t)2 * n + 2);
+    p.resize((size_t)n + 1);
+    r.resize((size_t)n + 1);
+    c.resize((size_t)n + 1);
+    if (n)
+      chk(mpsa_heff1_outkeys(&bl, &br, &ba, &mw, itk.data(), (int)itk.size(), n, &n, q.data(), p.data(), r.data(),
+                             c.data()));
+  }
   out_t = from_query(n, 1, q, p, r, c);
   out_t.cplx = At.cplx;
   out_b = Buf(E.pool, out_t.len(), E.st);
@@ -411,11 +459,30 @@ void heff1(Eng& E, const HTab& Lt, const double* dL, const HTab& At, const doubl
   mpsa_plan* pl = E.find_plan(key);
   if (!pl) {
     mpsa_blocks by = out_t.view();
+    ScopeTimer tp(E.t_plan);
     chk(mpsa_heff1_plan(&pl, &bl, &br, &ba, &by, &mw, itk.data(), (int)itk.size(), E.h1prec));
     E.put_plan(key, pl);
   }
   chk(mpsa_plan_bind_env(pl, dL, dR, (mpsa_stream)E.st));
   chk(mpsa_heff_apply(pl, dA, 0, out_b.p, 0, 1, (mpsa_stream)E.st));
+}
+
+// the 1 x 1 blocks of a closed left environment, one per vector
+std::vector<cplx> env_scalars(Eng& E, const HTab& Lt, const Buf& Lb) {
+  const int cp = E.planes == 2;
+  std::vector<double> h((size_t)std::max<int64_t>(Lt.len(), 1));
+  if (Lt.len() > 0)
+    hchk(hipMemcpyAsync(h.data(), Lb.p, sizeof(double) * (size_t)Lt.len(), hipMemcpyDeviceToHost, E.st),
+         "cheb: overlap");
+  E.sync();
+  std::vector<cplx> out((size_t)E.nvec, cplx(0.0, 0.0));
+  for (int64_t i = 0; i < Lt.n(); ++i) {
+    const uint64_t v = ns_of(Lt.q[2 * i]);
+    if (v >= (uint64_t)E.nvec || Lt.rows[(size_t)i] < 1 || Lt.cols[(size_t)i] < 1) continue;
+    const int64_t o = Lt.off[(size_t)i];
+    out[(size_t)v] = cplx(h[(size_t)o], cp ? h[(size_t)(o + Lt.total)] : 0.0);
+  }
+  return out;
 }
 
 // computeOverlap(a, b) per vector: <b|a> (the upper tensor b is conjugated)
@@ -431,19 +498,7 @@ std::vector<cplx> overlap(Eng& E, const DMps& a, const DMps& b) {
     Lt = nt;
     Lb = std::move(nb);
   }
-  std::vector<double> h((size_t)std::max<int64_t>(Lt.len(), 1));
-  if (Lt.len() > 0)
-    hchk(hipMemcpyAsync(h.data(), Lb.p, sizeof(double) * (size_t)Lt.len(), hipMemcpyDeviceToHost, E.st),
-         "cheb: overlap");
-  E.sync();
-  std::vector<cplx> out((size_t)E.nvec, cplx(0.0, 0.0));
-  for (int64_t i = 0; i < Lt.n(); ++i) {
-    const uint64_t v = ns_of(Lt.q[2 * i]);
-    if (v >= (uint64_t)E.nvec || Lt.rows[(size_t)i] < 1 || Lt.cols[(size_t)i] < 1) continue;
-    const int64_t o = Lt.off[(size_t)i];
-    out[(size_t)v] = cplx(h[(size_t)o], cp ? h[(size_t)(o + Lt.total)] : 0.0);
-  }
-  return out;
+  return env_scalars(E, Lt, Lb);
 }
 
 // ------------------------------------------------------------ prepareSite
@@ -699,15 +754,28 @@ void truncate_mps(Eng& E, DMps& M, unsigned chimax, double delta) {
 }
 
 // ------------------------------------------------------------ applyMPO
-DMps apply_mpo(Eng& E, const DMps& M, const std::vector<HMpo>& W) {
-  DMps O = new_mps(E);
-  std::map<std::pair<uint64_t, uint32_t>, uint64_t> new_left;  // (a, wl) -> new charge
+// The left bond of the applied state carried from one site to the next:
+// (a, wl) -> new charge, and the sizes of the coarse blocks per new charge.
+struct ApplyBond {
+  std::map<std::pair<uint64_t, uint32_t>, uint64_t> new_left;
   std::map<uint64_t, std::vector<int64_t>> left_sizes;
+};
+
+ApplyBond apply_start(const DMps& M, const std::vector<HMpo>& W) {
+  ApplyBond b;
   for (int64_t i = 0; i < M.A[0].n(); ++i) {  // site 0: each vector's boundary charge keeps its value
-    new_left[{M.A[0].q[2 * i], 0u}] = M.A[0].q[2 * i];
-    left_sizes[M.A[0].q[2 * i]] = std::vector<int64_t>(W[0].Dl, 1);
+    b.new_left[{M.A[0].q[2 * i], 0u}] = M.A[0].q[2 * i];
+    b.left_sizes[M.A[0].q[2 * i]] = std::vector<int64_t>(W[0].Dl, 1);
   }
-  for (int s = 0; s < E.N; ++s) {
+  return b;
+}
+
+// site s of applyMPO(W, M) into (NT, nb); advances the bond state
+void apply_site(Eng& E, ApplyBond& bond, const DMps& M, const std::vector<HMpo>& W, int s, HTab& NT_out,
+                Buf& nb_out) {
+  auto& new_left = bond.new_left;
+  auto& left_sizes = bond.left_sizes;
+  {
     const HTab& T = M.A[(size_t)s];
     const HMpo& w = W[(size_t)s];
     const auto& lmap = E.itk[(size_t)s];
@@ -797,12 +865,40 @@ DMps apply_mpo(Eng& E, const DMps& M, const std::vector<HMpo>& W) {
                rows, cols, rows, NT.rows[(size_t)t], pc.v);
     }
     run_copies(E, cp, M.d[(size_t)s].p, nb.p, nullptr);
-    O.A[(size_t)s] = NT;
-    O.d[(size_t)s] = std::move(nb);
+    NT_out = NT;
+    nb_out = std::move(nb);
     left_sizes = right_sizes;
     new_left = new_right;
   }
+}
+
+DMps apply_mpo(Eng& E, const DMps& M, const std::vector<HMpo>& W) {
+  DMps O = new_mps(E);
+  ApplyBond bond = apply_start(M, W);
+  for (int s = 0; s < E.N; ++s) apply_site(E, bond, M, W, s, O.A[(size_t)s], O.d[(size_t)s]);
   return O;
+}
+
+std::vector<double> real_parts(const std::vector<cplx>& v);
+
+// computeOverlap(t, t) of t = applyMPO(W, M) per vector, with t made one site
+// at a time and folded into the running left environment at once: the
+// applied state (bond w D) is never held whole.  Same launches, in the same
+// order, as overlap(E, t, t) on the whole applied state.
+std::vector<double> applied_norm_sq(Eng& E, const DMps& M, const std::vector<HMpo>& W) {
+  // the applied state's boundary charges are the state's own
+  HTab Lt = lmin_tab_of(M, E.planes == 2), Bt;
+  Buf Lb = identity_buf(E, Lt), Bb;
+  ApplyBond bond = apply_start(M, W);
+  for (int s = 0; s < E.N; ++s) {
+    apply_site(E, bond, M, W, s, Bt, Bb);
+    HTab nt;
+    Buf nb;
+    env_layer(E, Lt, Lb.p, Bt, Bb.p, Bt, Bb.p, E.Id[(size_t)s], E.itk[(size_t)s], 1, nt, nb);
+    Lt = nt;
+    Lb = std::move(nb);
+  }
+  return real_parts(env_scalars(E, Lt, Lb));
 }
 
 // ------------------------------------------------------------ MPS sums
@@ -992,11 +1088,8 @@ std::vector<double> real_parts(const std::vector<cplx>& v) {
 DMps compress_single_site(Eng& E, const DMps& mps, unsigned chimax, int maxstep, double delta,
                           std::vector<double>& conv) {
   const int N = E.N;
-  std::vector<double> n1sq;
-  {
-    DMps t = apply_mpo(E, mps, E.W);
-    n1sq = real_parts(overlap(E, t, t));
-  }
+  // n1^2 = computeOverlap(applyMPO(W, mps), applyMPO(W, mps)), site by site
+  const std::vector<double> n1sq = applied_norm_sq(E, mps, E.W);
   DMps tmps = copy_mps(E, mps);
   for (int s = 0; s < N - 1; ++s) prepare_site(E, tmps, s, true, true);
   for (int s = N - 1; s > 0; --s) prepare_site(E, tmps, s, false, true);
@@ -1388,6 +1481,19 @@ int mpsa_cheb_stats(const mpsa_cheb* E, int64_t* plans_built, int64_t* pool_byte
     for (auto& a : E->pool.all) b += (int64_t)a.second;
     *pool_bytes = b;
   }
+  return MPSA_OK;
+}
+
+int mpsa_cheb_profile(const mpsa_cheb* E, double* out5) {
+  if (!E || !out5) {
+    set_last_error("cheb_profile: null argument");
+    return MPSA_EINVAL;
+  }
+  out5[0] = E->t_plan;
+  out5[1] = E->t_keys;
+  out5[2] = E->t_sync;
+  out5[3] = (double)E->n_sync;
+  out5[4] = (double)E->plans.size();
   return MPSA_OK;
 }
 

@@ -113,6 +113,7 @@ def lib():
         L.mpsa_cheb_moments.argtypes = [P, C.c_int, C.c_uint, C.c_int, C.c_double, P, P, P, P, P]
         L.mpsa_cheb_site.argtypes = [P, C.c_int, C.c_int, C.c_int, P, P, P, P, P, P, P, P]
         L.mpsa_cheb_stats.argtypes = [P, P, P]
+        L.mpsa_cheb_profile.argtypes = [P, P]
         L.mpsa_cheb_destroy.argtypes = [P]
         L.mpsa_dot.argtypes = [P, P, C.c_int64, C.c_int, P, P]
         L.mpsa_axpby.argtypes = [P, P, P, P, C.c_int64, C.c_int, P]
@@ -709,7 +710,10 @@ class Cheb:
     def stats(self):
         a, b = np.zeros(1, np.int64), np.zeros(1, np.int64)
         _check(lib().mpsa_cheb_stats(self.h, _ptr(a), _ptr(b)))
-        return dict(plans_built=int(a[0]), pool_bytes=int(b[0]))
+        t = np.zeros(5)
+        _check(lib().mpsa_cheb_profile(self.h, _ptr(t)))
+        return dict(plans_built=int(a[0]), pool_bytes=int(b[0]), host_plan_s=float(t[0]), host_keys_s=float(t[1]),
+                    host_wait_s=float(t[2]), waits=int(t[3]), plans_cached=int(t[4]))
 
 
 def fill_gauss(buf, seed, n=None):
