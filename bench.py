@@ -143,6 +143,61 @@ def cpu_baseline(prob, workload, flops_alg, flops_ref):
             "one_how": one_how}
 
 
+def u1_bonds(torch, steps, warmup):
+    """The U(1) bonds of the other workloads beside the headline, each with
+    its GPU matvec (FP64 and BF16X3, tuned tiles) and the CPU baseline on the
+    same bond (oracle tuple loop: 1 warm-up, median of 5, all cores and one
+    thread):
+      xxz64  : XXZ L = 32, D = 64 -- one matvec and the 10-step Lanczos of
+               the plain-method workload, both timed on GPU and CPU;
+      tv512  : t-V L = 100, D = 512 -- also 64 vectors per batched matvec;
+      hub1024: Fermi-Hubbard L = 80, D = 1024 (U(1) x U(1)) -- the bond of
+               the Chebyshev workload, real FP64."""
+    from mpstools_amd import bench_problem, native
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    out = {}
+    for name, nvecs in (("xxz64", (1,)), ("tv512", (1, 64)), ("hub1024", (1, 16))):
+        model, L, N, D, seed = WORKLOADS[name]
+        t0 = time.perf_counter()
+        prob = bench_problem.u1_centre(model, L, N, D, seed, precisions=("f64", "bf16x3"))
+        row = {"psi_blocks": prob["psi_blocks"], "psi_block_dims": prob["psi_block_dims"]}
+        for pr in ("f64", "bf16x3"):
+            plan = prob["plan_" + pr]
+            info = plan.info()
+            for nv in nvecs:
+                plan.autotune(nvec=nv, reps=3)
+                v = prob["psi"]
+                x = v.repeat(nv) if nv > 1 else v
+                ms, _ = time_apply(torch, plan, x, torch.zeros_like(x), steps, warmup, nv, v.numel() if nv > 1 else 0)
+                row[f"{pr}_nvec{nv}"] = {"ms": round(ms, 4), "tflops": round(nv * info["flops_alg"] / ms / 1e9, 3)}
+            row["flops_alg"] = info["flops_alg"]
+            row["flops_ref"] = info["flops_ref"]
+        cb = cpu_baseline(prob, name, row["flops_alg"], row["flops_ref"])
+        row["cpu"] = {"ms_all_cores": round(cb["t_all"] * 1e3, 3), "ms_one_thread": round(cb["t_one"] * 1e3, 3),
+                      "cores": cb["cores"], "blas": cb["blas"], "kind": "port",
+                      "sample": "oracle tuple loop on the same bond, 1 warm-up then the median of 5"}
+        row["gpu_f64_vs_cpu_all_cores"] = round(cb["t_all"] * 1e3 / row["f64_nvec1"]["ms"], 1)
+        if name == "xxz64":
+            # the plain-method workload: one matvec plus a 10-step Lanczos
+            plan = prob["plan_f64"]
+            torch.cuda.synchronize()
+            t_gpu = _median_time(lambda: native.lanczos(plan, prob["psi"], 10, 1e-12, True, want_state=False))
+            Lb = prob["Lt"].unpack(prob["dL"].cpu().numpy())
+            Rb = prob["Rt"].unpack(prob["dR"].cpu().numpy())
+            psi = prob["vt"].unpack(prob["psi"].cpu().numpy())
+            c, mpo, itk = prob["c"], prob["mpo"], prob["itk"]
+            t_cpu = _median_time(lambda: pyoracle.lanczos_twosite(Lb, Rb, psi, mpo[c], mpo[c + 1], itk, itk, 10, 1e-12,
+                                                                  True, nthreads=cb["cores"]))
+            row["lanczos10"] = {"gpu_ms": round(t_gpu * 1e3, 3), "cpu_ms_all_cores": round(t_cpu * 1e3, 3)}
+        row["build_s"] = round(time.perf_counter() - t0, 1)
+        log(name, json.dumps(row))
+        out[name] = row
+        del prob
+        native.lib().mpsa_release_pools()
+    return out
+
+
 def lanczos_compare(torch, prob, mixed, args):
     """Bond ground state, the reference's plain Lanczos (TSLanzcosEngine rules)
     run to convergence (|dE| < 1e-10 between tests) in three ways:
@@ -241,6 +296,7 @@ def main():
                     help="time the split and environment update of the bond and estimate a sweep")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the plans' heuristic tile classes instead of autotuning them")
+    ap.add_argument("--no-u1", action="store_true", help="skip the U(1) bonds of the other workloads")
     ap.add_argument("--calib", action="store_true",
                     help="run a known-size axpby (2 GiB read, 1 GiB written) for HBM counter calibration")
     args = ap.parse_args()
@@ -330,6 +386,9 @@ def main():
     sweep = None
     if args.sweep_estimate and lan is not None and args.workload == "dense":
         sweep = sweep_estimate(torch, prob, lan, args.L, args.D)
+    u1 = None
+    if args.workload == "dense" and not args.no_u1:
+        u1 = u1_bonds(torch, args.steps, args.warmup)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline: oracle contractTwosite on host cores (1 warm-up, median of 5; all cores and 1 thread)")
@@ -376,6 +435,8 @@ def main():
         out["lanczos"] = lan
     if sweep is not None:
         out["sweep_estimate"] = sweep
+    if u1 is not None:
+        out["u1_bonds"] = u1
     print(json.dumps(out), flush=True)
 
 

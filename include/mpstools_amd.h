@@ -219,6 +219,16 @@ int mpsa_plan_vector_lengths(const mpsa_plan* plan, int64_t* x_len, int64_t* y_l
  * and states). */
 int mpsa_lanczos(mpsa_plan* plan, const double* dpsi0, double* dstate, int itmax, double delta, int testenergy,
                  int target, double* energies, int cap, int* nit, int* nritz, mpsa_stream stream);
+/* mpsa_lanczos with the options of the reference's projected run
+ * (TSLanzcosEngine::simulate(itmax, delta, testenergy, state, delta_ortho),
+ * src/lanzcos.hpp:281-414) for both convergence tests: nproj states at
+ * dproj + p*proj_stride projected out of the start vector and of every new
+ * recurrence vector, reortho = 1 for full FP64 reorthogonalisation; see
+ * mpsa_lanczos_batched_ex for the semantics (with testenergy the run is that
+ * entry point with one vector). */
+int mpsa_lanczos_ex(mpsa_plan* plan, const double* dpsi0, double* dstate, int itmax, double delta, int testenergy,
+                    int target, double* energies, int cap, int* nit, int* nritz, const double* dproj, int nproj,
+                    int64_t proj_stride, int reortho, mpsa_stream stream);
 /* nvec independent runs of mpsa_lanczos's recurrence (Ritz-value convergence
  * test only) from the vectors dpsi0 + v*stride, v < nvec (the reference runs
  * one TSLanzcosEngine per vector, src/lanzcos.hpp:165-278).  Every step is one

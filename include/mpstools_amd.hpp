@@ -734,6 +734,21 @@ class TSLanzcosEngine {
   void setAinit(const BlockMatrix<2, KeyType, T>& A) { _Ainit = A; }
 
   int simulate(const int itmax, const double delta, const bool testenergy) {
+    return simulate_impl(itmax, delta, testenergy, nullptr);
+  }
+  // The projected run (src/lanzcos.hpp:281-414): the start vector and every
+  // new Lanczos vector lose their component along `state`.  delta_ortho is
+  // accepted and unused, as in the reference (its Gram-Schmidt block is
+  // commented out).  The state's blocks must lie in the run's vector space.
+  int simulate(const int itmax, const double delta, const bool testenergy, BlockMatrix<2, KeyType, T>& state,
+               const T delta_ortho) {
+    (void)delta_ortho;
+    return simulate_impl(itmax, delta, testenergy, &state);
+  }
+
+ private:
+  int simulate_impl(const int itmax, const double delta, const bool testenergy,
+                    const BlockMatrix<2, KeyType, T>* state) {
     using namespace detail;
     Tab tl = table_of(_L), tr = table_of(_R), tx = table_of(_Ainit);
     Mpo ml = mpo_of(_mpol), mr = mpo_of(_mpor);
@@ -785,11 +800,25 @@ class TSLanzcosEngine {
       upload(full, _space, dx);
     }
     mpsa_check(mpsa_plan_bind_env(plan, dL.p, dR.p, nullptr), "TSLanzcosEngine");
+    _proj.reset();
+    if (state) {
+      BlockMatrix<2, KeyType, T> full;
+      for (auto& kv : space) full[kv.first] = matrix<T>((size_t)kv.second.first, (size_t)kv.second.second);
+      for (auto& kv : *state) {
+        auto f = space.find(kv.first);
+        if (f == space.end() || (int64_t)kv.second.size1() != f->second.first ||
+            (int64_t)kv.second.size2() != f->second.second)
+          mps_exception("TSLanzcosEngine::simulate: the state has a block outside the Lanczos vector space");
+        full[kv.first] = kv.second;
+      }
+      _proj.reset(new DevBuf((size_t)_space.len()));
+      upload(full, _space, *_proj);
+    }
     _es.assign((size_t)itmax + 2, 0.0);
     int nit = 0, nr = 0;
     _gs.reset(new DevBuf((size_t)_space.len()));
-    mpsa_check(mpsa_lanczos(plan, dx.p, _gs->p, itmax, delta, testenergy ? 1 : 0, 0, _es.data(), itmax + 2, &nit, &nr,
-                            nullptr),
+    mpsa_check(mpsa_lanczos_ex(plan, dx.p, _gs->p, itmax, delta, testenergy ? 1 : 0, 0, _es.data(), itmax + 2, &nit,
+                               &nr, _proj ? _proj->p : nullptr, _proj ? 1 : 0, _space.len(), 0, nullptr),
                "TSLanzcosEngine::simulate");
     _es.resize((size_t)nr);
     _dx.reset(new DevBuf((size_t)_space.len()));
@@ -803,6 +832,8 @@ class TSLanzcosEngine {
     if (_verbose == 1 && !_es.empty()) std::cout << "E=" << _es[0] << " after " << nit << " iterations; ";
     return nit;
   }
+
+ public:
 
   T E(const int i) const { return T(_es.at((size_t)i)); }  // Ritz values are real (Hermitian H_eff)
 
@@ -819,8 +850,8 @@ class TSLanzcosEngine {
     DevBuf st((size_t)_space.len());
     std::vector<double> es(_es.size() + 2);
     int nit = 0, nr = 0;
-    mpsa_check(mpsa_lanczos(_plan_keep->p, _dx->p, st.p, (int)_es.size(), 0.0, 1, j, es.data(), (int)es.size(), &nit,
-                            &nr, nullptr),
+    mpsa_check(mpsa_lanczos_ex(_plan_keep->p, _dx->p, st.p, (int)_es.size(), 0.0, 1, j, es.data(), (int)es.size(),
+                               &nit, &nr, _proj ? _proj->p : nullptr, _proj ? 1 : 0, _space.len(), 0, nullptr),
                "TSLanzcosEngine::getState");
     download(out, _space, st);
     return out;
@@ -842,7 +873,7 @@ class TSLanzcosEngine {
   int _prec;
   std::vector<double> _es;
   detail::Tab _space;
-  std::unique_ptr<detail::DevBuf> _gs, _dx, _keepL, _keepR;
+  std::unique_ptr<detail::DevBuf> _gs, _dx, _keepL, _keepR, _proj;
   std::unique_ptr<PlanHolder> _plan_keep;
 };
 

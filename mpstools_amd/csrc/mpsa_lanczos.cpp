@@ -233,8 +233,23 @@ int mpsa_tridiag_eig(int n, double* d, const double* e, double* z) {
 
 int mpsa_lanczos(mpsa_plan* plan, const double* dpsi0, double* dstate, int itmax, double delta, int testenergy,
                  int target, double* energies, int cap, int* nit, int* nritz, mpsa_stream stream) {
+  return mpsa_lanczos_ex(plan, dpsi0, dstate, itmax, delta, testenergy, target, energies, cap, nit, nritz, nullptr, 0,
+                         0, 0, stream);
+}
+
+int mpsa_lanczos_ex(mpsa_plan* plan, const double* dpsi0, double* dstate, int itmax, double delta, int testenergy,
+                    int target, double* energies, int cap, int* nit, int* nritz, const double* dproj, int nproj,
+                    int64_t proj_stride, int reortho, mpsa_stream stream) {
+  std::vector<void*> small;  // device tables, freed on exit
+  auto cleanup = [&]() {
+    if (!small.empty()) (void)hipStreamSynchronize((hipStream_t)stream);
+    for (void* p : small) (void)hipFree(p);
+    small.clear();
+  };
   try {
     if (!plan || !dpsi0 || itmax < 1 || !nit || !nritz) throw LErr(MPSA_EINVAL, "lanczos: bad arguments");
+    if (nproj < 0 || (nproj > 0 && !dproj) || (reortho != 0 && reortho != 1))
+      throw LErr(MPSA_EINVAL, "lanczos: bad projection arguments");
     int64_t xl = 0, yl = 0;
     int same = 0;
     chk(mpsa_plan_vector_lengths(plan, &xl, &yl, &same));
@@ -245,15 +260,63 @@ int mpsa_lanczos(mpsa_plan* plan, const double* dpsi0, double* dstate, int itmax
     // per test -- the same steps, stops and Ritz vector as the loop below,
     // without a host round trip per step.
     if (testenergy)
-      return mpsa_lanczos_batched(plan, dpsi0, std::max<int64_t>(n, 1), 1, dstate, itmax, delta, target, energies,
-                                  cap, nit, nritz, 1, stream);
+      return mpsa_lanczos_batched_ex(plan, dpsi0, std::max<int64_t>(n, 1), 1, dstate, itmax, delta, target,
+                                     energies, cap, nit, nritz, 1, dproj, nproj, proj_stride, 0, reortho, stream);
+    if (nproj > 0 && proj_stride < n) throw LErr(MPSA_EINVAL, "lanczos: projection stride smaller than the vector");
     hipStream_t st = (hipStream_t)stream;
     Vecs V{{}, n, st};
+    auto dalloc = [&](size_t bytes) {
+      void* p = nullptr;
+      if (hipMalloc(&p, std::max<size_t>(bytes, 16)) != hipSuccess) throw LErr(MPSA_ENOMEM, "lanczos alloc");
+      small.push_back(p);
+      return p;
+    };
+    // projection set (orthonormalised once: s and i s on a complex plan) and
+    // the device copy of the basis table for reorthogonalisation
+    const bool cplx = plan_complex(plan);
+    const int nq = cplx ? 2 * nproj : nproj;
+    const int nbmax = std::max(nq, reortho ? itmax + 1 : 0);
+    const int nparts = (int)std::max<int64_t>(1, std::min<int64_t>(256, (n + 8191) / 8192));
+    double* ppart = nbmax > 0 ? (double*)dalloc(sizeof(double) * (size_t)nbmax * nparts) : nullptr;
+    double* pcoef = nbmax > 0 ? (double*)dalloc(sizeof(double) * (size_t)nbmax) : nullptr;
+    double** ptab = nq > 0 ? (double**)dalloc(sizeof(double*) * (size_t)nq) : nullptr;
+    double** btab = reortho ? (double**)dalloc(sizeof(double*) * (size_t)(itmax + 1)) : nullptr;
+    std::vector<double*> hq((size_t)nq), hb;
+    hb.reserve((size_t)itmax + 1);  // stable storage: its entries are copied to btab asynchronously
+    if (nproj > 0) {
+      const int64_t e = n / 2;
+      for (int p = 0; p < nproj; ++p) {
+        double* Q = hq[(size_t)(cplx ? 2 * p : p)] = V.get();
+        hchk(hipMemcpyAsync(Q, dproj + p * proj_stride, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, st),
+             "copy projection state");
+        if (cplx) {
+          double* R = hq[(size_t)(2 * p + 1)] = V.get();
+          hchk(launch_axpby(-1.0, 0.0, Q + e, 0.0, 0.0, R, e, 0, st), "i s");
+          hchk(launch_axpby(1.0, 0.0, Q, 0.0, 0.0, R + e, e, 0, st), "i s");
+        }
+      }
+      hchk(hipMemcpyAsync(ptab, hq.data(), sizeof(double*) * (size_t)nq, hipMemcpyHostToDevice, st), "state table");
+      for (int p = 0; p < nq; ++p) {
+        double* Q = hq[(size_t)p];
+        const double n0 = ddot(Q, Q, n, st);
+        for (int pass = 0; pass < 2 && p > 0; ++pass)
+          hchk(launch_project_out(Q, ptab, 0, nullptr, p, nbmax, n, n, 1, ppart, nparts, pcoef, st), "state GS");
+        const double n1 = ddot(Q, Q, n, st);
+        if (!(n0 > 0.0) || !(n1 > 1e-24 * n0))
+          throw LErr(MPSA_EINVAL, "lanczos: projection state " + std::to_string(cplx ? p / 2 : p) +
+                                      " is zero or linearly dependent on the earlier ones");
+        axpby(0.0, Q, 1.0 / std::sqrt(n1), Q, n, st);
+      }
+    }
+    auto project = [&](double* W) {
+      if (nq > 0) hchk(launch_project_out(W, ptab, 0, nullptr, nq, nbmax, n, n, 1, ppart, nparts, pcoef, st), "project");
+    };
     std::vector<double*> As;
     double* A0 = nullptr;
     double* A1 = V.get();
     double* A2 = V.get();
     hchk(hipMemcpyAsync(A1, dpsi0, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, st), "copy psi0");
+    project(A1);
     std::vector<double> es, ks;
     int it = 0;
     double k = 0.0, e = 0.0, eold = 1e100;
@@ -290,6 +353,15 @@ int mpsa_lanczos(mpsa_plan* plan, const double* dpsi0, double* dstate, int itmax
       // A3 = (-e A1 + A2) + (-k A0), built in A2's storage
       axpby(-e, A1, 1.0, A2, n, st);
       if (!first) axpby(-k, A0, 1.0, A2, n, st);
+      if (reortho) {  // against the basis A_0 .. A_it, twice
+        hb.push_back(A1);
+        hchk(hipMemcpyAsync(btab + (hb.size() - 1), &hb.back(), sizeof(double*), hipMemcpyHostToDevice, st),
+             "basis table");
+        for (int pass = 0; pass < 2; ++pass)
+          hchk(launch_project_out(A2, btab, 0, nullptr, (int)hb.size(), nbmax, n, n, 1, ppart, nparts, pcoef, st),
+               "reorthogonalise");
+      }
+      project(A2);
       first = false;
       A0 = A1;
       A1 = A2;
@@ -312,11 +384,14 @@ int mpsa_lanczos(mpsa_plan* plan, const double* dpsi0, double* dstate, int itmax
       }
     }
     hchk(hipStreamSynchronize(st), "lanczos sync");
+    cleanup();
     return MPSA_OK;
   } catch (const LErr& e) {
+    cleanup();
     set_last_error(e.what());
     return e.code;
   } catch (const std::exception& e) {
+    cleanup();
     set_last_error(e.what());
     return MPSA_EINVAL;
   }

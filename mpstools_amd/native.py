@@ -79,6 +79,8 @@ def lib():
         L.mpsa_lanczos.argtypes = [P, P, P, C.c_int, C.c_double, C.c_int, C.c_int, P, C.c_int, P, P, P]
         L.mpsa_lanczos_batched.argtypes = [P, P, C.c_int64, C.c_int, P, C.c_int, C.c_double, C.c_int, P, C.c_int,
                                             P, P, C.c_int, P]
+        L.mpsa_lanczos_ex.argtypes = [P, P, P, C.c_int, C.c_double, C.c_int, C.c_int, P, C.c_int, P, P, P, C.c_int,
+                                       C.c_int64, C.c_int, P]
         L.mpsa_lanczos_batched_ex.argtypes = [P, P, C.c_int64, C.c_int, P, C.c_int, C.c_double, C.c_int, P, C.c_int,
                                                P, P, C.c_int, P, C.c_int, C.c_int64, C.c_int64, C.c_int, P]
         L.mpsa_chebyshev_moments.argtypes = [P, P, C.c_int, C.c_double, C.c_double, P, P, P]
@@ -408,16 +410,21 @@ class EnvPlan:
         _check(lib().mpsa_env_apply(self.h, _dptr(dBL), _dptr(dA), _dptr(dAp), _dptr(dout), _stream()))
 
 
-def lanczos(plan, dpsi0, itmax, delta, testenergy=True, target=0, want_state=True):
-    """TSLanzcosEngine::simulate + getState on the device (src/lanzcos.hpp:165-278)."""
+def lanczos(plan, dpsi0, itmax, delta, testenergy=True, target=0, want_state=True, proj=None, nproj=0,
+            proj_stride=0, reortho=False):
+    """TSLanzcosEngine::simulate + getState on the device (src/lanzcos.hpp:165-278);
+    with nproj > 0 the projected run against the states in `proj` (state p at
+    offset p * proj_stride, src/lanzcos.hpp:281-414); reortho adds full FP64
+    reorthogonalisation."""
     import torch
     cap = itmax + 2
     E = np.zeros(cap)
     nit = np.zeros(1, np.int32)
     nr = np.zeros(1, np.int32)
     state = torch.zeros_like(dpsi0) if want_state else None
-    _check(lib().mpsa_lanczos(plan.h, _dptr(dpsi0), _dptr(state), int(itmax), float(delta), int(testenergy),
-                              int(target), _ptr(E), cap, _ptr(nit), _ptr(nr), _stream()))
+    _check(lib().mpsa_lanczos_ex(plan.h, _dptr(dpsi0), _dptr(state), int(itmax), float(delta), int(testenergy),
+                                 int(target), _ptr(E), cap, _ptr(nit), _ptr(nr), _dptr(proj), int(nproj),
+                                 int(proj_stride), int(bool(reortho)), _stream()))
     return dict(it=int(nit[0]), energies=E[:int(nr[0])].copy(), state=state)
 
 

@@ -289,6 +289,22 @@ int main() {
       for (size_t i = 0; i < kv.second.a.size(); ++i) ov += f->second.data()[i] * kv.second.a[i];
     }
     report("TSLanzcosEngine state overlap", std::fabs(std::fabs(ov) - 1.0), 1e-6);
+    // the projected run against that ground state (src/lanzcos.hpp:281-414)
+    TSLanzcosEngine<AbKeyType, Real> lp(L, R, mps[(size_t)c], mps[(size_t)c + 1], heis(c, N), heis(c + 1, N), itk, itk, 0);
+    lp.simulate(200, 1e-11, true, gs, 0.0);
+    const auto ogs_dev = to_oracle<2>(gs);
+    auto rp = oracle::lanczos<double>(oracle::twosite(to_oracle<1>(mps[(size_t)c]), to_oracle<1>(mps[(size_t)c + 1])), mv,
+                                      200, 1e-11, true, &ogs_dev);
+    report("TSLanzcosEngine projected E0", std::fabs(lp.E(0) - rp.es[0]), 1e-9);
+    report("TSLanzcosEngine projected above E0", lp.E(0) > lan.E(0) + 1e-8 ? 0.0 : 1.0, 0.0);
+    auto xs = lp.getState(0);
+    double ovp = 0;
+    for (auto& kv : xs) {
+      auto f = gs.find(kv.first);
+      if (f == gs.end()) continue;
+      for (size_t i = 0; i < kv.second.size1() * kv.second.size2(); ++i) ovp += f->second.data()[i] * kv.second.data()[i];
+    }
+    report("TSLanzcosEngine projected state orthogonal", std::fabs(ovp), 1e-9);
   }
   // complex128 blocks (planar on the device) against the oracle's complex template
   {

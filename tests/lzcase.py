@@ -78,6 +78,23 @@ def run(plan, dev, host, x0, itmax, delta, proj=None, nproj=0, proj_vstride=0, r
                 states=host(ds).reshape(nvec, n))
 
 
+def run_single(plan, dev, host, x0, itmax, delta, testenergy, proj=None, nproj=0, reortho=False, target=0,
+               stream=None):
+    """mpsa_lanczos_ex on one start vector (host array)."""
+    x0 = np.asarray(x0, np.float64)
+    n = x0.size
+    cap = itmax + 2
+    E = np.zeros(cap)
+    nit, nr = np.zeros(1, np.int32), np.zeros(1, np.int32)
+    dx, ds = dev(x0), dev(np.zeros(n))
+    dp = dev(np.asarray(proj, np.float64).ravel()) if proj is not None else None
+    native._check(native.lib().mpsa_lanczos_ex(
+        plan.h, C.c_void_p(dx.data_ptr()), C.c_void_p(ds.data_ptr()), itmax, float(delta), int(testenergy), target,
+        native._ptr(E), cap, native._ptr(nit), native._ptr(nr), C.c_void_p(dp.data_ptr()) if dp is not None else None,
+        nproj, n, int(reortho), stream))
+    return dict(it=int(nit[0]), energies=E[:int(nr[0])].copy(), state=host(ds))
+
+
 def _close(a, b, tol):
     return abs(a - b) <= tol * max(1.0, abs(b))
 
@@ -106,6 +123,13 @@ def check_projection(oracle, K, dev, host, stream=None):
     r3 = run(plan, dev, host, np.stack([x0, x0]), 300, 1e-12, proj=np.stack([g0, g1]), nproj=1, proj_vstride=1,
              stream=stream)
     assert _close(r3["energies"][0][0], E[1], 1e-9) and _close(r3["energies"][1][0], E[0], 1e-9)
+    # the residual test (host-driven loop) projected the same way
+    r4 = run_single(plan, dev, host, x0, 300, 1e-9, False, proj=g0, nproj=1, stream=stream)
+    o4 = oracle.lanczos_twosite(K["Lb"], K["Rb"], K["psi"], K["mpo"][c], K["mpo"][c + 1], K["itk"][c],
+                                K["itk"][c + 1], 300, 1e-9, False, state=vt.unpack(g0))
+    assert r4["it"] == o4["it"]
+    assert np.max(np.abs(r4["energies"] - o4["energies"])) <= 1e-9 * max(1.0, np.max(np.abs(o4["energies"])))
+    assert _close(r4["energies"][0], E[1], 1e-8) and abs(np.dot(g0, r4["state"])) <= 1e-9
     # a zero state and a dependent pair are errors
     for bad, nb in [(np.zeros_like(g0), 1), (np.stack([g0, -2.5 * g0]), 2)]:
         try:
@@ -130,6 +154,9 @@ def check_reortho(oracle, K, dev, host, stream=None):
     # once the extreme Ritz values converge): this case needs the option
     p = run(plan, dev, host, x0, m, 0.0, reortho=False, stream=stream)
     assert np.max(np.abs(p["energies"][0] - E)) > 1e-6 * scale
+    # the host-driven loop (residual test) reorthogonalises the same way
+    r = run_single(plan, dev, host, x0, m, 0.0, False, reortho=True, stream=stream)
+    assert r["it"] == m and np.max(np.abs(r["energies"] - E)) <= 1e-9 * scale
     # a short run: the oracle's (plain) Ritz values to rounding
     r = run(plan, dev, host, x0, 12, 1e-12, reortho=True, stream=stream)
     o = oracle.lanczos_twosite(K["Lb"], K["Rb"], K["psi"], K["mpo"][c], K["mpo"][c + 1], K["itk"][c],
