@@ -454,6 +454,15 @@ int mpsa_cheb_destroy(mpsa_cheb* c);
  * X*Y = sum conj(Y) X (src/BlockTensor.hpp:520-544), computed in FP64 with a
  * fixed-order two-level reduction; result written to host *out (re, im). */
 int mpsa_dot(const double* dx, const double* dy, int64_t n, int is_complex, double* out, mpsa_stream stream);
+/* mpsa_dot for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc); no
+ * counterpart in the reference.  The partial sums live in a file-scope device
+ * array (1024 pairs, 16 KiB per GPU, any n accepted), the result (re, im) is
+ * written to the device pointer dout on `stream` with no host synchronisation,
+ * bitwise equal to mpsa_dot's.  The array is shared by every call on one GPU:
+ * calls that overlap on one GPU race. */
+int mpsa_dot_noalloc(const double* dx, const double* dy, int64_t n, int is_complex, double* dout,
+                     mpsa_stream stream);
 /* Seeded N(0, 1) entries (no reference counterpart: the reference's
  * initialisers are time-seeded, src/BlockTensor.hpp:394-404):
  * x[i] = sqrt(-2 ln u_{2i}) cos(2 pi u_{2i+1}) with u_j = ((SplitMix64(seed +
