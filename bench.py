@@ -198,6 +198,247 @@ def u1_bonds(torch, steps, warmup):
     return out
 
 
+def chebyshev(torch, args):
+    """The Chebyshev workload: spinful Fermi-Hubbard chain (t = 1, U = 4,
+    U(1) x U(1) charges, d = 4), L = 80, half filling, MPO rescaled into
+    [-1, 1], nvec seeded start vectors (device generator, canonicalised on
+    the device) of bond dimension D in complex128, moments of
+    simulateSSCompress (src/container.hpp:2343-2470) with chimax = D and 2
+    compression sweeps.  A moment n >= 2 is one compressSingleSite of H t_{n-1},
+    the sum 2 H t_{n-1} - t_{n-2}, one compress of it and the overlaps; its wall
+    time (moment 2 of a 3-moment run) is the figure, measured with the
+    one-site contraction in FP64 (complex128) and in FP32 (complex64).  The CPU
+    baseline is the oracle's numpy restatement (oracle/pycheb.py) on a bounded
+    sample: one start vector, real, D = 32, the same chain."""
+    from mpstools_amd import models, native
+    L, D, nv = 80, args.cheb_D, args.cheb_nvec
+    itk = [models.itokey_u1xu1()] * L
+    N = models.pack_charge(L // 2, L // 2)
+    mpo = models.rescale_mpo(models.hubbard_mpo(L, 1.0, 4.0), 4.0 * (L - 1) + 4.0 * L, 0.0)
+    out = {"workload": f"Fermi-Hubbard t=1 U=4, L={L}, d=4, U(1)xU(1), MPO w={mpo[1].Dr}, D={D}, {nv} start "
+                       f"vectors, complex128 states, chimax=D, 2 compression sweeps"}
+    for name, prec in (("complex128", native.F64), ("complex64_contraction", native.F32)):
+        t0 = time.perf_counter()
+        e = native.ChebRandom(L, itk, N, D, nv, mpo, args.seed + 2, cplx=True, h1_precision=prec)
+        torch.cuda.synchronize()
+        setup = time.perf_counter() - t0
+        r = e.moments(3, D, 2)
+        st = e.stats()
+        out[name] = {"s_per_moment": round(float(r["moment_s"][2]), 3),
+                     "moment_s": [round(float(x), 3) for x in r["moment_s"]], "setup_s": round(setup, 2),
+                     "mu": [[round(float(z.real), 10), round(float(z.imag), 10)] for z in r["moments"][0]],
+                     "host_plan_s": round(st["host_plan_s"], 2), "host_keys_s": round(st["host_keys_s"], 2),
+                     "device_bytes": st["pool_bytes"]}
+        log("chebyshev", name, json.dumps(out[name]))
+        del e
+        native.lib().mpsa_release_pools()
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pycheb
+        from mpstools_amd import synth
+        Dc = 32
+        mps, _ = synth.random_mps(L, itk, N, Dc, 7)
+        for i in range(L - 1, 0, -1):
+            mps[i], mps[i - 1] = synth.right_normalize(mps[i], mps[i - 1])
+        mps[0] = mps[0].scaled(1.0 / mps[0].norm())
+        t0 = time.perf_counter()
+        pycheb.chebyshev_moments(mpo, pycheb.MPS(mps, itk), 3, Dc, 2)
+        tc = time.perf_counter() - t0
+        e = native.ChebRandom(L, itk, N, Dc, 1, mpo, args.seed + 3, cplx=False)
+        r = e.moments(3, Dc, 2)
+        out["cpu_baseline"] = {"s_3_moments": round(tc, 3), "kind": "port",
+                               "sample": f"oracle numpy restatement, 1 real start vector, D={Dc}, 3 moments",
+                               "gpu_s_3_moments_same_sample": round(float(r["s"]), 3)}
+        del e
+    return out
+
+
+def lanczos_compare(torch, prob, mixed, args):
+    """Bond ground state, the reference's plain Lanczos (TSLanzcosEngine rules)
+    run to convergence (|dE| < 1e-10 between tests) in three ways:
+      fp64   : every matvec in FP64;
+      mixed  : matvecs in `mixed` precision, recurrence/tridiagonal solve in FP64;
+      refined: the mixed Ritz vector followed by a short FP64 Lanczos (at most
+               `refine_it` FP64 matvecs), the final FP64 refinement step.
+    |dE|/L is reported for mixed and refined against fp64."""
+    from mpstools_amd import native
+    L = prob["L"]
+    psi = prob["psi"]
+    it_max, delta = args.lanczos_itmax, 1e-10
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g64 = native.lanczos(prob["plan_f64"], psi, it_max, delta, True)
+    t64 = time.perf_counter() - t0
+    log(f"lanczos f64: {g64['it']} it, E={g64['energies'][0]:.12f}, {t64:.2f}s")
+    t0 = time.perf_counter()
+    gm = native.lanczos(prob["plan_" + mixed], psi, it_max, delta, True)
+    tm = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    gr = native.lanczos(prob["plan_f64"], gm["state"], args.refine_it, delta, True, want_state=False)
+    tr = time.perf_counter() - t0
+    e64, emix, eref = g64["energies"][0], gm["energies"][0], gr["energies"][0]
+    log(f"lanczos {mixed}: {gm['it']} it E={emix:.12f} {tm:.2f}s; fp64 refinement {gr['it']} it "
+        f"E={eref:.12f} {tr:.2f}s")
+    native.lib().mpsa_release_pools()
+    return {"E_fp64": e64, "it_fp64": g64["it"], "s_fp64": round(t64, 4),
+            "E_mixed": emix, "it_mixed": gm["it"], "s_mixed": round(tm, 4),
+            "E_refined": eref, "it_refine_fp64": gr["it"], "s_refine": round(tr, 4),
+            "dE_per_site_mixed": abs(emix - e64) / L, "dE_per_site_refined": abs(eref - e64) / L,
+            "meets_1e-9": bool(abs(eref - e64) / L <= 1e-9),
+            "solve_speedup": round(t64 / (tm + tr), 3)}
+
+
+def sweep_segment(torch, prob, mixed, args, nbonds):
+    """nbonds consecutive full-D bond steps of a left-to-right two-site sweep
+    (DMRGengine::doStep, src/container.hpp:293-337), each timed on the
+    device from its parts:
+      lanczos : the mixed-precision Lanczos from the bond's two-site state,
+                then at most refine_it FP64 steps from its Ritz vector;
+      split   : split of the refined ground state keeping chi = D
+                (src/MPSfunctions.hpp:1142-1278), lambda into the right tensor;
+      env     : L[b] = addContractionLayer(L[b-1], A_b) (:442-493);
+      twosite : the next bond's state BlockMatrix<2>(lambda V_{b+1}, A_{b+2})
+                (src/BlockTensor.hpp:285-301) -- the reference's sweep makes
+                it the same way;
+    plus building the bond's H_eff plans.  A sweep is 2 (L - 2) bond steps;
+    the measured steps are bulk bonds, so sweep_s = 2 (L - 2) x the mean
+    measured step is an upper estimate (bonds near the ends are smaller)."""
+    from mpstools_amd import native
+    c, mpo, itk, D = prob["c"], prob["mpo"], prob["itk"], prob["D"]
+    Lt, dL = prob["Lt"], prob["dL"]
+    vt, psi = prob["vt"], prob["psi"]
+    steps = []
+    for b in range(c, c + nbonds):
+        rec = {"bond": b}
+        t0 = time.perf_counter()
+        Rt, dR = (prob["Rt"], prob["dR"]) if b == c else prob["R"][b + 2]
+        if b == c:
+            pm, p64 = prob["plan_" + mixed], prob["plan_f64"]
+        else:
+            pm = native.HeffPlan(Lt, Rt, vt, vt, (mpo[b], mpo[b + 1]), (itk, itk), native.PREC[mixed])
+            pm.bind_env(dL, dR)
+            p64 = native.HeffPlan(Lt, Rt, vt, vt, (mpo[b], mpo[b + 1]), (itk, itk), native.F64)
+            p64.bind_env(dL, dR)
+        torch.cuda.synchronize()
+        rec["plans_s"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        gm = native.lanczos(pm, psi, args.lanczos_itmax, 1e-10, True)
+        gr = native.lanczos(p64, gm["state"], args.refine_it, 1e-10, True)
+        torch.cuda.synchronize()
+        rec["lanczos_s"] = time.perf_counter() - t0
+        rec["E"] = gr["energies"][0]
+        rec["it"] = gm["it"] + gr["it"]
+        t0 = time.perf_counter()
+        sp = native.split(vt, gr["state"], itk, itk, direction=1, chimax=D)
+        torch.cuda.synchronize()
+        rec["split_s"] = time.perf_counter() - t0
+        rec["chi"], rec["tw"] = sp["chi"], sp["tw"]
+        At, dA = sp["left"]
+        Bt, dB = sp["right"]
+        t0 = time.perf_counter()
+        outt = native.env_outkeys(Lt, At, mpo[b], itk, 1)
+        ep = native.EnvPlan(Lt, At, mpo[b], itk, 1, outt)
+        dLn = torch.zeros(max(outt.total, 1), dtype=torch.float64, device="cuda")
+        ep.apply(dL, dA, dLn)
+        torch.cuda.synchronize()
+        rec["env_s"] = time.perf_counter() - t0
+        Lt, dL = outt, dLn
+        t0 = time.perf_counter()
+        if b + 1 < c + nbonds:
+            Ct, dC = prob["sites"][b + 2]
+            xt = native.twosite_outkeys(Bt, Ct)
+            dx = torch.zeros(max(xt.total, 1), dtype=torch.float64, device="cuda")
+            native.twosite_product(Bt, dB, Ct, dC, xt, dx)
+            yt = native.heff2_outkeys(Lt, prob["R"][b + 3][0], xt, mpo[b + 1], mpo[b + 2], itk, itk)
+            vt = xt.union(yt)
+            psi = torch.zeros(max(vt.total, 1), dtype=torch.float64, device="cuda")
+            for k_, (o_new, o_old, n_) in enumerate(_block_moves(xt, vt)):
+                psi[o_new:o_new + n_] = dx[o_old:o_old + n_]
+        torch.cuda.synchronize()
+        rec["twosite_s"] = time.perf_counter() - t0
+        rec["step_s"] = sum(rec[k] for k in ("plans_s", "lanczos_s", "split_s", "env_s", "twosite_s"))
+        log(f"bond {b}: E={rec['E']:.10f} it={rec['it']} chi={rec['chi']} " +
+            " ".join(f"{k}={rec[k]:.3f}" for k in ("plans_s", "lanczos_s", "split_s", "env_s", "twosite_s", "step_s")))
+        steps.append({k: (round(v, 4) if isinstance(v, float) and k != "E" and k != "tw" else v)
+                      for k, v in rec.items()})
+        del pm, p64
+        native.lib().mpsa_release_pools()
+    mean = sum(r["step_s"] for r in steps) / len(steps)
+    L = prob["L"]
+    return {"bond_steps_measured": len(steps), "steps": steps, "mean_bond_step_s": round(mean, 4),
+            "bond_steps_per_sweep": 2 * (L - 2), "sweep_s": round(2 * (L - 2) * mean, 1),
+            "sweep_s_kind": f"measured {len(steps)} consecutive full-D bulk bond steps (mixed {mixed} Lanczos + "
+                            f"FP64 refinement, split chi = D, env update, two-site product, plan builds), "
+                            f"extrapolated to 2 (L - 2) = {2 * (L - 2)} steps (upper estimate: end bonds are "
+                            f"smaller)"}
+
+
+def _block_moves(src: "Table", dst: "Table"):
+    """(dst offset, src offset, elements) of every block of src inside dst."""
+    where = {k: i for i, k in enumerate(src.keys())}
+    out = []
+    for i, k in enumerate(dst.keys()):
+        if k in where:
+            j = where[k]
+            out.append((int(dst.offsets[i]), int(src.offsets[j]), int(src.rows[j] * src.cols[j])))
+    return out
+
+
+def u1_bonds(torch, steps, warmup):
+    """The U(1) bonds of the other workloads beside the headline, each with
+    its GPU matvec (FP64 and BF16X3, tuned tiles) and the CPU baseline on the
+    same bond (oracle tuple loop: 1 warm-up, median of 5, all cores and one
+    thread):
+      xxz64  : XXZ L = 32, D = 64 -- one matvec and the 10-step Lanczos of
+               the plain-method workload, both timed on GPU and CPU;
+      tv512  : t-V L = 100, D = 512 -- also 64 vectors per batched matvec;
+      hub1024: Fermi-Hubbard L = 80, D = 1024 (U(1) x U(1)) -- the bond of
+               the Chebyshev workload, real FP64."""
+    from mpstools_amd import bench_problem, native
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    out = {}
+    for name, nvecs in (("xxz64", (1,)), ("tv512", (1, 64)), ("hub1024", (1, 16))):
+        model, L, N, D, seed = WORKLOADS[name]
+        t0 = time.perf_counter()
+        prob = bench_problem.u1_centre(model, L, N, D, seed, precisions=("f64", "bf16x3"))
+        row = {"psi_blocks": prob["psi_blocks"], "psi_block_dims": prob["psi_block_dims"]}
+        for pr in ("f64", "bf16x3"):
+            plan = prob["plan_" + pr]
+            info = plan.info()
+            for nv in nvecs:
+                plan.autotune(nvec=nv, reps=3)
+                v = prob["psi"]
+                x = v.repeat(nv) if nv > 1 else v
+                ms, _ = time_apply(torch, plan, x, torch.zeros_like(x), steps, warmup, nv, v.numel() if nv > 1 else 0)
+                row[f"{pr}_nvec{nv}"] = {"ms": round(ms, 4), "tflops": round(nv * info["flops_alg"] / ms / 1e9, 3)}
+            row["flops_alg"] = info["flops_alg"]
+            row["flops_ref"] = info["flops_ref"]
+        cb = cpu_baseline(prob, name, row["flops_alg"], row["flops_ref"])
+        row["cpu"] = {"ms_all_cores": round(cb["t_all"] * 1e3, 3), "ms_one_thread": round(cb["t_one"] * 1e3, 3),
+                      "cores": cb["cores"], "blas": cb["blas"], "kind": "port",
+                      "sample": "oracle tuple loop on the same bond, 1 warm-up then the median of 5"}
+        row["gpu_f64_vs_cpu_all_cores"] = round(cb["t_all"] * 1e3 / row["f64_nvec1"]["ms"], 1)
+        if name == "xxz64":
+            # the plain-method workload: one matvec plus a 10-step Lanczos
+            plan = prob["plan_f64"]
+            torch.cuda.synchronize()
+            t_gpu = _median_time(lambda: native.lanczos(plan, prob["psi"], 10, 1e-12, True, want_state=False))
+            Lb = prob["Lt"].unpack(prob["dL"].cpu().numpy())
+            Rb = prob["Rt"].unpack(prob["dR"].cpu().numpy())
+            psi = prob["vt"].unpack(prob["psi"].cpu().numpy())
+            c, mpo, itk = prob["c"], prob["mpo"], prob["itk"]
+            t_cpu = _median_time(lambda: pyoracle.lanczos_twosite(Lb, Rb, psi, mpo[c], mpo[c + 1], itk, itk, 10, 1e-12,
+                                                                  True, nthreads=cb["cores"]))
+            row["lanczos10"] = {"gpu_ms": round(t_gpu * 1e3, 3), "cpu_ms_all_cores": round(t_cpu * 1e3, 3)}
+        row["build_s"] = round(time.perf_counter() - t0, 1)
+        log(name, json.dumps(row))
+        out[name] = row
+        del prob
+        native.lib().mpsa_release_pools()
+    return out
+
+
 def lanczos_compare(torch, prob, mixed, args):
     """Bond ground state, the reference's plain Lanczos (TSLanzcosEngine rules)
     run to convergence (|dE| < 1e-10 between tests) in three ways:
@@ -292,11 +533,13 @@ def main():
     ap.add_argument("--workload", default="dense", choices=sorted(WORKLOADS),
                     help="dense: the D=2048 dense bond (the headline); the others are U(1) bonds")
     ap.add_argument("--nvec", type=int, default=1, help="vectors per batched apply")
-    ap.add_argument("--sweep-estimate", action="store_true",
-                    help="time the split and environment update of the bond and estimate a sweep")
+    ap.add_argument("--sweep-bonds", type=int, default=3,
+                    help="full-D bond steps of a sweep measured on the dense bond (0 skips)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the plans' heuristic tile classes instead of autotuning them")
     ap.add_argument("--no-u1", action="store_true", help="skip the U(1) bonds of the other workloads")
+    ap.add_argument("--cheb-D", type=int, default=1024, help="bond dimension of the Chebyshev workload (0 skips)")
+    ap.add_argument("--cheb-nvec", type=int, default=16, help="start vectors of the Chebyshev workload")
     ap.add_argument("--calib", action="store_true",
                     help="run a known-size axpby (2 GiB read, 1 GiB written) for HBM counter calibration")
     args = ap.parse_args()
@@ -312,7 +555,7 @@ def main():
     if args.workload == "dense":
         log(f"building dense Heisenberg centre problem L={args.L} D={args.D}")
         prob = bench_problem.dense_heisenberg_centre(D=args.D, L=args.L, seed=args.seed, precisions=precs,
-                                                     verbose=True)
+                                                     verbose=True, keep_right=args.sweep_bonds)
         workload = (f"dense Heisenberg J=1 centre bond, L={args.L}, w=5, d=2, D={args.D}, seeded Gaussian MPS, "
                     f"QR-normalised, GPU-built environments")
     else:
@@ -384,11 +627,14 @@ def main():
     if not args.no_lanczos and "f64" in res:
         lan = lanczos_compare(torch, prob, mixed, args)
     sweep = None
-    if args.sweep_estimate and lan is not None and args.workload == "dense":
-        sweep = sweep_estimate(torch, prob, lan, args.L, args.D)
+    if args.sweep_bonds > 0 and args.workload == "dense" and "f64" in res:
+        sweep = sweep_segment(torch, prob, mixed, args, args.sweep_bonds)
     u1 = None
     if args.workload == "dense" and not args.no_u1:
         u1 = u1_bonds(torch, args.steps, args.warmup)
+    cheb = None
+    if args.workload == "dense" and args.cheb_D > 0:
+        cheb = chebyshev(torch, args)
     cpu = None
     if not args.no_cpu:
         log("cpu baseline: oracle contractTwosite on host cores (1 warm-up, median of 5; all cores and 1 thread)")
@@ -434,9 +680,11 @@ def main():
     if lan is not None:
         out["lanczos"] = lan
     if sweep is not None:
-        out["sweep_estimate"] = sweep
+        out["dmrg_sweep"] = sweep
     if u1 is not None:
         out["u1_bonds"] = u1
+    if cheb is not None:
+        out["chebyshev"] = cheb
     print(json.dumps(out), flush=True)
 
 

@@ -395,6 +395,43 @@ int mpsa_chain_env_table(const mpsa_chain* chain, int side, int index, int64_t c
                          uint32_t* p, int64_t* rows, int64_t* cols, int64_t* offsets);
 int mpsa_chain_destroy(mpsa_chain* chain);
 
+/* ---- Versioned checkpoint of block-sparse tensors (in place of the Boost
+ * archives of MPSBase::save/load, src/MPSTypes.hpp:377-426, and
+ * Container::write/read, src/container.hpp:87-146) ----
+ * A file holds a list of tensors (key table + packed FP64 data, all real or
+ * all complex128 planar; ns = 1 for MPS tensors and environments, 2 for
+ * two-site states) and caller metadata text.  Format version 1: magic,
+ * version, flags, then per tensor its table, data and an FNV-1a 64 hash; a
+ * trailer hash over the whole file.  mpsa_ckpt_open checks all of them and
+ * refuses a truncated, corrupt or other-version file; the handle holds the
+ * contents until mpsa_ckpt_close.  data_on_device / dst_on_device: the
+ * buffers are device (1) or host (0) memory. */
+typedef struct mpsa_ckpt mpsa_ckpt;
+int mpsa_ckpt_write(const char* path, int ntensors, const mpsa_blocks* tables, const double* const* data,
+                    int data_on_device, const char* meta, mpsa_stream stream);
+int mpsa_ckpt_open(const char* path, mpsa_ckpt** out, int* ntensors, int* is_complex, int64_t* meta_bytes);
+int mpsa_ckpt_meta(const mpsa_ckpt* c, char* dst, int64_t cap);
+/* Table of tensor i (arrays sized by a first call with NULL arrays); *len =
+ * doubles of its data. */
+int mpsa_ckpt_table(const mpsa_ckpt* c, int i, int* ns, int64_t* n_blocks, int64_t* len, uint64_t* q, uint32_t* p,
+                    int64_t* rows, int64_t* cols, int64_t* offsets);
+int mpsa_ckpt_read(const mpsa_ckpt* c, int i, double* dst, int dst_on_device, mpsa_stream stream);
+int mpsa_ckpt_close(mpsa_ckpt* c);
+
+/* ---- Two-site product BlockMatrix<2,KeyType,T>(B1, B2) (src/BlockTensor.hpp:285-301) ----
+ * For every block pair whose B1 right charge equals the B2 left charge, the
+ * product B1 B2 keyed (B1 left, B2 right, s1 = B1's index, s2 = B2's index);
+ * a key produced by two pairs keeps the later pair's product, as the
+ * reference's assignment does.  b1, b2: ns = 1 tables; out: an ns = 2 table
+ * holding every produced key (mpsa_twosite_outkeys lists them in key order,
+ * cap = 0 queries the count).  All three real FP64 or all complex128 planar.
+ * Blocks of out that no pair produces are zeroed.  FP64 grouped GEMM, one
+ * launch; returns when done. */
+int mpsa_twosite_outkeys(const mpsa_blocks* b1, const mpsa_blocks* b2, int64_t cap, int64_t* n, uint64_t* q,
+                         uint32_t* p, int64_t* rows, int64_t* cols);
+int mpsa_twosite_product(const mpsa_blocks* b1, const double* d1, const mpsa_blocks* b2, const double* d2,
+                         const mpsa_blocks* out, double* dout, mpsa_stream stream);
+
 /* ---- Chebyshev moments with variational MPS compression ----
  * ChebychevEngine::simulateSSCompress (src/container.hpp:2343-2470) and the
  * MPS functions under it, for nvec start vectors at once.  The engine holds
@@ -448,6 +485,10 @@ int mpsa_cheb_stats(const mpsa_cheb* c, int64_t* plans_built, int64_t* pool_byte
  * plans, seconds in output-key queries, seconds waiting on the stream, number
  * of waits, plans cached now}. */
 int mpsa_cheb_profile(const mpsa_cheb* engine, double* out5);
+/* Wall seconds of each moment of the last mpsa_cheb_moments call (moment 0:
+ * the overlap; 1: compressSingleSite; n >= 2: compressSingleSite, the sum
+ * and compress); *n = their count, at most cap written. */
+int mpsa_cheb_moment_times(const mpsa_cheb* engine, double* out, int cap, int* n);
 int mpsa_cheb_destroy(mpsa_cheb* c);
 
 /* ---- Lanczos vector algebra over packed block vectors (BlockTensor.hpp:443-544) ----

@@ -76,7 +76,12 @@ def _add_plan(torch, out, pr, W, itk, seed):
     out["plan_" + pr] = plan
 
 
-def dense_heisenberg_centre(D=2048, L=128, seed=20260003, build_envs="sweep", precisions=("f64",), verbose=False):
+def dense_heisenberg_centre(D=2048, L=128, seed=20260003, build_envs="sweep", precisions=("f64",), verbose=False,
+                            keep_right=0):
+    """keep_right = n also keeps, for a measured run of n bond steps to the
+    right of the centre, the right-normalised site tensors A[c+2 .. c+1+n]
+    (out["sites"][s] = (Table, device buffer)) and the right environments
+    R[c+3 .. c+2+n] (out["R"][s] = (Table, device buffer))."""
     import torch
     dev = "cuda"
     mpo = models.xxz_mpo(L, 1.0, 1.0, 0.0)
@@ -86,6 +91,7 @@ def dense_heisenberg_centre(D=2048, L=128, seed=20260003, build_envs="sweep", pr
     g = torch.Generator(device=dev)
     g.manual_seed(int(seed))
     plans = {}
+    kept_sites, kept_R = {}, {}
 
     def env_step(Et, dE, s, direction):
         dl, dr = dims[s], dims[s + 1]
@@ -110,6 +116,8 @@ def dense_heisenberg_centre(D=2048, L=128, seed=20260003, build_envs="sweep", pr
             dA[o:o + dl * dr] = blocks[i].T.contiguous().reshape(-1)  # column-major
         dout = torch.zeros(max(outt.total, 1), dtype=torch.float64, device=dev)
         plan.apply(dE, dA, dout)
+        if direction <= 0 and c + 2 <= s <= c + 1 + keep_right:
+            kept_sites[s] = (At, dA)
         return outt, dout
 
     if build_envs == "sweep":
@@ -122,6 +130,8 @@ def dense_heisenberg_centre(D=2048, L=128, seed=20260003, build_envs="sweep", pr
         # R[L] = identity on the last bond (getRMax), aux 0
         Et, dE = _lmin_table(), torch.ones(1, dtype=torch.float64, device=dev)
         for s in range(L - 1, c + 1, -1):
+            if c + 3 <= s + 1 <= c + 2 + keep_right:
+                kept_R[s + 1] = (Et, dE)  # R[s + 1], before site s is added
             Et, dE = env_step(Et, dE, s, 0)
             if verbose and s % 16 == 0:
                 print(f"[bench_problem] right env site {s}", file=sys.stderr, flush=True)
@@ -146,7 +156,8 @@ def dense_heisenberg_centre(D=2048, L=128, seed=20260003, build_envs="sweep", pr
         o = int(vt.offsets[i])
         psi[o:o + dl * dr] = blk.T.contiguous().reshape(-1)
     psi /= psi.norm()
-    out = dict(mpo=mpo, itk=itk, c=c, dims=dims, Lt=Lt, Rt=Rt, dL=dL, dR=dR, xt=xt, yt=yt, vt=vt, psi=psi, L=L, D=D)
+    out = dict(mpo=mpo, itk=itk, c=c, dims=dims, Lt=Lt, Rt=Rt, dL=dL, dR=dR, xt=xt, yt=yt, vt=vt, psi=psi, L=L, D=D,
+               sites=kept_sites, R=kept_R)
     for pr in precisions:
         _add_plan(torch, out, pr, (mpo[c], mpo[c + 1]), (itk, itk), seed)
     return out

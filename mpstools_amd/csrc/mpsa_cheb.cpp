@@ -60,7 +60,7 @@ typedef std::array<uint64_t, 3> Key;  // (left charge, right charge, physical in
 // by work queued later on the same stream.
 struct Pool {
   std::multimap<size_t, void*> free_;
-  std::vector<std::pair<void*, size_t>> all;
+  std::unordered_map<void*, size_t> all;  // every allocation and its size
   size_t idle = 0;
   void* get(size_t bytes) {
     bytes = std::max<size_t>((bytes + 255) / 256 * 256, 256);
@@ -77,13 +77,12 @@ struct Pool {
       release_idle();
       if (hipMalloc(&p, bytes) != hipSuccess) throw HErr(MPSA_ENOMEM, "cheb: device allocation failed");
     }
-    all.push_back({p, bytes});
+    all[p] = bytes;
     return p;
   }
   size_t size_of(void* p) const {
-    for (auto& a : all)
-      if (a.first == p) return a.second;
-    return 0;
+    auto f = all.find(p);
+    return f == all.end() ? 0 : f->second;
   }
   void put(void* p) {
     const size_t b = size_of(p);
@@ -93,11 +92,7 @@ struct Pool {
   void release_idle() {
     for (auto& kv : free_) {
       (void)hipFree(kv.second);
-      for (size_t i = 0; i < all.size(); ++i)
-        if (all[i].first == kv.second) {
-          all.erase(all.begin() + (long)i);
-          break;
-        }
+      all.erase(kv.second);
     }
     free_.clear();
     idle = 0;
@@ -195,6 +190,7 @@ struct mpsa_cheb {
     std::string key;
     mpsa_plan* plan;
     int64_t bytes;
+    HTab out;  // the plan's output table (the output-key query of its structure)
   };
   std::list<PlanEntry> plans;
   std::unordered_map<std::string, std::list<PlanEntry>::iterator> plan_index;
@@ -205,6 +201,7 @@ struct mpsa_cheb {
   int64_t plans_built = 0;
   // host time (seconds): plan builds, output-key queries, waits on the stream
   double t_plan = 0.0, t_keys = 0.0, t_sync = 0.0;
+  std::vector<double> moment_s;  // wall seconds of each moment of the last moment loop
   int64_t n_sync = 0;
   ~mpsa_cheb() {
     if (st) (void)hipStreamSynchronize(st);
@@ -228,16 +225,16 @@ struct mpsa_cheb {
     }
     return b;
   }
-  mpsa_plan* find_plan(const std::string& k) {
+  const PlanEntry* find_plan(const std::string& k) {
     auto f = plan_index.find(k);
     if (f == plan_index.end()) return nullptr;
     plans.splice(plans.begin(), plans, f->second);
-    return f->second->plan;
+    return &*f->second;
   }
-  void put_plan(const std::string& k, mpsa_plan* p) {
+  void put_plan(const std::string& k, mpsa_plan* p, const HTab& out) {
     int64_t ws = 0;
     (void)mpsa_plan_info(p, nullptr, nullptr, nullptr, nullptr, &ws);
-    plans.push_front(PlanEntry{k, p, ws});
+    plans.push_front(PlanEntry{k, p, ws, out});
     plan_index[k] = plans.begin();
     plan_bytes += ws;
     ++plans_built;
@@ -399,34 +396,40 @@ void env_layer(Eng& E, const HTab& Et, const double* dE, const HTab& At, const d
                const double* dAp, const HMpo& W, const std::vector<uint64_t>& itk, int dir, HTab& out_t, Buf& out_b) {
   mpsa_blocks be = Et.view(), ba = At.view(), bp = Apt.view();
   mpsa_mpo mw = W.view();
-  int64_t n = 0;
-  std::vector<uint64_t> q;
-  std::vector<uint32_t> p;
-  std::vector<int64_t> r, c;
-  {
-    ScopeTimer tk(E.t_keys);
-    chk(mpsa_env_outkeys(&be, &ba, &bp, &mw, itk.data(), (int)itk.size(), dir, 0, &n, nullptr, nullptr, nullptr,
-                         nullptr));
-    q.resize((size_t)2 * n + 2);
-    p.resize((size_t)n + 1);
-    r.resize((size_t)n + 1);
-    c.resize((size_t)n + 1);
-    if (n)
-      chk(mpsa_env_outkeys(&be, &ba, &bp, &mw, itk.data(), (int)itk.size(), dir, n, &n, q.data(), p.data(),
-                           r.data(), c.data()));
-  }
-  out_t = from_query(n, 1, q, p, r, c);
-  out_t.cplx = Et.cplx;
+  // a plan and its output table are fixed by the structure of the operands
   const std::string key = "E" + std::to_string(dir > 0) + sig_tab(Et) + sig_tab(At) + sig_tab(Apt) + sig_mpo(W);
-  mpsa_plan* pl = E.find_plan(key);
-  if (!pl) {
-    mpsa_blocks bo = out_t.view();
-    ScopeTimer tp(E.t_plan);
-    chk(mpsa_env_plan(&pl, &be, &ba, &bp, &mw, itk.data(), (int)itk.size(), dir, &bo, MPSA_F64));
-    E.put_plan(key, pl);
+  const auto* pe = E.find_plan(key);
+  mpsa_plan* pl = pe ? pe->plan : nullptr;
+  if (pe) {
+    out_t = pe->out;
+  } else {
+    int64_t n = 0;
+    std::vector<uint64_t> q;
+    std::vector<uint32_t> p;
+    std::vector<int64_t> r, c;
+    {
+      ScopeTimer tk(E.t_keys);
+      chk(mpsa_env_outkeys(&be, &ba, &bp, &mw, itk.data(), (int)itk.size(), dir, 0, &n, nullptr, nullptr, nullptr,
+                           nullptr));
+      q.resize((size_t)2 * n + 2);
+      p.resize((size_t)n + 1);
+      r.resize((size_t)n + 1);
+      c.resize((size_t)n + 1);
+      if (n)
+        chk(mpsa_env_outkeys(&be, &ba, &bp, &mw, itk.data(), (int)itk.size(), dir, n, &n, q.data(), p.data(),
+                             r.data(), c.data()));
+    }
+    out_t = from_query(n, 1, q, p, r, c);
+    out_t.cplx = Et.cplx;
+    if (n) {
+      mpsa_blocks bo = out_t.view();
+      ScopeTimer tp(E.t_plan);
+      chk(mpsa_env_plan(&pl, &be, &ba, &bp, &mw, itk.data(), (int)itk.size(), dir, &bo, MPSA_F64));
+      E.put_plan(key, pl, out_t);
+    }
   }
   out_b = Buf(E.pool, out_t.len(), E.st);
-  if (n) chk(mpsa_env_apply(pl, dE, dA, dAp, out_b.p, (mpsa_stream)E.st));
+  if (out_t.n() && pl) chk(mpsa_env_apply(pl, dE, dA, dAp, out_b.p, (mpsa_stream)E.st));
 }
 
 // contractSingle(L, A, W, R): the one-site H_eff applied to A; the output table
@@ -435,34 +438,39 @@ void heff1(Eng& E, const HTab& Lt, const double* dL, const HTab& At, const doubl
            const double* dR, const HMpo& W, const std::vector<uint64_t>& itk, HTab& out_t, Buf& out_b) {
   mpsa_blocks bl = Lt.view(), br = Rt.view(), ba = At.view();
   mpsa_mpo mw = W.view();
-  int64_t n = 0;
-  std::vector<uint64_t> q;
-  std::vector<uint32_t> p;
-  std::vector<int64_t> r, c;
-  {
-    ScopeTimer tk(E.t_keys);
-    chk(mpsa_heff1_outkeys(&bl, &br, &ba, &mw, itk.data(), (int)itk.size(), 0, &n, nullptr, nullptr, nullptr,
-                           nullptr));
-    q.resize((size_t)2 * n + 2);
-    p.resize((size_t)n + 1);
-    r.resize((size_t)n + 1);
-    c.resize((size_t)n + 1);
-    if (n)
-      chk(mpsa_heff1_outkeys(&bl, &br, &ba, &mw, itk.data(), (int)itk.size(), n, &n, q.data(), p.data(), r.data(),
-                             c.data()));
-  }
-  out_t = from_query(n, 1, q, p, r, c);
-  out_t.cplx = At.cplx;
-  out_b = Buf(E.pool, out_t.len(), E.st);
-  if (!n) return;
   const std::string key = "H" + std::to_string(E.h1prec) + sig_tab(Lt) + sig_tab(Rt) + sig_tab(At) + sig_mpo(W);
-  mpsa_plan* pl = E.find_plan(key);
-  if (!pl) {
-    mpsa_blocks by = out_t.view();
-    ScopeTimer tp(E.t_plan);
-    chk(mpsa_heff1_plan(&pl, &bl, &br, &ba, &by, &mw, itk.data(), (int)itk.size(), E.h1prec));
-    E.put_plan(key, pl);
+  const auto* pe = E.find_plan(key);
+  mpsa_plan* pl = pe ? pe->plan : nullptr;
+  if (pe) {
+    out_t = pe->out;
+  } else {
+    int64_t n = 0;
+    std::vector<uint64_t> q;
+    std::vector<uint32_t> p;
+    std::vector<int64_t> r, c;
+    {
+      ScopeTimer tk(E.t_keys);
+      chk(mpsa_heff1_outkeys(&bl, &br, &ba, &mw, itk.data(), (int)itk.size(), 0, &n, nullptr, nullptr, nullptr,
+                             nullptr));
+      q.resize((size_t)2 * n + 2);
+      p.resize((size_t)n + 1);
+      r.resize((size_t)n + 1);
+      c.resize((size_t)n + 1);
+      if (n)
+        chk(mpsa_heff1_outkeys(&bl, &br, &ba, &mw, itk.data(), (int)itk.size(), n, &n, q.data(), p.data(), r.data(),
+                               c.data()));
+    }
+    out_t = from_query(n, 1, q, p, r, c);
+    out_t.cplx = At.cplx;
+    if (n) {
+      mpsa_blocks by = out_t.view();
+      ScopeTimer tp(E.t_plan);
+      chk(mpsa_heff1_plan(&pl, &bl, &br, &ba, &by, &mw, itk.data(), (int)itk.size(), E.h1prec));
+      E.put_plan(key, pl, out_t);
+    }
   }
+  out_b = Buf(E.pool, out_t.len(), E.st);
+  if (!out_t.n() || !pl) return;
   chk(mpsa_plan_bind_env(pl, dL, dR, (mpsa_stream)E.st));
   chk(mpsa_heff_apply(pl, dA, 0, out_b.p, 0, 1, (mpsa_stream)E.st));
 }
@@ -504,7 +512,14 @@ std::vector<cplx> overlap(Eng& E, const DMps& a, const DMps& b) {
 // ------------------------------------------------------------ prepareSite
 // prepareSite(mps, site, "l" or "r", multiply = true) for every sector of the
 // site at once; lambdas kept on the host when asked (for a later truncate).
+// The sector factorisation is the SVD (U, S V^dagger) when the Schmidt values
+// are kept, else a Householder QR (Q, R) or LQ: the site is the same
+// isometry up to a unitary of the k kept directions and the neighbour takes
+// the rest, so the state and everything computed from it (overlaps, norms,
+// the compression's optimum, later Schmidt values) are the reference's; only
+// the gauge of the two tensors differs.
 void prepare_site(Eng& E, DMps& M, int site, bool left, bool keep_lambda) {
+  const bool use_qr = !keep_lambda;
   const int sa = left ? site : site - 1, sb = left ? site + 1 : site;  // (A, B) = (site, site+1) or (site-1, site)
   const HTab &At = M.A[(size_t)sa], &Bt = M.A[(size_t)sb];
   const HTab& X = left ? At : Bt;  // the tensor factorised
@@ -534,7 +549,8 @@ void prepare_site(Eng& E, DMps& M, int site, bool left, bool keep_lambda) {
       }
     }
     S.k = std::min(S.m, S.n);
-    if (std::min(S.m, S.n) > kSvdMaxR) throw HErr(MPSA_EINVAL, "cheb: charge sector too large for the batched SVD");
+    if (!use_qr && std::min(S.m, S.n) > kSvdMaxR)
+      throw HErr(MPSA_EINVAL, "cheb: charge sector too large for the batched SVD");
     S.mo = tm;
     tm += align64(S.m * S.n);
     S.uo = tu;
@@ -544,7 +560,10 @@ void prepare_site(Eng& E, DMps& M, int site, bool left, bool keep_lambda) {
     S.so = ts;
     ts += S.k;
     S.wo = tw;
-    tw += align64(std::max(S.m, S.n) * S.k + S.k * S.k);
+    const int64_t qp = left ? S.m : S.n;  // rows of the factorised X = M or M^H
+    const int64_t sp = std::max(S.m, S.n);  // the SVD kernel reduces tall sectors (p >= 2 k) by QR first
+    tw += use_qr ? align64(S.m * S.n + qp * S.k + S.k)
+                 : align64(sp >= 2 * S.k ? 2 * sp * S.k + 2 * S.k * S.k + S.k : sp * S.k + S.k * S.k);
   }
   const int pl = E.planes;
   Buf Mb(E.pool, pl * tm, E.st), Ub(E.pool, pl * tu, E.st), Vb(E.pool, pl * tv, E.st), Sb(E.pool, ts, E.st),
@@ -579,9 +598,14 @@ void prepare_site(Eng& E, DMps& M, int site, bool left, bool keep_lambda) {
   }
   {
     Buf dt = E.upload(tasks);
-    hchk(launch_svd_batched(pl == 2, (const SvdTask*)dt.p, (int)tasks.size(), Mb.p, tm, Ub.p, tu, Vb.p, tv, Sb.p,
-                            Wb.p, tw, E.st),
-         "cheb: sector SVD");
+    if (use_qr)
+      hchk(launch_qr_batched(pl == 2, left ? 0 : 1, (const SvdTask*)dt.p, (int)tasks.size(), Mb.p, tm, Ub.p, tu, Vb.p,
+                             tv, Sb.p, Wb.p, tw, E.st),
+           "cheb: sector QR");
+    else
+      hchk(launch_svd_batched(pl == 2, (const SvdTask*)dt.p, (int)tasks.size(), Mb.p, tm, Ub.p, tu, Vb.p, tv, Sb.p,
+                              Wb.p, tw, E.st),
+           "cheb: sector SVD");
   }
   // new tables: the factorised tensor keeps U (l) or V^dagger (r) with k
   // columns (rows); the neighbour's blocks of the sector charge take k rows
@@ -1091,7 +1115,9 @@ DMps compress_single_site(Eng& E, const DMps& mps, unsigned chimax, int maxstep,
   // n1^2 = computeOverlap(applyMPO(W, mps), applyMPO(W, mps)), site by site
   const std::vector<double> n1sq = applied_norm_sq(E, mps, E.W);
   DMps tmps = copy_mps(E, mps);
-  for (int s = 0; s < N - 1; ++s) prepare_site(E, tmps, s, true, true);
+  // the right-to-left pass sets every bond's Schmidt values: the left pass
+  // only brings the state to left-canonical form
+  for (int s = 0; s < N - 1; ++s) prepare_site(E, tmps, s, true, false);
   for (int s = N - 1; s > 0; --s) prepare_site(E, tmps, s, false, true);
   const unsigned D = N > 1 ? E.W[1].Dr : 1;
   truncate_mps(E, tmps, (unsigned)std::floor((double)chimax / (double)D), delta);
@@ -1110,7 +1136,9 @@ DMps compress_single_site(Eng& E, const DMps& mps, unsigned chimax, int maxstep,
 // canonicalised in place, the compressed state is returned
 DMps compress(Eng& E, DMps& mps, unsigned chimax, int maxstep, double delta, std::vector<double>& conv) {
   const int N = E.N;
-  for (int s = 0; s < N - 1; ++s) prepare_site(E, mps, s, true, true);
+  // the right-to-left pass starts at site N - 2, so bond N - 2 keeps the
+  // Schmidt values of the left pass (as in the reference)
+  for (int s = 0; s < N - 1; ++s) prepare_site(E, mps, s, true, s == N - 2);
   for (int s = N - 2; s > 0; --s) prepare_site(E, mps, s, false, true);
   DMps comp = copy_mps(E, mps);
   truncate_mps(E, comp, chimax, delta);
@@ -1216,20 +1244,25 @@ int mpsa_cheb_create(mpsa_cheb** out, int nsites, int nvec, const mpsa_blocks* s
       }
       HTab T = table_from(nm, E->planes == 2);
       Buf buf(E->pool, T.len(), E->st);
-      std::vector<Copy2D> dummy;
+      // one batched copy per vector: its blocks into the shared table
+      std::vector<std::vector<Copy2D>> per_vec((size_t)nvec);
+      std::vector<int64_t> tot((size_t)nvec, 0);
+      for (int v = 0; v < nvec; ++v) {
+        const mpsa_blocks& b = sites[(size_t)v * nsites + s];
+        for (int64_t i = 0; i < b.n; ++i)
+          tot[(size_t)v] = std::max<int64_t>(tot[(size_t)v], b.offset[i] + b.rows[i] * b.cols[i]);
+      }
       for (int64_t t = 0; t < T.n(); ++t) {
         const Src& sr = src[key_of(T, t)];
         const mpsa_blocks& b = sites[(size_t)sr.v * nsites + s];
-        int64_t tot = 0;
-        for (int64_t i = 0; i < b.n; ++i) tot = std::max<int64_t>(tot, b.offset[i] + b.rows[i] * b.cols[i]);
-        const double* base = dA[(size_t)sr.v * nsites + s];
+        add_copy(per_vec[(size_t)sr.v], *E, b.offset[sr.i], tot[(size_t)sr.v], T.off[(size_t)t], T.total,
+                 b.rows[sr.i], b.cols[sr.i], b.rows[sr.i], b.rows[sr.i]);
+      }
+      for (int v = 0; v < nvec; ++v) {
+        if (per_vec[(size_t)v].empty()) continue;
+        const double* base = dA[(size_t)v * nsites + s];
         if (!base) throw HErr(MPSA_EINVAL, "cheb_create: null site buffer");
-        const int64_t len = b.rows[sr.i] * b.cols[sr.i];
-        for (int pl = 0; pl < E->planes; ++pl)
-          if (len)
-            hchk(hipMemcpyAsync(buf.p + T.off[(size_t)t] + pl * T.total, base + b.offset[sr.i] + pl * tot,
-                                sizeof(double) * (size_t)len, hipMemcpyDeviceToDevice, E->st),
-                 "cheb_create: copy");
+        run_copies(*E, per_vec[(size_t)v], base, buf.p, nullptr);
       }
       M.A[(size_t)s] = T;
       M.d[(size_t)s] = std::move(buf);
@@ -1369,6 +1402,13 @@ int mpsa_cheb_moments(mpsa_cheb* E, int nmom, unsigned chimax, int ncmax, double
       if (arr) arr[(size_t)v * nmom + idx] = x;
     };
     const int nrec = 2 * nmom - 1;
+    // wall time of every moment (each ends with overlaps read back to the host)
+    E->moment_s.assign((size_t)nmom, 0.0);
+    auto lap = [&, t = t0c](int idx) mutable {
+      const auto now = std::chrono::steady_clock::now();
+      E->moment_s[(size_t)idx] = std::chrono::duration<double>(now - t).count();
+      t = now;
+    };
     DMps& f = slot_at(*E, 0, true);
     DMps t0 = copy_mps(*E, f);
     auto mu0 = overlap(*E, f, t0);
@@ -1377,6 +1417,7 @@ int mpsa_cheb_moments(mpsa_cheb* E, int nmom, unsigned chimax, int ncmax, double
       put_r(conv1, 0, v, 0.0);
       put_r(conv2, 0, v, 0.0);
     }
+    lap(0);
     if (nmom > 1) {
       std::vector<double> c1;
       DMps t1 = compress_single_site(*E, t0, chimax, ncmax, delta, c1);
@@ -1390,6 +1431,7 @@ int mpsa_cheb_moments(mpsa_cheb* E, int nmom, unsigned chimax, int ncmax, double
         put_r(conv1, 1, v, c1[(size_t)v]);
         put_r(conv2, 1, v, 0.0);
       }
+      lap(1);
       for (int n = 2; n < nmom; ++n) {
         DMps tm = compress_single_site(*E, t1, chimax, ncmax, delta, c1);
         DMps t2 = axpby_mps(*E, 2.0, tm, -1.0, t0);
@@ -1408,6 +1450,7 @@ int mpsa_cheb_moments(mpsa_cheb* E, int nmom, unsigned chimax, int ncmax, double
         }
         t0 = std::move(t1);
         t1 = std::move(t2c);
+        lap(n);
       }
       slot_at(*E, 1, false) = std::move(t1);  // the last moment's state
     }
@@ -1494,6 +1537,16 @@ int mpsa_cheb_profile(const mpsa_cheb* E, double* out5) {
   out5[2] = E->t_sync;
   out5[3] = (double)E->n_sync;
   out5[4] = (double)E->plans.size();
+  return MPSA_OK;
+}
+
+int mpsa_cheb_moment_times(const mpsa_cheb* E, double* out, int cap, int* n) {
+  if (!E || !n || cap < 0 || (cap > 0 && !out)) {
+    set_last_error("cheb_moment_times: bad arguments");
+    return MPSA_EINVAL;
+  }
+  *n = (int)E->moment_s.size();
+  for (int i = 0; i < cap && i < *n; ++i) out[i] = E->moment_s[(size_t)i];
   return MPSA_OK;
 }
 

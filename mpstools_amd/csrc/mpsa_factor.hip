@@ -34,7 +34,7 @@ namespace mpsa {
 
 namespace {
 
-constexpr int kThreads = 256;
+constexpr int kThreads = 512;
 constexpr int kWaves = kThreads / 64;
 
 __device__ __forceinline__ double wave_sum(double x) {
@@ -77,6 +77,129 @@ __device__ __forceinline__ void rotate_cols(const Mat<C>& X, int p, int i, int j
   }
 }
 
+// Householder reflections of X (p x r, planar, ld p) in place, one per
+// column j < k = min(p, r): H_j = I - beta_j v_j v_j^H with v_j = x - alpha e_j,
+// alpha = -e^{i arg x_j} |x| (x = rows j.. of column j), beta_j = 2 / |v_j|^2
+// (0 for a zero column: H_j = I).  Column j becomes (alpha, 0, ...), the
+// columns after it are reflected (one wave per column); v_j goes to rows j..
+// of column j of Hv, beta_j to beta[j].  NT threads; red holds NT / 64
+// doubles, hv 3.
+template <bool C, int NT>
+__device__ void house_reflect(const Mat<C>& X, const Mat<C>& Hv, double* beta, int p, int r, double* red,
+                              double* hv) {
+  constexpr int NW = NT / 64;
+  const int k = p < r ? p : r;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int j = 0; j < k; ++j) {
+    double a = 0.0;
+    for (int i = j + tid; i < p; i += NT) {
+      const double x = X.r(i, j);
+      a += x * x;
+      if (C) {
+        const double y = X.m(i, j);
+        a += y * y;
+      }
+    }
+    a = wave_sum(a);
+    if (lane == 0) red[wave] = a;
+    __syncthreads();
+    if (tid == 0) {
+      double nn = 0.0;
+      for (int w = 0; w < NW; ++w) nn += red[w];
+      const double x0r = X.r(j, j), x0i = C ? X.m(j, j) : 0.0;
+      const double a0 = sqrt(x0r * x0r + x0i * x0i), nrm = sqrt(nn);
+      double ar = 0.0, ai = 0.0, b = 0.0;
+      if (nn > 0.0) {
+        const double er = a0 > 0.0 ? x0r / a0 : 1.0, ei = a0 > 0.0 ? x0i / a0 : 0.0;
+        ar = -er * nrm;
+        ai = -ei * nrm;
+        const double v0r = x0r - ar, v0i = x0i - ai;
+        const double vn = nn - a0 * a0 + v0r * v0r + v0i * v0i;
+        b = vn > 0.0 ? 2.0 / vn : 0.0;
+      }
+      hv[0] = ar;
+      hv[1] = ai;
+      hv[2] = b;
+      beta[j] = b;
+    }
+    __syncthreads();
+    const double ar = hv[0], ai = hv[1], b = hv[2];
+    for (int i = j + tid; i < p; i += NT) {
+      double v_r = X.r(i, j), v_i = C ? X.m(i, j) : 0.0;
+      if (i == j) {
+        v_r -= ar;
+        v_i -= ai;
+      }
+      Hv.r(i, j) = v_r;
+      if (C) Hv.m(i, j) = v_i;
+    }
+    __syncthreads();
+    for (int c = j + 1 + wave; c < r; c += NW) {
+      double wr_ = 0.0, wi_ = 0.0;
+      for (int i = j + lane; i < p; i += 64) {
+        const double v_r = Hv.r(i, j), x_r = X.r(i, c);
+        wr_ += v_r * x_r;
+        if (C) {
+          const double v_i = Hv.m(i, j), x_i = X.m(i, c);
+          wr_ += v_i * x_i;
+          wi_ += v_r * x_i - v_i * x_r;  // conj(v) x
+        }
+      }
+      wr_ = wave_sum(wr_);
+      if (C) wi_ = wave_sum(wi_);
+      const double fr = b * wr_, fi = b * wi_;
+      for (int i = j + lane; i < p; i += 64) {
+        const double v_r = Hv.r(i, j);
+        const double v_i = C ? Hv.m(i, j) : 0.0;
+        X.r(i, c) -= fr * v_r - fi * v_i;
+        if (C) X.m(i, c) -= fr * v_i + fi * v_r;
+      }
+    }
+    for (int i = j + tid; i < p; i += NT) {
+      X.r(i, j) = i == j ? ar : 0.0;
+      if (C) X.m(i, j) = i == j ? ai : 0.0;
+    }
+    __syncthreads();
+  }
+}
+
+// Y <- H_0 H_1 ... H_{k-1} Y for the ncols columns of Y (p rows), applying
+// H_j for j = k-1 .. 0; with from_j only columns >= j (Y = [I; 0] there, as
+// LAPACK's xORGQR forms Q).
+template <bool C, int NT>
+__device__ void house_apply(const Mat<C>& Y, int ncols, bool from_j, const Mat<C>& Hv, const double* beta, int p,
+                            int k) {
+  constexpr int NW = NT / 64;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int j = k - 1; j >= 0; --j) {
+    const double b = beta[j];
+    if (b != 0.0) {
+      for (int c = (from_j ? j : 0) + wave; c < ncols; c += NW) {
+        double wr_ = 0.0, wi_ = 0.0;
+        for (int i = j + lane; i < p; i += 64) {
+          const double v_r = Hv.r(i, j), y_r = Y.r(i, c);
+          wr_ += v_r * y_r;
+          if (C) {
+            const double v_i = Hv.m(i, j), y_i = Y.m(i, c);
+            wr_ += v_i * y_i;
+            wi_ += v_r * y_i - v_i * y_r;
+          }
+        }
+        wr_ = wave_sum(wr_);
+        if (C) wi_ = wave_sum(wi_);
+        const double fr = b * wr_, fi = b * wi_;
+        for (int i = j + lane; i < p; i += 64) {
+          const double v_r = Hv.r(i, j);
+          const double v_i = C ? Hv.m(i, j) : 0.0;
+          Y.r(i, c) -= fr * v_r - fi * v_i;
+          if (C) Y.m(i, c) -= fr * v_i + fi * v_r;
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 }  // namespace
 
 template <bool C>
@@ -90,14 +213,24 @@ __global__ __launch_bounds__(kThreads) void svd_jacobi_kernel(const SvdTask* __r
   __shared__ int rank[kSvdMaxR];
   __shared__ int rotated;
   __shared__ double red[kWaves], redc[kWaves];
+  __shared__ double hv[3];
   const SvdTask t = tasks[blockIdx.x];
   const int m = t.m, n = t.n;
   if (m <= 0 || n <= 0) return;
   const bool wide = m < n;
   const int p = wide ? n : m, r = wide ? m : n, k = r;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  // Tall X (p >= 2 r) is reduced first, X = Q R by Householder QR, and the
+  // rotations run on the r x r R (r rows per column pair instead of p); the
+  // rotated R goes back through Q at the end.  Workspace: X (p x r), V
+  // (r x r), and for the reduction the vectors (p x r), R (r x r), betas (r).
+  const bool pre = p >= 2 * r;
   Mat<C> X{W + t.w_off, W + t.w_off + w_plane, p};
   Mat<C> V{W + t.w_off + (int64_t)p * r, W + t.w_off + (int64_t)p * r + w_plane, r};
+  const int64_t o_hv = (int64_t)p * r + (int64_t)r * r, o_rm = 2 * (int64_t)p * r + (int64_t)r * r;
+  Mat<C> Hv{W + t.w_off + o_hv, W + t.w_off + o_hv + w_plane, p};
+  Mat<C> Rm{W + t.w_off + o_rm, W + t.w_off + o_rm + w_plane, r};
+  double* const beta = W + t.w_off + o_rm + (int64_t)r * r;
   // X = M or M^H, V = identity
   for (int64_t e = tid; e < (int64_t)p * r; e += kThreads) {
     const int i = (int)(e % p), j = (int)(e / p);
@@ -111,7 +244,19 @@ __global__ __launch_bounds__(kThreads) void svd_jacobi_kernel(const SvdTask* __r
     if (C) V.m(i, j) = 0.0;
   }
   __syncthreads();
-  const double tol = sqrt((double)p) * DBL_EPSILON;
+  if (pre) {
+    house_reflect<C, kThreads>(X, Hv, beta, p, r, red, hv);
+    for (int64_t e = tid; e < (int64_t)r * r; e += kThreads) {
+      const int i = (int)(e % r), j = (int)(e / r);
+      Rm.r(i, j) = i <= j ? X.r(i, j) : 0.0;
+      if (C) Rm.m(i, j) = i <= j ? X.m(i, j) : 0.0;
+    }
+    __syncthreads();
+  }
+  // the matrix the rotations act on: X itself, or R
+  const Mat<C> J = pre ? Rm : X;
+  const int pj = pre ? r : p;
+  const double tol = sqrt((double)pj) * DBL_EPSILON;
   const int rr = r + (r & 1);
   for (int sweep = 0; sweep < max_sweeps && r > 1; ++sweep) {
     if (tid == 0) rotated = 0;
@@ -127,13 +272,13 @@ __global__ __launch_bounds__(kThreads) void svd_jacobi_kernel(const SvdTask* __r
         }
         if (j >= r) continue;
         double a = 0.0, b = 0.0, gr = 0.0, gi = 0.0;
-        for (int row = lane; row < p; row += 64) {
-          const double xir = X.r(row, i), xjr = X.r(row, j);
+        for (int row = lane; row < pj; row += 64) {
+          const double xir = J.r(row, i), xjr = J.r(row, j);
           a += xir * xir;
           b += xjr * xjr;
           gr += xir * xjr;
           if (C) {
-            const double xii = X.m(row, i), xji = X.m(row, j);
+            const double xii = J.m(row, i), xji = J.m(row, j);
             a += xii * xii;
             b += xji * xji;
             gr += xii * xji;
@@ -150,7 +295,7 @@ __global__ __launch_bounds__(kThreads) void svd_jacobi_kernel(const SvdTask* __r
         const double zeta = (b - a) / (2.0 * g);
         const double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
         const double c = 1.0 / sqrt(1.0 + tt * tt), s = c * tt;
-        rotate_cols<C>(X, p, i, j, c, s, er, ei, lane);
+        rotate_cols<C>(J, pj, i, j, c, s, er, ei, lane);
         rotate_cols<C>(V, r, i, j, c, s, er, ei, lane);
         if (lane == 0) rotated = 1;
       }
@@ -159,6 +304,15 @@ __global__ __launch_bounds__(kThreads) void svd_jacobi_kernel(const SvdTask* __r
     const int any = rotated;
     __syncthreads();
     if (!any) break;
+  }
+  if (pre) {  // X = Q [R V; 0]: the rotated columns back through the reflections
+    for (int64_t e = tid; e < (int64_t)p * r; e += kThreads) {
+      const int i = (int)(e % p), j = (int)(e / p);
+      X.r(i, j) = i < r ? Rm.r(i, j) : 0.0;
+      if (C) X.m(i, j) = i < r ? Rm.m(i, j) : 0.0;
+    }
+    __syncthreads();
+    house_apply<C, kThreads>(X, r, false, Hv, beta, p, r);
   }
   // singular values and their descending order
   for (int j = wave; j < r; j += kWaves) {
@@ -319,6 +473,103 @@ __global__ __launch_bounds__(kThreads) void svd_jacobi_kernel(const SvdTask* __r
       }
     }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Batched Householder QR of the same sector matrices, for the gauge steps
+// that need an isometry but no Schmidt values: left (right = 0) M = Q R with
+// Q (m x k) orthonormal columns, R (k x n) upper trapezoidal; right
+// (right = 1) M = L Q' from the QR of M^H = Q R: L = R^H (m x k), Q' = Q^H
+// (k x n) orthonormal rows; k = min(m, n).  Written in the SVD's output
+// format with S = 1: U = Q (left) or L (right), V^dagger = R (left) or Q'
+// (right), so that U S V^dagger = M as for the SVD.
+//
+// X = M or M^H (p x r) in the workspace; for j < k the reflection
+// H_j = I - beta_j v_j v_j^H (v_j at rows j.. of column j of the vector
+// store, beta_j = 2 / |v_j|^2, v_j = x - alpha e_j with alpha = -e^{i arg x_j}
+// |x|) zeroes column j below the diagonal, applied to columns j+1.. one wave
+// per column; then Q = H_0 ... H_{k-1} [I; 0] is accumulated backwards
+// (columns j.. of the partial product, as LAPACK's xORGQR does).
+// Workspace per task: p r (X) + p k (vectors) + k (betas) elements per plane.
+constexpr int kQrThreads = 1024;
+constexpr int kQrWaves = kQrThreads / 64;
+
+template <bool C>
+__global__ __launch_bounds__(kQrThreads) void qr_householder_kernel(const SvdTask* __restrict__ tasks, int right,
+                                                                    const double* __restrict__ M, int64_t m_plane,
+                                                                    double* __restrict__ U, int64_t u_plane,
+                                                                    double* __restrict__ Vt, int64_t vt_plane,
+                                                                    double* __restrict__ S, double* __restrict__ W,
+                                                                    int64_t w_plane) {
+  __shared__ double red[kQrWaves];
+  __shared__ double hv[3];
+  const SvdTask t = tasks[blockIdx.x];
+  const int m = t.m, n = t.n;
+  if (m <= 0 || n <= 0) return;
+  const int p = right ? n : m, r = right ? m : n, k = p < r ? p : r;
+  const int tid = threadIdx.x;
+  // X (p x r) at 0, vectors (p x k) at p r, betas at p r + p k (real plane)
+  Mat<C> X{W + t.w_off, W + t.w_off + w_plane, p};
+  Mat<C> Hv{W + t.w_off + (int64_t)p * r, W + t.w_off + (int64_t)p * r + w_plane, p};
+  double* const beta = W + t.w_off + (int64_t)p * r + (int64_t)p * k;
+  for (int64_t e = tid; e < (int64_t)p * r; e += kQrThreads) {
+    const int i = (int)(e % p), j = (int)(e / p);
+    const int64_t src = right ? (j + (int64_t)i * t.lda) : (i + (int64_t)j * t.lda);
+    X.r(i, j) = M[t.a_off + src];
+    if (C) X.m(i, j) = right ? -M[t.a_off + m_plane + src] : M[t.a_off + m_plane + src];
+  }
+  __syncthreads();
+  house_reflect<C, kQrThreads>(X, Hv, beta, p, r, red, hv);
+  // R (k x r) = the upper trapezoid of X; written before X is reused for Q
+  double* Ur = U + t.u_off;
+  double* Ui = U + t.u_off + u_plane;
+  double* Vr = Vt + t.vt_off;
+  double* Vi = Vt + t.vt_off + vt_plane;
+  for (int64_t e = tid; e < (int64_t)k * r; e += kQrThreads) {
+    const int i = (int)(e % k), c = (int)(e / k);
+    const double rr = i <= c ? X.r(i, c) : 0.0;
+    const double ri = (C && i <= c) ? X.m(i, c) : 0.0;
+    if (!right) {  // V^dagger (k x n) = R
+      Vr[i + (int64_t)c * k] = rr;
+      if (C) Vi[i + (int64_t)c * k] = ri;
+    } else {  // U (m x k) = R^H: U[c, i] = conj(R[i, c])
+      Ur[c + (int64_t)i * m] = rr;
+      if (C) Ui[c + (int64_t)i * m] = -ri;
+    }
+  }
+  for (int i = tid; i < k; i += kQrThreads) S[t.s_off + i] = 1.0;
+  __syncthreads();
+  // Q (p x k) in X's storage: [I; 0], then H_{k-1} .. H_0 applied to columns j..
+  for (int64_t e = tid; e < (int64_t)p * k; e += kQrThreads) {
+    const int i = (int)(e % p), c = (int)(e / p);
+    X.r(i, c) = i == c ? 1.0 : 0.0;
+    if (C) X.m(i, c) = 0.0;
+  }
+  __syncthreads();
+  house_apply<C, kQrThreads>(X, k, true, Hv, beta, p, k);
+  for (int64_t e = tid; e < (int64_t)p * k; e += kQrThreads) {
+    const int i = (int)(e % p), c = (int)(e / p);
+    if (!right) {  // U (m x k) = Q
+      Ur[i + (int64_t)c * m] = X.r(i, c);
+      if (C) Ui[i + (int64_t)c * m] = X.m(i, c);
+    } else {  // V^dagger (k x n) = Q^H: Vt[c, i] = conj(Q[i, c])
+      Vr[c + (int64_t)i * k] = X.r(i, c);
+      if (C) Vi[c + (int64_t)i * k] = -X.m(i, c);
+    }
+  }
+}
+
+hipError_t launch_qr_batched(int cplx, int right, const SvdTask* tasks, int ntasks, const double* M, int64_t m_plane,
+                             double* U, int64_t u_plane, double* Vt, int64_t vt_plane, double* S, double* W,
+                             int64_t w_plane, hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  if (cplx)
+    qr_householder_kernel<true><<<ntasks, kQrThreads, 0, st>>>(tasks, right, M, m_plane, U, u_plane, Vt, vt_plane, S,
+                                                               W, w_plane);
+  else
+    qr_householder_kernel<false><<<ntasks, kQrThreads, 0, st>>>(tasks, right, M, m_plane, U, u_plane, Vt, vt_plane,
+                                                                S, W, w_plane);
+  return hipGetLastError();
 }
 
 hipError_t launch_svd_batched(int cplx, const SvdTask* tasks, int ntasks, const double* M, int64_t m_plane,

@@ -106,8 +106,10 @@ hipError_t launch_fixphase(double* U, int64_t m, int64_t ldu, double* Vt, int64_
                            hipStream_t st);
 // Batched thin SVD of small sector matrices (mpsa_factor.hip): M (m x n, ld
 // lda) at a_off -> U (m x k, ld m) at u_off, S (k) at s_off, V^dagger (k x n,
-// ld k) at vt_off, k = min(m, n); workspace of p * r + r * r elements at
-// w_off (p = max(m, n), r = min(m, n) <= kSvdMaxR).  Complex data planar.
+// ld k) at vt_off, k = min(m, n); workspace at w_off of p * r + r * r
+// elements, or 2 p r + 2 r r + r when p >= 2 r (tall sectors are reduced by a
+// QR first), per plane (p = max(m, n), r = min(m, n) <= kSvdMaxR).  Complex
+// data planar.
 constexpr int kSvdMaxR = 1024;
 struct SvdTask {
   int64_t a_off, u_off, vt_off, s_off, w_off;
@@ -116,6 +118,14 @@ struct SvdTask {
 hipError_t launch_svd_batched(int cplx, const SvdTask* tasks, int ntasks, const double* M, int64_t m_plane,
                               double* U, int64_t u_plane, double* Vt, int64_t vt_plane, double* S, double* W,
                               int64_t w_plane, hipStream_t st);
+// Batched Householder QR of the same tasks in the SVD's output format with
+// S = 1: right = 0: U = Q (m x k), V^dagger = R (k x n); right = 1: M = L Q'
+// with U = L (m x k), V^dagger = Q' (k x n, orthonormal rows).  Workspace
+// p * r + p * k + k elements per plane at w_off, p = rows of X (m, or n when
+// right), r = its columns.
+hipError_t launch_qr_batched(int cplx, int right, const SvdTask* tasks, int ntasks, const double* M, int64_t m_plane,
+                             double* U, int64_t u_plane, double* Vt, int64_t vt_plane, double* S, double* W,
+                             int64_t w_plane, hipStream_t st);
 // batched Lanczos vector algebra (device-resident coefficients and step counter)
 hipError_t launch_bdot(const double* x, const double* y, int64_t n, int64_t vs, int nvec, double* partial, int nparts,
                        double* out, int ld, const int* step, int root, hipStream_t st);

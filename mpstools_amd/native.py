@@ -58,6 +58,8 @@ def lib():
         L.mpsa_last_error.restype = C.c_char_p
         L.mpsa_version.restype = C.c_char_p
         L.mpsa_heff2_outkeys.argtypes = [P, P, P, P, P, P, C.c_int, P, C.c_int, C.c_int64, P, P, P, P, P]
+        L.mpsa_twosite_outkeys.argtypes = [P, P, C.c_int64, P, P, P, P, P]
+        L.mpsa_twosite_product.argtypes = [P, P, P, P, P, P, P]
         L.mpsa_heff1_outkeys.argtypes = [P, P, P, P, P, C.c_int, C.c_int64, P, P, P, P, P]
         L.mpsa_env_outkeys.argtypes = [P, P, P, P, P, C.c_int, C.c_int, C.c_int64, P, P, P, P, P]
         L.mpsa_heff2_plan.argtypes = [P, P, P, P, P, P, P, P, C.c_int, P, C.c_int, C.c_int]
@@ -116,6 +118,7 @@ def lib():
         L.mpsa_cheb_site.argtypes = [P, C.c_int, C.c_int, C.c_int, P, P, P, P, P, P, P, P]
         L.mpsa_cheb_stats.argtypes = [P, P, P]
         L.mpsa_cheb_profile.argtypes = [P, P]
+        L.mpsa_cheb_moment_times.argtypes = [P, P, C.c_int, P]
         L.mpsa_cheb_destroy.argtypes = [P]
         L.mpsa_dot.argtypes = [P, P, C.c_int64, C.c_int, P, P]
         L.mpsa_axpby.argtypes = [P, P, P, P, C.c_int64, C.c_int, P]
@@ -235,6 +238,18 @@ def heff2_outkeys(Lt: Table, Rt: Table, xt: Table, Wl, Wr, itl, itr, align=64) -
                           [C.byref(Lt.s), C.byref(Rt.s), C.byref(xt.s), C.byref(ml.s), C.byref(mr.s), _ptr(a),
                            len(a), _ptr(b), len(b)], 2)
     return Table.from_keys(2, q, p, r, c, align)
+
+
+def twosite_outkeys(t1: Table, t2: Table, align=64) -> Table:
+    """Keys of BlockMatrix<2>(B1, B2) (src/BlockTensor.hpp:285-301)."""
+    q, p, r, c = _outkeys(lib().mpsa_twosite_outkeys, [C.byref(t1.s), C.byref(t2.s)], 2)
+    return Table.from_keys(2, q, p, r, c, align)
+
+
+def twosite_product(t1: Table, d1, t2: Table, d2, tout: Table, dout):
+    """dout = BlockMatrix<2>(B1, B2) on the device (mpsa_twosite_product)."""
+    _check(lib().mpsa_twosite_product(C.byref(t1.s), _dptr(d1), C.byref(t2.s), _dptr(d2), C.byref(tout.s),
+                                      _dptr(dout), _stream()))
 
 
 def heff1_outkeys(Lt: Table, Rt: Table, xt: Table, W, itk, align=64) -> Table:
@@ -696,8 +711,11 @@ class Cheb:
         c1, c2, sec = np.zeros(self.nvec * nmom), np.zeros(self.nvec * nmom), np.zeros(1)
         _check(lib().mpsa_cheb_moments(self.h, int(nmom), int(chimax), int(ncmax), float(delta), _ptr(mu), _ptr(rc),
                                        _ptr(c1), _ptr(c2), _ptr(sec)))
+        lap, n = np.zeros(nmom), np.zeros(1, np.int32)
+        _check(lib().mpsa_cheb_moment_times(self.h, _ptr(lap), nmom, _ptr(n)))
         return dict(moments=self._cv(mu).reshape(self.nvec, nmom), recons=self._cv(rc).reshape(self.nvec, -1),
-                    conv1=c1.reshape(self.nvec, nmom), conv2=c2.reshape(self.nvec, nmom), s=float(sec[0]))
+                    conv1=c1.reshape(self.nvec, nmom), conv2=c2.reshape(self.nvec, nmom), s=float(sec[0]),
+                    moment_s=lap[:int(n[0])].copy())
 
     def site(self, slot, vec, site):
         nb, ln = np.zeros(1, np.int64), np.zeros(1, np.int64)
@@ -858,6 +876,20 @@ def contract_twosite(L: Blocks, R: Blocks, psi: Blocks, Wl, Wr, itl, itr, precis
     plan.apply(dx, dy)
     torch.cuda.synchronize()
     return yt.unpack(dy.cpu().numpy())
+
+
+def twosite(B1: Blocks, B2: Blocks) -> Blocks:
+    """BlockMatrix<2>(B1, B2) (src/BlockTensor.hpp:285-301) computed on the
+    device; complex when either input is complex."""
+    import torch
+    B1, B2 = _promote(B1, B2)
+    t1, t2 = Table.from_blocks(B1), Table.from_blocks(B2)
+    tout = _out_table(twosite_outkeys(t1, t2), t1.is_complex)
+    d1, d2 = _to_dev(t1, B1), _to_dev(t2, B2)
+    dout = torch.zeros(max(tout.length, 1), dtype=torch.float64, device="cuda")
+    twosite_product(t1, d1, t2, d2, tout, dout)
+    torch.cuda.synchronize()
+    return tout.unpack(dout.cpu().numpy())
 
 
 def contract_single(L: Blocks, A: Blocks, W, R: Blocks, itk, precision=F64) -> Blocks:

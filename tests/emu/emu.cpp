@@ -27,6 +27,7 @@
 #include <dlfcn.h>
 
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -873,6 +874,98 @@ hipError_t launch_fixphase(double* U, int64_t m, int64_t ldu, double* Vt, int64_
   });
   return hipSuccess;
 }
+// batched Householder QR (mpsa_factor.hip's contract): X = M (right = 0) or
+// M^H (right = 1), p x r, X = Q R; outputs U = Q, V^dagger = R (left) or
+// U = R^H, V^dagger = Q^H (right), S = 1
+hipError_t launch_qr_batched(int cplx, int right, const SvdTask* tasks, int ntasks, const double* M, int64_t m_plane,
+                             double* U, int64_t u_plane, double* Vt, int64_t vt_plane, double* S, double* W,
+                             int64_t w_plane, hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  submit(st, "qr batched", [=]() {
+    if (!chk_range(tasks, 0, ntasks - 1, "qr tasks")) return;
+    typedef std::complex<double> Z;
+    const int pl = cplx ? 2 : 1;
+    for (int ti = 0; ti < ntasks; ++ti) {
+      const SvdTask t = tasks[ti];
+      const int m = t.m, n = t.n, k = std::min(m, n);
+      if (m <= 0 || n <= 0) continue;
+      const int p = right ? n : m, r = right ? m : n;
+      if (t.lda < m) {
+        report("qr: task outside the kernel's limits");
+        return;
+      }
+      bool ok = true;
+      for (int q = 0; q < pl; ++q) {
+        ok = chk_range(M, t.a_off + q * m_plane, t.a_off + q * m_plane + (int64_t)t.lda * (n - 1) + m - 1, "qr M") && ok;
+        ok = chk_range(U, t.u_off + q * u_plane, t.u_off + q * u_plane + (int64_t)m * k - 1, "qr U") && ok;
+        ok = chk_range(Vt, t.vt_off + q * vt_plane, t.vt_off + q * vt_plane + (int64_t)k * n - 1, "qr Vt") && ok;
+        ok = chk_range(W, t.w_off + q * w_plane,
+                       t.w_off + q * w_plane + (int64_t)p * r + (int64_t)p * k + k - 1, "qr W") && ok;
+      }
+      ok = chk_range(S, t.s_off, t.s_off + k - 1, "qr S") && ok;
+      if (!ok) return;
+      auto mat = [&](int i, int j) {  // M[i, j]
+        const int64_t o = t.a_off + i + (int64_t)j * t.lda;
+        return Z(M[o], cplx ? M[o + m_plane] : 0.0);
+      };
+      std::vector<Z> X((size_t)p * r), Q((size_t)p * k, Z(0.0));
+      for (int j = 0; j < r; ++j)
+        for (int i = 0; i < p; ++i) X[(size_t)i + (size_t)j * p] = right ? std::conj(mat(j, i)) : mat(i, j);
+      std::vector<std::vector<Z>> vs;
+      std::vector<double> betas;
+      for (int j = 0; j < k; ++j) {
+        double nn = 0.0;
+        for (int i = j; i < p; ++i) nn += std::norm(X[(size_t)i + (size_t)j * p]);
+        const Z x0 = X[(size_t)j + (size_t)j * p];
+        const double a0 = std::abs(x0);
+        Z alpha(0.0);
+        std::vector<Z> v((size_t)p, Z(0.0));
+        double b = 0.0;
+        if (nn > 0.0) {
+          alpha = -(a0 > 0.0 ? x0 / a0 : Z(1.0)) * std::sqrt(nn);
+          for (int i = j; i < p; ++i) v[(size_t)i] = X[(size_t)i + (size_t)j * p];
+          v[(size_t)j] -= alpha;
+          double vn = 0.0;
+          for (int i = j; i < p; ++i) vn += std::norm(v[(size_t)i]);
+          b = vn > 0.0 ? 2.0 / vn : 0.0;
+        }
+        for (int c = j + 1; c < r; ++c) {
+          Z w(0.0);
+          for (int i = j; i < p; ++i) w += std::conj(v[(size_t)i]) * X[(size_t)i + (size_t)c * p];
+          for (int i = j; i < p; ++i) X[(size_t)i + (size_t)c * p] -= b * w * v[(size_t)i];
+        }
+        for (int i = j; i < p; ++i) X[(size_t)i + (size_t)j * p] = i == j ? alpha : Z(0.0);
+        vs.push_back(v);
+        betas.push_back(b);
+      }
+      for (int c = 0; c < k; ++c) Q[(size_t)c + (size_t)c * p] = 1.0;
+      for (int j = k - 1; j >= 0; --j)
+        for (int c = j; c < k; ++c) {
+          Z w(0.0);
+          for (int i = j; i < p; ++i) w += std::conj(vs[(size_t)j][(size_t)i]) * Q[(size_t)i + (size_t)c * p];
+          for (int i = j; i < p; ++i) Q[(size_t)i + (size_t)c * p] -= betas[(size_t)j] * w * vs[(size_t)j][(size_t)i];
+        }
+      auto put = [&](double* base, int64_t plane, int64_t off, Z z) {
+        base[off] = z.real();
+        if (cplx) base[off + plane] = z.imag();
+      };
+      for (int c = 0; c < r; ++c)
+        for (int i = 0; i < k; ++i) {
+          const Z rv = i <= c ? X[(size_t)i + (size_t)c * p] : Z(0.0);
+          if (!right) put(Vt, vt_plane, t.vt_off + i + (int64_t)c * k, rv);
+          else put(U, u_plane, t.u_off + c + (int64_t)i * m, std::conj(rv));
+        }
+      for (int c = 0; c < k; ++c)
+        for (int i = 0; i < p; ++i) {
+          const Z qv = Q[(size_t)i + (size_t)c * p];
+          if (!right) put(U, u_plane, t.u_off + i + (int64_t)c * m, qv);
+          else put(Vt, vt_plane, t.vt_off + c + (int64_t)i * k, std::conj(qv));
+        }
+      for (int i = 0; i < k; ++i) S[t.s_off + i] = 1.0;
+    }
+  });
+  return hipSuccess;
+}
 // batched thin SVD (mpsa_factor.hip's contract): LAPACK dgesvd / zgesvd per
 // task, singular values descending, FixPhase on real data
 hipError_t launch_svd_batched(int cplx, const SvdTask* tasks, int ntasks, const double* M, int64_t m_plane, double* U,
@@ -904,7 +997,8 @@ hipError_t launch_svd_batched(int cplx, const SvdTask* tasks, int ntasks, const 
         ok = chk_range(M, t.a_off + q * m_plane, t.a_off + q * m_plane + (int64_t)t.lda * (n - 1) + m - 1, "svd M") && ok;
         ok = chk_range(U, t.u_off + q * u_plane, t.u_off + q * u_plane + (int64_t)m * k - 1, "svd U") && ok;
         ok = chk_range(Vt, t.vt_off + q * vt_plane, t.vt_off + q * vt_plane + (int64_t)k * n - 1, "svd Vt") && ok;
-        ok = chk_range(W, t.w_off + q * w_plane, t.w_off + q * w_plane + (int64_t)p * k + (int64_t)k * k - 1, "svd W") && ok;
+        const int64_t wl = p >= 2 * k ? 2 * (int64_t)p * k + 2 * (int64_t)k * k + k : (int64_t)p * k + (int64_t)k * k;
+        ok = chk_range(W, t.w_off + q * w_plane, t.w_off + q * w_plane + wl - 1, "svd W") && ok;
       }
       ok = chk_range(S, t.s_off, t.s_off + k - 1, "svd S") && ok;
       if (!ok) return;

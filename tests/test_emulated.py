@@ -567,3 +567,29 @@ def test_emulated_lanczos_complex_projection(lib, oracle):
     import lzcase
     dev, host = _emu_io(lib)
     lzcase.check_complex_projection(oracle, lzcase.case(oracle, "tv", 12, 6, 16, 116), dev, host, 116)
+
+
+@pytest.mark.parametrize("model,cplx", [("xxz", False), ("hubbard", False), ("hubbard", True)])
+def test_emulated_twosite_product(lib, oracle, model, cplx):
+    """BlockMatrix<2>(A_c, A_{c+1}) on the device layer (mpsa_twosite_product)
+    against the host product of blocks.twosite: the same keys, shapes and
+    values to rounding; an output block no pair produces is zeroed."""
+    import complexcase as cc
+    from mpstools_amd.blocks import twosite
+    P = problems.build(oracle, model, 8 if model == "xxz" else 6, 4 if model == "xxz" else 6, 12, 41)
+    c = P["c"]
+    A1, A2 = P["mps"][c], P["mps"][c + 1]
+    if cplx:
+        A1, A2 = cc.with_imag(A1, 3), cc.with_imag(A2, 4)
+    want = twosite(A1, A2)
+    t1, t2 = native.Table.from_blocks(A1), native.Table.from_blocks(A2)
+    to = native.twosite_outkeys(t1, t2)
+    if cplx:
+        to = to.as_complex()
+    assert to.keys() == want.keys()
+    d1, d2 = DevBuf(lib, t1.pack(A1)), DevBuf(lib, t2.pack(A2))
+    dout = DevBuf(lib, np.full(to.length, 7.0))  # stale contents must not survive
+    native.twosite_product(t1, d1, t2, d2, to, dout)
+    got = to.unpack(dout.host())
+    for k in want.keys():
+        assert np.abs(np.asarray(got[k]) - np.asarray(want[k])).max() <= 1e-13 * max(1.0, np.abs(want[k]).max())
