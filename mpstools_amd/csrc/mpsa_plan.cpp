@@ -529,8 +529,11 @@ static std::vector<Tuple> tuples_env(const Tab& BL, const Tab& A, const Tab& Aup
   const int idir = direction > 0 ? 1 : 0;
   std::map<uint64_t, int> outgoing;
   for (auto& k : A.key) outgoing[idir ? k.q1 : k.q0] = 1;
-  std::map<uint32_t, std::vector<int>> blits;
-  for (int i = 0; i < (int)BL.n; ++i) blits[BL.key[(size_t)i].p0].push_back(i);
+  // BL blocks by (aux index, inner charge), each list in table order: the
+  // loop below visits exactly the blocks the reference's scan matches, in
+  // the same order, without scanning the others
+  std::map<std::pair<uint32_t, uint64_t>, std::vector<int>> blits;
+  for (int i = 0; i < (int)BL.n; ++i) blits[{BL.key[(size_t)i].p0, BL.key[(size_t)i].q1}].push_back(i);
   for (auto& pk : outgoing) {
     const uint64_t p = pk.first;
     for (auto& o : w) {
@@ -541,14 +544,10 @@ static std::vector<Tuple> tuples_env(const Tab& BL, const Tab& A, const Tab& Aup
         BKey kb = direction > 0 ? BKey{p - kyout, p, oi.out, 0} : BKey{p, p + kyout, oi.out, 0};
         int ib = A.find(kb);
         if (ib < 0) continue;
-        auto bl = blits.find(o_in);
+        auto bl = blits.find({o_in, direction > 0 ? kb.q0 : kb.q1});
         if (bl == blits.end()) continue;
         for (int iBL : bl->second) {
           const BKey& k = BL.key[(size_t)iBL];
-          if (direction > 0 && k.p0 != ko[0]) continue;
-          if (direction <= 0 && k.p0 != ko[1]) continue;
-          if (direction > 0 && k.q1 != kb.q0) continue;
-          if (direction <= 0 && k.q1 != kb.q1) continue;
           const uint64_t kyin = itk[oi.in];
           BKey ka = direction > 0 ? BKey{k.q0, k.q0 + kyin, oi.in, 0} : BKey{k.q0 - kyin, k.q0, oi.in, 0};
           int ia = Aup.find(ka);
