@@ -316,7 +316,12 @@ int mpsa_pool_set_cap(int64_t cap_bytes);
  * (dropDangling, src/MPSfunctions.hpp:1444-1484).  direction > 0 multiplies
  * lambda into the right tensor, direction <= 0 into the left one
  * (DMRGengine::doStep, src/container.hpp:321-328).  The result owns its
- * device buffers until mpsa_split_destroy. */
+ * device buffers until mpsa_split_destroy.  When chimax is below the number
+ * of singular values, sectors of at least 1024 x 1024 are decomposed by the
+ * density-matrix eigendecomposition (syevd of the Gram matrix of the side
+ * that stays an isometry) instead of the SVD: the kept set agrees with the
+ * SVD's except among directions of weight below ~1e-16, and values below
+ * 1e-10 of the sector's largest count as zero. */
 typedef struct mpsa_split_result mpsa_split_result;
 int mpsa_split(const mpsa_blocks* psi, const double* dpsi, const uint64_t* itokey_l, int dl, const uint64_t* itokey_r,
                int dr, int direction, unsigned chimax, int do_trunc, double tr_weight, int normalize,
@@ -489,6 +494,19 @@ int mpsa_cheb_profile(const mpsa_cheb* engine, double* out5);
  * the overlap; 1: compressSingleSite; n >= 2: compressSingleSite, the sum
  * and compress); *n = their count, at most cap written. */
 int mpsa_cheb_moment_times(const mpsa_cheb* engine, double* out, int cap, int* n);
+/* Charge sectors with min(m, n) >= k (default 1024, the batched kernels'
+ * limit) are factorised by rocSOLVER (gesvd, geqrf/ungqr, gelqf/unglq), one
+ * call each; smaller ones by the batched kernels in one launch.  k in
+ * [1, 1024]. */
+int mpsa_cheb_set_big_sector(mpsa_cheb* engine, int k);
+/* Sector-shape log of the canonicalisation launches (a profiling aid with no
+ * counterpart in the reference): cap > 0 clears the log and records up to cap
+ * launches from now on, cap = 0 clears it and stops.  Each record is 8 int64:
+ * kind (0 SVD, 1 QR/LQ in the batched kernels; 2, 3 the same on rocSOLVER),
+ * sectors, max m, max n, max k = min(m, n), and the sectors with k >= 64,
+ * >= 128, >= 256.  mpsa_cheb_factor_records copies min(cap, *n) records. */
+int mpsa_cheb_factor_log(mpsa_cheb* engine, int64_t cap);
+int mpsa_cheb_factor_records(const mpsa_cheb* engine, int64_t* out, int64_t cap, int64_t* n);
 int mpsa_cheb_destroy(mpsa_cheb* c);
 
 /* ---- Lanczos vector algebra over packed block vectors (BlockTensor.hpp:443-544) ----

@@ -26,6 +26,8 @@
 // Jacobi SVD (mpsa_factor.hip) and multiplies S V^dagger or U S into the
 // neighbour with one grouped GEMM.  Host work is table bookkeeping only.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include <algorithm>
 #include <array>
@@ -47,6 +49,10 @@
 
 using namespace mpsa;
 using namespace mpsa::host;
+
+#ifndef MPSA_BIG_SECTOR
+#define MPSA_BIG_SECTOR 1024
+#endif
 
 namespace {
 
@@ -194,15 +200,25 @@ struct mpsa_cheb {
   };
   std::list<PlanEntry> plans;
   std::unordered_map<std::string, std::list<PlanEntry>::iterator> plan_index;
-  static constexpr size_t kPlanCap = 4096;
-  static constexpr int64_t kPlanBytesCap = int64_t(24) << 30;
+  // One compression sweep of 16 vectors at D = 1024 meets ~320 structures of
+  // ~80 MB of workspace each, so the byte cap holds several sweeps' worth: a
+  // cap at one sweep's size makes the LRU rebuild every plan of the next.
+  static constexpr size_t kPlanCap = 8192;
+  static constexpr int64_t kPlanBytesCap = int64_t(96) << 30;
   int64_t plan_bytes = 0;
   std::vector<DMps> slot;
   int64_t plans_built = 0;
+  int64_t big_sector = MPSA_BIG_SECTOR;  // sectors with min(m, n) at least this go to rocSOLVER
   // host time (seconds): plan builds, output-key queries, waits on the stream
   double t_plan = 0.0, t_keys = 0.0, t_sync = 0.0;
   std::vector<double> moment_s;  // wall seconds of each moment of the last moment loop
   int64_t n_sync = 0;
+  // sector shapes of every factorisation launch, while logging is on (up to
+  // flog_cap records of kFlogWidth values: kind, sectors, max m, max n, max k,
+  // sectors with k >= 64, >= 128, >= 256)
+  static constexpr int kFlogWidth = 8;
+  std::vector<int64_t> flog;
+  int64_t flog_cap = 0;
   ~mpsa_cheb() {
     if (st) (void)hipStreamSynchronize(st);
     for (auto& e : plans) mpsa_plan_destroy(e.plan);
@@ -509,6 +525,76 @@ std::vector<cplx> overlap(Eng& E, const DMps& a, const DMps& b) {
   return env_scalars(E, Lt, Lb);
 }
 
+// ------------------------------------------------------------ large sectors
+// Sectors with k = min(m, n) >= the engine's big_sector (default
+// MPSA_BIG_SECTOR = 1024, the batched kernels' limit) are factorised by
+// rocSOLVER, one call each, on the engine stream; the others go to the batched
+// launch.  A lower threshold hands more sectors to rocSOLVER, which pays only
+// for sectors far larger than the Hubbard D = 1024 workload's (k < 512 there):
+// at k >= 96 or >= 192 the serial rocSOLVER calls cost more than the batched
+// launch they shorten.
+static_assert(MPSA_BIG_SECTOR >= 1 && MPSA_BIG_SECTOR <= kSvdMaxR, "sectors below the threshold go to the batched SVD");
+
+void rschk(rocblas_status s, const char* what) {
+  if (s != rocblas_status_success) throw HErr(MPSA_EHIP, std::string("cheb: ") + what + ": rocBLAS/rocSOLVER status " +
+                                                            std::to_string((int)s));
+}
+
+// One sector M (m x n, ld m, planar) into the batched factorisation's output
+// format: SVD (qr = false): U (m x k), S, V^dagger (k x n), FixPhase on real
+// data; QR (left) U = Q, V^dagger = R, or LQ (right) U = L, V^dagger = Q, S = 1.
+// M is destroyed.  info: one int of device memory for rocSOLVER's status.
+void big_factor(Eng& E, bool qr, bool left, int64_t m, int64_t n, double* M, int64_t m_plane, double* U,
+                int64_t u_plane, double* Vt, int64_t vt_plane, double* Sv, rocblas_int* info) {
+  const int64_t k = std::min(m, n);
+  const bool c = E.planes == 2;
+  rocblas_handle h = (rocblas_handle)rocblas_handle_current();
+  rschk(rocblas_set_stream(h, E.st), "set stream");
+  const rocblas_int mi = (rocblas_int)m, ni = (rocblas_int)n, ki = (rocblas_int)k;
+  Buf Az, tau(E.pool, 2 * k, E.st, false), ework(E.pool, std::max<int64_t>(k, 1), E.st, false);
+  double* A = M;
+  if (c) {
+    Az = Buf(E.pool, 2 * m * n, E.st, false);
+    hchk(launch_pack_z(M, M + m_plane, m * n, Az.p, E.st), "cheb: pack sector");
+    A = Az.p;
+  }
+  auto* Az_ = reinterpret_cast<rocblas_double_complex*>(A);
+  auto* tz = reinterpret_cast<rocblas_double_complex*>(tau.p);
+  if (!qr) {
+    if (!c) {
+      rschk(rocsolver_dgesvd(h, rocblas_svect_singular, rocblas_svect_singular, mi, ni, A, mi, Sv, U, mi, Vt, ki,
+                             ework.p, rocblas_outofplace, info),
+            "dgesvd");
+      hchk(launch_fixphase(U, m, m, Vt, k, n, k, E.st), "cheb: fixphase");
+    } else {
+      Buf uz(E.pool, 2 * m * k, E.st, false), vz(E.pool, 2 * k * n, E.st, false);
+      rschk(rocsolver_zgesvd(h, rocblas_svect_singular, rocblas_svect_singular, mi, ni, Az_, mi, Sv,
+                             reinterpret_cast<rocblas_double_complex*>(uz.p), mi,
+                             reinterpret_cast<rocblas_double_complex*>(vz.p), ki, ework.p, rocblas_outofplace, info),
+            "zgesvd");
+      hchk(launch_unpack_mat(uz.p, 1, m, m, k, 0, U, m, u_plane, E.st), "cheb: unpack U");
+      hchk(launch_unpack_mat(vz.p, 1, k, k, n, 0, Vt, k, vt_plane, E.st), "cheb: unpack Vt");
+    }
+    return;
+  }
+  if (left) {  // M = Q R
+    if (c) rschk(rocsolver_zgeqrf(h, mi, ni, Az_, mi, tz), "zgeqrf");
+    else rschk(rocsolver_dgeqrf(h, mi, ni, A, mi, tau.p), "dgeqrf");
+    hchk(launch_unpack_mat(A, c, m, k, n, 1, Vt, k, vt_plane, E.st), "cheb: unpack R");
+    if (c) rschk(rocsolver_zungqr(h, mi, ki, ki, Az_, mi, tz), "zungqr");
+    else rschk(rocsolver_dorgqr(h, mi, ki, ki, A, mi, tau.p), "dorgqr");
+    hchk(launch_unpack_mat(A, c, m, m, k, 0, U, m, u_plane, E.st), "cheb: unpack Q");
+  } else {  // M = L Q
+    if (c) rschk(rocsolver_zgelqf(h, mi, ni, Az_, mi, tz), "zgelqf");
+    else rschk(rocsolver_dgelqf(h, mi, ni, A, mi, tau.p), "dgelqf");
+    hchk(launch_unpack_mat(A, c, m, m, k, 2, U, m, u_plane, E.st), "cheb: unpack L");
+    if (c) rschk(rocsolver_zunglq(h, ki, ni, ki, Az_, mi, tz), "zunglq");
+    else rschk(rocsolver_dorglq(h, ki, ni, ki, A, mi, tau.p), "dorglq");
+    hchk(launch_unpack_mat(A, c, m, k, n, 0, Vt, k, vt_plane, E.st), "cheb: unpack Q'");
+  }
+  hchk(launch_fill(Sv, k, 1.0, E.st), "cheb: unit S");
+}
+
 // ------------------------------------------------------------ prepareSite
 // prepareSite(mps, site, "l" or "r", multiply = true) for every sector of the
 // site at once; lambdas kept on the host when asked (for a later truncate).
@@ -527,6 +613,7 @@ void prepare_site(Eng& E, DMps& M, int site, bool left, bool keep_lambda) {
     std::vector<int64_t> blk;  // blocks of X in key order
     std::vector<int64_t> off;  // their row (l) or column (r) offsets in the sector matrix
     int64_t m = 0, n = 0, k = 0, mo = 0, uo = 0, vo = 0, so = 0, wo = 0;
+    bool big = false;  // factorised by rocSOLVER
   };
   std::map<uint64_t, Sec> secs;
   for (int64_t i = 0; i < X.n(); ++i) {
@@ -549,8 +636,7 @@ void prepare_site(Eng& E, DMps& M, int site, bool left, bool keep_lambda) {
       }
     }
     S.k = std::min(S.m, S.n);
-    if (!use_qr && std::min(S.m, S.n) > kSvdMaxR)
-      throw HErr(MPSA_EINVAL, "cheb: charge sector too large for the batched SVD");
+    S.big = S.k >= E.big_sector;
     S.mo = tm;
     tm += align64(S.m * S.n);
     S.uo = tu;
@@ -561,9 +647,7 @@ void prepare_site(Eng& E, DMps& M, int site, bool left, bool keep_lambda) {
     ts += S.k;
     S.wo = tw;
     const int64_t qp = left ? S.m : S.n;  // rows of the factorised X = M or M^H
-    const int64_t sp = std::max(S.m, S.n);  // the SVD kernel reduces tall sectors (p >= 2 k) by QR first
-    tw += use_qr ? align64(S.m * S.n + qp * S.k + S.k)
-                 : align64(sp >= 2 * S.k ? 2 * sp * S.k + 2 * S.k * S.k + S.k : sp * S.k + S.k * S.k);
+    if (!S.big) tw += use_qr ? align64(S.m * S.n + qp * S.k + S.k) : align64(svd_workspace(S.m, S.n));
   }
   const int pl = E.planes;
   Buf Mb(E.pool, pl * tm, E.st), Ub(E.pool, pl * tu, E.st), Vb(E.pool, pl * tv, E.st), Sb(E.pool, ts, E.st),
@@ -583,8 +667,13 @@ void prepare_site(Eng& E, DMps& M, int site, bool left, bool keep_lambda) {
   }
   run_copies(E, g, dX, Mb.p, nullptr);
   std::vector<SvdTask> tasks;
+  int nbig = 0;
   for (auto& kv : secs) {
     const Sec& S = kv.second;
+    if (S.big) {
+      ++nbig;
+      continue;
+    }
     SvdTask t{};
     t.a_off = S.mo;
     t.u_off = S.uo;
@@ -596,6 +685,30 @@ void prepare_site(Eng& E, DMps& M, int site, bool left, bool keep_lambda) {
     t.lda = (int32_t)std::max<int64_t>(S.m, 1);
     tasks.push_back(t);
   }
+  // largest sectors first: a launch lasts as long as its largest sector, so
+  // the big ones must not wait for a free CU behind the small ones
+  std::stable_sort(tasks.begin(), tasks.end(), [](const SvdTask& a, const SvdTask& b) {
+    const int64_t ka = std::min(a.m, a.n), kb = std::min(b.m, b.n);
+    return (int64_t)a.m * a.n * ka > (int64_t)b.m * b.n * kb;
+  });
+  if (E.flog_cap > 0 && (int64_t)E.flog.size() < E.flog_cap * Eng::kFlogWidth) {
+    // kind: 0 SVD, 1 QR/LQ in the batched kernels; 2, 3 the same on rocSOLVER
+    for (int big = 0; big < 2; ++big) {
+      int64_t rec[Eng::kFlogWidth] = {(use_qr ? 1 : 0) + 2 * big, 0, 0, 0, 0, 0, 0, 0};
+      for (auto& kv : secs) {
+        const Sec& S = kv.second;
+        if (S.big != (big == 1) || S.k < 1) continue;
+        ++rec[1];
+        rec[2] = std::max(rec[2], S.m);
+        rec[3] = std::max(rec[3], S.n);
+        rec[4] = std::max(rec[4], S.k);
+        rec[5] += S.k >= 64;
+        rec[6] += S.k >= 128;
+        rec[7] += S.k >= 256;
+      }
+      if (rec[1] && (int64_t)E.flog.size() < E.flog_cap * Eng::kFlogWidth) E.flog.insert(E.flog.end(), rec, rec + 8);
+    }
+  }
   {
     Buf dt = E.upload(tasks);
     if (use_qr)
@@ -606,6 +719,26 @@ void prepare_site(Eng& E, DMps& M, int site, bool left, bool keep_lambda) {
       hchk(launch_svd_batched(pl == 2, (const SvdTask*)dt.p, (int)tasks.size(), Mb.p, tm, Ub.p, tu, Vb.p, tv, Sb.p,
                               Wb.p, tw, E.st),
            "cheb: sector SVD");
+  }
+  if (nbig) {
+    Buf dinfo(E.pool, nbig, E.st);  // one rocblas_int per sector (zero-filled doubles)
+    rocblas_int* info = reinterpret_cast<rocblas_int*>(dinfo.p);
+    int b = 0;
+    for (auto& kv : secs) {
+      const Sec& S = kv.second;
+      if (!S.big) continue;
+      big_factor(E, use_qr, left, S.m, S.n, Mb.p + S.mo, tm, Ub.p + S.uo, tu, Vb.p + S.vo, tv, Sb.p + S.so,
+                 info + 2 * b);
+      ++b;
+    }
+    if (!use_qr) {  // gesvd reports non-convergence in info
+      std::vector<rocblas_int> hi((size_t)2 * nbig);
+      hchk(hipMemcpyAsync(hi.data(), info, sizeof(rocblas_int) * hi.size(), hipMemcpyDeviceToHost, E.st),
+           "cheb: sector info");
+      E.sync();
+      for (int i = 0; i < nbig; ++i)
+        if (hi[(size_t)2 * i] != 0) throw HErr(MPSA_EINVAL, "cheb: rocSOLVER gesvd did not converge on a sector");
+    }
   }
   // new tables: the factorised tensor keeps U (l) or V^dagger (r) with k
   // columns (rows); the neighbour's blocks of the sector charge take k rows
@@ -1540,6 +1673,15 @@ int mpsa_cheb_profile(const mpsa_cheb* E, double* out5) {
   return MPSA_OK;
 }
 
+int mpsa_cheb_set_big_sector(mpsa_cheb* E, int k) {
+  if (!E || k < 1 || k > kSvdMaxR) {
+    set_last_error("cheb_set_big_sector: k must be in [1, 1024]");
+    return MPSA_EINVAL;
+  }
+  E->big_sector = k;
+  return MPSA_OK;
+}
+
 int mpsa_cheb_moment_times(const mpsa_cheb* E, double* out, int cap, int* n) {
   if (!E || !n || cap < 0 || (cap > 0 && !out)) {
     set_last_error("cheb_moment_times: bad arguments");
@@ -1547,6 +1689,27 @@ int mpsa_cheb_moment_times(const mpsa_cheb* E, double* out, int cap, int* n) {
   }
   *n = (int)E->moment_s.size();
   for (int i = 0; i < cap && i < *n; ++i) out[i] = E->moment_s[(size_t)i];
+  return MPSA_OK;
+}
+
+int mpsa_cheb_factor_log(mpsa_cheb* E, int64_t cap) {
+  if (!E || cap < 0) {
+    set_last_error("cheb_factor_log: bad arguments");
+    return MPSA_EINVAL;
+  }
+  E->flog.clear();
+  E->flog_cap = cap;
+  return MPSA_OK;
+}
+
+int mpsa_cheb_factor_records(const mpsa_cheb* E, int64_t* out, int64_t cap, int64_t* n) {
+  if (!E || !n || cap < 0 || (cap > 0 && !out)) {
+    set_last_error("cheb_factor_records: bad arguments");
+    return MPSA_EINVAL;
+  }
+  const int64_t w = mpsa_cheb::kFlogWidth;
+  *n = (int64_t)E->flog.size() / w;
+  for (int64_t i = 0; i < std::min(cap, *n) * w; ++i) out[i] = E->flog[(size_t)i];
   return MPSA_OK;
 }
 

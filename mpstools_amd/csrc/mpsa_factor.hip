@@ -4,23 +4,27 @@
 // 614-634) for MPS canonicalisation and truncation, where a site tensor splits
 // into hundreds of sectors of tens of rows each.
 //
-// Method: one-sided Jacobi (Hestenes) on the columns of X = M (m >= n) or
-// X = M^H (m < n), p x r with p >= r.  Column pairs are visited in the
-// round-robin order of a tournament (r/2 disjoint pairs per round, r - 1
-// rounds per sweep), one wave per pair, so a wave needs no barrier inside a
-// rotation; the workgroup synchronises between rounds.  A pair is rotated when
-// |x_i^H x_j| > tol sqrt(|x_i|^2 |x_j|^2) with tol = sqrt(p) eps; sweeps stop
+// Method: X = M (m >= n) or X = M^H (m < n), p x r with p >= r, is reduced
+// by Householder QR, X = Q R, and one-sided Jacobi (Hestenes) runs on the
+// columns of A = R^H (r x r; the Drmac-Veselic preconditioning without
+// pivoting).  Column pairs are visited in the round-robin order of a
+// tournament (r/2 disjoint pairs per round, r - 1 rounds per sweep), one wave
+// per pair, so a wave needs no barrier inside a rotation; the workgroup
+// synchronises between rounds.  A pair is rotated when
+// |a_i^H a_j| > tol sqrt(|a_i|^2 |a_j|^2) with tol = sqrt(r) eps; sweeps stop
 // when a whole sweep rotates nothing.  The 2 x 2 Hermitian Jacobi rotation
 // with gamma = |gamma| e^{i phi}, zeta = (b - a) / (2 |gamma|),
 // t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)), c = 1 / sqrt(1 + t^2), s = c t:
-//   x_i' = c x_i - s e^{-i phi} x_j,   x_j' = s e^{i phi} x_i + c x_j
-// makes the pair orthogonal; V accumulates the same rotations.  Then
-// sigma_j = |x_j|, sorted descending (ties by column index), and
-//   m >= n: U = X / sigma, V^dagger = V^H;   m < n: U = V, V^dagger = (X / sigma)^H.
-// Singular vectors of exactly zero singular values are completed to an
-// orthonormal basis (Gram-Schmidt of unit vectors, twice), as a LAPACK thin
-// SVD returns them.  Real data: FixPhase makes the first largest-|.| entry of
-// every row of V^dagger positive (the column of U flips with it).
+//   a_i' = c a_i - s e^{-i phi} a_j,   a_j' = s e^{i phi} a_i + c a_j
+// makes the pair orthogonal; Wm accumulates the same rotations.  Then
+// sigma_j = |a_j|, sorted descending (ties by column index), Z = A / sigma,
+// and X = (Q Wm) Sigma Z^H:
+//   m >= n: U = Q Wm, V^dagger = Z^H;   m < n: U = Z, V^dagger = (Q Wm)^H.
+// Singular vectors of exactly zero singular values (on the Z side; Q Wm is
+// orthonormal by construction) are completed to an orthonormal basis
+// (Gram-Schmidt of unit vectors, twice), as a LAPACK thin SVD returns them.
+// Real data: FixPhase makes the first largest-|.| entry of every row of
+// V^dagger positive (the column of U flips with it).
 //
 // Complex data is planar (real parts, then imaginary parts at a plane offset),
 // like every complex buffer of the library.
@@ -77,16 +81,88 @@ __device__ __forceinline__ void rotate_cols(const Mat<C>& X, int p, int i, int j
   }
 }
 
+// Rows per lane of the register-held column paths: a column of up to
+// 64 * kRpl rows lives in registers of one wave while it is reduced and
+// updated (one global read and one write instead of two reads and a write),
+// and a reflection vector of up to kRpl * 64 rows sits in LDS.
+constexpr int kRpl = 16;
+constexpr int kVecRows = 64 * kRpl;
+static_assert(kSvdMaxR <= kVecRows, "a Jacobi column of r <= kSvdMaxR rows fits the register path");
+
+// y <- y - f v for one column y (rows from..p-1) held by one wave, with
+// f = beta (v^H y) and v at vl (LDS, planar at + kVecRows, v[0] at row from).
+template <bool C>
+__device__ __forceinline__ void reflect_col_regs(const Mat<C>& Y, int c, int from, int p, const double* vl,
+                                                 double b, int lane) {
+  double yr[kRpl], yi[kRpl];
+  double wr_ = 0.0, wi_ = 0.0;
+#pragma unroll
+  for (int q = 0; q < kRpl; ++q) {
+    const int i = from + lane + 64 * q;
+    yr[q] = yi[q] = 0.0;
+    if (i < p) {
+      yr[q] = Y.r(i, c);
+      if (C) yi[q] = Y.m(i, c);
+      const double v_r = vl[i - from];
+      wr_ += v_r * yr[q];
+      if (C) {
+        const double v_i = vl[kVecRows + i - from];
+        wr_ += v_i * yi[q];
+        wi_ += v_r * yi[q] - v_i * yr[q];  // conj(v) y
+      }
+    }
+  }
+  wr_ = wave_sum(wr_);
+  if (C) wi_ = wave_sum(wi_);
+  const double fr = b * wr_, fi = b * wi_;
+#pragma unroll
+  for (int q = 0; q < kRpl; ++q) {
+    const int i = from + lane + 64 * q;
+    if (i < p) {
+      const double v_r = vl[i - from];
+      const double v_i = C ? vl[kVecRows + i - from] : 0.0;
+      Y.r(i, c) = yr[q] - (fr * v_r - fi * v_i);
+      if (C) Y.m(i, c) = yi[q] - (fr * v_i + fi * v_r);
+    }
+  }
+}
+
+// The same update with v read from its global column (any p).
+template <bool C>
+__device__ __forceinline__ void reflect_col_global(const Mat<C>& Y, int c, int from, int p, const Mat<C>& Hv, int j,
+                                                   double b, int lane) {
+  double wr_ = 0.0, wi_ = 0.0;
+  for (int i = from + lane; i < p; i += 64) {
+    const double v_r = Hv.r(i, j), y_r = Y.r(i, c);
+    wr_ += v_r * y_r;
+    if (C) {
+      const double v_i = Hv.m(i, j), y_i = Y.m(i, c);
+      wr_ += v_i * y_i;
+      wi_ += v_r * y_i - v_i * y_r;
+    }
+  }
+  wr_ = wave_sum(wr_);
+  if (C) wi_ = wave_sum(wi_);
+  const double fr = b * wr_, fi = b * wi_;
+  for (int i = from + lane; i < p; i += 64) {
+    const double v_r = Hv.r(i, j);
+    const double v_i = C ? Hv.m(i, j) : 0.0;
+    Y.r(i, c) -= fr * v_r - fi * v_i;
+    if (C) Y.m(i, c) -= fr * v_i + fi * v_r;
+  }
+}
+
 // Householder reflections of X (p x r, planar, ld p) in place, one per
 // column j < k = min(p, r): H_j = I - beta_j v_j v_j^H with v_j = x - alpha e_j,
 // alpha = -e^{i arg x_j} |x| (x = rows j.. of column j), beta_j = 2 / |v_j|^2
 // (0 for a zero column: H_j = I).  Column j becomes (alpha, 0, ...), the
-// columns after it are reflected (one wave per column); v_j goes to rows j..
-// of column j of Hv, beta_j to beta[j].  NT threads; red holds NT / 64
-// doubles, hv 3.
+// columns after it are reflected (one wave per column, the column in
+// registers and v_j in LDS when p - j <= kVecRows); v_j goes to rows j.. of
+// column j of Hv, beta_j to beta[j].  NT threads; red holds NT / 64 doubles,
+// hv 3, vl 2 kVecRows (LDS).
 template <bool C, int NT>
 __device__ void house_reflect(const Mat<C>& X, const Mat<C>& Hv, double* beta, int p, int r, double* red,
-                              double* hv) {
+                              double* hv, double* vl) {
   constexpr int NW = NT / 64;
   const int k = p < r ? p : r;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -124,6 +200,7 @@ __device__ void house_reflect(const Mat<C>& X, const Mat<C>& Hv, double* beta, i
     }
     __syncthreads();
     const double ar = hv[0], ai = hv[1], b = hv[2];
+    const bool in_lds = p - j <= kVecRows;
     for (int i = j + tid; i < p; i += NT) {
       double v_r = X.r(i, j), v_i = C ? X.m(i, j) : 0.0;
       if (i == j) {
@@ -132,28 +209,15 @@ __device__ void house_reflect(const Mat<C>& X, const Mat<C>& Hv, double* beta, i
       }
       Hv.r(i, j) = v_r;
       if (C) Hv.m(i, j) = v_i;
+      if (in_lds) {
+        vl[i - j] = v_r;
+        if (C) vl[kVecRows + i - j] = v_i;
+      }
     }
     __syncthreads();
     for (int c = j + 1 + wave; c < r; c += NW) {
-      double wr_ = 0.0, wi_ = 0.0;
-      for (int i = j + lane; i < p; i += 64) {
-        const double v_r = Hv.r(i, j), x_r = X.r(i, c);
-        wr_ += v_r * x_r;
-        if (C) {
-          const double v_i = Hv.m(i, j), x_i = X.m(i, c);
-          wr_ += v_i * x_i;
-          wi_ += v_r * x_i - v_i * x_r;  // conj(v) x
-        }
-      }
-      wr_ = wave_sum(wr_);
-      if (C) wi_ = wave_sum(wi_);
-      const double fr = b * wr_, fi = b * wi_;
-      for (int i = j + lane; i < p; i += 64) {
-        const double v_r = Hv.r(i, j);
-        const double v_i = C ? Hv.m(i, j) : 0.0;
-        X.r(i, c) -= fr * v_r - fi * v_i;
-        if (C) X.m(i, c) -= fr * v_i + fi * v_r;
-      }
+      if (in_lds) reflect_col_regs<C>(X, c, j, p, vl, b, lane);
+      else reflect_col_global<C>(X, c, j, p, Hv, j, b, lane);
     }
     for (int i = j + tid; i < p; i += NT) {
       X.r(i, j) = i == j ? ar : 0.0;
@@ -165,36 +229,26 @@ __device__ void house_reflect(const Mat<C>& X, const Mat<C>& Hv, double* beta, i
 
 // Y <- H_0 H_1 ... H_{k-1} Y for the ncols columns of Y (p rows), applying
 // H_j for j = k-1 .. 0; with from_j only columns >= j (Y = [I; 0] there, as
-// LAPACK's xORGQR forms Q).
+// LAPACK's xORGQR forms Q).  vl: 2 kVecRows doubles of LDS.
 template <bool C, int NT>
 __device__ void house_apply(const Mat<C>& Y, int ncols, bool from_j, const Mat<C>& Hv, const double* beta, int p,
-                            int k) {
+                            int k, double* vl) {
   constexpr int NW = NT / 64;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int j = k - 1; j >= 0; --j) {
     const double b = beta[j];
-    if (b != 0.0) {
-      for (int c = (from_j ? j : 0) + wave; c < ncols; c += NW) {
-        double wr_ = 0.0, wi_ = 0.0;
-        for (int i = j + lane; i < p; i += 64) {
-          const double v_r = Hv.r(i, j), y_r = Y.r(i, c);
-          wr_ += v_r * y_r;
-          if (C) {
-            const double v_i = Hv.m(i, j), y_i = Y.m(i, c);
-            wr_ += v_i * y_i;
-            wi_ += v_r * y_i - v_i * y_r;
-          }
-        }
-        wr_ = wave_sum(wr_);
-        if (C) wi_ = wave_sum(wi_);
-        const double fr = b * wr_, fi = b * wi_;
-        for (int i = j + lane; i < p; i += 64) {
-          const double v_r = Hv.r(i, j);
-          const double v_i = C ? Hv.m(i, j) : 0.0;
-          Y.r(i, c) -= fr * v_r - fi * v_i;
-          if (C) Y.m(i, c) -= fr * v_i + fi * v_r;
-        }
+    if (b == 0.0) continue;  // uniform across the workgroup
+    const bool in_lds = p - j <= kVecRows;
+    if (in_lds) {
+      for (int i = j + tid; i < p; i += NT) {
+        vl[i - j] = Hv.r(i, j);
+        if (C) vl[kVecRows + i - j] = Hv.m(i, j);
       }
+      __syncthreads();
+    }
+    for (int c = (from_j ? j : 0) + wave; c < ncols; c += NW) {
+      if (in_lds) reflect_col_regs<C>(Y, c, j, p, vl, b, lane);
+      else reflect_col_global<C>(Y, c, j, p, Hv, j, b, lane);
     }
     __syncthreads();
   }
@@ -214,49 +268,46 @@ __global__ __launch_bounds__(kThreads) void svd_jacobi_kernel(const SvdTask* __r
   __shared__ int rotated;
   __shared__ double red[kWaves], redc[kWaves];
   __shared__ double hv[3];
+  __shared__ double vl[2 * kVecRows];
   const SvdTask t = tasks[blockIdx.x];
   const int m = t.m, n = t.n;
   if (m <= 0 || n <= 0) return;
   const bool wide = m < n;
   const int p = wide ? n : m, r = wide ? m : n, k = r;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  // Tall X (p >= 2 r) is reduced first, X = Q R by Householder QR, and the
-  // rotations run on the r x r R (r rows per column pair instead of p); the
-  // rotated R goes back through Q at the end.  Workspace: X (p x r), V
-  // (r x r), and for the reduction the vectors (p x r), R (r x r), betas (r).
-  const bool pre = p >= 2 * r;
+  // X = Q R (Householder), then the rotations run on the columns of
+  // A = R^H (r x r), accumulated in Wm: A Wm = Z Sigma, so R = Wm Sigma Z^H
+  // and X = (Q Wm) Sigma Z^H.  On R^H the sweeps of a graded or rank-deficient
+  // sector are a third of those on R or X itself (the rows of a triangular
+  // factor are closer to orthogonal than its columns).  The p-side singular
+  // vectors Q Wm are orthonormal by construction; the r-side ones are the
+  // rotated columns of A divided by sigma.  Workspace: X (p x r), Wm (r x r),
+  // the reflection vectors (p x r), A (r x r), betas (r).
   Mat<C> X{W + t.w_off, W + t.w_off + w_plane, p};
-  Mat<C> V{W + t.w_off + (int64_t)p * r, W + t.w_off + (int64_t)p * r + w_plane, r};
-  const int64_t o_hv = (int64_t)p * r + (int64_t)r * r, o_rm = 2 * (int64_t)p * r + (int64_t)r * r;
+  Mat<C> Wm{W + t.w_off + (int64_t)p * r, W + t.w_off + (int64_t)p * r + w_plane, r};
+  const int64_t o_hv = (int64_t)p * r + (int64_t)r * r, o_a = 2 * (int64_t)p * r + (int64_t)r * r;
   Mat<C> Hv{W + t.w_off + o_hv, W + t.w_off + o_hv + w_plane, p};
-  Mat<C> Rm{W + t.w_off + o_rm, W + t.w_off + o_rm + w_plane, r};
-  double* const beta = W + t.w_off + o_rm + (int64_t)r * r;
-  // X = M or M^H, V = identity
+  Mat<C> A{W + t.w_off + o_a, W + t.w_off + o_a + w_plane, r};
+  double* const beta = W + t.w_off + o_a + (int64_t)r * r;
+  // X = M or M^H
   for (int64_t e = tid; e < (int64_t)p * r; e += kThreads) {
     const int i = (int)(e % p), j = (int)(e / p);
     const int64_t src = wide ? (j + (int64_t)i * t.lda) : (i + (int64_t)j * t.lda);
     X.r(i, j) = M[t.a_off + src];
     if (C) X.m(i, j) = wide ? -M[t.a_off + m_plane + src] : M[t.a_off + m_plane + src];
   }
+  __syncthreads();
+  house_reflect<C, kThreads>(X, Hv, beta, p, r, red, hv, vl);
+  // A = R^H (lower triangular), Wm = identity
   for (int64_t e = tid; e < (int64_t)r * r; e += kThreads) {
     const int i = (int)(e % r), j = (int)(e / r);
-    V.r(i, j) = i == j ? 1.0 : 0.0;
-    if (C) V.m(i, j) = 0.0;
+    A.r(i, j) = i >= j ? X.r(j, i) : 0.0;
+    if (C) A.m(i, j) = i >= j ? -X.m(j, i) : 0.0;
+    Wm.r(i, j) = i == j ? 1.0 : 0.0;
+    if (C) Wm.m(i, j) = 0.0;
   }
   __syncthreads();
-  if (pre) {
-    house_reflect<C, kThreads>(X, Hv, beta, p, r, red, hv);
-    for (int64_t e = tid; e < (int64_t)r * r; e += kThreads) {
-      const int i = (int)(e % r), j = (int)(e / r);
-      Rm.r(i, j) = i <= j ? X.r(i, j) : 0.0;
-      if (C) Rm.m(i, j) = i <= j ? X.m(i, j) : 0.0;
-    }
-    __syncthreads();
-  }
-  // the matrix the rotations act on: X itself, or R
-  const Mat<C> J = pre ? Rm : X;
-  const int pj = pre ? r : p;
-  const double tol = sqrt((double)pj) * DBL_EPSILON;
+  const double tol = sqrt((double)r) * DBL_EPSILON;
   const int rr = r + (r & 1);
   for (int sweep = 0; sweep < max_sweeps && r > 1; ++sweep) {
     if (tid == 0) rotated = 0;
@@ -271,18 +322,27 @@ __global__ __launch_bounds__(kThreads) void svd_jacobi_kernel(const SvdTask* __r
           j = x;
         }
         if (j >= r) continue;
+        // the pair's columns stay in registers (r <= kSvdMaxR = 64 kRpl rows)
+        double xir[kRpl], xii[kRpl], xjr[kRpl], xji[kRpl];
         double a = 0.0, b = 0.0, gr = 0.0, gi = 0.0;
-        for (int row = lane; row < pj; row += 64) {
-          const double xir = J.r(row, i), xjr = J.r(row, j);
-          a += xir * xir;
-          b += xjr * xjr;
-          gr += xir * xjr;
-          if (C) {
-            const double xii = J.m(row, i), xji = J.m(row, j);
-            a += xii * xii;
-            b += xji * xji;
-            gr += xii * xji;
-            gi += xir * xji - xii * xjr;  // x_i^H x_j = sum conj(x_i) x_j
+#pragma unroll
+        for (int q = 0; q < kRpl; ++q) {
+          const int row = lane + 64 * q;
+          xir[q] = xii[q] = xjr[q] = xji[q] = 0.0;
+          if (row < r) {
+            xir[q] = A.r(row, i);
+            xjr[q] = A.r(row, j);
+            a += xir[q] * xir[q];
+            b += xjr[q] * xjr[q];
+            gr += xir[q] * xjr[q];
+            if (C) {
+              xii[q] = A.m(row, i);
+              xji[q] = A.m(row, j);
+              a += xii[q] * xii[q];
+              b += xji[q] * xji[q];
+              gr += xii[q] * xji[q];
+              gi += xir[q] * xji[q] - xii[q] * xjr[q];  // a_i^H a_j = sum conj(a_i) a_j
+            }
           }
         }
         a = wave_sum(a);
@@ -295,8 +355,25 @@ __global__ __launch_bounds__(kThreads) void svd_jacobi_kernel(const SvdTask* __r
         const double zeta = (b - a) / (2.0 * g);
         const double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
         const double c = 1.0 / sqrt(1.0 + tt * tt), s = c * tt;
-        rotate_cols<C>(J, pj, i, j, c, s, er, ei, lane);
-        rotate_cols<C>(V, r, i, j, c, s, er, ei, lane);
+        // a_i' = c a_i - s conj(e) a_j, a_j' = s e a_i + c a_j (as rotate_cols)
+#pragma unroll
+        for (int q = 0; q < kRpl; ++q) {
+          const int row = lane + 64 * q;
+          if (row < r) {
+            if (C) {
+              const double cjr = er * xjr[q] + ei * xji[q], cji = er * xji[q] - ei * xjr[q];
+              const double eir = er * xir[q] - ei * xii[q], eii = er * xii[q] + ei * xir[q];
+              A.r(row, i) = c * xir[q] - s * cjr;
+              A.m(row, i) = c * xii[q] - s * cji;
+              A.r(row, j) = s * eir + c * xjr[q];
+              A.m(row, j) = s * eii + c * xji[q];
+            } else {
+              A.r(row, i) = c * xir[q] - s * er * xjr[q];
+              A.r(row, j) = s * er * xir[q] + c * xjr[q];
+            }
+          }
+        }
+        rotate_cols<C>(Wm, r, i, j, c, s, er, ei, lane);
         if (lane == 0) rotated = 1;
       }
       __syncthreads();
@@ -305,23 +382,22 @@ __global__ __launch_bounds__(kThreads) void svd_jacobi_kernel(const SvdTask* __r
     __syncthreads();
     if (!any) break;
   }
-  if (pre) {  // X = Q [R V; 0]: the rotated columns back through the reflections
-    for (int64_t e = tid; e < (int64_t)p * r; e += kThreads) {
-      const int i = (int)(e % p), j = (int)(e / p);
-      X.r(i, j) = i < r ? Rm.r(i, j) : 0.0;
-      if (C) X.m(i, j) = i < r ? Rm.m(i, j) : 0.0;
-    }
-    __syncthreads();
-    house_apply<C, kThreads>(X, r, false, Hv, beta, p, r);
+  // X = Q [Wm; 0]: the accumulated rotations back through the reflections
+  for (int64_t e = tid; e < (int64_t)p * r; e += kThreads) {
+    const int i = (int)(e % p), j = (int)(e / p);
+    X.r(i, j) = i < r ? Wm.r(i, j) : 0.0;
+    if (C) X.m(i, j) = i < r ? Wm.m(i, j) : 0.0;
   }
-  // singular values and their descending order
+  __syncthreads();
+  house_apply<C, kThreads>(X, r, false, Hv, beta, p, r, vl);
+  // singular values (column norms of the rotated A) and their descending order
   for (int j = wave; j < r; j += kWaves) {
     double a = 0.0;
-    for (int row = lane; row < p; row += 64) {
-      const double x = X.r(row, j);
+    for (int row = lane; row < r; row += 64) {
+      const double x = A.r(row, j);
       a += x * x;
       if (C) {
-        const double y = X.m(row, j);
+        const double y = A.m(row, j);
         a += y * y;
       }
     }
@@ -337,7 +413,8 @@ __global__ __launch_bounds__(kThreads) void svd_jacobi_kernel(const SvdTask* __r
     S[t.s_off + rk] = sj;
   }
   __syncthreads();
-  // U (m x k, ld m) and V^dagger (k x n, ld k); "vec" = the side built from X
+  // U (m x k, ld m) and V^dagger (k x n, ld k): the p side from Q Wm, the r
+  // side from A / sigma
   double* Ur = U + t.u_off;
   double* Ui = U + t.u_off + u_plane;
   double* Vr = Vt + t.vt_off;
@@ -345,42 +422,43 @@ __global__ __launch_bounds__(kThreads) void svd_jacobi_kernel(const SvdTask* __r
   for (int64_t e = tid; e < (int64_t)p * r; e += kThreads) {
     const int row = (int)(e % p), j = (int)(e / p);
     const int rk = rank[j];
-    const double sj = sig[j];
-    const double xr = sj > 0.0 ? X.r(row, j) / sj : 0.0;
-    const double xi = (C && sj > 0.0) ? X.m(row, j) / sj : 0.0;
-    if (!wide) {  // U[:, rk] = x_j / sigma_j
-      Ur[row + (int64_t)rk * m] = xr;
-      if (C) Ui[row + (int64_t)rk * m] = xi;
-    } else {  // V^dagger[rk, :] = (x_j / sigma_j)^H
-      Vr[rk + (int64_t)row * k] = xr;
-      if (C) Vi[rk + (int64_t)row * k] = -xi;
+    if (!wide) {  // U[:, rk] = (Q Wm)[:, j]
+      Ur[row + (int64_t)rk * m] = X.r(row, j);
+      if (C) Ui[row + (int64_t)rk * m] = X.m(row, j);
+    } else {  // V^dagger[rk, :] = (Q Wm)[:, j]^H
+      Vr[rk + (int64_t)row * k] = X.r(row, j);
+      if (C) Vi[rk + (int64_t)row * k] = -X.m(row, j);
     }
   }
   for (int64_t e = tid; e < (int64_t)r * r; e += kThreads) {
     const int row = (int)(e % r), j = (int)(e / r);
     const int rk = rank[j];
-    if (!wide) {  // V^dagger[rk, row] = conj(V[row, j])
-      Vr[rk + (int64_t)row * k] = V.r(row, j);
-      if (C) Vi[rk + (int64_t)row * k] = -V.m(row, j);
-    } else {  // U[row, rk] = V[row, j]
-      Ur[row + (int64_t)rk * m] = V.r(row, j);
-      if (C) Ui[row + (int64_t)rk * m] = V.m(row, j);
+    const double sj = sig[j];
+    const double zr = sj > 0.0 ? A.r(row, j) / sj : 0.0;
+    const double zi = (C && sj > 0.0) ? A.m(row, j) / sj : 0.0;
+    if (!wide) {  // V^dagger[rk, row] = conj(Z[row, j])
+      Vr[rk + (int64_t)row * k] = zr;
+      if (C) Vi[rk + (int64_t)row * k] = -zi;
+    } else {  // U[row, rk] = Z[row, j]
+      Ur[row + (int64_t)rk * m] = zr;
+      if (C) Ui[row + (int64_t)rk * m] = zi;
     }
   }
   __syncthreads();
-  // orthonormal completion of the vectors of zero singular values (descending
-  // order puts them last): vectors of length p stored at base + idx * vs with
-  // stride es between elements (U columns: es = 1, vs = m; V^dagger rows:
-  // es = k, vs = 1).  The conjugate of a V^dagger row is the vector.
-  const int64_t es = wide ? k : 1, vs = wide ? 1 : m;
-  double* Br = wide ? Vr : Ur;
-  double* Bi = wide ? Vi : Ui;
-  const double csign = wide ? -1.0 : 1.0;  // stored imaginary part = csign * vector imaginary part
+  // orthonormal completion of the r-side vectors of zero singular values
+  // (descending order puts them last): vectors of length r stored at
+  // base + idx * vs with stride es between elements (V^dagger rows: es = k,
+  // vs = 1; U columns: es = 1, vs = m).  The conjugate of a V^dagger row is
+  // the vector.
+  const int64_t es = wide ? 1 : k, vs = wide ? m : 1;
+  double* Br = wide ? Ur : Vr;
+  double* Bi = wide ? Ui : Vi;
+  const double csign = wide ? 1.0 : -1.0;  // stored imaginary part = csign * vector imaginary part
   for (int rk = 0; rk < k; ++rk) {
     if (S[t.s_off + rk] > 0.0) continue;
-    for (int cand = 0; cand < p; ++cand) {
+    for (int cand = 0; cand < r; ++cand) {
       // w = e_cand, then two passes of w -= (u_i^H w) u_i over i < rk
-      for (int row = tid; row < p; row += kThreads) {
+      for (int row = tid; row < r; row += kThreads) {
         Br[rk * vs + row * es] = row == cand ? 1.0 : 0.0;
         if (C) Bi[rk * vs + row * es] = 0.0;
       }
@@ -388,7 +466,7 @@ __global__ __launch_bounds__(kThreads) void svd_jacobi_kernel(const SvdTask* __r
       for (int pass = 0; pass < 2; ++pass)
         for (int i = 0; i < rk; ++i) {
           double dr = 0.0, di = 0.0;
-          for (int row = tid; row < p; row += kThreads) {
+          for (int row = tid; row < r; row += kThreads) {
             const double ur = Br[i * vs + row * es], wr = Br[rk * vs + row * es];
             dr += ur * wr;
             if (C) {
@@ -409,7 +487,7 @@ __global__ __launch_bounds__(kThreads) void svd_jacobi_kernel(const SvdTask* __r
             Dr += red[w];
             Di += redc[w];
           }
-          for (int row = tid; row < p; row += kThreads) {
+          for (int row = tid; row < r; row += kThreads) {
             const double ur = Br[i * vs + row * es];
             const double ui = C ? csign * Bi[i * vs + row * es] : 0.0;
             // w -= (Dr + i Di) u
@@ -419,7 +497,7 @@ __global__ __launch_bounds__(kThreads) void svd_jacobi_kernel(const SvdTask* __r
           __syncthreads();
         }
       double nn = 0.0;
-      for (int row = tid; row < p; row += kThreads) {
+      for (int row = tid; row < r; row += kThreads) {
         const double wr = Br[rk * vs + row * es];
         nn += wr * wr;
         if (C) {
@@ -435,7 +513,7 @@ __global__ __launch_bounds__(kThreads) void svd_jacobi_kernel(const SvdTask* __r
       __syncthreads();
       if (tot > 0.25) {
         const double inv = 1.0 / sqrt(tot);
-        for (int row = tid; row < p; row += kThreads) {
+        for (int row = tid; row < r; row += kThreads) {
           Br[rk * vs + row * es] *= inv;
           if (C) Bi[rk * vs + row * es] *= inv;
         }
@@ -491,7 +569,7 @@ __global__ __launch_bounds__(kThreads) void svd_jacobi_kernel(const SvdTask* __r
 // per column; then Q = H_0 ... H_{k-1} [I; 0] is accumulated backwards
 // (columns j.. of the partial product, as LAPACK's xORGQR does).
 // Workspace per task: p r (X) + p k (vectors) + k (betas) elements per plane.
-constexpr int kQrThreads = 1024;
+constexpr int kQrThreads = 512;
 constexpr int kQrWaves = kQrThreads / 64;
 
 template <bool C>
@@ -503,6 +581,7 @@ __global__ __launch_bounds__(kQrThreads) void qr_householder_kernel(const SvdTas
                                                                     int64_t w_plane) {
   __shared__ double red[kQrWaves];
   __shared__ double hv[3];
+  __shared__ double vl[2 * kVecRows];
   const SvdTask t = tasks[blockIdx.x];
   const int m = t.m, n = t.n;
   if (m <= 0 || n <= 0) return;
@@ -519,7 +598,7 @@ __global__ __launch_bounds__(kQrThreads) void qr_householder_kernel(const SvdTas
     if (C) X.m(i, j) = right ? -M[t.a_off + m_plane + src] : M[t.a_off + m_plane + src];
   }
   __syncthreads();
-  house_reflect<C, kQrThreads>(X, Hv, beta, p, r, red, hv);
+  house_reflect<C, kQrThreads>(X, Hv, beta, p, r, red, hv, vl);
   // R (k x r) = the upper trapezoid of X; written before X is reused for Q
   double* Ur = U + t.u_off;
   double* Ui = U + t.u_off + u_plane;
@@ -546,7 +625,7 @@ __global__ __launch_bounds__(kQrThreads) void qr_householder_kernel(const SvdTas
     if (C) X.m(i, c) = 0.0;
   }
   __syncthreads();
-  house_apply<C, kQrThreads>(X, k, true, Hv, beta, p, k);
+  house_apply<C, kQrThreads>(X, k, true, Hv, beta, p, k, vl);
   for (int64_t e = tid; e < (int64_t)p * k; e += kQrThreads) {
     const int i = (int)(e % p), c = (int)(e / p);
     if (!right) {  // U (m x k) = Q

@@ -92,6 +92,16 @@ int gemm_max_grid(int prec_in, int tile_class);
 void finalize_gemm_list(std::vector<GTask>& tasks, std::vector<GSeg>& segs, int prec, int* tiles, int* tile_class);
 // whether the plan's operands are complex128 (planar)
 bool plan_complex(const mpsa_plan* P);
+// the library's rocBLAS handle of the current device (a rocblas_handle),
+// created on first use; callers set its stream before each use
+void* rocblas_handle_current();
+// planar complex (re, im: n each) -> interleaved z (2 n)
+hipError_t launch_pack_z(const double* re, const double* im, int64_t n, double* z, hipStream_t st);
+// rows x cols window of A (real, or interleaved complex when cplx; ld lda) ->
+// planar out (ld ldo, imaginary plane at out + plane): mode 0 whole window,
+// 1 upper trapezoid (i <= j), 2 lower (i >= j), zeros elsewhere
+hipError_t launch_unpack_mat(const double* A, int cplx, int64_t lda, int64_t rows, int64_t cols, int mode,
+                             double* out, int64_t ldo, int64_t plane, hipStream_t st);
 // dense block copies of the split (column-major; see copy2d_kernel)
 struct Copy2D {
   int64_t src_off, dst_off;
@@ -104,13 +114,19 @@ hipError_t launch_copy2d(const Copy2D* tasks, int ntasks, int64_t max_elems, con
                          const double* scales, hipStream_t st);
 hipError_t launch_fixphase(double* U, int64_t m, int64_t ldu, double* Vt, int64_t k, int64_t n, int64_t ldv,
                            hipStream_t st);
+// out[j] = |A[:, j]| for the cols columns of A (rows x cols, ld lda)
+hipError_t launch_colnorms(const double* A, int64_t rows, int64_t cols, int64_t lda, double* out, hipStream_t st);
 // Batched thin SVD of small sector matrices (mpsa_factor.hip): M (m x n, ld
 // lda) at a_off -> U (m x k, ld m) at u_off, S (k) at s_off, V^dagger (k x n,
-// ld k) at vt_off, k = min(m, n); workspace at w_off of p * r + r * r
-// elements, or 2 p r + 2 r r + r when p >= 2 r (tall sectors are reduced by a
-// QR first), per plane (p = max(m, n), r = min(m, n) <= kSvdMaxR).  Complex
-// data planar.
+// ld k) at vt_off, k = min(m, n); workspace at w_off of svd_workspace(m, n)
+// elements per plane (p = max(m, n), r = min(m, n) <= kSvdMaxR: X and the
+// QR's reflection vectors, p x r each, the rotated R^H and the accumulated
+// rotations, r x r each, and r betas).  Complex data planar.
 constexpr int kSvdMaxR = 1024;
+inline int64_t svd_workspace(int64_t m, int64_t n) {
+  const int64_t p = m > n ? m : n, r = m > n ? n : m;
+  return 2 * p * r + 2 * r * r + r;
+}
 struct SvdTask {
   int64_t a_off, u_off, vt_off, s_off, w_off;
   int32_t m, n, lda, pad;
@@ -133,7 +149,13 @@ hipError_t launch_lz_normalize(const double* W, double* X, double* const* basis,
                                const int* step, int64_t n, int64_t vs, int nvec, hipStream_t st);
 hipError_t launch_lz_update(const double* Y, const double* X, double* const* basis, const double* kbuf,
                             const double* ebuf, int ld, const int* step, double* W, int64_t n, int64_t vs, int nvec,
-                            hipStream_t st);
+                            double* wpart, hipStream_t st);
+// lz_update with wpart writes nvec x lz_update_parts(n) partial sums of the new
+// W squared; launch_bdot_final reduces nparts partials per vector into
+// out[v * ld + *step] (sqrt when root), as launch_bdot's second pass
+int lz_update_parts(int64_t n);
+hipError_t launch_bdot_final(const double* partial, int nparts, int nvec, double* out, int ld, const int* step,
+                             int root, hipStream_t st);
 hipError_t launch_lz_step(int* step, hipStream_t st);
 // W_v -= sum_{i < nb} <Q_i(v), W_v> Q_i(v) with Q_i(v) = Q[i] + v * qvs,
 // nb = (cnt ? *cnt : 0) + add (at most nbmax); partial holds

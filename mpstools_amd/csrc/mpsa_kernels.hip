@@ -1798,9 +1798,16 @@ __global__ __launch_bounds__(256) void bdot_partial_kernel(const double* __restr
   const int v = blockIdx.y;
   const double* xv = x + (int64_t)v * vs;
   const double* yv = y + (int64_t)v * vs;
-  double a = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) a += yv[i] * xv[i];
-  sr[threadIdx.x] = a;
+  // two independent chains per thread (more loads in flight), fixed order
+  double a0 = 0.0, a1 = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  for (; i + stride < n; i += 2 * stride) {
+    a0 += yv[i] * xv[i];
+    a1 += yv[i + stride] * xv[i + stride];
+  }
+  if (i < n) a0 += yv[i] * xv[i];
+  sr[threadIdx.x] = a0 + a1;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) sr[threadIdx.x] += sr[threadIdx.x + w];
@@ -1833,20 +1840,36 @@ __global__ void lz_normalize_kernel(const double* __restrict__ W, double* __rest
   }
 }
 
-__global__ void lz_update_kernel(const double* __restrict__ Y, const double* __restrict__ X,
-                                 double* const* __restrict__ basis, const double* __restrict__ kbuf,
-                                 const double* __restrict__ ebuf, int ld, const int* __restrict__ step,
-                                 double* __restrict__ W, int64_t n, int64_t vs) {
+// W = Y - alpha X - beta basis[j-1]; with wpart, also the per-workgroup sums
+// of W^2 (the next step's norm, finished by bdot_final_kernel over gridDim.x
+// parts).  256 threads.
+__global__ __launch_bounds__(256) void lz_update_kernel(const double* __restrict__ Y, const double* __restrict__ X,
+                                                        double* const* __restrict__ basis,
+                                                        const double* __restrict__ kbuf,
+                                                        const double* __restrict__ ebuf, int ld,
+                                                        const int* __restrict__ step, double* __restrict__ W,
+                                                        int64_t n, int64_t vs, double* __restrict__ wpart) {
+  __shared__ double sr[256];
   const int v = blockIdx.y;
   const int j = *step;
   const double e = ebuf[(int64_t)v * ld + j], k = kbuf[(int64_t)v * ld + j];
   const int64_t o = (int64_t)v * vs;
   const double* A0 = j > 0 ? basis[j - 1] + o : nullptr;
+  double a = 0.0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
     double t = -e * X[o + i] + Y[o + i];
     if (A0) t = t + -k * A0[i];
     W[o + i] = t;
+    a += t * t;
   }
+  if (!wpart) return;  // uniform across the workgroup
+  sr[threadIdx.x] = a;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sr[threadIdx.x] += sr[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) wpart[(int64_t)v * gridDim.x + blockIdx.x] = sr[0];
 }
 
 // Projection of the recurrence vector against a set of orthonormal vectors
@@ -2303,8 +2326,16 @@ hipError_t launch_lz_normalize(const double* W, double* X, double* const* basis,
 
 hipError_t launch_lz_update(const double* Y, const double* X, double* const* basis, const double* kbuf,
                             const double* ebuf, int ld, const int* step, double* W, int64_t n, int64_t vs, int nvec,
-                            hipStream_t st) {
-  lz_update_kernel<<<dim3(vec_grid(n), nvec), 256, 0, st>>>(Y, X, basis, kbuf, ebuf, ld, step, W, n, vs);
+                            double* wpart, hipStream_t st) {
+  lz_update_kernel<<<dim3(vec_grid(n), nvec), 256, 0, st>>>(Y, X, basis, kbuf, ebuf, ld, step, W, n, vs, wpart);
+  return hipGetLastError();
+}
+
+int lz_update_parts(int64_t n) { return vec_grid(n); }
+
+hipError_t launch_bdot_final(const double* partial, int nparts, int nvec, double* out, int ld, const int* step,
+                             int root, hipStream_t st) {
+  bdot_final_kernel<<<nvec, 64, 0, st>>>(partial, nparts, out, ld, step, root);
   return hipGetLastError();
 }
 
@@ -2342,10 +2373,73 @@ hipError_t launch_copy2d(const Copy2D* tasks, int ntasks, int64_t max_elems, con
   return hipGetLastError();
 }
 
+// Library-factorisation plumbing (sectors handed to rocSOLVER): planar
+// complex -> interleaved, and a rows x cols window of a real or interleaved
+// complex matrix (ld lda) -> planar (ld ldo, imaginary plane at + plane),
+// keeping the whole window (mode 0), its upper (1: i <= j) or lower (2:
+// i >= j) trapezoid, zeros elsewhere.
+__global__ void pack_z_kernel(const double* __restrict__ re, const double* __restrict__ im, int64_t n,
+                              double* __restrict__ z) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    z[2 * i] = re[i];
+    z[2 * i + 1] = im[i];
+  }
+}
+__global__ void unpack_mat_kernel(const double* __restrict__ A, int cplx, int64_t lda, int64_t rows, int64_t cols,
+                                  int mode, double* __restrict__ out, int64_t ldo, int64_t plane) {
+  const int64_t n = rows * cols;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e % rows, j = e / rows;
+    const bool keep = mode == 0 || (mode == 1 && i <= j) || (mode == 2 && i >= j);
+    const int64_t src = i + j * lda;
+    out[i + j * ldo] = keep ? (cplx ? A[2 * src] : A[src]) : 0.0;
+    if (cplx) out[plane + i + j * ldo] = keep ? A[2 * src + 1] : 0.0;
+  }
+}
+
+hipError_t launch_pack_z(const double* re, const double* im, int64_t n, double* z, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  pack_z_kernel<<<vec_grid(n), 256, 0, st>>>(re, im, n, z);
+  return hipGetLastError();
+}
+
+hipError_t launch_unpack_mat(const double* A, int cplx, int64_t lda, int64_t rows, int64_t cols, int mode,
+                             double* out, int64_t ldo, int64_t plane, hipStream_t st) {
+  if (rows <= 0 || cols <= 0) return hipSuccess;
+  unpack_mat_kernel<<<vec_grid(rows * cols), 256, 0, st>>>(A, cplx, lda, rows, cols, mode, out, ldo, plane);
+  return hipGetLastError();
+}
+
 hipError_t launch_fixphase(double* U, int64_t m, int64_t ldu, double* Vt, int64_t k, int64_t n, int64_t ldv,
                            hipStream_t st) {
   if (k <= 0) return hipSuccess;
   fixphase_kernel<<<(unsigned)k, 256, 0, st>>>(U, m, ldu, Vt, n, ldv);
+  return hipGetLastError();
+}
+
+// Euclidean norm of every column of A (rows x cols, ld lda), one workgroup per
+// column: per-thread sums in a fixed stride, then a fixed-order tree.
+__global__ __launch_bounds__(256) void colnorm_kernel(const double* __restrict__ A, int64_t rows, int64_t lda,
+                                                      double* __restrict__ out) {
+  __shared__ double red[256];
+  const int64_t j = blockIdx.x;
+  double a = 0.0;
+  for (int64_t i = threadIdx.x; i < rows; i += 256) {
+    const double x = A[i + j * lda];
+    a += x * x;
+  }
+  red[threadIdx.x] = a;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[j] = sqrt(red[0]);
+}
+
+hipError_t launch_colnorms(const double* A, int64_t rows, int64_t cols, int64_t lda, double* out, hipStream_t st) {
+  if (cols <= 0) return hipSuccess;
+  colnorm_kernel<<<(unsigned)cols, 256, 0, st>>>(A, rows, lda, out);
   return hipGetLastError();
 }
 

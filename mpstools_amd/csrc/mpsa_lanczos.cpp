@@ -439,6 +439,9 @@ int mpsa_lanczos_batched_ex(mpsa_plan* plan, const double* dpsi0, int64_t stride
     if (use_graph) chk(mpsa_plan_set_timing(plan, 0));
     const int ld = itmax + 1;
     const int nparts = (int)std::max<int64_t>(1, std::min<int64_t>(256, (n + 8191) / 8192));
+    // the recurrence's own dot products: up to 1024 partial sums (4
+    // workgroups per CU on the large vectors)
+    const int dparts = (int)std::max<int64_t>(1, std::min<int64_t>(1024, (n + 8191) / 8192));
     auto dalloc = [&](size_t bytes) {
       void* p = nullptr;
       if (hipMalloc(&p, std::max<size_t>(bytes, 16)) != hipSuccess) throw LErr(MPSA_ENOMEM, "lanczos_batched alloc");
@@ -448,7 +451,7 @@ int mpsa_lanczos_batched_ex(mpsa_plan* plan, const double* dpsi0, int64_t stride
     int* dstep = (int*)dalloc(sizeof(int));
     double* kbuf = (double*)dalloc(sizeof(double) * (size_t)nvec * ld);
     double* ebuf = (double*)dalloc(sizeof(double) * (size_t)nvec * ld);
-    double* partial = (double*)dalloc(sizeof(double) * (size_t)nvec * nparts);
+    double* partial = (double*)dalloc(sizeof(double) * (size_t)nvec * std::max(nparts, dparts));
     double** table = (double**)dalloc(sizeof(double*) * (size_t)ld);
     const int zero = 0;
     hchk(hipMemcpy(dstep, &zero, sizeof(int), hipMemcpyHostToDevice), "step counter");
@@ -519,14 +522,26 @@ int mpsa_lanczos_batched_ex(mpsa_plan* plan, const double* dpsi0, int64_t stride
         hchk(launch_scale_rsqrt(Q, nrm + nvp, n, vs, nvp, cs), "state normalise");
       }
     }
+    // Without projections or reorthogonalisation nothing changes W between
+    // the recurrence update and the next step's norm, so the update sums
+    // |W|^2 as it writes W and the norm pass only adds the partial sums.
+    const bool fuse_norm = nproj == 0 && !reortho;
+    const int wparts = lz_update_parts(n);
+    double* wpart = fuse_norm ? (double*)dalloc(sizeof(double) * (size_t)nvec * wparts) : nullptr;
+    bool norm_ready = false;
     auto step_once = [&]() {
       if (nproj > 0)
         hchk(launch_project_out(W, ptab, qvs, nullptr, nq, nbmax, n, vs, nvec, ppart, nparts, pcoef, cs), "project");
-      hchk(launch_bdot(W, W, n, vs, nvec, partial, nparts, kbuf, ld, dstep, 1, cs), "norm");
+      if (norm_ready)  // |W|^2 summed by the previous step's recurrence update
+        hchk(launch_bdot_final(wpart, wparts, nvec, kbuf, ld, dstep, 1, cs), "norm");
+      else
+        hchk(launch_bdot(W, W, n, vs, nvec, partial, dparts, kbuf, ld, dstep, 1, cs), "norm");
       hchk(launch_lz_normalize(W, X, table, kbuf, ld, dstep, n, vs, nvec, cs), "normalise");
       chk(mpsa_heff_apply(plan, X, vs, Y, vs, nvec, (mpsa_stream)cs));
-      hchk(launch_bdot(Y, X, n, vs, nvec, partial, nparts, ebuf, ld, dstep, 0, cs), "alpha");
-      hchk(launch_lz_update(Y, X, table, kbuf, ebuf, ld, dstep, W, n, vs, nvec, cs), "recurrence");
+      hchk(launch_bdot(Y, X, n, vs, nvec, partial, dparts, ebuf, ld, dstep, 0, cs), "alpha");
+      hchk(launch_lz_update(Y, X, table, kbuf, ebuf, ld, dstep, W, n, vs, nvec, fuse_norm ? wpart : nullptr, cs),
+           "recurrence");
+      norm_ready = fuse_norm;
       // full reorthogonalisation against the basis 0..step, twice
       for (int pass = 0; reortho && pass < 2; ++pass)
         hchk(launch_project_out(W, table, vs, dstep, 1, nbmax, n, vs, nvec, ppart, nparts, pcoef, cs), "reorthogonalise");

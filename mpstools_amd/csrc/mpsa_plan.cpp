@@ -21,6 +21,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/mpstools_amd.h"
@@ -54,13 +55,23 @@ struct BKey {
   }
   bool operator==(const BKey& o) const { return q0 == o.q0 && q1 == o.q1 && p0 == o.p0 && p1 == o.p1; }
 };
+// hash for the lookup tables (their iteration order is never used)
+struct BKeyHash {
+  size_t operator()(const BKey& k) const {
+    uint64_t h = k.q0 * 0x9E3779B97F4A7C15ull;
+    h ^= (k.q1 + 0x632BE59BD9B4E019ull) * 0xC2B2AE3D27D4EB4Full;
+    h ^= ((uint64_t)k.p0 << 32 | k.p1) * 0x165667B19E3779F9ull;
+    return (size_t)(h ^ (h >> 29));
+  }
+};
+inline uint64_t pair_key(int a, int b) { return (uint64_t)(uint32_t)a << 32 | (uint32_t)b; }
 
 struct Tab {
   int64_t n = 0;
   int ns = 1;
   std::vector<BKey> key;
   std::vector<int64_t> rows, cols, off;
-  std::map<BKey, int> idx;
+  std::unordered_map<BKey, int, BKeyHash> idx;
   int64_t total = 0;  // extent of one plane: max(offset + rows * cols)
   int cplx = 0;       // complex128, planar: real parts, then imaginary parts at + total
   int find(const BKey& k) const {
@@ -81,12 +92,13 @@ static Tab parse(const mpsa_blocks* b, int ns_expect, const char* name) {
   t.n = b->n;
   t.ns = b->ns;
   t.cplx = b->is_complex;
+  t.idx.reserve((size_t)b->n);
+  t.key.reserve((size_t)b->n);
   for (int64_t i = 0; i < b->n; ++i) {
     BKey k{b->q[2 * i], b->q[2 * i + 1], b->p[b->ns * i], b->ns == 2 ? b->p[b->ns * i + 1] : 0u};
     if (b->rows[i] < 0 || b->cols[i] < 0 || b->offset[i] < 0 || b->rows[i] > (1 << 30) || b->cols[i] > (1 << 30))
       throw Error(MPSA_EINVAL, std::string(name) + ": bad block shape/offset");
-    if (t.idx.count(k)) throw Error(MPSA_EINVAL, std::string(name) + ": duplicate key");
-    t.idx[k] = (int)i;
+    if (!t.idx.emplace(k, (int)i).second) throw Error(MPSA_EINVAL, std::string(name) + ": duplicate key");
     t.key.push_back(k);
     t.rows.push_back(b->rows[i]);
     t.cols.push_back(b->cols[i]);
@@ -381,49 +393,47 @@ struct Built {
 static Built group_tuples(const std::vector<Tuple>& tup, const Tab* ytab, bool need_y,
                           std::map<BKey, std::pair<int64_t, int64_t>>* out_dims, bool plain = false) {
   Built b;
-  std::map<std::pair<int, int>, int> pidx;
-  std::map<std::pair<int, int>, int> sidx;  // (iout, iu) -> slot; iout = y index or key ordinal
-  std::map<BKey, int> outord;
+  // produced keys and their shapes (lookups hashed; the key list sorted)
+  std::unordered_map<BKey, std::pair<int64_t, int64_t>, BKeyHash> dims;
+  dims.reserve(tup.size() / 4 + 16);
   for (auto& t : tup) {
-    auto od = out_dims->find(t.out);
-    if (od == out_dims->end()) (*out_dims)[t.out] = {t.n_out_rows, t.n_out_cols};
-    else if (od->second.first != t.n_out_rows || od->second.second != t.n_out_cols)
+    auto ins = dims.emplace(t.out, std::make_pair(t.n_out_rows, t.n_out_cols));
+    if (!ins.second && (ins.first->second.first != t.n_out_rows || ins.first->second.second != t.n_out_cols))
       throw Error(MPSA_EINVAL, "inconsistent output block shapes for one key");
-    if (!outord.count(t.out)) outord[t.out] = 0;
   }
-  int ord = 0;
-  for (auto& kv : outord) kv.second = ord++;
-  for (auto& kv : outord) b.outkeys.push_back(kv.first);
+  b.outkeys.reserve(dims.size());
+  for (auto& kv : dims) b.outkeys.push_back(kv.first);
+  std::sort(b.outkeys.begin(), b.outkeys.end());
+  for (auto& k : b.outkeys) {
+    auto od = out_dims->find(k);
+    const auto& d = dims.at(k);
+    if (od == out_dims->end()) out_dims->emplace_hint(out_dims->end(), k, d);
+    else if (od->second != d) throw Error(MPSA_EINVAL, "inconsistent output block shapes for one key");
+  }
   if (!need_y) return b;
+  // products (ia, ib) and stage-2 slots (iout, iu) in first-seen order
+  std::unordered_map<uint64_t, int> pidx, sidx;
+  pidx.reserve(tup.size() / 2 + 16);
+  sidx.reserve(tup.size() / 2 + 16);
   int tix = 0;
   for (auto& t : tup) {
     int iout = ytab->find(t.out);
     if (iout < 0) throw Error(MPSA_ENOKEY, "output table lacks a produced key");
-    auto pk = plain ? std::make_pair(tix, -1) : std::make_pair(t.ia, t.ib);
-    int pi;
-    auto it = pidx.find(pk);
-    if (it == pidx.end()) {
-      pi = (int)b.prods.size();
-      pidx[pk] = pi;
-      b.prods.push_back(Product{t.ia, t.ib, t.m1, t.n1, t.k1, 0});
-    } else {
-      pi = it->second;
-    }
-    auto sk = plain ? std::make_pair(iout, tix) : std::make_pair(iout, t.iu);
+    const uint64_t pk = plain ? pair_key(tix, -1) : pair_key(t.ia, t.ib);
+    auto it = pidx.emplace(pk, (int)b.prods.size());
+    const int pi = it.first->second;
+    if (it.second) b.prods.push_back(Product{t.ia, t.ib, t.m1, t.n1, t.k1, 0});
+    const uint64_t sk = plain ? pair_key(iout, tix) : pair_key(iout, t.iu);
     ++tix;
-    int si;
-    auto jt = sidx.find(sk);
-    if (jt == sidx.end()) {
-      si = (int)b.slots.size();
-      sidx[sk] = si;
+    auto jt = sidx.emplace(sk, (int)b.slots.size());
+    const int si = jt.first->second;
+    if (jt.second) {
       T2Slot s;
       s.iout = iout;
       s.iu = t.iu;
       s.m = t.m1;
       s.n = t.n1;
       b.slots.push_back(s);
-    } else {
-      si = jt->second;
     }
     auto& terms = b.slots[(size_t)si].terms;
     bool found = false;
@@ -853,19 +863,20 @@ static void build_plan(mpsa_plan* P, Built& b, const Tab& Ptab, const Tab& Qtab,
   }
 
   // stage 2: one task per produced output block; segments in slot order
-  std::map<int, std::vector<int>> per_out;
-  for (int i = 0; i < (int)b.slots.size(); ++i) per_out[b.slots[(size_t)i].iout].push_back(i);
+  std::vector<std::vector<int>> per_out((size_t)Ytab.n);  // slots per output block, output blocks ascending
+  for (int i = 0; i < (int)b.slots.size(); ++i) per_out[(size_t)b.slots[(size_t)i].iout].push_back(i);
   std::vector<char> produced((size_t)Ytab.n, 0);
-  for (auto& kv : per_out) {
-    const int iy = kv.first;
+  for (int iy = 0; iy < (int)Ytab.n; ++iy) {
+    const std::vector<int>& kvs = per_out[(size_t)iy];
+    if (kvs.empty()) continue;
     produced[(size_t)iy] = 1;
     GTask t{};
     t.c_off = Ytab.off[(size_t)iy];
     t.m = (int32_t)Ytab.rows[(size_t)iy];
     t.n = (int32_t)Ytab.cols[(size_t)iy];
     t.seg0 = (int32_t)P->s2.segs.size();
-    t.nseg = (int32_t)kv.second.size();
-    for (int si : kv.second) {
+    t.nseg = (int32_t)kvs.size();
+    for (int si : kvs) {
       const T2Slot& sl = b.slots[(size_t)si];
       GSeg s{};
       int64_t toff;

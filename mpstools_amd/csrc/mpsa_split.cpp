@@ -12,23 +12,33 @@
 // reference's dgesvd, backward stable), the sign of every singular pair is
 // fixed by FixPhase, and the global truncation runs on the host over all
 // sectors' singular values exactly as the reference orders them.
-// Sectors with m >= n >= kPrecondMin are preconditioned: with V_e the
-// eigenvectors of the Gram matrix M^T M (rocSOLVER syevd), the columns of
-// B = M V_e are orthogonal up to rounding, so Jacobi on B stops after one or
-// two sweeps instead of tens; then M = U S (V_e V_B)^T.  The accuracy is then
-// that of gesvdj's stopping test, an absolute bound on the off-diagonal of
-// B^T B: it does not resolve singular values below about sqrt(eps) * sigma_max
-// once the large ones are converged.  Measured: a 128 x 64 sector with
-// singular values down to 3e-10 sigma_max, 6.8e-11 sigma_max error after one
-// sweep against 7e-15 for direct gesvdj (nine sweeps); a 4096 x 4096 sector,
-// backward error 1.6e-9 against direct gesvdj's 2.2e-9 (38 sweeps, 7.6x the
-// time).  So only sectors of at least kPrecondMin columns, where the direct
-// Jacobi SVD is itself tolerance-limited and slow, are preconditioned; wide
-// sectors (m < n) always take gesvdj directly.  The Jacobi pass on B
-// keeps the accuracy of a direct Jacobi SVD of M (the eigensolver only
-// supplies a rotation, whose rounding is part of B).  On a 4096 x 4096 sector
-// with singular values over ten decades: 1.3 s against 9.8 s, backward error
-// 1.6e-9 against 2.2e-9 relative, V orthonormal to 1.4e-13 against 4.8e-12.
+// Sectors with min(m, n) >= kPrecondMin (the dense D = 2048 bond's 4096 x 4096
+// sector) of a count-limited split (chimax below the number of singular
+// values) take the density-matrix eigendecomposition instead: with E the
+// eigenvectors (rocSOLVER syevd) of the Gram matrix of the side that stays a
+// bare isometry (M M^T when lambda goes right, M^T M when it goes left),
+// P = M^T E (or M E) holds sigma_e times the other side's singular vector in
+// column e, and sigma_e = |P[:, e]|.  The isometry is E itself (orthonormal to
+// syevd's rounding) and the lambda-carrying tensor is P (M = E P^T exactly, up
+// to E's orthogonality).  One level resolves sigma_e to SVD accuracy only
+// down to about sqrt(eps) sigma_max, since the Gram matrix squares them; so
+// the directions with eigenvalues below kDmTail^2 lambda_max are
+// diagonalised a second time from their own block Pc of P (Pc^T Pc has
+// entries of the tail's size, not lambda_max's): E_c <- E_c E2, Pc <- Pc E2.
+// What then limits the tail is its contamination by the large directions,
+// about eps sigma_max^2 / (kDmTail sigma_max) absolute (2e-12 sigma_max).
+// Directions outside M's range come out near eps sigma_max^2 / sigma_r
+// (sigma_r the smallest nonzero singular value), not an SVD's
+// eps sigma_max, so values below kDmZero = 1e-10 of the largest count as
+// zero: the rank holds while sigma_r > 2e-6 sigma_max, and a real singular
+// value below that level (weight under 1e-20) is dropped where the reference
+// would keep it.  An untruncated split, whose kept count is the number of
+// values above 1e-15, keeps gesvdj, which reveals the rank.  The route
+// replaced Jacobi (gesvdj) on B = M V_e after the same syevd, whose stopping
+// test bounds the off-diagonal of B^T B absolutely (backward error 1.6e-9
+// relative on the 4096 x 4096 sector) and which took 2.1 of the split's 2.9 s
+// there.
+
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
 #include <rocsolver/rocsolver.h>
@@ -36,7 +46,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -60,14 +72,19 @@ void rchk(rocblas_status s, const char* what) {
     throw SErr(MPSA_EHIP, std::string(what) + ": rocBLAS/rocSOLVER status " + std::to_string((int)s));
 }
 int64_t align64(int64_t x) { return (x + 63) / 64 * 64; }
-// smallest column count of a sector whose SVD is preconditioned (below it the
-// direct Jacobi SVD is cheap and converges to rounding level); the CPU
+// smallest min(m, n) of a sector split by the density-matrix route (below it
+// the direct Jacobi SVD is cheap and converges to rounding level); the CPU
 // emulation of the device layer builds with a smaller value so that its tests
 // reach this path at small sizes
 #ifndef MPSA_PRECOND_MIN
 #define MPSA_PRECOND_MIN 1024
 #endif
 constexpr int64_t kPrecondMin = MPSA_PRECOND_MIN;
+// the density-matrix route's zero level, relative to the sector's largest
+// singular value, and the level below which its second stage re-resolves
+// the singular values
+constexpr double kDmZero = 1e-10;
+constexpr double kDmTail = 1e-4;
 
 // one rocBLAS handle per device, created on first use
 rocblas_handle handle_for_current_device() {
@@ -107,6 +124,8 @@ struct Sector {
 };
 
 }  // namespace
+
+void* mpsa::rocblas_handle_current() { return (void*)handle_for_current_device(); }
 
 struct mpsa_split_result {
   std::vector<uint64_t> lq, rq;
@@ -213,24 +232,49 @@ int mpsa_split(const mpsa_blocks* psi, const double* dpsi, const uint64_t* itoke
     // SVD + FixPhase per sector (BoostSVD, lapack_routines.cpp:614-634)
     rocblas_handle h = handle_for_current_device();
     rchk(rocblas_set_stream(h, st), "rocblas_set_stream");
-    int64_t pre_nn = 0, pre_mn = 0, pre_n = 0;  // workspace of the preconditioned sectors
+    // Sectors with min(m, n) >= kPrecondMin take the density-matrix route:
+    // the Gram matrix of the side that stays a bare isometry (rho = M M^T
+    // when lambda goes right, G = M^T M when it goes left) is diagonalised by
+    // syevd, and P = M^T E (or M E) carries sigma_e times the other side's
+    // vector in its column e.  sigma_e = |P[:, e]| is the Rayleigh quotient of
+    // the exact Gram at the computed eigenvector, so its error is that of the
+    // product P (eps sigma_max absolute) plus a second-order term in the
+    // eigenvector's angle; the isometry is syevd's orthonormal E, and the
+    // lambda-carrying tensor is P itself (the division by sigma below is
+    // undone when lambda is multiplied back in).
+    // The route resolves singular values down to about sqrt(eps) sigma_max,
+    // not the rank of a rank-deficient sector, so it is taken only when the
+    // split is count-limited (chimax below the number of singular values):
+    // its kept set is then the top chimax values, while an untruncated split,
+    // whose kept count is the number of values above 1e-15, keeps gesvdj.
+    unsigned cm = chimax;
+    if (!cm) cm = do_trunc ? 100u : 200u;  // split :1257-1258
+    const bool count_limited = (int64_t)cm < tot_S;
+    auto dm_route = [&](const Sector& S) { return count_limited && std::min(S.m, S.n) >= kPrecondMin; };
+    int64_t big_ss = 0, big_mn = 0, big_s = 0;
+    const bool left_bare = direction > 0;
     for (auto& kv : secs) {
       const Sector& S = kv.second;
-      if (S.m >= S.n && S.n >= kPrecondMin) {
-        pre_nn = std::max(pre_nn, S.n * S.n);
-        pre_mn = std::max(pre_mn, S.m * S.n);
-        pre_n = std::max(pre_n, S.n);
+      if (dm_route(S)) {
+        const int64_t sd = left_bare ? S.m : S.n;
+        big_ss = std::max(big_ss, sd * sd);
+        big_mn = std::max(big_mn, S.m * S.n);
+        big_s = std::max(big_s, sd);
       }
     }
-    // Ge: V_e (in place of M^T M), Bw: B = M V_e, Vb: V_B^T, ev/ework:
-    // syevd's eigenvalues and workspace
-    DevArr<double> Ge, Bw, Vb, ev, ework;
-    if (pre_n) {
-      Ge.alloc((size_t)pre_nn);
-      Vb.alloc((size_t)pre_nn);
-      Bw.alloc((size_t)pre_mn);
-      ev.alloc((size_t)pre_n);
-      ework.alloc((size_t)pre_n);
+    // Ge: the Gram matrix, then its eigenvectors E; Pw: P; ev/ework: syevd's
+    // eigenvalues and workspace; nrm: the column norms of P
+    // Gc, T1, T2: the second level's Gram matrix and products
+    DevArr<double> Ge, Pw, ev, ework, nrm, Gc, T1, T2;
+    if (big_s) {
+      Ge.alloc((size_t)big_ss);
+      Pw.alloc((size_t)big_mn);
+      ev.alloc((size_t)big_s);
+      ework.alloc((size_t)big_s);
+      nrm.alloc((size_t)big_s);
+      Gc.alloc((size_t)big_ss);
+      T1.alloc((size_t)big_mn);
+      T2.alloc((size_t)big_ss);
     }
     auto check_info = [&](const char* what) {
       rocblas_int ih = 0;
@@ -239,32 +283,137 @@ int mpsa_split(const mpsa_blocks* psi, const double* dpsi, const uint64_t* itoke
       if (ih != 0) throw SErr(MPSA_EINVAL, std::string("split: ") + what + " did not converge (info " + std::to_string(ih) + ")");
     };
     const double one = 1.0, zero = 0.0;
+    std::vector<std::vector<double>> big_sig;    // host sources of the sorted sigmas, alive until the final sync
+    std::vector<std::vector<Copy2D>> big_moves;  // host sources of the gather tasks, likewise
+    std::vector<std::unique_ptr<DevArr<Copy2D>>> big_dmoves;
     for (auto& kv : secs) {
       Sector& S = kv.second;
       if (S.k == 0) continue;
       const rocblas_int m = (rocblas_int)S.m, n = (rocblas_int)S.n, k = (rocblas_int)S.k;
       double* Ms = M.p + S.off_M;
-      // abstol 0: converge to machine precision; 100 sweeps is far above what
-      // Jacobi needs (quadratic convergence), info reports a miss
-      if (S.m >= S.n && S.n >= kPrecondMin) {  // tall or square
-        rchk(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, n, n, m, &one, Ms, m, Ms, m, &zero,
-                           Ge.p, n),
-             "split: Gram");
-        rchk(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, n, Ge.p, n, ev.p, ework.p, info.p),
+      if (dm_route(S)) {
+        const rocblas_int sd = left_bare ? m : n;     // order of the Gram matrix
+        const int64_t prow = left_bare ? S.n : S.m;  // rows of P
+        if (left_bare)
+          rchk(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, m, m, n, &one, Ms, m, Ms, m,
+                             &zero, Ge.p, m),
+               "split: M M^T");
+        else
+          rchk(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, n, n, m, &one, Ms, m, Ms, m,
+                             &zero, Ge.p, n),
+               "split: M^T M");
+        rchk(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, sd, Ge.p, sd, ev.p, ework.p, info.p),
              "rocsolver_dsyevd");
         check_info("syevd");
-        rchk(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, m, n, n, &one, Ms, m, Ge.p, n, &zero,
-                           Bw.p, m),
-             "split: M V_e");
-        rchk(rocsolver_dgesvdj(h, rocblas_svect_singular, rocblas_svect_singular, m, n, Bw.p, m, 0.0, resid.p, 100,
-                               sweeps.p, Sg.p + S.off_S, U.p + S.off_U, m, Vb.p, n, info.p),
-             "rocsolver_dgesvdj");
-        check_info("gesvdj");
-        // V^T (k x n, k = n) = V_B^T V_e^T
-        rchk(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_transpose, n, n, n, &one, Vb.p, n, Ge.p, n,
-                           &zero, V.p + S.off_V, k),
-             "split: V^T");
+        if (left_bare)  // P = M^T E (n x m)
+          rchk(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, n, m, m, &one, Ms, m, Ge.p, m,
+                             &zero, Pw.p, n),
+               "split: M^T E");
+        else  // P = M E (m x n)
+          rchk(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, m, n, n, &one, Ms, m, Ge.p, n, &zero,
+                             Pw.p, m),
+               "split: M E");
+        // second level: the directions with eigenvalues below kDmTail^2
+        // lambda_max (the first sc columns, syevd's order is ascending) are
+        // diagonalised again from Pc = P[:, 0:sc] alone, whose Gram matrix
+        // Pc^T Pc has entries of their own size, not lambda_max's
+        std::vector<double> hev((size_t)sd);
+        hchk(hipMemcpyAsync(hev.data(), ev.p, sizeof(double) * (size_t)sd, hipMemcpyDeviceToHost, st),
+             "split: eigenvalues");
+        hchk(hipStreamSynchronize(st), "split: sync");
+        rocblas_int sc = 0;
+        while (sc < sd && hev[(size_t)sc] < kDmTail * kDmTail * hev[(size_t)sd - 1]) ++sc;
+        if (sc >= 2) {
+          rchk(rocblas_dgemm(h, rocblas_operation_transpose, rocblas_operation_none, sc, sc, (rocblas_int)prow, &one,
+                             Pw.p, (rocblas_int)prow, Pw.p, (rocblas_int)prow, &zero, Gc.p, sc),
+               "split: Pc^T Pc");
+          rchk(rocsolver_dsyevd(h, rocblas_evect_original, rocblas_fill_upper, sc, Gc.p, sc, ev.p, ework.p, info.p),
+               "rocsolver_dsyevd");
+          check_info("syevd");
+          rchk(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)prow, sc, sc, &one, Pw.p,
+                             (rocblas_int)prow, Gc.p, sc, &zero, T1.p, (rocblas_int)prow),
+               "split: Pc E2");
+          rchk(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, sd, sc, sc, &one, Ge.p, sd, Gc.p, sc,
+                             &zero, T2.p, sd),
+               "split: Ec E2");
+          hchk(hipMemcpyAsync(Pw.p, T1.p, sizeof(double) * (size_t)prow * sc, hipMemcpyDeviceToDevice, st),
+               "split: tail");
+          hchk(hipMemcpyAsync(Ge.p, T2.p, sizeof(double) * (size_t)sd * sc, hipMemcpyDeviceToDevice, st),
+               "split: tail");
+        }
+        hchk(launch_colnorms(Pw.p, prow, sd, prow, nrm.p, st), "split: sigma");
+        std::vector<double> hn((size_t)sd);
+        hchk(hipMemcpyAsync(hn.data(), nrm.p, sizeof(double) * (size_t)sd, hipMemcpyDeviceToHost, st), "split: sigma");
+        hchk(hipStreamSynchronize(st), "split: sync");
+        // rank: a direction outside the range of M gets |P[:, e]| of about
+        // eps sigma_max^2 / sigma_r (sigma_r its smallest nonzero singular
+        // value), not the eps sigma_max of an SVD, so values up to
+        // kDmZero sigma_max count as zero (they then fall under the 1e-15 cut)
+        double smax = 0.0;
+        for (double x : hn) smax = std::max(smax, x);
+        const double zero_below = kDmZero * smax;
+        for (double& x : hn)
+          if (x <= zero_below) x = 0.0;
+        // descending sigma; among equal values the larger eigenvalue index (syevd: ascending) first
+        std::vector<int> ord((size_t)sd);
+        for (int e = 0; e < sd; ++e) ord[(size_t)e] = sd - 1 - e;
+        std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return hn[(size_t)x] > hn[(size_t)y]; });
+        big_sig.emplace_back((size_t)k);
+        std::vector<double>& sg = big_sig.back();
+        std::vector<Copy2D> fromE, fromP;
+        for (int rk = 0; rk < k; ++rk) {
+          const int e = ord[(size_t)rk];
+          const double sig = hn[(size_t)e];
+          sg[(size_t)rk] = sig;
+          Copy2D ce{}, cp{};
+          ce.rs_off = ce.cs_off = cp.rs_off = cp.cs_off = -1;
+          ce.scale = 1.0;
+          cp.scale = sig > 0.0 ? 1.0 / sig : 0.0;
+          if (left_bare) {  // U[:, rk] = E[:, e]; V^T[rk, :] = P[:, e]^T / sigma
+            ce.src_off = (int64_t)e * m;
+            ce.dst_off = S.off_U + (int64_t)rk * m;
+            ce.rows = m;
+            ce.cols = 1;
+            ce.lds = ce.ldd = m;
+            cp.src_off = (int64_t)e * n;
+            cp.dst_off = S.off_V + rk;
+            cp.rows = 1;
+            cp.cols = n;
+            cp.lds = 1;
+            cp.ldd = k;
+          } else {  // U[:, rk] = P[:, e] / sigma; V^T[rk, :] = E[:, e]^T
+            cp.src_off = (int64_t)e * m;
+            cp.dst_off = S.off_U + (int64_t)rk * m;
+            cp.rows = m;
+            cp.cols = 1;
+            cp.lds = cp.ldd = m;
+            ce.src_off = (int64_t)e * n;
+            ce.dst_off = S.off_V + rk;
+            ce.rows = 1;
+            ce.cols = n;
+            ce.lds = 1;
+            ce.ldd = k;
+          }
+          fromE.push_back(ce);
+          fromP.push_back(cp);
+        }
+        hchk(hipMemcpyAsync(Sg.p + S.off_S, sg.data(), sizeof(double) * (size_t)k, hipMemcpyHostToDevice, st),
+             "split: sigma");
+        for (int src = 0; src < 2; ++src) {
+          big_moves.push_back(src == 0 ? fromE : fromP);
+          const std::vector<Copy2D>& mv = big_moves.back();
+          big_dmoves.emplace_back(new DevArr<Copy2D>());
+          big_dmoves.back()->alloc(mv.size());
+          hchk(hipMemcpyAsync(big_dmoves.back()->p, mv.data(), sizeof(Copy2D) * mv.size(), hipMemcpyHostToDevice, st),
+               "split: gather");
+          const bool to_u = (src == 0) == left_bare;
+          hchk(launch_copy2d(big_dmoves.back()->p, (int)mv.size(), to_u ? S.m : S.n, src == 0 ? Ge.p : Pw.p,
+                             to_u ? U.p : V.p, nullptr, st),
+               "split: gather");
+        }
       } else {
+        // abstol 0: converge to machine precision; 100 sweeps is far above what
+        // Jacobi needs (quadratic convergence), info reports a miss
         rchk(rocsolver_dgesvdj(h, rocblas_svect_singular, rocblas_svect_singular, m, n, Ms, m, 0.0, resid.p, 100,
                                sweeps.p, Sg.p + S.off_S, U.p + S.off_U, m, V.p + S.off_V, k, info.p),
              "rocsolver_dgesvdj");
@@ -287,8 +436,6 @@ int mpsa_split(const mpsa_blocks* psi, const double* dpsi, const uint64_t* itoke
                      [](const std::pair<double, uint64_t>& a, const std::pair<double, uint64_t>& b) {
                        return a.first < b.first;
                      });
-    unsigned cm = chimax;
-    if (!cm) cm = do_trunc ? 100u : 200u;  // split :1257-1258
     R = new mpsa_split_result();
     std::map<uint64_t, std::vector<double>> kept;
     double tw;

@@ -119,6 +119,9 @@ def lib():
         L.mpsa_cheb_stats.argtypes = [P, P, P]
         L.mpsa_cheb_profile.argtypes = [P, P]
         L.mpsa_cheb_moment_times.argtypes = [P, P, C.c_int, P]
+        L.mpsa_cheb_set_big_sector.argtypes = [P, C.c_int]
+        L.mpsa_cheb_factor_log.argtypes = [P, C.c_int64]
+        L.mpsa_cheb_factor_records.argtypes = [P, P, C.c_int64, P]
         L.mpsa_cheb_destroy.argtypes = [P]
         L.mpsa_dot.argtypes = [P, P, C.c_int64, C.c_int, P, P]
         L.mpsa_axpby.argtypes = [P, P, P, P, C.c_int64, C.c_int, P]
@@ -716,6 +719,27 @@ class Cheb:
         return dict(moments=self._cv(mu).reshape(self.nvec, nmom), recons=self._cv(rc).reshape(self.nvec, -1),
                     conv1=c1.reshape(self.nvec, nmom), conv2=c2.reshape(self.nvec, nmom), s=float(sec[0]),
                     moment_s=lap[:int(n[0])].copy())
+
+    def set_big_sector(self, k):
+        """Sectors with min(m, n) >= k go to rocSOLVER (default 1024)."""
+        _check(lib().mpsa_cheb_set_big_sector(self.h, int(k)))
+
+    def factor_log(self, cap):
+        """Record the sector shapes of the next cap canonicalisation launches
+        (0 stops and clears)."""
+        _check(lib().mpsa_cheb_factor_log(self.h, int(cap)))
+
+    def factor_records(self):
+        """Logged launches as dicts: kind ('svd', 'qr', 'svd_rocsolver',
+        'qr_rocsolver'), sectors, max_m, max_n, max_k, k64, k128, k256."""
+        n = np.zeros(1, np.int64)
+        _check(lib().mpsa_cheb_factor_records(self.h, None, 0, _ptr(n)))
+        out = np.zeros(8 * max(int(n[0]), 1), np.int64)
+        _check(lib().mpsa_cheb_factor_records(self.h, _ptr(out), int(n[0]), _ptr(n)))
+        kinds = ("svd", "qr", "svd_rocsolver", "qr_rocsolver")
+        names = ("sectors", "max_m", "max_n", "max_k", "k64", "k128", "k256")
+        return [dict(kind=kinds[int(r[0])], **{k: int(v) for k, v in zip(names, r[1:])})
+                for r in out[:8 * int(n[0])].reshape(-1, 8)]
 
     def site(self, slot, vec, site):
         nb, ln = np.zeros(1, np.int64), np.zeros(1, np.int64)

@@ -338,6 +338,172 @@ rocblas_status rocsolver_dgesvd(rocblas_handle, const rocblas_svect ls, const ro
   return rocblas_status_success;
 }
 
+// rocSOLVER QR / LQ / complex SVD by LAPACK (the engine's large-sector path);
+// every array is checked against its documented extent first
+template <typename F>
+static F lapack_fn(const char* name) {
+  void* h = lapack_lib();
+  return h ? (F)dlsym(h, name) : nullptr;
+}
+typedef void (*geqrf_fn)(const int*, const int*, void*, const int*, void*, void*, const int*, int*);
+typedef void (*orgqr_fn)(const int*, const int*, const int*, void*, const int*, const void*, void*, const int*, int*);
+static void qr_like(const char* what, const char* fname, int m, int n, void* A, int lda, void* tau, int esz,
+                    int ktau) {
+  submit(nullptr, what, [=]() {
+    if (m < 0 || n < 0 || lda < std::max(1, m)) {
+      report("qr: inconsistent arguments");
+      return;
+    }
+    bool ok = check(A, (size_t)esz * ((size_t)lda * (n - 1) + m), "qr A");
+    ok = check(tau, (size_t)esz * std::max(ktau, 1), "qr tau") && ok;
+    if (!ok || m == 0 || n == 0) return;
+    geqrf_fn f = lapack_fn<geqrf_fn>(fname);
+    if (!f) {
+      report("qr: no LAPACK library found");
+      return;
+    }
+    int M = m, N = n, LDA = lda, lw = -1, info = 0;
+    double wq[2] = {0, 0};
+    f(&M, &N, A, &LDA, tau, wq, &lw, &info);
+    lw = std::max(1, (int)wq[0]);
+    std::vector<double> work((size_t)2 * lw);
+    f(&M, &N, A, &LDA, tau, work.data(), &lw, &info);
+    if (info) report("qr: LAPACK info");
+  });
+}
+static void orgq_like(const char* what, const char* fname, int m, int n, int k, void* A, int lda, void* tau, int esz) {
+  submit(nullptr, what, [=]() {
+    if (m < 0 || n < 0 || k < 0 || lda < std::max(1, m)) {
+      report("orgq: inconsistent arguments");
+      return;
+    }
+    bool ok = check(A, (size_t)esz * ((size_t)lda * (n - 1) + m), "orgq A");
+    ok = check(tau, (size_t)esz * std::max(k, 1), "orgq tau") && ok;
+    if (!ok || m == 0 || n == 0) return;
+    orgqr_fn f = lapack_fn<orgqr_fn>(fname);
+    if (!f) {
+      report("orgq: no LAPACK library found");
+      return;
+    }
+    int M = m, N = n, K = k, LDA = lda, lw = -1, info = 0;
+    double wq[2] = {0, 0};
+    f(&M, &N, &K, A, &LDA, tau, wq, &lw, &info);
+    lw = std::max(1, (int)wq[0]);
+    std::vector<double> work((size_t)2 * lw);
+    f(&M, &N, &K, A, &LDA, tau, work.data(), &lw, &info);
+    if (info) report("orgq: LAPACK info");
+  });
+}
+rocblas_status rocsolver_dgeqrf(rocblas_handle, const rocblas_int m, const rocblas_int n, double* A,
+                                const rocblas_int lda, double* ipiv) {
+  qr_like("rocsolver_dgeqrf", "dgeqrf_", m, n, A, lda, ipiv, 8, std::min(m, n));
+  return rocblas_status_success;
+}
+rocblas_status rocsolver_zgeqrf(rocblas_handle, const rocblas_int m, const rocblas_int n, rocblas_double_complex* A,
+                                const rocblas_int lda, rocblas_double_complex* ipiv) {
+  qr_like("rocsolver_zgeqrf", "zgeqrf_", m, n, A, lda, ipiv, 16, std::min(m, n));
+  return rocblas_status_success;
+}
+rocblas_status rocsolver_dgelqf(rocblas_handle, const rocblas_int m, const rocblas_int n, double* A,
+                                const rocblas_int lda, double* ipiv) {
+  qr_like("rocsolver_dgelqf", "dgelqf_", m, n, A, lda, ipiv, 8, std::min(m, n));
+  return rocblas_status_success;
+}
+rocblas_status rocsolver_zgelqf(rocblas_handle, const rocblas_int m, const rocblas_int n, rocblas_double_complex* A,
+                                const rocblas_int lda, rocblas_double_complex* ipiv) {
+  qr_like("rocsolver_zgelqf", "zgelqf_", m, n, A, lda, ipiv, 16, std::min(m, n));
+  return rocblas_status_success;
+}
+rocblas_status rocsolver_dorgqr(rocblas_handle, const rocblas_int m, const rocblas_int n, const rocblas_int k,
+                                double* A, const rocblas_int lda, double* ipiv) {
+  orgq_like("rocsolver_dorgqr", "dorgqr_", m, n, k, A, lda, ipiv, 8);
+  return rocblas_status_success;
+}
+rocblas_status rocsolver_zungqr(rocblas_handle, const rocblas_int m, const rocblas_int n, const rocblas_int k,
+                                rocblas_double_complex* A, const rocblas_int lda, rocblas_double_complex* ipiv) {
+  orgq_like("rocsolver_zungqr", "zungqr_", m, n, k, A, lda, ipiv, 16);
+  return rocblas_status_success;
+}
+rocblas_status rocsolver_dorglq(rocblas_handle, const rocblas_int m, const rocblas_int n, const rocblas_int k,
+                                double* A, const rocblas_int lda, double* ipiv) {
+  orgq_like("rocsolver_dorglq", "dorglq_", m, n, k, A, lda, ipiv, 8);
+  return rocblas_status_success;
+}
+rocblas_status rocsolver_zunglq(rocblas_handle, const rocblas_int m, const rocblas_int n, const rocblas_int k,
+                                rocblas_double_complex* A, const rocblas_int lda, rocblas_double_complex* ipiv) {
+  orgq_like("rocsolver_zunglq", "zunglq_", m, n, k, A, lda, ipiv, 16);
+  return rocblas_status_success;
+}
+rocblas_status rocsolver_zgesvd(rocblas_handle, const rocblas_svect ls, const rocblas_svect rs, const rocblas_int m,
+                                const rocblas_int n, rocblas_double_complex* A, const rocblas_int lda, double* Sg,
+                                rocblas_double_complex* U, const rocblas_int ldu, rocblas_double_complex* V,
+                                const rocblas_int ldv, double* E, const rocblas_workmode, rocblas_int* info) {
+  submit(nullptr, "rocsolver_zgesvd", [=]() {
+    const int k = std::min(m, n);
+    if (ls != rocblas_svect_singular || rs != rocblas_svect_singular || m < 0 || n < 0 || lda < std::max(1, m) ||
+        ldu < std::max(1, m) || ldv < std::max(1, k)) {
+      report("zgesvd: unsupported or inconsistent arguments");
+      return;
+    }
+    bool ok = check(A, 16 * ((size_t)lda * (n - 1) + m), "zgesvd A");
+    ok = chk_range(Sg, 0, k - 1, "zgesvd S") && ok;
+    ok = check(U, 16 * ((size_t)ldu * (k - 1) + m), "zgesvd U") && ok;
+    ok = check(V, 16 * ((size_t)ldv * (n - 1) + k), "zgesvd V") && ok;
+    if (k > 1 && E) ok = chk_range(E, 0, k - 2, "zgesvd E") && ok;
+    ok = check(info, sizeof(rocblas_int), "zgesvd info") && ok;
+    if (!ok || k == 0) return;
+    typedef void (*zgesvd_fn)(const char*, const char*, const int*, const int*, void*, const int*, double*, void*,
+                              const int*, void*, const int*, void*, const int*, double*, int*);
+    zgesvd_fn f = lapack_fn<zgesvd_fn>("zgesvd_");
+    if (!f) {
+      report("zgesvd: no LAPACK library found");
+      return;
+    }
+    int M = m, N = n, LDA = lda, LDU = ldu, LDV = ldv, lw = -1, inf = 0;
+    std::vector<double> rwork((size_t)5 * k + 8);
+    double wq[2] = {0, 0};
+    f("S", "S", &M, &N, A, &LDA, Sg, U, &LDU, V, &LDV, wq, &lw, rwork.data(), &inf);
+    lw = std::max(1, (int)wq[0]);
+    std::vector<double> work((size_t)2 * lw);
+    f("S", "S", &M, &N, A, &LDA, Sg, U, &LDU, V, &LDV, work.data(), &lw, rwork.data(), &inf);
+    *info = inf;
+  });
+  return rocblas_status_success;
+}
+namespace mpsa {
+hipError_t launch_pack_z(const double* re, const double* im, int64_t n, double* z, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  submit(st, "pack_z", [=]() {
+    if (!chk_range(re, 0, n - 1, "pack re") || !chk_range(im, 0, n - 1, "pack im") ||
+        !chk_range(z, 0, 2 * n - 1, "pack z"))
+      return;
+    for (int64_t i = 0; i < n; ++i) {
+      z[2 * i] = re[i];
+      z[2 * i + 1] = im[i];
+    }
+  });
+  return hipSuccess;
+}
+hipError_t launch_unpack_mat(const double* A, int cplx, int64_t lda, int64_t rows, int64_t cols, int mode,
+                             double* out, int64_t ldo, int64_t plane, hipStream_t st) {
+  if (rows <= 0 || cols <= 0) return hipSuccess;
+  submit(st, "unpack_mat", [=]() {
+    const int64_t last = (rows - 1) + (cols - 1) * lda;
+    if (!chk_range(A, 0, cplx ? 2 * last + 1 : last, "unpack A") ||
+        !chk_range(out, 0, (rows - 1) + (cols - 1) * ldo, "unpack out") ||
+        (cplx && !chk_range(out + plane, 0, (rows - 1) + (cols - 1) * ldo, "unpack out im")))
+      return;
+    for (int64_t j = 0; j < cols; ++j)
+      for (int64_t i = 0; i < rows; ++i) {
+        const bool keep = mode == 0 || (mode == 1 && i <= j) || (mode == 2 && i >= j);
+        const int64_t src = i + j * lda;
+        out[i + j * ldo] = keep ? (cplx ? A[2 * src] : A[src]) : 0.0;
+        if (cplx) out[plane + i + j * ldo] = keep ? A[2 * src + 1] : 0.0;
+      }
+  });
+  return hipSuccess;
+}
+}  // namespace mpsa
 // rocBLAS dgemm (host pointer mode), computed with plain loops
 rocblas_status rocblas_dgemm(rocblas_handle, rocblas_operation ta, rocblas_operation tb, rocblas_int m,
                              rocblas_int n, rocblas_int k, const double* alpha, const double* A, rocblas_int lda,
@@ -729,10 +895,30 @@ hipError_t launch_lz_normalize(const double* W, double* X, double* const* basis,
   });
   return hipSuccess;
 }
+int lz_update_parts(int64_t n) {
+  const int64_t blocks = (n + 255) / 256;
+  return (int)(blocks < 1024 ? (blocks > 0 ? blocks : 1) : 1024);
+}
+hipError_t launch_bdot_final(const double* partial, int nparts, int nvec, double* out, int ld, const int* step,
+                             int root, hipStream_t st) {
+  submit(st, "bdot final", [=]() {
+    if (!check(step, 4, "bdot step") || !chk_range(partial, 0, (int64_t)nvec * nparts - 1, "bdot partial")) return;
+    const int j = *step;
+    for (int v = 0; v < nvec; ++v) {
+      double a = 0;
+      for (int q = 0; q < nparts; ++q) a += partial[(int64_t)v * nparts + q];
+      if (!check(out + (int64_t)v * ld + j, 8, "bdot out")) return;
+      out[(int64_t)v * ld + j] = root ? std::sqrt(a) : a;
+    }
+  });
+  return hipSuccess;
+}
 hipError_t launch_lz_update(const double* Y, const double* X, double* const* basis, const double* kbuf,
                             const double* ebuf, int ld, const int* step, double* W, int64_t n, int64_t vs, int nvec,
-                            hipStream_t st) {
+                            double* wpart, hipStream_t st) {
+  const int parts = lz_update_parts(n);
   submit(st, "lz_update", [=]() {
+    if (wpart && !chk_range(wpart, 0, (int64_t)nvec * parts - 1, "lz wpart")) return;
     if (!check(step, 4, "lz step")) return;
     const int j = *step;
     if (j > 0 && !check(basis + j - 1, 8, "lz basis table")) return;
@@ -744,11 +930,15 @@ hipError_t launch_lz_update(const double* Y, const double* X, double* const* bas
       if (!chk_range(Y + o, 0, n - 1, "lz Y") || !chk_range(X + o, 0, n - 1, "lz X") ||
           !chk_range(W + o, 0, n - 1, "lz W") || (A0 && !chk_range(A0, 0, n - 1, "lz A0")))
         return;
+      double a = 0.0;
       for (int64_t i = 0; i < n; ++i) {
         double t = -e * X[o + i] + Y[o + i];
         if (A0) t = t + -k * A0[i];
         W[o + i] = t;
+        a += t * t;
       }
+      if (wpart)  // the sum in the first part, zeros in the others
+        for (int q = 0; q < parts; ++q) wpart[(int64_t)v * parts + q] = q == 0 ? a : 0.0;
     }
   });
   return hipSuccess;
@@ -870,6 +1060,19 @@ hipError_t launch_fixphase(double* U, int64_t m, int64_t ldu, double* Vt, int64_
         if (!check(U + r + i * ldu, 8, "fixphase U")) return;
         U[r + i * ldu] *= phi;
       }
+    }
+  });
+  return hipSuccess;
+}
+hipError_t launch_colnorms(const double* A, int64_t rows, int64_t cols, int64_t lda, double* out, hipStream_t st) {
+  if (cols <= 0) return hipSuccess;
+  submit(st, "colnorms", [=]() {
+    if (!chk_range(A, 0, lda * (cols - 1) + rows - 1, "colnorms A") || !chk_range(out, 0, cols - 1, "colnorms out"))
+      return;
+    for (int64_t j = 0; j < cols; ++j) {
+      double a = 0.0;
+      for (int64_t i = 0; i < rows; ++i) a += A[i + j * lda] * A[i + j * lda];
+      out[j] = std::sqrt(a);
     }
   });
   return hipSuccess;
@@ -997,7 +1200,7 @@ hipError_t launch_svd_batched(int cplx, const SvdTask* tasks, int ntasks, const 
         ok = chk_range(M, t.a_off + q * m_plane, t.a_off + q * m_plane + (int64_t)t.lda * (n - 1) + m - 1, "svd M") && ok;
         ok = chk_range(U, t.u_off + q * u_plane, t.u_off + q * u_plane + (int64_t)m * k - 1, "svd U") && ok;
         ok = chk_range(Vt, t.vt_off + q * vt_plane, t.vt_off + q * vt_plane + (int64_t)k * n - 1, "svd Vt") && ok;
-        const int64_t wl = p >= 2 * k ? 2 * (int64_t)p * k + 2 * (int64_t)k * k + k : (int64_t)p * k + (int64_t)k * k;
+        const int64_t wl = svd_workspace(m, n);
         ok = chk_range(W, t.w_off + q * w_plane, t.w_off + q * w_plane + wl - 1, "svd W") && ok;
       }
       ok = chk_range(S, t.s_off, t.s_off + k - 1, "svd S") && ok;

@@ -311,16 +311,23 @@ def test_emulated_complex_plans_match_oracle(lib, oracle, model, L, N, D, seed):
             del ep
 
 
-@pytest.mark.parametrize("model,L,N,D,seed,dense,c", [("xxz", 12, 0, 48, 503, True, None),
-                                                      ("tv", 16, 8, 80, 507, False, None),
-                                                      ("xxz", 16, 0, 64, 508, True, 5)])
-def test_emulated_split_matches_oracle(lib, oracle, model, L, N, D, seed, dense, c):
-    """The split's sector SVDs, the Gram-preconditioned Jacobi paths included
-    (built here with the threshold at 64 columns, m >= n >= 64: the dense
-    centre bond is one 96 x 96 sector; the dense bond at sites 5, 6 of L = 16
-    is one wide 64 x 128 sector, taken by gesvdj directly), against
-    the numpy restatement: singular values within 1e-12 relative (LAPACK on
-    both sides here), the lambda-carrying tensor within 1e-10, isometries."""
+@pytest.mark.parametrize("model,L,N,D,seed,dense,c,chimax", [("xxz", 12, 0, 48, 503, True, None, 10 ** 6),
+                                                             ("xxz", 12, 0, 48, 503, True, None, 40),
+                                                             ("xxz", 12, 0, 48, 503, True, None, 60),
+                                                             ("tv", 16, 8, 80, 507, False, None, 10 ** 6),
+                                                             ("xxz", 16, 0, 64, 508, True, 5, 10 ** 6),
+                                                             ("xxz", 16, 0, 64, 508, True, 5, 50)])
+def test_emulated_split_matches_oracle(lib, oracle, model, L, N, D, seed, dense, c, chimax):
+    """The split's sector SVDs against the numpy restatement, with the
+    density-matrix route built here for sectors with min(m, n) >= 64: the
+    dense centre bond is one 96 x 96 sector of rank 48, the dense bond at
+    sites 5, 6 of L = 16 one wide 64 x 128 sector.  Untruncated
+    (chimax = 10^6) they go to gesvdj, which reveals the rank; count-limited
+    (chimax = 40, 50, 60) to the eigendecomposition of M M^T, where
+    chimax = 60 is above the rank (48) and needs the route's rank guard.  Bars: singular
+    values within 1e-12 relative or 100 eps lambda_max (LAPACK on both sides
+    here), the lambda-carrying tensor within 1e-10, isometries within 1e-12,
+    the bare vectors weighted by lambda / lambda_max within 1e-10."""
     import pysplit
     P = problems.build(oracle, model, L, N, D, seed, c=c, dense=dense)
     c, itk, psi = P["c"], P["itk"], P["psi"]
@@ -330,7 +337,7 @@ def test_emulated_split_matches_oracle(lib, oracle, model, L, N, D, seed, dense,
     itl, itr = native._itk(itk[c]), native._itk(itk[c + 1])
     h = C.c_void_p()
     native._check(Lb.mpsa_split(C.byref(t.s), dpsi.p, native._ptr(itl), len(itl), native._ptr(itr), len(itr), 1,
-                                10 ** 6, 0, 0.0, 1, C.byref(h), C.c_void_p(0)))
+                                chimax, 0, 0.0, 1, C.byref(h), C.c_void_p(0)))
     tw, chi = np.zeros(1), np.zeros(1, np.int32)
     nl, nr, nlam, ll, rl = (np.zeros(1, np.int64) for _ in range(5))
     native._check(Lb.mpsa_split_info(h, native._ptr(tw), native._ptr(chi), native._ptr(nl), native._ptr(nr),
@@ -349,8 +356,9 @@ def test_emulated_split_matches_oracle(lib, oracle, model, L, N, D, seed, dense,
     lq, ls, vals = np.zeros(k, np.uint64), np.zeros(k, np.int64), np.zeros(max(1, int(chi[0])))
     native._check(Lb.mpsa_split_lambda(h, native._ptr(lq), native._ptr(ls), native._ptr(vals)))
     Lb.mpsa_split_destroy(h)
-    left, lam, right, twr, chir = pysplit.split(psi, itk[c], itk[c + 1], chimax=10 ** 6, direction=1)
+    left, lam, right, twr, chir = pysplit.split(psi, itk[c], itk[c + 1], chimax=chimax, direction=1)
     assert int(chi[0]) == chir
+    assert abs(float(tw[0]) - twr) <= 1e-12
     if dense:  # one sector of 2 m(l) x 2 m(r), both >= 64: a preconditioned path ran
         assert 2 * int(t.rows.max()) >= 64 and 2 * int(t.cols.max()) >= 64
     o = 0
@@ -500,9 +508,45 @@ def test_emulated_chebyshev_engine_matches_oracle(lib, oracle, model, L, nvec, c
     eng2 = native.Cheb(sts, mpo, itk, alloc=al, to_host=th)
     chebcase.check_moments(eng2, sts, mpo, itk, 6, chimax, 1e-10)
     eng3 = native.Cheb(sts, mpo, itk, alloc=al, to_host=th)
+    eng3.factor_log(100000)
     r = eng3.moments(6, 10000, 2)
     assert np.abs(r["moments"] - chebcase.dense_moments(sts, mpo, itk, 6)).max() <= 1e-10
+    # the sector-shape log: this build sends sectors with k >= 3 to the
+    # rocSOLVER path, the rest to the batched kernels
+    recs = eng3.factor_records()
+    assert recs and {r_["kind"] for r_ in recs} <= {"svd", "qr", "svd_rocsolver", "qr_rocsolver"}
+    for r_ in recs:
+        assert r_["sectors"] >= r_["k64"] >= r_["k128"] >= r_["k256"] >= 0
+        assert 1 <= r_["max_k"] <= min(r_["max_m"], r_["max_n"])
+        assert (r_["max_k"] >= 3) == r_["kind"].endswith("rocsolver")
+    eng3.factor_log(0)
+    assert eng3.factor_records() == []
     del eng, eng2, eng3
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_emulated_chebyshev_engine_lossless_invariants(lib, cplx):
+    """The invariants the GPU test checks on large sectors, on the CPU
+    emulation at small size (Hubbard L = 8, D = 24, 2 vectors): compress
+    with chimax above every bond returns the state (conv <= 1e-10, <c|t> = 1),
+    and compressSingleSite with chimax above w D gives the exact applyMPO
+    state (|c - H t|^2 within 1e-10 of |H t|^2)."""
+    import chebcase
+    sts, itk = chebcase.states("hubbard", 8, 2, 77, D=24, cplx=cplx)
+    mpo = models.rescale_mpo(models.hubbard_mpo(8, 1.0, 4.0), 4.0 * 7 + 4.0 * 8, 0.0)
+    al, th = _emu_alloc(lib)
+    e = native.Cheb(sts, mpo, itk, alloc=al, to_host=th)
+    n0 = e.overlap(0, 0).real
+    e.copy(1, 0)
+    conv = e.compress(1, 10 ** 4, 2, 1e-14)
+    assert np.abs(conv).max() <= 1e-10
+    assert np.abs(e.overlap(1, 0) / n0 - 1.0).max() <= 1e-10
+    e.apply_mpo(3, 0)
+    e.compress_single_site(2, 0, 10 ** 5, 2, 1e-14)
+    h2 = e.overlap(3, 3).real
+    dist = e.overlap(2, 2).real + h2 - 2.0 * e.overlap(2, 3).real
+    assert np.abs(dist / h2).max() <= 1e-10
+    del e
 
 
 def test_emulated_device_canonicalisation(lib, oracle):

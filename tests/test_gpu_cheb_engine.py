@@ -34,15 +34,25 @@ def _say(*a):
     print("[case]", *a, flush=True)
 
 
+def _engine(sts, mpo, itk, big, **kw):
+    e = native.Cheb(sts, mpo, itk, **kw)
+    if big:
+        e.set_big_sector(big)
+    return e
+
+
+@pytest.mark.parametrize("big", [0, 3])
 @pytest.mark.parametrize("model,L,nvec,chimax,cplx", [("hubbard", 4, 2, 24, False), ("xxz", 6, 3, 15, False),
                                                       ("hubbard", 4, 2, 24, True), ("xxz", 8, 2, 40, True)])
-def test_cheb_engine_steps_and_moments_match_oracle(gpu, oracle, model, L, nvec, chimax, cplx):
-    _say("cheb engine", model, L, nvec, chimax, cplx)
+def test_cheb_engine_steps_and_moments_match_oracle(gpu, oracle, model, L, nvec, chimax, cplx, big):
+    """big = 3 sends every sector of at least 3 columns to the rocSOLVER path
+    (gesvd / geqrf+ungqr / gelqf+unglq) that the engine uses for large sectors."""
+    _say("cheb engine", model, L, nvec, chimax, cplx, big)
     sts, itk = chebcase.states(model, L, nvec, 40 + L, cplx=cplx)
     mpo = chebcase.scaled_mpo(model, L)
-    chebcase.check_steps(native.Cheb(sts, mpo, itk), sts, mpo, itk, chimax, 1e-10)
-    chebcase.check_moments(native.Cheb(sts, mpo, itk), sts, mpo, itk, 6, chimax, 1e-10)
-    r = native.Cheb(sts, mpo, itk).moments(6, 10000, 2)
+    chebcase.check_steps(_engine(sts, mpo, itk, big), sts, mpo, itk, chimax, 1e-10)
+    chebcase.check_moments(_engine(sts, mpo, itk, big), sts, mpo, itk, 6, chimax, 1e-10)
+    r = _engine(sts, mpo, itk, big).moments(6, 10000, 2)
     assert np.abs(r["moments"] - chebcase.dense_moments(sts, mpo, itk, 6)).max() <= 1e-10
 
 
@@ -64,7 +74,11 @@ def test_cheb_engine_mid_size_properties(gpu):
     bound = 4.0 * (L - 1) + 4.0 * L
     mpo = models.rescale_mpo(raw, bound, 0.0)
     eng = native.Cheb(sts, mpo, itk)
+    eng.factor_log(100000)
     r = eng.moments(5, 96, 2)
+    recs = eng.factor_records()  # sector shapes: all in the batched kernels at the default threshold
+    assert recs and {x["kind"] for x in recs} <= {"svd", "qr"}
+    assert all(1 <= x["max_k"] <= min(x["max_m"], x["max_n"]) for x in recs)
     mu = r["moments"]
     assert np.all(np.abs(mu) <= np.abs(mu[:, :1]) * (1 + 1e-3))
     # mu_2 = 2 <t1|t1> - mu_0 is reconstructed exactly from the compressed t1
@@ -77,3 +91,37 @@ def test_cheb_engine_mid_size_properties(gpu):
             for k, b in st[s].items():
                 dims[k[1]] = b.shape[1]
             assert sum(dims.values()) <= 96
+
+
+@pytest.mark.parametrize("cplx", [True, False])
+def test_cheb_engine_large_sector_invariants(gpu, cplx):
+    """Sector matrices of up to a few hundred rows (Hubbard L = 24, D = 512,
+    U(1) x U(1), 2 device-generated start vectors) through the batched
+    Householder QR and the Jacobi SVD on R^H: the start vectors have unit
+    norm after the device orthonormalisation; compress with chimax above
+    every bond (canonical form by SVD and QR, then sweeps) returns the same
+    state (conv <= 1e-10, <c|t> = 1 within 1e-10); compressSingleSite with
+    chimax above w D gives the exact applyMPO state (|c - H t|^2 within 1e-10
+    of |H t|^2)."""
+    _say("cheb engine large sectors", cplx)
+    L, D = 24, 512
+    itk = [models.itokey_u1xu1()] * L
+    N = models.pack_charge(L // 2, L // 2)
+    mpo = models.rescale_mpo(models.hubbard_mpo(L, 1.0, 4.0), 4.0 * (L - 1) + 4.0 * L, 0.0)
+    e = native.ChebRandom(L, itk, N, D, 2, mpo, 424242, cplx=cplx)
+    e.factor_log(100000)
+    assert np.abs(e.overlap(0, 0) - 1.0).max() <= 1e-12
+    e.copy(1, 0)
+    conv = e.compress(1, 8 * D, 2, 1e-14)
+    assert np.abs(conv).max() <= 1e-10
+    assert np.abs(e.overlap(1, 0) - 1.0).max() <= 1e-10
+    assert np.abs(e.overlap(1, 1) - 1.0).max() <= 1e-10
+    # compressSingleSite(H, t) with chimax above w D against the exact applyMPO
+    e.apply_mpo(3, 0)
+    e.compress_single_site(2, 0, 8 * 6 * D, 2, 1e-14)
+    h2 = e.overlap(3, 3).real
+    dist = e.overlap(2, 2).real + h2 - 2.0 * e.overlap(2, 3).real
+    assert np.abs(dist / h2).max() <= 1e-10
+    recs = e.factor_records()
+    assert {x["kind"] for x in recs} == {"svd", "qr"}
+    assert max(x["max_k"] for x in recs) >= 32 and max(x["max_m"] for x in recs) >= 100

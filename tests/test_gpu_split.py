@@ -148,10 +148,10 @@ def test_split_of_a_ground_state_truncated(gpu, oracle, model, L, N, D, seed, de
 
 
 def test_split_preconditioned_sector_within_tolerance(gpu, oracle):
-    """A 1024 x 1024 sector (the dense D = 512 bond) through the
-    Gram-preconditioned Jacobi SVD.  Its accuracy is that of gesvdj's stopping
-    test, an absolute bound on the off-diagonal of B^T B, not rounding level, so
-    the bars are: lambda within 1e-9 lambda_max, the lambda-carrying tensor and
+    """A 1024 x 1024 sector (the dense D = 512 bond), untruncated, so gesvdj
+    decides the rank.  Its accuracy is that of gesvdj's stopping test, an
+    absolute bound on the off-diagonal of M^T M, not rounding level, so the
+    bars are: lambda within 1e-9 lambda_max, the lambda-carrying tensor and
     the lambda-weighted bare vectors within 1e-9, both isometries within 1e-12,
     and the rebuilt psi within 1e-9 of its norm."""
     torch = gpu
@@ -176,3 +176,44 @@ def test_split_preconditioned_sector_within_tolerance(gpu, oracle):
     num = sum(float(np.sum((rebuilt[k] - psi[k]) ** 2)) for k in psi.keys())
     den = sum(float(np.sum(psi[k] ** 2)) for k in psi.keys())
     assert np.sqrt(num / den) <= 1e-9  # psi is normalised, so lambda is not rescaled
+
+
+@pytest.mark.parametrize("chimax,kept", [(600, 512), (300, 300)])
+@pytest.mark.parametrize("direction", [1, 0])
+def test_split_density_route_count_limited(gpu, oracle, direction, chimax, kept):
+    """The same 1024 x 1024 sector (rank 512: smallest nonzero singular
+    value 3.2e-4, the rest LAPACK noise below 3e-17) split count-limited, so
+    it takes the density-matrix route: syevd of M M^T (lambda right) or
+    M^T M (lambda left) and P = M^T E or M E.  chimax = 600 needs the
+    route's zero level (1e-10 sigma_max: the directions outside the range come
+    out near 3e-14 there), chimax = 300 cuts inside the spectrum.  Bars: kept count equal, truncated
+    weight within 1e-10, lambda within 1e-9 lambda_max, the lambda-carrying
+    tensor and the lambda-weighted bare vectors within 1e-9, the bare
+    isometry within 1e-12, and the truncated state within 1e-9 of the
+    oracle's."""
+    torch = gpu
+    _say("split density route 1024 x 1024, chimax", chimax, "direction", direction)
+    P = problems.build(oracle, "xxz", 22, 0, 512, 509, dense=True)
+    c, itk, psi = P["c"], P["itk"], P["psi"]
+    t = native.Table.from_blocks(psi)
+    dpsi = torch.from_numpy(t.pack(psi)).cuda()
+    out = native.split(t, dpsi, itk[c], itk[c + 1], direction=direction, chimax=chimax)
+    left, lam, right, twr, chi = pysplit.split(psi, itk[c], itk[c + 1], chimax=chimax, direction=direction)
+    assert out["chi"] == chi == kept
+    assert abs(out["tw"] - twr) <= 1e-10
+    lmax = max(float(np.max(v)) for v in lam.values())
+    for qc in lam:
+        assert np.max(np.abs(np.asarray(out["lam"][qc]) - np.asarray(lam[qc]))) <= 1e-9 * lmax
+    if direction > 0:
+        _cmp_blocks(*out["right"], right, 1e-9)
+        _cmp_weighted(out["left"], left, out["lam"], lam, "left", 1e-9)
+        _check_isometry(out["left"], out["lam"], "left", 1e-12)
+    else:
+        _cmp_blocks(*out["left"], left, 1e-9)
+        _cmp_weighted(out["right"], right, out["lam"], lam, "right", 1e-9)
+        _check_isometry(out["right"], out["lam"], "right", 1e-12)
+    gl = out["left"][0].unpack(out["left"][1].cpu().numpy())
+    gr = out["right"][0].unpack(out["right"][1].cpu().numpy())
+    got, want = twosite(gl, gr), twosite(left, right)
+    assert got.keys() == want.keys()
+    assert max(np.max(np.abs(got[k] - want[k])) for k in want.keys()) <= 1e-9
