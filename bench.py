@@ -55,6 +55,24 @@ def time_apply(torch, plan, x, y, steps, warmup, nvec=1, stride=0):
     return ms, st
 
 
+def time_apply_stream(torch, plan, x, y, steps, warmup):
+    """Wall time per untimed single-vector apply on a stream of its own,
+    where the plan replays its captured graph of launches."""
+    s = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s):
+        for _ in range(warmup):
+            plan.apply(x, y)
+        s.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            plan.apply(x, y)
+        s.synchronize()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+    torch.cuda.synchronize()
+    return ms
+
+
 TILE = {0: "256x128 stream, 4 waves", 1: "64x64", 2: "128x128", 3: "256x256 pipelined, 4 waves", 4: "32x32"}
 KERNEL = {0: "gemm_bf16_stream_kernel", 1: "gemm_bf16_kernel", 2: "gemm_bf16_kernel", 3: "gemm_bf16_pipe_kernel",
           4: "gemm_bf16_kernel"}
@@ -169,8 +187,14 @@ def u1_bonds(torch, steps, warmup):
                 plan.autotune(nvec=nv, reps=3)
                 v = prob["psi"]
                 x = v.repeat(nv) if nv > 1 else v
-                ms, _ = time_apply(torch, plan, x, torch.zeros_like(x), steps, warmup, nv, v.numel() if nv > 1 else 0)
+                ms, st = time_apply(torch, plan, x, torch.zeros_like(x), steps, warmup, nv,
+                                    v.numel() if nv > 1 else 0)
                 row[f"{pr}_nvec{nv}"] = {"ms": round(ms, 4), "tflops": round(nv * info["flops_alg"] / ms / 1e9, 3)}
+                if nv == 1:
+                    # the same apply untimed on its own stream (graph replay), beside its event-timed stages
+                    row[f"{pr}_nvec1"]["ms_graph_replay"] = round(
+                        time_apply_stream(torch, plan, x, torch.zeros_like(x), steps, warmup), 4)
+                    row[f"{pr}_nvec1"]["stages_ms_events"] = round(float(sum(st)), 4)
             row["flops_alg"] = info["flops_alg"]
             row["flops_ref"] = info["flops_ref"]
         cb = cpu_baseline(prob, name, row["flops_alg"], row["flops_ref"])
@@ -217,7 +241,9 @@ def chebyshev(torch, args):
     mpo = models.rescale_mpo(models.hubbard_mpo(L, 1.0, 4.0), 4.0 * (L - 1) + 4.0 * L, 0.0)
     out = {"workload": f"Fermi-Hubbard t=1 U=4, L={L}, d=4, U(1)xU(1), MPO w={mpo[1].Dr}, D={D}, {nv} start "
                        f"vectors, complex128 states, chimax=D, 2 compression sweeps"}
-    for name, prec in (("complex128", native.F64), ("complex64_contraction", native.F32)):
+    legs = {"complex128": native.F64, "complex64_contraction": native.F32}
+    for name in args.cheb_legs.split(","):
+        prec = legs[name]
         t0 = time.perf_counter()
         e = native.ChebRandom(L, itk, N, D, nv, mpo, args.seed + 2, cplx=True, h1_precision=prec)
         torch.cuda.synchronize()
@@ -384,137 +410,6 @@ def _block_moves(src: "Table", dst: "Table"):
     return out
 
 
-def u1_bonds(torch, steps, warmup):
-    """The U(1) bonds of the other workloads beside the headline, each with
-    its GPU matvec (FP64 and BF16X3, tuned tiles) and the CPU baseline on the
-    same bond (oracle tuple loop: 1 warm-up, median of 5, all cores and one
-    thread):
-      xxz64  : XXZ L = 32, D = 64 -- one matvec and the 10-step Lanczos of
-               the plain-method workload, both timed on GPU and CPU;
-      tv512  : t-V L = 100, D = 512 -- also 64 vectors per batched matvec;
-      hub1024: Fermi-Hubbard L = 80, D = 1024 (U(1) x U(1)) -- the bond of
-               the Chebyshev workload, real FP64."""
-    from mpstools_amd import bench_problem, native
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import pyoracle
-    out = {}
-    for name, nvecs in (("xxz64", (1,)), ("tv512", (1, 64)), ("hub1024", (1, 16))):
-        model, L, N, D, seed = WORKLOADS[name]
-        t0 = time.perf_counter()
-        prob = bench_problem.u1_centre(model, L, N, D, seed, precisions=("f64", "bf16x3"))
-        row = {"psi_blocks": prob["psi_blocks"], "psi_block_dims": prob["psi_block_dims"]}
-        for pr in ("f64", "bf16x3"):
-            plan = prob["plan_" + pr]
-            info = plan.info()
-            for nv in nvecs:
-                plan.autotune(nvec=nv, reps=3)
-                v = prob["psi"]
-                x = v.repeat(nv) if nv > 1 else v
-                ms, _ = time_apply(torch, plan, x, torch.zeros_like(x), steps, warmup, nv, v.numel() if nv > 1 else 0)
-                row[f"{pr}_nvec{nv}"] = {"ms": round(ms, 4), "tflops": round(nv * info["flops_alg"] / ms / 1e9, 3)}
-            row["flops_alg"] = info["flops_alg"]
-            row["flops_ref"] = info["flops_ref"]
-        cb = cpu_baseline(prob, name, row["flops_alg"], row["flops_ref"])
-        row["cpu"] = {"ms_all_cores": round(cb["t_all"] * 1e3, 3), "ms_one_thread": round(cb["t_one"] * 1e3, 3),
-                      "cores": cb["cores"], "blas": cb["blas"], "kind": "port",
-                      "sample": "oracle tuple loop on the same bond, 1 warm-up then the median of 5"}
-        row["gpu_f64_vs_cpu_all_cores"] = round(cb["t_all"] * 1e3 / row["f64_nvec1"]["ms"], 1)
-        if name == "xxz64":
-            # the plain-method workload: one matvec plus a 10-step Lanczos
-            plan = prob["plan_f64"]
-            torch.cuda.synchronize()
-            t_gpu = _median_time(lambda: native.lanczos(plan, prob["psi"], 10, 1e-12, True, want_state=False))
-            Lb = prob["Lt"].unpack(prob["dL"].cpu().numpy())
-            Rb = prob["Rt"].unpack(prob["dR"].cpu().numpy())
-            psi = prob["vt"].unpack(prob["psi"].cpu().numpy())
-            c, mpo, itk = prob["c"], prob["mpo"], prob["itk"]
-            t_cpu = _median_time(lambda: pyoracle.lanczos_twosite(Lb, Rb, psi, mpo[c], mpo[c + 1], itk, itk, 10, 1e-12,
-                                                                  True, nthreads=cb["cores"]))
-            row["lanczos10"] = {"gpu_ms": round(t_gpu * 1e3, 3), "cpu_ms_all_cores": round(t_cpu * 1e3, 3)}
-        row["build_s"] = round(time.perf_counter() - t0, 1)
-        log(name, json.dumps(row))
-        out[name] = row
-        del prob
-        native.lib().mpsa_release_pools()
-    return out
-
-
-def lanczos_compare(torch, prob, mixed, args):
-    """Bond ground state, the reference's plain Lanczos (TSLanzcosEngine rules)
-    run to convergence (|dE| < 1e-10 between tests) in three ways:
-      fp64   : every matvec in FP64;
-      mixed  : matvecs in `mixed` precision, recurrence/tridiagonal solve in FP64;
-      refined: the mixed Ritz vector followed by a short FP64 Lanczos (at most
-               `refine_it` FP64 matvecs), the final FP64 refinement step.
-    |dE|/L is reported for mixed and refined against fp64."""
-    from mpstools_amd import native
-    L = prob["L"]
-    psi = prob["psi"]
-    it_max, delta = args.lanczos_itmax, 1e-10
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    g64 = native.lanczos(prob["plan_f64"], psi, it_max, delta, True)
-    t64 = time.perf_counter() - t0
-    log(f"lanczos f64: {g64['it']} it, E={g64['energies'][0]:.12f}, {t64:.2f}s")
-    t0 = time.perf_counter()
-    gm = native.lanczos(prob["plan_" + mixed], psi, it_max, delta, True)
-    tm = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    gr = native.lanczos(prob["plan_f64"], gm["state"], args.refine_it, delta, True, want_state=False)
-    tr = time.perf_counter() - t0
-    e64, emix, eref = g64["energies"][0], gm["energies"][0], gr["energies"][0]
-    log(f"lanczos {mixed}: {gm['it']} it E={emix:.12f} {tm:.2f}s; fp64 refinement {gr['it']} it "
-        f"E={eref:.12f} {tr:.2f}s")
-    native.lib().mpsa_release_pools()
-    return {"E_fp64": e64, "it_fp64": g64["it"], "s_fp64": round(t64, 4),
-            "E_mixed": emix, "it_mixed": gm["it"], "s_mixed": round(tm, 4),
-            "E_refined": eref, "it_refine_fp64": gr["it"], "s_refine": round(tr, 4),
-            "dE_per_site_mixed": abs(emix - e64) / L, "dE_per_site_refined": abs(eref - e64) / L,
-            "meets_1e-9": bool(abs(eref - e64) / L <= 1e-9),
-            "solve_speedup": round(t64 / (tm + tr), 3)}
-
-
-def sweep_estimate(torch, prob, lan, L, D):
-    """Cost of one full-D bond step of a two-site sweep (DMRGengine::doStep,
-    src/container.hpp:293-337) from its measured parts on the bench bond: the
-    mixed Lanczos with FP64 refinement (from lanczos_compare), the split of
-    the two-site tensor keeping chi = D (split, src/MPSfunctions.hpp:1142-1278:
-    one 2D x 2D FP64 SVD here), and the environment update with the new left
-    tensor (addContractionLayer, :442-493).  A sweep is 2 (L - 2) such steps;
-    the bonds within log2(D) sites of the ends are cheaper, so the product is
-    an upper estimate."""
-    from mpstools_amd import native
-    vt, itk, c, mpo = prob["vt"], prob["itk"], prob["c"], prob["mpo"]
-    x = prob["psi"]
-    native.split(vt, x, itk, itk, direction=1, chimax=D)  # warm-up (library set-up)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    out = native.split(vt, x, itk, itk, direction=1, chimax=D)
-    torch.cuda.synchronize()
-    t_split = time.perf_counter() - t0
-    At, dA = out["left"]
-    outt = native.env_outkeys(prob["Lt"], At, mpo[c], itk, 1)
-    ep = native.EnvPlan(prob["Lt"], At, mpo[c], itk, 1, outt)
-    dout = torch.zeros(max(outt.total, 1), dtype=torch.float64, device="cuda")
-    ep.apply(prob["dL"], dA, dout)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(3):
-        ep.apply(prob["dL"], dA, dout)
-    torch.cuda.synchronize()
-    t_env = (time.perf_counter() - t0) / 3
-    t_lan = lan["s_mixed"] + lan["s_refine"]
-    step = t_lan + t_split + t_env
-    log(f"bond step: lanczos {t_lan:.3f} s + split {t_split:.3f} s + env {t_env * 1e3:.1f} ms; "
-        f"sweep estimate {2 * (L - 2) * step:.1f} s")
-    return {"lanczos_s": round(t_lan, 4), "split_s": round(t_split, 4), "env_update_s": round(t_env, 5),
-            "bond_step_s": round(step, 4), "bond_steps": 2 * (L - 2), "sweep_s_estimate": round(2 * (L - 2) * step, 2),
-            "kept": out["chi"], "truncated_weight": out["tw"],
-            "note": "estimate from the measured parts of one full-D bond step (mixed Lanczos + FP64 "
-                    "refinement, split with chi = D, left environment update); the device sweep driver is "
-                    "switched off in this build"}
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -540,6 +435,8 @@ def main():
     ap.add_argument("--no-u1", action="store_true", help="skip the U(1) bonds of the other workloads")
     ap.add_argument("--cheb-D", type=int, default=1024, help="bond dimension of the Chebyshev workload (0 skips)")
     ap.add_argument("--cheb-nvec", type=int, default=16, help="start vectors of the Chebyshev workload")
+    ap.add_argument("--cheb-legs", default="complex128,complex64_contraction",
+                    help="Chebyshev legs to run: complex128 and/or complex64_contraction")
     ap.add_argument("--calib", action="store_true",
                     help="run a known-size axpby (2 GiB read, 1 GiB written) for HBM counter calibration")
     args = ap.parse_args()

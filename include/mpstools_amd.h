@@ -487,8 +487,9 @@ int mpsa_cheb_site(mpsa_cheb* c, int slot, int vec, int site, int64_t* n_blocks,
 /* Plans built so far and device bytes held by the engine's buffer pool. */
 int mpsa_cheb_stats(const mpsa_cheb* c, int64_t* plans_built, int64_t* pool_bytes);
 /* Host-side time of the engine since its creation: out5 = {seconds building
- * plans, seconds in output-key queries, seconds waiting on the stream, number
- * of waits, plans cached now}. */
+ * plans (with their output keys: the engine enumerates each structure once),
+ * seconds in separate output-key queries (none in this engine), seconds
+ * waiting on the stream, number of waits, plans cached now}. */
 int mpsa_cheb_profile(const mpsa_cheb* engine, double* out5);
 /* Wall seconds of each moment of the last mpsa_cheb_moments call (moment 0:
  * the overlap; 1: compressSingleSite; n >= 2: compressSingleSite, the sum

@@ -419,30 +419,21 @@ void env_layer(Eng& E, const HTab& Et, const double* dE, const HTab& At, const d
   if (pe) {
     out_t = pe->out;
   } else {
-    int64_t n = 0;
-    std::vector<uint64_t> q;
-    std::vector<uint32_t> p;
-    std::vector<int64_t> r, c;
+    // keys and plan in one pass; the output table takes this engine's layout
+    const int cplx_ = Et.cplx;
+    auto make_out = [&](int64_t n, const uint64_t* q, const uint32_t* p, const int64_t* r, const int64_t* c) {
+      std::vector<uint64_t> vq(q, q + 2 * n);
+      std::vector<uint32_t> vp(p, p + n);
+      std::vector<int64_t> vr(r, r + n), vc(c, c + n);
+      out_t = from_query(n, 1, vq, vp, vr, vc);
+      out_t.cplx = cplx_;
+      return out_t.view();
+    };
     {
-      ScopeTimer tk(E.t_keys);
-      chk(mpsa_env_outkeys(&be, &ba, &bp, &mw, itk.data(), (int)itk.size(), dir, 0, &n, nullptr, nullptr, nullptr,
-                           nullptr));
-      q.resize((size_t)2 * n + 2);
-      p.resize((size_t)n + 1);
-      r.resize((size_t)n + 1);
-      c.resize((size_t)n + 1);
-      if (n)
-        chk(mpsa_env_outkeys(&be, &ba, &bp, &mw, itk.data(), (int)itk.size(), dir, n, &n, q.data(), p.data(),
-                             r.data(), c.data()));
-    }
-    out_t = from_query(n, 1, q, p, r, c);
-    out_t.cplx = Et.cplx;
-    if (n) {
-      mpsa_blocks bo = out_t.view();
       ScopeTimer tp(E.t_plan);
-      chk(mpsa_env_plan(&pl, &be, &ba, &bp, &mw, itk.data(), (int)itk.size(), dir, &bo, MPSA_F64));
-      E.put_plan(key, pl, out_t);
+      chk(env_keys_plan(&pl, &be, &ba, &bp, &mw, itk.data(), (int)itk.size(), dir, make_out));
     }
+    if (pl) E.put_plan(key, pl, out_t);
   }
   out_b = Buf(E.pool, out_t.len(), E.st);
   if (out_t.n() && pl) chk(mpsa_env_apply(pl, dE, dA, dAp, out_b.p, (mpsa_stream)E.st));
@@ -460,30 +451,21 @@ void heff1(Eng& E, const HTab& Lt, const double* dL, const HTab& At, const doubl
   if (pe) {
     out_t = pe->out;
   } else {
-    int64_t n = 0;
-    std::vector<uint64_t> q;
-    std::vector<uint32_t> p;
-    std::vector<int64_t> r, c;
+    // keys and plan in one pass; the output table takes this engine's layout
+    const int cplx_ = At.cplx;
+    auto make_out = [&](int64_t n, const uint64_t* q, const uint32_t* p, const int64_t* r, const int64_t* c) {
+      std::vector<uint64_t> vq(q, q + 2 * n);
+      std::vector<uint32_t> vp(p, p + n);
+      std::vector<int64_t> vr(r, r + n), vc(c, c + n);
+      out_t = from_query(n, 1, vq, vp, vr, vc);
+      out_t.cplx = cplx_;
+      return out_t.view();
+    };
     {
-      ScopeTimer tk(E.t_keys);
-      chk(mpsa_heff1_outkeys(&bl, &br, &ba, &mw, itk.data(), (int)itk.size(), 0, &n, nullptr, nullptr, nullptr,
-                             nullptr));
-      q.resize((size_t)2 * n + 2);
-      p.resize((size_t)n + 1);
-      r.resize((size_t)n + 1);
-      c.resize((size_t)n + 1);
-      if (n)
-        chk(mpsa_heff1_outkeys(&bl, &br, &ba, &mw, itk.data(), (int)itk.size(), n, &n, q.data(), p.data(), r.data(),
-                               c.data()));
-    }
-    out_t = from_query(n, 1, q, p, r, c);
-    out_t.cplx = At.cplx;
-    if (n) {
-      mpsa_blocks by = out_t.view();
       ScopeTimer tp(E.t_plan);
-      chk(mpsa_heff1_plan(&pl, &bl, &br, &ba, &by, &mw, itk.data(), (int)itk.size(), E.h1prec));
-      E.put_plan(key, pl, out_t);
+      chk(heff1_keys_plan(&pl, &bl, &br, &ba, &mw, itk.data(), (int)itk.size(), E.h1prec, make_out));
     }
+    if (pl) E.put_plan(key, pl, out_t);
   }
   out_b = Buf(E.pool, out_t.len(), E.st);
   if (!out_t.n() || !pl) return;

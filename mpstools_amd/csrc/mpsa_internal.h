@@ -5,9 +5,12 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <limits>
+#include <functional>
 #include <vector>
 
 struct mpsa_plan;
+struct mpsa_blocks;
+struct mpsa_mpo;
 
 namespace mpsa {
 
@@ -92,6 +95,19 @@ int gemm_max_grid(int prec_in, int tile_class);
 void finalize_gemm_list(std::vector<GTask>& tasks, std::vector<GSeg>& segs, int prec, int* tiles, int* tile_class);
 // whether the plan's operands are complex128 (planar)
 bool plan_complex(const mpsa_plan* P);
+// One-pass planners (mpsa_plan.cpp) for callers that build the output table
+// themselves: the structure's produced keys, in key order with their shapes,
+// go to make_out, which returns the output table the plan is then built
+// against; one tuple enumeration serves both (the C ABI's *_outkeys + *_plan
+// enumerate twice).  No plan (*plan = nullptr) when nothing is produced.
+// Returns an MPSA_* code, with the message in mpsa_last_error.
+typedef std::function<mpsa_blocks(int64_t n, const uint64_t* q, const uint32_t* p, const int64_t* rows,
+                                  const int64_t* cols)>
+    OutTableFn;
+int env_keys_plan(mpsa_plan** plan, const mpsa_blocks* BL, const mpsa_blocks* A, const mpsa_blocks* Ap,
+                  const mpsa_mpo* W, const uint64_t* itk, int d, int direction, const OutTableFn& make_out);
+int heff1_keys_plan(mpsa_plan** plan, const mpsa_blocks* L, const mpsa_blocks* R, const mpsa_blocks* x,
+                    const mpsa_mpo* W, const uint64_t* itk, int d, int precision, const OutTableFn& make_out);
 // the library's rocBLAS handle of the current device (a rocblas_handle),
 // created on first use; callers set its stream before each use
 void* rocblas_handle_current();

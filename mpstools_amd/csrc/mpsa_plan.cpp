@@ -203,12 +203,25 @@ struct mpsa_plan {
   int timing = 0;
   int on_device = 0;
   int xy_same = 0;
+  // the launches of one apply replayed from a graph (H_eff plans): valid for
+  // the vectors, strides, vector count and buffers it was captured with
+  hipGraphExec_t gexec = nullptr;
+  const double* g_dx = nullptr;
+  double* g_dy = nullptr;
+  int64_t g_xs = 0, g_ys = 0;
+  int g_nvec = 0;
+  const void *g_ws = nullptr, *g_xc = nullptr;
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    gexec = nullptr;
+  }
   // timing: one set of 5 events per timed apply (up to kTimedApplies kept)
   static constexpr int kTimedApplies = 512;
   std::vector<std::array<hipEvent_t, 5>> evs;
   int nrec = 0;    // applies recorded since the last summary
   int cur = -1;    // event set of the last timed apply
   ~mpsa_plan() {
+    drop_graph();
     for (auto& set : evs)
       for (auto& e : set)
         if (e) (void)hipEventDestroy(e);
@@ -388,12 +401,10 @@ struct Built {
   double flops_ref = 0;
 };
 
-// plain = true keeps the reference's schedule: one product per tuple
-// (F_ref flops), no sharing of (P, Q) products and no mix.
-static Built group_tuples(const std::vector<Tuple>& tup, const Tab* ytab, bool need_y,
-                          std::map<BKey, std::pair<int64_t, int64_t>>* out_dims, bool plain = false) {
-  Built b;
-  // produced keys and their shapes (lookups hashed; the key list sorted)
+// The produced keys (sorted) and their shapes.
+static void produced_keys(const std::vector<Tuple>& tup, std::vector<BKey>& outkeys,
+                          std::map<BKey, std::pair<int64_t, int64_t>>* out_dims) {
+  // lookups hashed; the key list sorted
   std::unordered_map<BKey, std::pair<int64_t, int64_t>, BKeyHash> dims;
   dims.reserve(tup.size() / 4 + 16);
   for (auto& t : tup) {
@@ -401,17 +412,22 @@ static Built group_tuples(const std::vector<Tuple>& tup, const Tab* ytab, bool n
     if (!ins.second && (ins.first->second.first != t.n_out_rows || ins.first->second.second != t.n_out_cols))
       throw Error(MPSA_EINVAL, "inconsistent output block shapes for one key");
   }
-  b.outkeys.reserve(dims.size());
-  for (auto& kv : dims) b.outkeys.push_back(kv.first);
-  std::sort(b.outkeys.begin(), b.outkeys.end());
-  for (auto& k : b.outkeys) {
+  outkeys.clear();
+  outkeys.reserve(dims.size());
+  for (auto& kv : dims) outkeys.push_back(kv.first);
+  std::sort(outkeys.begin(), outkeys.end());
+  for (auto& k : outkeys) {
     auto od = out_dims->find(k);
     const auto& d = dims.at(k);
     if (od == out_dims->end()) out_dims->emplace_hint(out_dims->end(), k, d);
     else if (od->second != d) throw Error(MPSA_EINVAL, "inconsistent output block shapes for one key");
   }
-  if (!need_y) return b;
-  // products (ia, ib) and stage-2 slots (iout, iu) in first-seen order
+}
+
+// Products (ia, ib) and stage-2 slots (iout, iu) in first-seen order, against
+// the output table ytab.  plain = true keeps the reference's schedule: one
+// product per tuple (F_ref flops), no sharing of (P, Q) products and no mix.
+static void group_products(const std::vector<Tuple>& tup, const Tab* ytab, bool plain, Built& b) {
   std::unordered_map<uint64_t, int> pidx, sidx;
   pidx.reserve(tup.size() / 2 + 16);
   sidx.reserve(tup.size() / 2 + 16);
@@ -444,6 +460,13 @@ static Built group_tuples(const std::vector<Tuple>& tup, const Tab* ytab, bool n
       }
     if (!found) terms.push_back({pi, t.coef});
   }
+}
+
+static Built group_tuples(const std::vector<Tuple>& tup, const Tab* ytab, bool need_y,
+                          std::map<BKey, std::pair<int64_t, int64_t>>* out_dims, bool plain = false) {
+  Built b;
+  produced_keys(tup, b.outkeys, out_dims);
+  if (need_y) group_products(tup, ytab, plain, b);
   return b;
 }
 
@@ -986,6 +1009,8 @@ static void ensure_ws(mpsa_plan* P, int nvec, hipStream_t st) {
 
 bool plan_complex(const mpsa_plan* P) { return P && P->cplx; }
 
+
+
 }  // namespace mpsa
 
 // =================================================================== C ABI
@@ -995,6 +1020,7 @@ bool plan_complex(const mpsa_plan* P) { return P && P->cplx; }
 // the copy itself is queued on `st` ahead of every later launch.
 static void set_stage_class(mpsa_plan* P, Stage& S, int prec, int cls, hipStream_t st) {
   if (S.tile_class == cls) return;
+  P->drop_graph();  // the captured launches name the old class's kernel and grid
   if (P->on_device && S.d_tasks && !S.tasks.empty()) HIPCHK(hipStreamSynchronize(st));
   assign_tiles(S, prec, cls);
   if (P->on_device && S.d_tasks && !S.tasks.empty()) {
@@ -1107,6 +1133,7 @@ static int complex_tables(std::initializer_list<const Tab*> t) {
   return 1;
 }
 
+
 static int plan_heff(mpsa_plan** plan, const mpsa_blocks* L, const mpsa_blocks* R, const mpsa_blocks* x,
                      const mpsa_blocks* y, const std::vector<Tuple>& tup, double fref, int nsx, int precision) {
   (void)nsx;
@@ -1213,6 +1240,7 @@ int mpsa_plan_bind_env(mpsa_plan* P, const double* dL, const double* dR, mpsa_st
     if (!P || P->kind != 0) throw Error(MPSA_EINVAL, "bind_env: not an H_eff plan");
     if ((P->L_total > 0 && !dL) || (P->R_total > 0 && !dR)) throw Error(MPSA_EINVAL, "bind_env: null environment");
     hipStream_t st = (hipStream_t)stream;
+    P->drop_graph();  // the captured launches name the old environment buffers
     ensure_device(P);
     P->dL = dL;
     P->dR = dR;
@@ -1238,6 +1266,26 @@ int mpsa_plan_bind_env(mpsa_plan* P, const double* dL, const double* dR, mpsa_st
   })
 }
 
+// The launches of one H_eff apply: repack (reduced precision), stage 1, mix,
+// zeroing of unproduced output blocks, stage 2; ev[1..4] recorded between
+// them when timing.
+static void apply_launches(mpsa_plan* P, const double* dx, int64_t x_stride, double* dy, int64_t y_stride, int nvec,
+                           hipStream_t st, hipEvent_t* ev) {
+  const int64_t xcs = P->xc_per_vec * P->planes, wss = P->ws_per_vec * P->planes;
+  if (P->prec != F64)
+    HIPCHK(launch_repack(P->prec, P->d_rpx, (int)P->rpx.size(), P->rpx_chunks, nvec, dx, x_stride, P->xc, xcs, st,
+                         P->xc_per_vec));
+  if (ev) HIPCHK(hipEventRecord(ev[1], st));
+  launch_stage(P, 1, dx, x_stride, dy, y_stride, nvec, st);
+  if (ev) HIPCHK(hipEventRecord(ev[2], st));
+  HIPCHK(launch_mix(P->prec, P->d_mix, (int)P->mix.size(), P->d_terms, P->mix_chunks, nvec, P->ws, P->ws, wss, st,
+                    P->ws_per_vec));
+  HIPCHK(launch_zero_ranges(P->d_zero, (int)P->zero.size(), dy, y_stride, nvec, st));
+  if (ev) HIPCHK(hipEventRecord(ev[3], st));
+  launch_stage(P, 2, dx, x_stride, dy, y_stride, nvec, st);
+  if (ev) HIPCHK(hipEventRecord(ev[4], st));
+}
+
 int mpsa_heff_apply(mpsa_plan* P, const double* dx, int64_t x_stride, double* dy, int64_t y_stride, int nvec,
                     mpsa_stream stream) {
   MPSA_TRY({
@@ -1249,6 +1297,48 @@ int mpsa_heff_apply(mpsa_plan* P, const double* dx, int64_t x_stride, double* dy
       throw Error(MPSA_EINVAL, "apply: vector stride smaller than the table extent");
     hipStream_t st = (hipStream_t)stream;
     ensure_ws(P, nvec, st);
+    // Untimed applies on a stream of the caller's own run from a graph of
+    // their launches (captured on the first apply with these arguments), so
+    // that the dependent launches follow each other without host gaps.  A
+    // stream that is already being captured (a caller's graph) or the null
+    // stream takes the launches directly.
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (st) HIPCHK(hipStreamIsCapturing(st, &cap));
+    const bool graphable = !P->timing && st && cap == hipStreamCaptureStatusNone;
+    if (graphable && P->gexec && P->g_dx == dx && P->g_dy == dy && P->g_xs == x_stride && P->g_ys == y_stride &&
+        P->g_nvec == nvec && P->g_ws == P->ws && P->g_xc == P->xc) {
+      HIPCHK(hipGraphLaunch(P->gexec, st));
+      return MPSA_OK;
+    }
+    if (graphable) {
+      P->drop_graph();
+      HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      try {
+        apply_launches(P, dx, x_stride, dy, y_stride, nvec, st, nullptr);
+      } catch (...) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(st, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamEndCapture(st, &g));
+      const hipError_t ie = hipGraphInstantiate(&P->gexec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) {
+        P->gexec = nullptr;
+        throw Error(MPSA_EHIP, std::string("apply: graph instantiate: ") + hipGetErrorString(ie));
+      }
+      P->g_dx = dx;
+      P->g_dy = dy;
+      P->g_xs = x_stride;
+      P->g_ys = y_stride;
+      P->g_nvec = nvec;
+      P->g_ws = P->ws;
+      P->g_xc = P->xc;
+      HIPCHK(hipGraphLaunch(P->gexec, st));
+      return MPSA_OK;
+    }
     hipEvent_t* ev = nullptr;
     if (P->timing) {
       const int slot = std::min(P->nrec, mpsa_plan::kTimedApplies - 1);
@@ -1262,19 +1352,7 @@ int mpsa_heff_apply(mpsa_plan* P, const double* dx, int64_t x_stride, double* dy
       P->nrec = slot + 1;
       HIPCHK(hipEventRecord(ev[0], st));
     }
-    const int64_t xcs = P->xc_per_vec * P->planes, wss = P->ws_per_vec * P->planes;
-    if (P->prec != F64)
-      HIPCHK(launch_repack(P->prec, P->d_rpx, (int)P->rpx.size(), P->rpx_chunks, nvec, dx, x_stride, P->xc, xcs, st,
-                           P->xc_per_vec));
-    if (ev) HIPCHK(hipEventRecord(ev[1], st));
-    launch_stage(P, 1, dx, x_stride, dy, y_stride, nvec, st);
-    if (ev) HIPCHK(hipEventRecord(ev[2], st));
-    HIPCHK(launch_mix(P->prec, P->d_mix, (int)P->mix.size(), P->d_terms, P->mix_chunks, nvec, P->ws, P->ws, wss, st,
-                      P->ws_per_vec));
-    HIPCHK(launch_zero_ranges(P->d_zero, (int)P->zero.size(), dy, y_stride, nvec, st));
-    if (ev) HIPCHK(hipEventRecord(ev[3], st));
-    launch_stage(P, 2, dx, x_stride, dy, y_stride, nvec, st);
-    if (ev) HIPCHK(hipEventRecord(ev[4], st));
+    apply_launches(P, dx, x_stride, dy, y_stride, nvec, st, ev);
   })
 }
 
@@ -1510,3 +1588,100 @@ int mpsa_axpby(const double* a, const double* dx, const double* b, double* dy, i
 }
 
 }  // extern "C"
+
+namespace mpsa {
+
+// The produced keys of a structure handed to the caller, who returns the
+// output table (its own block layout) that the plan is then built against;
+// one tuple enumeration and one key pass for both (see mpsa_internal.h).
+static mpsa_blocks keys_to_table(const std::vector<BKey>& keys,
+                                 const std::map<BKey, std::pair<int64_t, int64_t>>& dims, const OutTableFn& make_out) {
+  const int64_t n = (int64_t)keys.size();
+  std::vector<uint64_t> q((size_t)2 * n + 2);
+  std::vector<uint32_t> p((size_t)n + 1);
+  std::vector<int64_t> r((size_t)n + 1), c((size_t)n + 1);
+  int64_t nout = 0;
+  write_keys(keys, dims, 1, n, &nout, q.data(), p.data(), r.data(), c.data());
+  return make_out(n, q.data(), p.data(), r.data(), c.data());
+}
+
+static void env_keys_plan_impl(mpsa_plan** plan, const mpsa_blocks* BL, const mpsa_blocks* A, const mpsa_blocks* Ap,
+                               const mpsa_mpo* W, const uint64_t* itk, int d, int direction,
+                               const OutTableFn& make_out) {
+  if (!plan) throw Error(MPSA_EINVAL, "null plan pointer");
+  *plan = nullptr;
+  check_itk(itk, d, "itokey");
+  Tab tb = parse(BL, 1, "BL"), ta = parse(A, 1, "A");
+  if (ta.n == 0) throw Error(MPSA_EINVAL, "Error in addContractionLayer: A is empty.");
+  Tab tap = Ap ? parse(Ap, 1, "Ap") : ta;
+  if (Ap && tap.n == 0) throw Error(MPSA_EINVAL, "Error in addContractionLayer: Ap is empty.");
+  auto g = parse_mpo(W, d, "W");
+  double fref;
+  auto tup = tuples_env(tb, ta, tap, g, itk, direction, &fref);
+  std::map<BKey, std::pair<int64_t, int64_t>> dims;
+  Built b;
+  produced_keys(tup, b.outkeys, &dims);
+  const mpsa_blocks out = keys_to_table(b.outkeys, dims, make_out);
+  if (b.outkeys.empty()) return;
+  Tab to = parse(&out, 1, "out");
+  std::unique_ptr<mpsa_plan> P(new mpsa_plan());
+  P->kind = 2;
+  P->prec = F64;
+  P->direction = direction;
+  P->cplx = complex_tables({&tb, &ta, &tap, &to});
+  P->L_total = tb.total;
+  P->x_total = ta.total;
+  P->R_total = tap.total;
+  P->y_total = to.total;
+  group_products(tup, &to, false, b);
+  build_plan(P.get(), b, tb, ta, tap, to, 2, direction);
+  P->flops_ref = P->cplx ? 4 * fref : fref;
+  *plan = P.release();
+}
+
+static void heff1_keys_plan_impl(mpsa_plan** plan, const mpsa_blocks* L, const mpsa_blocks* R,
+                                 const mpsa_blocks* x, const mpsa_mpo* W, const uint64_t* itk, int d, int precision,
+                                 const OutTableFn& make_out) {
+  if (!plan) throw Error(MPSA_EINVAL, "null plan pointer");
+  *plan = nullptr;
+  check_itk(itk, d, "itokey");
+  if (!x || x->ns != 1) throw Error(MPSA_EINVAL, "heff1: x must be a one-site table");
+  if (precision < 0 || precision > 3)
+    throw Error(MPSA_EINVAL, "precision must be MPSA_F64, MPSA_F32, MPSA_BF16 or MPSA_BF16X3");
+  Tab tl = parse(L, 1, "L"), tr = parse(R, 1, "R"), tx = parse(x, 1, "x");
+  auto g = parse_mpo(W, d, "W");
+  double fref;
+  auto tup = tuples_heff1(tl, tr, tx, g, itk, &fref);
+  std::map<BKey, std::pair<int64_t, int64_t>> dims;
+  Built b;
+  produced_keys(tup, b.outkeys, &dims);
+  const mpsa_blocks y = keys_to_table(b.outkeys, dims, make_out);
+  if (b.outkeys.empty()) return;
+  Tab ty = parse(&y, 1, "y");
+  std::unique_ptr<mpsa_plan> P(new mpsa_plan());
+  P->kind = 0;
+  P->prec = precision;
+  P->cplx = complex_tables({&tl, &tr, &tx, &ty});
+  P->L_total = tl.total;
+  P->R_total = tr.total;
+  P->x_total = tx.total;
+  P->y_total = ty.total;
+  group_products(tup, &ty, false, b);
+  build_plan(P.get(), b, tl, tx, tr, ty, 0, 1);
+  P->flops_ref = P->cplx ? 4 * fref : fref;
+  P->xy_same = (tx.n == ty.n && tx.ns == ty.ns && tx.rows == ty.rows && tx.cols == ty.cols && tx.off == ty.off &&
+                tx.key == ty.key) ? 1 : 0;
+  *plan = P.release();
+}
+
+int env_keys_plan(mpsa_plan** plan, const mpsa_blocks* BL, const mpsa_blocks* A, const mpsa_blocks* Ap,
+                  const mpsa_mpo* W, const uint64_t* itk, int d, int direction, const OutTableFn& make_out) {
+  MPSA_TRY(env_keys_plan_impl(plan, BL, A, Ap, W, itk, d, direction, make_out))
+}
+
+int heff1_keys_plan(mpsa_plan** plan, const mpsa_blocks* L, const mpsa_blocks* R, const mpsa_blocks* x,
+                    const mpsa_mpo* W, const uint64_t* itk, int d, int precision, const OutTableFn& make_out) {
+  MPSA_TRY(heff1_keys_plan_impl(plan, L, R, x, W, itk, d, precision, make_out))
+}
+
+}  // namespace mpsa

@@ -202,6 +202,10 @@ hipError_t hipStreamBeginCapture(hipStream_t s, hipStreamCaptureMode) {
   S(s)->cap.clear();
   return hipSuccess;
 }
+hipError_t hipStreamIsCapturing(hipStream_t s, hipStreamCaptureStatus* st) {
+  *st = S(s)->capturing ? hipStreamCaptureStatusActive : hipStreamCaptureStatusNone;
+  return hipSuccess;
+}
 hipError_t hipStreamEndCapture(hipStream_t s, hipGraph_t* g) {
   Graph* G = new Graph();
   G->ops = S(s)->cap;
