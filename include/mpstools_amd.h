@@ -500,6 +500,14 @@ int mpsa_cheb_moment_times(const mpsa_cheb* engine, double* out, int cap, int* n
  * call each; smaller ones by the batched kernels in one launch.  k in
  * [1, 1024]. */
 int mpsa_cheb_set_big_sector(mpsa_cheb* engine, int k);
+/* Sectors with r = min(m, n) >= r_min (default 96) and max(m, n) <= 1024
+ * are factorised by several workgroups each: the Householder reflections in
+ * panels of 8 columns, Q and the SVD's back-transform column by column, and
+ * (SVD steps, r <= 512) the blocked Jacobi sweeps, one launch per round of
+ * the block tournament and one host wait per sweep.  r_min = 0 keeps every
+ * sector in the one-workgroup kernels.  Same factorisation either way (no
+ * counterpart in the reference: a scheduling choice). */
+int mpsa_cheb_set_block_jacobi(mpsa_cheb* engine, int r_min);
 /* Sector-shape log of the canonicalisation launches (a profiling aid with no
  * counterpart in the reference): cap > 0 clears the log and records up to cap
  * launches from now on, cap = 0 clears it and stops.  Each record is 8 int64:

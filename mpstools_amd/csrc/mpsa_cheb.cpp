@@ -50,6 +50,9 @@
 using namespace mpsa;
 using namespace mpsa::host;
 
+#ifndef MPSA_BLOCK_JACOBI_MIN
+#define MPSA_BLOCK_JACOBI_MIN 96
+#endif
 #ifndef MPSA_BIG_SECTOR
 #define MPSA_BIG_SECTOR 1024
 #endif
@@ -209,6 +212,9 @@ struct mpsa_cheb {
   std::vector<DMps> slot;
   int64_t plans_built = 0;
   int64_t big_sector = MPSA_BIG_SECTOR;  // sectors with min(m, n) at least this go to rocSOLVER
+  // sectors with min(m, n) at least this (and at most kSvdBlockMaxR) take the
+  // blocked Jacobi SVD, several workgroups per sector; 0: never
+  int64_t block_min_r = MPSA_BLOCK_JACOBI_MIN;
   // host time (seconds): plan builds, output-key queries, waits on the stream
   double t_plan = 0.0, t_keys = 0.0, t_sync = 0.0;
   std::vector<double> moment_s;  // wall seconds of each moment of the last moment loop
@@ -577,6 +583,93 @@ void big_factor(Eng& E, bool qr, bool left, int64_t m, int64_t n, double* M, int
   hchk(launch_fill(Sv, k, 1.0, E.st), "cheb: unit S");
 }
 
+// Chunk table of launch_house_panels / launch_house_apply: sector s has its
+// workgroups at base[s] .. base[s + 1]; returns the largest min(m, n).
+int house_table(const std::vector<SvdTask>& ts, int layout, std::vector<int>& base) {
+  base.assign(ts.size() + 1, 0);
+  int kmax = 0;
+  for (size_t i = 0; i < ts.size(); ++i) {
+    base[i + 1] = base[i] + house_chunks(ts[i].m, ts[i].n, layout);
+    kmax = std::max(kmax, (int)std::min(ts[i].m, ts[i].n));
+  }
+  return kmax;
+}
+
+// The SVD of the large sectors in phases (mpsa_factor.hip): the load, the
+// reflections in panels over several workgroups per sector, the init, sweeps
+// of block-tournament rounds until a sweep rotates nothing (one host wait per
+// sweep), the back-transform over several workgroups, the outputs.
+void svd_blocked(Eng& E, bool cplx, const std::vector<SvdTask>& big, const double* M, int64_t m_plane, double* U,
+                 int64_t u_plane, double* Vt, int64_t vt_plane, double* S, double* W, int64_t w_plane) {
+  const int nb = (int)big.size();
+  std::vector<int> hbase;
+  const int kmax = house_table(big, 0, hbase);
+  Buf dt = E.upload(big), dh = E.upload(hbase);
+  Buf flags(E.pool, nb, E.st);
+  const SvdTask* t = (const SvdTask*)dt.p;
+  auto phase = [&](int ph, const char* what) {
+    hchk(launch_svd_phase(cplx, ph, t, nb, M, m_plane, U, u_plane, Vt, vt_plane, S, W, w_plane, E.st), what);
+  };
+  phase(kPhLoad, "cheb: sector SVD load");
+  hchk(launch_house_panels(cplx, 0, t, nb, (const int*)dh.p, hbase.back(), kmax, W, w_plane, E.st),
+       "cheb: sector SVD reflections");
+  phase(kPhInit, "cheb: sector SVD init");
+  // sweeps over the sectors still rotating: a sweep that rotates nothing in a
+  // sector leaves it as it is, so later sweeps would not rotate it either
+  std::vector<SvdTask> act = big;
+  std::vector<double> hf((size_t)nb);
+  for (int sweep = 0; sweep < 60 && !act.empty(); ++sweep) {
+    const int na = (int)act.size();
+    std::vector<int> base((size_t)na + 1, 0);
+    int rounds = 0, rmax = 1;
+    for (int i = 0; i < na; ++i) {
+      const SvdTask& a = act[(size_t)i];
+      base[(size_t)i + 1] = base[(size_t)i] + svd_block_pairs(a.m, a.n);
+      rounds = std::max(rounds, svd_block_rounds(a.m, a.n));
+      rmax = std::max(rmax, (int)std::min(a.m, a.n));
+    }
+    Buf da = E.upload(act), db = E.upload(base);
+    for (int round = 0; round < rounds; ++round)
+      hchk(launch_svd_block_round(cplx, (const SvdTask*)da.p, na, (const int*)db.p, base.back(), round, W, w_plane,
+                                  flags.p, rmax, E.st),
+           "cheb: sector SVD round");
+    hchk(hipMemcpyAsync(hf.data(), flags.p, sizeof(double) * (size_t)na, hipMemcpyDeviceToHost, E.st),
+         "cheb: sector SVD flags");
+    E.sync();
+    std::vector<SvdTask> next;
+    for (int i = 0; i < na; ++i)
+      if (hf[(size_t)i] != 0.0) next.push_back(act[(size_t)i]);
+    act.swap(next);
+    if (!act.empty()) hchk(launch_fill(flags.p, na, 0.0, E.st), "cheb: sector SVD flags");
+  }
+  phase(kPhBackInit, "cheb: sector SVD back-transform");
+  hchk(launch_house_apply(cplx, 0, t, nb, (const int*)dh.p, hbase.back(), W, w_plane, E.st),
+       "cheb: sector SVD back-transform");
+  phase(kPhOut, "cheb: sector SVD outputs");
+}
+
+// The QR (right = 0) or LQ (right = 1) of the large sectors the same way:
+// load, reflections in panels, R and Q := [I; 0], Q formed over several
+// workgroups per sector, outputs.
+void qr_blocked(Eng& E, bool cplx, int right, const std::vector<SvdTask>& big, const double* M, int64_t m_plane,
+                double* U, int64_t u_plane, double* Vt, int64_t vt_plane, double* S, double* W, int64_t w_plane) {
+  const int nb = (int)big.size();
+  std::vector<int> hbase;
+  const int kmax = house_table(big, 1 + right, hbase);
+  Buf dt = E.upload(big), dh = E.upload(hbase);
+  const SvdTask* t = (const SvdTask*)dt.p;
+  auto phase = [&](int ph, const char* what) {
+    hchk(launch_qr_phase(cplx, right, ph, t, nb, M, m_plane, U, u_plane, Vt, vt_plane, S, W, w_plane, E.st), what);
+  };
+  phase(kPhLoad, "cheb: sector QR load");
+  hchk(launch_house_panels(cplx, 1 + right, t, nb, (const int*)dh.p, hbase.back(), kmax, W, w_plane, E.st),
+       "cheb: sector QR reflections");
+  phase(kPhInit, "cheb: sector QR init");
+  hchk(launch_house_apply(cplx, 1 + right, t, nb, (const int*)dh.p, hbase.back(), W, w_plane, E.st),
+       "cheb: sector QR formation");
+  phase(kPhOut, "cheb: sector QR outputs");
+}
+
 // ------------------------------------------------------------ prepareSite
 // prepareSite(mps, site, "l" or "r", multiply = true) for every sector of the
 // site at once; lambdas kept on the host when asked (for a later truncate).
@@ -690,6 +783,22 @@ void prepare_site(Eng& E, DMps& M, int site, bool left, bool keep_lambda) {
       }
       if (rec[1] && (int64_t)E.flog.size() < E.flog_cap * Eng::kFlogWidth) E.flog.insert(E.flog.end(), rec, rec + 8);
     }
+  }
+  std::vector<SvdTask> blocked;  // large sectors: several workgroups each
+  if (E.block_min_r > 0) {
+    std::vector<SvdTask> rest;
+    for (const SvdTask& t : tasks) {
+      const int64_t r = std::min(t.m, t.n), p = std::max(t.m, t.n);
+      const bool ok = r >= E.block_min_r && p <= kVecRowsMax && (use_qr || r <= kSvdBlockMaxR);
+      (ok ? blocked : rest).push_back(t);
+    }
+    tasks.swap(rest);
+  }
+  if (!blocked.empty()) {
+    if (use_qr)
+      qr_blocked(E, pl == 2, left ? 0 : 1, blocked, Mb.p, tm, Ub.p, tu, Vb.p, tv, Sb.p, Wb.p, tw);
+    else
+      svd_blocked(E, pl == 2, blocked, Mb.p, tm, Ub.p, tu, Vb.p, tv, Sb.p, Wb.p, tw);
   }
   {
     Buf dt = E.upload(tasks);
@@ -1671,6 +1780,15 @@ int mpsa_cheb_moment_times(const mpsa_cheb* E, double* out, int cap, int* n) {
   }
   *n = (int)E->moment_s.size();
   for (int i = 0; i < cap && i < *n; ++i) out[i] = E->moment_s[(size_t)i];
+  return MPSA_OK;
+}
+
+int mpsa_cheb_set_block_jacobi(mpsa_cheb* E, int r) {
+  if (!E || r < 0) {
+    set_last_error("cheb_set_block_jacobi: r must be >= 0");
+    return MPSA_EINVAL;
+  }
+  E->block_min_r = r;
   return MPSA_OK;
 }
 

@@ -160,13 +160,14 @@ __device__ __forceinline__ void reflect_col_global(const Mat<C>& Y, int c, int f
 // registers and v_j in LDS when p - j <= kVecRows); v_j goes to rows j.. of
 // column j of Hv, beta_j to beta[j].  NT threads; red holds NT / 64 doubles,
 // hv 3, vl 2 kVecRows (LDS).
+// house_reflect_range does reflections jb .. je - 1 and applies each to the
+// columns after it up to c_end - 1 (the panel of the multi-workgroup path).
 template <bool C, int NT>
-__device__ void house_reflect(const Mat<C>& X, const Mat<C>& Hv, double* beta, int p, int r, double* red,
-                              double* hv, double* vl) {
+__device__ void house_reflect_range(const Mat<C>& X, const Mat<C>& Hv, double* beta, int p, int jb, int je, int c_end,
+                                    double* red, double* hv, double* vl) {
   constexpr int NW = NT / 64;
-  const int k = p < r ? p : r;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int j = 0; j < k; ++j) {
+  for (int j = jb; j < je; ++j) {
     double a = 0.0;
     for (int i = j + tid; i < p; i += NT) {
       const double x = X.r(i, j);
@@ -215,7 +216,7 @@ __device__ void house_reflect(const Mat<C>& X, const Mat<C>& Hv, double* beta, i
       }
     }
     __syncthreads();
-    for (int c = j + 1 + wave; c < r; c += NW) {
+    for (int c = j + 1 + wave; c < c_end; c += NW) {
       if (in_lds) reflect_col_regs<C>(X, c, j, p, vl, b, lane);
       else reflect_col_global<C>(X, c, j, p, Hv, j, b, lane);
     }
@@ -225,6 +226,12 @@ __device__ void house_reflect(const Mat<C>& X, const Mat<C>& Hv, double* beta, i
     }
     __syncthreads();
   }
+}
+
+template <bool C, int NT>
+__device__ void house_reflect(const Mat<C>& X, const Mat<C>& Hv, double* beta, int p, int r, double* red,
+                              double* hv, double* vl) {
+  house_reflect_range<C, NT>(X, Hv, beta, p, 0, p < r ? p : r, r, red, hv, vl);
 }
 
 // Y <- H_0 H_1 ... H_{k-1} Y for the ncols columns of Y (p rows), applying
@@ -262,7 +269,7 @@ __global__ __launch_bounds__(kThreads) void svd_jacobi_kernel(const SvdTask* __r
                                                               double* __restrict__ U, int64_t u_plane,
                                                               double* __restrict__ Vt, int64_t vt_plane,
                                                               double* __restrict__ S, double* __restrict__ W,
-                                                              int64_t w_plane, int max_sweeps) {
+                                                              int64_t w_plane, int max_sweeps, int phases) {
   __shared__ double sig[kSvdMaxR];
   __shared__ int rank[kSvdMaxR];
   __shared__ int rotated;
@@ -289,24 +296,33 @@ __global__ __launch_bounds__(kThreads) void svd_jacobi_kernel(const SvdTask* __r
   Mat<C> Hv{W + t.w_off + o_hv, W + t.w_off + o_hv + w_plane, p};
   Mat<C> A{W + t.w_off + o_a, W + t.w_off + o_a + w_plane, r};
   double* const beta = W + t.w_off + o_a + (int64_t)r * r;
+  // phases (kPh* bits; kPhAll: the whole SVD): the multi-workgroup path runs
+  // the load, then house_panel/update kernels for the reflections, the init,
+  // block rounds for the sweeps, the back-transform's init, house_apply_kernel,
+  // and the outputs, each phase reading what the previous left in the workspace
   // X = M or M^H
-  for (int64_t e = tid; e < (int64_t)p * r; e += kThreads) {
-    const int i = (int)(e % p), j = (int)(e / p);
-    const int64_t src = wide ? (j + (int64_t)i * t.lda) : (i + (int64_t)j * t.lda);
-    X.r(i, j) = M[t.a_off + src];
-    if (C) X.m(i, j) = wide ? -M[t.a_off + m_plane + src] : M[t.a_off + m_plane + src];
+  if (phases & kPhLoad) {
+    for (int64_t e = tid; e < (int64_t)p * r; e += kThreads) {
+      const int i = (int)(e % p), j = (int)(e / p);
+      const int64_t src = wide ? (j + (int64_t)i * t.lda) : (i + (int64_t)j * t.lda);
+      X.r(i, j) = M[t.a_off + src];
+      if (C) X.m(i, j) = wide ? -M[t.a_off + m_plane + src] : M[t.a_off + m_plane + src];
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  house_reflect<C, kThreads>(X, Hv, beta, p, r, red, hv, vl);
+  if (phases & kPhReflect) house_reflect<C, kThreads>(X, Hv, beta, p, r, red, hv, vl);
   // A = R^H (lower triangular), Wm = identity
-  for (int64_t e = tid; e < (int64_t)r * r; e += kThreads) {
-    const int i = (int)(e % r), j = (int)(e / r);
-    A.r(i, j) = i >= j ? X.r(j, i) : 0.0;
-    if (C) A.m(i, j) = i >= j ? -X.m(j, i) : 0.0;
-    Wm.r(i, j) = i == j ? 1.0 : 0.0;
-    if (C) Wm.m(i, j) = 0.0;
+  if (phases & kPhInit) {
+    for (int64_t e = tid; e < (int64_t)r * r; e += kThreads) {
+      const int i = (int)(e % r), j = (int)(e / r);
+      A.r(i, j) = i >= j ? X.r(j, i) : 0.0;
+      if (C) A.m(i, j) = i >= j ? -X.m(j, i) : 0.0;
+      Wm.r(i, j) = i == j ? 1.0 : 0.0;
+      if (C) Wm.m(i, j) = 0.0;
+    }
+    __syncthreads();
   }
-  __syncthreads();
+  if (phases & kPhSweep) {
   const double tol = sqrt((double)r) * DBL_EPSILON;
   const int rr = r + (r & 1);
   for (int sweep = 0; sweep < max_sweeps && r > 1; ++sweep) {
@@ -382,14 +398,18 @@ __global__ __launch_bounds__(kThreads) void svd_jacobi_kernel(const SvdTask* __r
     __syncthreads();
     if (!any) break;
   }
-  // X = Q [Wm; 0]: the accumulated rotations back through the reflections
-  for (int64_t e = tid; e < (int64_t)p * r; e += kThreads) {
-    const int i = (int)(e % p), j = (int)(e / p);
-    X.r(i, j) = i < r ? Wm.r(i, j) : 0.0;
-    if (C) X.m(i, j) = i < r ? Wm.m(i, j) : 0.0;
   }
-  __syncthreads();
-  house_apply<C, kThreads>(X, r, false, Hv, beta, p, r, vl);
+  // X = Q [Wm; 0]: the accumulated rotations back through the reflections
+  if (phases & kPhBackInit) {
+    for (int64_t e = tid; e < (int64_t)p * r; e += kThreads) {
+      const int i = (int)(e % p), j = (int)(e / p);
+      X.r(i, j) = i < r ? Wm.r(i, j) : 0.0;
+      if (C) X.m(i, j) = i < r ? Wm.m(i, j) : 0.0;
+    }
+    __syncthreads();
+  }
+  if (phases & kPhApply) house_apply<C, kThreads>(X, r, false, Hv, beta, p, r, vl);
+  if (!(phases & kPhOut)) return;
   // singular values (column norms of the rotated A) and their descending order
   for (int j = wave; j < r; j += kWaves) {
     double a = 0.0;
@@ -554,6 +574,177 @@ __global__ __launch_bounds__(kThreads) void svd_jacobi_kernel(const SvdTask* __r
 }
 
 // ---------------------------------------------------------------------------
+// Block Jacobi rounds for large sectors, several workgroups per sector.
+// After svd_jacobi_kernel's reduction (mode 1) the r columns of A = R^H are
+// split into blocks of kBj columns; one launch runs one round of the block
+// tournament over every large sector at once, one workgroup per block pair
+// (I, J): it loads the 2 kBj columns into LDS, runs one inner sweep of the
+// scalar kernel's rotations over all pairs of those columns (same rotation,
+// same threshold), writes them back and applies the accumulated 2 kBj x 2 kBj
+// rotation to the same columns of Wm.  Sectors have their block pairs at
+// pair_base[s] .. pair_base[s + 1] of the grid; rotated[s] is set when any
+// pair of the sector rotated (the host repeats sweeps until none does, then
+// svd_jacobi_kernel mode 2 finishes).  r <= kBjMaxR.
+constexpr int kBj = 8;
+constexpr int kBjMaxR = 512;
+
+template <bool C>
+__global__ __launch_bounds__(kThreads) void svd_block_round_kernel(const SvdTask* __restrict__ tasks, int ntasks,
+                                                                   const int* __restrict__ pair_base, int round,
+                                                                   double* __restrict__ W, int64_t w_plane,
+                                                                   double* __restrict__ rotated, int rmax) {
+  // the block pair's columns: 2 kBj x rmax per plane of dynamic LDS (rmax: the
+  // launch's largest r, so that small launches fit two workgroups per CU)
+  extern __shared__ double blk_lds[];
+  double* const Lr = blk_lds;
+  double* const Li = blk_lds + 2 * kBj * rmax;
+  __shared__ double Vr[2 * kBj * 2 * kBj], Vi[2 * kBj * 2 * kBj];
+  __shared__ int any;
+  // sector of this workgroup: pair_base is ascending, pair_base[0] = 0
+  int lo = 0, hi = ntasks - 1;
+  const int b = blockIdx.x;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pair_base[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  const int sidx = lo;
+  const SvdTask t = tasks[sidx];
+  const int q = b - pair_base[sidx];
+  const int m = t.m, n = t.n;
+  const int p = m < n ? n : m, r = m < n ? m : n;
+  const int nb = (r + kBj - 1) / kBj, nbb = nb + (nb & 1);
+  if (round >= nbb - 1 || q >= nbb / 2 || r > rmax) return;
+  int I = (round + q) % (nbb - 1);
+  int J = q == 0 ? nbb - 1 : (round + nbb - 1 - q) % (nbb - 1);
+  if (I > J) {
+    const int x = I;
+    I = J;
+    J = x;
+  }
+  // the bye of an odd block count; a sector of one block runs it alone
+  if (J >= nb && nb > 1) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int i0 = I * kBj, wI = min(kBj, r - i0), j0 = J * kBj, wJ = J < nb ? min(kBj, r - j0) : 0, w = wI + wJ;
+  auto colof = [&](int c) { return c < wI ? i0 + c : j0 + (c - wI); };
+  const int64_t o_wm = (int64_t)p * r, o_a = 2 * (int64_t)p * r + (int64_t)r * r;
+  Mat<C> Wm{W + t.w_off + o_wm, W + t.w_off + o_wm + w_plane, r};
+  Mat<C> A{W + t.w_off + o_a, W + t.w_off + o_a + w_plane, r};
+  // the block pair's columns of A into LDS (column c at c * r), V = identity
+  for (int e = tid; e < w * r; e += kThreads) {
+    const int c = e / r, row = e - c * r;
+    Lr[c * r + row] = A.r(row, colof(c));
+    if (C) Li[c * r + row] = A.m(row, colof(c));
+  }
+  for (int e = tid; e < w * w; e += kThreads) {
+    Vr[e] = (e % w) == (e / w) ? 1.0 : 0.0;
+    Vi[e] = 0.0;
+  }
+  if (tid == 0) any = 0;
+  __syncthreads();
+  const double tol = sqrt((double)r) * DBL_EPSILON;
+  const int ww = w + (w & 1);
+  for (int sub = 0; sub < ww - 1; ++sub) {
+    for (int pq = wave; pq < ww / 2; pq += kWaves) {
+      int i = (sub + pq) % (ww - 1);
+      int j = pq == 0 ? ww - 1 : (sub + ww - 1 - pq) % (ww - 1);
+      if (i > j) {
+        const int x = i;
+        i = j;
+        j = x;
+      }
+      if (j >= w) continue;
+      double a = 0.0, bb = 0.0, gr = 0.0, gi = 0.0;
+      for (int row = lane; row < r; row += 64) {
+        const double xir = Lr[i * r + row], xjr = Lr[j * r + row];
+        a += xir * xir;
+        bb += xjr * xjr;
+        gr += xir * xjr;
+        if (C) {
+          const double xii = Li[i * r + row], xji = Li[j * r + row];
+          a += xii * xii;
+          bb += xji * xji;
+          gr += xii * xji;
+          gi += xir * xji - xii * xjr;
+        }
+      }
+      a = wave_sum(a);
+      bb = wave_sum(bb);
+      gr = wave_sum(gr);
+      if (C) gi = wave_sum(gi);
+      const double g = C ? sqrt(gr * gr + gi * gi) : fabs(gr);
+      if (!(g > tol * sqrt(a * bb)) || a == 0.0 || bb == 0.0) continue;
+      const double er = gr / g, ei = C ? gi / g : 0.0;
+      const double zeta = (bb - a) / (2.0 * g);
+      const double tt = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+      const double c = 1.0 / sqrt(1.0 + tt * tt), sn = c * tt;
+      // a_i' = c a_i - s conj(e) a_j, a_j' = s e a_i + c a_j (as rotate_cols), on the columns and on V
+      for (int row = lane; row < r; row += 64) {
+        const double xir = Lr[i * r + row], xjr = Lr[j * r + row];
+        if (C) {
+          const double xii = Li[i * r + row], xji = Li[j * r + row];
+          const double cjr = er * xjr + ei * xji, cji = er * xji - ei * xjr;
+          const double eir = er * xir - ei * xii, eii = er * xii + ei * xir;
+          Lr[i * r + row] = c * xir - sn * cjr;
+          Li[i * r + row] = c * xii - sn * cji;
+          Lr[j * r + row] = sn * eir + c * xjr;
+          Li[j * r + row] = sn * eii + c * xji;
+        } else {
+          Lr[i * r + row] = c * xir - sn * er * xjr;
+          Lr[j * r + row] = sn * er * xir + c * xjr;
+        }
+      }
+      if (lane < w) {
+        const int row = lane;
+        const double vir = Vr[i * w + row], vjr = Vr[j * w + row];
+        if (C) {
+          const double vii = Vi[i * w + row], vji = Vi[j * w + row];
+          const double cjr = er * vjr + ei * vji, cji = er * vji - ei * vjr;
+          const double eir = er * vir - ei * vii, eii = er * vii + ei * vir;
+          Vr[i * w + row] = c * vir - sn * cjr;
+          Vi[i * w + row] = c * vii - sn * cji;
+          Vr[j * w + row] = sn * eir + c * vjr;
+          Vi[j * w + row] = sn * eii + c * vji;
+        } else {
+          Vr[i * w + row] = c * vir - sn * er * vjr;
+          Vr[j * w + row] = sn * er * vir + c * vjr;
+        }
+      }
+      if (lane == 0) any = 1;
+    }
+    __syncthreads();
+  }
+  if (!any) return;  // uniform: read after the last barrier; nothing changed
+  // columns back to A; Wm[:, cols] <- Wm[:, cols] V (V column c: entries c * w + row)
+  for (int e = tid; e < w * r; e += kThreads) {
+    const int c = e / r, row = e - c * r;
+    A.r(row, colof(c)) = Lr[c * r + row];
+    if (C) A.m(row, colof(c)) = Li[c * r + row];
+  }
+  for (int row = tid; row < r; row += kThreads) {
+    double orr[2 * kBj], oi[2 * kBj];
+#pragma unroll
+    for (int c = 0; c < 2 * kBj; ++c) {
+      orr[c] = c < w ? Wm.r(row, colof(c)) : 0.0;
+      oi[c] = (C && c < w) ? Wm.m(row, colof(c)) : 0.0;
+    }
+    for (int c = 0; c < w; ++c) {
+      double sr = 0.0, si = 0.0;
+#pragma unroll
+      for (int c2 = 0; c2 < 2 * kBj; ++c2) {
+        if (c2 >= w) break;
+        const double vr = Vr[c * w + c2], vi = Vi[c * w + c2];  // V[c2, c]
+        sr += orr[c2] * vr - (C ? oi[c2] * vi : 0.0);
+        if (C) si += orr[c2] * vi + oi[c2] * vr;
+      }
+      Wm.r(row, colof(c)) = sr;
+      if (C) Wm.m(row, colof(c)) = si;
+    }
+  }
+  if (tid == 0) rotated[sidx] = 1.0;
+}
+
+// ---------------------------------------------------------------------------
 // Batched Householder QR of the same sector matrices, for the gauge steps
 // that need an isometry but no Schmidt values: left (right = 0) M = Q R with
 // Q (m x k) orthonormal columns, R (k x n) upper trapezoidal; right
@@ -578,7 +769,7 @@ __global__ __launch_bounds__(kQrThreads) void qr_householder_kernel(const SvdTas
                                                                     double* __restrict__ U, int64_t u_plane,
                                                                     double* __restrict__ Vt, int64_t vt_plane,
                                                                     double* __restrict__ S, double* __restrict__ W,
-                                                                    int64_t w_plane) {
+                                                                    int64_t w_plane, int phases) {
   __shared__ double red[kQrWaves];
   __shared__ double hv[3];
   __shared__ double vl[2 * kVecRows];
@@ -591,19 +782,25 @@ __global__ __launch_bounds__(kQrThreads) void qr_householder_kernel(const SvdTas
   Mat<C> X{W + t.w_off, W + t.w_off + w_plane, p};
   Mat<C> Hv{W + t.w_off + (int64_t)p * r, W + t.w_off + (int64_t)p * r + w_plane, p};
   double* const beta = W + t.w_off + (int64_t)p * r + (int64_t)p * k;
-  for (int64_t e = tid; e < (int64_t)p * r; e += kQrThreads) {
-    const int i = (int)(e % p), j = (int)(e / p);
-    const int64_t src = right ? (j + (int64_t)i * t.lda) : (i + (int64_t)j * t.lda);
-    X.r(i, j) = M[t.a_off + src];
-    if (C) X.m(i, j) = right ? -M[t.a_off + m_plane + src] : M[t.a_off + m_plane + src];
+  // phases: kPhLoad, kPhReflect, kPhInit (R out, Q := [I; 0]), kPhApply, kPhOut
+  // (the multi-workgroup path runs the reflections and the apply in
+  // house_panel/update_kernel and house_apply_kernel between them)
+  if (phases & kPhLoad) {
+    for (int64_t e = tid; e < (int64_t)p * r; e += kQrThreads) {
+      const int i = (int)(e % p), j = (int)(e / p);
+      const int64_t src = right ? (j + (int64_t)i * t.lda) : (i + (int64_t)j * t.lda);
+      X.r(i, j) = M[t.a_off + src];
+      if (C) X.m(i, j) = right ? -M[t.a_off + m_plane + src] : M[t.a_off + m_plane + src];
+    }
+    __syncthreads();
   }
-  __syncthreads();
-  house_reflect<C, kQrThreads>(X, Hv, beta, p, r, red, hv, vl);
+  if (phases & kPhReflect) house_reflect<C, kQrThreads>(X, Hv, beta, p, r, red, hv, vl);
   // R (k x r) = the upper trapezoid of X; written before X is reused for Q
   double* Ur = U + t.u_off;
   double* Ui = U + t.u_off + u_plane;
   double* Vr = Vt + t.vt_off;
   double* Vi = Vt + t.vt_off + vt_plane;
+  if (phases & kPhInit) {
   for (int64_t e = tid; e < (int64_t)k * r; e += kQrThreads) {
     const int i = (int)(e % k), c = (int)(e / k);
     const double rr = i <= c ? X.r(i, c) : 0.0;
@@ -625,7 +822,9 @@ __global__ __launch_bounds__(kQrThreads) void qr_householder_kernel(const SvdTas
     if (C) X.m(i, c) = 0.0;
   }
   __syncthreads();
-  house_apply<C, kQrThreads>(X, k, true, Hv, beta, p, k, vl);
+  }
+  if (phases & kPhApply) house_apply<C, kQrThreads>(X, k, true, Hv, beta, p, k, vl);
+  if (!(phases & kPhOut)) return;
   for (int64_t e = tid; e < (int64_t)p * k; e += kQrThreads) {
     const int i = (int)(e % p), c = (int)(e / p);
     if (!right) {  // U (m x k) = Q
@@ -638,17 +837,295 @@ __global__ __launch_bounds__(kQrThreads) void qr_householder_kernel(const SvdTas
   }
 }
 
+// ---------------------------------------------------------------------------
+// Multi-workgroup Householder reflections for large sectors (p <= kVecRows).
+// The one-workgroup kernels above keep a sector on one CU for its whole
+// factorisation; here the reflections run in panels of kHp: per panel one
+// launch factors the panel's columns (one workgroup per sector,
+// house_reflect_range) and one applies its reflections to the later columns,
+// one column per wave held in registers through the kHp reflections, the
+// panel's vectors in LDS (house_update_kernel); house_apply_kernel forms
+// H_0 ... H_{k-1} Y column by column the same way, every column through all
+// reflections in one launch.  Workspace layouts (layout): 0 the SVD's (X, Wm,
+// vectors, A, betas; p = max(m, n)), 1 / 2 the QR's left / right (X, vectors,
+// betas; p = m / n).  Sector s has its 8-column chunks at
+// col_base[s] .. col_base[s + 1] of the grid.
+constexpr int kHp = 8;
+constexpr int kHcols = kThreads / 64;  // columns per workgroup, one per wave
+
+template <bool C>
+struct HView {
+  int p, r, k;
+  Mat<C> X, Hv;
+  double* beta;
+};
+
+template <bool C>
+__device__ __forceinline__ HView<C> house_view(const SvdTask& t, int layout, double* W, int64_t w_plane) {
+  HView<C> v;
+  const bool sw = layout == 0 ? t.m < t.n : layout == 2;
+  v.p = sw ? t.n : t.m;
+  v.r = sw ? t.m : t.n;
+  v.k = v.p < v.r ? v.p : v.r;
+  const int64_t o_hv = layout == 0 ? (int64_t)v.p * v.r + (int64_t)v.r * v.r : (int64_t)v.p * v.r;
+  const int64_t o_b = layout == 0 ? 2 * (int64_t)v.p * v.r + 2 * (int64_t)v.r * v.r
+                                  : (int64_t)v.p * v.r + (int64_t)v.p * v.k;
+  v.X = Mat<C>{W + t.w_off, W + t.w_off + w_plane, v.p};
+  v.Hv = Mat<C>{W + t.w_off + o_hv, W + t.w_off + o_hv + w_plane, v.p};
+  v.beta = W + t.w_off + o_b;
+  return v;
+}
+
+__device__ __forceinline__ int task_of(const int* __restrict__ base, int ntasks, int b) {
+  int lo = 0, hi = ntasks - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (base[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// vectors j0 .. j1 - 1 into LDS, rows 0 .. p - 1, zero above row j
+template <bool C>
+__device__ __forceinline__ void panel_to_lds(const HView<C>& v, int j0, int j1, double* pl) {
+  for (int e = threadIdx.x; e < (j1 - j0) * v.p; e += kThreads) {
+    const int jj = e / v.p, i = e - jj * v.p, j = j0 + jj;
+    pl[jj * kVecRows + i] = i >= j ? v.Hv.r(i, j) : 0.0;
+    if (C) pl[(kHp + jj) * kVecRows + i] = i >= j ? v.Hv.m(i, j) : 0.0;
+  }
+}
+
+// y <- H_j y for the register-held column y (rows lane + 64 q), v_j in LDS
+template <bool C>
+__device__ __forceinline__ void reflect_regs_full(double* yr, double* yi, const double* pl, int jj, double b, int p,
+                                                  int lane) {
+  double wr_ = 0.0, wi_ = 0.0;
+#pragma unroll
+  for (int q = 0; q < kRpl; ++q) {
+    const int i = lane + 64 * q;
+    if (i < p) {
+      const double v_r = pl[jj * kVecRows + i];
+      wr_ += v_r * yr[q];
+      if (C) {
+        const double v_i = pl[(kHp + jj) * kVecRows + i];
+        wr_ += v_i * yi[q];
+        wi_ += v_r * yi[q] - v_i * yr[q];
+      }
+    }
+  }
+  wr_ = wave_sum(wr_);
+  if (C) wi_ = wave_sum(wi_);
+  const double fr = b * wr_, fi = b * wi_;
+#pragma unroll
+  for (int q = 0; q < kRpl; ++q) {
+    const int i = lane + 64 * q;
+    if (i < p) {
+      const double v_r = pl[jj * kVecRows + i];
+      const double v_i = C ? pl[(kHp + jj) * kVecRows + i] : 0.0;
+      yr[q] -= fr * v_r - fi * v_i;
+      if (C) yi[q] -= fr * v_i + fi * v_r;
+    }
+  }
+}
+
+template <bool C>
+__device__ __forceinline__ void col_load(const HView<C>& v, int c, double* yr, double* yi, int lane) {
+#pragma unroll
+  for (int q = 0; q < kRpl; ++q) {
+    const int i = lane + 64 * q;
+    yr[q] = i < v.p ? v.X.r(i, c) : 0.0;
+    yi[q] = (C && i < v.p) ? v.X.m(i, c) : 0.0;
+  }
+}
+
+template <bool C>
+__device__ __forceinline__ void col_store(const HView<C>& v, int c, const double* yr, const double* yi, int lane) {
+#pragma unroll
+  for (int q = 0; q < kRpl; ++q) {
+    const int i = lane + 64 * q;
+    if (i < v.p) {
+      v.X.r(i, c) = yr[q];
+      if (C) v.X.m(i, c) = yi[q];
+    }
+  }
+}
+
+template <bool C>
+__global__ __launch_bounds__(kThreads) void house_panel_kernel(const SvdTask* __restrict__ tasks, int layout, int j0,
+                                                               double* __restrict__ W, int64_t w_plane) {
+  __shared__ double red[kWaves];
+  __shared__ double hv[3];
+  __shared__ double vl[2 * kVecRows];
+  const HView<C> v = house_view<C>(tasks[blockIdx.x], layout, W, w_plane);
+  if (j0 >= v.k) return;
+  const int j1 = min(j0 + kHp, v.k);
+  house_reflect_range<C, kThreads>(v.X, v.Hv, v.beta, v.p, j0, j1, j1, red, hv, vl);
+}
+
+template <bool C>
+__global__ __launch_bounds__(kThreads) void house_update_kernel(const SvdTask* __restrict__ tasks, int ntasks,
+                                                                const int* __restrict__ col_base, int layout, int j0,
+                                                                double* __restrict__ W, int64_t w_plane) {
+  __shared__ double pl[(C ? 2 : 1) * kHp * kVecRows];
+  const int s = task_of(col_base, ntasks, blockIdx.x);
+  const HView<C> v = house_view<C>(tasks[s], layout, W, w_plane);
+  const int c0 = (blockIdx.x - col_base[s]) * kHcols;
+  if (j0 >= v.k) return;
+  const int j1 = min(j0 + kHp, v.k);
+  if (c0 + kHcols <= j1 || c0 >= v.r) return;  // no column of this workgroup is after the panel
+  panel_to_lds<C>(v, j0, j1, pl);
+  __syncthreads();
+  const int lane = threadIdx.x & 63, c = c0 + (threadIdx.x >> 6);
+  if (c < j1 || c >= v.r) return;
+  double yr[kRpl], yi[kRpl];
+  col_load<C>(v, c, yr, yi, lane);
+  for (int j = j0; j < j1; ++j) {
+    const double b = v.beta[j];
+    if (b != 0.0) reflect_regs_full<C>(yr, yi, pl, j - j0, b, v.p, lane);
+  }
+  col_store<C>(v, c, yr, yi, lane);
+}
+
+// Y <- H_0 ... H_{k-1} Y on X's first ncols columns (layout 0: all r,
+// columns from 0; QR layouts: the k columns of [I; 0], reflection j skipped on
+// columns before j, which it leaves unchanged, as LAPACK's xORGQR)
+template <bool C>
+__global__ __launch_bounds__(kThreads) void house_apply_kernel(const SvdTask* __restrict__ tasks, int ntasks,
+                                                               const int* __restrict__ col_base, int layout,
+                                                               double* __restrict__ W, int64_t w_plane) {
+  __shared__ double pl[(C ? 2 : 1) * kHp * kVecRows];
+  const int s = task_of(col_base, ntasks, blockIdx.x);
+  const HView<C> v = house_view<C>(tasks[s], layout, W, w_plane);
+  const bool from_j = layout != 0;
+  const int ncols = from_j ? v.k : v.r;
+  const int c0 = (blockIdx.x - col_base[s]) * kHcols;
+  if (c0 >= ncols) return;
+  const int lane = threadIdx.x & 63, c = c0 + (threadIdx.x >> 6);
+  const bool mine = c < ncols;
+  double yr[kRpl], yi[kRpl];
+  if (mine) col_load<C>(v, c, yr, yi, lane);
+  const int cmax = min(c0 + kHcols, ncols) - 1;
+  for (int j0 = ((v.k - 1) / kHp) * kHp; j0 >= 0; j0 -= kHp) {
+    if (from_j && cmax < j0) continue;  // uniform: every column here is before the panel
+    const int j1 = min(j0 + kHp, v.k);
+    __syncthreads();  // the previous panel's vectors are no longer read
+    panel_to_lds<C>(v, j0, j1, pl);
+    __syncthreads();
+    if (mine)
+      for (int j = j1 - 1; j >= j0; --j) {
+        const double b = v.beta[j];
+        if (b != 0.0 && !(from_j && c < j)) reflect_regs_full<C>(yr, yi, pl, j - j0, b, v.p, lane);
+      }
+  }
+  if (mine) col_store<C>(v, c, yr, yi, lane);
+}
+
 hipError_t launch_qr_batched(int cplx, int right, const SvdTask* tasks, int ntasks, const double* M, int64_t m_plane,
                              double* U, int64_t u_plane, double* Vt, int64_t vt_plane, double* S, double* W,
                              int64_t w_plane, hipStream_t st) {
   if (ntasks <= 0) return hipSuccess;
   if (cplx)
     qr_householder_kernel<true><<<ntasks, kQrThreads, 0, st>>>(tasks, right, M, m_plane, U, u_plane, Vt, vt_plane, S,
-                                                               W, w_plane);
+                                                               W, w_plane, kPhAll);
   else
     qr_householder_kernel<false><<<ntasks, kQrThreads, 0, st>>>(tasks, right, M, m_plane, U, u_plane, Vt, vt_plane,
-                                                                S, W, w_plane);
+                                                                S, W, w_plane, kPhAll);
   return hipGetLastError();
+}
+
+hipError_t launch_qr_phase(int cplx, int right, int phases, const SvdTask* tasks, int ntasks, const double* M,
+                           int64_t m_plane, double* U, int64_t u_plane, double* Vt, int64_t vt_plane, double* S,
+                           double* W, int64_t w_plane, hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  if (cplx)
+    qr_householder_kernel<true><<<ntasks, kQrThreads, 0, st>>>(tasks, right, M, m_plane, U, u_plane, Vt, vt_plane, S,
+                                                               W, w_plane, phases);
+  else
+    qr_householder_kernel<false><<<ntasks, kQrThreads, 0, st>>>(tasks, right, M, m_plane, U, u_plane, Vt, vt_plane,
+                                                                S, W, w_plane, phases);
+  return hipGetLastError();
+}
+
+hipError_t launch_svd_phase(int cplx, int phases, const SvdTask* tasks, int ntasks, const double* M, int64_t m_plane,
+                            double* U, int64_t u_plane, double* Vt, int64_t vt_plane, double* S, double* W,
+                            int64_t w_plane, hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  if (cplx)
+    svd_jacobi_kernel<true><<<ntasks, kThreads, 0, st>>>(tasks, M, m_plane, U, u_plane, Vt, vt_plane, S, W, w_plane,
+                                                          60, phases);
+  else
+    svd_jacobi_kernel<false><<<ntasks, kThreads, 0, st>>>(tasks, M, m_plane, U, u_plane, Vt, vt_plane, S, W,
+                                                           w_plane, 60, phases);
+  return hipGetLastError();
+}
+
+hipError_t launch_svd_block_round(int cplx, const SvdTask* tasks, int ntasks, const int* pair_base, int npairs,
+                                  int round, double* W, int64_t w_plane, double* rotated, int rmax, hipStream_t st) {
+  if (ntasks <= 0 || npairs <= 0) return hipSuccess;
+  if (rmax < 1 || rmax > kBjMaxR) return hipErrorInvalidValue;
+  static bool attr[2] = {false, false};  // the dynamic LDS limit, raised once per kernel
+  const size_t lds = sizeof(double) * (cplx ? 2 : 1) * 2 * kBj * (size_t)rmax;
+  const size_t lds_max = sizeof(double) * 2 * 2 * kBj * (size_t)kBjMaxR;
+  if (!attr[cplx ? 1 : 0]) {
+    const hipError_t e =
+        cplx ? hipFuncSetAttribute((const void*)svd_block_round_kernel<true>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_max)
+             : hipFuncSetAttribute((const void*)svd_block_round_kernel<false>,
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)(lds_max / 2));
+    if (e != hipSuccess) return e;
+    attr[cplx ? 1 : 0] = true;
+  }
+  if (cplx)
+    svd_block_round_kernel<true><<<npairs, kThreads, lds, st>>>(tasks, ntasks, pair_base, round, W, w_plane, rotated,
+                                                                rmax);
+  else
+    svd_block_round_kernel<false><<<npairs, kThreads, lds, st>>>(tasks, ntasks, pair_base, round, W, w_plane,
+                                                                 rotated, rmax);
+  return hipGetLastError();
+}
+
+hipError_t launch_house_panels(int cplx, int layout, const SvdTask* tasks, int ntasks, const int* col_base,
+                               int nchunks, int kmax, double* W, int64_t w_plane, hipStream_t st) {
+  if (ntasks <= 0 || nchunks <= 0) return hipSuccess;
+  for (int j0 = 0; j0 < kmax; j0 += kHp) {
+    if (cplx) {
+      house_panel_kernel<true><<<ntasks, kThreads, 0, st>>>(tasks, layout, j0, W, w_plane);
+      house_update_kernel<true><<<nchunks, kThreads, 0, st>>>(tasks, ntasks, col_base, layout, j0, W, w_plane);
+    } else {
+      house_panel_kernel<false><<<ntasks, kThreads, 0, st>>>(tasks, layout, j0, W, w_plane);
+      house_update_kernel<false><<<nchunks, kThreads, 0, st>>>(tasks, ntasks, col_base, layout, j0, W, w_plane);
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_house_apply(int cplx, int layout, const SvdTask* tasks, int ntasks, const int* col_base,
+                              int nchunks, double* W, int64_t w_plane, hipStream_t st) {
+  if (ntasks <= 0 || nchunks <= 0) return hipSuccess;
+  if (cplx)
+    house_apply_kernel<true><<<nchunks, kThreads, 0, st>>>(tasks, ntasks, col_base, layout, W, w_plane);
+  else
+    house_apply_kernel<false><<<nchunks, kThreads, 0, st>>>(tasks, ntasks, col_base, layout, W, w_plane);
+  return hipGetLastError();
+}
+
+int house_chunks(int m, int n, int layout) {
+  const int r = layout == 0 ? (m < n ? m : n) : (layout == 1 ? n : m);
+  return (r + kHcols - 1) / kHcols;
+}
+
+int svd_block_pairs(int m, int n) {
+  const int r = m < n ? m : n;
+  const int nb = (r + kBj - 1) / kBj;
+  return (nb + (nb & 1)) / 2;
+}
+int svd_block_rounds(int m, int n) {
+  const int r = m < n ? m : n;
+  const int nb = (r + kBj - 1) / kBj;
+  return nb + (nb & 1) - 1;
 }
 
 hipError_t launch_svd_batched(int cplx, const SvdTask* tasks, int ntasks, const double* M, int64_t m_plane,
@@ -657,10 +1134,10 @@ hipError_t launch_svd_batched(int cplx, const SvdTask* tasks, int ntasks, const 
   if (ntasks <= 0) return hipSuccess;
   if (cplx)
     svd_jacobi_kernel<true><<<ntasks, kThreads, 0, st>>>(tasks, M, m_plane, U, u_plane, Vt, vt_plane, S, W, w_plane,
-                                                          60);
+                                                          60, kPhAll);
   else
     svd_jacobi_kernel<false><<<ntasks, kThreads, 0, st>>>(tasks, M, m_plane, U, u_plane, Vt, vt_plane, S, W,
-                                                           w_plane, 60);
+                                                           w_plane, 60, kPhAll);
   return hipGetLastError();
 }
 

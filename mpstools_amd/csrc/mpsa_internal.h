@@ -150,6 +150,42 @@ struct SvdTask {
 hipError_t launch_svd_batched(int cplx, const SvdTask* tasks, int ntasks, const double* M, int64_t m_plane,
                               double* U, int64_t u_plane, double* Vt, int64_t vt_plane, double* S, double* W,
                               int64_t w_plane, hipStream_t st);
+// The same SVD in phases for large sectors (the kPh* bits below), several
+// workgroups per sector for the reflections, the sweeps and the
+// back-transform: the sweeps are launch_svd_block_round launches (one per
+// round of the block tournament over all tasks; svd_block_rounds(m, n) rounds
+// per sweep, svd_block_pairs(m, n) workgroups per task, their bases in
+// pair_base[ntasks + 1], rotated[task] set when a rotation happened) until a
+// sweep rotates nothing.  min(m, n) <= kSvdBlockMaxR.  Same outputs and
+// workspace as launch_svd_batched.
+constexpr int kSvdBlockMaxR = 512;
+hipError_t launch_svd_phase(int cplx, int phases, const SvdTask* tasks, int ntasks, const double* M, int64_t m_plane,
+                            double* U, int64_t u_plane, double* Vt, int64_t vt_plane, double* S, double* W,
+                            int64_t w_plane, hipStream_t st);
+// rmax: the largest min(m, n) of the launch's tasks (sizes its LDS)
+hipError_t launch_svd_block_round(int cplx, const SvdTask* tasks, int ntasks, const int* pair_base, int npairs,
+                                  int round, double* W, int64_t w_plane, double* rotated, int rmax, hipStream_t st);
+int svd_block_pairs(int m, int n);
+// Phases of svd_jacobi_kernel / qr_householder_kernel (bit set; kPhAll: the
+// whole factorisation in one launch).  The multi-workgroup path for sectors
+// with max(m, n) <= kVecRowsMax runs kPhLoad, launch_house_panels (in place of
+// kPhReflect), kPhInit, then for the SVD block rounds (kPhSweep),
+// kPhBackInit, and launch_house_apply (in place of kPhApply), kPhOut.
+constexpr int kPhLoad = 1, kPhReflect = 2, kPhInit = 4, kPhSweep = 8, kPhBackInit = 16, kPhApply = 32, kPhOut = 64;
+constexpr int kPhAll = 127;
+constexpr int kVecRowsMax = 1024;
+hipError_t launch_qr_phase(int cplx, int right, int phases, const SvdTask* tasks, int ntasks, const double* M,
+                           int64_t m_plane, double* U, int64_t u_plane, double* Vt, int64_t vt_plane, double* S,
+                           double* W, int64_t w_plane, hipStream_t st);
+// workspace layout: 0 the SVD's, 1 / 2 the left / right QR's; col_base[s] ..
+// col_base[s + 1]: sector s's house_chunks(m, n, layout) workgroups; kmax: the largest
+// min(m, n)
+hipError_t launch_house_panels(int cplx, int layout, const SvdTask* tasks, int ntasks, const int* col_base,
+                               int nchunks, int kmax, double* W, int64_t w_plane, hipStream_t st);
+hipError_t launch_house_apply(int cplx, int layout, const SvdTask* tasks, int ntasks, const int* col_base,
+                              int nchunks, double* W, int64_t w_plane, hipStream_t st);
+int house_chunks(int m, int n, int layout);
+int svd_block_rounds(int m, int n);
 // Batched Householder QR of the same tasks in the SVD's output format with
 // S = 1: right = 0: U = Q (m x k), V^dagger = R (k x n); right = 1: M = L Q'
 // with U = L (m x k), V^dagger = Q' (k x n, orthonormal rows).  Workspace
@@ -183,19 +219,20 @@ hipError_t launch_scale_rsqrt(double* X, const double* nrm2, int64_t n, int64_t 
 hipError_t launch_lz_combine(double* const* basis, const double* c, int ldc, const int* m, double* out, int64_t n,
                              int64_t vs, int nvec, hipStream_t st);
 
-// Workgroup tile of each (precision, tile class).  FP64/FP32: 0 = 128x128,
-// 1 = 64x64, 2 = 32x32.  BF16/BF16X3: 0 = 256x128, 1 = 64x64, 2 = 128x128,
+// Tile of each (precision, tile class).  FP64/FP32: 0 = 128x128, 1 = 64x64,
+// 2 = 32x32 per workgroup; 3 = 16x16 and 4 = 32x32 per wave (four waves per
+// workgroup, no LDS).  BF16/BF16X3: 0 = 256x128, 1 = 64x64, 2 = 128x128,
 // 3 = 256x256, 4 = 32x32.
 constexpr int kMaxTileClasses = 5;
 inline void tile_dims(int prec, int cls, int* bm, int* bn) {
   static const int bf[5][2] = {{256, 128}, {64, 64}, {128, 128}, {256, 256}, {32, 32}};
-  static const int fp[3][2] = {{128, 128}, {64, 64}, {32, 32}};
+  static const int fp[5][2] = {{128, 128}, {64, 64}, {32, 32}, {16, 16}, {32, 32}};
   const bool b = prec == BF16 || prec == BF16X3;
-  const int c = cls < 0 ? 0 : (b ? (cls > 4 ? 4 : cls) : (cls > 2 ? 2 : cls));
+  const int c = cls < 0 ? 0 : (cls > 4 ? 4 : cls);
   *bm = b ? bf[c][0] : fp[c][0];
   *bn = b ? bf[c][1] : fp[c][1];
 }
-inline int num_tile_classes(int prec) { return (prec == BF16 || prec == BF16X3) ? 5 : 3; }
+inline int num_tile_classes(int) { return 5; }
 void set_last_error(const char* msg);
 
 }  // namespace mpsa

@@ -254,6 +254,96 @@ __global__ __launch_bounds__(256, 2) void gemm16_kernel(const GTask* __restrict_
   }
 }
 
+// ---------------------------------------------------------------------------
+// FP64 / FP32 grouped GEMM for small sector blocks: one WTxWT output tile per
+// wave (WT = 16 or 32: 1 or 2x2 16x16x4 MFMA tiles), four independent waves
+// per workgroup, no LDS and no barrier.  Each MFMA takes its A fragment
+// (lane: row l16, k lq) and B fragment (k lq, column l16) straight from
+// global memory (the blocks of a small sector stay in L1/L2); four k-steps of
+// loads are issued ahead of their MFMAs, and a 32-wide tile reuses each
+// fragment for two MFMAs.  Tiles are numbered as for gemm16_kernel
+// (m-fastest inside a task) and every output element is accumulated over k
+// in the same order, so FP64 results equal gemm16_kernel's bit for bit;
+// loads clamp to valid addresses and zero what lies outside the block.
+// ---------------------------------------------------------------------------
+template <typename T, typename TOut, bool TA, bool TB, int WT>
+__global__ __launch_bounds__(256) void gemm_wave_kernel(const GTask* __restrict__ tasks, int ntasks,
+                                                        const GSeg* __restrict__ segs, int tiles_per_vec,
+                                                        int total_tiles, const T* __restrict__ A, int64_t a_vs,
+                                                        const T* __restrict__ B, int64_t b_vs,
+                                                        TOut* __restrict__ C, int64_t c_vs) {
+  typedef typename Acc4<T>::type V4;
+  constexpr int NT = WT / 16;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int l16 = lane & 15, lq = lane >> 4;
+  const int nw = gridDim.x * 4;
+  for (int g = xcd_logical_id() * 4 + wave; g < total_tiles; g += nw) {
+    const int v = g / tiles_per_vec;
+    const int t = g - v * tiles_per_vec;
+    const GTask tk = tasks[find_task(tasks, ntasks, t, &GTask::tile0)];
+    const int local = t - tk.tile0;
+    const int row0 = (local % tk.tiles_m) * WT, col0 = (local / tk.tiles_m) * WT;
+    const int m = tk.m, n = tk.n;
+    bool rok[NT], cok[NT];
+    int rowc[NT], colc[NT];  // always-valid addresses
+#pragma unroll
+    for (int i = 0; i < NT; ++i) {
+      const int row = row0 + 16 * i + l16, col = col0 + 16 * i + l16;
+      rok[i] = row < m;
+      cok[i] = col < n;
+      rowc[i] = rok[i] ? row : m - 1;
+      colc[i] = cok[i] ? col : n - 1;
+    }
+    V4 acc[NT][NT];
+#pragma unroll
+    for (int i = 0; i < NT; ++i)
+#pragma unroll
+      for (int j = 0; j < NT; ++j) acc[i][j] = V4{0, 0, 0, 0};
+    const int send = tk.seg0 + tk.nseg;
+    for (int si = tk.seg0; si < send; ++si) {
+      const GSeg sg = segs[uni(si)];
+      const int k = uni(sg.k), lda = uni(sg.lda), ldb = uni(sg.ldb);
+      const T* Ab = A + (int64_t)v * a_vs + sg.a_off;
+      const T* Bb = B + (int64_t)v * b_vs + sg.b_off;
+      const T alpha = (T)sg.alpha;
+      for (int k0 = 0; k0 < k; k0 += 16) {
+        T af[4][NT], bf[4][NT];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int kk = k0 + 4 * q + lq;
+          const bool kok = kk < k;
+          const int kc = kok ? kk : k - 1;
+#pragma unroll
+          for (int i = 0; i < NT; ++i) {
+            const T a = TA ? Ab[kc + (int64_t)rowc[i] * lda] : Ab[rowc[i] + (int64_t)kc * lda];
+            const T b = TB ? Bb[colc[i] + (int64_t)kc * ldb] : Bb[kc + (int64_t)colc[i] * ldb];
+            af[q][i] = (rok[i] && kok) ? a * alpha : (T)0;
+            bf[q][i] = (cok[i] && kok) ? b : (T)0;
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          if (k0 + 4 * q < k)  // uniform: k0, q and k are the same in every lane
+#pragma unroll
+            for (int i = 0; i < NT; ++i)
+#pragma unroll
+              for (int j = 0; j < NT; ++j) acc[i][j] = mfma16(af[q][i], bf[q][j], acc[i][j]);
+      }
+    }
+    TOut* Cb = C + (int64_t)v * c_vs + tk.c_off;
+#pragma unroll
+    for (int i = 0; i < NT; ++i)
+#pragma unroll
+      for (int j = 0; j < NT; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int orow = row0 + 16 * i + acc_row<T>(lq, r);
+          const int ocol = col0 + 16 * j + l16;
+          if (orow < m && ocol < n) Cb[orow + (int64_t)ocol * m] = cvt<TOut>(acc[i][j][r]);
+        }
+  }
+}
+
 // Output store of the bf16 kernels: doubles as-is, bf16 as one plane or a
 // hi/lo pair (hi = bf16(x), lo = bf16(x - hi)).
 template <typename TOut, bool SPLIT>
@@ -2096,6 +2186,27 @@ int gemm_max_grid(int prec_in, int tile_class) {
   return num_cu() * 2;
 }
 
+template <typename T, typename TOut, int WT>
+static hipError_t runwave(int tr, const GTask* tasks, int ntasks, const GSeg* segs, int tpv, int nvec, const void* A,
+                          int64_t a_vs, const void* B, int64_t b_vs, void* C, int64_t c_vs, hipStream_t st) {
+  const int total = tpv * nvec;
+  if (total <= 0) return hipSuccess;
+  int grid = num_cu() * (WT == 16 ? 8 : 4);  // 32 (16 per wave) or 16 waves per CU
+  const int need = (total + 3) / 4;
+  if (grid > need) grid = need;
+  if (grid >= 8) grid &= ~7;
+  const T* a = (const T*)A;
+  const T* b = (const T*)B;
+  TOut* c = (TOut*)C;
+  switch (tr) {
+    case 0: gemm_wave_kernel<T, TOut, false, false, WT><<<grid, 256, 0, st>>>(tasks, ntasks, segs, tpv, total, a, a_vs, b, b_vs, c, c_vs); break;
+    case 1: gemm_wave_kernel<T, TOut, false, true, WT><<<grid, 256, 0, st>>>(tasks, ntasks, segs, tpv, total, a, a_vs, b, b_vs, c, c_vs); break;
+    case 2: gemm_wave_kernel<T, TOut, true, false, WT><<<grid, 256, 0, st>>>(tasks, ntasks, segs, tpv, total, a, a_vs, b, b_vs, c, c_vs); break;
+    default: gemm_wave_kernel<T, TOut, true, true, WT><<<grid, 256, 0, st>>>(tasks, ntasks, segs, tpv, total, a, a_vs, b, b_vs, c, c_vs); break;
+  }
+  return hipGetLastError();
+}
+
 template <typename T, typename TOut, int BM, int BN>
 static hipError_t run16(int tr, const GTask* tasks, int ntasks, const GSeg* segs, int tpv, int nvec, const void* A,
                         int64_t a_vs, const void* B, int64_t b_vs, void* C, int64_t c_vs, hipStream_t st) {
@@ -2165,6 +2276,10 @@ hipError_t launch_gemm(int prec_in, int out_kind, int tr, int tile_class, const 
       return run16<T_, TO_, 128, 128>(tr, tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, st); \
     if (tile_class == 2)                                                                                     \
       return run16<T_, TO_, 32, 32>(tr, tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, st);   \
+    if (tile_class == 3)                                                                                     \
+      return runwave<T_, TO_, 16>(tr, tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, st);     \
+    if (tile_class == 4)                                                                                     \
+      return runwave<T_, TO_, 32>(tr, tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, st);     \
     return run16<T_, TO_, 64, 64>(tr, tasks, ntasks, segs, tiles_per_vec, nvec, A, a_vs, B, b_vs, C, c_vs, st);     \
   } while (0)
   if (prec_in == F64) MPSA_RUN16(double, double);

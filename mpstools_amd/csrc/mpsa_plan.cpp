@@ -266,7 +266,8 @@ static bool fits_pipe_tile(const std::vector<GTask>& t, const std::vector<GSeg>&
 // 256x256 class only where fits_pipe_tile holds.
 static bool class_admissible(const std::vector<GTask>& t, const std::vector<GSeg>& segs, int prec, int cls) {
   if (cls < 0 || cls >= num_tile_classes(prec)) return false;
-  return cls != 3 || fits_pipe_tile(t, segs);
+  const bool bf = prec == BF16 || prec == BF16X3;
+  return !bf || cls != 3 || fits_pipe_tile(t, segs);
 }
 
 static int pick_class(const std::vector<GTask>& t, const std::vector<GSeg>& segs, int prec) {
@@ -274,8 +275,10 @@ static int pick_class(const std::vector<GTask>& t, const std::vector<GSeg>& segs
   int best_area = 0;
   double fallback_fill = -1.0;
   const bool pipe_ok = fits_pipe_tile(t, segs);
+  const bool bfp = prec == BF16 || prec == BF16X3;
+  double fill_cls[kMaxTileClasses] = {0, 0, 0, 0, 0};
   for (int cls = 0; cls < num_tile_classes(prec); ++cls) {
-    if (cls == 3 && !pipe_ok) continue;
+    if (bfp && cls == 3 && !pipe_ok) continue;
     int bm, bn;
     tile_dims(prec, cls, &bm, &bn);
     double useful = 0, padded = 0;
@@ -284,6 +287,8 @@ static int pick_class(const std::vector<GTask>& t, const std::vector<GSeg>& segs
       padded += (double)((x.m + bm - 1) / bm) * ((x.n + bn - 1) / bn) * (double)bm * bn;
     }
     const double fill = padded > 0 ? useful / padded : 1.0;
+    fill_cls[cls] = fill;
+    if (!bfp && cls >= 3) continue;  // the wave tiles: decided below (class 4 by the autotuner only)
     if (fill >= 0.8 && bm * bn > best_area) {
       best = cls;
       best_area = bm * bn;
@@ -293,7 +298,13 @@ static int pick_class(const std::vector<GTask>& t, const std::vector<GSeg>& segs
       fallback_fill = fill;
     }
   }
-  return best >= 0 ? best : fallback;
+  const int pick = best >= 0 ? best : fallback;
+  // FP64/FP32 stages of blocks mostly below 16 rows or columns (the 32x32
+  // tile under 40 % full): the barrier-free 16x16 wave tile (measured 1.9x
+  // on the 16-vector Hubbard D = 1024 bond, slower than 32x32 on the t-V and
+  // XXZ bonds whose blocks reach 80 and 196)
+  if (!bfp && pick == 2 && fill_cls[2] < 0.4) return 3;
+  return pick;
 }
 
 // Number the workgroup tiles of a stage for tile class cls, in task order

@@ -120,6 +120,7 @@ def lib():
         L.mpsa_cheb_profile.argtypes = [P, P]
         L.mpsa_cheb_moment_times.argtypes = [P, P, C.c_int, P]
         L.mpsa_cheb_set_big_sector.argtypes = [P, C.c_int]
+        L.mpsa_cheb_set_block_jacobi.argtypes = [P, C.c_int]
         L.mpsa_cheb_factor_log.argtypes = [P, C.c_int64]
         L.mpsa_cheb_factor_records.argtypes = [P, P, C.c_int64, P]
         L.mpsa_cheb_destroy.argtypes = [P]
@@ -723,6 +724,11 @@ class Cheb:
     def set_big_sector(self, k):
         """Sectors with min(m, n) >= k go to rocSOLVER (default 1024)."""
         _check(lib().mpsa_cheb_set_big_sector(self.h, int(k)))
+
+    def set_block_jacobi(self, r_min):
+        """Sectors with min(m, n) >= r_min (<= 512) of an SVD step take the
+        blocked Jacobi SVD across several workgroups (0: never; default 96)."""
+        _check(lib().mpsa_cheb_set_block_jacobi(self.h, int(r_min)))
 
     def factor_log(self, cap):
         """Record the sector shapes of the next cap canonicalisation launches

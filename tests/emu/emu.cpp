@@ -665,7 +665,8 @@ hipError_t launch_gemm(int prec_in, int out_kind, int tr, int tile_class, const 
                        const GSeg* segs, int tiles_per_vec, int nvec, const void* A, int64_t a_vs, const void* B,
                        int64_t b_vs, void* C, int64_t c_vs, hipStream_t st, Planes) {
   if (tiles_per_vec * nvec <= 0) return hipSuccess;
-  const int bm = tile_class == 0 ? 128 : (tile_class == 2 ? 32 : 64);
+  int bm = 64, bn = 64;
+  tile_dims(prec_in, tile_class, &bm, &bn);
   const bool TA = (tr >> 1) & 1, TB = tr & 1;
   if (prec_in == F64) {
     submit(st, "gemm f64", [=]() {
@@ -1265,6 +1266,88 @@ hipError_t launch_svd_batched(int cplx, const SvdTask* tasks, int ntasks, const 
     }
   });
   return hipSuccess;
+}
+// the multi-workgroup factorisation's phases: the phases before the outputs
+// leave nothing the emulation needs (they check their tasks), no round
+// rotates, and kPhOut is the whole SVD or QR (LAPACK) from M
+static void chk_phase_tasks(hipStream_t st, const SvdTask* tasks, int ntasks, int rmax) {
+  submit(st, "factor phase", [=]() {
+    if (!chk_range(tasks, 0, ntasks - 1, "factor tasks")) return;
+    for (int i = 0; i < ntasks; ++i) {
+      if (std::min(tasks[i].m, tasks[i].n) > rmax) report("factor phase: task above the blocked path's limit");
+      if (std::max(tasks[i].m, tasks[i].n) > kVecRowsMax) report("factor phase: task above the register path's rows");
+    }
+  });
+}
+hipError_t launch_svd_phase(int cplx, int phases, const SvdTask* tasks, int ntasks, const double* M, int64_t m_plane,
+                            double* U, int64_t u_plane, double* Vt, int64_t vt_plane, double* S, double* W,
+                            int64_t w_plane, hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  if (!(phases & kPhOut)) {
+    chk_phase_tasks(st, tasks, ntasks, kSvdBlockMaxR);
+    return hipSuccess;
+  }
+  return launch_svd_batched(cplx, tasks, ntasks, M, m_plane, U, u_plane, Vt, vt_plane, S, W, w_plane, st);
+}
+hipError_t launch_qr_phase(int cplx, int right, int phases, const SvdTask* tasks, int ntasks, const double* M,
+                           int64_t m_plane, double* U, int64_t u_plane, double* Vt, int64_t vt_plane, double* S,
+                           double* W, int64_t w_plane, hipStream_t st) {
+  if (ntasks <= 0) return hipSuccess;
+  if (!(phases & kPhOut)) {
+    chk_phase_tasks(st, tasks, ntasks, kVecRowsMax);
+    return hipSuccess;
+  }
+  return launch_qr_batched(cplx, right, tasks, ntasks, M, m_plane, U, u_plane, Vt, vt_plane, S, W, w_plane, st);
+}
+static void chk_house_table(hipStream_t st, const SvdTask* tasks, int ntasks, const int* col_base, int nchunks,
+                            int layout) {
+  submit(st, "house chunks", [=]() {
+    if (!chk_range(tasks, 0, ntasks - 1, "house tasks") || !chk_range(col_base, 0, ntasks, "house chunk base"))
+      return;
+    if (layout < 0 || layout > 2) report("house: bad layout");
+    if (col_base[0] != 0 || col_base[ntasks] != nchunks) report("house: bad chunk table");
+    for (int i = 0; i < ntasks; ++i)
+      if (col_base[i + 1] - col_base[i] != house_chunks(tasks[i].m, tasks[i].n, layout))
+        report("house: chunk count differs from house_chunks");
+  });
+}
+hipError_t launch_house_panels(int, int layout, const SvdTask* tasks, int ntasks, const int* col_base, int nchunks,
+                               int kmax, double*, int64_t, hipStream_t st) {
+  if (ntasks <= 0 || nchunks <= 0) return hipSuccess;
+  if (kmax < 1) report("house panels: kmax < 1");
+  chk_house_table(st, tasks, ntasks, col_base, nchunks, layout);
+  return hipSuccess;
+}
+hipError_t launch_house_apply(int, int layout, const SvdTask* tasks, int ntasks, const int* col_base, int nchunks,
+                              double*, int64_t, hipStream_t st) {
+  if (ntasks <= 0 || nchunks <= 0) return hipSuccess;
+  chk_house_table(st, tasks, ntasks, col_base, nchunks, layout);
+  return hipSuccess;
+}
+int house_chunks(int m, int n, int layout) {
+  const int r = layout == 0 ? std::min(m, n) : (layout == 1 ? n : m);
+  return (r + 7) / 8;
+}
+hipError_t launch_svd_block_round(int, const SvdTask* tasks, int ntasks, const int* pair_base, int npairs, int round,
+                                  double*, int64_t, double* rotated, int rmax, hipStream_t st) {
+  if (rmax < 1 || rmax > kSvdBlockMaxR) return hipErrorInvalidValue;
+  submit(st, "svd block round", [=]() {
+    for (int i = 0; i < ntasks; ++i)
+      if (std::min(tasks[i].m, tasks[i].n) > rmax) report("svd block round: task above rmax");
+    if (!chk_range(tasks, 0, ntasks - 1, "svd tasks") || !chk_range(pair_base, 0, ntasks, "svd pair base") ||
+        !chk_range(rotated, 0, ntasks - 1, "svd rotated"))
+      return;
+    if (pair_base[0] != 0 || pair_base[ntasks] != npairs || round < 0) report("svd block round: bad pair table");
+  });
+  return hipSuccess;
+}
+int svd_block_pairs(int m, int n) {
+  const int r = std::min(m, n), nb = (r + 7) / 8;
+  return (nb + (nb & 1)) / 2;
+}
+int svd_block_rounds(int m, int n) {
+  const int r = std::min(m, n), nb = (r + 7) / 8;
+  return nb + (nb & 1) - 1;
 }
 int gemm_max_grid(int, int) { return 512; }
 

@@ -158,7 +158,8 @@ def test_emulation_reports_planted_mistakes(emu):
 
 
 def test_emulated_tile_classes_agree_and_autotune_keeps_ties(lib, oracle):
-    """Every FP64 tile class of both GEMM stages gives the same bits and the
+    """Every FP64 tile class of both GEMM stages (128, 64, 32 per workgroup,
+    16 and 32 per wave) gives the same bits and the
     oracle's contraction (doContraction, src/MPSfunctions.hpp:726-804); bad
     classes are refused; the autotuner times every admissible class and, with
     the emulation's zero-duration events, keeps the plan's own classes."""
@@ -172,8 +173,8 @@ def test_emulated_tile_classes_agree_and_autotune_keeps_ties(lib, oracle):
     plan.bind_env(dL, dR)
     x = DevBuf(lib, vt.pack(psi))
     outs = {}
-    for c1 in (0, 1, 2):
-        for c2 in (0, 1, 2):
+    for c1 in (0, 1, 2, 3, 4):
+        for c2 in (0, 1, 2, 3, 4):
             plan.set_tile_class(1, c1)
             plan.set_tile_class(2, c2)
             assert (plan.stage_info()["tile_class1"], plan.stage_info()["tile_class2"]) == (c1, c2)
@@ -187,13 +188,13 @@ def test_emulated_tile_classes_agree_and_autotune_keeps_ties(lib, oracle):
     num = sum(np.sum((y0[k] - y_ref[k]) ** 2) for k in y_ref.keys())
     den = sum(np.sum(y_ref[k] ** 2) for k in y_ref.keys())
     assert np.sqrt(num / den) <= 1e-12
-    for bad in ((1, 3), (2, 4), (1, -1), (3, 0)):
+    for bad in ((1, 5), (2, 6), (1, -1), (3, 0)):
         with pytest.raises(native.MpsaError):
             plan.set_tile_class(*bad)
     plan.set_tile_class(1, 1)
     plan.set_tile_class(2, 0)
     ms = plan.autotune(nvec=2, reps=1)
-    assert set(ms) == {(1, 0), (1, 1), (1, 2), (2, 0), (2, 1), (2, 2)}
+    assert set(ms) == {(s_, c_) for s_ in (1, 2) for c_ in range(5)}
     assert (plan.stage_info()["tile_class1"], plan.stage_info()["tile_class2"]) == (1, 0)
     y = DevBuf(lib, np.zeros(vt.total))
     plan.apply(x, y)
@@ -521,7 +522,15 @@ def test_emulated_chebyshev_engine_matches_oracle(lib, oracle, model, L, nvec, c
         assert (r_["max_k"] >= 3) == r_["kind"].endswith("rocsolver")
     eng3.factor_log(0)
     assert eng3.factor_records() == []
-    del eng, eng2, eng3
+    # every sector below the rocSOLVER threshold through the blocked SVD's
+    # phases and rounds (pair table, per-sweep flags, finish)
+    eng4 = native.Cheb(sts, mpo, itk, alloc=al, to_host=th)
+    eng4.set_block_jacobi(1)
+    r4 = eng4.moments(6, 10000, 2)
+    assert np.abs(r4["moments"] - r["moments"]).max() <= 1e-12
+    with pytest.raises(native.MpsaError):
+        eng4.set_block_jacobi(-1)
+    del eng, eng2, eng3, eng4
 
 
 @pytest.mark.parametrize("cplx", [False, True])

@@ -1,7 +1,7 @@
 """Tile-class selection of H_eff plans on the GPU (mpsa_plan_set_tile_class,
 mpsa_plan_autotune).
 
-Bars: in FP64 every tile class (128x128, 64x64, 32x32) of both grouped GEMM
+Bars: in FP64 every tile class (128x128, 64x64, 32x32 per workgroup, 16x16 and 32x32 per wave) of both grouped GEMM
 stages gives the same bits (each output element is accumulated over k in the
 same order whatever the tile) and the oracle's contraction (doContraction,
 src/MPSfunctions.hpp:726-804) within 1e-12; in BF16X3 every admissible class, the 256x256 one included,
@@ -54,7 +54,7 @@ def _rel(vt, y, ref):
 
 def _admissible(plan, stage, prec):
     out = []
-    for cls in range(5 if prec in (native.BF16, native.BF16X3) else 3):
+    for cls in range(5):
         try:
             plan.set_tile_class(stage, cls)
             out.append(cls)
@@ -68,15 +68,15 @@ def test_fp64_tile_classes_give_identical_bits(gpu, oracle, model, L, N, D, seed
     _say("fp64 tile classes", model, L, N, D)
     plan, vt, x, ref, keep = _setup(gpu, oracle, model, L, N, D, seed, dense, native.F64)
     outs = {}
-    for c1 in (0, 1, 2):
-        for c2 in (0, 1, 2):
+    for c1 in (0, 1, 2, 3, 4):
+        for c2 in (0, 1, 2, 3, 4):
             plan.set_tile_class(1, c1)
             plan.set_tile_class(2, c2)
             outs[(c1, c2)] = _apply(plan, vt, x)
     y0 = outs[(0, 0)]
     assert all(torch.equal(y, y0) for y in outs.values())
     assert _rel(vt, y0, ref) <= 1e-12
-    for bad in ((1, 3), (2, 4), (0, 0)):
+    for bad in ((1, 5), (2, 6), (0, 0)):
         with pytest.raises(native.MpsaError):
             plan.set_tile_class(*bad)
 

@@ -36,17 +36,22 @@ def _say(*a):
 
 def _engine(sts, mpo, itk, big, **kw):
     e = native.Cheb(sts, mpo, itk, **kw)
-    if big:
+    if big == "blocked":
+        e.set_block_jacobi(2)
+    elif big:
         e.set_big_sector(big)
     return e
 
 
-@pytest.mark.parametrize("big", [0, 3])
+@pytest.mark.parametrize("big", [0, 3, "blocked"])
 @pytest.mark.parametrize("model,L,nvec,chimax,cplx", [("hubbard", 4, 2, 24, False), ("xxz", 6, 3, 15, False),
                                                       ("hubbard", 4, 2, 24, True), ("xxz", 8, 2, 40, True)])
 def test_cheb_engine_steps_and_moments_match_oracle(gpu, oracle, model, L, nvec, chimax, cplx, big):
     """big = 3 sends every sector of at least 3 columns to the rocSOLVER path
-    (gesvd / geqrf+ungqr / gelqf+unglq) that the engine uses for large sectors."""
+    (gesvd / geqrf+ungqr / gelqf+unglq) that the engine uses for large sectors;
+    "blocked" sends every SVD sector of at least 2 columns to the blocked
+    Jacobi SVD (several workgroups per sector) that the engine uses for
+    sectors of 96 to 512 columns."""
     _say("cheb engine", model, L, nvec, chimax, cplx, big)
     sts, itk = chebcase.states(model, L, nvec, 40 + L, cplx=cplx)
     mpo = chebcase.scaled_mpo(model, L)
@@ -93,8 +98,8 @@ def test_cheb_engine_mid_size_properties(gpu):
             assert sum(dims.values()) <= 96
 
 
-@pytest.mark.parametrize("cplx", [True, False])
-def test_cheb_engine_large_sector_invariants(gpu, cplx):
+@pytest.mark.parametrize("cplx,block", [(True, None), (False, None), (True, 16), (False, 0)])
+def test_cheb_engine_large_sector_invariants(gpu, cplx, block):
     """Sector matrices of up to a few hundred rows (Hubbard L = 24, D = 512,
     U(1) x U(1), 2 device-generated start vectors) through the batched
     Householder QR and the Jacobi SVD on R^H: the start vectors have unit
@@ -102,13 +107,16 @@ def test_cheb_engine_large_sector_invariants(gpu, cplx):
     every bond (canonical form by SVD and QR, then sweeps) returns the same
     state (conv <= 1e-10, <c|t> = 1 within 1e-10); compressSingleSite with
     chimax above w D gives the exact applyMPO state (|c - H t|^2 within 1e-10
-    of |H t|^2)."""
-    _say("cheb engine large sectors", cplx)
+    of |H t|^2).  block: the blocked Jacobi SVD's threshold (None: the
+    default 96, 0: every sector in the one-workgroup kernel)."""
+    _say("cheb engine large sectors", cplx, block)
     L, D = 24, 512
     itk = [models.itokey_u1xu1()] * L
     N = models.pack_charge(L // 2, L // 2)
     mpo = models.rescale_mpo(models.hubbard_mpo(L, 1.0, 4.0), 4.0 * (L - 1) + 4.0 * L, 0.0)
     e = native.ChebRandom(L, itk, N, D, 2, mpo, 424242, cplx=cplx)
+    if block is not None:
+        e.set_block_jacobi(block)
     e.factor_log(100000)
     assert np.abs(e.overlap(0, 0) - 1.0).max() <= 1e-12
     e.copy(1, 0)
