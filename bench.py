@@ -246,6 +246,8 @@ def chebyshev(torch, args):
         prec = legs[name]
         t0 = time.perf_counter()
         e = native.ChebRandom(L, itk, N, D, nv, mpo, args.seed + 2, cplx=True, h1_precision=prec)
+        if args.cheb_block_min is not None:
+            e.set_block_jacobi(args.cheb_block_min)
         torch.cuda.synchronize()
         setup = time.perf_counter() - t0
         r = e.moments(3, D, 2)
@@ -435,6 +437,9 @@ def main():
     ap.add_argument("--no-u1", action="store_true", help="skip the U(1) bonds of the other workloads")
     ap.add_argument("--cheb-D", type=int, default=1024, help="bond dimension of the Chebyshev workload (0 skips)")
     ap.add_argument("--cheb-nvec", type=int, default=16, help="start vectors of the Chebyshev workload")
+    ap.add_argument("--cheb-block-min", type=int, default=None,
+                    help="sectors of at least this many columns are factorised by several workgroups "
+                         "(engine default 32; 0: one workgroup per sector)")
     ap.add_argument("--cheb-legs", default="complex128,complex64_contraction",
                     help="Chebyshev legs to run: complex128 and/or complex64_contraction")
     ap.add_argument("--calib", action="store_true",

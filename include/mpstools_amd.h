@@ -500,7 +500,7 @@ int mpsa_cheb_moment_times(const mpsa_cheb* engine, double* out, int cap, int* n
  * call each; smaller ones by the batched kernels in one launch.  k in
  * [1, 1024]. */
 int mpsa_cheb_set_big_sector(mpsa_cheb* engine, int k);
-/* Sectors with r = min(m, n) >= r_min (default 96) and max(m, n) <= 1024
+/* Sectors with r = min(m, n) >= r_min (default 32) and max(m, n) <= 1024
  * are factorised by several workgroups each: the Householder reflections in
  * panels of 8 columns, Q and the SVD's back-transform column by column, and
  * (SVD steps, r <= 512) the blocked Jacobi sweeps, one launch per round of

@@ -51,7 +51,7 @@ using namespace mpsa;
 using namespace mpsa::host;
 
 #ifndef MPSA_BLOCK_JACOBI_MIN
-#define MPSA_BLOCK_JACOBI_MIN 96
+#define MPSA_BLOCK_JACOBI_MIN 32
 #endif
 #ifndef MPSA_BIG_SECTOR
 #define MPSA_BIG_SECTOR 1024
@@ -629,7 +629,7 @@ void svd_blocked(Eng& E, bool cplx, const std::vector<SvdTask>& big, const doubl
       rmax = std::max(rmax, (int)std::min(a.m, a.n));
     }
     Buf da = E.upload(act), db = E.upload(base);
-    for (int round = 0; round < rounds; ++round)
+    for (int round = -1; round < rounds; ++round)
       hchk(launch_svd_block_round(cplx, (const SvdTask*)da.p, na, (const int*)db.p, base.back(), round, W, w_plane,
                                   flags.p, rmax, E.st),
            "cheb: sector SVD round");

@@ -575,18 +575,22 @@ __global__ __launch_bounds__(kThreads) void svd_jacobi_kernel(const SvdTask* __r
 
 // ---------------------------------------------------------------------------
 // Block Jacobi rounds for large sectors, several workgroups per sector.
-// After svd_jacobi_kernel's reduction (mode 1) the r columns of A = R^H are
+// After the reduction and init phases the r columns of A = R^H are
 // split into blocks of kBj columns; one launch runs one round of the block
 // tournament over every large sector at once, one workgroup per block pair
-// (I, J): it loads the 2 kBj columns into LDS, runs one inner sweep of the
-// scalar kernel's rotations over all pairs of those columns (same rotation,
-// same threshold), writes them back and applies the accumulated 2 kBj x 2 kBj
-// rotation to the same columns of Wm.  Sectors have their block pairs at
-// pair_base[s] .. pair_base[s + 1] of the grid; rotated[s] is set when any
-// pair of the sector rotated (the host repeats sweeps until none does, then
-// svd_jacobi_kernel mode 2 finishes).  r <= kBjMaxR.
+// (I, J): it loads the 2 kBj columns into LDS, rotates every pair (i in I,
+// j in J) once with the scalar kernel's rotation and threshold, writes them
+// back and applies the accumulated 2 kBj x 2 kBj rotation to the same columns
+// of Wm.  The launch with round < 0 does the pairs inside each block (a
+// tournament over its columns, one workgroup per block), so a sweep (that
+// launch, then the rounds) visits every pair of columns once.  Sectors have
+// their block pairs at pair_base[s] .. pair_base[s + 1] of the grid (twice
+// that for the intra-block launch); rotated[s] is set when any pair of the
+// sector rotated (the host repeats sweeps until none does, then the
+// back-transform and output phases finish).  r <= kBjMaxR.
 constexpr int kBj = 8;
 constexpr int kBjMaxR = 512;
+static_assert(kBj == kWaves, "one wave per column of block I in the rounds across blocks");
 
 template <bool C>
 __global__ __launch_bounds__(kThreads) void svd_block_round_kernel(const SvdTask* __restrict__ tasks, int ntasks,
@@ -600,30 +604,39 @@ __global__ __launch_bounds__(kThreads) void svd_block_round_kernel(const SvdTask
   double* const Li = blk_lds + 2 * kBj * rmax;
   __shared__ double Vr[2 * kBj * 2 * kBj], Vi[2 * kBj * 2 * kBj];
   __shared__ int any;
-  // sector of this workgroup: pair_base is ascending, pair_base[0] = 0
+  // sector of this workgroup: pair_base is ascending, pair_base[0] = 0; the
+  // intra-block launch (round < 0) has two workgroups per block pair slot
   int lo = 0, hi = ntasks - 1;
-  const int b = blockIdx.x;
+  const bool intra = round < 0;
+  const int b = blockIdx.x, bs = intra ? b >> 1 : b;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (pair_base[mid] <= b) lo = mid;
+    if (pair_base[mid] <= bs) lo = mid;
     else hi = mid - 1;
   }
   const int sidx = lo;
   const SvdTask t = tasks[sidx];
-  const int q = b - pair_base[sidx];
+  const int q = b - (intra ? 2 : 1) * pair_base[sidx];
   const int m = t.m, n = t.n;
   const int p = m < n ? n : m, r = m < n ? m : n;
   const int nb = (r + kBj - 1) / kBj, nbb = nb + (nb & 1);
-  if (round >= nbb - 1 || q >= nbb / 2 || r > rmax) return;
-  int I = (round + q) % (nbb - 1);
-  int J = q == 0 ? nbb - 1 : (round + nbb - 1 - q) % (nbb - 1);
-  if (I > J) {
-    const int x = I;
-    I = J;
-    J = x;
+  if (r > rmax) return;
+  int I, J;
+  if (intra) {  // the pairs inside block q
+    if (q >= nb) return;
+    I = q;
+    J = nb;
+  } else {  // the pairs across blocks I and J of the tournament's round
+    if (round >= nbb - 1 || q >= nbb / 2) return;
+    I = (round + q) % (nbb - 1);
+    J = q == 0 ? nbb - 1 : (round + nbb - 1 - q) % (nbb - 1);
+    if (I > J) {
+      const int x = I;
+      I = J;
+      J = x;
+    }
+    if (J >= nb) return;  // the bye of an odd block count
   }
-  // the bye of an odd block count; a sector of one block runs it alone
-  if (J >= nb && nb > 1) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int i0 = I * kBj, wI = min(kBj, r - i0), j0 = J * kBj, wJ = J < nb ? min(kBj, r - j0) : 0, w = wI + wJ;
   auto colof = [&](int c) { return c < wI ? i0 + c : j0 + (c - wI); };
@@ -644,14 +657,24 @@ __global__ __launch_bounds__(kThreads) void svd_block_round_kernel(const SvdTask
   __syncthreads();
   const double tol = sqrt((double)r) * DBL_EPSILON;
   const int ww = w + (w & 1);
-  for (int sub = 0; sub < ww - 1; ++sub) {
-    for (int pq = wave; pq < ww / 2; pq += kWaves) {
-      int i = (sub + pq) % (ww - 1);
-      int j = pq == 0 ? ww - 1 : (sub + ww - 1 - pq) % (ww - 1);
-      if (i > j) {
-        const int x = i;
-        i = j;
-        j = x;
+  // intra: a tournament over the block's wI columns; across: kBj sub-rounds,
+  // wave a pairs column a of I with column (a + sub) mod kBj of J
+  const int nsub = intra ? ww - 1 : kBj;
+  for (int sub = 0; sub < nsub; ++sub) {
+    for (int pq = wave; pq < (intra ? ww / 2 : kBj); pq += kWaves) {
+      int i, j;
+      if (intra) {
+        i = (sub + pq) % (ww - 1);
+        j = pq == 0 ? ww - 1 : (sub + ww - 1 - pq) % (ww - 1);
+        if (i > j) {
+          const int x = i;
+          i = j;
+          j = x;
+        }
+      } else {
+        i = pq;
+        j = wI + (pq + sub) % kBj;
+        if (i >= wI) continue;
       }
       if (j >= w) continue;
       double a = 0.0, bb = 0.0, gr = 0.0, gi = 0.0;
@@ -1076,12 +1099,13 @@ hipError_t launch_svd_block_round(int cplx, const SvdTask* tasks, int ntasks, co
     if (e != hipSuccess) return e;
     attr[cplx ? 1 : 0] = true;
   }
+  const int grid = round < 0 ? 2 * npairs : npairs;
   if (cplx)
-    svd_block_round_kernel<true><<<npairs, kThreads, lds, st>>>(tasks, ntasks, pair_base, round, W, w_plane, rotated,
-                                                                rmax);
+    svd_block_round_kernel<true><<<grid, kThreads, lds, st>>>(tasks, ntasks, pair_base, round, W, w_plane, rotated,
+                                                              rmax);
   else
-    svd_block_round_kernel<false><<<npairs, kThreads, lds, st>>>(tasks, ntasks, pair_base, round, W, w_plane,
-                                                                 rotated, rmax);
+    svd_block_round_kernel<false><<<grid, kThreads, lds, st>>>(tasks, ntasks, pair_base, round, W, w_plane, rotated,
+                                                               rmax);
   return hipGetLastError();
 }
 

@@ -162,7 +162,9 @@ constexpr int kSvdBlockMaxR = 512;
 hipError_t launch_svd_phase(int cplx, int phases, const SvdTask* tasks, int ntasks, const double* M, int64_t m_plane,
                             double* U, int64_t u_plane, double* Vt, int64_t vt_plane, double* S, double* W,
                             int64_t w_plane, hipStream_t st);
-// rmax: the largest min(m, n) of the launch's tasks (sizes its LDS)
+// rmax: the largest min(m, n) of the launch's tasks (sizes its LDS); round -1:
+// the pairs inside each block, rounds 0 .. svd_block_rounds - 1: the pairs
+// across the blocks the tournament pairs in that round
 hipError_t launch_svd_block_round(int cplx, const SvdTask* tasks, int ntasks, const int* pair_base, int npairs,
                                   int round, double* W, int64_t w_plane, double* rotated, int rmax, hipStream_t st);
 int svd_block_pairs(int m, int n);

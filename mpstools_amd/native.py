@@ -726,8 +726,10 @@ class Cheb:
         _check(lib().mpsa_cheb_set_big_sector(self.h, int(k)))
 
     def set_block_jacobi(self, r_min):
-        """Sectors with min(m, n) >= r_min (<= 512) of an SVD step take the
-        blocked Jacobi SVD across several workgroups (0: never; default 96)."""
+        """Sectors with min(m, n) >= r_min and max(m, n) <= 1024 are
+        factorised by several workgroups each (Householder panels, Q
+        formation, and for SVD steps with min(m, n) <= 512 the block Jacobi
+        rounds); 0: one workgroup per sector; default 32."""
         _check(lib().mpsa_cheb_set_block_jacobi(self.h, int(r_min)))
 
     def factor_log(self, cap):

@@ -1337,7 +1337,7 @@ hipError_t launch_svd_block_round(int, const SvdTask* tasks, int ntasks, const i
     if (!chk_range(tasks, 0, ntasks - 1, "svd tasks") || !chk_range(pair_base, 0, ntasks, "svd pair base") ||
         !chk_range(rotated, 0, ntasks - 1, "svd rotated"))
       return;
-    if (pair_base[0] != 0 || pair_base[ntasks] != npairs || round < 0) report("svd block round: bad pair table");
+    if (pair_base[0] != 0 || pair_base[ntasks] != npairs || round < -1) report("svd block round: bad pair table");
   });
   return hipSuccess;
 }

@@ -51,7 +51,7 @@ def test_cheb_engine_steps_and_moments_match_oracle(gpu, oracle, model, L, nvec,
     (gesvd / geqrf+ungqr / gelqf+unglq) that the engine uses for large sectors;
     "blocked" sends every SVD sector of at least 2 columns to the blocked
     Jacobi SVD (several workgroups per sector) that the engine uses for
-    sectors of 96 to 512 columns."""
+    sectors of 32 to 512 columns (QR and LQ: 32 or more)."""
     _say("cheb engine", model, L, nvec, chimax, cplx, big)
     sts, itk = chebcase.states(model, L, nvec, 40 + L, cplx=cplx)
     mpo = chebcase.scaled_mpo(model, L)
@@ -108,7 +108,7 @@ def test_cheb_engine_large_sector_invariants(gpu, cplx, block):
     state (conv <= 1e-10, <c|t> = 1 within 1e-10); compressSingleSite with
     chimax above w D gives the exact applyMPO state (|c - H t|^2 within 1e-10
     of |H t|^2).  block: the blocked Jacobi SVD's threshold (None: the
-    default 96, 0: every sector in the one-workgroup kernel)."""
+    default 32, 0: every sector in the one-workgroup kernel)."""
     _say("cheb engine large sectors", cplx, block)
     L, D = 24, 512
     itk = [models.itokey_u1xu1()] * L
