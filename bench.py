@@ -79,7 +79,12 @@ KERNEL = {0: "gemm_bf16_stream_kernel", 1: "gemm_bf16_kernel", 2: "gemm_bf16_ker
 # U(1) bonds: (model, L, N, D, seed) -- the t-V chain at half filling, the XXZ
 # chain of the gradient workload, the small XXZ chain
 WORKLOADS = {"dense": None, "tv512": ("tv", 100, 50, 512, 20260002), "xxz1024": ("xxz", 64, 32, 1024, 20260004),
-             "xxz64": ("xxz", 32, 16, 64, 20260001), "hub1024": ("hubbard", 80, 80, 1024, 20260005)}
+             "xxz64": ("xxz", 32, 16, 64, 20260001), "hub1024": ("hubbard", 80, 80, 1024, 20260005),
+             "tv1700": ("tv", 100, 50, 8000, 20260006)}
+# sector profiles other than the default: tv1700 keeps the 24 heaviest charge
+# sectors of every bond, apportions 8000 over them by weight and clamps each
+# into [20, 120] (centre bond: 24 sectors, sum 1698)
+PROFILES = {"tv1700": {"clamp": (20, 120), "nsec": 24}}
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "gemm_bf16_pipe_dense_D2048_hbm_traffic.txt")
 
 
@@ -169,17 +174,20 @@ def u1_bonds(torch, steps, warmup):
       xxz64  : XXZ L = 32, D = 64 -- one matvec and the 10-step Lanczos of
                the plain-method workload, both timed on GPU and CPU;
       tv512  : t-V L = 100, D = 512 -- also 64 vectors per batched matvec;
+      tv1700 : t-V L = 100, 24 sectors per bond clamped to [20, 120]
+               (centre sum 1698) -- also 64 vectors per batched matvec;
       hub1024: Fermi-Hubbard L = 80, D = 1024 (U(1) x U(1)) -- the bond of
                the Chebyshev workload, real FP64."""
     from mpstools_amd import bench_problem, native
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     out = {}
-    for name, nvecs in (("xxz64", (1,)), ("tv512", (1, 64)), ("hub1024", (1, 16))):
+    for name, nvecs in (("xxz64", (1,)), ("tv512", (1, 64)), ("tv1700", (1, 64)), ("hub1024", (1, 16))):
         model, L, N, D, seed = WORKLOADS[name]
         t0 = time.perf_counter()
-        prob = bench_problem.u1_centre(model, L, N, D, seed, precisions=("f64", "bf16x3"))
-        row = {"psi_blocks": prob["psi_blocks"], "psi_block_dims": prob["psi_block_dims"]}
+        prob = bench_problem.u1_centre(model, L, N, D, seed, precisions=("f64", "bf16x3"), **PROFILES.get(name, {}))
+        row = {"psi_blocks": prob["psi_blocks"], "psi_block_dims": prob["psi_block_dims"],
+               "centre_D": prob["centre_D"]}
         for pr in ("f64", "bf16x3"):
             plan = prob["plan_" + pr]
             info = plan.info()
@@ -463,12 +471,15 @@ def main():
     else:
         model, L, N, D, seed = WORKLOADS[args.workload]
         log(f"building U(1) {model} centre problem L={L} N={N} D={D}")
-        prob = bench_problem.u1_centre(model, L, N, D, seed, precisions=precs, verbose=True)
+        prob = bench_problem.u1_centre(model, L, N, D, seed, precisions=precs, verbose=True,
+                                       **PROFILES.get(args.workload, {}))
         name = {"tv": "U(1) t-V t=1 V=2", "xxz": "U(1) XXZ Jz=Jxy=1",
                 "hubbard": "U(1)xU(1) Fermi-Hubbard t=1 U=4 (real FP64, N_up=N_dn)"}[model]
+        dlabel = f"D={D}" if args.workload not in PROFILES else (
+            f"sum D={prob['centre_D']} (24 heaviest sectors per bond, dims clamped to [20, 120])")
         workload = (f"{name} centre bond, L={L}, N={N}, w={6 if model == 'hubbard' else 5}, "
                     f"d={4 if model == 'hubbard' else 2}, "
-                    f"D={D} over charge sectors ({prob['psi_blocks']} psi blocks, dims "
+                    f"{dlabel} over charge sectors ({prob['psi_blocks']} psi blocks, dims "
                     f"{prob['psi_block_dims'][0]}-{prob['psi_block_dims'][1]}), seeded MPS, GPU-built environments")
     torch.cuda.synchronize()
     log(f"problem built in {time.perf_counter() - t0:.1f}s")

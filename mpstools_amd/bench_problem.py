@@ -163,12 +163,14 @@ def dense_heisenberg_centre(D=2048, L=128, seed=20260003, build_envs="sweep", pr
     return out
 
 
-def u1_centre(model="tv", L=100, N=50, D=512, seed=20260002, precisions=("f64",), verbose=False):
+def u1_centre(model="tv", L=100, N=50, D=512, seed=20260002, precisions=("f64",), verbose=False, clamp=None,
+              nsec=None):
     """Centre bond of a U(1)-symmetric chain: t-V (t = 1, V = 2, mu = 0), XXZ
     (Jz = Jxy = 1, hz = 0) or Fermi-Hubbard (t = 1, U = 4, U(1) x U(1)), N
     particles (up spins; electrons for Hubbard, half of them up), bond
     dimension D split over charge sectors by the package's profile
-    (synth.bond_dims).
+    (synth.bond_dims; with clamp = (lo, hi) and nsec: the nsec heaviest sectors
+    of every bond, D apportioned over them, dimensions clamped into [lo, hi]).
 
     The MPS is this package's seeded generator (synth.random_mps, host numpy),
     brought to mixed-canonical form around (c, c+1) on the host; the
@@ -194,7 +196,7 @@ def u1_centre(model="tv", L=100, N=50, D=512, seed=20260002, precisions=("f64",)
     else:
         itk = [models.itokey_u1()] * L
     c = L // 2 - 1
-    mps, dims = synth.random_mps(L, itk, N, D, seed)
+    mps, dims = synth.random_mps(L, itk, N, D, seed, clamp=clamp, nsec=nsec)
     mps = synth.mixed_canonical(mps, c)
 
     def run(Et, dE, s, direction):
@@ -227,7 +229,7 @@ def u1_centre(model="tv", L=100, N=50, D=512, seed=20260002, precisions=("f64",)
     vt = xt.union(yt)
     psi = torch.from_numpy(vt.pack(psi_b)).to(dev)
     out = dict(mpo=mpo, itk=itk[c], c=c, dims=dims, Lt=Lt, Rt=Rt, dL=dL, dR=dR, xt=xt, yt=yt, vt=vt, psi=psi, L=L,
-               D=D, psi_blocks=len(xt.keys()), psi_block_dims=(int(min(xt.rows.min(), xt.cols.min())),
+               D=D, centre_D=int(sum(dims[c + 1].values())), psi_blocks=len(xt.keys()), psi_block_dims=(int(min(xt.rows.min(), xt.cols.min())),
                                                              int(max(xt.rows.max(), xt.cols.max()))))
     for pr in precisions:
         _add_plan(torch, out, pr, (mpo[c], mpo[c + 1]), (itk[c], itk[c + 1]), seed)

@@ -61,14 +61,23 @@ def _apportion(D, weights):
     return base
 
 
-def bond_dims(L, itk, Ntot, D):
-    """Per-bond {charge: dim} for bonds 0..L (see module docstring)."""
+def bond_dims(L, itk, Ntot, D, clamp=None, nsec=None):
+    """Per-bond {charge: dim} for bonds 0..L (see module docstring).  With
+    clamp = (lo, hi) and nsec, a bond keeps its nsec heaviest sectors (weight
+    cl * cr, ties by charge), apportions D over them by weight and clamps each
+    dimension into [lo, hi] (and to the exact count), so sector sizes are
+    bounded below as well as above."""
     cl, cr = _counts_left(L, itk), _counts_right(L, itk, Ntot)
     dims = []
     for l in range(L + 1):
         qs = sorted(set(cl[l]) & set(cr[l]))
         mmax = {q: min(cl[l][q], cr[l][q]) for q in qs}
-        if sum(mmax.values()) <= D:
+        if clamp is not None:
+            w = {q: cl[l][q] * cr[l][q] for q in qs}
+            keep = sorted(qs, key=lambda q: (-w[q], q))[:nsec or len(qs)]
+            a = _apportion(D, {q: w[q] for q in keep}) if keep else {}
+            d = {q: min(max(a[q], clamp[0]), clamp[1], mmax[q]) for q in keep}
+        elif sum(mmax.values()) <= D:
             d = dict(mmax)
         else:
             w = {q: cl[l][q] * cr[l][q] for q in qs}
@@ -161,10 +170,10 @@ def right_normalize(A: Blocks, Aprev: Blocks | None):
     return out, carry
 
 
-def random_mps(L, itk, Ntot, D, seed, dtype=np.float64):
+def random_mps(L, itk, Ntot, D, seed, dtype=np.float64, clamp=None, nsec=None):
     """Seeded random MPS (list of Blocks) with the sector profile above."""
     rng = np.random.Generator(np.random.PCG64(seed))
-    dims = bond_dims(L, itk, Ntot, D)
+    dims = bond_dims(L, itk, Ntot, D, clamp, nsec)
     return [random_site(rng, dims[i], dims[i + 1], itk[i], dtype) for i in range(L)], dims
 
 
