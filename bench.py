@@ -260,11 +260,12 @@ def chebyshev(torch, args):
         setup = time.perf_counter() - t0
         r = e.moments(3, D, 2)
         st = e.stats()
+        host_timers = e.timers()
         out[name] = {"s_per_moment": round(float(r["moment_s"][2]), 3),
                      "moment_s": [round(float(x), 3) for x in r["moment_s"]], "setup_s": round(setup, 2),
                      "mu": [[round(float(z.real), 10), round(float(z.imag), 10)] for z in r["moments"][0]],
                      "host_plan_s": round(st["host_plan_s"], 2), "host_keys_s": round(st["host_keys_s"], 2),
-                     "device_bytes": st["pool_bytes"]}
+                     "device_bytes": st["pool_bytes"], "host_timers": host_timers}
         log("chebyshev", name, json.dumps(out[name]))
         del e
         native.lib().mpsa_release_pools()

@@ -499,6 +499,13 @@ int mpsa_cheb_moment_times(const mpsa_cheb* engine, double* out, int cap, int* n
  * limit) are factorised by rocSOLVER (gesvd, geqrf/ungqr, gelqf/unglq), one
  * call each; smaller ones by the batched kernels in one launch.  k in
  * [1, 1024]. */
+/* Host-side timers of the engine since its creation, the first n of: seconds
+ * in the one-site H_eff steps, the environment layers, the site
+ * factorisations (prepareSite), stream waits, device allocation and release
+ * (hipMalloc / hipFree), then the number of allocations and releases, seconds
+ * building plans, the number of stream waits, plans cached (no counterpart in
+ * the reference: diagnostics). */
+int mpsa_cheb_timers(const mpsa_cheb* engine, double* out, int n);
 int mpsa_cheb_set_big_sector(mpsa_cheb* engine, int k);
 /* Sectors with r = min(m, n) >= r_min (default 32) and max(m, n) <= 1024
  * are factorised by several workgroups each: the Householder reflections in

@@ -71,6 +71,8 @@ struct Pool {
   std::multimap<size_t, void*> free_;
   std::unordered_map<void*, size_t> all;  // every allocation and its size
   size_t idle = 0;
+  double t_alloc = 0.0;  // host seconds in hipMalloc / hipFree, and their count
+  int64_t n_alloc = 0;
   void* get(size_t bytes) {
     bytes = std::max<size_t>((bytes + 255) / 256 * 256, 256);
     auto it = free_.lower_bound(bytes);
@@ -81,11 +83,14 @@ struct Pool {
       return p;
     }
     void* p = nullptr;
+    const auto t0 = std::chrono::steady_clock::now();
     if (hipMalloc(&p, bytes) != hipSuccess) {
       (void)hipGetLastError();
       release_idle();
       if (hipMalloc(&p, bytes) != hipSuccess) throw HErr(MPSA_ENOMEM, "cheb: device allocation failed");
     }
+    t_alloc += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    ++n_alloc;
     all[p] = bytes;
     return p;
   }
@@ -99,12 +104,15 @@ struct Pool {
     idle += b;
   }
   void release_idle() {
+    const auto t0 = std::chrono::steady_clock::now();
     for (auto& kv : free_) {
       (void)hipFree(kv.second);
       all.erase(kv.second);
+      ++n_alloc;
     }
     free_.clear();
     idle = 0;
+    t_alloc += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
   ~Pool() {
     for (auto& a : all) (void)hipFree(a.first);
@@ -163,23 +171,65 @@ HTab table_from(const std::map<Key, std::pair<int64_t, int64_t>>& m, int cplx_) 
   return t;
 }
 
+// Plan-cache keys: a 128-bit digest of the operands' structure (keys, block
+// dimensions, MPO entries) instead of the structure itself, which for a
+// batch of vectors is megabytes per table and was copied and hashed on every
+// call.  Two independent 64-bit lanes, each a multiply-rotate mix of every
+// 8-byte word with a final avalanche (the murmur3 finaliser), plus the
+// lengths: a collision between two of the engine's few thousand structures
+// has probability of order 1e-31.
+struct Digest {
+  uint64_t a = 0x243f6a8885a308d3ull, b = 0x13198a2e03707344ull;
+  static uint64_t rotl(uint64_t x, int r) { return (x << r) | (x >> (64 - r)); }
+  void word(uint64_t w) {
+    a = rotl(a ^ (w * 0x87c37b91114253d5ull), 31) * 0x4cf5ad432745937full + 0x52dce729ull;
+    b = rotl(b + (w * 0x9e3779b97f4a7c15ull), 27) * 0xc2b2ae3d27d4eb4full ^ 0x38495ab5ull;
+  }
+  void bytes(const void* p, size_t n) {
+    const unsigned char* c = (const unsigned char*)p;
+    size_t i = 0;
+    for (; i + 8 <= n; i += 8) {
+      uint64_t w;
+      std::memcpy(&w, c + i, 8);
+      word(w);
+    }
+    uint64_t w = 0;
+    if (i < n) std::memcpy(&w, c + i, n - i);
+    word(w ^ ((uint64_t)n << 56));
+  }
+  static uint64_t fmix(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    return k ^ (k >> 33);
+  }
+  std::string str() const {
+    const uint64_t x[2] = {fmix(a ^ rotl(b, 17)), fmix(b ^ rotl(a, 41))};
+    return std::string((const char*)x, sizeof(x));
+  }
+};
 std::string sig_tab(const HTab& t) {
-  std::string s;
-  s.append((const char*)t.q.data(), t.q.size() * 8);
-  s.append((const char*)t.p.data(), t.p.size() * 4);
-  s.append((const char*)t.rows.data(), t.rows.size() * 8);
-  s.append((const char*)t.cols.data(), t.cols.size() * 8);
-  s += "|" + std::to_string(t.n()) + "," + std::to_string(t.cplx) + "|";
-  return s;
+  Digest d;
+  d.bytes(t.q.data(), t.q.size() * 8);
+  d.bytes(t.p.data(), t.p.size() * 4);
+  d.bytes(t.rows.data(), t.rows.size() * 8);
+  d.bytes(t.cols.data(), t.cols.size() * 8);
+  d.word((uint64_t)t.n());
+  d.word((uint64_t)t.cplx);
+  return d.str();
 }
 std::string sig_mpo(const HMpo& w) {
-  std::string s = std::to_string(w.Dl) + "," + std::to_string(w.Dr) + ":";
-  s.append((const char*)w.wl.data(), w.wl.size() * 4);
-  s.append((const char*)w.wr.data(), w.wr.size() * 4);
-  s.append((const char*)w.out.data(), w.out.size() * 4);
-  s.append((const char*)w.in.data(), w.in.size() * 4);
-  s.append((const char*)w.val.data(), w.val.size() * 8);
-  return s + "|" + std::to_string(w.val.size()) + "|";
+  Digest d;
+  d.word((uint64_t)w.Dl);
+  d.word((uint64_t)w.Dr);
+  d.bytes(w.wl.data(), w.wl.size() * 4);
+  d.bytes(w.wr.data(), w.wr.size() * 4);
+  d.bytes(w.out.data(), w.out.size() * 4);
+  d.bytes(w.in.data(), w.in.size() * 4);
+  d.bytes(w.val.data(), w.val.size() * 8);
+  d.word((uint64_t)w.val.size());
+  return d.str();
 }
 
 }  // namespace
@@ -217,6 +267,7 @@ struct mpsa_cheb {
   int64_t block_min_r = MPSA_BLOCK_JACOBI_MIN;
   // host time (seconds): plan builds, output-key queries, waits on the stream
   double t_plan = 0.0, t_keys = 0.0, t_sync = 0.0;
+  double t_heff = 0.0, t_env = 0.0, t_prep = 0.0;  // host seconds in heff1, env_layer, prepare_site
   std::vector<double> moment_s;  // wall seconds of each moment of the last moment loop
   int64_t n_sync = 0;
   // sector shapes of every factorisation launch, while logging is on (up to
@@ -225,10 +276,18 @@ struct mpsa_cheb {
   static constexpr int kFlogWidth = 8;
   std::vector<int64_t> flog;
   int64_t flog_cap = 0;
+  // Pinned staging ring for uploads: a copy from pinned memory is queued
+  // without waiting, where a pageable source makes the runtime stage it
+  // synchronously.  A region is rewritten only after a stream wait that
+  // follows its copy (the ring wraps through sync()).
+  static constexpr size_t kRing = size_t(64) << 20;
+  char* ring = nullptr;
+  size_t ring_at = 0;
   ~mpsa_cheb() {
     if (st) (void)hipStreamSynchronize(st);
     for (auto& e : plans) mpsa_plan_destroy(e.plan);
     slot.clear();
+    if (ring) (void)hipHostFree(ring);
   }
   void sync() {
     const auto t0 = std::chrono::steady_clock::now();
@@ -236,15 +295,31 @@ struct mpsa_cheb {
     t_sync += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     ++n_sync;
     keep.clear();
+    ring_at = 0;
   }
   template <typename T>
   Buf upload(const std::vector<T>& v) {
     Buf b(pool, (int64_t)((sizeof(T) * std::max<size_t>(v.size(), 1) + 7) / 8), st, false);
-    if (!v.empty()) {
-      auto h = std::make_shared<std::vector<T>>(v);
-      hchk(hipMemcpyAsync(b.p, h->data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st), "cheb: upload");
-      keep.push_back(h);
+    if (v.empty()) return b;
+    const size_t bytes = sizeof(T) * v.size();
+    if (!ring && hipHostMalloc((void**)&ring, kRing, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      ring = nullptr;
     }
+    if (ring && bytes <= kRing / 4) {
+      size_t at = (ring_at + 255) / 256 * 256;
+      if (at + bytes > kRing) {
+        sync();  // every earlier copy out of the ring has completed
+        at = 0;
+      }
+      std::memcpy(ring + at, v.data(), bytes);
+      hchk(hipMemcpyAsync(b.p, ring + at, bytes, hipMemcpyHostToDevice, st), "cheb: upload");
+      ring_at = at + bytes;
+      return b;
+    }
+    auto h = std::make_shared<std::vector<T>>(v);
+    hchk(hipMemcpyAsync(b.p, h->data(), bytes, hipMemcpyHostToDevice, st), "cheb: upload");
+    keep.push_back(h);
     return b;
   }
   const PlanEntry* find_plan(const std::string& k) {
@@ -416,6 +491,7 @@ Buf identity_buf(Eng& E, const HTab& t) {
 // E' = addContractionLayer(E, A, W, dir) with the upper (conjugated) tensor Ap
 void env_layer(Eng& E, const HTab& Et, const double* dE, const HTab& At, const double* dA, const HTab& Apt,
                const double* dAp, const HMpo& W, const std::vector<uint64_t>& itk, int dir, HTab& out_t, Buf& out_b) {
+  ScopeTimer tt(E.t_env);
   mpsa_blocks be = Et.view(), ba = At.view(), bp = Apt.view();
   mpsa_mpo mw = W.view();
   // a plan and its output table are fixed by the structure of the operands
@@ -449,6 +525,7 @@ void env_layer(Eng& E, const HTab& Et, const double* dE, const HTab& At, const d
 // is the produced key set in key order
 void heff1(Eng& E, const HTab& Lt, const double* dL, const HTab& At, const double* dA, const HTab& Rt,
            const double* dR, const HMpo& W, const std::vector<uint64_t>& itk, HTab& out_t, Buf& out_b) {
+  ScopeTimer tt(E.t_heff);
   mpsa_blocks bl = Lt.view(), br = Rt.view(), ba = At.view();
   mpsa_mpo mw = W.view();
   const std::string key = "H" + std::to_string(E.h1prec) + sig_tab(Lt) + sig_tab(Rt) + sig_tab(At) + sig_mpo(W);
@@ -680,6 +757,7 @@ void qr_blocked(Eng& E, bool cplx, int right, const std::vector<SvdTask>& big, c
 // the compression's optimum, later Schmidt values) are the reference's; only
 // the gauge of the two tensors differs.
 void prepare_site(Eng& E, DMps& M, int site, bool left, bool keep_lambda) {
+  ScopeTimer tt(E.t_prep);
   const bool use_qr = !keep_lambda;
   const int sa = left ? site : site - 1, sb = left ? site + 1 : site;  // (A, B) = (site, site+1) or (site-1, site)
   const HTab &At = M.A[(size_t)sa], &Bt = M.A[(size_t)sb];
@@ -834,32 +912,63 @@ void prepare_site(Eng& E, DMps& M, int site, bool left, bool keep_lambda) {
   // new tables: the factorised tensor keeps U (l) or V^dagger (r) with k
   // columns (rows); the neighbour's blocks of the sector charge take k rows
   // (columns)
-  std::map<Key, std::pair<int64_t, int64_t>> nx, ny;
-  for (int64_t i = 0; i < X.n(); ++i) {
-    const Key kk = key_of(X, i);
-    const Sec& S = secs[left ? kk[1] : kk[0]];
-    nx[kk] = left ? std::make_pair(X.rows[(size_t)i], S.k) : std::make_pair(S.k, X.cols[(size_t)i]);
-  }
   const HTab& Y = left ? Bt : At;  // the neighbour
-  for (int64_t i = 0; i < Y.n(); ++i) {
-    const Key kk = key_of(Y, i);
-    const uint64_t q = left ? kk[0] : kk[1];
-    auto it = secs.find(q);
-    if (it == secs.end()) ny[kk] = {Y.rows[(size_t)i], Y.cols[(size_t)i]};
-    else ny[kk] = left ? std::make_pair(it->second.k, Y.cols[(size_t)i]) : std::make_pair(Y.rows[(size_t)i], it->second.k);
-  }
-  HTab NX = table_from(nx, pl == 2), NY = table_from(ny, pl == 2);
-  Buf dNX(E.pool, NX.len(), E.st), dNY(E.pool, NY.len(), E.st);
+  auto x_dims = [&](int64_t i) {
+    const Sec& S = secs[left ? X.q[2 * i + 1] : X.q[2 * i]];
+    return left ? std::make_pair(X.rows[(size_t)i], S.k) : std::make_pair(S.k, X.cols[(size_t)i]);
+  };
+  auto y_dims = [&](int64_t i) {
+    auto it = secs.find(left ? Y.q[2 * i] : Y.q[2 * i + 1]);
+    if (it == secs.end()) return std::make_pair(Y.rows[(size_t)i], Y.cols[(size_t)i]);
+    return left ? std::make_pair(it->second.k, Y.cols[(size_t)i]) : std::make_pair(Y.rows[(size_t)i], it->second.k);
+  };
+  // The new tables have the keys of X and Y.  Tables in key order (the
+  // engine's own) give them in the same order, so block i of X is block i of
+  // NX and block t of NY is block t of Y; otherwise go through key maps.
+  auto in_order = [](const HTab& t) {
+    for (int64_t i = 1; i < t.n(); ++i)
+      if (!(key_of(t, i - 1) < key_of(t, i))) return false;
+    return true;
+  };
+  const bool ordered = in_order(X) && in_order(Y);
+  HTab NX, NY;
   std::map<Key, int64_t> nx_index, y_index;
-  for (int64_t t = 0; t < NX.n(); ++t) nx_index[key_of(NX, t)] = t;
-  for (int64_t t = 0; t < Y.n(); ++t) y_index[key_of(Y, t)] = t;
+  if (ordered) {
+    auto rebuild = [&](const HTab& src, auto dims_of) {
+      HTab t;
+      t.ns = 1;
+      t.cplx = pl == 2;
+      t.q = src.q;
+      t.p = src.p;
+      t.rows.resize((size_t)src.n());
+      t.cols.resize((size_t)src.n());
+      for (int64_t i = 0; i < src.n(); ++i) {
+        const auto rc = dims_of(i);
+        t.rows[(size_t)i] = rc.first;
+        t.cols[(size_t)i] = rc.second;
+      }
+      t.pack();
+      return t;
+    };
+    NX = rebuild(X, x_dims);
+    NY = rebuild(Y, y_dims);
+  } else {
+    std::map<Key, std::pair<int64_t, int64_t>> nx, ny;
+    for (int64_t i = 0; i < X.n(); ++i) nx[key_of(X, i)] = x_dims(i);
+    for (int64_t i = 0; i < Y.n(); ++i) ny[key_of(Y, i)] = y_dims(i);
+    NX = table_from(nx, pl == 2);
+    NY = table_from(ny, pl == 2);
+    for (int64_t t = 0; t < NX.n(); ++t) nx_index[key_of(NX, t)] = t;
+    for (int64_t t = 0; t < Y.n(); ++t) y_index[key_of(Y, t)] = t;
+  }
+  Buf dNX(E.pool, NX.len(), E.st), dNY(E.pool, NY.len(), E.st);
   // factorised tensor from U / V^dagger
   std::vector<Copy2D> c1;
   for (auto& kv : secs) {
     const Sec& S = kv.second;
     for (size_t j = 0; j < S.blk.size(); ++j) {
       const int64_t i = S.blk[j];
-      const int64_t b = nx_index.at(key_of(X, i));
+      const int64_t b = ordered ? i : nx_index.at(key_of(X, i));
       if (left)
         add_copy(c1, E, S.uo + S.off[j], tu, NX.off[(size_t)b], NX.total, X.rows[(size_t)i], S.k, S.m,
                  X.rows[(size_t)i]);
@@ -882,7 +991,7 @@ void prepare_site(Eng& E, DMps& M, int site, bool left, bool keep_lambda) {
   const double* dY = left ? M.d[(size_t)sb].p : M.d[(size_t)sa].p;
   for (int64_t t = 0; t < NY.n(); ++t) {
     const Key kk = key_of(NY, t);
-    const int64_t i = y_index.at(kk);
+    const int64_t i = ordered ? t : y_index.at(kk);
     const uint64_t q = left ? kk[0] : kk[1];
     auto it = secs.find(q);
     if (it == secs.end()) {
@@ -1761,6 +1870,17 @@ int mpsa_cheb_profile(const mpsa_cheb* E, double* out5) {
   out5[2] = E->t_sync;
   out5[3] = (double)E->n_sync;
   out5[4] = (double)E->plans.size();
+  return MPSA_OK;
+}
+
+int mpsa_cheb_timers(const mpsa_cheb* E, double* out, int n) {
+  if (!E || !out || n < 0) {
+    set_last_error("cheb_timers: null argument or n < 0");
+    return MPSA_EINVAL;
+  }
+  const double v[9] = {E->t_heff, E->t_env, E->t_prep, E->t_sync, E->pool.t_alloc, (double)E->pool.n_alloc,
+                       E->t_plan, (double)E->n_sync, (double)E->plans.size()};
+  for (int i = 0; i < n && i < 9; ++i) out[i] = v[i];
   return MPSA_OK;
 }
 

@@ -121,6 +121,7 @@ def lib():
         L.mpsa_cheb_moment_times.argtypes = [P, P, C.c_int, P]
         L.mpsa_cheb_set_big_sector.argtypes = [P, C.c_int]
         L.mpsa_cheb_set_block_jacobi.argtypes = [P, C.c_int]
+        L.mpsa_cheb_timers.argtypes = [P, C.POINTER(C.c_double), C.c_int]
         L.mpsa_cheb_factor_log.argtypes = [P, C.c_int64]
         L.mpsa_cheb_factor_records.argtypes = [P, P, C.c_int64, P]
         L.mpsa_cheb_destroy.argtypes = [P]
@@ -724,6 +725,13 @@ class Cheb:
     def set_big_sector(self, k):
         """Sectors with min(m, n) >= k go to rocSOLVER (default 1024)."""
         _check(lib().mpsa_cheb_set_big_sector(self.h, int(k)))
+
+    def timers(self):
+        """Host-side seconds and counts of the engine since its creation."""
+        out = (C.c_double * 9)()
+        _check(lib().mpsa_cheb_timers(self.h, out, 9))
+        k = ("heff_s", "env_s", "prepare_s", "sync_s", "alloc_s", "allocs", "plan_s", "syncs", "plans")
+        return {n: (int(v) if n in ("allocs", "syncs", "plans") else round(v, 4)) for n, v in zip(k, out)}
 
     def set_block_jacobi(self, r_min):
         """Sectors with min(m, n) >= r_min and max(m, n) <= 1024 are

@@ -150,6 +150,16 @@ hipError_t hipMalloc(void** p, size_t bytes) {
   *p = m;
   return hipSuccess;
 }
+// pinned host memory is ordinary host memory here
+hipError_t hipHostMalloc(void** p, size_t bytes, unsigned int) {
+  if (!p) return hipErrorInvalidValue;
+  *p = aligned_alloc(256, ((bytes ? bytes : 1) + 255) / 256 * 256);
+  return *p ? hipSuccess : hipErrorOutOfMemory;
+}
+hipError_t hipHostFree(void* p) {
+  free(p);
+  return hipSuccess;
+}
 hipError_t hipFree(void* p) {
   if (!p) return hipSuccess;
   auto it = g_alloc.find((uintptr_t)p);

@@ -528,6 +528,9 @@ def test_emulated_chebyshev_engine_matches_oracle(lib, oracle, model, L, nvec, c
     eng4.set_block_jacobi(1)
     r4 = eng4.moments(6, 10000, 2)
     assert np.abs(r4["moments"] - r["moments"]).max() <= 1e-12
+    tm = eng4.timers()
+    assert tm["heff_s"] > 0 and tm["env_s"] > 0 and tm["prepare_s"] > 0 and tm["plans"] > 0
+    assert tm["syncs"] >= 1 and tm["allocs"] >= 0 and tm["alloc_s"] >= 0
     with pytest.raises(native.MpsaError):
         eng4.set_block_jacobi(-1)
     del eng, eng2, eng3, eng4
