@@ -204,8 +204,12 @@ struct mpsa_plan {
   int on_device = 0;
   int xy_same = 0;
   // the launches of one apply replayed from a graph (H_eff plans): valid for
-  // the vectors, strides, vector count and buffers it was captured with
+  // the vectors, strides, vector count and buffers it was captured with.  The
+  // g_* fields hold the last untimed apply's arguments (g_valid); a graph is
+  // recorded on the second apply in a row with the same arguments, so a plan
+  // applied once per environment (bind_env drops both) never pays for one.
   hipGraphExec_t gexec = nullptr;
+  bool g_valid = false;
   const double* g_dx = nullptr;
   double* g_dy = nullptr;
   int64_t g_xs = 0, g_ys = 0;
@@ -214,6 +218,7 @@ struct mpsa_plan {
   void drop_graph() {
     if (gexec) (void)hipGraphExecDestroy(gexec);
     gexec = nullptr;
+    g_valid = false;
   }
   // timing: one set of 5 events per timed apply (up to kTimedApplies kept)
   static constexpr int kTimedApplies = 512;
@@ -1309,20 +1314,31 @@ int mpsa_heff_apply(mpsa_plan* P, const double* dx, int64_t x_stride, double* dy
     hipStream_t st = (hipStream_t)stream;
     ensure_ws(P, nvec, st);
     // Untimed applies on a stream of the caller's own run from a graph of
-    // their launches (captured on the first apply with these arguments), so
+    // their launches (recorded on the second apply in a row with the same
+    // arguments, replayed from the third on), so
     // that the dependent launches follow each other without host gaps.  A
     // stream that is already being captured (a caller's graph) or the null
     // stream takes the launches directly.
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     if (st) HIPCHK(hipStreamIsCapturing(st, &cap));
     const bool graphable = !P->timing && st && cap == hipStreamCaptureStatusNone;
-    if (graphable && P->gexec && P->g_dx == dx && P->g_dy == dy && P->g_xs == x_stride && P->g_ys == y_stride &&
-        P->g_nvec == nvec && P->g_ws == P->ws && P->g_xc == P->xc) {
+    const bool same = P->g_valid && P->g_dx == dx && P->g_dy == dy && P->g_xs == x_stride &&
+                      P->g_ys == y_stride && P->g_nvec == nvec && P->g_ws == P->ws && P->g_xc == P->xc;
+    if (graphable && P->gexec && same) {
       HIPCHK(hipGraphLaunch(P->gexec, st));
       return MPSA_OK;
     }
-    if (graphable) {
+    if (graphable && !same) {  // first apply with these arguments: direct launches, remembered
       P->drop_graph();
+      P->g_valid = true;
+      P->g_dx = dx;
+      P->g_dy = dy;
+      P->g_xs = x_stride;
+      P->g_ys = y_stride;
+      P->g_nvec = nvec;
+      P->g_ws = P->ws;
+      P->g_xc = P->xc;
+    } else if (graphable) {  // the second in a row: record the graph
       HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
       try {
         apply_launches(P, dx, x_stride, dy, y_stride, nvec, st, nullptr);
@@ -1340,13 +1356,6 @@ int mpsa_heff_apply(mpsa_plan* P, const double* dx, int64_t x_stride, double* dy
         P->gexec = nullptr;
         throw Error(MPSA_EHIP, std::string("apply: graph instantiate: ") + hipGetErrorString(ie));
       }
-      P->g_dx = dx;
-      P->g_dy = dy;
-      P->g_xs = x_stride;
-      P->g_ys = y_stride;
-      P->g_nvec = nvec;
-      P->g_ws = P->ws;
-      P->g_xc = P->xc;
       HIPCHK(hipGraphLaunch(P->gexec, st));
       return MPSA_OK;
     }

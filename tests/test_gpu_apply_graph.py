@@ -1,8 +1,9 @@
 """H_eff applies replayed from the plan's own graph (mpsa_heff_apply on a
 caller stream that is not being captured): the replayed apply equals the
 directly launched one bit for bit (FP64 and BF16X3, one and several
-vectors), a change of vectors, vector count or environment (bind_env)
-re-captures instead of replaying stale arguments, and applies inside a
+vectors; the graph is recorded on the second apply in a row with the same
+arguments), a change of vectors, vector count or environment (bind_env)
+records anew instead of replaying stale arguments, and applies inside a
 caller's capture (the batched Lanczos) still run directly."""
 import numpy as np
 import pytest
@@ -41,7 +42,8 @@ def test_graph_apply_equals_direct(gpu, oracle, prec):
     plan.apply(x, y_ref, nvec, stride, stride)  # null stream: direct launches
     torch.cuda.synchronize()
     s = torch.cuda.Stream()
-    # the first apply on the stream captures, the next ones replay
+    # the first apply on the stream launches directly, the second records the
+    # graph, the next ones replay it
     with torch.cuda.stream(s):
         yb = torch.zeros(nvec * stride, dtype=torch.float64, device="cuda")
         for _ in range(4):
@@ -50,24 +52,35 @@ def test_graph_apply_equals_direct(gpu, oracle, prec):
             for v in range(nvec):
                 o = v * stride
                 assert torch.equal(yb[o:o + vt.total], y_ref[o:o + vt.total])
-        # one vector from the same buffers: a new capture, equal to the direct one-vector apply
+        # one vector from the same buffers: direct, then recorded and replayed,
+        # each equal to the direct one-vector apply on the null stream
         y1 = torch.zeros(stride, dtype=torch.float64, device="cuda")
-        plan.apply(x, y1, 1, stride, stride)
-        s.synchronize()
+        y1s = []
+        for _ in range(3):
+            y1.zero_()
+            plan.apply(x, y1, 1, stride, stride)
+            s.synchronize()
+            y1s.append(y1.clone())
     y1_ref = torch.zeros(stride, dtype=torch.float64, device="cuda")
     plan.apply(x, y1_ref, 1, stride, stride)
     torch.cuda.synchronize()
-    assert torch.equal(y1[:vt.total], y1_ref[:vt.total])
+    for y in y1s:
+        assert torch.equal(y[:vt.total], y1_ref[:vt.total])
     # a new environment: the captured launches must not keep the old one
     with torch.cuda.stream(s):
         plan.bind_env(2.0 * dL, dR)
         yb2 = torch.zeros(nvec * stride, dtype=torch.float64, device="cuda")
-        plan.apply(x, yb2, nvec, stride, stride)
-        s.synchronize()
+        yb2s = []
+        for _ in range(3):  # direct, recorded, replayed
+            yb2.zero_()
+            plan.apply(x, yb2, nvec, stride, stride)
+            s.synchronize()
+            yb2s.append(yb2.clone())
     # doubling L is exact in every precision (bf16 hi/lo splits scale by 2 too)
-    for v in range(nvec):
-        o = v * stride
-        assert torch.equal(yb2[o:o + vt.total], 2.0 * y_ref[o:o + vt.total])
+    for yb2 in yb2s:
+        for v in range(nvec):
+            o = v * stride
+            assert torch.equal(yb2[o:o + vt.total], 2.0 * y_ref[o:o + vt.total])
     torch.cuda.synchronize()
 
 
