@@ -863,7 +863,10 @@ static void build_plan(mpsa_plan* P, Built& b, const Tab& Ptab, const Tab& Qtab,
     for (int i = 0; i < (int)Qtab.n; ++i) xoff[(size_t)i] = Qtab.off[(size_t)i];
   }
 
-  // stage 1
+  // stage 1 (one task and one segment per product; stage 2: one segment per slot)
+  P->s1.tasks.reserve(P->s1.tasks.size() + b.prods.size());
+  P->s1.segs.reserve(P->s1.segs.size() + b.prods.size());
+  P->s2.segs.reserve(P->s2.segs.size() + b.slots.size());
   for (auto& pr : b.prods) {
     GTask t{};
     t.c_off = pr.ws_off;
