@@ -875,6 +875,7 @@ __global__ __launch_bounds__(kQrThreads) void qr_householder_kernel(const SvdTas
 // col_base[s] .. col_base[s + 1] of the grid.
 constexpr int kHp = 8;
 constexpr int kHcols = kThreads / 64;  // columns per workgroup, one per wave
+static_assert(kVecRows == kVecRowsMax, "the host routes sectors of at most kVecRowsMax rows to these kernels");
 
 template <bool C>
 struct HView {
